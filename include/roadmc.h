@@ -1,0 +1,182 @@
+/*
+ * libroadmc: MI355X (gfx950) Monte Carlo road evaluation, C ABI.
+ *
+ * The entry points replace the reference's GPU dispatch functions
+ * (include/gpuCode/SimulateGPU.h) over plain data: the reference takes Eigen
+ * objects and a Simulator object graph; this ABI takes the same arrays as
+ * pointers and sizes.  Inputs and outputs are host memory owned by the caller;
+ * the context owns device memory and reuses it across calls.  Every function
+ * returns ROADMC_OK (0) or an error code and, on error, writes no output;
+ * roadmc_last_error() returns the message.
+ *
+ * Random streams are Philox4x32-10 keyed by the problem's seed; the layout is
+ * documented in transportoptimiser_amd/csrc/rng.h.  Results are deterministic
+ * for a given problem.
+ */
+#ifndef ROADMC_H
+#define ROADMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ROADMC_OK 0
+#define ROADMC_ERR_INVALID 1     /* bad argument or size */
+#define ROADMC_ERR_HIP 2         /* HIP runtime error */
+#define ROADMC_ERR_OOM 3         /* device allocation failed */
+#define ROADMC_ERR_NUMERIC 4     /* singular system, non-finite input */
+#define ROADMC_ERR_UNSUPPORTED 5 /* valid request this build does not run */
+
+/* Growth laws (the reference has two different ones, see the header of
+ * mte_kernels.hip). */
+#define ROADMC_LAW_LOGNORMAL 0 /* n = n'(1 + r(1 - n'/K)) exp(s z - s^2/2) */
+#define ROADMC_LAW_REF_CPU 1   /* Simulator.cpp:712-728 */
+#define ROADMC_LAW_REF_GPU 2   /* SimulateGPU.cu:691-709 */
+
+/* Which population a path returns. */
+#define ROADMC_END_POST_GROWTH 0 /* Simulator.cpp:772-774 */
+#define ROADMC_END_PRE_GROWTH 1  /* SimulateGPU.cu:712-715 */
+
+#define ROADMC_PREC_FP32 0
+#define ROADMC_PREC_FP64 1
+
+/* Largest patch count the tiled kernels take (the population tile of one
+ * workgroup must fit in LDS). */
+#define ROADMC_MAX_PATCHES 1024
+
+typedef struct roadmc_ctx roadmc_ctx;
+
+/* One species on one road: the arrays SpeciesRoadPatches holds
+ * (include/road/SpeciesRoadPatches.h:294-306) and the packed parameters of
+ * SimulateGPU.cu:3690-3714. */
+typedef struct {
+  int32_t n_patches;        /* P, 1..ROADMC_MAX_PATCHES */
+  const double* init_pop;   /* [P] initial population per patch */
+  const double* capacity;   /* [P] carrying capacity, > 0 */
+  const double* growth;     /* [P] intrinsic growth rate (LOGNORMAL law);
+                               NULL reads as 0 */
+  const double* trans;      /* [P*P] column-major, (i, j) = P(origin i ->
+                               destination j), rows sum to 1 */
+  const double* surv;       /* [P*P] column-major survival for the control in
+                               force (survProbs[last] in the reference); NULL
+                               means no road mortality */
+  double params[8];         /* 0 growth current, 1 growth mean, 2 growth noise
+                               SD, 3 threshold, 4 reversion, 5 jump size,
+                               6 jump probability, 7 local variability */
+  double noise_sd;          /* LOGNORMAL year noise sigma */
+  double threshold_ratio;   /* MTE: P(total_T >= ratio * total_0) */
+} roadmc_species;
+
+/* One road.  The scalars the reference reads from the optimiser object
+ * (SimulateGPU.cu:3667-3675). */
+typedef struct {
+  int32_t n_species;
+  const roadmc_species* species; /* [n_species] */
+  int32_t n_years;               /* T >= 1 */
+  double time_step;              /* dt */
+  int64_t n_paths;               /* N >= 1, < 2^31 */
+  int32_t growth_law;            /* ROADMC_LAW_* */
+  int32_t endpoint;              /* ROADMC_END_* */
+  int32_t precision;             /* ROADMC_PREC_* */
+  double drop_tol;               /* movement entries <= drop_tol are removed
+                                    (dense2Sparse uses 1e-5); 0 keeps all */
+  uint64_t seed;
+  uint32_t road_id;              /* < 2^24, selects an independent stream */
+} roadmc_mte_problem;
+
+/* Outputs; each pointer may be NULL. */
+typedef struct {
+  double* mte_prob;   /* [S] fraction of paths with total_T >= ratio*total_0 */
+  double* end_mean;   /* [S] mean end total (Simulator.cpp:184-186) */
+  double* end_sd;     /* [S] population SD of end totals (1/N) */
+  double* patch_mean; /* [sum_s P_s] mean end population per patch */
+  double* end_pops;   /* [N*S] column-major per-path end totals (the
+                         reference's endPops matrix) */
+} roadmc_mte_result;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* SimulateGPU::deviceCount (SimulateGPU.h:197). */
+int roadmc_device_count(int* count);
+int roadmc_ctx_create(int device, roadmc_ctx** out);
+void roadmc_ctx_destroy(roadmc_ctx* ctx);
+const char* roadmc_last_error(const roadmc_ctx* ctx);
+/* Test hook: the next n device allocations of this context fail with
+ * ROADMC_ERR_OOM. */
+int roadmc_debug_fail_allocs(roadmc_ctx* ctx, int n);
+
+/* ---- MTE ---------------------------------------------------------------- */
+
+/* SimulateGPU::simulateMTECUDA (SimulateGPU.h:305-309, SimulateGPU.cu:
+ * 3650-3864): all species of one road on the fused tiled kernels. */
+int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
+                   roadmc_mte_result* result);
+
+/* The same evaluation with the reference's GPU design: per-call device
+ * allocation, random streams pre-generated into HBM, one thread per path over
+ * a CSC matrix (SimulateGPU.cu:595-717, 3720-3843).  Kept as the baseline of
+ * the batched path; results agree with roadmc_mte_run to FP32 rounding. */
+int roadmc_mte_run_plain(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
+                         roadmc_mte_result* result);
+
+/* A whole GA generation: every species of every road in one persistent
+ * launch (RoadGA.cpp:1687-1748 evaluates such roads one call at a time). */
+int roadmc_mte_run_batch(roadmc_ctx* ctx, const roadmc_mte_problem* roads,
+                         int32_t n_roads, roadmc_mte_result* results);
+
+/* Batched evaluation split into upload / launch / download, so a caller can
+ * relaunch a generation whose inputs already sit in HBM.  launch() replays a
+ * captured HIP graph on the context's stream and returns without waiting. */
+typedef struct roadmc_batch roadmc_batch;
+int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* roads,
+                         int32_t n_roads, roadmc_batch** out);
+int roadmc_batch_launch(roadmc_batch* batch);
+int roadmc_batch_sync(roadmc_batch* batch);
+int roadmc_batch_fetch(roadmc_batch* batch, roadmc_mte_result* results);
+/* Average duration in ms of the persistent kernel over the last n launches,
+ * measured with HIP events on the stream the kernel runs on. */
+int roadmc_batch_kernel_ms(roadmc_batch* batch, double* ms);
+void roadmc_batch_destroy(roadmc_batch* batch);
+
+/* Tile shapes: rows of paths per workgroup for each patch-count bucket.
+ * roadmc_autotune times the candidates on synthetic roads and stores the
+ * winners in the context; roadmc_get_tiles reports them. */
+int roadmc_autotune(roadmc_ctx* ctx, int32_t n_paths, int32_t n_years);
+int roadmc_get_tiles(const roadmc_ctx* ctx, int32_t* bucket_max_p,
+                     int32_t* rows, int32_t* n_buckets);
+
+/* ---- helpers restated from the reference -------------------------------- */
+
+/* SimulateGPU::dense2Sparse (SimulateGPU.cu:5973-5996): column-major dense
+ * to CSC, keeping entries > tol.  nnz_out receives the count; val and row_idx
+ * need rows*cols capacity. */
+int roadmc_dense_to_csc(const float* dense, int32_t rows, int32_t cols,
+                        float tol, float* val, int32_t* row_idx,
+                        int32_t* per_col, int32_t* nnz_out);
+
+/* Utility::NormalCDFInverse (Utility.cpp:165-193). */
+int roadmc_normal_cdf_inverse(double p, double* out);
+
+/* Penalty of Simulator::simulateMTE (Simulator.cpp:197-212) summed over
+ * species. */
+int roadmc_mte_penalty(int32_t n_species, const double* threshold,
+                       const double* end_mean, const double* end_sd,
+                       double confidence_interval,
+                       const double* initial_population,
+                       const double* cost_per_animal, double* penalty);
+
+/* Writes the patch-noise normals z(path, year, patch) for paths [0, n_paths),
+ * years [0, n_years), patches [0, n_patches) to out[(path*n_years + year)*
+ * n_patches + patch] (the reference's rgr layout, SimulateGPU.cu:703-705),
+ * computed on the device. */
+int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
+                         uint32_t species, int32_t n_paths, int32_t n_years,
+                         int32_t n_patches, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROADMC_H */

@@ -1,0 +1,98 @@
+/*
+ * CPU oracle for the Monte Carlo road-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker (or
+ * the timed CPU baseline).  The product library libroadmc never links it.
+ *
+ * It restates, in plain C with FP64 (and an FP32 mode), the reference
+ * functions on the path:
+ *   - Simulator::simulateMTEPath + naturalBirthDeath
+ *     (include/main/Simulator.cpp:686-775): the REF_CPU growth law;
+ *   - mteKernel (include/gpuCode/SimulateGPU.cu:595-717): the REF_GPU growth
+ *     law and the pre-growth endpoint;
+ *   - SimulateGPU::dense2Sparse (SimulateGPU.cu:5973-5996): CSC with a drop
+ *     tolerance;
+ *   - Utility::NormalCDFInverse (include/utilities/Utility.cpp:165-193) and the
+ *     penalty of Simulator::simulateMTE (Simulator.cpp:184-212);
+ *   - the price SDEs of forwardPathKernel / expPVPath / singlePathValue
+ *     (SimulateGPU.cu:256-279, 930-952; include/valuation/Uncertainty.cpp:83-112);
+ *   - the regression normal equations and the linear solve of multiQuadReg /
+ *     solveLinearSystem (SimulateGPU.cu:6092-6207, 6297-6358), with correct
+ *     partial pivoting;
+ *   - the adjoint and MLP references used by the gradient tests.
+ *
+ * Random numbers follow the Philox4x32-10 stream contract documented in
+ * transportoptimiser_amd/csrc/rng.h; the Philox here is an independent
+ * implementation, pinned by the Random123 known-answer vectors.
+ *
+ * Pinning status of each function is listed in oracle/README.md.
+ */
+#ifndef ROADMC_ORACLE_H
+#define ROADMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10: out = philox(ctr, key). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]);
+
+/* Patch-noise normal z(path, year, patch) of the stream contract, FP64
+ * transform (fp32 != 0: binary32 transform). */
+double orc_patch_normal(uint64_t seed, uint32_t tag, uint32_t path,
+                        uint32_t year, uint32_t patch, int fp32);
+
+/* Path-scalar draws of kind 1: z_brownian, z_jumpsize, u_jump. */
+void orc_path_scalars(uint64_t seed, uint32_t tag, uint32_t path,
+                      uint32_t year, double out[3]);
+
+/* One species of one road.  Matrices are column-major (Eigen default) with
+ * entry (i, j) = origin i -> destination j. */
+typedef struct {
+  int32_t n_patches;
+  const double* trans;   /* [P*P] */
+  const double* surv;    /* [P*P] or NULL */
+  const double* init_pop;
+  const double* capacity;
+  const double* growth;  /* [P] or NULL */
+  double params[8];      /* SimulateGPU.cu:3690-3714 order */
+  double noise_sd;
+} orc_species;
+
+enum { ORC_LAW_LOGNORMAL = 0, ORC_LAW_REF_CPU = 1, ORC_LAW_REF_GPU = 2 };
+enum { ORC_END_POST = 0, ORC_END_PRE = 1 };
+
+/* Simulates paths [path_begin, path_end) of one species.  Writes each path's
+ * end total to end_totals[path - path_begin] and, when patch_sum is non-NULL,
+ * adds the end vector of every path to patch_sum[P].  Returns 0 on success. */
+int orc_mte_paths(const orc_species* sp, uint32_t species_index,
+                  uint32_t road_id, int32_t n_years, double time_step,
+                  int32_t law, int32_t endpoint, double drop_tol,
+                  uint64_t seed, int64_t path_begin, int64_t path_end,
+                  int fp32, int n_threads, double* end_totals,
+                  double* patch_sum);
+
+/* SimulateGPU::dense2Sparse on a column-major float matrix: keeps entries
+ * strictly greater than tol.  Returns nnz. */
+int32_t orc_dense_to_csc(const float* dense, int32_t rows, int32_t cols,
+                         float tol, float* val, int32_t* row_idx,
+                         int32_t* per_col);
+
+/* Utility::NormalCDFInverse (Abramowitz and Stegun 26.2.23).  Returns NAN
+ * outside (0, 1). */
+double orc_normal_cdf_inverse(double p);
+
+/* Penalty of Simulator::simulateMTE for one species. */
+double orc_mte_penalty(double threshold, double end_mean, double end_sd,
+                       double confidence_interval, double initial_population,
+                       double cost_per_animal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

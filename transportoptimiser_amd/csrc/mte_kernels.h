@@ -1,0 +1,77 @@
+// Host-visible declarations of the MTE kernels (mte_kernels.hip).
+#ifndef ROADMC_MTE_KERNELS_H
+#define ROADMC_MTE_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace roadmc {
+
+// One (road, species) pair of a launch.  Device arrays are of the launch's
+// element type (float for FP32, double for FP64).
+struct MteJob {
+  const void* mpack;   // movement matrix in MFMA B-operand order, Pp*Pp
+  const void* cols;    // 4*Pp: capacity, 1/capacity, growth rate, initial pop
+  double* end_pops;    // [N] per-path end totals, or nullptr
+  double* partial;     // [n_tiles][3 + Pp]: count, sum, sum of squares,
+                       // per-patch sums of one tile
+  double params[8];    // packed species parameters (SimulateGPU.cu:3690-3714)
+  double sigma;        // LOGNORMAL year-noise SD
+  double dt;           // time step
+  double thr_total;    // threshold_ratio * initial total
+  int32_t P, Pp, N, T, BM, law, endpoint, n_tiles;
+  uint32_t tag0, tag1, key0, key1;
+};
+
+constexpr int kMteThreads = 512;  // 8 waves per workgroup
+
+// Element budget BM*Pp of one workgroup's population tile.
+constexpr int kMteTileElemsF32 = 32768;
+constexpr int kMteTileElemsF64 = 16384;
+// Row granularity (MFMA tile height) of the FP32 / FP64 kernels.
+constexpr int kMteRowsF32 = 32;
+constexpr int kMteRowsF64 = 16;
+
+size_t mte_lds_bytes(bool fp64, int BM, int Pp);
+// Packed index of movement entry (k = origin, j = destination).
+size_t mte_pack_index(bool fp64, int Pp, int k, int j);
+
+hipError_t mte_set_lds_limit(bool fp64, size_t bytes);
+hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
+                             int n_work, size_t lds, hipStream_t s);
+hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
+                                 const int2* work, int n_work, int* counter,
+                                 int grid, size_t lds, hipStream_t s);
+hipError_t launch_mte_reduce(const MteJob* jobs, int n_jobs, double* out,
+                             const int64_t* out_off, hipStream_t s);
+hipError_t launch_fill_i32(int* p, int value, int n, hipStream_t s);
+
+// Reference-design baseline (one thread per path, pre-generated streams).
+hipError_t launch_gen_patch_normals(bool fp64, void* out, int N, int T, int P,
+                                    uint32_t tag, uint32_t k0, uint32_t k1,
+                                    hipStream_t s);
+hipError_t launch_gen_path_scalars(bool fp64, void* brown, void* jsize,
+                                   void* jump, int N, int T, uint32_t tag,
+                                   uint32_t k0, uint32_t k1, hipStream_t s);
+struct PlainArgs {
+  int N, T, P, law, endpoint;
+  double dt, sigma;
+  const void* rgr;
+  const void* brown;
+  const void* jsize;
+  const void* jump;
+  const void* params;  // 8
+  const void* init;
+  const void* caps;
+  const void* rates;
+  const void* val;
+  const int* row_idx;
+  const int* per_col;
+  void* path_pops;  // [N][2][P]
+  void* eps;        // [N]
+};
+hipError_t launch_mte_plain(bool fp64, const PlainArgs& a, hipStream_t s);
+
+}  // namespace roadmc
+
+#endif

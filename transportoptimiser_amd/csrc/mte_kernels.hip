@@ -1,0 +1,611 @@
+// MTE Monte Carlo kernels for gfx950 (MI355X).
+//
+// What the reference computes (include/gpuCode/SimulateGPU.cu:595-717,
+// include/main/Simulator.cpp:686-775): for every path and year, the patch
+// population vector n is moved by the movement-and-mortality matrix
+// R = transProbs o survProbs (n' = R^T n, i.e. n'_j = sum_k n_k R[k][j]) and
+// then grown by a stochastic logistic law.  The reference runs one thread per
+// path with a serial CSC loop and reads pre-generated cuRAND arrays.
+//
+// How this file computes it: the paths of one species form an [N x P] state
+// matrix and a year is the product [N x P] . [P x P] followed by an
+// elementwise epilogue.  One workgroup (8 waves) owns BM paths for all T
+// years:
+//   - the BM x P population tile stays in LDS for the whole simulation, with
+//     the k index permuted inside groups of 8 so that a lane's MFMA A operands
+//     for 4 consecutive k-steps are one 16-byte LDS read;
+//   - R is pre-packed in MFMA B-operand order, so each wave's B load is one
+//     coalesced 1 KB read that L2 serves to every workgroup of the road;
+//   - products run on v_mfma_f32_32x32x2_f32 (FP32 accumulate) or, for the
+//     FP64 checker path, v_mfma_f64_16x16x4_f64;
+//   - the epilogue draws the year noise from counter-based Philox in
+//     registers (no random arrays in HBM), applies the growth law, and writes
+//     the new population back to LDS; after the last year it reduces the
+//     tallies in FP64.
+// Growth laws: LOGNORMAL n = n'(1 + r(1 - n'/K)) exp(s z - s^2/2);
+// REF_CPU  n = n'(1 + g(K - n')/K), g = dt (mu + sd z) / 100
+//          (Simulator.cpp:712-728);
+// REF_GPU  the growth-rate SDE of SimulateGPU.cu:686-709.
+#include <hip/hip_runtime.h>
+
+#include "mte_kernels.h"
+#include "rng.h"
+
+namespace roadmc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+enum { kLawLognormal = 0, kLawRefCpu = 1, kLawRefGpu = 2 };
+enum { kEndPost = 0, kEndPre = 1 };
+
+// v_mfma_f32_32x32x2_f32: A lane l = A[l%32][l/32]; B lane l = B[l/32][l%32];
+// D item r of lane l = D[8(r/4) + 4(l/32) + r%4][l%32].
+struct TraitsF32 {
+  using T = float;
+  using VecT = floatx4;
+  using AccT = floatx16;
+  static constexpr int TM = 32, TN = 32, KV = 4, NACC = 16, NT = 4, PAD = 4;
+  static constexpr bool kQuadItems = true;  // items 4q..4q+3 are 4 paths
+  __device__ static AccT mfma(float a, float b, AccT c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int acc_row(int lane, int r) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+  }
+  __device__ static int acc_col(int lane) { return lane & 31; }
+  __device__ static int a_row(int lane) { return lane & 31; }
+  __device__ static int k_lane(int lane) { return lane >> 5; }
+  // k = 8g + 2s + h  ->  8g + 4h + s
+  __host__ __device__ static int perm(int k) {
+    return (k & ~7) | ((k & 1) << 2) | ((k >> 1) & 3);
+  }
+  __device__ static float expo(float x) { return __expf(x); }
+  __device__ static void normal4(uint32_t q, uint32_t y, uint32_t p,
+                                 uint32_t tag, uint32_t k0, uint32_t k1,
+                                 float z[4]) {
+    rmc_normal4_f(q, y, p, tag, k0, k1, z);
+  }
+};
+
+// v_mfma_f64_16x16x4_f64: A lane l = A[l%16][l/16]; B lane l = B[l/16][l%16];
+// D item r of lane l = D[l/16 + 4r][l%16].
+struct TraitsF64 {
+  using T = double;
+  using VecT = doublex2;
+  using AccT = doublex4;
+  static constexpr int TM = 16, TN = 16, KV = 2, NACC = 4, NT = 8, PAD = 2;
+  static constexpr bool kQuadItems = false;
+  __device__ static AccT mfma(double a, double b, AccT c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int acc_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  __device__ static int acc_col(int lane) { return lane & 15; }
+  __device__ static int a_row(int lane) { return lane & 15; }
+  __device__ static int k_lane(int lane) { return lane >> 4; }
+  // k = 8g + 4s + h  ->  8g + 2h + s
+  __host__ __device__ static int perm(int k) {
+    return (k & ~7) | ((k & 3) << 1) | ((k >> 2) & 1);
+  }
+  __device__ static double expo(double x) { return exp(x); }
+  __device__ static void normal4(uint32_t q, uint32_t y, uint32_t p,
+                                 uint32_t tag, uint32_t k0, uint32_t k1,
+                                 double z[4]) {
+    rmc_normal4_d(q, y, p, tag, k0, k1, z);
+  }
+};
+
+__host__ __device__ static inline size_t align16(size_t x) {
+  return (x + 15) & ~(size_t)15;
+}
+
+template <class TR>
+__host__ __device__ static inline size_t lds_bytes_t(int BM, int Pp) {
+  using T = typename TR::T;
+  return align16((size_t)BM * (Pp + TR::PAD) * sizeof(T)) +
+         align16((size_t)BM * sizeof(T)) + 8 * 4 * sizeof(double);
+}
+
+size_t mte_lds_bytes(bool fp64, int BM, int Pp) {
+  return fp64 ? lds_bytes_t<TraitsF64>(BM, Pp) : lds_bytes_t<TraitsF32>(BM, Pp);
+}
+
+size_t mte_pack_index(bool fp64, int Pp, int k, int j) {
+  // Element (k, j) lands where lane l of the wave handling column tile
+  // j / TN reads it as B operand s of k-group k / 8.
+  const int kg = k >> 3, kk = k & 7;
+  const int nkg = Pp >> 3;
+  if (!fp64) {
+    const int s = kk >> 1, h = kk & 1;
+    const int lane = h * 32 + (j & 31), cb = j >> 5;
+    return (((size_t)cb * nkg + kg) * 64 + lane) * 4 + s;
+  }
+  const int s = kk >> 2, h = kk & 3;
+  const int lane = h * 16 + (j & 15), cb = j >> 4;
+  return (((size_t)cb * nkg + kg) * 64 + lane) * 2 + s;
+}
+
+template <typename T>
+__device__ __forceinline__ T grow(int law, T x, T z, T K, T Kinv, T r, T sig,
+                                  T half_var, T dt, T mu, T sd, T grm, T lv) {
+  if (law == kLawRefCpu) {
+    const T g = dt * (mu + sd * z) * (T)0.01;
+    return x * ((T)1 + g * (K - x) * Kinv);
+  }
+  if (law == kLawRefGpu) {
+    const T g = lv * z * grm + grm;
+    return x * ((T)1 + g * (K - x) * Kinv);
+  }
+  T e;
+  if constexpr (sizeof(T) == 4)
+    e = __expf(sig * z - half_var);
+  else
+    e = exp(sig * z - half_var);
+  return x * ((T)1 + r * ((T)1 - x * Kinv)) * e;
+}
+
+template <typename T>
+__device__ __forceinline__ void path_scalars(uint32_t path, uint32_t year,
+                                             uint32_t tag, uint32_t k0,
+                                             uint32_t k1, T* zb, T* zj,
+                                             T* uj) {
+  const rmc_u4 x = rmc_philox(path, year, 0u, tag, k0, k1);
+  if constexpr (sizeof(T) == 4) {
+    rmc_box_muller_f(rmc_u01f(x.x), rmc_u01f(x.y), zb, zj);
+    *uj = rmc_u01f(x.z);
+  } else {
+    rmc_box_muller_d(rmc_u01d(x.x), rmc_u01d(x.y), zb, zj);
+    *uj = rmc_u01d(x.z);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Simulates one tile (BM paths of one job) for all years and writes the
+// tile's tallies.  Every thread of the workgroup calls it.
+template <class TR>
+__device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
+                                         char* smem) {
+  using T = typename TR::T;
+  using VecT = typename TR::VecT;
+  using AccT = typename TR::AccT;
+  constexpr int TM = TR::TM, TN = TR::TN, KV = TR::KV, NACC = TR::NACC,
+                NT = TR::NT;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // Wave index as a scalar: the per-wave tile loops below then branch on
+  // SGPRs instead of exec masks.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nthreads = blockDim.x, nwaves = nthreads >> 6;
+  const int BM = jb.BM, Pp = jb.Pp, LDA = Pp + TR::PAD;
+  const int p0 = tile * BM;
+
+  T* A = reinterpret_cast<T*>(smem);
+  T* grm = reinterpret_cast<T*>(smem + align16((size_t)BM * LDA * sizeof(T)));
+  double* red = reinterpret_cast<double*>(
+      smem + align16((size_t)BM * LDA * sizeof(T)) +
+      align16((size_t)BM * sizeof(T)));
+
+  const T* cols = reinterpret_cast<const T*>(jb.cols);
+  const T* capK = cols;
+  const T* capKinv = cols + Pp;
+  const T* rate = cols + 2 * Pp;
+  const T* init = cols + 3 * Pp;
+
+  const T mu = (T)jb.params[1], sd = (T)jb.params[2], rev = (T)jb.params[4];
+  const T jmp = (T)jb.params[5], jprob = (T)jb.params[6], lv = (T)jb.params[7];
+  const T dt = (T)jb.dt, sig = (T)jb.sigma, half_var = (T)0.5 * sig * sig;
+  const int law = jb.law;
+
+  for (int idx = tid; idx < BM * Pp; idx += nthreads) {
+    const int r = idx / Pp, c = idx - r * Pp;
+    A[r * LDA + TR::perm(c)] = init[c];
+  }
+  for (int r = tid; r < BM; r += nthreads) grm[r] = (T)jb.params[0];
+  __syncthreads();
+
+  // Output tiles of this wave: a contiguous run of the BM/TM x Pp/TN tiles.
+  const int ntc = Pp / TN;
+  const int ntiles = (BM / TM) * ntc;
+  const int base = ntiles / nwaves, extra = ntiles % nwaves;
+  const int my_n = base + (wave < extra ? 1 : 0);
+  const int my_t0 = wave * base + (wave < extra ? wave : extra);
+  int trow[NT], tcol[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int ti = my_t0 + t;
+    trow[t] = ti / ntc;
+    tcol[t] = ti - (ti / ntc) * ntc;
+  }
+
+  const int nkg = Pp >> 3;
+  const VecT* Bp = reinterpret_cast<const VecT*>(jb.mpack);
+  const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
+
+  AccT acc[NT];
+  for (int year = 0; year < jb.T; ++year) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = AccT{};
+
+    VecT bcur[NT], bnxt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      if (t < my_n) bcur[t] = Bp[((size_t)tcol[t] * nkg) * 64 + lane];
+    for (int kg = 0; kg < nkg; ++kg) {
+      if (kg + 1 < nkg) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          if (t < my_n)
+            bnxt[t] = Bp[((size_t)tcol[t] * nkg + kg + 1) * 64 + lane];
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if (t < my_n) {
+          const VecT a = *reinterpret_cast<const VecT*>(
+              A + (trow[t] * TM + arow) * LDA + kg * 8 + akoff);
+#pragma unroll
+          for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a[s], bcur[t][s], acc[t]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bcur[t] = bnxt[t];
+    }
+    __syncthreads();  // every wave has finished reading this year's state
+
+    if (law == kLawRefGpu) {
+      // Growth-rate SDE per path (SimulateGPU.cu:686-701).
+      for (int r = tid; r < BM; r += nthreads) {
+        T zb, zj, uj;
+        path_scalars<T>((uint32_t)(p0 + r), (uint32_t)year, jb.tag1, jb.key0,
+                        jb.key1, &zb, &zj, &uj);
+        const T g = grm[r];
+        const T jump = (uj < jprob) ? (T)1 : (T)0;
+        const T brown = zb * sd;
+        const T jsz = zj * jmp * jmp - jmp * jmp / (T)2;
+        grm[r] = g + rev * (mu - g) * dt + g * brown +
+                 (TR::expo(jsz) - (T)1) * g * jump;
+      }
+      __syncthreads();
+    }
+
+    const bool keep_pre = (year == jb.T - 1) && (jb.endpoint == kEndPre);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (t < my_n) {
+        const int col = tcol[t] * TN + TR::acc_col(lane);
+        const int pcol = TR::perm(col);
+        const T K = capK[col], Ki = capKinv[col], rr = rate[col];
+#pragma unroll
+        for (int r4 = 0; r4 < NACC; r4 += 4) {
+          T z[4];
+          if constexpr (TR::kQuadItems) {
+            const int row0 = trow[t] * TM + TR::acc_row(lane, r4);
+            TR::normal4((uint32_t)(p0 + row0) >> 2, (uint32_t)year,
+                        (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
+          }
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int row = trow[t] * TM + TR::acc_row(lane, r4 + s);
+            T zz;
+            if constexpr (TR::kQuadItems) {
+              zz = z[s];
+            } else {
+              const uint32_t path = (uint32_t)(p0 + row);
+              TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0,
+                          jb.key0, jb.key1, z);
+              zz = z[path & 3u];
+            }
+            const T x = acc[t][r4 + s];
+            const T n = grow<T>(law, x, zz, K, Ki, rr, sig, half_var, dt, mu,
+                                sd, grm[row], lv);
+            A[row * LDA + pcol] = keep_pre ? x : n;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Tallies in FP64: per-path totals, exceedance count, sum, sum of squares.
+  double cnt = 0.0, sum = 0.0, sq = 0.0;
+  for (int r = tid; r < BM; r += nthreads) {
+    const int path = p0 + r;
+    if (path < jb.N) {
+      double tot = 0.0;
+      for (int c = 0; c < Pp; ++c) tot += (double)A[r * LDA + c];
+      if (jb.end_pops) jb.end_pops[path] = tot;
+      cnt += (tot >= jb.thr_total) ? 1.0 : 0.0;
+      sum += tot;
+      sq += tot * tot;
+    }
+  }
+  cnt = wave_sum(cnt);
+  sum = wave_sum(sum);
+  sq = wave_sum(sq);
+  if (lane == 0) {
+    red[wave * 3 + 0] = cnt;
+    red[wave * 3 + 1] = sum;
+    red[wave * 3 + 2] = sq;
+  }
+  __syncthreads();
+  double* part = jb.partial + (size_t)tile * (3 + Pp);
+  if (tid < 3) {
+    double v = 0.0;
+    for (int w = 0; w < nwaves; ++w) v += red[w * 3 + tid];
+    part[tid] = v;
+  }
+  const int nvalid = min(BM, jb.N - p0);
+  for (int c = tid; c < Pp; c += nthreads) {
+    const int pc = TR::perm(c);
+    double s = 0.0;
+    for (int r = 0; r < nvalid; ++r) s += (double)A[r * LDA + pc];
+    part[3 + c] = s;
+  }
+  __syncthreads();  // LDS is reused by the next tile
+}
+
+template <class TR>
+__global__ __launch_bounds__(kMteThreads) void mte_direct_kernel(
+    const MteJob* __restrict__ jobs, const int2* __restrict__ work) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int2 w = work[blockIdx.x];
+  mte_tile<TR>(jobs[w.x], w.y, smem);
+}
+
+// Persistent form: a grid sized to the machine pulls (job, tile) items from a
+// queue ordered by decreasing cost.  Every workgroup leaves the loop once the
+// counter passes n_work.
+template <class TR>
+__global__ __launch_bounds__(kMteThreads) void mte_persistent_kernel(
+    const MteJob* __restrict__ jobs, const int2* __restrict__ work, int n_work,
+    int* __restrict__ counter) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = next;
+    __syncthreads();
+    if (w >= n_work) break;
+    const int2 it = work[w];
+    mte_tile<TR>(jobs[it.x], it.y, smem);
+  }
+}
+
+// Sums the tile partials of each job in a fixed order (deterministic).
+__global__ void mte_reduce_kernel(const MteJob* __restrict__ jobs,
+                                  double* __restrict__ out,
+                                  const int64_t* __restrict__ out_off) {
+  const MteJob& jb = jobs[blockIdx.x];
+  const int W = 3 + jb.Pp;
+  double* o = out + out_off[blockIdx.x];
+  for (int e = threadIdx.x; e < W; e += blockDim.x) {
+    double s = 0.0;
+    for (int t = 0; t < jb.n_tiles; ++t) s += jb.partial[(size_t)t * W + e];
+    o[e] = s;
+  }
+}
+
+__global__ void fill_i32_kernel(int* p, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t mte_set_lds_limit(bool fp64, size_t bytes) {
+  hipError_t e;
+  if (fp64) {
+    e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&mte_direct_kernel<TraitsF64>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&mte_persistent_kernel<TraitsF64>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  }
+  e = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&mte_direct_kernel<TraitsF32>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&mte_persistent_kernel<TraitsF32>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
+                             int n_work, size_t lds, hipStream_t s) {
+  if (n_work <= 0) return hipSuccess;
+  if (fp64)
+    hipLaunchKernelGGL(mte_direct_kernel<TraitsF64>, dim3(n_work),
+                       dim3(kMteThreads), lds, s, jobs, work);
+  else
+    hipLaunchKernelGGL(mte_direct_kernel<TraitsF32>, dim3(n_work),
+                       dim3(kMteThreads), lds, s, jobs, work);
+  return hipGetLastError();
+}
+
+hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
+                                 const int2* work, int n_work, int* counter,
+                                 int grid, size_t lds, hipStream_t s) {
+  if (n_work <= 0) return hipSuccess;
+  if (fp64)
+    hipLaunchKernelGGL(mte_persistent_kernel<TraitsF64>, dim3(grid),
+                       dim3(kMteThreads), lds, s, jobs, work, n_work, counter);
+  else
+    hipLaunchKernelGGL(mte_persistent_kernel<TraitsF32>, dim3(grid),
+                       dim3(kMteThreads), lds, s, jobs, work, n_work, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_mte_reduce(const MteJob* jobs, int n_jobs, double* out,
+                             const int64_t* out_off, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mte_reduce_kernel, dim3(n_jobs), dim3(256), 0, s, jobs,
+                     out, out_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, int value, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                     p, value, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reference-design baseline.  These kernels keep the structure of mteKernel
+// (SimulateGPU.cu:595-717) and of its host driver (SimulateGPU.cu:3720-3843):
+// the random streams are written to HBM first (there by curandGenerate*), then
+// one thread per path runs the years with its state in a path-major
+// [N][2][P] ping-pong buffer and a serial CSC loop.  They draw the same
+// Philox numbers as the tiled kernels.
+
+template <typename T>
+__global__ void gen_patch_normals_kernel(T* __restrict__ out, int N, int Ty,
+                                         int P, uint32_t tag, uint32_t k0,
+                                         uint32_t k1) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nq = (N + 3) / 4;
+  if (idx >= nq * Ty * P) return;
+  const int j = (int)(idx % P);
+  const int64_t rest = idx / P;
+  const int t = (int)(rest % Ty);
+  const int q = (int)(rest / Ty);
+  T z[4];
+  if constexpr (sizeof(T) == 4)
+    rmc_normal4_f((uint32_t)q, (uint32_t)t, (uint32_t)j, tag, k0, k1, z);
+  else
+    rmc_normal4_d((uint32_t)q, (uint32_t)t, (uint32_t)j, tag, k0, k1, z);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int path = 4 * q + s;
+    if (path < N) out[((int64_t)path * Ty + t) * P + j] = z[s];
+  }
+}
+
+template <typename T>
+__global__ void gen_path_scalars_kernel(T* __restrict__ brown,
+                                        T* __restrict__ jsize,
+                                        T* __restrict__ jump, int N, int Ty,
+                                        uint32_t tag, uint32_t k0,
+                                        uint32_t k1) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)N * Ty) return;
+  const int t = (int)(idx % Ty);
+  const int path = (int)(idx / Ty);
+  T zb, zj, uj;
+  path_scalars<T>((uint32_t)path, (uint32_t)t, tag, k0, k1, &zb, &zj, &uj);
+  brown[idx] = zb;
+  jsize[idx] = zj;
+  jump[idx] = uj;
+}
+
+template <typename T>
+__global__ void mte_plain_kernel(PlainArgs a) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ii >= a.N) return;
+  const int P = a.P, Ty = a.T;
+  const T* rgr = static_cast<const T*>(a.rgr);
+  const T* prm = static_cast<const T*>(a.params);
+  const T* init = static_cast<const T*>(a.init);
+  const T* caps = static_cast<const T*>(a.caps);
+  const T* rates = static_cast<const T*>(a.rates);
+  const T* val = static_cast<const T*>(a.val);
+  T* cur = static_cast<T*>(a.path_pops) + (size_t)ii * 2 * P;
+  T* nxt = cur + P;
+  const T dt = (T)a.dt, sig = (T)a.sigma, half_var = (T)0.5 * sig * sig;
+  for (int j = 0; j < P; ++j) cur[j] = init[j];
+  T grm = prm[0];
+  for (int t = 0; t < Ty; ++t) {
+    for (int k = 0; k < P; ++k) nxt[k] = (T)0;
+    int it = 0;
+    for (int k = 0; k < P; ++k) {
+      for (int l = 0; l < a.per_col[k]; ++l) {
+        nxt[k] += cur[a.row_idx[it]] * val[it];
+        ++it;
+      }
+    }
+    if (a.law == kLawRefGpu) {
+      const size_t ix = (size_t)ii * Ty + t;
+      const T u = static_cast<const T*>(a.jump)[ix];
+      const T jump = (u < prm[6]) ? (T)1 : (T)0;
+      const T brown = static_cast<const T*>(a.brown)[ix] * prm[2];
+      const T jsz = static_cast<const T*>(a.jsize)[ix] * prm[5] * prm[5] -
+                    prm[5] * prm[5] / (T)2;
+      T e;
+      if constexpr (sizeof(T) == 4)
+        e = __expf(jsz);
+      else
+        e = exp(jsz);
+      grm = grm + prm[4] * (prm[1] - grm) * dt + grm * brown +
+            (e - (T)1) * grm * jump;
+    }
+    for (int k = 0; k < P; ++k) {
+      const T z = rgr[((size_t)ii * Ty + t) * P + k];
+      const T x = nxt[k];
+      cur[k] = grow<T>(a.law, x, z, caps[k], (T)1 / caps[k], rates[k], sig,
+                       half_var, dt, prm[1], prm[2], grm, prm[7]);
+    }
+  }
+  const T* v = (a.endpoint == kEndPre) ? nxt : cur;
+  T tot = (T)0;
+  for (int j = 0; j < P; ++j) tot += v[j];
+  static_cast<T*>(a.eps)[ii] = tot;
+}
+
+hipError_t launch_gen_patch_normals(bool fp64, void* out, int N, int T, int P,
+                                    uint32_t tag, uint32_t k0, uint32_t k1,
+                                    hipStream_t s) {
+  const int64_t total = (int64_t)((N + 3) / 4) * T * P;
+  if (total <= 0) return hipSuccess;
+  const int64_t blocks = (total + 255) / 256;
+  if (fp64)
+    hipLaunchKernelGGL(gen_patch_normals_kernel<double>, dim3((unsigned)blocks),
+                       dim3(256), 0, s, static_cast<double*>(out), N, T, P,
+                       tag, k0, k1);
+  else
+    hipLaunchKernelGGL(gen_patch_normals_kernel<float>, dim3((unsigned)blocks),
+                       dim3(256), 0, s, static_cast<float*>(out), N, T, P, tag,
+                       k0, k1);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_path_scalars(bool fp64, void* brown, void* jsize,
+                                   void* jump, int N, int T, uint32_t tag,
+                                   uint32_t k0, uint32_t k1, hipStream_t s) {
+  const int64_t total = (int64_t)N * T;
+  if (total <= 0) return hipSuccess;
+  const int64_t blocks = (total + 255) / 256;
+  if (fp64)
+    hipLaunchKernelGGL(gen_path_scalars_kernel<double>, dim3((unsigned)blocks),
+                       dim3(256), 0, s, static_cast<double*>(brown),
+                       static_cast<double*>(jsize), static_cast<double*>(jump),
+                       N, T, tag, k0, k1);
+  else
+    hipLaunchKernelGGL(gen_path_scalars_kernel<float>, dim3((unsigned)blocks),
+                       dim3(256), 0, s, static_cast<float*>(brown),
+                       static_cast<float*>(jsize), static_cast<float*>(jump),
+                       N, T, tag, k0, k1);
+  return hipGetLastError();
+}
+
+hipError_t launch_mte_plain(bool fp64, const PlainArgs& a, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  // The reference launches min(N, maxThreadsPerBlock) threads per block
+  // (SimulateGPU.cu:3824-3833).
+  const int tpb = a.N < 1024 ? a.N : 1024;
+  const int blocks = (a.N + tpb - 1) / tpb;
+  if (fp64)
+    hipLaunchKernelGGL(mte_plain_kernel<double>, dim3(blocks), dim3(tpb), 0, s,
+                       a);
+  else
+    hipLaunchKernelGGL(mte_plain_kernel<float>, dim3(blocks), dim3(tpb), 0, s,
+                       a);
+  return hipGetLastError();
+}
+
+}  // namespace roadmc

@@ -1,0 +1,760 @@
+// C-ABI host code for the MTE path: problem validation, packing of the
+// movement matrices into MFMA operand order, launch of the tiled kernels, the
+// batched/graph path and the reference-design baseline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "mte_kernels.h"
+#include "rng.h"
+#include "roadmc.h"
+
+using roadmc::MteJob;
+
+namespace {
+
+constexpr int kBuckets = 4;
+const int kBucketMax[kBuckets] = {128, 256, 512, 1024};
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int bucket_of(int Pp) {
+  for (int b = 0; b < kBuckets; ++b)
+    if (Pp <= kBucketMax[b]) return b;
+  return kBuckets - 1;
+}
+
+bool finite_array(const double* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr) {
+  if (!pr) return rmc_fail(ctx, ROADMC_ERR_INVALID, "problem is NULL");
+  if (pr->n_species < 1 || pr->n_species > 63 || !pr->species)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_species must be in [1, 63]");
+  if (pr->n_years < 1)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_years must be >= 1");
+  if (pr->n_paths < 1 || pr->n_paths > 0x7fffffffLL)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_paths must be in [1, 2^31)");
+  if (pr->growth_law < ROADMC_LAW_LOGNORMAL || pr->growth_law > ROADMC_LAW_REF_GPU)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown growth_law");
+  if (pr->endpoint != ROADMC_END_POST_GROWTH && pr->endpoint != ROADMC_END_PRE_GROWTH)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown endpoint");
+  if (pr->precision != ROADMC_PREC_FP32 && pr->precision != ROADMC_PREC_FP64)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown precision");
+  if (!(pr->drop_tol >= 0.0) || !std::isfinite(pr->time_step))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "drop_tol/time_step invalid");
+  if (pr->road_id >= (1u << 24))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "road_id must be < 2^24");
+  for (int s = 0; s < pr->n_species; ++s) {
+    const roadmc_species& sp = pr->species[s];
+    const int P = sp.n_patches;
+    if (P < 1 || P > ROADMC_MAX_PATCHES)
+      return rmc_fail(ctx, ROADMC_ERR_INVALID,
+                      "n_patches must be in [1, ROADMC_MAX_PATCHES]");
+    if (!sp.init_pop || !sp.capacity || !sp.trans)
+      return rmc_fail(ctx, ROADMC_ERR_INVALID,
+                      "init_pop, capacity and trans are required");
+    for (int j = 0; j < P; ++j)
+      if (!(sp.capacity[j] > 0.0) || !std::isfinite(sp.capacity[j]))
+        return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "capacity must be finite and > 0");
+    if (!finite_array(sp.init_pop, P) || !finite_array(sp.trans, (size_t)P * P) ||
+        (sp.surv && !finite_array(sp.surv, (size_t)P * P)) ||
+        (sp.growth && !finite_array(sp.growth, P)) || !finite_array(sp.params, 8) ||
+        !std::isfinite(sp.noise_sd) || !std::isfinite(sp.threshold_ratio))
+      return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite input");
+  }
+  return ROADMC_OK;
+}
+
+// Movement-and-mortality entry (origin k, destination j) as the reference
+// forms it: T o S (SimulateGPU.cu:3783-3787), with dense2Sparse's drop rule
+// applied to the binary32 value (SimulateGPU.cu:5985).
+inline double movement(const roadmc_species& sp, int P, int k, int j, double tol) {
+  const size_t cm = (size_t)j * P + k;
+  double v = sp.trans[cm] * (sp.surv ? sp.surv[cm] : 1.0);
+  if (tol > 0.0 && !((float)v > (float)tol)) v = 0.0;
+  return v;
+}
+
+template <typename T>
+void pack_species(const roadmc_species& sp, int Pp, double tol, bool fp64, T* mp,
+                  T* cols) {
+  const int P = sp.n_patches;
+  std::fill(mp, mp + (size_t)Pp * Pp, (T)0);
+  for (int k = 0; k < P; ++k)
+    for (int j = 0; j < P; ++j)
+      mp[roadmc::mte_pack_index(fp64, Pp, k, j)] = (T)movement(sp, P, k, j, tol);
+  for (int j = 0; j < Pp; ++j) {
+    const bool in = j < P;
+    cols[j] = in ? (T)sp.capacity[j] : (T)1;
+    cols[Pp + j] = in ? (T)(1.0 / sp.capacity[j]) : (T)1;
+    cols[2 * Pp + j] = (in && sp.growth) ? (T)sp.growth[j] : (T)0;
+    cols[3 * Pp + j] = in ? (T)sp.init_pop[j] : (T)0;
+  }
+}
+
+int rows_for(const roadmc_ctx* ctx, bool fp64, int Pp, int64_t N) {
+  if (fp64) {
+    int bm = (roadmc::kMteTileElemsF64 / Pp) / roadmc::kMteRowsF64 * roadmc::kMteRowsF64;
+    bm = std::max(roadmc::kMteRowsF64, std::min(bm, 256));
+    const int cap = round_up((int)std::min<int64_t>(N, 256), roadmc::kMteRowsF64);
+    return std::min(bm, cap);
+  }
+  int bm = ctx->tile_rows[bucket_of(Pp)];
+  const int lim = (roadmc::kMteTileElemsF32 / Pp) / roadmc::kMteRowsF32 * roadmc::kMteRowsF32;
+  bm = std::max(roadmc::kMteRowsF32, std::min(bm, lim));
+  const int cap = round_up((int)std::min<int64_t>(N, 256), roadmc::kMteRowsF32);
+  return std::min(bm, cap);
+}
+
+// Host-side description of a set of (road, species) jobs, shared by the
+// single-road and batched entry points.
+struct JobSet {
+  bool fp64 = false;
+  std::vector<MteJob> jobs;
+  std::vector<int> road_of, species_of;
+  std::vector<int64_t> mpack_off, cols_off, part_off, out_off, endpop_off;
+  std::vector<int2> work;
+  size_t mpack_elems = 0, cols_elems = 0, part_elems = 0, out_elems = 0,
+         endpop_elems = 0;
+  size_t lds = 0;
+};
+
+void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roads,
+               bool want_end_pops, JobSet& js) {
+  js.fp64 = roads[0].precision == ROADMC_PREC_FP64;
+  for (int r = 0; r < n_roads; ++r) {
+    const roadmc_mte_problem& pr = roads[r];
+    for (int s = 0; s < pr.n_species; ++s) {
+      const roadmc_species& sp = pr.species[s];
+      MteJob jb{};
+      jb.P = sp.n_patches;
+      jb.Pp = round_up(jb.P, 32);
+      jb.N = (int32_t)pr.n_paths;
+      jb.T = pr.n_years;
+      jb.BM = rows_for(ctx, js.fp64, jb.Pp, pr.n_paths);
+      jb.law = pr.growth_law;
+      jb.endpoint = pr.endpoint;
+      jb.n_tiles = (int)((pr.n_paths + jb.BM - 1) / jb.BM);
+      for (int i = 0; i < 8; ++i) jb.params[i] = sp.params[i];
+      jb.sigma = sp.noise_sd;
+      jb.dt = pr.time_step;
+      double tot0 = 0.0;
+      for (int j = 0; j < jb.P; ++j) tot0 += sp.init_pop[j];
+      jb.thr_total = sp.threshold_ratio * tot0;
+      jb.tag0 = rmc_tag(pr.road_id, (uint32_t)s, RMC_KIND_PATCH);
+      jb.tag1 = rmc_tag(pr.road_id, (uint32_t)s, RMC_KIND_PATH);
+      jb.key0 = (uint32_t)(pr.seed & 0xffffffffu);
+      jb.key1 = (uint32_t)(pr.seed >> 32);
+      js.mpack_off.push_back((int64_t)js.mpack_elems);
+      js.cols_off.push_back((int64_t)js.cols_elems);
+      js.part_off.push_back((int64_t)js.part_elems);
+      js.out_off.push_back((int64_t)js.out_elems);
+      js.endpop_off.push_back(want_end_pops ? (int64_t)js.endpop_elems : -1);
+      js.mpack_elems += (size_t)jb.Pp * jb.Pp;
+      js.cols_elems += 4 * (size_t)jb.Pp;
+      js.part_elems += (size_t)jb.n_tiles * (3 + jb.Pp);
+      js.out_elems += 3 + (size_t)jb.Pp;
+      if (want_end_pops) js.endpop_elems += (size_t)jb.N;
+      js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.fp64, jb.BM, jb.Pp));
+      js.jobs.push_back(jb);
+      js.road_of.push_back(r);
+      js.species_of.push_back(s);
+    }
+  }
+  // Work items, most expensive tiles first (cost ~ BM * Pp^2 * T).
+  std::vector<int> order(js.jobs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const MteJob &x = js.jobs[a], &y = js.jobs[b];
+    return (double)x.BM * x.Pp * x.Pp * x.T > (double)y.BM * y.Pp * y.Pp * y.T;
+  });
+  for (int j : order)
+    for (int t = 0; t < js.jobs[j].n_tiles; ++t) js.work.push_back(make_int2(j, t));
+}
+
+// Device buffers of one job set.
+using JobBuffers = rmc_job_buffers;
+
+template <typename T>
+int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
+                JobBuffers& b, bool persistent_bufs) {
+  const size_t es = sizeof(T);
+  int rc;
+  auto grow = [&](rmc_devbuf& buf, size_t bytes) {
+    return persistent_bufs ? rmc_buf_alloc_exact(ctx, buf, bytes)
+                           : rmc_buf_ensure(ctx, buf, bytes);
+  };
+  if ((rc = grow(b.mpack, js.mpack_elems * es)) ||
+      (rc = grow(b.cols, js.cols_elems * es)) ||
+      (rc = grow(b.partial, js.part_elems * sizeof(double))) ||
+      (rc = grow(b.out, js.out_elems * sizeof(double))) ||
+      (rc = grow(b.endpop, std::max<size_t>(js.endpop_elems, 1) * sizeof(double))) ||
+      (rc = grow(b.jobs, js.jobs.size() * sizeof(MteJob))) ||
+      (rc = grow(b.work, js.work.size() * sizeof(int2))) ||
+      (rc = grow(b.off, js.jobs.size() * sizeof(int64_t))) ||
+      (rc = grow(b.counter, 64)))
+    return rc;
+  std::vector<T> mp(js.mpack_elems), cols(js.cols_elems);
+  for (size_t j = 0; j < js.jobs.size(); ++j) {
+    const roadmc_mte_problem& pr = roads[js.road_of[j]];
+    pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, pr.drop_tol, js.fp64,
+                    mp.data() + js.mpack_off[j], cols.data() + js.cols_off[j]);
+    MteJob& jb = js.jobs[j];
+    jb.mpack = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];
+    jb.cols = static_cast<T*>(b.cols.ptr) + js.cols_off[j];
+    jb.partial = static_cast<double*>(b.partial.ptr) + js.part_off[j];
+    jb.end_pops = js.endpop_off[j] >= 0
+                      ? static_cast<double*>(b.endpop.ptr) + js.endpop_off[j]
+                      : nullptr;
+  }
+  hipStream_t s = ctx->stream;
+  RMC_HIP(ctx, hipMemcpyAsync(b.mpack.ptr, mp.data(), mp.size() * es,
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(b.cols.ptr, cols.data(), cols.size() * es,
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(b.jobs.ptr, js.jobs.data(), js.jobs.size() * sizeof(MteJob),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(b.work.ptr, js.work.data(), js.work.size() * sizeof(int2),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(b.off.ptr, js.out_off.data(),
+                              js.out_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  // The host vectors die at return: wait for the copies.
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  return ROADMC_OK;
+}
+
+int ensure_lds_limit(roadmc_ctx* ctx, bool fp64, size_t lds) {
+  size_t& cur = fp64 ? ctx->lds_limit_f64 : ctx->lds_limit_f32;
+  if (lds > cur) {
+    RMC_HIP(ctx, roadmc::mte_set_lds_limit(fp64, lds));
+    cur = lds;
+  }
+  return ROADMC_OK;
+}
+
+// Writes caller outputs from the reduced per-job sums; nothing is written
+// before every device step has succeeded.
+void write_results(const JobSet& js, const std::vector<double>& out,
+                   const std::vector<double>& endpop, const roadmc_mte_problem* roads,
+                   roadmc_mte_result* results) {
+  std::vector<int64_t> patch_base(js.jobs.size());
+  {
+    int last_road = -1;
+    int64_t acc = 0;
+    for (size_t j = 0; j < js.jobs.size(); ++j) {
+      if (js.road_of[j] != last_road) {
+        acc = 0;
+        last_road = js.road_of[j];
+      }
+      patch_base[j] = acc;
+      acc += js.jobs[j].P;
+    }
+  }
+  for (size_t j = 0; j < js.jobs.size(); ++j) {
+    const MteJob& jb = js.jobs[j];
+    const roadmc_mte_result& res = results[js.road_of[j]];
+    const int s = js.species_of[j];
+    const double* o = out.data() + js.out_off[j];
+    const double N = (double)jb.N;
+    const double mean = o[1] / N;
+    const double var = std::max(0.0, o[2] / N - mean * mean);
+    if (res.mte_prob) res.mte_prob[s] = o[0] / N;
+    if (res.end_mean) res.end_mean[s] = mean;
+    if (res.end_sd) res.end_sd[s] = std::sqrt(var);
+    if (res.patch_mean)
+      for (int p = 0; p < jb.P; ++p) res.patch_mean[patch_base[j] + p] = o[3 + p] / N;
+    if (res.end_pops && js.endpop_off[j] >= 0)
+      std::memcpy(res.end_pops + (size_t)s * jb.N, endpop.data() + js.endpop_off[j],
+                  sizeof(double) * jb.N);
+  }
+  (void)roads;
+}
+
+int fetch_results(roadmc_ctx* ctx, const JobSet& js, JobBuffers& b,
+                  const roadmc_mte_problem* roads, roadmc_mte_result* results) {
+  std::vector<double> out(js.out_elems), endpop(js.endpop_elems);
+  hipStream_t s = ctx->stream;
+  RMC_HIP(ctx, hipMemcpyAsync(out.data(), b.out.ptr, out.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  if (!endpop.empty())
+    RMC_HIP(ctx, hipMemcpyAsync(endpop.data(), b.endpop.ptr, endpop.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  for (double v : out)
+    if (!std::isfinite(v))
+      return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite population (diverging growth law?)");
+  write_results(js, out, endpop, roads, results);
+  return ROADMC_OK;
+}
+
+bool wants_end_pops(const roadmc_mte_result* results, int n) {
+  for (int i = 0; i < n; ++i)
+    if (results[i].end_pops) return true;
+  return false;
+}
+
+}  // namespace
+
+extern "C" int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
+                              roadmc_mte_result* result) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!result) return rmc_fail(ctx, ROADMC_ERR_INVALID, "result is NULL");
+  int rc = validate_problem(ctx, problem);
+  if (rc) return rc;
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  JobSet js;
+  plan_jobs(ctx, problem, 1, result->end_pops != nullptr, js);
+  JobBuffers& b = ctx->mte_bufs;
+  rc = js.fp64 ? upload_jobs<double>(ctx, problem, js, b, false)
+               : upload_jobs<float>(ctx, problem, js, b, false);
+  if (rc) return rc;
+  if ((rc = ensure_lds_limit(ctx, js.fp64, js.lds))) return rc;
+  RMC_HIP(ctx, roadmc::launch_mte_direct(js.fp64, static_cast<MteJob*>(b.jobs.ptr),
+                                         static_cast<int2*>(b.work.ptr), (int)js.work.size(),
+                                         js.lds, ctx->stream));
+  RMC_HIP(ctx, roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
+                                         static_cast<double*>(b.out.ptr),
+                                         static_cast<int64_t*>(b.off.ptr), ctx->stream));
+  return fetch_results(ctx, js, b, problem, result);
+}
+
+// ---------------------------------------------------------------------------
+// Batched generation.
+
+struct roadmc_batch {
+  roadmc_ctx* ctx = nullptr;
+  JobSet js;
+  JobBuffers bufs;
+  std::vector<roadmc_mte_problem> roads;  // shallow copies, used for output shapes
+  std::vector<std::vector<roadmc_species>> species;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int grid = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t timed_from = 0;
+};
+
+namespace {
+
+int capture_batch_graph(roadmc_batch* bt) {
+  roadmc_ctx* ctx = bt->ctx;
+  hipStream_t s = ctx->stream;
+  JobSet& js = bt->js;
+  JobBuffers& b = bt->bufs;
+  RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = roadmc::launch_fill_i32(static_cast<int*>(b.counter.ptr), 0, 16, s);
+  if (e == hipSuccess)
+    e = roadmc::launch_mte_persistent(js.fp64, static_cast<MteJob*>(b.jobs.ptr),
+                                      static_cast<int2*>(b.work.ptr), (int)js.work.size(),
+                                      static_cast<int*>(b.counter.ptr), bt->grid, js.lds, s);
+  if (e == hipSuccess)
+    e = roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
+                                  static_cast<double*>(b.out.ptr),
+                                  static_cast<int64_t*>(b.off.ptr), s);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+  }
+  RMC_HIP(ctx, e2);
+  bt->graph = g;
+  RMC_HIP(ctx, hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
+  return ROADMC_OK;
+}
+
+}  // namespace
+
+extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* roads,
+                                    int32_t n_roads, roadmc_batch** out) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!out || !roads || n_roads < 1)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "roads/out invalid");
+  *out = nullptr;
+  for (int r = 0; r < n_roads; ++r) {
+    int rc = validate_problem(ctx, &roads[r]);
+    if (rc) return rc;
+    if (roads[r].precision != roads[0].precision)
+      return rmc_fail(ctx, ROADMC_ERR_INVALID, "all roads of a batch need one precision");
+  }
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  roadmc_batch* bt = new roadmc_batch();
+  bt->ctx = ctx;
+  plan_jobs(ctx, roads, n_roads, true, bt->js);
+  int rc = bt->js.fp64 ? upload_jobs<double>(ctx, roads, bt->js, bt->bufs, true)
+                       : upload_jobs<float>(ctx, roads, bt->js, bt->bufs, true);
+  if (!rc) rc = ensure_lds_limit(ctx, bt->js.fp64, bt->js.lds);
+  if (!rc) {
+    // One workgroup per CU fits the LDS of the largest tile; small tiles
+    // leave room for a second one.
+    const int per_cu = bt->js.lds * 2 <= 160 * 1024 ? 2 : 1;
+    bt->grid = ctx->n_cu * per_cu;
+    bt->roads.assign(roads, roads + n_roads);
+    rc = capture_batch_graph(bt);
+  }
+  if (rc) {
+    roadmc_batch_destroy(bt);
+    return rc;
+  }
+  *out = bt;
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_batch_launch(roadmc_batch* bt) {
+  if (!bt) return ROADMC_ERR_INVALID;
+  roadmc_ctx* ctx = bt->ctx;
+  hipEvent_t a = nullptr, z = nullptr;
+  RMC_HIP(ctx, hipEventCreate(&a));
+  RMC_HIP(ctx, hipEventCreate(&z));
+  RMC_HIP(ctx, hipEventRecord(a, ctx->stream));
+  RMC_HIP(ctx, hipGraphLaunch(bt->exec, ctx->stream));
+  RMC_HIP(ctx, hipEventRecord(z, ctx->stream));
+  bt->events.emplace_back(a, z);
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_batch_sync(roadmc_batch* bt) {
+  if (!bt) return ROADMC_ERR_INVALID;
+  RMC_HIP(bt->ctx, hipStreamSynchronize(bt->ctx->stream));
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_batch_kernel_ms(roadmc_batch* bt, double* ms) {
+  if (!bt || !ms) return ROADMC_ERR_INVALID;
+  roadmc_ctx* ctx = bt->ctx;
+  RMC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  size_t n = 0;
+  for (size_t i = bt->timed_from; i < bt->events.size(); ++i) {
+    float t = 0.f;
+    RMC_HIP(ctx, hipEventElapsedTime(&t, bt->events[i].first, bt->events[i].second));
+    tot += t;
+    ++n;
+  }
+  bt->timed_from = bt->events.size();
+  if (!n) return rmc_fail(ctx, ROADMC_ERR_INVALID, "no launches since the last query");
+  *ms = tot / (double)n;
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_batch_fetch(roadmc_batch* bt, roadmc_mte_result* results) {
+  if (!bt || !results) return ROADMC_ERR_INVALID;
+  return fetch_results(bt->ctx, bt->js, bt->bufs, bt->roads.data(), results);
+}
+
+extern "C" void roadmc_batch_destroy(roadmc_batch* bt) {
+  if (!bt) return;
+  roadmc_ctx* ctx = bt->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& ev : bt->events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  rmc_devbuf* all[] = {&bt->bufs.mpack, &bt->bufs.cols, &bt->bufs.partial, &bt->bufs.out,
+                       &bt->bufs.endpop, &bt->bufs.jobs, &bt->bufs.work, &bt->bufs.off,
+                       &bt->bufs.counter};
+  for (rmc_devbuf* d : all) rmc_buf_free(*d);
+  delete bt;
+}
+
+extern "C" int roadmc_mte_run_batch(roadmc_ctx* ctx, const roadmc_mte_problem* roads,
+                                    int32_t n_roads, roadmc_mte_result* results) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!results) return rmc_fail(ctx, ROADMC_ERR_INVALID, "results is NULL");
+  roadmc_batch* bt = nullptr;
+  int rc = roadmc_batch_prepare(ctx, roads, n_roads, &bt);
+  if (rc) return rc;
+  rc = roadmc_batch_launch(bt);
+  if (!rc) rc = roadmc_batch_fetch(bt, results);
+  roadmc_batch_destroy(bt);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Tile autotuner: for each patch-count bucket, time every row count the LDS
+// allows on a synthetic road and keep the fastest per path.
+
+extern "C" int roadmc_autotune(roadmc_ctx* ctx, int32_t n_paths, int32_t n_years) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (n_paths < 256 || n_years < 1)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "autotune needs n_paths >= 256");
+  const int saved[kBuckets] = {ctx->tile_rows[0], ctx->tile_rows[1], ctx->tile_rows[2],
+                               ctx->tile_rows[3]};
+  int best_rows[kBuckets];
+  for (int b = 0; b < kBuckets; ++b) {
+    const int P = kBucketMax[b];
+    std::vector<double> trans((size_t)P * P), init(P, 100.0), cap(P, 200.0), gr(P, 0.05);
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j)
+        trans[(size_t)j * P + i] = (i == j ? 0.5 : 0.5 / (P - 1));
+    roadmc_species sp{};
+    sp.n_patches = P;
+    sp.init_pop = init.data();
+    sp.capacity = cap.data();
+    sp.growth = gr.data();
+    sp.trans = trans.data();
+    sp.noise_sd = 0.1;
+    sp.threshold_ratio = 0.5;
+    roadmc_mte_problem pr{};
+    pr.n_species = 1;
+    pr.species = &sp;
+    pr.n_years = n_years;
+    pr.time_step = 1.0;
+    pr.n_paths = n_paths;
+    pr.seed = 12345;
+    double best = 1e300;
+    best_rows[b] = saved[b];
+    for (int rows = 32; rows <= 256; rows *= 2) {
+      if (rows * P > roadmc::kMteTileElemsF32) break;
+      ctx->tile_rows[b] = rows;
+      roadmc_batch* bt = nullptr;
+      int rc = roadmc_batch_prepare(ctx, &pr, 1, &bt);
+      if (rc) {
+        for (int i = 0; i < kBuckets; ++i) ctx->tile_rows[i] = saved[i];
+        return rc;
+      }
+      double ms = 0.0;
+      rc = roadmc_batch_launch(bt);  // warm-up
+      if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
+      if (!rc) rc = roadmc_batch_launch(bt);
+      if (!rc) rc = roadmc_batch_launch(bt);
+      if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
+      roadmc_batch_destroy(bt);
+      if (rc) {
+        for (int i = 0; i < kBuckets; ++i) ctx->tile_rows[i] = saved[i];
+        return rc;
+      }
+      if (ms < best) {
+        best = ms;
+        best_rows[b] = rows;
+      }
+    }
+    ctx->tile_rows[b] = saved[b];
+  }
+  for (int b = 0; b < kBuckets; ++b) ctx->tile_rows[b] = best_rows[b];
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_get_tiles(const roadmc_ctx* ctx, int32_t* bucket_max_p, int32_t* rows,
+                                int32_t* n_buckets) {
+  if (!ctx || !n_buckets) return ROADMC_ERR_INVALID;
+  *n_buckets = kBuckets;
+  for (int b = 0; b < kBuckets; ++b) {
+    if (bucket_max_p) bucket_max_p[b] = kBucketMax[b];
+    if (rows) rows[b] = ctx->tile_rows[b];
+  }
+  return ROADMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-design baseline (SimulateGPU.cu:3680-3843): per species, allocate,
+// pre-generate the streams into HBM, build the CSC matrix on the host, run one
+// thread per path, copy the per-path totals back, free.
+
+namespace {
+
+template <typename T>
+int plain_species(roadmc_ctx* ctx, const roadmc_mte_problem& pr, int s,
+                  std::vector<double>& eps_out, std::vector<double>* patch_sum) {
+  const roadmc_species& sp = pr.species[s];
+  const int P = sp.n_patches, N = (int)pr.n_paths, Ty = pr.n_years;
+  const bool fp64 = sizeof(T) == 8;
+  const uint32_t k0 = (uint32_t)(pr.seed & 0xffffffffu), k1 = (uint32_t)(pr.seed >> 32);
+  // CSC of the binary32 movement matrix.  The reference keeps entries
+  // > 1e-5; here entries > drop_tol (0 keeps every non-zero).
+  std::vector<float> dense((size_t)P * P);
+  for (int k = 0; k < P; ++k)
+    for (int j = 0; j < P; ++j) dense[(size_t)j * P + k] = (float)movement(sp, P, k, j, 0.0);
+  std::vector<float> valf((size_t)P * P);
+  std::vector<int32_t> row_idx((size_t)P * P), per_col(P);
+  int32_t nnz = 0;
+  roadmc_dense_to_csc(dense.data(), P, P, (float)pr.drop_tol, valf.data(), row_idx.data(),
+                      per_col.data(), &nnz);
+  std::vector<T> val(nnz > 0 ? nnz : 1), prm(8), init(P), caps(P), rates(P);
+  if (fp64) {
+    // Double-precision values for the FP64 variant, same sparsity.
+    int it = 0;
+    for (int c = 0; c < P; ++c)
+      for (int l = 0; l < per_col[c]; ++l, ++it)
+        val[it] = (T)movement(sp, P, row_idx[it], c, 0.0);
+  } else {
+    for (int i = 0; i < nnz; ++i) val[i] = (T)valf[i];
+  }
+  for (int i = 0; i < 8; ++i) prm[i] = (T)sp.params[i];
+  for (int j = 0; j < P; ++j) {
+    init[j] = (T)sp.init_pop[j];
+    caps[j] = (T)sp.capacity[j];
+    rates[j] = sp.growth ? (T)sp.growth[j] : (T)0;
+  }
+  void *d_rgr = nullptr, *d_br = nullptr, *d_js = nullptr, *d_ju = nullptr, *d_prm = nullptr,
+       *d_init = nullptr, *d_caps = nullptr, *d_rates = nullptr, *d_val = nullptr,
+       *d_pp = nullptr, *d_eps = nullptr;
+  int *d_row = nullptr, *d_col = nullptr;
+  const size_t es = sizeof(T);
+  int rc = ROADMC_OK;
+  hipStream_t st = ctx->stream;
+  auto cleanup = [&]() {
+    void* all[] = {d_rgr, d_br, d_js, d_ju, d_prm, d_init, d_caps, d_rates, d_val, d_pp,
+                   d_eps, d_row, d_col};
+    for (void* p : all)
+      if (p) (void)hipFree(p);
+  };
+  if ((rc = rmc_malloc(ctx, &d_rgr, (size_t)N * Ty * P * es)) ||
+      (rc = rmc_malloc(ctx, &d_br, (size_t)N * Ty * es)) ||
+      (rc = rmc_malloc(ctx, &d_js, (size_t)N * Ty * es)) ||
+      (rc = rmc_malloc(ctx, &d_ju, (size_t)N * Ty * es)) ||
+      (rc = rmc_malloc(ctx, &d_prm, 8 * es)) || (rc = rmc_malloc(ctx, &d_init, P * es)) ||
+      (rc = rmc_malloc(ctx, &d_caps, P * es)) || (rc = rmc_malloc(ctx, &d_rates, P * es)) ||
+      (rc = rmc_malloc(ctx, &d_val, val.size() * es)) ||
+      (rc = rmc_malloc(ctx, (void**)&d_row, row_idx.size() * sizeof(int))) ||
+      (rc = rmc_malloc(ctx, (void**)&d_col, per_col.size() * sizeof(int))) ||
+      (rc = rmc_malloc(ctx, &d_pp, (size_t)N * 2 * P * es)) ||
+      (rc = rmc_malloc(ctx, &d_eps, (size_t)N * es))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = roadmc::launch_gen_patch_normals(
+      fp64, d_rgr, N, Ty, P, rmc_tag(pr.road_id, (uint32_t)s, RMC_KIND_PATCH), k0, k1, st);
+  if (e == hipSuccess)
+    e = roadmc::launch_gen_path_scalars(fp64, d_br, d_js, d_ju, N, Ty,
+                                        rmc_tag(pr.road_id, (uint32_t)s, RMC_KIND_PATH), k0,
+                                        k1, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_prm, prm.data(), 8 * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_init, init.data(), P * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_caps, caps.data(), P * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_rates, rates.data(), P * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_val, val.data(), val.size() * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_row, row_idx.data(), row_idx.size() * sizeof(int),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_col, per_col.data(), per_col.size() * sizeof(int),
+                       hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    roadmc::PlainArgs a{};
+    a.N = N;
+    a.T = Ty;
+    a.P = P;
+    a.law = pr.growth_law;
+    a.endpoint = pr.endpoint;
+    a.dt = pr.time_step;
+    a.sigma = sp.noise_sd;
+    a.rgr = d_rgr;
+    a.brown = d_br;
+    a.jsize = d_js;
+    a.jump = d_ju;
+    a.params = d_prm;
+    a.init = d_init;
+    a.caps = d_caps;
+    a.rates = d_rates;
+    a.val = d_val;
+    a.row_idx = d_row;
+    a.per_col = d_col;
+    a.path_pops = d_pp;
+    a.eps = d_eps;
+    e = roadmc::launch_mte_plain(fp64, a, st);
+  }
+  std::vector<T> eps(N);
+  if (e == hipSuccess) e = hipMemcpyAsync(eps.data(), d_eps, N * es, hipMemcpyDeviceToHost, st);
+  std::vector<T> pops;
+  if (e == hipSuccess && patch_sum) {
+    pops.resize((size_t)N * 2 * P);
+    e = hipMemcpyAsync(pops.data(), d_pp, pops.size() * es, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess)
+    return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("plain path: ") + hipGetErrorString(e));
+  eps_out.resize(N);
+  for (int i = 0; i < N; ++i) eps_out[i] = (double)eps[i];
+  if (patch_sum) {
+    patch_sum->assign(P, 0.0);
+    const int slot = pr.endpoint == ROADMC_END_PRE_GROWTH ? 1 : 0;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < P; ++j) (*patch_sum)[j] += (double)pops[((size_t)i * 2 + slot) * P + j];
+  }
+  return ROADMC_OK;
+}
+
+}  // namespace
+
+extern "C" int roadmc_mte_run_plain(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
+                                    roadmc_mte_result* result) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!result) return rmc_fail(ctx, ROADMC_ERR_INVALID, "result is NULL");
+  int rc = validate_problem(ctx, problem);
+  if (rc) return rc;
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const int S = problem->n_species;
+  const int64_t N = problem->n_paths;
+  std::vector<std::vector<double>> eps(S), psum(S);
+  for (int s = 0; s < S; ++s) {
+    rc = problem->precision == ROADMC_PREC_FP64
+             ? plain_species<double>(ctx, *problem, s, eps[s],
+                                     result->patch_mean ? &psum[s] : nullptr)
+             : plain_species<float>(ctx, *problem, s, eps[s],
+                                    result->patch_mean ? &psum[s] : nullptr);
+    if (rc) return rc;
+  }
+  // Statistics as Simulator::simulateMTE forms them (Simulator.cpp:184-191).
+  int64_t pbase = 0;
+  for (int s = 0; s < S; ++s) {
+    const roadmc_species& sp = problem->species[s];
+    double tot0 = 0.0;
+    for (int j = 0; j < sp.n_patches; ++j) tot0 += sp.init_pop[j];
+    const double thr = sp.threshold_ratio * tot0;
+    double sum = 0.0, cnt = 0.0;
+    for (double v : eps[s]) {
+      sum += v;
+      cnt += v >= thr ? 1.0 : 0.0;
+    }
+    const double mean = sum / (double)N;
+    double ss = 0.0;
+    for (double v : eps[s]) ss += (v - mean) * (v - mean);
+    if (result->mte_prob) result->mte_prob[s] = cnt / (double)N;
+    if (result->end_mean) result->end_mean[s] = mean;
+    if (result->end_sd) result->end_sd[s] = std::sqrt(ss / (double)N);
+    if (result->end_pops) std::memcpy(result->end_pops + s * N, eps[s].data(), sizeof(double) * N);
+    if (result->patch_mean)
+      for (int j = 0; j < sp.n_patches; ++j) result->patch_mean[pbase + j] = psum[s][j] / (double)N;
+    pbase += sp.n_patches;
+  }
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
+                                    uint32_t species, int32_t n_paths, int32_t n_years,
+                                    int32_t n_patches, float* out) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!out || n_paths < 1 || n_years < 1 || n_patches < 1 || road_id >= (1u << 24) ||
+      species > 63)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "roadmc_patch_normals: bad arguments");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t n = (size_t)n_paths * n_years * n_patches;
+  void* d = nullptr;
+  int rc = rmc_malloc(ctx, &d, n * sizeof(float));
+  if (rc) return rc;
+  hipError_t e = roadmc::launch_gen_patch_normals(
+      false, d, n_paths, n_years, n_patches, rmc_tag(road_id, species, RMC_KIND_PATCH),
+      (uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32), ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("patch normals: ") + hipGetErrorString(e));
+  return ROADMC_OK;
+}

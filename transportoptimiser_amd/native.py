@@ -1,0 +1,129 @@
+"""ctypes binding of libroadmc (include/roadmc.h).
+
+The library is loaded from ``transportoptimiser_amd/lib/libroadmc.so``; if it
+is missing, :func:`lib` raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libroadmc.so"
+
+OK = 0
+ERR_INVALID = 1
+ERR_HIP = 2
+ERR_OOM = 3
+ERR_NUMERIC = 4
+ERR_UNSUPPORTED = 5
+
+LAW_LOGNORMAL = 0
+LAW_REF_CPU = 1
+LAW_REF_GPU = 2
+END_POST_GROWTH = 0
+END_PRE_GROWTH = 1
+PREC_FP32 = 0
+PREC_FP64 = 1
+MAX_PATCHES = 1024
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int32),
+        ("init_pop", _dp),
+        ("capacity", _dp),
+        ("growth", _dp),
+        ("trans", _dp),
+        ("surv", _dp),
+        ("params", C.c_double * 8),
+        ("noise_sd", C.c_double),
+        ("threshold_ratio", C.c_double),
+    ]
+
+
+class MteProblem(C.Structure):
+    _fields_ = [
+        ("n_species", C.c_int32),
+        ("species", C.POINTER(Species)),
+        ("n_years", C.c_int32),
+        ("time_step", C.c_double),
+        ("n_paths", C.c_int64),
+        ("growth_law", C.c_int32),
+        ("endpoint", C.c_int32),
+        ("precision", C.c_int32),
+        ("drop_tol", C.c_double),
+        ("seed", C.c_uint64),
+        ("road_id", C.c_uint32),
+    ]
+
+
+class MteResult(C.Structure):
+    _fields_ = [
+        ("mte_prob", _dp),
+        ("end_mean", _dp),
+        ("end_sd", _dp),
+        ("patch_mean", _dp),
+        ("end_pops", _dp),
+    ]
+
+
+# Every symbol include/roadmc.h declares: name -> (restype, argtypes).
+SIGNATURES: dict[str, tuple] = {
+    "roadmc_device_count": (C.c_int, [_ip]),
+    "roadmc_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "roadmc_ctx_destroy": (None, [C.c_void_p]),
+    "roadmc_last_error": (C.c_char_p, [C.c_void_p]),
+    "roadmc_debug_fail_allocs": (C.c_int, [C.c_void_p, C.c_int]),
+    "roadmc_mte_run": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.POINTER(MteResult)]),
+    "roadmc_mte_run_plain": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.POINTER(MteResult)]),
+    "roadmc_mte_run_batch": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
+                                       C.POINTER(MteResult)]),
+    "roadmc_batch_prepare": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
+                                       C.POINTER(C.c_void_p)]),
+    "roadmc_batch_launch": (C.c_int, [C.c_void_p]),
+    "roadmc_batch_sync": (C.c_int, [C.c_void_p]),
+    "roadmc_batch_fetch": (C.c_int, [C.c_void_p, C.POINTER(MteResult)]),
+    "roadmc_batch_kernel_ms": (C.c_int, [C.c_void_p, _dp]),
+    "roadmc_batch_destroy": (None, [C.c_void_p]),
+    "roadmc_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "roadmc_get_tiles": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "roadmc_dense_to_csc": (C.c_int, [_fp, C.c_int32, C.c_int32, C.c_float, _fp, _ip, _ip, _ip]),
+    "roadmc_normal_cdf_inverse": (C.c_int, [C.c_double, _dp]),
+    "roadmc_mte_penalty": (C.c_int, [C.c_int32, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]),
+    "roadmc_patch_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.c_int32, C.c_int32, C.c_int32, _fp]),
+}
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Loads libroadmc.so once; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: run `python -m transportoptimiser_amd.build` first")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+class RoadmcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libroadmc error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int, ctx=None) -> None:
+    if code != OK:
+        msg = lib().roadmc_last_error(ctx).decode() if ctx else ""
+        raise RoadmcError(code, msg)
