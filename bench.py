@@ -1,0 +1,235 @@
+#!/usr/bin/env python
+"""Road MTE evaluations per second for one GA generation on one MI355X.
+
+Workload (one "step"): a generation of 512 candidate roads, patch counts
+log-uniform in [64, 1024], 2 species, 50 years, 20,000 Monte Carlo paths per
+road, FP32 arithmetic (FP32-accumulating MFMA), synthetic landscapes.  The
+batched path evaluates the whole generation in one replay of a captured HIP
+graph around a persistent kernel; inputs are resident in HBM when the timed
+region starts.
+
+Reported beside the headline value:
+  plain    the reference's GPU design (one road per call, per-call device
+           allocation, random streams pre-generated into HBM, one thread per
+           path over CSC), timed on a sample of roads and extrapolated by P^2;
+  single   the tiled kernel called once per road through roadmc_mte_run;
+  accuracy max |dMTE| of the FP32 generation against the FP64 kernels;
+  roofline FP32 MFMA flops 2 N P^2 T S of the persistent kernel over its
+           event-timed duration;
+  cpu_baseline the CPU oracle (FP64, OpenMP over paths) on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "road MTE evals/sec, batched vs plain GPU, 1 MI355X at |ΔMTE|≤2e-3"
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def road_flops(r) -> float:
+    return float(sum(2.0 * r.n_paths * sd.n_patches ** 2 * r.n_years for sd in r.species))
+
+
+def pick_sample(roads, quantiles):
+    P = np.array([r.species[0].n_patches for r in roads])
+    order = np.argsort(P, kind="stable")
+    idx = sorted({int(order[min(len(order) - 1, int(q * (len(order) - 1)))]) for q in quantiles})
+    return idx
+
+
+def fit_p2(sizes, times):
+    """Least-squares t = a + b P^2 (a >= 0)."""
+    A = np.stack([np.ones(len(sizes)), np.asarray(sizes, float) ** 2], axis=1)
+    coef, *_ = np.linalg.lstsq(A, np.asarray(times, float), rcond=None)
+    a, b = float(coef[0]), float(coef[1])
+    if a < 0 or b <= 0:
+        a, b = 0.0, float(np.sum(times) / np.sum(np.asarray(sizes, float) ** 2))
+    return a, b
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--roads", type=int, default=512)
+    ap.add_argument("--paths", type=int, default=20000)
+    ap.add_argument("--years", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--autotune", action="store_true", help="tune tile rows first")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+              "multi-GPU path (roads shard across GPUs with no communication; see "
+              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from transportoptimiser_amd import landscape
+    from transportoptimiser_amd import native as nat
+    from transportoptimiser_amd.api import Context
+
+    t0 = time.perf_counter()
+    workers = max(1, min(8, (os.cpu_count() or 2) // 2))
+    roads = landscape.config3(seed=args.seed, n_roads=args.roads, n_paths=args.paths,
+                              n_years=args.years, workers=workers)
+    flops = sum(road_flops(r) for r in roads)
+    sizes = [r.species[0].n_patches for r in roads]
+    log(f"generated {len(roads)} roads in {time.perf_counter() - t0:.1f}s; "
+        f"sum P^2 = {sum(p * p for p in sizes)}, {flops / 1e12:.1f} TFLOP per generation")
+
+    ctx = Context(0)
+    if args.autotune:
+        log(f"autotuned tile rows: {ctx.autotune(n_paths=args.paths, n_years=5)}")
+    tiles = ctx.tiles()
+
+    batch = ctx.prepare_batch(roads)
+    for _ in range(args.warmup):
+        batch.launch()
+        batch.sync()
+    if args.warmup:
+        batch.kernel_ms()  # drop warm-up launches from the event average
+    batch.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    batch.sync()
+    t_end = time.perf_counter()
+    ms_per_step = (t_end - t_start) * 1e3 / args.steps
+    kernel_ms = batch.kernel_ms()
+    value = len(roads) / (ms_per_step / 1e3)
+    log(f"batched: {ms_per_step:.1f} ms/generation, graph {kernel_ms:.1f} ms, "
+        f"{value:.1f} roads/s, {flops / kernel_ms / 1e9:.1f} TFLOP/s")
+    out32 = batch.fetch(end_pops=False, patch_mean=False)
+    batch.close()
+
+    accuracy = None
+    if not args.no_fp64:
+        t = time.perf_counter()
+        for r in roads:
+            r.precision = nat.PREC_FP64
+        out64 = ctx.simulate_mte_batch(roads, end_pops=False, patch_mean=False)
+        for r in roads:
+            r.precision = nat.PREC_FP32
+        d = np.array([np.max(np.abs(a.mte_prob - b.mte_prob)) for a, b in zip(out32, out64)])
+        dm = np.array([np.max(np.abs(a.end_mean - b.end_mean) / b.end_mean)
+                       for a, b in zip(out32, out64)])
+        accuracy = {"max_abs_dMTE_vs_fp64": float(d.max()), "mean_abs_dMTE": float(d.mean()),
+                    "max_rel_dmean_vs_fp64": float(dm.max()), "roads_checked": len(roads),
+                    "fp64_generation_s": round(time.perf_counter() - t, 2)}
+        log(f"accuracy vs FP64 kernels: {accuracy}")
+
+    single = None
+    if not args.no_single:
+        t = time.perf_counter()
+        for r in roads:
+            ctx.simulate_mte(r, end_pops=False, patch_mean=False)
+        s = time.perf_counter() - t
+        single = {"evals_per_s": len(roads) / s, "s_per_generation": round(s, 3),
+                  "ratio_batched_over_single": value / (len(roads) / s),
+                  "how": "roadmc_mte_run once per road: host packing, H2D, direct launch, D2H"}
+        log(f"single-road tiled calls: {single}")
+
+    plain = None
+    if not args.no_plain:
+        idx = pick_sample(roads, [0.0, 0.12, 0.25])
+        ts, ps = [], []
+        for i in idx:
+            t = time.perf_counter()
+            ctx.simulate_mte(roads[i], plain=True, end_pops=False, patch_mean=False)
+            ts.append(time.perf_counter() - t)
+            ps.append(sizes[i])
+            log(f"plain road P={sizes[i]}: {ts[-1]:.2f}s")
+        a, b = fit_p2(ps, ts)
+        est = float(sum(a + b * p * p for p in sizes))
+        plain = {"evals_per_s": len(roads) / est, "est_s_per_generation": round(est, 1),
+                 "ratio_batched_over_plain": value / (len(roads) / est),
+                 "sample": f"{len(idx)} roads of the generation, P={ps}, timed {np.round(ts, 2).tolist()} s; "
+                           f"generation time extrapolated as sum over roads of a + b P^2",
+                 "how": "roadmc_mte_run_plain: per-call hipMalloc, streams pre-generated in HBM, "
+                        "one thread per path over CSC (SimulateGPU.cu:3650-3864 call shape)"}
+        log(f"plain: {plain}")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        idx = pick_sample(roads, [0.1, 0.3, 0.5, 0.7])
+        n_sample = max(1, args.paths // 5)
+        ts, ps = [], []
+        for i in idx:
+            r = roads[i]
+            t = time.perf_counter()
+            for s, sd in enumerate(r.species):
+                pyoracle.mte_species(sd, s, r.road_id, r.n_years, r.time_step, r.growth_law,
+                                     r.endpoint, r.drop_tol, r.seed, 0, n_sample,
+                                     n_threads=threads)
+            ts.append((time.perf_counter() - t) * args.paths / n_sample)
+            ps.append(sizes[i])
+        a, b = fit_p2(ps, ts)
+        est = float(sum(a + b * p * p for p in sizes))
+        cpu = {"value": len(roads) / est, "unit": "road evals/s", "cores": threads,
+               "kind": "port",
+               "sample": f"CPU oracle (FP64, OpenMP over paths) on {len(idx)} roads of the "
+                         f"generation (P={ps}), {n_sample} of {args.paths} paths each, both "
+                         f"species; per-road time scaled by paths and extrapolated over the "
+                         f"512 roads as a + b P^2"}
+        log(f"cpu baseline: {cpu}")
+
+    achieved = flops / (kernel_ms / 1e3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "road evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"GA generation: {len(roads)} roads, P log-uniform [64,1024], "
+                               f"{len(roads[0].species)} species, {args.years} years, "
+                               f"{args.paths} paths, one persistent graph launch",
+                   "n_roads": len(roads), "paths": args.paths, "years": args.years,
+                   "species": len(roads[0].species), "tile_rows": tiles},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
+                     "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "mte_persistent_kernel<TraitsF32>",
+                     "kernel_ms": round(kernel_ms, 3),
+                     "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+        "plain": plain,
+        "single": single,
+        "accuracy": accuracy,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -39,3 +39,24 @@ def test_configs_are_deterministic():
     g = ls.config3(seed=7, n_roads=4, n_paths=100)
     assert [len(r.species) for r in g] == [2, 2, 2, 2]
     assert [r.road_id for r in g] == [0, 1, 2, 3]
+
+
+def test_vectorised_crossing_matches_segment_test():
+    for seed in range(4):
+        rng = np.random.default_rng(seed)
+        P = 150
+        xy = rng.uniform(0, 10, size=(P, 2))
+        road = ls.random_polyline(rng, 10.0)
+        a = np.broadcast_to(xy[:, None, :], (P, P, 2))
+        b = np.broadcast_to(xy[None, :, :], (P, P, 2))
+        ref = np.zeros((P, P), dtype=bool)
+        for s in range(road.shape[0] - 1):
+            ref |= ls._segments_cross(a, b, road[s], road[s + 1])
+        assert np.array_equal(ref, ls.crossed_links(xy, road))
+
+
+def test_parallel_generation_equals_serial():
+    a = ls.config3(seed=2, n_roads=6, n_paths=10)
+    b = ls.config3(seed=2, n_roads=6, n_paths=10, workers=3)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.species[1].surv, y.species[1].surv)
