@@ -175,6 +175,99 @@ int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
                          uint32_t species, int32_t n_paths, int32_t n_years,
                          int32_t n_patches, float* out);
 
+/* ---- ROV: price paths and regression Monte Carlo --------------------------- */
+
+/* Price laws of one exogenous uncertainty. */
+#define ROADMC_SDE_MRJD 0 /* x += k(m - x)dt + x s z + (e^J' - 1) x 1[u < p],
+                             J' = J^2 z' - J^2/2 (SimulateGPU.cu:930-952) */
+#define ROADMC_SDE_OU 1   /* x += k(m - x)dt + s sqrt(dt) z */
+#define ROADMC_SDE_GBM 2  /* x *= exp((m - s^2/2)dt + s sqrt(dt) z) */
+
+#define ROADMC_ROV_SWITCHING 0 /* flow control per step (backwardInduction,
+                                  SimulateGPU.cu:2333-2784) */
+#define ROADMC_ROV_AMERICAN 1  /* optimal stopping of a put on uncertainty 0
+                                  (Longstaff and Schwartz 2001) */
+
+#define ROADMC_VALUE_REGRESSED 0 /* carry the regressed value back
+                                    (SimulateGPU.cu:2757-2771) */
+#define ROADMC_VALUE_REALIZED 1  /* carry realized cash flows back */
+
+#define ROADMC_MAX_UNCERTAINTIES 8
+#define ROADMC_MAX_CONTROLS 8
+#define ROADMC_MAX_BASIS 35 /* cubic in 4 dimensions */
+
+/* The six packed parameters of SimulateGPU.cu:4020-4070 plus the law. */
+typedef struct {
+  int32_t kind;          /* ROADMC_SDE_* */
+  double x0;             /* current value */
+  double mean;           /* long-run mean (MRJD/OU) or drift (GBM) */
+  double sd;             /* volatility */
+  double reversion;      /* mean-reversion strength */
+  double jump_size;      /* MRJD jump size J */
+  double jump_prob;      /* MRJD jump probability per step */
+} roadmc_uncertainty;
+
+typedef struct {
+  int32_t n_uncertainties;            /* U, 1..ROADMC_MAX_UNCERTAINTIES */
+  const roadmc_uncertainty* unc;      /* [U] */
+  const double* correlation;          /* [U*U] row-major, NULL = independent */
+  int32_t n_controls;                 /* C, 1..ROADMC_MAX_CONTROLS (SWITCHING) */
+  const double* flow;                 /* [C] flow rate per control, Q[c] */
+  double unit_cost;                   /* unit profit = unit_cost + sum_u w_u x_u */
+  const double* profit_weight;        /* [U] (fuel costs, -revenue * ore) */
+  int32_t traffic_index;              /* uncertainty scaling the flow, or -1 */
+  int32_t n_steps;                    /* T decision steps */
+  double time_step;                   /* dt */
+  double rrr;                         /* per-step discount 1/(1 + rrr dt / 100) */
+  double discount;                    /* if > 0, the per-step discount factor */
+  int64_t n_paths;                    /* N */
+  int32_t basis_degree;               /* 1..3: all monomials up to this degree */
+  int32_t standardize;                /* scale features per step to zero mean,
+                                         unit variance before the fit */
+  int32_t itm_only;                   /* fit only in-the-money paths
+                                         (keepValidPoints, SimulateGPU.cu:5999) */
+  int32_t pooled;                     /* 1: one fit on all paths; 0: one fit per
+                                         randomly drawn control (randControls) */
+  double hysteresis;                  /* switch only if payoff < h * best
+                                         (0.975 in SimulateGPU.cu:2763) */
+  int32_t mode;                       /* ROADMC_ROV_* */
+  int32_t value_mode;                 /* ROADMC_VALUE_* */
+  double strike;                      /* AMERICAN: payoff max(strike - x_0, 0) */
+  uint64_t seed;
+  const double* states;               /* optional pre-generated paths
+                                         [(T+1)][U][N]; NULL simulates them */
+  const int32_t* controls;            /* optional random controls [T][N] */
+} roadmc_rov_problem;
+
+typedef struct {
+  double* value;         /* [1] time-0 value (SWITCHING: expected cost, <= 0
+                            is profit; AMERICAN: option value) */
+  double* value_sd;      /* [1] SD of the time-0 value over paths */
+  int32_t* first_control;/* [1] optimal control at time 0 */
+  double* coeffs;        /* [T][C][K] fitted coefficients (raw basis when
+                            standardize is 0); K from roadmc_rov_basis_size */
+  double* cond_exp;      /* [(T+1)][N] value per path and step */
+  int32_t* opt_control;  /* [T][N] chosen control per path and step */
+  double* states;        /* [(T+1)][U][N] simulated paths */
+} roadmc_rov_result;
+
+/* Number of basis functions of a degree-d polynomial in n variables. */
+int roadmc_rov_basis_size(int32_t n_vars, int32_t degree, int32_t* size);
+
+/* SimulateGPU::simulateROVCUDA (SimulateGPU.h:340-344): forward paths, then
+ * backward induction with per-step FP64 regression, captured in one HIP
+ * graph. */
+int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* problem,
+                   roadmc_rov_result* result);
+
+/* SimulateGPU::expPV (SimulateGPU.h:206, SimulateGPU.cu:3163-3259): expected
+ * present value sum_t (1 + g)^t x_t / (1 + r)^t of one uncontrolled MRJD
+ * price with Brownian SD s*dt (the reference's parameterisation), and its SD
+ * over paths (1/N; the reference omits the 1/N, Uncertainty.cpp:75). */
+int roadmc_exp_pv(roadmc_ctx* ctx, const roadmc_uncertainty* unc,
+                  int32_t n_steps, double time_step, double rrr, double growth,
+                  int64_t n_paths, uint64_t seed, double* mean, double* sd);
+
 #ifdef __cplusplus
 }
 #endif

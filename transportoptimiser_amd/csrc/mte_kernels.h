@@ -39,9 +39,13 @@ size_t mte_pack_index(bool fp64, int Pp, int k, int j);
 hipError_t mte_set_lds_limit(bool fp64, size_t bytes);
 hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
                              int n_work, size_t lds, hipStream_t s);
+// Work queues: items work[qbeg[q] .. qbeg[q+1]) form queue q; counters[q]
+// must be zero at launch.
+constexpr int kMteQueues = 8;  // one per XCD of an MI355X
 hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
-                                 const int2* work, int n_work, int* counter,
-                                 int grid, size_t lds, hipStream_t s);
+                                 const int2* work, const int* qbeg, int nq,
+                                 int* counters, int grid, size_t lds,
+                                 hipStream_t s);
 hipError_t launch_mte_reduce(const MteJob* jobs, int n_jobs, double* out,
                              const int64_t* out_off, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int value, int n, hipStream_t s);

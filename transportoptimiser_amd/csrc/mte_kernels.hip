@@ -140,7 +140,8 @@ __device__ __forceinline__ T grow(int law, T x, T z, T K, T Kinv, T r, T sig,
   }
   T e;
   if constexpr (sizeof(T) == 4)
-    e = __expf(sig * z - half_var);
+    // sig and half_var arrive pre-scaled by log2(e): one FMA and v_exp_f32.
+    e = __builtin_amdgcn_exp2f(fmaf(sig, z, -half_var));
   else
     e = exp(sig * z - half_var);
   return x * ((T)1 + r * ((T)1 - x * Kinv)) * e;
@@ -165,6 +166,69 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Global-address-space view of a device pointer, so that loads compile to
+// global_load (not flat_load).
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(1))) V* as_global(const V* p) {
+  return (const __attribute__((address_space(1))) V*)(p);
+}
+
+// One year's product for the NTILE output tiles of a wave:
+// acc[t] += A[rows of tile t, :] . M[:, cols of tile t].
+// Two register stages ping-pong over the k-groups (the k-group count Pp/8 is a
+// multiple of 4): the B operands and LDS A operands of k-group kg+1 are in
+// flight while the MFMAs of k-group kg run.
+// On entry b0 holds the B operands of k-group 0; on exit it holds them again
+// (the last stage-0 load re-fetches k-group 0, which the next year starts
+// with, so that load's latency is hidden behind the epilogue).
+template <class TR, int NTILE>
+__device__ __forceinline__ void year_product(typename TR::AccT* acc,
+                                             typename TR::VecT* b0,
+                                             const typename TR::VecT* Bp,
+                                             const typename TR::T* A,
+                                             const int* trow, const int* tcol,
+                                             int nkg, int LDA, int arow,
+                                             int akoff, int lane) {
+  using T = typename TR::T;
+  using VecT = typename TR::VecT;
+  constexpr int TM = TR::TM, KV = TR::KV;
+  const __attribute__((address_space(1))) VecT* bp[NTILE];
+  const T* ap[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) {
+    bp[t] = as_global(Bp + (size_t)tcol[t] * nkg * 64 + lane);
+    ap[t] = A + (trow[t] * TM + arow) * LDA + akoff;
+  }
+  VecT b1[NTILE], a0[NTILE], a1[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
+  for (int kg = 0; kg < nkg; kg += 2) {
+    // stage 1 loads (k-group kg+1 < nkg always: nkg is even)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      b1[t] = bp[t][(size_t)(kg + 1) * 64];
+      a1[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 1) * 8);
+    }
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[t][s], b0[t][s], acc[t]);
+    }
+    // stage 0 loads for k-group kg+2; after the last pass, k-group 0 again
+    const int k2 = (kg + 2 < nkg) ? kg + 2 : 0;
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      b0[t] = bp[t][(size_t)k2 * 64];
+      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + k2 * 8);
+    }
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[t][s], b1[t][s], acc[t]);
+    }
+  }
 }
 
 // Simulates one tile (BM paths of one job) for all years and writes the
@@ -199,7 +263,12 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
 
   const T mu = (T)jb.params[1], sd = (T)jb.params[2], rev = (T)jb.params[4];
   const T jmp = (T)jb.params[5], jprob = (T)jb.params[6], lv = (T)jb.params[7];
-  const T dt = (T)jb.dt, sig = (T)jb.sigma, half_var = (T)0.5 * sig * sig;
+  const T dt = (T)jb.dt;
+  // Lognormal factor exp(s z - s^2/2): the FP32 path evaluates it as
+  // exp2(s' z - h') with s' = s log2(e), h' = (s^2/2) log2(e).
+  const T l2e = sizeof(T) == 4 ? (T)1.4426950408889634 : (T)1;
+  const T sig = (T)(jb.sigma * (sizeof(T) == 4 ? 1.4426950408889634 : 1.0));
+  const T half_var = (T)(0.5 * jb.sigma * jb.sigma) * l2e;
   const int law = jb.law;
 
   for (int idx = tid; idx < BM * Pp; idx += nthreads) {
@@ -228,32 +297,31 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
   const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
 
   AccT acc[NT];
+  VecT b0[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+    if (t < my_n) b0[t] = as_global(Bp)[(size_t)tcol[t] * nkg * 64 + lane];
   for (int year = 0; year < jb.T; ++year) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = AccT{};
-
-    VecT bcur[NT], bnxt[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-      if (t < my_n) bcur[t] = Bp[((size_t)tcol[t] * nkg) * 64 + lane];
-    for (int kg = 0; kg < nkg; ++kg) {
-      if (kg + 1 < nkg) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-          if (t < my_n)
-            bnxt[t] = Bp[((size_t)tcol[t] * nkg + kg + 1) * 64 + lane];
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        if (t < my_n) {
-          const VecT a = *reinterpret_cast<const VecT*>(
-              A + (trow[t] * TM + arow) * LDA + kg * 8 + akoff);
-#pragma unroll
-          for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a[s], bcur[t][s], acc[t]);
+    // One branch per year on the (wave-uniform) tile count; the product
+    // loops themselves are straight-line code.
+    switch (my_n) {
+      case 1: year_product<TR, 1>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      case 2: year_product<TR, 2>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      case 3: year_product<TR, 3>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      case 4: year_product<TR, 4>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      default:
+        if constexpr (NT > 4) {
+          switch (my_n) {
+            case 5: year_product<TR, 5>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            case 6: year_product<TR, 6>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            case 7: year_product<TR, 7>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            case 8: year_product<TR, 8>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            default: break;
+          }
         }
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) bcur[t] = bnxt[t];
+        break;
     }
     __syncthreads();  // every wave has finished reading this year's state
 
@@ -301,8 +369,9 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
               zz = z[path & 3u];
             }
             const T x = acc[t][r4 + s];
+            const T g = (law == kLawRefGpu) ? grm[row] : (T)0;
             const T n = grow<T>(law, x, zz, K, Ki, rr, sig, half_var, dt, mu,
-                                sd, grm[row], lv);
+                                sd, g, lv);
             A[row * LDA + pcol] = keep_pre ? x : n;
           }
         }
@@ -357,23 +426,31 @@ __global__ __launch_bounds__(kMteThreads) void mte_direct_kernel(
   mte_tile<TR>(jobs[w.x], w.y, smem);
 }
 
-// Persistent form: a grid sized to the machine pulls (job, tile) items from a
-// queue ordered by decreasing cost.  Every workgroup leaves the loop once the
-// counter passes n_work.
+// Persistent form.  The work items are split into nq queues, one per XCD,
+// each holding whole jobs with their tiles contiguous, so that the CUs of an
+// XCD share one movement matrix in their L2.  Workgroups are dispatched to
+// XCDs round robin (workgroup b runs on XCD b % 8), so workgroup b first
+// drains queue b % nq and then takes what is left in the others.  Every
+// workgroup leaves each queue once its counter passes the queue length.
 template <class TR>
 __global__ __launch_bounds__(kMteThreads) void mte_persistent_kernel(
-    const MteJob* __restrict__ jobs, const int2* __restrict__ work, int n_work,
-    int* __restrict__ counter) {
+    const MteJob* __restrict__ jobs, const int2* __restrict__ work,
+    const int* __restrict__ qbeg, int nq, int* __restrict__ counters) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   __shared__ int next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-    __syncthreads();
-    const int w = next;
-    __syncthreads();
-    if (w >= n_work) break;
-    const int2 it = work[w];
-    mte_tile<TR>(jobs[it.x], it.y, smem);
+  const int home = (int)(blockIdx.x % (unsigned)nq);
+  for (int qi = 0; qi < nq; ++qi) {
+    const int q = (home + qi) % nq;
+    const int base = qbeg[q], len = qbeg[q + 1] - base;
+    for (;;) {
+      if (threadIdx.x == 0) next = atomicAdd(&counters[q], 1);
+      __syncthreads();
+      const int w = next;
+      __syncthreads();
+      if (w >= len) break;
+      const int2 it = work[base + w];
+      mte_tile<TR>(jobs[it.x], it.y, smem);
+    }
   }
 }
 
@@ -429,15 +506,18 @@ hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
 }
 
 hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
-                                 const int2* work, int n_work, int* counter,
-                                 int grid, size_t lds, hipStream_t s) {
-  if (n_work <= 0) return hipSuccess;
+                                 const int2* work, const int* qbeg, int nq,
+                                 int* counters, int grid, size_t lds,
+                                 hipStream_t s) {
+  if (nq <= 0 || grid <= 0) return hipSuccess;
   if (fp64)
     hipLaunchKernelGGL(mte_persistent_kernel<TraitsF64>, dim3(grid),
-                       dim3(kMteThreads), lds, s, jobs, work, n_work, counter);
+                       dim3(kMteThreads), lds, s, jobs, work, qbeg, nq,
+                       counters);
   else
     hipLaunchKernelGGL(mte_persistent_kernel<TraitsF32>, dim3(grid),
-                       dim3(kMteThreads), lds, s, jobs, work, n_work, counter);
+                       dim3(kMteThreads), lds, s, jobs, work, qbeg, nq,
+                       counters);
   return hipGetLastError();
 }
 
@@ -517,7 +597,10 @@ __global__ void mte_plain_kernel(PlainArgs a) {
   const T* val = static_cast<const T*>(a.val);
   T* cur = static_cast<T*>(a.path_pops) + (size_t)ii * 2 * P;
   T* nxt = cur + P;
-  const T dt = (T)a.dt, sig = (T)a.sigma, half_var = (T)0.5 * sig * sig;
+  const T dt = (T)a.dt;
+  const T l2e = sizeof(T) == 4 ? (T)1.4426950408889634 : (T)1;
+  const T sig = (T)(a.sigma * (sizeof(T) == 4 ? 1.4426950408889634 : 1.0));
+  const T half_var = (T)(0.5 * a.sigma * a.sigma) * l2e;
   for (int j = 0; j < P; ++j) cur[j] = init[j];
   T grm = prm[0];
   for (int t = 0; t < Ty; ++t) {

@@ -103,11 +103,12 @@ RMC_HD_STATIC double rmc_u01d(uint32_t x) {
 /* Device Box-Muller in binary32: 2 normals from 2 uniforms. */
 __device__ __forceinline__ void rmc_box_muller_f(float u0, float u1, float* z0,
                                                  float* z1) {
-  const float r = __fsqrt_rn(-2.0f * __logf(u0));
-  float s, c;
-  sincospif(2.0f * u1, &s, &c);
-  *z0 = r * c;
-  *z1 = r * s;
+  // v_log_f32 is log2; v_sqrt_f32 is the 1-ulp hardware square root.
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u0));
+  // v_sin_f32 / v_cos_f32 take their argument in revolutions: sin(2 pi u1)
+  // for u1 in (0, 1), no range reduction needed.
+  *z0 = r * __builtin_amdgcn_cosf(u1);
+  *z1 = r * __builtin_amdgcn_sinf(u1);
 }
 
 __device__ __forceinline__ void rmc_box_muller_d(double u0, double u1,

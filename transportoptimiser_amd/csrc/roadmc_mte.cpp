@@ -124,6 +124,7 @@ struct JobSet {
   std::vector<int> road_of, species_of;
   std::vector<int64_t> mpack_off, cols_off, part_off, out_off, endpop_off;
   std::vector<int2> work;
+  std::vector<int> qbeg;  // per-XCD queue boundaries into work (persistent)
   size_t mpack_elems = 0, cols_elems = 0, part_elems = 0, out_elems = 0,
          endpop_elems = 0;
   size_t lds = 0;
@@ -171,15 +172,31 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       js.species_of.push_back(s);
     }
   }
-  // Work items, most expensive tiles first (cost ~ BM * Pp^2 * T).
+  // Jobs in decreasing cost (cost ~ n_tiles * BM * Pp^2 * T), assigned to
+  // the per-XCD queues by longest-processing-time-first; each queue lists
+  // its jobs' tiles contiguously.
   std::vector<int> order(js.jobs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    const MteJob &x = js.jobs[a], &y = js.jobs[b];
-    return (double)x.BM * x.Pp * x.Pp * x.T > (double)y.BM * y.Pp * y.Pp * y.T;
-  });
-  for (int j : order)
-    for (int t = 0; t < js.jobs[j].n_tiles; ++t) js.work.push_back(make_int2(j, t));
+  auto cost = [&](int j) {
+    const MteJob& x = js.jobs[j];
+    return (double)x.n_tiles * x.BM * x.Pp * x.Pp * x.T;
+  };
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return cost(a) > cost(b); });
+  const int nq = roadmc::kMteQueues;
+  std::vector<std::vector<int>> queues(nq);
+  std::vector<double> load(nq, 0.0);
+  for (int j : order) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    queues[q].push_back(j);
+    load[q] += cost(j);
+  }
+  js.qbeg.assign(1, 0);
+  for (int q = 0; q < nq; ++q) {
+    for (int j : queues[q])
+      for (int t = 0; t < js.jobs[j].n_tiles; ++t) js.work.push_back(make_int2(j, t));
+    js.qbeg.push_back((int)js.work.size());
+  }
 }
 
 // Device buffers of one job set.
@@ -202,7 +219,7 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
       (rc = grow(b.jobs, js.jobs.size() * sizeof(MteJob))) ||
       (rc = grow(b.work, js.work.size() * sizeof(int2))) ||
       (rc = grow(b.off, js.jobs.size() * sizeof(int64_t))) ||
-      (rc = grow(b.counter, 64)))
+      (rc = grow(b.counter, 256)))
     return rc;
   std::vector<T> mp(js.mpack_elems), cols(js.cols_elems);
   for (size_t j = 0; j < js.jobs.size(); ++j) {
@@ -228,6 +245,9 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
                               hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(b.off.ptr, js.out_off.data(),
                               js.out_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  // Queue boundaries live after the 16 queue counters in the counter buffer.
+  RMC_HIP(ctx, hipMemcpyAsync(static_cast<int*>(b.counter.ptr) + 16, js.qbeg.data(),
+                              js.qbeg.size() * sizeof(int), hipMemcpyHostToDevice, s));
   // The host vectors die at return: wait for the copies.
   RMC_HIP(ctx, hipStreamSynchronize(s));
   return ROADMC_OK;
@@ -354,9 +374,10 @@ int capture_batch_graph(roadmc_batch* bt) {
   RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   hipError_t e = roadmc::launch_fill_i32(static_cast<int*>(b.counter.ptr), 0, 16, s);
   if (e == hipSuccess)
-    e = roadmc::launch_mte_persistent(js.fp64, static_cast<MteJob*>(b.jobs.ptr),
-                                      static_cast<int2*>(b.work.ptr), (int)js.work.size(),
-                                      static_cast<int*>(b.counter.ptr), bt->grid, js.lds, s);
+    e = roadmc::launch_mte_persistent(
+        js.fp64, static_cast<MteJob*>(b.jobs.ptr), static_cast<int2*>(b.work.ptr),
+        static_cast<int*>(b.counter.ptr) + 16, (int)js.qbeg.size() - 1,
+        static_cast<int*>(b.counter.ptr), bt->grid, js.lds, s);
   if (e == hipSuccess)
     e = roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
                                   static_cast<double*>(b.out.ptr),
