@@ -91,6 +91,34 @@ double orc_mte_penalty(double threshold, double end_mean, double end_sd,
                        double confidence_interval, double initial_population,
                        double cost_per_animal);
 
+/* ---- ROV ------------------------------------------------------------- */
+
+/* Price paths of U uncertainties (par[u*7..] = kind, x0, mean, sd, reversion,
+ * jump size, jump probability; chol = lower Cholesky factor, U x U) and the
+ * random controls, on the ROV Philox stream.  states [(T+1)][U][N],
+ * controls [T][N] (may be NULL). */
+int orc_rov_forward(int32_t n_unc, const double* par, const double* chol,
+                    int32_t n_controls, int32_t n_steps, double dt,
+                    int64_t n_paths, uint64_t seed, double* states,
+                    int32_t* controls);
+
+typedef struct {
+  int32_t n_unc, n_steps, n_controls, deg, mode, value_mode, pooled, itm_only,
+      standardize, traffic;
+  int64_t n_paths;
+  double dt, rrr, discount, hysteresis, unit_cost, strike;
+  double weight[8];
+  double flow[8];
+} orc_rov_spec;
+
+/* Backward induction (see rov_oracle.c).  out = {value, control, SD}. */
+int orc_rov_backward(const orc_rov_spec* s, const double* states,
+                     const int32_t* controls, double* out, double* V,
+                     double* beta);
+
+/* A x = b by Gaussian elimination with partial pivoting (A, b overwritten). */
+int orc_solve(int n, double* A, double* b, double* x);
+
 #ifdef __cplusplus
 }
 #endif

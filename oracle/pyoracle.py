@@ -156,3 +156,96 @@ def normal_cdf_inverse(p: float) -> float:
 
 def mte_penalty(threshold, mean, sd, ci, init_pop, cost) -> float:
     return lib().orc_mte_penalty(threshold, mean, sd, ci, init_pop, cost)
+
+
+# ---- ROV --------------------------------------------------------------------
+
+class OrcRovSpec(C.Structure):
+    _fields_ = [
+        ("n_unc", C.c_int32), ("n_steps", C.c_int32), ("n_controls", C.c_int32),
+        ("deg", C.c_int32), ("mode", C.c_int32), ("value_mode", C.c_int32),
+        ("pooled", C.c_int32), ("itm_only", C.c_int32), ("standardize", C.c_int32),
+        ("traffic", C.c_int32), ("n_paths", C.c_int64), ("dt", C.c_double),
+        ("rrr", C.c_double), ("discount", C.c_double), ("hysteresis", C.c_double),
+        ("unit_cost", C.c_double), ("strike", C.c_double), ("weight", C.c_double * 8),
+        ("flow", C.c_double * 8),
+    ]
+
+
+def _rov_lib():
+    h = lib()
+    if not getattr(h, "_rov_ready", False):
+        h.orc_rov_forward.argtypes = [C.c_int32, _dp, _dp, C.c_int32, C.c_int32, C.c_double,
+                                      C.c_int64, C.c_uint64, _dp, _ip]
+        h.orc_rov_forward.restype = C.c_int
+        h.orc_rov_backward.argtypes = [C.POINTER(OrcRovSpec), _dp, _ip, _dp, _dp, _dp]
+        h.orc_rov_backward.restype = C.c_int
+        h.orc_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        h.orc_solve.restype = C.c_int
+        h._rov_ready = True
+    return h
+
+
+def solve(A, b):
+    A = np.array(A, dtype=np.float64, order="C")
+    b = np.array(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    rc = _rov_lib().orc_solve(len(b), A.ctypes.data_as(_dp), b.ctypes.data_as(_dp),
+                              x.ctypes.data_as(_dp))
+    if rc:
+        raise np.linalg.LinAlgError("singular")
+    return x
+
+
+def _chol(pr):
+    U = len(pr.uncertainties)
+    if pr.correlation is None:
+        return np.eye(U)
+    return np.linalg.cholesky(np.asarray(pr.correlation, dtype=np.float64))
+
+
+def rov_forward(pr):
+    """States (T+1, U, N) and controls (T, N) of an ROV problem on its stream."""
+    U, T, N = len(pr.uncertainties), pr.n_steps, pr.n_paths
+    par = np.array([[u.kind, u.x0, u.mean, u.sd, u.reversion, u.jump_size, u.jump_prob]
+                    for u in pr.uncertainties], dtype=np.float64)
+    L = np.ascontiguousarray(_chol(pr))
+    states = np.zeros((T + 1, U, N))
+    ctrl = np.zeros((T, N), dtype=np.int32)
+    rc = _rov_lib().orc_rov_forward(U, par.ctypes.data_as(_dp), L.ctypes.data_as(_dp),
+                                    pr.n_controls, T, pr.time_step, N, pr.seed,
+                                    states.ctypes.data_as(_dp), ctrl.ctypes.data_as(_ip))
+    if rc:
+        raise RuntimeError("orc_rov_forward failed")
+    return states, ctrl
+
+
+def rov_backward(pr, states, controls):
+    """Backward induction; returns dict(value, first_control, value_sd, V, beta)."""
+    U, T, N = len(pr.uncertainties), pr.n_steps, pr.n_paths
+    s = OrcRovSpec()
+    s.n_unc, s.n_steps, s.n_controls = U, T, len(pr.flow)
+    s.deg, s.mode, s.value_mode = pr.basis_degree, pr.mode, pr.value_mode
+    s.pooled, s.itm_only, s.standardize = pr.pooled, pr.itm_only, pr.standardize
+    s.traffic = pr.traffic_index if pr.mode == 0 else -1
+    s.n_paths, s.dt, s.rrr, s.discount = N, pr.time_step, pr.rrr, pr.discount
+    s.hysteresis, s.unit_cost, s.strike = pr.hysteresis, pr.unit_cost, pr.strike
+    w = pr.profit_weight if pr.profit_weight is not None else np.zeros(U)
+    for i in range(U):
+        s.weight[i] = float(w[i])
+    for i, f in enumerate(pr.flow):
+        s.flow[i] = float(f)
+    K = pr.basis_size()
+    G = 1 if (pr.mode == 1 or pr.pooled) else len(pr.flow)
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    ct = np.ascontiguousarray(controls, dtype=np.int32)
+    out = np.zeros(3)
+    V = np.zeros((T + 1, N))
+    beta = np.zeros((T, G, K))
+    rc = _rov_lib().orc_rov_backward(C.byref(s), st.ctypes.data_as(_dp), ct.ctypes.data_as(_ip),
+                                     out.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                                     beta.ctypes.data_as(_dp))
+    if rc:
+        raise RuntimeError("orc_rov_backward failed")
+    return {"value": out[0], "first_control": int(out[1]), "value_sd": out[2], "V": V,
+            "beta": beta}

@@ -1,0 +1,368 @@
+/*
+ * CPU oracle, ROV part.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restates forwardPathKernel's price law (SimulateGPU.cu:930-952), randControls
+ * (:720-730), the normal equations of multiQuadReg (:6092-6207) solved with
+ * proper partial pivoting (the reference's pivot test is broken, :6312),
+ * keepValidPoints (:5999-6043), backwardInduction's decision rule
+ * (:2333-2784) and firstPeriodInduction (:2210-2330), on the Philox stream of
+ * transportoptimiser_amd/csrc/rov_kernels.hip.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define MAXU 8
+#define MAXC 8
+#define MAXK 35
+
+static double u01(uint32_t x) { return ((double)(x >> 9) + 0.5) / 8388608.0; }
+static const double kTwoPi = 6.283185307179586476925286766559;
+
+static void normal4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
+                    uint32_t tag, double z[4]) {
+  uint32_t ctr[4], key[2], x[4];
+  double r;
+  ctr[0] = c0;
+  ctr[1] = c1;
+  ctr[2] = c2;
+  ctr[3] = tag;
+  key[0] = (uint32_t)(seed & 0xffffffffu);
+  key[1] = (uint32_t)(seed >> 32);
+  orc_philox4x32_10(ctr, key, x);
+  r = sqrt(-2.0 * log(u01(x[0])));
+  z[0] = r * cos(kTwoPi * u01(x[1]));
+  z[1] = r * sin(kTwoPi * u01(x[1]));
+  r = sqrt(-2.0 * log(u01(x[2])));
+  z[2] = r * cos(kTwoPi * u01(x[3]));
+  z[3] = r * sin(kTwoPi * u01(x[3]));
+}
+
+static void words(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
+                  uint32_t tag, uint32_t out[4]) {
+  uint32_t ctr[4], key[2];
+  ctr[0] = c0;
+  ctr[1] = c1;
+  ctr[2] = c2;
+  ctr[3] = tag;
+  key[0] = (uint32_t)(seed & 0xffffffffu);
+  key[1] = (uint32_t)(seed >> 32);
+  orc_philox4x32_10(ctr, key, out);
+}
+
+int orc_rov_forward(int32_t n_unc, const double* par /* [U][7] */,
+                    const double* chol /* [U][U] lower */, int32_t n_controls,
+                    int32_t n_steps, double dt, int64_t n_paths, uint64_t seed,
+                    double* states, int32_t* controls) {
+  const int U = n_unc, T = n_steps;
+  const int64_t N = n_paths;
+  const uint32_t tag = (0u << 8) | (0u << 2) | 2u;
+  int64_t p;
+  if (U < 1 || U > MAXU) return 1;
+  for (p = 0; p < N; ++p) {
+    double x[MAXU];
+    int t, u, b, i;
+    for (u = 0; u < U; ++u) {
+      x[u] = par[u * 7 + 1];
+      states[(size_t)u * N + p] = x[u];
+    }
+    for (t = 0; t < T; ++t) {
+      double z[MAXU], zj[MAXU], uj[MAXU], zc[MAXU];
+      for (b = 0; b < (U + 3) / 4; ++b) {
+        double w[4];
+        uint32_t r[4];
+        normal4(seed, (uint32_t)p, (uint32_t)t, (uint32_t)b, tag, w);
+        for (i = 0; i < 4 && 4 * b + i < U; ++i) z[4 * b + i] = w[i];
+        normal4(seed, (uint32_t)p, (uint32_t)t, (uint32_t)(2 + b), tag, w);
+        for (i = 0; i < 4 && 4 * b + i < U; ++i) zj[4 * b + i] = w[i];
+        words(seed, (uint32_t)p, (uint32_t)t, (uint32_t)(4 + b), tag, r);
+        for (i = 0; i < 4 && 4 * b + i < U; ++i) uj[4 * b + i] = u01(r[i]);
+      }
+      for (u = 0; u < U; ++u) {
+        double s = 0.0;
+        int v;
+        for (v = 0; v <= u; ++v) s += chol[u * U + v] * z[v];
+        zc[u] = s;
+      }
+      for (u = 0; u < U; ++u) {
+        const double* q = par + u * 7;
+        const int kind = (int)q[0];
+        const double xv = x[u];
+        if (kind == 1) {
+          x[u] = xv + q[4] * (q[2] - xv) * dt + q[3] * sqrt(dt) * zc[u];
+        } else if (kind == 2) {
+          x[u] = xv * exp((q[2] - 0.5 * q[3] * q[3]) * dt + q[3] * sqrt(dt) * zc[u]);
+        } else {
+          const double jump = (uj[u] < q[6]) ? 1.0 : 0.0;
+          const double jsz = zj[u] * q[5] * q[5] - q[5] * q[5] / 2.0;
+          x[u] = xv + q[4] * (q[2] - xv) * dt + xv * q[3] * zc[u] +
+                 (exp(jsz) - 1.0) * xv * jump;
+        }
+        states[((size_t)(t + 1) * U + u) * N + p] = x[u];
+      }
+      if (controls) {
+        uint32_t r[4];
+        int c;
+        words(seed, (uint32_t)p, (uint32_t)t, 6u, tag, r);
+        c = (int)(u01(r[0]) * n_controls);
+        if (c >= n_controls) c = n_controls - 1;
+        controls[(size_t)t * N + p] = c;
+      }
+    }
+  }
+  return 0;
+}
+
+static int basis(const double* x, int U, int deg, double* phi) {
+  int k = 0, i, j, l;
+  phi[k++] = 1.0;
+  if (deg >= 1)
+    for (i = 0; i < U; ++i) phi[k++] = x[i];
+  if (deg >= 2)
+    for (i = 0; i < U; ++i)
+      for (j = i; j < U; ++j) phi[k++] = x[i] * x[j];
+  if (deg >= 3)
+    for (i = 0; i < U; ++i)
+      for (j = i; j < U; ++j)
+        for (l = j; l < U; ++l) phi[k++] = x[i] * x[j] * x[l];
+  return k;
+}
+
+/* Solves A x = b (n x n, row-major, overwritten) by Gaussian elimination with
+ * partial pivoting on the largest |A[i][col]|.  Returns 0 on success. */
+int orc_solve(int n, double* A, double* b, double* x) {
+  int col, i, j;
+  for (col = 0; col < n; ++col) {
+    int best = col;
+    double bv = fabs(A[col * n + col]);
+    for (i = col + 1; i < n; ++i)
+      if (fabs(A[i * n + col]) > bv) {
+        bv = fabs(A[i * n + col]);
+        best = i;
+      }
+    if (!(bv > 1e-300)) return 1;
+    if (best != col) {
+      for (j = 0; j < n; ++j) {
+        double t = A[col * n + j];
+        A[col * n + j] = A[best * n + j];
+        A[best * n + j] = t;
+      }
+      {
+        double t = b[col];
+        b[col] = b[best];
+        b[best] = t;
+      }
+    }
+    for (i = col + 1; i < n; ++i) {
+      const double f = A[i * n + col] / A[col * n + col];
+      for (j = col; j < n; ++j) A[i * n + j] -= f * A[col * n + j];
+      b[i] -= f * b[col];
+    }
+  }
+  for (i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (j = i + 1; j < n; ++j) s -= A[i * n + j] * x[j];
+    x[i] = s / A[i * n + i];
+  }
+  return 0;
+}
+
+/* Backward induction on given states/controls.  Layouts as in
+ * roadmc_rov_problem.  Outputs: out[3] = {value, first control, SD};
+ * V [(T+1)][N]; beta [T][G][K] (G = 1 when pooled or american). */
+int orc_rov_backward(const orc_rov_spec* s, const double* states,
+                     const int32_t* controls, double* out, double* V,
+                     double* beta) {
+  const int U = s->n_unc, T = s->n_steps, C = s->mode == 1 ? 2 : s->n_controls;
+  const int64_t N = s->n_paths;
+  const int G = (s->mode == 1 || s->pooled) ? 1 : C;
+  const double disc = s->discount > 0 ? s->discount : 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  int K, t, c;
+  int64_t p;
+  double* scale = NULL;
+  {
+    double x[MAXU], phi[MAXK];
+    int u;
+    for (u = 0; u < U; ++u) x[u] = 0.0;
+    K = basis(x, U, s->deg, phi);
+  }
+  if (s->standardize) {
+    scale = (double*)malloc(sizeof(double) * (size_t)(T + 1) * U * 2);
+    for (t = 0; t <= T; ++t) {
+      int u;
+      for (u = 0; u < U; ++u) {
+        const double* xs = states + ((size_t)t * U + u) * N;
+        double a = 0, b = 0, m, v, sd;
+        for (p = 0; p < N; ++p) {
+          a += xs[p];
+          b += xs[p] * xs[p];
+        }
+        m = a / N;
+        v = b / N - m * m;
+        sd = v > 1e-300 ? sqrt(v) : 1.0;
+        scale[((size_t)t * U + u) * 2] = m;
+        scale[((size_t)t * U + u) * 2 + 1] = (sd > 1e-12 * (fabs(m) + 1.0)) ? 1.0 / sd : 1.0;
+      }
+    }
+  }
+  memset(beta, 0, sizeof(double) * (size_t)T * G * K);
+#define X(tt, uu, pp) states[((size_t)(tt) * U + (uu)) * N + (pp)]
+  /* terminal step */
+  if (s->mode == 1) {
+    for (p = 0; p < N; ++p) V[(size_t)T * N + p] = fmax(s->strike - X(T, 0, p), 0.0);
+  } else {
+    for (p = 0; p < N; ++p) {
+      double up = s->unit_cost, m = 1.0, best, pay;
+      int u, bc = 0;
+      for (u = 0; u < U; ++u) up += s->weight[u] * X(T - 1, u, p);
+      if (s->traffic >= 0) m = X(T - 1, s->traffic, p);
+      best = fmin(s->flow[0] * m * up, 0.0);
+      for (c = 1; c < C; ++c) {
+        pay = fmin(s->flow[c] * m * up, 0.0);
+        if (pay < s->hysteresis * best) {
+          best = pay;
+          bc = c;
+        }
+      }
+      (void)bc;
+      V[(size_t)(T - 1) * N + p] = best;
+    }
+  }
+  {
+    const int t_hi = s->mode == 1 ? T - 1 : T - 2;
+    for (t = t_hi; t >= 1; --t) {
+      /* normal equations per group */
+      double* A = (double*)calloc((size_t)G * K * K, sizeof(double));
+      double* bb = (double*)calloc((size_t)G * K, sizeof(double));
+      double* cnt = (double*)calloc((size_t)G, sizeof(double));
+      int g, i, j;
+      for (p = 0; p < N; ++p) {
+        double x[MAXU], phi[MAXK], y;
+        int u;
+        if (s->mode == 1) {
+          const double e = s->strike - X(t, 0, p);
+          if (!(e > 0.0) && s->itm_only) continue;
+          g = 0;
+        } else {
+          double up = s->unit_cost;
+          for (u = 0; u < U; ++u) up += s->weight[u] * X(t, u, p);
+          if (s->itm_only && up > 0.0) continue;
+          g = s->pooled ? 0 : controls[(size_t)t * N + p];
+        }
+        for (u = 0; u < U; ++u) {
+          double v = X(t, u, p);
+          if (scale) v = (v - scale[((size_t)t * U + u) * 2]) * scale[((size_t)t * U + u) * 2 + 1];
+          x[u] = v;
+        }
+        basis(x, U, s->deg, phi);
+        y = V[(size_t)(t + 1) * N + p];
+        for (i = 0; i < K; ++i) {
+          for (j = 0; j < K; ++j) A[((size_t)g * K + i) * K + j] += phi[i] * phi[j];
+          bb[(size_t)g * K + i] += phi[i] * y;
+        }
+        cnt[g] += 1.0;
+      }
+      for (g = 0; g < G; ++g) {
+        double* coef = beta + ((size_t)t * G + g) * K;
+        if (cnt[g] >= K) {
+          if (orc_solve(K, A + (size_t)g * K * K, bb + (size_t)g * K, coef) != 0)
+            for (i = 0; i < K; ++i) coef[i] = 0.0;
+        }
+      }
+      free(A);
+      free(bb);
+      free(cnt);
+      /* decisions */
+      for (p = 0; p < N; ++p) {
+        double x[MAXU], phi[MAXK];
+        int u;
+        for (u = 0; u < U; ++u) {
+          double v = X(t, u, p);
+          if (scale) v = (v - scale[((size_t)t * U + u) * 2]) * scale[((size_t)t * U + u) * 2 + 1];
+          x[u] = v;
+        }
+        basis(x, U, s->deg, phi);
+        if (s->mode == 1) {
+          const double e = fmax(s->strike - X(t, 0, p), 0.0);
+          const double later = disc * V[(size_t)(t + 1) * N + p];
+          int ex = 0;
+          if (e > 0.0 || !s->itm_only) {
+            const double* b = beta + (size_t)t * K;
+            double cont = 0.0;
+            for (i = 0; i < K; ++i) cont += phi[i] * b[i];
+            ex = e > disc * cont;
+          }
+          V[(size_t)t * N + p] = ex ? e : later;
+        } else {
+          double up = s->unit_cost, m = 1.0, cur[MAXC], pay[MAXC], best;
+          int bc = 0;
+          for (u = 0; u < U; ++u) up += s->weight[u] * X(t, u, p);
+          if (s->traffic >= 0) m = X(t, s->traffic, p);
+          for (c = 0; c < C; ++c) {
+            const double* b = beta + ((size_t)t * G + (s->pooled ? 0 : c)) * K;
+            double cont = 0.0;
+            for (i = 0; i < K; ++i) cont += phi[i] * b[i];
+            cont *= disc;
+            if (cont > 0.0) cont = 0.0;
+            cur[c] = s->flow[c] * m * up;
+            pay[c] = cur[c] + cont;
+          }
+          best = pay[0];
+          for (c = 1; c < C; ++c)
+            if (pay[c] < s->hysteresis * best) {
+              best = pay[c];
+              bc = c;
+            }
+          V[(size_t)t * N + p] = s->value_mode == 1
+                                     ? cur[bc] + disc * V[(size_t)(t + 1) * N + p]
+                                     : best;
+        }
+      }
+    }
+  }
+  /* time 0 */
+  {
+    double s1[MAXC] = {0}, s2[MAXC] = {0}, n[MAXC] = {0}, mean[MAXC], sd[MAXC];
+    const int GG = (s->mode == 1 || s->pooled) ? 1 : C;
+    for (p = 0; p < N; ++p) {
+      const int g = GG == 1 ? 0 : controls[p];
+      const double v = disc * V[(size_t)N + p];
+      s1[g] += v;
+      s2[g] += v * v;
+      n[g] += 1.0;
+    }
+    for (c = 0; c < GG; ++c) {
+      mean[c] = n[c] > 0 ? s1[c] / n[c] : 0.0;
+      sd[c] = n[c] > 0 ? sqrt(fmax(s2[c] / n[c] - mean[c] * mean[c], 0.0)) : 0.0;
+    }
+    if (s->mode == 1) {
+      out[0] = mean[0];
+      out[1] = 0;
+      out[2] = sd[0];
+    } else {
+      double up = s->unit_cost, m = 1.0, best = 0.0;
+      int u, bc = -1;
+      for (u = 0; u < U; ++u) up += s->weight[u] * X(0, u, 0);
+      if (s->traffic >= 0) m = X(0, s->traffic, 0);
+      for (c = 0; c < C; ++c) {
+        const int g = GG == 1 ? 0 : c;
+        double pay;
+        if (n[g] <= 0) continue;
+        pay = mean[g] + s->flow[c] * m * up;
+        if (pay > 0.0) pay = 0.0;
+        if (bc < 0 || pay < best) {
+          best = pay;
+          bc = c;
+        }
+      }
+      out[0] = best;
+      out[1] = bc < 0 ? 0 : bc;
+      out[2] = bc < 0 ? 0.0 : sd[GG == 1 ? 0 : bc];
+    }
+  }
+#undef X
+  free(scale);
+  return 0;
+}

@@ -490,6 +490,16 @@ __global__ void exp_pv_kernel(ExpPvArgs a) {
   a.out[p] = value;
 }
 
+hipError_t rov_set_lds_limits() {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rov_assemble_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)rov_assemble_lds(kRovMaxK));
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rov_solve_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             kRovMaxC * (kRovMaxK * (kRovMaxK + 1) / 2 + kRovMaxK + 1) * 8);
+}
+
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
   hipLaunchKernelGGL(rov_forward_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a);

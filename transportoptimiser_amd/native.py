@@ -71,6 +71,69 @@ class MteResult(C.Structure):
     ]
 
 
+SDE_MRJD = 0
+SDE_OU = 1
+SDE_GBM = 2
+ROV_SWITCHING = 0
+ROV_AMERICAN = 1
+VALUE_REGRESSED = 0
+VALUE_REALIZED = 1
+_i32p = C.POINTER(C.c_int32)
+
+
+class Uncertainty(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("x0", C.c_double),
+        ("mean", C.c_double),
+        ("sd", C.c_double),
+        ("reversion", C.c_double),
+        ("jump_size", C.c_double),
+        ("jump_prob", C.c_double),
+    ]
+
+
+class RovProblem(C.Structure):
+    _fields_ = [
+        ("n_uncertainties", C.c_int32),
+        ("unc", C.POINTER(Uncertainty)),
+        ("correlation", _dp),
+        ("n_controls", C.c_int32),
+        ("flow", _dp),
+        ("unit_cost", C.c_double),
+        ("profit_weight", _dp),
+        ("traffic_index", C.c_int32),
+        ("n_steps", C.c_int32),
+        ("time_step", C.c_double),
+        ("rrr", C.c_double),
+        ("discount", C.c_double),
+        ("n_paths", C.c_int64),
+        ("basis_degree", C.c_int32),
+        ("standardize", C.c_int32),
+        ("itm_only", C.c_int32),
+        ("pooled", C.c_int32),
+        ("hysteresis", C.c_double),
+        ("mode", C.c_int32),
+        ("value_mode", C.c_int32),
+        ("strike", C.c_double),
+        ("seed", C.c_uint64),
+        ("states", _dp),
+        ("controls", _i32p),
+    ]
+
+
+class RovResult(C.Structure):
+    _fields_ = [
+        ("value", _dp),
+        ("value_sd", _dp),
+        ("first_control", _i32p),
+        ("coeffs", _dp),
+        ("cond_exp", _dp),
+        ("opt_control", _i32p),
+        ("states", _dp),
+    ]
+
+
 # Every symbol include/roadmc.h declares: name -> (restype, argtypes).
 SIGNATURES: dict[str, tuple] = {
     "roadmc_device_count": (C.c_int, [_ip]),
@@ -96,6 +159,10 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_mte_penalty": (C.c_int, [C.c_int32, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]),
     "roadmc_patch_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                        C.c_int32, C.c_int32, C.c_int32, _fp]),
+    "roadmc_rov_basis_size": (C.c_int, [C.c_int32, C.c_int32, _ip]),
+    "roadmc_rov_run": (C.c_int, [C.c_void_p, C.POINTER(RovProblem), C.POINTER(RovResult)]),
+    "roadmc_exp_pv": (C.c_int, [C.c_void_p, C.POINTER(Uncertainty), C.c_int32, C.c_double,
+                                C.c_double, C.c_double, C.c_int64, C.c_uint64, _dp, _dp]),
 }
 
 _LIB = None
