@@ -175,6 +175,24 @@ int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
                          uint32_t species, int32_t n_paths, int32_t n_years,
                          int32_t n_patches, float* out);
 
+/* ---- Adjoint ---------------------------------------------------------------- */
+
+/* Gradient of J = d_objective * mean total end population (one species,
+ * LOGNORMAL law, post-growth endpoint, common random numbers) with respect to
+ * the crossing mortality m (surv = 1 - m) and the growth rates r.  No
+ * counterpart in the reference, which has no derivatives; it extends
+ * simulateMTECUDA (SimulateGPU.cu:3650-3864).  The forward pass stores the
+ * state every checkpoint_every years; the backward pass recomputes each
+ * segment from its checkpoint. */
+typedef struct roadmc_adjoint roadmc_adjoint;
+int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
+                           int32_t checkpoint_every, roadmc_adjoint** out,
+                           double* objective);
+/* grad_mortality: [P*P] column-major like trans; grad_growth: [P]. */
+int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
+                           double* grad_mortality, double* grad_growth);
+void roadmc_adjoint_destroy(roadmc_adjoint* adj);
+
 /* ---- ROV: price paths and regression Monte Carlo --------------------------- */
 
 /* Price laws of one exogenous uncertainty. */

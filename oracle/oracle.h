@@ -119,6 +119,15 @@ int orc_rov_backward(const orc_rov_spec* s, const double* states,
 /* A x = b by Gaussian elimination with partial pivoting (A, b overwritten). */
 int orc_solve(int n, double* A, double* b, double* x);
 
+/* ---- Adjoint ---------------------------------------------------------- */
+
+/* J = mean total end population (LOGNORMAL law, post-growth), its gradient
+ * with respect to the crossing mortality (column-major [P*P], S = 1 - m) and
+ * the growth rates [P]. */
+int orc_mte_adjoint(const orc_species* sp, uint32_t road_id, int32_t n_years,
+                    int64_t n_paths, uint64_t seed, double* objective,
+                    double* grad_m, double* grad_r);
+
 #ifdef __cplusplus
 }
 #endif

@@ -249,3 +249,38 @@ def rov_backward(pr, states, controls):
         raise RuntimeError("orc_rov_backward failed")
     return {"value": out[0], "first_control": int(out[1]), "value_sd": out[2], "V": V,
             "beta": beta}
+
+
+# ---- Adjoint ----------------------------------------------------------------
+
+def mte_adjoint(sd, road_id: int, n_years: int, n_paths: int, seed: int):
+    """FP64 objective (mean total end population) and its gradients with
+    respect to crossing mortality (P, P) and growth rates (P,)."""
+    h = lib()
+    h.orc_mte_adjoint.argtypes = [C.POINTER(OrcSpecies), C.c_uint32, C.c_int32, C.c_int64,
+                                  C.c_uint64, _dp, _dp, _dp]
+    h.orc_mte_adjoint.restype = C.c_int
+    keep = []
+
+    def arr(a, order="C"):
+        if a is None:
+            return None
+        b = np.require(np.asarray(a, dtype=np.float64), requirements=[order])
+        keep.append(b)
+        return b.ctypes.data_as(_dp)
+
+    sp = OrcSpecies()
+    sp.n_patches = sd.n_patches
+    sp.trans = arr(sd.trans, "F")
+    sp.surv = arr(sd.surv, "F")
+    sp.init_pop = arr(sd.init_pop)
+    sp.capacity = arr(sd.capacity)
+    sp.growth = arr(sd.growth)
+    sp.noise_sd = float(sd.noise_sd)
+    P = sd.n_patches
+    obj = C.c_double()
+    gm = np.zeros(P * P)
+    gr = np.zeros(P)
+    h.orc_mte_adjoint(C.byref(sp), road_id, n_years, n_paths, seed, C.byref(obj),
+                      gm.ctypes.data_as(_dp), gr.ctypes.data_as(_dp))
+    return obj.value, gm.reshape(P, P, order="F"), gr

@@ -1,0 +1,196 @@
+// Device building blocks shared by the MTE and adjoint kernels: the MFMA
+// traits (operand and accumulator lane layouts of the two matrix
+// instructions used), the k-permuted LDS layout, the packed B-operand order,
+// the growth law and the per-year product loop.
+#ifndef ROADMC_MTE_COMMON_H
+#define ROADMC_MTE_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include "mte_kernels.h"
+#include "rng.h"
+
+namespace roadmc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+enum { kLawLognormal = 0, kLawRefCpu = 1, kLawRefGpu = 2 };
+enum { kEndPost = 0, kEndPre = 1 };
+
+// v_mfma_f32_32x32x2_f32: A lane l = A[l%32][l/32]; B lane l = B[l/32][l%32];
+// D item r of lane l = D[8(r/4) + 4(l/32) + r%4][l%32].
+struct TraitsF32 {
+  using T = float;
+  using VecT = floatx4;
+  using AccT = floatx16;
+  static constexpr int TM = 32, TN = 32, KV = 4, NACC = 16, NT = 4, PAD = 4;
+  static constexpr bool kQuadItems = true;  // items 4q..4q+3 are 4 paths
+  __device__ static AccT mfma(float a, float b, AccT c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int acc_row(int lane, int r) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+  }
+  __device__ static int acc_col(int lane) { return lane & 31; }
+  __device__ static int a_row(int lane) { return lane & 31; }
+  __device__ static int k_lane(int lane) { return lane >> 5; }
+  // k = 8g + 2s + h  ->  8g + 4h + s
+  __host__ __device__ static int perm(int k) {
+    return (k & ~7) | ((k & 1) << 2) | ((k >> 1) & 3);
+  }
+  __device__ static float expo(float x) { return __expf(x); }
+  __device__ static void normal4(uint32_t q, uint32_t y, uint32_t p,
+                                 uint32_t tag, uint32_t k0, uint32_t k1,
+                                 float z[4]) {
+    rmc_normal4_f(q, y, p, tag, k0, k1, z);
+  }
+};
+
+// v_mfma_f64_16x16x4_f64: A lane l = A[l%16][l/16]; B lane l = B[l/16][l%16];
+// D item r of lane l = D[l/16 + 4r][l%16].
+struct TraitsF64 {
+  using T = double;
+  using VecT = doublex2;
+  using AccT = doublex4;
+  static constexpr int TM = 16, TN = 16, KV = 2, NACC = 4, NT = 8, PAD = 2;
+  static constexpr bool kQuadItems = false;
+  __device__ static AccT mfma(double a, double b, AccT c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int acc_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  __device__ static int acc_col(int lane) { return lane & 15; }
+  __device__ static int a_row(int lane) { return lane & 15; }
+  __device__ static int k_lane(int lane) { return lane >> 4; }
+  // k = 8g + 4s + h  ->  8g + 2h + s
+  __host__ __device__ static int perm(int k) {
+    return (k & ~7) | ((k & 3) << 1) | ((k >> 2) & 1);
+  }
+  __device__ static double expo(double x) { return exp(x); }
+  __device__ static void normal4(uint32_t q, uint32_t y, uint32_t p,
+                                 uint32_t tag, uint32_t k0, uint32_t k1,
+                                 double z[4]) {
+    rmc_normal4_d(q, y, p, tag, k0, k1, z);
+  }
+};
+
+__host__ __device__ static inline size_t align16(size_t x) {
+  return (x + 15) & ~(size_t)15;
+}
+
+template <class TR>
+__host__ __device__ static inline size_t lds_bytes_t(int BM, int Pp) {
+  using T = typename TR::T;
+  return align16((size_t)BM * (Pp + TR::PAD) * sizeof(T)) +
+         align16((size_t)BM * sizeof(T)) + 8 * 4 * sizeof(double);
+}
+
+template <typename T>
+__device__ __forceinline__ T grow(int law, T x, T z, T K, T Kinv, T r, T sig,
+                                  T half_var, T dt, T mu, T sd, T grm, T lv) {
+  if (law == kLawRefCpu) {
+    const T g = dt * (mu + sd * z) * (T)0.01;
+    return x * ((T)1 + g * (K - x) * Kinv);
+  }
+  if (law == kLawRefGpu) {
+    const T g = lv * z * grm + grm;
+    return x * ((T)1 + g * (K - x) * Kinv);
+  }
+  T e;
+  if constexpr (sizeof(T) == 4)
+    // sig and half_var arrive pre-scaled by log2(e): one FMA and v_exp_f32.
+    e = __builtin_amdgcn_exp2f(fmaf(sig, z, -half_var));
+  else
+    e = exp(sig * z - half_var);
+  return x * ((T)1 + r * ((T)1 - x * Kinv)) * e;
+}
+
+template <typename T>
+__device__ __forceinline__ void path_scalars(uint32_t path, uint32_t year,
+                                             uint32_t tag, uint32_t k0,
+                                             uint32_t k1, T* zb, T* zj,
+                                             T* uj) {
+  const rmc_u4 x = rmc_philox(path, year, 0u, tag, k0, k1);
+  if constexpr (sizeof(T) == 4) {
+    rmc_box_muller_f(rmc_u01f(x.x), rmc_u01f(x.y), zb, zj);
+    *uj = rmc_u01f(x.z);
+  } else {
+    rmc_box_muller_d(rmc_u01d(x.x), rmc_u01d(x.y), zb, zj);
+    *uj = rmc_u01d(x.z);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Global-address-space view of a device pointer, so that loads compile to
+// global_load (not flat_load).
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(1))) V* as_global(const V* p) {
+  return (const __attribute__((address_space(1))) V*)(p);
+}
+
+// One year's product for the NTILE output tiles of a wave:
+// acc[t] += A[rows of tile t, :] . M[:, cols of tile t].
+// Two register stages ping-pong over the k-groups (the k-group count Pp/8 is a
+// multiple of 4): the B operands and LDS A operands of k-group kg+1 are in
+// flight while the MFMAs of k-group kg run.
+// On entry b0 holds the B operands of k-group 0; on exit it holds them again
+// (the last stage-0 load re-fetches k-group 0, which the next year starts
+// with, so that load's latency is hidden behind the epilogue).
+template <class TR, int NTILE>
+__device__ __forceinline__ void year_product(typename TR::AccT* acc,
+                                             typename TR::VecT* b0,
+                                             const typename TR::VecT* Bp,
+                                             const typename TR::T* A,
+                                             const int* trow, const int* tcol,
+                                             int nkg, int LDA, int arow,
+                                             int akoff, int lane) {
+  using T = typename TR::T;
+  using VecT = typename TR::VecT;
+  constexpr int TM = TR::TM, KV = TR::KV;
+  const __attribute__((address_space(1))) VecT* bp[NTILE];
+  const T* ap[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) {
+    bp[t] = as_global(Bp + (size_t)tcol[t] * nkg * 64 + lane);
+    ap[t] = A + (trow[t] * TM + arow) * LDA + akoff;
+  }
+  VecT b1[NTILE], a0[NTILE], a1[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
+  for (int kg = 0; kg < nkg; kg += 2) {
+    // stage 1 loads (k-group kg+1 < nkg always: nkg is even)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      b1[t] = bp[t][(size_t)(kg + 1) * 64];
+      a1[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 1) * 8);
+    }
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[t][s], b0[t][s], acc[t]);
+    }
+    // stage 0 loads for k-group kg+2; after the last pass, k-group 0 again
+    const int k2 = (kg + 2 < nkg) ? kg + 2 : 0;
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      b0[t] = bp[t][(size_t)k2 * 64];
+      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + k2 * 8);
+    }
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+#pragma unroll
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[t][s], b1[t][s], acc[t]);
+    }
+  }
+}
+
+}  // namespace roadmc
+
+#endif

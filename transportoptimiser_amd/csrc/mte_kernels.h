@@ -13,6 +13,8 @@ struct MteJob {
   const void* mpack;   // movement matrix in MFMA B-operand order, Pp*Pp
   const void* cols;    // 4*Pp: capacity, 1/capacity, growth rate, initial pop
   double* end_pops;    // [N] per-path end totals, or nullptr
+  void* ckpt;          // adjoint checkpoints [T/ckpt_every][N][Pp], or nullptr
+  int32_t ckpt_every;
   double* partial;     // [n_tiles][3 + Pp]: count, sum, sum of squares,
                        // per-patch sums of one tile
   double params[8];    // packed species parameters (SimulateGPU.cu:3690-3714)

@@ -779,3 +779,235 @@ extern "C" int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t roa
     return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("patch normals: ") + hipGetErrorString(e));
   return ROADMC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Adjoint: forward with checkpoints, backward by segments.
+
+#include "adj_kernels.h"
+
+struct roadmc_adjoint {
+  roadmc_ctx* ctx = nullptr;
+  bool fp64 = false;
+  MteJob job{};
+  int P = 0, L = 10;
+  std::vector<double> trans;  // column-major, for dJ/dm = -T o dJ/dM
+  rmc_job_buffers bufs;       // mpack, cols, partial, out, endpop(unused), jobs, work, off
+  rmc_devbuf mtpack, ckpt, ns, np, mu, lam, gradr, gradm, gpart, rpart;
+};
+
+namespace {
+
+template <typename T>
+int adjoint_upload(roadmc_adjoint* adj, const roadmc_species& sp, double tol) {
+  roadmc_ctx* ctx = adj->ctx;
+  const int Pp = adj->job.Pp, P = adj->P;
+  const bool fp64 = adj->fp64;
+  std::vector<T> mp((size_t)Pp * Pp), mt((size_t)Pp * Pp, (T)0), cols(4 * (size_t)Pp);
+  pack_species<T>(sp, Pp, tol, fp64, mp.data(), cols.data());
+  for (int k = 0; k < P; ++k)
+    for (int j = 0; j < P; ++j)
+      mt[roadmc::mte_pack_index(fp64, Pp, j, k)] = (T)movement(sp, P, k, j, tol);
+  int rc;
+  if ((rc = rmc_buf_alloc_exact(ctx, adj->bufs.mpack, mp.size() * sizeof(T))) ||
+      (rc = rmc_buf_alloc_exact(ctx, adj->mtpack, mt.size() * sizeof(T))) ||
+      (rc = rmc_buf_alloc_exact(ctx, adj->bufs.cols, cols.size() * sizeof(T))))
+    return rc;
+  RMC_HIP(ctx, hipMemcpy(adj->bufs.mpack.ptr, mp.data(), mp.size() * sizeof(T),
+                         hipMemcpyHostToDevice));
+  RMC_HIP(ctx, hipMemcpy(adj->mtpack.ptr, mt.data(), mt.size() * sizeof(T),
+                         hipMemcpyHostToDevice));
+  RMC_HIP(ctx, hipMemcpy(adj->bufs.cols.ptr, cols.data(), cols.size() * sizeof(T),
+                         hipMemcpyHostToDevice));
+  return ROADMC_OK;
+}
+
+}  // namespace
+
+extern "C" void roadmc_adjoint_destroy(roadmc_adjoint* adj) {
+  if (!adj) return;
+  (void)hipStreamSynchronize(adj->ctx->stream);
+  rmc_devbuf* all[] = {&adj->bufs.mpack, &adj->bufs.cols, &adj->bufs.partial, &adj->bufs.out,
+                       &adj->bufs.endpop, &adj->bufs.jobs, &adj->bufs.work, &adj->bufs.off,
+                       &adj->bufs.counter, &adj->mtpack, &adj->ckpt, &adj->ns, &adj->np,
+                       &adj->mu, &adj->lam, &adj->gradr, &adj->gradm, &adj->gpart,
+                       &adj->rpart};
+  for (rmc_devbuf* d : all) rmc_buf_free(*d);
+  delete adj;
+}
+
+extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem* pr,
+                                      int32_t checkpoint_every, roadmc_adjoint** out,
+                                      double* objective) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!out || !objective) return rmc_fail(ctx, ROADMC_ERR_INVALID, "out/objective is NULL");
+  *out = nullptr;
+  int rc = validate_problem(ctx, pr);
+  if (rc) return rc;
+  if (pr->n_species != 1 || pr->growth_law != ROADMC_LAW_LOGNORMAL ||
+      pr->endpoint != ROADMC_END_POST_GROWTH || pr->drop_tol != 0.0)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "adjoint: one species, LOGNORMAL law, post-growth endpoint, drop_tol 0");
+  if (checkpoint_every < 1) return rmc_fail(ctx, ROADMC_ERR_INVALID, "checkpoint_every < 1");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const roadmc_species& sp = pr->species[0];
+  roadmc_adjoint* adj = new roadmc_adjoint();
+  adj->ctx = ctx;
+  adj->fp64 = pr->precision == ROADMC_PREC_FP64;
+  adj->P = sp.n_patches;
+  adj->L = checkpoint_every;
+  adj->trans.assign(sp.trans, sp.trans + (size_t)adj->P * adj->P);
+  MteJob& jb = adj->job;
+  jb.P = sp.n_patches;
+  jb.Pp = round_up(jb.P, 64);  // the gradient GEMM works in 64 x 64 blocks
+  jb.N = (int32_t)pr->n_paths;
+  jb.T = pr->n_years;
+  jb.BM = rows_for(ctx, adj->fp64, jb.Pp, pr->n_paths);
+  jb.law = ROADMC_LAW_LOGNORMAL;
+  jb.endpoint = ROADMC_END_POST_GROWTH;
+  jb.n_tiles = (int)((pr->n_paths + jb.BM - 1) / jb.BM);
+  for (int i = 0; i < 8; ++i) jb.params[i] = sp.params[i];
+  jb.sigma = sp.noise_sd;
+  jb.dt = pr->time_step;
+  double tot0 = 0.0;
+  for (int j = 0; j < jb.P; ++j) tot0 += sp.init_pop[j];
+  jb.thr_total = sp.threshold_ratio * tot0;
+  jb.tag0 = rmc_tag(pr->road_id, 0u, RMC_KIND_PATCH);
+  jb.tag1 = rmc_tag(pr->road_id, 0u, RMC_KIND_PATH);
+  jb.key0 = (uint32_t)(pr->seed & 0xffffffffu);
+  jb.key1 = (uint32_t)(pr->seed >> 32);
+  jb.ckpt_every = checkpoint_every;
+  const size_t es = adj->fp64 ? 8 : 4;
+  const int nck = (jb.T + checkpoint_every - 1) / checkpoint_every;
+  rc = adj->fp64 ? adjoint_upload<double>(adj, sp, 0.0) : adjoint_upload<float>(adj, sp, 0.0);
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->ckpt, (size_t)nck * jb.N * jb.Pp * es);
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.partial,
+                                    (size_t)jb.n_tiles * (3 + jb.Pp) * sizeof(double));
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.out, (3 + (size_t)jb.Pp) * sizeof(double));
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.jobs, sizeof(MteJob));
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.work, (size_t)jb.n_tiles * sizeof(int2));
+  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.off, sizeof(int64_t));
+  if (rc) {
+    roadmc_adjoint_destroy(adj);
+    return rc;
+  }
+  jb.mpack = adj->bufs.mpack.ptr;
+  jb.cols = adj->bufs.cols.ptr;
+  jb.partial = static_cast<double*>(adj->bufs.partial.ptr);
+  jb.end_pops = nullptr;
+  jb.ckpt = adj->ckpt.ptr;
+  std::vector<int2> work(jb.n_tiles);
+  for (int t = 0; t < jb.n_tiles; ++t) work[t] = make_int2(0, t);
+  const int64_t off0 = 0;
+  const size_t lds = roadmc::mte_lds_bytes(adj->fp64, jb.BM, jb.Pp);
+  hipStream_t s = ctx->stream;
+  auto fail = [&](int code) {
+    roadmc_adjoint_destroy(adj);
+    return code;
+  };
+  if ((rc = ensure_lds_limit(ctx, adj->fp64, lds))) return fail(rc);
+  hipError_t e = hipMemcpyAsync(adj->bufs.jobs.ptr, &jb, sizeof(MteJob), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(adj->bufs.work.ptr, work.data(), work.size() * sizeof(int2),
+                       hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(adj->bufs.off.ptr, &off0, sizeof(off0), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = roadmc::launch_mte_direct(adj->fp64, static_cast<MteJob*>(adj->bufs.jobs.ptr),
+                                  static_cast<int2*>(adj->bufs.work.ptr), jb.n_tiles, lds, s);
+  if (e == hipSuccess)
+    e = roadmc::launch_mte_reduce(static_cast<MteJob*>(adj->bufs.jobs.ptr), 1,
+                                  static_cast<double*>(adj->bufs.out.ptr),
+                                  static_cast<int64_t*>(adj->bufs.off.ptr), s);
+  double sums[3] = {0, 0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sums, adj->bufs.out.ptr, sizeof(sums), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    rmc_fail(ctx, ROADMC_ERR_HIP, std::string("adjoint forward: ") + hipGetErrorString(e));
+    return fail(ROADMC_ERR_HIP);
+  }
+  if (!std::isfinite(sums[1])) {
+    rmc_fail(ctx, ROADMC_ERR_NUMERIC, "adjoint forward: non-finite population");
+    return fail(ROADMC_ERR_NUMERIC);
+  }
+  *objective = sums[1] / (double)jb.N;
+  *out = adj;
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
+                                      double* grad_mortality, double* grad_growth) {
+  if (!adj) return ROADMC_ERR_INVALID;
+  roadmc_ctx* ctx = adj->ctx;
+  if (!grad_mortality && !grad_growth)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "no gradient output requested");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const MteJob& j0 = adj->job;
+  const int Pp = j0.Pp, N = j0.N, L = adj->L, P = adj->P;
+  const size_t es = adj->fp64 ? 8 : 4;
+  const size_t plane = (size_t)N * Pp;
+  const int nsplit = 64;
+  int rc;
+  if ((rc = rmc_buf_ensure(ctx, adj->ns, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, adj->np, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, adj->mu, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, adj->lam, plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, adj->gradr, (size_t)Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, adj->gradm, (size_t)Pp * Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, adj->gpart, (size_t)nsplit * Pp * Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, adj->rpart, (size_t)j0.n_tiles * Pp * sizeof(double))))
+    return rc;
+  hipStream_t s = ctx->stream;
+  RMC_HIP(ctx, hipMemsetAsync(adj->gradr.ptr, 0, (size_t)Pp * sizeof(double), s));
+  RMC_HIP(ctx, hipMemsetAsync(adj->gradm.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
+  roadmc::AdjJob a{};
+  a.mpack = adj->bufs.mpack.ptr;
+  a.mtpack = adj->mtpack.ptr;
+  a.cols = adj->bufs.cols.ptr;
+  a.ckpt = adj->ckpt.ptr;
+  a.ns = adj->ns.ptr;
+  a.np = adj->np.ptr;
+  a.mu = adj->mu.ptr;
+  a.lam = adj->lam.ptr;
+  a.grad_r = static_cast<double*>(adj->gradr.ptr);
+  a.grad_part = static_cast<double*>(adj->rpart.ptr);
+  a.sigma = j0.sigma;
+  a.dobj = d_objective;
+  a.P = P;
+  a.Pp = Pp;
+  a.N = N;
+  a.T = j0.T;
+  a.BM = j0.BM;
+  a.L = L;
+  a.tag0 = j0.tag0;
+  a.key0 = j0.key0;
+  a.key1 = j0.key1;
+  const size_t lds = roadmc::adj_lds_bytes(adj->fp64, j0.BM, Pp);
+  RMC_HIP(ctx, roadmc::adj_set_lds_limit(adj->fp64, lds));
+  const int nseg = (j0.T + L - 1) / L;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    a.y0 = sg * L;
+    a.y1 = std::min(j0.T, a.y0 + L) - 1;
+    RMC_HIP(ctx, roadmc::launch_adj_segment(adj->fp64, a, j0.n_tiles, lds, s));
+    const int K = (a.y1 - a.y0 + 1) * N;
+    RMC_HIP(ctx, roadmc::launch_adj_gradm(adj->fp64, adj->ns.ptr, adj->mu.ptr, K, Pp, nsplit,
+                                          static_cast<double*>(adj->gpart.ptr),
+                                          static_cast<double*>(adj->gradm.ptr), s));
+  }
+  std::vector<double> gm((size_t)Pp * Pp), gr(Pp);
+  RMC_HIP(ctx, hipMemcpyAsync(gm.data(), adj->gradm.ptr, gm.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipMemcpyAsync(gr.data(), adj->gradr.ptr, gr.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  for (double v : gr)
+    if (!std::isfinite(v)) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite gradient");
+  // M = T o (1 - m): dJ/dm_kj = -T_kj dJ/dM_kj; outputs column-major like trans.
+  if (grad_mortality)
+    for (int k = 0; k < P; ++k)
+      for (int j = 0; j < P; ++j)
+        grad_mortality[(size_t)j * P + k] = -adj->trans[(size_t)j * P + k] * gm[(size_t)k * Pp + j];
+  if (grad_growth)
+    for (int j = 0; j < P; ++j) grad_growth[j] = gr[j];
+  return ROADMC_OK;
+}

@@ -159,6 +159,10 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_mte_penalty": (C.c_int, [C.c_int32, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]),
     "roadmc_patch_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                        C.c_int32, C.c_int32, C.c_int32, _fp]),
+    "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
+                                         C.POINTER(C.c_void_p), _dp]),
+    "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
+    "roadmc_adjoint_destroy": (None, [C.c_void_p]),
     "roadmc_rov_basis_size": (C.c_int, [C.c_int32, C.c_int32, _ip]),
     "roadmc_rov_run": (C.c_int, [C.c_void_p, C.POINTER(RovProblem), C.POINTER(RovResult)]),
     "roadmc_exp_pv": (C.c_int, [C.c_void_p, C.POINTER(Uncertainty), C.c_int32, C.c_double,
