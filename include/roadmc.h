@@ -286,6 +286,33 @@ int roadmc_exp_pv(roadmc_ctx* ctx, const roadmc_uncertainty* unc,
                   int32_t n_steps, double time_step, double rrr, double growth,
                   int64_t n_paths, uint64_t seed, double* mean, double* sd);
 
+/* ---- Learned surrogate --------------------------------------------------- */
+
+/* An ensemble of n_members MLPs n_in -> 64 -> 64 -> n_out (tanh), trained in
+ * FP32 by full-batch Adam on the mean squared error with this library's
+ * kernels.  It stands in for the reference's kNN/local-linear surrogate fit
+ * (SimulateGPU::buildSurrogateMTECUDA/ROVCUDA, SimulateGPU.cu:5533-5968,
+ * interpolated by Utility::interpolateSurrogate, Utility.cpp:253-363);
+ * callers fall back to full Monte Carlo where the members disagree.
+ * Parameters of a member: W1[n_in][64], b1[64], W2[64][64], b2[64],
+ * W3[64][n_out], b3[n_out]. */
+typedef struct roadmc_mlp roadmc_mlp;
+int roadmc_mlp_create(roadmc_ctx* ctx, int32_t n_in, int32_t n_hidden,
+                      int32_t n_out, int32_t n_members, roadmc_mlp** out);
+void roadmc_mlp_destroy(roadmc_mlp* net);
+int roadmc_mlp_n_params(const roadmc_mlp* net, int32_t* n);
+/* params [n_members][n_params]; setting them resets the Adam state. */
+int roadmc_mlp_set_params(roadmc_mlp* net, const float* params);
+int roadmc_mlp_get_params(roadmc_mlp* net, float* params);
+/* Gradient of the mean squared error over (x [n][n_in], y [n][n_out]):
+ * grad [n_members][n_params], loss [n_members]. */
+int roadmc_mlp_gradient(roadmc_mlp* net, const float* x, const float* y,
+                        int32_t n, float* grad, float* loss);
+int roadmc_mlp_train(roadmc_mlp* net, const float* x, const float* y,
+                     int32_t n, int32_t steps, float lr, float* loss);
+/* out [n_members][n][n_out]. */
+int roadmc_mlp_predict(roadmc_mlp* net, const float* x, int32_t n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

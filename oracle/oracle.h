@@ -128,6 +128,12 @@ int orc_mte_adjoint(const orc_species* sp, uint32_t road_id, int32_t n_years,
                     int64_t n_paths, uint64_t seed, double* objective,
                     double* grad_m, double* grad_r);
 
+/* ---- Surrogate MLP ------------------------------------------------------ */
+
+/* Mean squared error over n samples and its gradient (grad may be NULL). */
+double orc_mlp_loss_grad(int n_in, int H, int n_out, const double* w,
+                         const double* X, const double* Y, int n, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

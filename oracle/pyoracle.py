@@ -284,3 +284,18 @@ def mte_adjoint(sd, road_id: int, n_years: int, n_paths: int, seed: int):
     h.orc_mte_adjoint(C.byref(sp), road_id, n_years, n_paths, seed, C.byref(obj),
                       gm.ctypes.data_as(_dp), gr.ctypes.data_as(_dp))
     return obj.value, gm.reshape(P, P, order="F"), gr
+
+
+# ---- Surrogate MLP ----------------------------------------------------------
+
+def mlp_loss_grad(w, X, Y, n_in: int, n_out: int, H: int = 64):
+    h = lib()
+    h.orc_mlp_loss_grad.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp]
+    h.orc_mlp_loss_grad.restype = C.c_double
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    g = np.zeros_like(w)
+    loss = h.orc_mlp_loss_grad(n_in, H, n_out, w.ctypes.data_as(_dp), X.ctypes.data_as(_dp),
+                               Y.ctypes.data_as(_dp), len(X), g.ctypes.data_as(_dp))
+    return loss, g

@@ -163,6 +163,15 @@ SIGNATURES: dict[str, tuple] = {
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     "roadmc_adjoint_destroy": (None, [C.c_void_p]),
+    "roadmc_mlp_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    "roadmc_mlp_destroy": (None, [C.c_void_p]),
+    "roadmc_mlp_n_params": (C.c_int, [C.c_void_p, _ip]),
+    "roadmc_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "roadmc_mlp_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "roadmc_mlp_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]),
+    "roadmc_mlp_train": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_int32, C.c_float, _fp]),
+    "roadmc_mlp_predict": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
     "roadmc_rov_basis_size": (C.c_int, [C.c_int32, C.c_int32, _ip]),
     "roadmc_rov_run": (C.c_int, [C.c_void_p, C.POINTER(RovProblem), C.POINTER(RovResult)]),
     "roadmc_exp_pv": (C.c_int, [C.c_void_p, C.POINTER(Uncertainty), C.c_int32, C.c_double,
