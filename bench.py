@@ -62,6 +62,80 @@ def fit_p2(sizes, times):
     return a, b
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "mte_persistent_generation512_traffic.txt")
+
+
+def read_traffic(kernel_ms: float):
+    """HBM bytes per launch of the persistent kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a calibrated copy);
+    None when that summary is absent."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        return float(vals["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def run_extras(ctx) -> dict:
+    """The other configurations of the hot path, each timed once after a
+    warm-up: the large single road (3 x 256 patches, 100 years, 100k paths),
+    the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis) and
+    the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10)."""
+    from transportoptimiser_amd import landscape
+    from transportoptimiser_amd.adjoint import mte_gradient
+    from transportoptimiser_amd.rov import config5, simulate_rov
+
+    out = {}
+    pr = landscape.config2()
+    fl = road_flops(pr)
+    ctx.simulate_mte(pr, end_pops=False, patch_mean=False)
+    with ctx.prepare_batch([pr]) as b:
+        b.launch()
+        b.sync()
+        b.kernel_ms()
+        b.launch()
+        ms = b.kernel_ms()
+    out["config2_road"] = {"ms": round(ms, 2), "tflops": round(fl / ms / 1e9, 1),
+                           "frac_fp32_mfma": round(fl / ms / 1e9 / FP32_MFMA_PEAK_TFLOPS, 3),
+                           "what": "3 species x 256 patches, 100 years, 100,000 paths"}
+    log(f"config2 road: {out['config2_road']}")
+    rp = config5()
+    simulate_rov(ctx, rp)
+    t = time.perf_counter()
+    r = simulate_rov(ctx, rp)
+    ms = (time.perf_counter() - t) * 1e3
+    N, T, U = rp.n_paths, rp.n_steps, len(rp.uncertainties)
+    # Bytes the valuation must move: states written once and read by the
+    # moments pass, then per step read by assembly and decision, plus the
+    # value and control arrays.
+    nbytes = 8.0 * N * (T + 1) * U * 3 + 4.0 * N * T * 3 + 8.0 * N * (T + 1) * 3
+    out["rov_config5"] = {"ms": round(ms, 2), "value": round(r["value"], 3),
+                          "value_sd": round(r["value_sd"], 3),
+                          "gbps_algorithmic": round(nbytes / ms / 1e6, 1),
+                          "what": "2 OU + 1 GBM, corr 0.3, 2 controls, 50 steps, 100k paths, "
+                                  "cubic LSM, FP64 solve; host call incl. graph capture"}
+    log(f"rov config5: {out['rov_config5']}")
+    ap = landscape.RoadProblem([landscape.make_species(np.random.default_rng(4), 256,
+                                                       mort_max=0.4)],
+                               n_years=100, n_paths=50000, seed=4)
+    mte_gradient(ctx, ap, 10)
+    t = time.perf_counter()
+    mte_gradient(ctx, ap, 10)
+    ms = (time.perf_counter() - t) * 1e3
+    # Forward + recompute + two backward contractions: 4 x 2 N P^2 T.
+    afl = 4 * 2.0 * 50000 * 256 ** 2 * 100
+    out["adjoint_config4"] = {"ms": round(ms, 2), "tflops": round(afl / ms / 1e9, 1),
+                              "what": "256 patches, 100 years, 50,000 paths, checkpoint 10; "
+                                      "forward + backward, FP32"}
+    log(f"adjoint config4: {out['adjoint_config4']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +150,8 @@ def main() -> int:
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the config-2 road, ROV and adjoint records")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -196,7 +272,10 @@ def main() -> int:
                          f"512 roads as a + b P^2"}
         log(f"cpu baseline: {cpu}")
 
+    extras = None if args.no_extras else run_extras(ctx)
+
     achieved = flops / (kernel_ms / 1e3) / 1e12
+    traffic = read_traffic(kernel_ms)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -217,7 +296,7 @@ def main() -> int:
                    "species": len(roads[0].species), "tile_rows": tiles},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
                      "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
                      "kernel": "mte_persistent_kernel<TraitsF32>",
                      "kernel_ms": round(kernel_ms, 3),
                      "flops_per_launch": flops},
@@ -225,6 +304,7 @@ def main() -> int:
         "plain": plain,
         "single": single,
         "accuracy": accuracy,
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

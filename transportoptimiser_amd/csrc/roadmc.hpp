@@ -41,6 +41,53 @@ struct SpeciesRoadPatches {
   double thresholdRatio = 0.5;
 };
 
+// Column-major int matrix (Eigen::MatrixXi's storage order).
+struct MatrixI {
+  int64_t rows = 0, cols = 0;
+  std::vector<int32_t> data;
+  MatrixI() = default;
+  MatrixI(int64_t r, int64_t c) : rows(r), cols(c), data((size_t)(r * c), 0) {}
+  int32_t& operator()(int64_t r, int64_t c) { return data[(size_t)(c * rows + r)]; }
+  int32_t operator()(int64_t r, int64_t c) const { return data[(size_t)(c * rows + r)]; }
+};
+
+// What SimulateGPU::expPV reads from an Uncertainty (SimulateGPU.cu:3163-3259,
+// multipliers already applied) and the two results it stores back on it.
+struct Uncertainty {
+  roadmc_uncertainty params{};
+  double expPV = 0.0;
+  double expPVSD = 0.0;
+};
+
+// Economic inputs of the valuation; the reference reads these through
+// sim->getRoad()->getOptimiser() (SimulateGPU.cu:3903-3936).
+struct RovInputs {
+  std::vector<roadmc_uncertainty> uncertainties;
+  std::vector<double> correlation;   // U x U row-major, empty = independent
+  std::vector<double> flows;         // per control
+  double unitCost = 0.0;
+  std::vector<double> profitWeights; // per uncertainty
+  int trafficIndex = -1;
+  int nYears = 50;
+  double timeStep = 1.0;
+  double rrr = 7.0;
+  int64_t noPaths = 10000;
+  int basisDegree = 3;
+  bool standardize = true;
+  bool itmOnly = true;
+  bool pooled = false;
+  double hysteresis = 1.0;
+  int valueMode = ROADMC_VALUE_REGRESSED;
+  uint64_t seed = 0;
+};
+
+// The road attributes simulateROVCUDA stores (SimulateGPU.cu:5461-5490).
+struct RovStats {
+  double value = 0.0;
+  double valueSD = 0.0;
+  int firstControl = 0;
+};
+
 struct SimulatorInputs {
   int nYears = 50;
   double timeStep = 1.0;
@@ -106,6 +153,81 @@ class SimulateGPU {
     res.end_pops = endPops.data.data();
     roadmc_ctx* ctx = context(device);
     if (roadmc_mte_run(ctx, &pr, &res) != ROADMC_OK) fail(roadmc_last_error(ctx));
+  }
+
+  // SimulateGPU::simulateROVCUDA (SimulateGPU.h:340-344) for the price-state
+  // model: condExp is (nYears+1) x noPaths and optCont nYears x noPaths
+  // (step-major, like the reference's T x N matrices); regressions receives
+  // nYears * controls * K coefficients.  Outputs are resized here.
+  static void simulateROVCUDA(const RovInputs& in, Matrix& condExp, MatrixI& optCont,
+                              std::vector<double>& regressions, RovStats& stats,
+                              int device = 0) {
+    const int U = (int)in.uncertainties.size();
+    const int C = (int)in.flows.size();
+    if ((int)in.profitWeights.size() != U) fail("simulateROVCUDA: one profit weight per uncertainty");
+    if (!in.correlation.empty() && (int)in.correlation.size() != U * U)
+      fail("simulateROVCUDA: correlation must be U x U");
+    int32_t K = 0;
+    if (roadmc_rov_basis_size(U, in.basisDegree, &K) != ROADMC_OK) fail("simulateROVCUDA: basis");
+    roadmc_rov_problem pr{};
+    pr.n_uncertainties = U;
+    pr.unc = in.uncertainties.data();
+    pr.correlation = in.correlation.empty() ? nullptr : in.correlation.data();
+    pr.n_controls = C;
+    pr.flow = in.flows.data();
+    pr.unit_cost = in.unitCost;
+    pr.profit_weight = in.profitWeights.data();
+    pr.traffic_index = in.trafficIndex;
+    pr.n_steps = in.nYears;
+    pr.time_step = in.timeStep;
+    pr.rrr = in.rrr;
+    pr.n_paths = in.noPaths;
+    pr.basis_degree = in.basisDegree;
+    pr.standardize = in.standardize ? 1 : 0;
+    pr.itm_only = in.itmOnly ? 1 : 0;
+    pr.pooled = in.pooled ? 1 : 0;
+    pr.hysteresis = in.hysteresis;
+    pr.mode = ROADMC_ROV_SWITCHING;
+    pr.value_mode = in.valueMode;
+    pr.seed = in.seed;
+    Matrix ce(in.nYears + 1, in.noPaths);
+    MatrixI oc(in.nYears, in.noPaths);
+    std::vector<double> reg((size_t)in.nYears * C * K);
+    // The library writes [step][path] row-major, which is the transpose of
+    // a column-major (steps x paths) matrix; stage and reorder.
+    std::vector<double> ce_rm(ce.data.size());
+    std::vector<int32_t> oc_rm(oc.data.size());
+    RovStats st;
+    roadmc_rov_result res{};
+    res.value = &st.value;
+    res.value_sd = &st.valueSD;
+    res.first_control = &st.firstControl;
+    res.coeffs = reg.data();
+    res.cond_exp = ce_rm.data();
+    res.opt_control = oc_rm.data();
+    roadmc_ctx* ctx = context(device);
+    if (roadmc_rov_run(ctx, &pr, &res) != ROADMC_OK) fail(roadmc_last_error(ctx));
+    for (int64_t t = 0; t <= in.nYears; ++t)
+      for (int64_t i = 0; i < in.noPaths; ++i) ce(t, i) = ce_rm[(size_t)(t * in.noPaths + i)];
+    for (int64_t t = 0; t < in.nYears; ++t)
+      for (int64_t i = 0; i < in.noPaths; ++i) oc(t, i) = oc_rm[(size_t)(t * in.noPaths + i)];
+    condExp = std::move(ce);
+    optCont = std::move(oc);
+    regressions = std::move(reg);
+    stats = st;
+  }
+
+  // SimulateGPU::expPV (SimulateGPU.h:206): stores the expected present value
+  // and its SD on the uncertainty.
+  static void expPV(Uncertainty& u, int nYears, double timeStep, double rrr, double growth,
+                    int64_t noPaths, uint64_t seed, int device = 0) {
+    roadmc_ctx* ctx = context(device);
+    double m = 0.0, sd = 0.0;
+    if (roadmc_exp_pv(ctx, &u.params, nYears, timeStep, rrr, growth, noPaths, seed, &m, &sd) !=
+        ROADMC_OK)
+      fail(roadmc_last_error(ctx));
+    u.expPV = m;
+    u.expPVSD = sd;
   }
 
  private:

@@ -7,6 +7,21 @@ for the MTE kernel into HBM bytes.
 
 ``python transportoptimiser_amd/profiling.py summarize <stats.csv> [...]``
 prints the kernel-trace statistics rows of the named kernels.
+
+``python transportoptimiser_amd/profiling.py uniform --patches P`` runs one
+batched launch of 8 roads that all have P patches (2 species, 50 years,
+20,000 paths), the per-size workload used for counter profiles of the
+persistent MTE kernel.
+
+``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
+per-kernel call count and duration statistics of a rocprofv3 kernel trace
+(the rocpd SQLite output).
+
+``python transportoptimiser_amd/profiling.py traffic --kernel NAME --fetch F.db
+--write W.db --copy-fetch CF.db --copy-write CW.db --copy-bytes B`` prints the
+HBM bytes per dispatch of kernel NAME: FETCH_SIZE and WRITE_SIZE (KiB per
+dispatch), each scaled by B divided by what the same counter reads for the
+measured B-byte copy.
 """
 from __future__ import annotations
 
@@ -27,6 +42,25 @@ def copy(nbytes: int) -> None:
     print(f"copied {n * 4} bytes twice (second copy is the measured one)")
 
 
+def uniform(P: int, n_roads: int = 8) -> None:
+    import os
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import landscape as ls
+    from transportoptimiser_amd.api import Context
+    rng = np.random.default_rng(P)
+    roads = [ls.RoadProblem([ls.make_species(rng, P) for _ in range(2)], n_years=50,
+                            n_paths=20000, seed=1, road_id=r) for r in range(n_roads)]
+    flops = sum(2.0 * r.n_paths * P * P * r.n_years * len(r.species) for r in roads)
+    ctx = Context(0)
+    with ctx.prepare_batch(roads) as b:
+        b.launch()
+        b.sync()
+        ms = b.kernel_ms()
+    ctx.close()
+    print(f"P={P}: {n_roads} roads, kernel {ms:.1f} ms, {flops / ms / 1e9:.1f} TFLOP/s", flush=True)
+
+
 def summarize(paths: list[str]) -> None:
     for p in paths:
         with open(p, newline="") as f:
@@ -36,6 +70,64 @@ def summarize(paths: list[str]) -> None:
                                                 "AverageNs", "Percentage", "MinNs", "MaxNs")})
 
 
+def _db(path: str):
+    import sqlite3
+    return sqlite3.connect(f"file:{path}?mode=ro", uri=True)
+
+
+def dbstats(path: str) -> None:
+    c = _db(path)
+    rows = list(c.execute("select name, count(*), sum(duration), avg(duration), min(duration), "
+                          "max(duration) from kernels group by name order by sum(duration) desc"))
+    total = sum(r[2] for r in rows) or 1.0
+    print(f"{'calls':>6} {'total_ms':>11} {'avg_us':>11} {'min_us':>11} {'max_us':>11} {'pct':>6}  kernel")
+    for name, n, tot, avg, mn, mx in rows:
+        print(f"{n:>6} {tot / 1e6:>11.3f} {avg / 1e3:>11.1f} {mn / 1e3:>11.1f} {mx / 1e3:>11.1f} "
+              f"{100 * tot / total:>6.2f}  {name[:110]}")
+
+
+def counters(path: str, kernel: str) -> None:
+    c = _db(path)
+    rows = c.execute("select dispatch_id, kernel_name, counter_name, sum(value) from "
+                     "counters_collection group by dispatch_id, counter_name "
+                     "order by dispatch_id, counter_name")
+    for d, name, cn, v in rows:
+        if kernel in name:
+            print(f"dispatch {d}: {cn} = {v:.6e}")
+
+
+def _counter(path: str, kernel: str, counter: str) -> list[float]:
+    c = _db(path)
+    rows = c.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                     "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,))
+    return [float(v) for _, name, v in rows if kernel in name]
+
+
+def traffic(a) -> None:
+    fetch = _counter(a.fetch, a.kernel, "FETCH_SIZE")
+    write = _counter(a.write, a.kernel, "WRITE_SIZE")
+    cf = max(_counter(a.copy_fetch, "copyBuffer", "FETCH_SIZE"))
+    cw = max(_counter(a.copy_write, "copyBuffer", "WRITE_SIZE"))
+    if not fetch or not write:
+        raise SystemExit(f"no dispatch of {a.kernel} in the counter files")
+    kf = a.copy_bytes / (cf * 1024.0)
+    kw = a.copy_bytes / (cw * 1024.0)
+    f_kib = sum(fetch) / len(fetch)
+    w_kib = sum(write) / len(write)
+    hbm = f_kib * 1024.0 * kf + w_kib * 1024.0 * kw
+    print(f"kernel: {a.kernel}")
+    print(f"dispatches_fetch: {len(fetch)}")
+    print(f"dispatches_write: {len(write)}")
+    print(f"fetch_kib_per_dispatch: {f_kib:.1f}")
+    print(f"write_kib_per_dispatch: {w_kib:.1f}")
+    print(f"copy_bytes: {a.copy_bytes}")
+    print(f"copy_fetch_kib: {cf:.1f}")
+    print(f"copy_write_kib: {cw:.1f}")
+    print(f"fetch_scale: {kf:.4f}")
+    print(f"write_scale: {kw:.4f}")
+    print(f"hbm_bytes_per_launch: {hbm:.6e}")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -43,9 +135,32 @@ def main() -> int:
     c.add_argument("--bytes", type=int, default=1 << 30)
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
+    u = sub.add_parser("uniform")
+    u.add_argument("--patches", type=int, required=True)
+    u.add_argument("--roads", type=int, default=8)
+    d = sub.add_parser("dbstats")
+    d.add_argument("path")
+    k = sub.add_parser("counters")
+    k.add_argument("path")
+    k.add_argument("--kernel", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--copy-bytes", type=int, required=True)
     a = ap.parse_args()
     if a.cmd == "copy":
         copy(a.bytes)
+    elif a.cmd == "uniform":
+        uniform(a.patches, a.roads)
+    elif a.cmd == "dbstats":
+        dbstats(a.path)
+    elif a.cmd == "counters":
+        counters(a.path, a.kernel)
+    elif a.cmd == "traffic":
+        traffic(a)
     else:
         summarize(a.paths)
     return 0
