@@ -87,14 +87,14 @@ __host__ __device__ static inline size_t lds_bytes_t(int BM, int Pp) {
          align16((size_t)BM * sizeof(T)) + 8 * 4 * sizeof(double);
 }
 
-template <typename T>
-__device__ __forceinline__ T grow(int law, T x, T z, T K, T Kinv, T r, T sig,
+template <typename T, int LAW>
+__device__ __forceinline__ T grow(T x, T z, T K, T Kinv, T r, T sig,
                                   T half_var, T dt, T mu, T sd, T grm, T lv) {
-  if (law == kLawRefCpu) {
+  if constexpr (LAW == kLawRefCpu) {
     const T g = dt * (mu + sd * z) * (T)0.01;
     return x * ((T)1 + g * (K - x) * Kinv);
   }
-  if (law == kLawRefGpu) {
+  if constexpr (LAW == kLawRefGpu) {
     const T g = lv * z * grm + grm;
     return x * ((T)1 + g * (K - x) * Kinv);
   }

@@ -55,10 +55,11 @@ size_t mte_pack_index(bool fp64, int Pp, int k, int j) {
 
 
 // Simulates one tile (BM paths of one job) for all years and writes the
-// tile's tallies.  Every thread of the workgroup calls it.
-template <class TR>
-__device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
-                                         char* smem) {
+// tile's tallies.  Every thread of the workgroup calls it.  The growth law is
+// a template parameter so that the epilogue has no per-element branches.
+template <class TR, int LAW>
+__device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
+                                             char* smem) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   using AccT = typename TR::AccT;
@@ -79,9 +80,7 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
       align16((size_t)BM * sizeof(T)));
 
   const T* cols = reinterpret_cast<const T*>(jb.cols);
-  const T* capK = cols;
-  const T* capKinv = cols + Pp;
-  const T* rate = cols + 2 * Pp;
+  // cols holds K, 1/K, r and the initial population, Pp values each.
   const T* init = cols + 3 * Pp;
 
   const T mu = (T)jb.params[1], sd = (T)jb.params[2], rev = (T)jb.params[4];
@@ -92,7 +91,6 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
   const T l2e = sizeof(T) == 4 ? (T)1.4426950408889634 : (T)1;
   const T sig = (T)(jb.sigma * (sizeof(T) == 4 ? 1.4426950408889634 : 1.0));
   const T half_var = (T)(0.5 * jb.sigma * jb.sigma) * l2e;
-  const int law = jb.law;
 
   for (int idx = tid; idx < BM * Pp; idx += nthreads) {
     const int r = idx / Pp, c = idx - r * Pp;
@@ -125,6 +123,14 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
   for (int t = 0; t < NT; ++t)
     if (t < my_n) b0[t] = as_global(Bp)[(size_t)tcol[t] * nkg * 64 + lane];
   for (int year = 0; year < jb.T; ++year) {
+    // Opaque per-year copies of loop-invariant scalars.  Addresses derived
+    // from them (the 64 LDS write addresses and 16 Philox counters of a lane,
+    // the B and A operand pointers) are recomputed each year by a few integer
+    // ops instead of being hoisted out of the year loop into registers that
+    // would then spill.
+    int LDAy = LDA, p0y = p0;
+    const T* colsy = cols;
+    asm volatile("" : "+s"(LDAy), "+s"(p0y), "+s"(colsy));
     if (jb.ckpt && year % jb.ckpt_every == 0) {
       // Adjoint checkpoint: the state entering this year, natural column order.
       T* ck = static_cast<T*>(jb.ckpt) + ((size_t)(year / jb.ckpt_every) * jb.N) * Pp;
@@ -138,17 +144,17 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
     // One branch per year on the (wave-uniform) tile count; the product
     // loops themselves are straight-line code.
     switch (my_n) {
-      case 1: year_product<TR, 1>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 2: year_product<TR, 2>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 3: year_product<TR, 3>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 4: year_product<TR, 4>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      case 1: year_product<TR, 1>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+      case 2: year_product<TR, 2>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+      case 3: year_product<TR, 3>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+      case 4: year_product<TR, 4>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
       default:
         if constexpr (NT > 4) {
           switch (my_n) {
-            case 5: year_product<TR, 5>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 6: year_product<TR, 6>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 7: year_product<TR, 7>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 8: year_product<TR, 8>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            case 5: year_product<TR, 5>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+            case 6: year_product<TR, 6>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+            case 7: year_product<TR, 7>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+            case 8: year_product<TR, 8>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
             default: break;
           }
         }
@@ -156,7 +162,7 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
     }
     __syncthreads();  // every wave has finished reading this year's state
 
-    if (law == kLawRefGpu) {
+    if constexpr (LAW == kLawRefGpu) {
       // Growth-rate SDE per path (SimulateGPU.cu:686-701).
       for (int r = tid; r < BM; r += nthreads) {
         T zb, zj, uj;
@@ -178,13 +184,13 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
       if (t < my_n) {
         const int col = tcol[t] * TN + TR::acc_col(lane);
         const int pcol = TR::perm(col);
-        const T K = capK[col], Ki = capKinv[col], rr = rate[col];
+        const T K = colsy[col], Ki = colsy[Pp + col], rr = colsy[2 * Pp + col];
 #pragma unroll
         for (int r4 = 0; r4 < NACC; r4 += 4) {
           T z[4];
           if constexpr (TR::kQuadItems) {
             const int row0 = trow[t] * TM + TR::acc_row(lane, r4);
-            TR::normal4((uint32_t)(p0 + row0) >> 2, (uint32_t)year,
+            TR::normal4((uint32_t)(p0y + row0) >> 2, (uint32_t)year,
                         (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
           }
 #pragma unroll
@@ -194,16 +200,17 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
             if constexpr (TR::kQuadItems) {
               zz = z[s];
             } else {
-              const uint32_t path = (uint32_t)(p0 + row);
+              const uint32_t path = (uint32_t)(p0y + row);
               TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0,
                           jb.key0, jb.key1, z);
               zz = z[path & 3u];
             }
             const T x = acc[t][r4 + s];
-            const T g = (law == kLawRefGpu) ? grm[row] : (T)0;
-            const T n = grow<T>(law, x, zz, K, Ki, rr, sig, half_var, dt, mu,
-                                sd, g, lv);
-            A[row * LDA + pcol] = keep_pre ? x : n;
+            T g = (T)0;
+            if constexpr (LAW == kLawRefGpu) g = grm[row];
+            const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
+                                     sd, g, lv);
+            A[row * LDAy + pcol] = keep_pre ? x : n;
           }
         }
       }
@@ -247,6 +254,15 @@ __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile,
     part[3 + c] = s;
   }
   __syncthreads();  // LDS is reused by the next tile
+}
+
+template <class TR>
+__device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile, char* smem) {
+  switch (jb.law) {
+    case kLawRefCpu: mte_tile_law<TR, kLawRefCpu>(jb, tile, smem); break;
+    case kLawRefGpu: mte_tile_law<TR, kLawRefGpu>(jb, tile, smem); break;
+    default: mte_tile_law<TR, kLawLognormal>(jb, tile, smem); break;
+  }
 }
 
 template <class TR>
@@ -461,8 +477,16 @@ __global__ void mte_plain_kernel(PlainArgs a) {
     for (int k = 0; k < P; ++k) {
       const T z = rgr[((size_t)ii * Ty + t) * P + k];
       const T x = nxt[k];
-      cur[k] = grow<T>(a.law, x, z, caps[k], (T)1 / caps[k], rates[k], sig,
-                       half_var, dt, prm[1], prm[2], grm, prm[7]);
+      const T Ki = (T)1 / caps[k];
+      if (a.law == kLawRefCpu)
+        cur[k] = grow<T, kLawRefCpu>(x, z, caps[k], Ki, rates[k], sig, half_var, dt, prm[1],
+                                     prm[2], grm, prm[7]);
+      else if (a.law == kLawRefGpu)
+        cur[k] = grow<T, kLawRefGpu>(x, z, caps[k], Ki, rates[k], sig, half_var, dt, prm[1],
+                                     prm[2], grm, prm[7]);
+      else
+        cur[k] = grow<T, kLawLognormal>(x, z, caps[k], Ki, rates[k], sig, half_var, dt,
+                                        prm[1], prm[2], grm, prm[7]);
     }
   }
   const T* v = (a.endpoint == kEndPre) ? nxt : cur;
