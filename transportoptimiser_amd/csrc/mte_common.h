@@ -80,11 +80,16 @@ __host__ __device__ static inline size_t align16(size_t x) {
   return (x + 15) & ~(size_t)15;
 }
 
+// LDS of one MTE workgroup: the BM x (Pp + PAD) population tile, the per-row
+// growth-rate means, 8 x 4 doubles for the tally reduction and, when
+// remainder tiles are split into k-quarters, 4 partial accumulators per
+// remainder tile.
 template <class TR>
-__host__ __device__ static inline size_t lds_bytes_t(int BM, int Pp) {
+__host__ __device__ static inline size_t lds_bytes_t(int BM, int Pp, int n_rem_split) {
   using T = typename TR::T;
   return align16((size_t)BM * (Pp + TR::PAD) * sizeof(T)) +
-         align16((size_t)BM * sizeof(T)) + 8 * 4 * sizeof(double);
+         align16((size_t)BM * sizeof(T)) + 8 * 4 * sizeof(double) +
+         (size_t)n_rem_split * 4 * TR::NACC * 64 * sizeof(T);
 }
 
 template <typename T, int LAW>
@@ -188,6 +193,35 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
 #pragma unroll
       for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[t][s], b1[t][s], acc[t]);
     }
+  }
+}
+
+// acc += A[rows of tile (trow, tcol), k-groups kg0 .. kg1) . M[those k-groups,
+// cols of the tile] for one output tile, kg1 > kg0: one k-group per step with
+// the next step's operands in flight (the last step re-reads its own, so the
+// loop has no tail branch).
+template <class TR>
+__device__ __forceinline__ void range_product(typename TR::AccT& acc,
+                                              const typename TR::VecT* Bp,
+                                              const typename TR::T* A, int trow,
+                                              int tcol, int nkg, int kg0, int kg1,
+                                              int LDA, int arow, int akoff, int lane) {
+  using T = typename TR::T;
+  using VecT = typename TR::VecT;
+  constexpr int TM = TR::TM, KV = TR::KV;
+  const __attribute__((address_space(1))) VecT* bp =
+      as_global(Bp + (size_t)tcol * nkg * 64 + lane);
+  const T* ap = A + (trow * TM + arow) * LDA + akoff;
+  VecT b = bp[(size_t)kg0 * 64];
+  VecT a = *reinterpret_cast<const VecT*>(ap + kg0 * 8);
+  for (int kg = kg0; kg < kg1; ++kg) {
+    const int kn = min(kg + 1, kg1 - 1);
+    const VecT bn = bp[(size_t)kn * 64];
+    const VecT an = *reinterpret_cast<const VecT*>(ap + kn * 8);
+#pragma unroll
+    for (int s = 0; s < KV; ++s) acc = TR::mfma(a[s], b[s], acc);
+    a = an;
+    b = bn;
   }
 }
 

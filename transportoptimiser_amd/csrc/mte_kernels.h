@@ -22,6 +22,8 @@ struct MteJob {
   double dt;           // time step
   double thr_total;    // threshold_ratio * initial total
   int32_t P, Pp, N, T, BM, law, endpoint, n_tiles;
+  int32_t Pk;          // contraction extent: P rounded up to 16 (<= Pp)
+  int32_t split;       // split the ntiles % 4 remainder tiles into k-quarters
   uint32_t tag0, tag1, key0, key1;
 };
 
@@ -34,9 +36,12 @@ constexpr int kMteTileElemsF64 = 16384;
 constexpr int kMteRowsF32 = 32;
 constexpr int kMteRowsF64 = 16;
 
-size_t mte_lds_bytes(bool fp64, int BM, int Pp);
-// Packed index of movement entry (k = origin, j = destination).
-size_t mte_pack_index(bool fp64, int Pp, int k, int j);
+// LDS bytes of one workgroup; n_rem_split is the number of remainder tiles
+// split into k-quarters (0 when the job does not split).
+size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split);
+// Packed index of movement entry (k = origin, j = destination) in a matrix
+// packed with Kp / 8 k-groups per column block (Kp: the contraction extent).
+size_t mte_pack_index(bool fp64, int Kp, int k, int j);
 
 hipError_t mte_set_lds_limit(bool fp64, size_t bytes);
 hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,

@@ -34,15 +34,16 @@
 
 namespace roadmc {
 
-size_t mte_lds_bytes(bool fp64, int BM, int Pp) {
-  return fp64 ? lds_bytes_t<TraitsF64>(BM, Pp) : lds_bytes_t<TraitsF32>(BM, Pp);
+size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split) {
+  return fp64 ? lds_bytes_t<TraitsF64>(BM, Pp, n_rem_split)
+              : lds_bytes_t<TraitsF32>(BM, Pp, n_rem_split);
 }
 
-size_t mte_pack_index(bool fp64, int Pp, int k, int j) {
+size_t mte_pack_index(bool fp64, int Kp, int k, int j) {
   // Element (k, j) lands where lane l of the wave handling column tile
   // j / TN reads it as B operand s of k-group k / 8.
   const int kg = k >> 3, kk = k & 7;
-  const int nkg = Pp >> 3;
+  const int nkg = Kp >> 3;
   if (!fp64) {
     const int s = kk >> 1, h = kk & 1;
     const int lane = h * 32 + (j & 31), cb = j >> 5;
@@ -99,12 +100,34 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   for (int r = tid; r < BM; r += nthreads) grm[r] = (T)jb.params[0];
   __syncthreads();
 
-  // Output tiles of this wave: a contiguous run of the BM/TM x Pp/TN tiles.
+  // Output tiles (TM x TN blocks of the BM x Pp result) and who computes
+  // them.  Waves w and w + 4 run on the same SIMD, so the work that has to be
+  // even is each SIMD's, not each wave's.
+  //  - Without splitting: wave w takes a contiguous run of ceil or floor of
+  //    ntiles / 8 tiles.
+  //  - With splitting (jb.split): each SIMD takes q = ntiles / 4 whole tiles
+  //    (ceil(q/2) in wave s, the rest in wave s + 4), and each of the
+  //    ntiles % 4 remaining tiles is cut into four k-quarters, quarter s going
+  //    to wave s + 4.  The quarters meet in LDS, where wave i + 4 sums the four
+  //    partial products of remainder tile i in a fixed order and applies its
+  //    epilogue.
   const int ntc = Pp / TN;
   const int ntiles = (BM / TM) * ntc;
-  const int base = ntiles / nwaves, extra = ntiles % nwaves;
-  const int my_n = base + (wave < extra ? 1 : 0);
-  const int my_t0 = wave * base + (wave < extra ? wave : extra);
+  const int simd = wave & 3, upper = wave >> 2;
+  int my_n, my_t0, n_rem = 0, q4 = 0;
+  if (jb.split) {
+    q4 = ntiles >> 2;
+    n_rem = ntiles & 3;
+    const int qa = (q4 + 1) >> 1;
+    my_n = upper ? q4 - qa : qa;
+    my_t0 = simd * q4 + (upper ? qa : 0);
+  } else {
+    const int base = ntiles / nwaves, extra = ntiles % nwaves;
+    my_n = base + (wave < extra ? 1 : 0);
+    my_t0 = wave * base + (wave < extra ? wave : extra);
+  }
+  const int my_np = upper ? n_rem : 0;  // k-quarter pieces of this wave
+  T* qpart = reinterpret_cast<T*>(reinterpret_cast<char*>(red) + 8 * 4 * sizeof(double));
   int trow[NT], tcol[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -113,7 +136,9 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     tcol[t] = ti - (ti / ntc) * ntc;
   }
 
-  const int nkg = Pp >> 3;
+  // The contraction runs over Pk = P rounded up to 16 (the packed matrix has
+  // Pk / 8 k-groups); output columns run over Pp = P rounded up to 32.
+  const int nkg = jb.Pk >> 3;
   const VecT* Bp = reinterpret_cast<const VecT*>(jb.mpack);
   const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
 
@@ -138,6 +163,19 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
         const int r = idx / Pp, c = idx - r * Pp;
         if (p0 + r < jb.N) ck[(size_t)(p0 + r) * Pp + c] = A[r * LDA + TR::perm(c)];
       }
+    }
+    // k-quarter pieces of the remainder tiles first (their partial products
+    // go to LDS): the wave with pieces then overlaps its whole tiles with the
+    // other wave on its SIMD instead of finishing alone on a one-tile chain.
+    for (int i = 0; i < my_np; ++i) {
+      const int ti = 4 * q4 + i;
+      const int tr = ti / ntc, tc = ti - (ti / ntc) * ntc;
+      AccT pacc = AccT{};
+      range_product<TR>(pacc, Bp, A, tr, tc, nkg, (simd * nkg) >> 2, ((simd + 1) * nkg) >> 2,
+                        LDAy, arow, akoff, lane);
+      T* slot = qpart + (size_t)((i * 4 + simd) * NACC) * 64 + lane;
+#pragma unroll
+      for (int r = 0; r < NACC; ++r) slot[r * 64] = pacc[r];
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = AccT{};
@@ -179,41 +217,53 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     }
 
     const bool keep_pre = (year == jb.T - 1) && (jb.endpoint == kEndPre);
+    // Noise, growth and write-back of one output tile.
+    auto epilogue = [&](const AccT& a, const int tr, const int tc) {
+      const int col = tc * TN + TR::acc_col(lane);
+      const int pcol = TR::perm(col);
+      const T K = colsy[col], Ki = colsy[Pp + col], rr = colsy[2 * Pp + col];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      if (t < my_n) {
-        const int col = tcol[t] * TN + TR::acc_col(lane);
-        const int pcol = TR::perm(col);
-        const T K = colsy[col], Ki = colsy[Pp + col], rr = colsy[2 * Pp + col];
+      for (int r4 = 0; r4 < NACC; r4 += 4) {
+        T z[4];
+        if constexpr (TR::kQuadItems) {
+          const int row0 = tr * TM + TR::acc_row(lane, r4);
+          TR::normal4((uint32_t)(p0y + row0) >> 2, (uint32_t)year,
+                      (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
+        }
 #pragma unroll
-        for (int r4 = 0; r4 < NACC; r4 += 4) {
-          T z[4];
+        for (int s = 0; s < 4; ++s) {
+          const int row = tr * TM + TR::acc_row(lane, r4 + s);
+          T zz;
           if constexpr (TR::kQuadItems) {
-            const int row0 = trow[t] * TM + TR::acc_row(lane, r4);
-            TR::normal4((uint32_t)(p0y + row0) >> 2, (uint32_t)year,
-                        (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
+            zz = z[s];
+          } else {
+            const uint32_t path = (uint32_t)(p0y + row);
+            TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0,
+                        jb.key0, jb.key1, z);
+            zz = z[path & 3u];
           }
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const int row = trow[t] * TM + TR::acc_row(lane, r4 + s);
-            T zz;
-            if constexpr (TR::kQuadItems) {
-              zz = z[s];
-            } else {
-              const uint32_t path = (uint32_t)(p0y + row);
-              TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0,
-                          jb.key0, jb.key1, z);
-              zz = z[path & 3u];
-            }
-            const T x = acc[t][r4 + s];
-            T g = (T)0;
-            if constexpr (LAW == kLawRefGpu) g = grm[row];
-            const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
-                                     sd, g, lv);
-            A[row * LDAy + pcol] = keep_pre ? x : n;
-          }
+          const T x = a[r4 + s];
+          T g = (T)0;
+          if constexpr (LAW == kLawRefGpu) g = grm[row];
+          const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
+                                   sd, g, lv);
+          A[row * LDAy + pcol] = keep_pre ? x : n;
         }
       }
+    };
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      if (t < my_n) epilogue(acc[t], trow[t], tcol[t]);
+    if (upper && simd < n_rem) {
+      // Remainder tile `simd`: the four quarters in order, then its epilogue.
+      AccT sum = AccT{};
+      const T* slot = qpart + (size_t)(simd * 4 * NACC) * 64 + lane;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+        for (int r = 0; r < NACC; ++r) sum[r] += slot[(qq * NACC + r) * 64];
+      const int ti = 4 * q4 + simd;
+      epilogue(sum, ti / ntc, ti - (ti / ntc) * ntc);
     }
     __syncthreads();
   }

@@ -59,6 +59,15 @@ typedef struct {
   uint32_t x, y, z, w;
 } rmc_u4;
 
+/* a ^ b ^ c: one v_bitop3_b32 (truth table 0x96) in gfx950 device code. */
+RMC_HD_STATIC uint32_t rmc_xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 /* Philox4x32 with 10 rounds: round r uses key + r*(W0, W1). */
 RMC_HD_STATIC rmc_u4 rmc_philox(uint32_t c0, uint32_t c1, uint32_t c2,
                                 uint32_t c3, uint32_t k0, uint32_t k1) {
@@ -68,8 +77,8 @@ RMC_HD_STATIC rmc_u4 rmc_philox(uint32_t c0, uint32_t c1, uint32_t c2,
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)RMC_PHILOX_M0 * (uint64_t)c0;
     const uint64_t p1 = (uint64_t)RMC_PHILOX_M1 * (uint64_t)c2;
-    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n0 = rmc_xor3((uint32_t)(p1 >> 32), c1, k0);
+    const uint32_t n2 = rmc_xor3((uint32_t)(p0 >> 32), c3, k1);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;

@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kBuckets = 4;
 const int kBucketMax[kBuckets] = {128, 256, 512, 1024};
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950 LDS per CU
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -85,14 +86,16 @@ inline double movement(const roadmc_species& sp, int P, int k, int j, double tol
   return v;
 }
 
+// Packs the movement matrix (Pp output columns, Kp contraction rows) in MFMA
+// B-operand order and the per-patch columns K, 1/K, r, n0.
 template <typename T>
-void pack_species(const roadmc_species& sp, int Pp, double tol, bool fp64, T* mp,
+void pack_species(const roadmc_species& sp, int Pp, int Kp, double tol, bool fp64, T* mp,
                   T* cols) {
   const int P = sp.n_patches;
-  std::fill(mp, mp + (size_t)Pp * Pp, (T)0);
+  std::fill(mp, mp + (size_t)Pp * Kp, (T)0);
   for (int k = 0; k < P; ++k)
     for (int j = 0; j < P; ++j)
-      mp[roadmc::mte_pack_index(fp64, Pp, k, j)] = (T)movement(sp, P, k, j, tol);
+      mp[roadmc::mte_pack_index(fp64, Kp, k, j)] = (T)movement(sp, P, k, j, tol);
   for (int j = 0; j < Pp; ++j) {
     const bool in = j < P;
     cols[j] = in ? (T)sp.capacity[j] : (T)1;
@@ -140,6 +143,7 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       MteJob jb{};
       jb.P = sp.n_patches;
       jb.Pp = round_up(jb.P, 32);
+      jb.Pk = round_up(jb.P, 16);
       jb.N = (int32_t)pr.n_paths;
       jb.T = pr.n_years;
       jb.BM = rows_for(ctx, js.fp64, jb.Pp, pr.n_paths);
@@ -161,12 +165,22 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       js.part_off.push_back((int64_t)js.part_elems);
       js.out_off.push_back((int64_t)js.out_elems);
       js.endpop_off.push_back(want_end_pops ? (int64_t)js.endpop_elems : -1);
-      js.mpack_elems += (size_t)jb.Pp * jb.Pp;
+      js.mpack_elems += (size_t)jb.Pp * jb.Pk;
       js.cols_elems += 4 * (size_t)jb.Pp;
       js.part_elems += (size_t)jb.n_tiles * (3 + jb.Pp);
       js.out_elems += 3 + (size_t)jb.Pp;
       if (want_end_pops) js.endpop_elems += (size_t)jb.N;
-      js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.fp64, jb.BM, jb.Pp));
+      // Remainder tiles (ntiles % 4) would leave one SIMD with more work
+      // than the others; split them into k-quarters when LDS has room for
+      // the partial products (FP32 kernel, at least 8 k-groups).
+      const int ntiles = (jb.BM / roadmc::kMteRowsF32) * (jb.Pp / 32);
+      const int rem = ntiles % 4;
+      jb.split = 0;
+      if (!js.fp64 && rem && (jb.Pk >> 3) >= 8 &&
+          roadmc::mte_lds_bytes(false, jb.BM, jb.Pp, rem) <= kLdsPerCu)
+        jb.split = 1;
+      js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.fp64, jb.BM, jb.Pp,
+                                                      jb.split ? rem : 0));
       js.jobs.push_back(jb);
       js.road_of.push_back(r);
       js.species_of.push_back(s);
@@ -224,8 +238,8 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
   std::vector<T> mp(js.mpack_elems), cols(js.cols_elems);
   for (size_t j = 0; j < js.jobs.size(); ++j) {
     const roadmc_mte_problem& pr = roads[js.road_of[j]];
-    pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, pr.drop_tol, js.fp64,
-                    mp.data() + js.mpack_off[j], cols.data() + js.cols_off[j]);
+    pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, js.jobs[j].Pk, pr.drop_tol,
+                    js.fp64, mp.data() + js.mpack_off[j], cols.data() + js.cols_off[j]);
     MteJob& jb = js.jobs[j];
     jb.mpack = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];
     jb.cols = static_cast<T*>(b.cols.ptr) + js.cols_off[j];
@@ -418,7 +432,7 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
   if (!rc) {
     // One workgroup per CU fits the LDS of the largest tile; small tiles
     // leave room for a second one.
-    const int per_cu = bt->js.lds * 2 <= 160 * 1024 ? 2 : 1;
+    const int per_cu = bt->js.lds * 2 <= kLdsPerCu ? 2 : 1;
     bt->grid = ctx->n_cu * per_cu;
     bt->roads.assign(roads, roads + n_roads);
     rc = capture_batch_graph(bt);
@@ -803,7 +817,7 @@ int adjoint_upload(roadmc_adjoint* adj, const roadmc_species& sp, double tol) {
   const int Pp = adj->job.Pp, P = adj->P;
   const bool fp64 = adj->fp64;
   std::vector<T> mp((size_t)Pp * Pp), mt((size_t)Pp * Pp, (T)0), cols(4 * (size_t)Pp);
-  pack_species<T>(sp, Pp, tol, fp64, mp.data(), cols.data());
+  pack_species<T>(sp, Pp, Pp, tol, fp64, mp.data(), cols.data());
   for (int k = 0; k < P; ++k)
     for (int j = 0; j < P; ++j)
       mt[roadmc::mte_pack_index(fp64, Pp, j, k)] = (T)movement(sp, P, k, j, tol);
@@ -859,6 +873,8 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   MteJob& jb = adj->job;
   jb.P = sp.n_patches;
   jb.Pp = round_up(jb.P, 64);  // the gradient GEMM works in 64 x 64 blocks
+  jb.Pk = jb.Pp;               // the adjoint kernels contract over Pp
+  jb.split = 0;
   jb.N = (int32_t)pr->n_paths;
   jb.T = pr->n_years;
   jb.BM = rows_for(ctx, adj->fp64, jb.Pp, pr->n_paths);
@@ -898,7 +914,7 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   std::vector<int2> work(jb.n_tiles);
   for (int t = 0; t < jb.n_tiles; ++t) work[t] = make_int2(0, t);
   const int64_t off0 = 0;
-  const size_t lds = roadmc::mte_lds_bytes(adj->fp64, jb.BM, jb.Pp);
+  const size_t lds = roadmc::mte_lds_bytes(adj->fp64, jb.BM, jb.Pp, 0);
   hipStream_t s = ctx->stream;
   auto fail = [&](int code) {
     roadmc_adjoint_destroy(adj);
