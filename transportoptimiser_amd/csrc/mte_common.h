@@ -176,6 +176,9 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
       b1[t] = bp[t][(size_t)(kg + 1) * 64];
       a1[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 1) * 8);
     }
+    // Keep the loads ahead of the MFMAs that do not need them: the scheduler
+    // would otherwise sink them next to their use and expose their latency.
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
 #pragma unroll
@@ -188,6 +191,7 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
       b0[t] = bp[t][(size_t)k2 * 64];
       a0[t] = *reinterpret_cast<const VecT*>(ap[t] + k2 * 8);
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
 #pragma unroll
@@ -197,9 +201,9 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
 }
 
 // acc += A[rows of tile (trow, tcol), k-groups kg0 .. kg1) . M[those k-groups,
-// cols of the tile] for one output tile, kg1 > kg0: one k-group per step with
-// the next step's operands in flight (the last step re-reads its own, so the
-// loop has no tail branch).
+// cols of the tile] for one output tile, kg1 > kg0, with the same two-stage
+// ping-pong as year_product (the operands of the next k-group are in flight
+// while the MFMAs of the current one run) and an odd tail step.
 template <class TR>
 __device__ __forceinline__ void range_product(typename TR::AccT& acc,
                                               const typename TR::VecT* Bp,
@@ -212,16 +216,25 @@ __device__ __forceinline__ void range_product(typename TR::AccT& acc,
   const __attribute__((address_space(1))) VecT* bp =
       as_global(Bp + (size_t)tcol * nkg * 64 + lane);
   const T* ap = A + (trow * TM + arow) * LDA + akoff;
-  VecT b = bp[(size_t)kg0 * 64];
-  VecT a = *reinterpret_cast<const VecT*>(ap + kg0 * 8);
-  for (int kg = kg0; kg < kg1; ++kg) {
-    const int kn = min(kg + 1, kg1 - 1);
-    const VecT bn = bp[(size_t)kn * 64];
-    const VecT an = *reinterpret_cast<const VecT*>(ap + kn * 8);
+  VecT b0 = bp[(size_t)kg0 * 64];
+  VecT a0 = *reinterpret_cast<const VecT*>(ap + kg0 * 8);
+  int kg = kg0;
+  for (; kg + 1 < kg1; kg += 2) {
+    const VecT b1 = bp[(size_t)(kg + 1) * 64];
+    const VecT a1 = *reinterpret_cast<const VecT*>(ap + (kg + 1) * 8);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int s = 0; s < KV; ++s) acc = TR::mfma(a[s], b[s], acc);
-    a = an;
-    b = bn;
+    for (int s = 0; s < KV; ++s) acc = TR::mfma(a0[s], b0[s], acc);
+    const int k2 = min(kg + 2, kg1 - 1);
+    b0 = bp[(size_t)k2 * 64];
+    a0 = *reinterpret_cast<const VecT*>(ap + k2 * 8);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < KV; ++s) acc = TR::mfma(a1[s], b1[s], acc);
+  }
+  if (kg < kg1) {
+#pragma unroll
+    for (int s = 0; s < KV; ++s) acc = TR::mfma(a0[s], b0[s], acc);
   }
 }
 
