@@ -95,24 +95,24 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   AccT acc[NT];
   VecT b0[NT];
 
-  auto product = [&](const VecT* Bp) {
+  auto product = [&](const PackedB& Bm) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       acc[t] = AccT{};
-      if (t < my_n) b0[t] = as_global(Bp)[(size_t)tcol[t] * nkg * 64 + lane];
+      if (t < my_n) b0[t] = load_b<VecT>(Bm, b_voff(Bm, tcol[t], lane), 0);
     }
     switch (my_n) {
-      case 1: year_product<TR, 1>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 2: year_product<TR, 2>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 3: year_product<TR, 3>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-      case 4: year_product<TR, 4>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+      case 1: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+      case 2: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+      case 3: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+      case 4: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
       default:
         if constexpr (NT > 4) {
           switch (my_n) {
-            case 5: year_product<TR, 5>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 6: year_product<TR, 6>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 7: year_product<TR, 7>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
-            case 8: year_product<TR, 8>(acc, b0, Bp, A, trow, tcol, nkg, LDA, arow, akoff, lane); break;
+            case 5: year_product<TR, 5>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+            case 6: year_product<TR, 6>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+            case 7: year_product<TR, 7>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+            case 8: year_product<TR, 8>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
             default: break;
           }
         }
@@ -120,8 +120,8 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     }
   };
 
-  const VecT* Mp = static_cast<const VecT*>(jb.mpack);
-  const VecT* Mtp = static_cast<const VecT*>(jb.mtpack);
+  const PackedB Mp = packed_b(jb.mpack, nkg, (size_t)Pp * Pp * sizeof(T));
+  const PackedB Mtp = packed_b(jb.mtpack, nkg, (size_t)Pp * Pp * sizeof(T));
 
   // Recompute the segment: ns[i] = n_{y0+i}, np[i] = n'_{y0+i+1}.
   for (int yr = jb.y0; yr <= jb.y1; ++yr) {

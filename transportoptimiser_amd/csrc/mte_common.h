@@ -140,6 +140,35 @@ __device__ __forceinline__ const __attribute__((address_space(1))) V* as_global(
   return (const __attribute__((address_space(1))) V*)(p);
 }
 
+// Movement matrix of one job in MFMA B-operand order, read through a buffer
+// resource: the k-group step goes into the scalar offset and the tile's
+// column block into a 32-bit vector offset, so operand loads need no vector
+// address arithmetic.  Reads past `bytes` return zero instead of faulting.
+struct PackedB {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int nkg;  // k-groups per column block
+};
+
+__device__ __forceinline__ PackedB packed_b(const void* p, int nkg, size_t bytes) {
+  PackedB b;
+  // Word 3 of the descriptor: raw 32-bit data format of the gfx9 family.
+  b.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+  b.nkg = nkg;
+  return b;
+}
+
+// Byte offset of lane `lane`'s 16-byte operand in column block `tcol`.
+__device__ __forceinline__ uint32_t b_voff(const PackedB& B, int tcol, int lane) {
+  return ((uint32_t)tcol * (uint32_t)B.nkg * 64u + (uint32_t)lane) * 16u;
+}
+
+template <class VecT>
+__device__ __forceinline__ VecT load_b(const PackedB& B, uint32_t voff, int kg) {
+  static_assert(sizeof(VecT) == 16, "B operands are 16-byte vectors");
+  return __builtin_bit_cast(
+      VecT, __builtin_amdgcn_raw_buffer_load_b128(B.rsrc, (int)voff, kg * 64 * 16, 0));
+}
+
 // One year's product for the NTILE output tiles of a wave:
 // acc[t] += A[rows of tile t, :] . M[:, cols of tile t].
 // Two register stages ping-pong over the k-groups (the k-group count Pp/8 is a
@@ -151,21 +180,22 @@ __device__ __forceinline__ const __attribute__((address_space(1))) V* as_global(
 template <class TR, int NTILE>
 __device__ __forceinline__ void year_product(typename TR::AccT* acc,
                                              typename TR::VecT* b0,
-                                             const typename TR::VecT* Bp,
+                                             const PackedB& B,
                                              const typename TR::T* A,
                                              const int* trow, const int* tcol,
-                                             int nkg, int LDA, int arow,
-                                             int akoff, int lane) {
+                                             int LDA, int arow, int akoff, int lane) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   constexpr int TM = TR::TM, KV = TR::KV;
-  const __attribute__((address_space(1))) VecT* bp[NTILE];
+  const int nkg = B.nkg;
+  uint32_t boff[NTILE];
   const T* ap[NTILE];
 #pragma unroll
   for (int t = 0; t < NTILE; ++t) {
-    bp[t] = as_global(Bp + (size_t)tcol[t] * nkg * 64 + lane);
+    boff[t] = b_voff(B, tcol[t], lane);
     ap[t] = A + (trow[t] * TM + arow) * LDA + akoff;
   }
+  auto ldb = [&](int t, int kg) -> VecT { return load_b<VecT>(B, boff[t], kg); };
   VecT b1[NTILE], a0[NTILE], a1[NTILE];
 #pragma unroll
   for (int t = 0; t < NTILE; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
@@ -173,7 +203,7 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
     // stage 1 loads (k-group kg+1 < nkg always: nkg is even)
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
-      b1[t] = bp[t][(size_t)(kg + 1) * 64];
+      b1[t] = ldb(t, kg + 1);
       a1[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 1) * 8);
     }
     // Keep the loads ahead of the MFMAs that do not need them: the scheduler
@@ -184,12 +214,15 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
 #pragma unroll
       for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[t][s], b0[t][s], acc[t]);
     }
-    // stage 0 loads for k-group kg+2; after the last pass, k-group 0 again
+    // stage 0 loads for k-group kg+2.  After the last pass b0 re-fetches
+    // k-group 0, which the next year starts with; a0 then reads the 8 values
+    // past column Pk - 1 of its row (inside the LDS allocation, never used:
+    // the next call reloads a0), which keeps its address an immediate offset.
     const int k2 = (kg + 2 < nkg) ? kg + 2 : 0;
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
-      b0[t] = bp[t][(size_t)k2 * 64];
-      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + k2 * 8);
+      b0[t] = ldb(t, k2);
+      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 2) * 8);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -205,28 +238,26 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
 // ping-pong as year_product (the operands of the next k-group are in flight
 // while the MFMAs of the current one run) and an odd tail step.
 template <class TR>
-__device__ __forceinline__ void range_product(typename TR::AccT& acc,
-                                              const typename TR::VecT* Bp,
+__device__ __forceinline__ void range_product(typename TR::AccT& acc, const PackedB& B,
                                               const typename TR::T* A, int trow,
-                                              int tcol, int nkg, int kg0, int kg1,
+                                              int tcol, int kg0, int kg1,
                                               int LDA, int arow, int akoff, int lane) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   constexpr int TM = TR::TM, KV = TR::KV;
-  const __attribute__((address_space(1))) VecT* bp =
-      as_global(Bp + (size_t)tcol * nkg * 64 + lane);
+  const uint32_t boff = b_voff(B, tcol, lane);
   const T* ap = A + (trow * TM + arow) * LDA + akoff;
-  VecT b0 = bp[(size_t)kg0 * 64];
+  VecT b0 = load_b<VecT>(B, boff, kg0);
   VecT a0 = *reinterpret_cast<const VecT*>(ap + kg0 * 8);
   int kg = kg0;
   for (; kg + 1 < kg1; kg += 2) {
-    const VecT b1 = bp[(size_t)(kg + 1) * 64];
+    const VecT b1 = load_b<VecT>(B, boff, kg + 1);
     const VecT a1 = *reinterpret_cast<const VecT*>(ap + (kg + 1) * 8);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int s = 0; s < KV; ++s) acc = TR::mfma(a0[s], b0[s], acc);
     const int k2 = min(kg + 2, kg1 - 1);
-    b0 = bp[(size_t)k2 * 64];
+    b0 = load_b<VecT>(B, boff, k2);
     a0 = *reinterpret_cast<const VecT*>(ap + k2 * 8);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

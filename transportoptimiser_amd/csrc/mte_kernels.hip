@@ -139,14 +139,14 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   // The contraction runs over Pk = P rounded up to 16 (the packed matrix has
   // Pk / 8 k-groups); output columns run over Pp = P rounded up to 32.
   const int nkg = jb.Pk >> 3;
-  const VecT* Bp = reinterpret_cast<const VecT*>(jb.mpack);
+  const PackedB Bm = packed_b(jb.mpack, nkg, (size_t)Pp * jb.Pk * sizeof(T));
   const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
 
   AccT acc[NT];
   VecT b0[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t)
-    if (t < my_n) b0[t] = as_global(Bp)[(size_t)tcol[t] * nkg * 64 + lane];
+    if (t < my_n) b0[t] = load_b<VecT>(Bm, b_voff(Bm, tcol[t], lane), 0);
   for (int year = 0; year < jb.T; ++year) {
     // Opaque per-year copies of loop-invariant scalars.  Addresses derived
     // from them (the 64 LDS write addresses and 16 Philox counters of a lane,
@@ -171,7 +171,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
       const int ti = 4 * q4 + i;
       const int tr = ti / ntc, tc = ti - (ti / ntc) * ntc;
       AccT pacc = AccT{};
-      range_product<TR>(pacc, Bp, A, tr, tc, nkg, (simd * nkg) >> 2, ((simd + 1) * nkg) >> 2,
+      range_product<TR>(pacc, Bm, A, tr, tc, (simd * nkg) >> 2, ((simd + 1) * nkg) >> 2,
                         LDAy, arow, akoff, lane);
       T* slot = qpart + (size_t)((i * 4 + simd) * NACC) * 64 + lane;
 #pragma unroll
@@ -182,17 +182,17 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     // One branch per year on the (wave-uniform) tile count; the product
     // loops themselves are straight-line code.
     switch (my_n) {
-      case 1: year_product<TR, 1>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-      case 2: year_product<TR, 2>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-      case 3: year_product<TR, 3>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-      case 4: year_product<TR, 4>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+      case 1: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 2: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 3: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 4: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
       default:
         if constexpr (NT > 4) {
           switch (my_n) {
-            case 5: year_product<TR, 5>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-            case 6: year_product<TR, 6>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-            case 7: year_product<TR, 7>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
-            case 8: year_product<TR, 8>(acc, b0, Bp, A, trow, tcol, nkg, LDAy, arow, akoff, lane); break;
+            case 5: year_product<TR, 5>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+            case 6: year_product<TR, 6>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+            case 7: year_product<TR, 7>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+            case 8: year_product<TR, 8>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
             default: break;
           }
         }
