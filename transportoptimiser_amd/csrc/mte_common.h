@@ -177,7 +177,7 @@ __device__ __forceinline__ VecT load_b(const PackedB& B, uint32_t voff, int kg) 
 // On entry b0 holds the B operands of k-group 0; on exit it holds them again
 // (the last stage-0 load re-fetches k-group 0, which the next year starts
 // with, so that load's latency is hidden behind the epilogue).
-template <class TR, int NTILE>
+template <class TR, int NTILE, bool SHARED_A = false>
 __device__ __forceinline__ void year_product(typename TR::AccT* acc,
                                              typename TR::VecT* b0,
                                              const PackedB& B,
@@ -187,32 +187,34 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   constexpr int TM = TR::TM, KV = TR::KV;
+  // With SHARED_A all NTILE tiles lie in one row block: one A fragment per
+  // k-group serves every tile.
+  constexpr int NA = SHARED_A ? 1 : NTILE;
   const int nkg = B.nkg;
   uint32_t boff[NTILE];
-  const T* ap[NTILE];
+  const T* ap[NA];
 #pragma unroll
-  for (int t = 0; t < NTILE; ++t) {
-    boff[t] = b_voff(B, tcol[t], lane);
-    ap[t] = A + (trow[t] * TM + arow) * LDA + akoff;
-  }
+  for (int t = 0; t < NTILE; ++t) boff[t] = b_voff(B, tcol[t], lane);
+#pragma unroll
+  for (int t = 0; t < NA; ++t) ap[t] = A + (trow[t] * TM + arow) * LDA + akoff;
   auto ldb = [&](int t, int kg) -> VecT { return load_b<VecT>(B, boff[t], kg); };
-  VecT b1[NTILE], a0[NTILE], a1[NTILE];
+  auto ia = [](int t) { return SHARED_A ? 0 : t; };
+  VecT b1[NTILE], a0[NA], a1[NA];
 #pragma unroll
-  for (int t = 0; t < NTILE; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
+  for (int t = 0; t < NA; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
   for (int kg = 0; kg < nkg; kg += 2) {
     // stage 1 loads (k-group kg+1 < nkg always: nkg is even)
 #pragma unroll
-    for (int t = 0; t < NTILE; ++t) {
-      b1[t] = ldb(t, kg + 1);
-      a1[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 1) * 8);
-    }
+    for (int t = 0; t < NTILE; ++t) b1[t] = ldb(t, kg + 1);
+#pragma unroll
+    for (int t = 0; t < NA; ++t) a1[t] = *reinterpret_cast<const VecT*>(ap[t] + 8);
     // Keep the loads ahead of the MFMAs that do not need them: the scheduler
     // would otherwise sink them next to their use and expose their latency.
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
 #pragma unroll
-      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[t][s], b0[t][s], acc[t]);
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[ia(t)][s], b0[t][s], acc[t]);
     }
     // stage 0 loads for k-group kg+2.  After the last pass b0 re-fetches
     // k-group 0, which the next year starts with; a0 then reads the 8 values
@@ -220,15 +222,17 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
     // the next call reloads a0), which keeps its address an immediate offset.
     const int k2 = (kg + 2 < nkg) ? kg + 2 : 0;
 #pragma unroll
-    for (int t = 0; t < NTILE; ++t) {
-      b0[t] = ldb(t, k2);
-      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + (kg + 2) * 8);
+    for (int t = 0; t < NTILE; ++t) b0[t] = ldb(t, k2);
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      a0[t] = *reinterpret_cast<const VecT*>(ap[t] + 16);
+      ap[t] += 16;
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
 #pragma unroll
-      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[t][s], b1[t][s], acc[t]);
+      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[ia(t)][s], b1[t][s], acc[t]);
     }
   }
 }

@@ -142,6 +142,10 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   const PackedB Bm = packed_b(jb.mpack, nkg, (size_t)Pp * jb.Pk * sizeof(T));
   const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
 
+  // All of this wave's tiles in one row block (contiguous runs usually are):
+  // they can share one A fragment per k-group.
+  const bool shared_a = my_n > 1 && my_t0 / ntc == (my_t0 + my_n - 1) / ntc;
+
   AccT acc[NT];
   VecT b0[NT];
 #pragma unroll
@@ -179,13 +183,16 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = AccT{};
-    // One branch per year on the (wave-uniform) tile count; the product
-    // loops themselves are straight-line code.
-    switch (my_n) {
-      case 1: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 2: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 3: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 4: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+    // One branch per year on the (wave-uniform) tile count and row sharing;
+    // the product loops themselves are straight-line code.
+    switch (my_n * 2 + (shared_a ? 1 : 0)) {
+      case 2: case 3: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 4: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 5: year_product<TR, 2, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 6: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 7: year_product<TR, 3, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 8: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+      case 9: year_product<TR, 4, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
       default:
         if constexpr (NT > 4) {
           switch (my_n) {
