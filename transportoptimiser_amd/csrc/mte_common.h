@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mte_kernels.h"
 #include "rng.h"
 
@@ -170,10 +172,10 @@ __device__ __forceinline__ VecT load_b(const PackedB& B, uint32_t voff, int kg) 
 }
 
 // One year's product for the NTILE output tiles of a wave:
-// acc[t] += A[rows of tile t, :] . M[:, cols of tile t].
-// Two register stages ping-pong over the k-groups (the k-group count Pp/8 is a
-// multiple of 4): the B operands and LDS A operands of k-group kg+1 are in
-// flight while the MFMAs of k-group kg run.
+// acc[t] = A[rows of tile t, :] . M[:, cols of tile t].
+// Two register stages ping-pong over the k-groups (the k-group count B.nkg is
+// even): the B operands and LDS A operands of k-group kg+1 are in flight while
+// the MFMAs of k-group kg run.
 // On entry b0 holds the B operands of k-group 0; on exit it holds them again
 // (the last stage-0 load re-fetches k-group 0, which the next year starts
 // with, so that load's latency is hidden behind the epilogue).
@@ -186,6 +188,7 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
                                              int LDA, int arow, int akoff, int lane) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
+  using AccT = typename TR::AccT;
   constexpr int TM = TR::TM, KV = TR::KV;
   // With SHARED_A all NTILE tiles lie in one row block: one A fragment per
   // k-group serves every tile.
@@ -202,7 +205,9 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
   VecT b1[NTILE], a0[NA], a1[NA];
 #pragma unroll
   for (int t = 0; t < NA; ++t) a0[t] = *reinterpret_cast<const VecT*>(ap[t]);
-  for (int kg = 0; kg < nkg; kg += 2) {
+  // Two k-groups per step.  FIRST: the step's first MFMA per tile starts
+  // from a zero accumulator (an inline constant, no register clearing).
+  auto step = [&](const int kg, auto first) {
     // stage 1 loads (k-group kg+1 < nkg always: nkg is even)
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) b1[t] = ldb(t, kg + 1);
@@ -213,10 +218,14 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int t = 0; t < NTILE; ++t) {
+      if constexpr (decltype(first)::value)
+        acc[t] = TR::mfma(a0[ia(t)][0], b0[t][0], AccT{});
+      else
+        acc[t] = TR::mfma(a0[ia(t)][0], b0[t][0], acc[t]);
 #pragma unroll
-      for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a0[ia(t)][s], b0[t][s], acc[t]);
+      for (int s = 1; s < KV; ++s) acc[t] = TR::mfma(a0[ia(t)][s], b0[t][s], acc[t]);
     }
-    // stage 0 loads for k-group kg+2.  After the last pass b0 re-fetches
+    // stage 0 loads for k-group kg+2.  After the last step b0 re-fetches
     // k-group 0, which the next year starts with; a0 then reads the 8 values
     // past column Pk - 1 of its row (inside the LDS allocation, never used:
     // the next call reloads a0), which keeps its address an immediate offset.
@@ -234,7 +243,9 @@ __device__ __forceinline__ void year_product(typename TR::AccT* acc,
 #pragma unroll
       for (int s = 0; s < KV; ++s) acc[t] = TR::mfma(a1[ia(t)][s], b1[t][s], acc[t]);
     }
-  }
+  };
+  step(0, std::true_type{});
+  for (int kg = 2; kg < nkg; kg += 2) step(kg, std::false_type{});
 }
 
 // acc += A[rows of tile (trow, tcol), k-groups kg0 .. kg1) . M[those k-groups,

@@ -181,8 +181,6 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
 #pragma unroll
       for (int r = 0; r < NACC; ++r) slot[r * 64] = pacc[r];
     }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = AccT{};
     // One branch per year on the (wave-uniform) tile count and row sharing;
     // the product loops themselves are straight-line code.
     switch (my_n * 2 + (shared_a ? 1 : 0)) {
@@ -223,38 +221,56 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
       __syncthreads();
     }
 
+    // Fresh opaque copies for the epilogue: nothing it derives from them can
+    // be computed before (and kept live across) the product loop.
+    int LDAe = LDA, p0e = p0, lanee = lane;
+    const T* colse = cols;
+    asm volatile("" : "+s"(LDAe), "+s"(p0e), "+s"(colse), "+v"(lanee));
     const bool keep_pre = (year == jb.T - 1) && (jb.endpoint == kEndPre);
+    // FP32 LOGNORMAL: n = x (a - b x) exp2(s' z - h') with a = 1 + r and
+    // b = r / K; the pre-growth endpoint year uses s' = h' = 0, a = 1, b = 0,
+    // which returns x exactly, instead of a per-element select.
+    constexpr bool kFastLn = LAW == kLawLognormal && sizeof(T) == 4;
+    const T sig_y = (kFastLn && keep_pre) ? (T)0 : sig;
+    const T hv_y = (kFastLn && keep_pre) ? (T)0 : half_var;
+    const T rscale = (kFastLn && keep_pre) ? (T)0 : (T)1;
     // Noise, growth and write-back of one output tile.
     auto epilogue = [&](const AccT& a, const int tr, const int tc) {
-      const int col = tc * TN + TR::acc_col(lane);
+      const int col = tc * TN + TR::acc_col(lanee);
       const int pcol = TR::perm(col);
-      const T K = colsy[col], Ki = colsy[Pp + col], rr = colsy[2 * Pp + col];
+      const T K = colse[col], Ki = colse[Pp + col], rr = colse[2 * Pp + col];
+      const T ga = (T)1 + rr * rscale, gb = rr * rscale * Ki;
 #pragma unroll
       for (int r4 = 0; r4 < NACC; r4 += 4) {
         T z[4];
         if constexpr (TR::kQuadItems) {
-          const int row0 = tr * TM + TR::acc_row(lane, r4);
-          TR::normal4((uint32_t)(p0y + row0) >> 2, (uint32_t)year,
+          const int row0 = tr * TM + TR::acc_row(lanee, r4);
+          TR::normal4((uint32_t)(p0e + row0) >> 2, (uint32_t)year,
                       (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          const int row = tr * TM + TR::acc_row(lane, r4 + s);
+          const int row = tr * TM + TR::acc_row(lanee, r4 + s);
           T zz;
           if constexpr (TR::kQuadItems) {
             zz = z[s];
           } else {
-            const uint32_t path = (uint32_t)(p0y + row);
+            const uint32_t path = (uint32_t)(p0e + row);
             TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0,
                         jb.key0, jb.key1, z);
             zz = z[path & 3u];
           }
           const T x = a[r4 + s];
-          T g = (T)0;
-          if constexpr (LAW == kLawRefGpu) g = grm[row];
-          const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
-                                   sd, g, lv);
-          A[row * LDAy + pcol] = keep_pre ? x : n;
+          if constexpr (kFastLn) {
+            const T e = __builtin_amdgcn_exp2f(fmaf(sig_y, zz, -hv_y));
+            A[row * LDAe + pcol] = x * fmaf(-gb, x, ga) * e;
+          } else {
+            T g = (T)0;
+            if constexpr (LAW == kLawRefGpu) g = grm[row];
+            const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
+                                     sd, g, lv);
+            A[row * LDAe + pcol] = keep_pre ? x : n;
+          }
         }
       }
     };
@@ -264,7 +280,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     if (upper && simd < n_rem) {
       // Remainder tile `simd`: the four quarters in order, then its epilogue.
       AccT sum = AccT{};
-      const T* slot = qpart + (size_t)(simd * 4 * NACC) * 64 + lane;
+      const T* slot = qpart + (size_t)(simd * 4 * NACC) * 64 + lanee;
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq)
 #pragma unroll

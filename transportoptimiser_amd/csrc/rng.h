@@ -101,7 +101,12 @@ RMC_HD_STATIC uint32_t rmc_tag(uint32_t road_id, uint32_t species,
 
 /* Uniform in (0,1): 23 random bits plus a half-step offset, exact in float. */
 RMC_HD_STATIC float rmc_u01f(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // (m + 1/2) 2^-23 as one FMA; exact, so equal to the expression below.
+  return __builtin_fmaf((float)(x >> 9), 1.0f / 8388608.0f, 1.0f / 16777216.0f);
+#else
   return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f);
+#endif
 }
 
 RMC_HD_STATIC double rmc_u01d(uint32_t x) {
