@@ -100,6 +100,8 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->adj_bufs.e,     &ctx->adj_bufs.f,    &ctx->adj_bufs.g,
                         &ctx->adj_bufs.h};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
+  if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
+  rmc_buf_free(ctx->rov_graph.param);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

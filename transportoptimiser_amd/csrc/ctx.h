@@ -30,6 +30,14 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } rov_bufs;
+  // Backward-induction graph of the last ROV shape; its kernels read their
+  // parameters from rov_param (device), so one graph serves every problem of
+  // that shape.
+  struct {
+    hipGraphExec_t exec = nullptr;
+    long long key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    rmc_devbuf param;
+  } rov_graph;
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } adj_bufs;

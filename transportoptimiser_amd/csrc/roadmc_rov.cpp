@@ -115,7 +115,7 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
       (rc = rmc_buf_ensure(ctx, B.c, (size_t)(T + 1) * N * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.d, (size_t)T * N * sizeof(int))) ||
       (rc = rmc_buf_ensure(ctx, B.e, (size_t)(T + 1) * U * 2 * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, (size_t)nblk * G * E * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.f, ((size_t)nblk + 1) * G * E * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.g, (size_t)T * G * K * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.h, ((size_t)T * G + 8) * sizeof(int) + 64)))
     return rc;
@@ -198,6 +198,8 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   a.V = d_V;
   a.optc = d_optc;
   a.partial = static_cast<double*>(B.f.ptr);
+  a.sums = a.partial + (size_t)nblk * G * E;
+  a.nblk = nblk;
   a.beta = static_cast<double*>(B.g.ptr);
   a.status = d_status;
 
@@ -206,33 +208,52 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   RMC_HIP(ctx, hipMemsetAsync(d_status, 0, (size_t)T * G * sizeof(int), s));
 
   RMC_HIP(ctx, roadmc::rov_set_lds_limits());
-  // Backward induction as one graph: terminal decision, then fit + decide
-  // for every interior step, then time 0.
-  RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipSuccess;
-  if (american) {
-    e = roadmc::launch_rov_step(a, T, false, true, nblk, s);
-    for (int t = T - 1; t >= 1 && e == hipSuccess; --t)
-      e = roadmc::launch_rov_step(a, t, true, false, nblk, s);
-  } else {
-    e = roadmc::launch_rov_step(a, T - 1, false, true, nblk, s);
-    for (int t = T - 2; t >= 1 && e == hipSuccess; --t)
-      e = roadmc::launch_rov_step(a, t, true, false, nblk, s);
+  a.out = d_out;
+  // Parameters go to the device block the captured kernels read.
+  auto& RG = ctx->rov_graph;
+  if ((rc = rmc_buf_ensure(ctx, RG.param, sizeof(roadmc::RovStepArgs)))) return rc;
+  const roadmc::RovStepArgs* pa = static_cast<const roadmc::RovStepArgs*>(RG.param.ptr);
+  RMC_HIP(ctx, hipMemcpyAsync(RG.param.ptr, &a, sizeof(a), hipMemcpyHostToDevice, s));
+  // The graph depends only on the launch shapes and the parameter block's
+  // address.
+  const long long key[8] = {N, T, K, G * 1000LL + E, C, american ? 1 : 0,
+                            (long long)(uintptr_t)pa, a.nblk};
+  if (!RG.exec || std::memcmp(key, RG.key, sizeof(key)) != 0) {
+    if (RG.exec) {
+      (void)hipGraphExecDestroy(RG.exec);
+      RG.exec = nullptr;
+    }
+    // Backward induction as one graph: terminal decision, then fit + decide
+    // for every interior step, then time 0.
+    RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    if (american) {
+      e = roadmc::launch_rov_step(a, pa, T, false, true, s);
+      for (int t = T - 1; t >= 1 && e == hipSuccess; --t)
+        e = roadmc::launch_rov_step(a, pa, t, true, false, s);
+    } else {
+      e = roadmc::launch_rov_step(a, pa, T - 1, false, true, s);
+      for (int t = T - 2; t >= 1 && e == hipSuccess; --t)
+        e = roadmc::launch_rov_step(a, pa, t, true, false, s);
+    }
+    if (e == hipSuccess) e = roadmc::launch_rov_first(pa, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rmc_fail(ctx, ROADMC_ERR_HIP,
+                      std::string("rov capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+    }
+    e = hipGraphInstantiate(&RG.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      RG.exec = nullptr;
+      return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("rov graph: ") + hipGetErrorString(e));
+    }
+    std::memcpy(RG.key, key, sizeof(key));
   }
-  if (e == hipSuccess) e = roadmc::launch_rov_first(a, d_out, s);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(s, &graph);
-  if (e != hipSuccess || ec != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rmc_fail(ctx, ROADMC_ERR_HIP,
-                    std::string("rov capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
-  }
-  hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+  hipError_t e = hipGraphLaunch(RG.exec, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   if (e != hipSuccess)
     return rmc_fail(ctx, ROADMC_ERR_HIP, std::string("rov graph: ") + hipGetErrorString(e));
 

@@ -31,9 +31,12 @@ struct RovStepArgs {
   const int* controls;   // [T][N]
   double* V;             // [(T+1)][N]
   int* optc;             // [T][N]
-  double* partial;       // [nblk][G][E]
+  double* partial;       // [G][E][nblk] block partial sums of one step
+  double* sums;          // [G][E] their fixed-order totals
+  int nblk;              // assembly blocks
   double* beta;          // [T][G][K]
   int* status;           // [T][G]
+  double* out;           // [3] value, control, SD of time 0
 };
 
 struct ExpPvArgs {
@@ -48,9 +51,12 @@ hipError_t rov_set_lds_limits();
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s);
 hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double* scale,
                               hipStream_t s);
-hipError_t launch_rov_step(const RovStepArgs& a, int t, bool fit, bool terminal, int nblk,
-                           hipStream_t s);
-hipError_t launch_rov_first(const RovStepArgs& a, double* out, hipStream_t s);
+// The step kernels read their parameters from the device copy `pa` of `a`
+// (a supplies the launch shapes), so a captured graph of them stays valid
+// for any problem of the same shape.
+hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, bool fit,
+                           bool terminal, hipStream_t s);
+hipError_t launch_rov_first(const RovStepArgs* pa, hipStream_t s);
 hipError_t launch_exp_pv(const ExpPvArgs& a, hipStream_t s);
 
 }  // namespace roadmc

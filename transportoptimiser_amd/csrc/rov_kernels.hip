@@ -30,50 +30,61 @@ __device__ __forceinline__ void rov_normals(uint32_t p, uint32_t t, uint32_t blk
   rmc_normal4_d(p, t, blk, tag, k0, k1, z);
 }
 
-__global__ void rov_forward_kernel(RovForwardArgs a) {
+__global__ __launch_bounds__(256) void rov_forward_kernel(RovForwardArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.N) return;
   const int U = a.U;
+  // Per-uncertainty arrays are indexed only by unrolled loop counters, so
+  // they stay in registers.
   double x[kRovMaxU];
-  for (int u = 0; u < U; ++u) {
-    x[u] = a.par[u * 7 + 1];
-    a.states[(size_t)u * a.N + p] = x[u];
+#pragma unroll
+  for (int u = 0; u < kRovMaxU; ++u) {
+    x[u] = 0.0;
+    if (u < U) {
+      x[u] = a.par[u * 7 + 1];
+      a.states[(size_t)u * a.N + p] = x[u];
+    }
   }
   const double dt = a.dt, sqdt = sqrt(a.dt);
   for (int t = 0; t < a.T; ++t) {
     double z[kRovMaxU], zj[kRovMaxU], uj[kRovMaxU];
-    for (int b = 0; b < (U + 3) / 4; ++b) {
+#pragma unroll
+    for (int b = 0; b < kRovMaxU / 4; ++b) {
+      if (4 * b >= U) break;
       double w[4];
       rov_normals((uint32_t)p, (uint32_t)t, (uint32_t)b, a.tag, a.k0, a.k1, w);
-      for (int i = 0; i < 4 && 4 * b + i < U; ++i) z[4 * b + i] = w[i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) z[4 * b + i] = w[i];
       rov_normals((uint32_t)p, (uint32_t)t, (uint32_t)(2 + b), a.tag, a.k0, a.k1, w);
-      for (int i = 0; i < 4 && 4 * b + i < U; ++i) zj[4 * b + i] = w[i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) zj[4 * b + i] = w[i];
       const rmc_u4 r = rmc_philox((uint32_t)p, (uint32_t)t, (uint32_t)(4 + b), a.tag, a.k0, a.k1);
-      const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
-      for (int i = 0; i < 4 && 4 * b + i < U; ++i) uj[4 * b + i] = rmc_u01d(rw[i]);
+      uj[4 * b + 0] = rmc_u01d(r.x);
+      uj[4 * b + 1] = rmc_u01d(r.y);
+      uj[4 * b + 2] = rmc_u01d(r.z);
+      uj[4 * b + 3] = rmc_u01d(r.w);
     }
-    // Correlated diffusion normals: L z with L the Cholesky factor.
-    double zc[kRovMaxU];
-    for (int u = 0; u < U; ++u) {
-      double s = 0.0;
-      for (int v = 0; v <= u; ++v) s += a.chol[u * kRovMaxU + v] * z[v];
-      zc[u] = s;
-    }
-    for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int u = 0; u < kRovMaxU; ++u) {
+      if (u >= U) break;
+      // Correlated diffusion normal: row u of L z, L the Cholesky factor.
+      double zc = 0.0;
+#pragma unroll
+      for (int v = 0; v <= u; ++v) zc += a.chol[u * kRovMaxU + v] * z[v];
       const double* q = a.par + u * 7;  // kind, x0, mean, sd, reversion, J, p
       const int kind = (int)q[0];
       const double xv = x[u];
       double nx;
       if (kind == kSdeOu) {
-        nx = xv + q[4] * (q[2] - xv) * dt + q[3] * sqdt * zc[u];
+        nx = xv + q[4] * (q[2] - xv) * dt + q[3] * sqdt * zc;
       } else if (kind == kSdeGbm) {
-        nx = xv * exp((q[2] - 0.5 * q[3] * q[3]) * dt + q[3] * sqdt * zc[u]);
+        nx = xv * exp((q[2] - 0.5 * q[3] * q[3]) * dt + q[3] * sqdt * zc);
       } else {
         // SimulateGPU.cu:930-952: Brownian term x * sd * z, jump size
         // J' = J^2 z' - J^2 / 2 applied when the uniform is below p.
         const double jump = (uj[u] < q[6]) ? 1.0 : 0.0;
         const double jsz = zj[u] * q[5] * q[5] - q[5] * q[5] / 2.0;
-        nx = xv + q[4] * (q[2] - xv) * dt + xv * q[3] * zc[u] + (exp(jsz) - 1.0) * xv * jump;
+        nx = xv + q[4] * (q[2] - xv) * dt + xv * q[3] * zc + (exp(jsz) - 1.0) * xv * jump;
       }
       x[u] = nx;
       a.states[((size_t)(t + 1) * U + u) * a.N + p] = nx;
@@ -88,62 +99,83 @@ __global__ void rov_forward_kernel(RovForwardArgs a) {
   }
 }
 
-// Mean and 1/SD of every state dimension at every step (feature scaling).
-__global__ void rov_moments_kernel(const double* __restrict__ states, int N, int U,
-                                   double* __restrict__ scale) {
-  const int t = blockIdx.x;
-  __shared__ double s1[256], s2[256];
-  for (int u = 0; u < U; ++u) {
-    const double* x = states + ((size_t)t * U + u) * N;
-    double a = 0.0, b = 0.0;
-    for (int p = threadIdx.x; p < N; p += blockDim.x) {
-      a += x[p];
-      b += x[p] * x[p];
-    }
-    s1[threadIdx.x] = a;
-    s2[threadIdx.x] = b;
-    __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) {
-        s1[threadIdx.x] += s1[threadIdx.x + off];
-        s2[threadIdx.x] += s2[threadIdx.x + off];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const double m = s1[0] / N;
-      const double v = s2[0] / N - m * m;
-      const double sd = v > 1e-300 ? sqrt(v) : 1.0;
-      scale[((size_t)t * U + u) * 2 + 0] = m;
-      scale[((size_t)t * U + u) * 2 + 1] = (sd > 1e-12 * (fabs(m) + 1.0)) ? 1.0 / sd : 1.0;
+// Mean and 1/SD of every state dimension at every step (feature scaling):
+// one 1024-thread block per (step, dimension), fixed-order tree reduction.
+__global__ __launch_bounds__(1024) void rov_moments_kernel(const double* __restrict__ states,
+                                                           int N, int U,
+                                                           double* __restrict__ scale) {
+  const int tu = blockIdx.x;  // t * U + u
+  __shared__ double s1[1024], s2[1024];
+  const double* x = states + (size_t)tu * N;
+  double a = 0.0, b = 0.0;
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
+    const double v = x[p];
+    a += v;
+    b += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      s1[threadIdx.x] += s1[threadIdx.x + off];
+      s2[threadIdx.x] += s2[threadIdx.x + off];
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = s1[0] / N;
+    const double v = s2[0] / N - m * m;
+    const double sd = v > 1e-300 ? sqrt(v) : 1.0;
+    scale[(size_t)tu * 2 + 0] = m;
+    scale[(size_t)tu * 2 + 1] = (sd > 1e-12 * (fabs(m) + 1.0)) ? 1.0 / sd : 1.0;
   }
 }
 
 // All monomials of degree <= deg in U variables, in graded order:
-// 1, x_i, x_i x_j (i <= j), x_i x_j x_k (i <= j <= k).
-__device__ __forceinline__ int rov_basis(const double* x, int U, int deg, double* phi) {
+// 1, x_i, x_i x_j (i <= j), x_i x_j x_k (i <= j <= k); f(k, value) is called
+// for each in turn.  The loops run over the compile-time maximum with guards,
+// so x stays in registers.
+template <class F>
+__device__ __forceinline__ int rov_basis_visit(const double (&x)[kRovMaxU], int U, int deg,
+                                               F&& f) {
   int k = 0;
-  phi[k++] = 1.0;
-  if (deg >= 1)
-    for (int i = 0; i < U; ++i) phi[k++] = x[i];
-  if (deg >= 2)
-    for (int i = 0; i < U; ++i)
-      for (int j = i; j < U; ++j) phi[k++] = x[i] * x[j];
-  if (deg >= 3)
-    for (int i = 0; i < U; ++i)
-      for (int j = i; j < U; ++j)
-        for (int l = j; l < U; ++l) phi[k++] = x[i] * x[j] * x[l];
+  f(k++, 1.0);
+  if (deg >= 1) {
+#pragma unroll
+    for (int i = 0; i < kRovMaxU; ++i)
+      if (i < U) f(k++, x[i]);
+  }
+  if (deg >= 2) {
+#pragma unroll
+    for (int i = 0; i < kRovMaxU; ++i)
+#pragma unroll
+      for (int j = i; j < kRovMaxU; ++j)
+        if (j < U) f(k++, x[i] * x[j]);
+  }
+  if (deg >= 3) {
+#pragma unroll
+    for (int i = 0; i < kRovMaxU; ++i)
+#pragma unroll
+      for (int j = i; j < kRovMaxU; ++j)
+#pragma unroll
+        for (int l = j; l < kRovMaxU; ++l)
+          if (l < U) f(k++, x[i] * x[j] * x[l]);
+  }
   return k;
 }
 
-__device__ __forceinline__ void rov_features(const RovStepArgs& a, int t, int p, double* x) {
-  for (int u = 0; u < a.U; ++u) {
-    double v = a.states[((size_t)t * a.U + u) * a.N + p];
-    if (a.scale) {
-      const double* sc = a.scale + ((size_t)t * a.U + u) * 2;
-      v = (v - sc[0]) * sc[1];
+__device__ __forceinline__ void rov_features(const RovStepArgs& a, int t, int p,
+                                             double (&x)[kRovMaxU]) {
+#pragma unroll
+  for (int u = 0; u < kRovMaxU; ++u) {
+    double v = 0.0;
+    if (u < a.U) {
+      v = a.states[((size_t)t * a.U + u) * a.N + p];
+      if (a.scale) {
+        const double* sc = a.scale + ((size_t)t * a.U + u) * 2;
+        v = (v - sc[0]) * sc[1];
+      }
     }
     x[u] = v;
   }
@@ -171,170 +203,242 @@ __device__ __forceinline__ int rov_fit_group(const RovStepArgs& a, int t, int p)
 
 // Per-block partial sums of the normal equations of step t for every fit
 // group: A (upper triangle, row-major), b, and the point count.
-__global__ __launch_bounds__(256) void rov_assemble_kernel(RovStepArgs a, int t) {
+//
+// The block's 256 points form Phi = [phi_0 .. phi_{K-1}, y] (256 x KP, KP =
+// K + 1 rounded up to 16, zero padded) in LDS; for group g the sums are the
+// Gram matrix Phi_g^T Phi with Phi_g the rows of the group's points, built
+// from 16 x 16 tiles on v_mfma_f64_16x16x4_f64 (4 points per MFMA, points in
+// a fixed order).  Column K of the Gram matrix is b, entry (0, 0) the count.
+constexpr int kRovAsmPoints = 256;
+
+__host__ __device__ inline int rov_kp(int K) { return (K + 1 + 15) / 16 * 16; }
+
+__global__ __launch_bounds__(256) void rov_assemble_kernel(const RovStepArgs* __restrict__ pa,
+                                                           int t) {
+  typedef double doublex4 __attribute__((ext_vector_type(4)));
+  const RovStepArgs& a = *pa;
   extern __shared__ double sm[];
-  const int K = a.K, E = a.E, B = blockDim.x;
-  double* sphi = sm;               // [K][B]
-  double* sy = sphi + (size_t)K * B;
-  int* sg = reinterpret_cast<int*>(sy + B);
-  const int p = blockIdx.x * B + threadIdx.x;
+  const int K = a.K, E = a.E, KP = rov_kp(K), LD = KP + 1;  // +1: rotate banks per row
+  double* phi = sm;  // [256][LD]
+  int* sg = reinterpret_cast<int*>(phi + (size_t)kRovAsmPoints * LD);
+  const int q = threadIdx.x;
+  const int p = blockIdx.x * kRovAsmPoints + q;
+  double* row = phi + (size_t)q * LD;
   int g = -1;
+  for (int k = 0; k < KP; ++k) row[k] = 0.0;
   if (p < a.N) {
     g = rov_fit_group(a, t, p);
-    double x[kRovMaxU], phi[kRovMaxK];
+    double x[kRovMaxU];
     rov_features(a, t, p, x);
-    rov_basis(x, a.U, a.deg, phi);
-    for (int k = 0; k < K; ++k) sphi[(size_t)k * B + threadIdx.x] = phi[k];
-    sy[threadIdx.x] = a.V[(size_t)(t + 1) * a.N + p];
-  } else {
-    for (int k = 0; k < K; ++k) sphi[(size_t)k * B + threadIdx.x] = 0.0;
-    sy[threadIdx.x] = 0.0;
+    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { row[k] = v; });
+    row[K] = a.V[(size_t)(t + 1) * a.N + p];
   }
-  sg[threadIdx.x] = g;
+  sg[q] = g;
   __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nt = KP / 16, ntile = nt * (nt + 1) / 2, njobs = a.G * ntile;
   const int n_tri = K * (K + 1) / 2;
-  for (int e = threadIdx.x; e < a.G * E; e += B) {
-    const int grp = e / E, ee = e - grp * E;
-    double acc = 0.0;
-    if (ee < n_tri) {
-      // (i, j), i <= j, of the upper triangle in row-major order.
-      int i = 0, rem = ee;
-      while (rem >= K - i) {
-        rem -= K - i;
-        ++i;
-      }
-      const int j = i + rem;
-      const double* pi = sphi + (size_t)i * B;
-      const double* pj = sphi + (size_t)j * B;
-      for (int q = 0; q < B; ++q)
-        if (sg[q] == grp) acc += pi[q] * pj[q];
-    } else if (ee < n_tri + K) {
-      const double* pi = sphi + (size_t)(ee - n_tri) * B;
-      for (int q = 0; q < B; ++q)
-        if (sg[q] == grp) acc += pi[q] * sy[q];
-    } else {
-      for (int q = 0; q < B; ++q)
-        if (sg[q] == grp) acc += 1.0;
+  const int kq = lane >> 4, li = lane & 15;
+  for (int job = wave; job < njobs; job += 4) {
+    const int grp = job / ntile;
+    int I = 0, rem = job - grp * ntile;
+    while (rem >= nt - I) {
+      rem -= nt - I;
+      ++I;
     }
-    a.partial[((size_t)blockIdx.x * a.G + grp) * E + ee] = acc;
+    const int J = I + rem;
+    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < kRovAsmPoints; q0 += 4) {
+      const double* r = phi + (size_t)(q0 + kq) * LD;
+      const double av = sg[q0 + kq] == grp ? r[I * 16 + li] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, r[J * 16 + li], acc, 0, 0, 0);
+    }
+    // Item r of lane l is Gram entry (I*16 + l/16 + 4r, J*16 + l%16).
+    const int j = J * 16 + li;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int i = I * 16 + kq + 4 * rr;
+      int e = -1;
+      if (i < K && j < K && i <= j)
+        e = i * K - i * (i - 1) / 2 + (j - i);
+      else if (i < K && j == K)
+        e = n_tri + i;
+      double* dst = a.partial + (size_t)grp * E * a.nblk + blockIdx.x;
+      if (e >= 0) dst[(size_t)e * a.nblk] = acc[rr];
+      if (i == 0 && j == 0) dst[(size_t)(n_tri + K) * a.nblk] = acc[rr];
+    }
   }
 }
 
-// Reduces the block partials in a fixed order and solves each group's K x K
-// system in FP64: Cholesky, or LU with partial pivoting when the matrix is not
-// numerically positive definite.  Groups with fewer points than K keep zero
+// sums[e] = sum over blocks of partial[e][.], one wave per entry: lane l adds
+// blocks l, l + 64, ... in order, then a fixed butterfly across lanes, so
+// the totals do not depend on timing.
+__global__ __launch_bounds__(64) void rov_reduce_kernel(const RovStepArgs* __restrict__ pa) {
+  const RovStepArgs& a = *pa;
+  const int e = blockIdx.x, lane = threadIdx.x;
+  const double* src = a.partial + (size_t)e * a.nblk;
+  double s = 0.0;
+  for (int b = lane; b < a.nblk; b += 64) s += src[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) a.sums[e] = s;
+}
+
+// Value of v in lane `src` (a compile-time constant in the unrolled loops
+// that use it) broadcast to every lane.
+__device__ __forceinline__ double bcast_lane(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Solves each group's K x K normal equations in FP64, one wave per group:
+// Cholesky with the rows in registers (lane i owns row i; K is a template
+// parameter, one instance per basis size the problem validation admits, so
+// the loops are straight-line code), or, when the matrix is not numerically
+// positive definite, Gaussian elimination with partial pivoting on
+// |M[i][col]| (the pivot search the reference intends,
+// SimulateGPU.cu:6307-6316).  Groups with fewer points than K keep zero
 // coefficients (multiQuadReg skips them, SimulateGPU.cu:6101).
-__global__ __launch_bounds__(256) void rov_solve_kernel(RovStepArgs a, int t, int nblk) {
+template <int K>
+__global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __restrict__ pa, int t) {
+  const RovStepArgs& a = *pa;
   extern __shared__ double sm[];
-  const int K = a.K, E = a.E, G = a.G;
-  double* sums = sm;  // [G][E]
-  for (int e = threadIdx.x; e < G * E; e += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += a.partial[(size_t)b * G * E + e];
-    sums[e] = s;
+  const int E = a.E, grp = blockIdx.x, lane = threadIdx.x;
+  double* M = sm;            // [K][K]
+  double* L = M + K * K;     // [K][K]
+  double* r = L + K * K;     // [K]
+  double* y = r + K;         // [K]
+  const double* S = a.sums + (size_t)grp * E;
+  const int n_tri = K * (K + 1) / 2;
+  for (int e = lane; e < n_tri; e += 64) {
+    int i = 0, rem = e;
+    while (rem >= K - i) {
+      rem -= K - i;
+      ++i;
+    }
+    const int j = i + rem;
+    M[i * K + j] = S[e];
+    M[j * K + i] = S[e];
   }
+  for (int i = lane; i < K; i += 64) r[i] = S[n_tri + i];
   __syncthreads();
-  if (threadIdx.x < G) {
-    const int grp = threadIdx.x;
-    double* coef = a.beta + ((size_t)t * G + grp) * K;
-    const double* S = sums + (size_t)grp * E;
-    const int n_tri = K * (K + 1) / 2;
-    double M[kRovMaxK * kRovMaxK], r[kRovMaxK];
-    for (int i = 0, e = 0; i < K; ++i)
-      for (int j = i; j < K; ++j, ++e) {
-        M[i * K + j] = S[e];
-        M[j * K + i] = S[e];
-      }
-    for (int i = 0; i < K; ++i) r[i] = S[n_tri + i];
-    const double count = S[n_tri + K];
-    int status = 0;
-    if (count < (double)K) {
-      for (int i = 0; i < K; ++i) coef[i] = 0.0;
-      status = 1;
-    } else {
-      // Cholesky M = L L^T in place (lower triangle).
-      double L[kRovMaxK * kRovMaxK];
-      bool spd = true;
-      double dmax = 0.0;
-      for (int i = 0; i < K; ++i) dmax = fmax(dmax, fabs(M[i * K + i]));
-      for (int j = 0; j < K && spd; ++j) {
-        double d = M[j * K + j];
-        for (int k = 0; k < j; ++k) d -= L[j * K + k] * L[j * K + k];
-        if (!(d > 1e-13 * dmax)) {
-          spd = false;
-          break;
-        }
-        const double ljj = sqrt(d);
-        L[j * K + j] = ljj;
-        for (int i = j + 1; i < K; ++i) {
-          double s = M[i * K + j];
-          for (int k = 0; k < j; ++k) s -= L[i * K + k] * L[j * K + k];
-          L[i * K + j] = s / ljj;
-        }
-      }
-      if (spd) {
-        double y[kRovMaxK];
-        for (int i = 0; i < K; ++i) {
-          double s = r[i];
-          for (int k = 0; k < i; ++k) s -= L[i * K + k] * y[k];
-          y[i] = s / L[i * K + i];
-        }
-        for (int i = K - 1; i >= 0; --i) {
-          double s = y[i];
-          for (int k = i + 1; k < K; ++k) s -= L[k * K + i] * coef[k];
-          coef[i] = s / L[i * K + i];
-        }
+  double* coef = a.beta + ((size_t)t * a.G + grp) * K;
+  const double count = S[n_tri + K];
+  if (count < (double)K) {
+    for (int i = lane; i < K; i += 64) coef[i] = 0.0;
+    if (lane == 0) a.status[(size_t)t * a.G + grp] = 1;
+    return;
+  }
+  double dmax = 0.0;
+  for (int i = 0; i < K; ++i) dmax = fmax(dmax, fabs(M[i * K + i]));
+  // Cholesky in registers: lane i holds row i of M (m) and of L (l).  Every
+  // loop is unrolled over the compile-time bound, so the rows stay in
+  // registers and each cross-lane value is a v_readlane broadcast from a
+  // constant lane (no LDS round trip on the critical path).
+  double m[K], l[K], invd[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m[k] = lane < K ? M[lane * K + k] : 0.0;
+    l[k] = 0.0;
+    invd[k] = 0.0;
+  }
+  bool spd = true;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (spd) {
+      const double djj = bcast_lane(m[j], j);
+      if (djj > 1e-13 * dmax) {
+        const double inv = 1.0 / sqrt(djj);
+        invd[j] = inv;
+        const double lij = m[j] * inv;  // L[lane][j] (lane j: L[j][j])
+        l[j] = lij;
+#pragma unroll
+        for (int k = j + 1; k < K; ++k)
+          m[k] = fma(-lij, bcast_lane(lij, k), m[k]);
       } else {
-        // Gaussian elimination with partial pivoting on |M[i][col]| (the
-        // pivot search the reference intends, SimulateGPU.cu:6307-6316).
-        int piv_ok = 1;
-        for (int col = 0; col < K; ++col) {
-          int best = col;
-          double bv = fabs(M[col * K + col]);
-          for (int i = col + 1; i < K; ++i)
-            if (fabs(M[i * K + col]) > bv) {
-              bv = fabs(M[i * K + col]);
-              best = i;
-            }
-          if (!(bv > 1e-300)) {
-            piv_ok = 0;
-            break;
-          }
-          if (best != col) {
-            for (int j = 0; j < K; ++j) {
-              const double tmp = M[col * K + j];
-              M[col * K + j] = M[best * K + j];
-              M[best * K + j] = tmp;
-            }
-            const double tr = r[col];
-            r[col] = r[best];
-            r[best] = tr;
-          }
-          for (int i = col + 1; i < K; ++i) {
-            const double f = M[i * K + col] / M[col * K + col];
-            for (int j = col; j < K; ++j) M[i * K + j] -= f * M[col * K + j];
-            r[i] -= f * r[col];
-          }
-        }
-        if (piv_ok) {
-          for (int i = K - 1; i >= 0; --i) {
-            double s = r[i];
-            for (int j = i + 1; j < K; ++j) s -= M[i * K + j] * coef[j];
-            coef[i] = s / M[i * K + i];
-          }
-          status = 2;
-        } else {
-          for (int i = 0; i < K; ++i) coef[i] = 0.0;
-          status = 3;
-        }
+        spd = false;
       }
     }
-    a.status[(size_t)t * G + grp] = status;
+  }
+  if (spd) {
+    // L y = r: y_j from lane j, removed from the lanes below (1/L_jj = invd).
+    double rv = lane < K ? r[lane] : 0.0, yv = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const double yj = bcast_lane(rv, j) * invd[j];
+      if (lane == j) yv = yj;
+      if (lane > j) rv = fma(-l[j], yj, rv);
+    }
+    // L^T coef = y: coef_j from lane j, removed from the lanes above; lane i
+    // needs L[j][i], a column of L: stage the rows of L in LDS.
+    if (lane < K) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) L[lane * K + k] = l[k];
+    }
+    __syncthreads();
+    double cv = 0.0;
+#pragma unroll
+    for (int j = K - 1; j >= 0; --j) {
+      const double cj = bcast_lane(yv, j) * invd[j];
+      if (lane == j) cv = cj;
+      if (lane < j) yv = fma(-L[j * K + lane], cj, yv);
+    }
+    if (lane < K) coef[lane] = cv;
+    if (lane == 0) a.status[(size_t)t * a.G + grp] = 0;
+    return;
+  }
+  if (lane != 0) return;
+  // Rare fallback: pivoted elimination by one lane.
+  int piv_ok = 1;
+  for (int col = 0; col < K; ++col) {
+    int best = col;
+    double bv = fabs(M[col * K + col]);
+    for (int i = col + 1; i < K; ++i)
+      if (fabs(M[i * K + col]) > bv) {
+        bv = fabs(M[i * K + col]);
+        best = i;
+      }
+    if (!(bv > 1e-300)) {
+      piv_ok = 0;
+      break;
+    }
+    if (best != col) {
+      for (int j = 0; j < K; ++j) {
+        const double tmp = M[col * K + j];
+        M[col * K + j] = M[best * K + j];
+        M[best * K + j] = tmp;
+      }
+      const double tr = r[col];
+      r[col] = r[best];
+      r[best] = tr;
+    }
+    for (int i = col + 1; i < K; ++i) {
+      const double f = M[i * K + col] / M[col * K + col];
+      for (int j = col; j < K; ++j) M[i * K + j] -= f * M[col * K + j];
+      r[i] -= f * r[col];
+    }
+  }
+  if (piv_ok) {
+    for (int i = K - 1; i >= 0; --i) {
+      double s = r[i];
+      for (int j = i + 1; j < K; ++j) s -= M[i * K + j] * y[j];
+      y[i] = s / M[i * K + i];
+    }
+    for (int i = 0; i < K; ++i) coef[i] = y[i];
+    a.status[(size_t)t * a.G + grp] = 2;
+  } else {
+    for (int i = 0; i < K; ++i) coef[i] = 0.0;
+    a.status[(size_t)t * a.G + grp] = 3;
   }
 }
 
 // Decision of step t for every path.
-__global__ void rov_decide_kernel(RovStepArgs a, int t, int terminal) {
+__global__ __launch_bounds__(256) void rov_decide_kernel(const RovStepArgs* __restrict__ pa, int t,
+                                                         int terminal) {
+  const RovStepArgs& a = *pa;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.N) return;
   const double disc = a.disc;
@@ -349,12 +453,11 @@ __global__ void rov_decide_kernel(RovStepArgs a, int t, int terminal) {
     const double later = disc * V[(size_t)(t + 1) * a.N + p];
     int ex = 0;
     if (e > 0.0 || !a.itm_only) {
-      double x[kRovMaxU], phi[kRovMaxK];
+      double x[kRovMaxU];
       rov_features(a, t, p, x);
-      const int K = rov_basis(x, a.U, a.deg, phi);
-      const double* b = a.beta + (size_t)t * a.G * K;
+      const double* b = a.beta + (size_t)t * a.G * a.K;
       double cont = 0.0;
-      for (int k = 0; k < K; ++k) cont += phi[k] * b[k];
+      rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { cont = fma(v, b[k], cont); });
       cont *= disc;
       ex = e > cont ? 1 : 0;
     }
@@ -371,75 +474,101 @@ __global__ void rov_decide_kernel(RovStepArgs a, int t, int terminal) {
   // Flow control (backwardInduction, SimulateGPU.cu:2333-2784).
   const double up = rov_unit_profit(a, t, p);
   const double m = rov_flow_scale(a, t, p);
-  double pay[kRovMaxC], cur[kRovMaxC];
-  double x[kRovMaxU], phi[kRovMaxK];
-  int K = 0;
+  // Continuation value of every fit group, one pass over the basis.
+  double cont[kRovMaxC];
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) cont[c] = 0.0;
   if (!terminal) {
+    double x[kRovMaxU];
     rov_features(a, t, p, x);
-    K = rov_basis(x, a.U, a.deg, phi);
+    const double* b = a.beta + (size_t)t * a.G * a.K;
+    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) {
+#pragma unroll
+      for (int c = 0; c < kRovMaxC; ++c)
+        if (c < a.G) cont[c] = fma(v, b[(size_t)c * a.K + k], cont[c]);
+    });
   }
-  for (int c = 0; c < a.C; ++c) {
-    cur[c] = a.flow[c] * m * up;
+  double best = 0.0, best_cur = 0.0, best_pay = 0.0;
+  int bc = 0;
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) {
+    if (c >= a.C) break;
+    const double cur = a.flow[c] * m * up;
+    double pay;
     if (terminal) {
       // Terminal step: payoff clamped at zero (SimulateGPU.cu:2385-2416).
-      pay[c] = fmin(cur[c], 0.0);
+      pay = fmin(cur, 0.0);
     } else {
-      const double* b = a.beta + ((size_t)t * a.G + (a.pooled ? 0 : c)) * K;
-      double cont = 0.0;
-      for (int k = 0; k < K; ++k) cont += phi[k] * b[k];
-      cont *= disc;                 // SimulateGPU.cu:2703
-      if (cont > 0.0) cont = 0.0;   // SimulateGPU.cu:2712-2714
-      pay[c] = cur[c] + cont;
+      double cv = cont[a.pooled ? 0 : c] * disc;  // SimulateGPU.cu:2703
+      if (cv > 0.0) cv = 0.0;                      // SimulateGPU.cu:2712-2714
+      pay = cur + cv;
     }
-  }
-  double best = pay[0];
-  int bc = 0;
-  for (int c = 1; c < a.C; ++c) {
-    if (pay[c] < a.hysteresis * best) {  // SimulateGPU.cu:2757-2771
-      best = pay[c];
+    if (c == 0 || pay < a.hysteresis * best) {  // SimulateGPU.cu:2757-2771
+      best = pay;
       bc = c;
+      best_cur = cur;
+      best_pay = pay;
     }
   }
   double v = best;
   if (a.value_mode == kValueRealized)
-    v = terminal ? pay[bc] : cur[bc] + disc * V[(size_t)(t + 1) * a.N + p];
+    v = terminal ? best_pay : best_cur + disc * V[(size_t)(t + 1) * a.N + p];
   V[(size_t)t * a.N + p] = v;
   a.optc[(size_t)t * a.N + p] = bc;
 }
 
 // Time 0 (firstPeriodInduction, SimulateGPU.cu:2210-2330): per initial
 // control, the mean discounted value of the paths that took it, plus the
-// deterministic current payoff; out = {value, control, SD}.
-__global__ void rov_first_kernel(RovStepArgs a, double* out) {
-  __shared__ double s1[kRovMaxC][256], s2[kRovMaxC][256], sn[kRovMaxC][256];
+// deterministic current payoff; out = {value, control, SD}.  One 1024-thread
+// block, one pass over the paths, fixed-order tree reduction.
+__global__ __launch_bounds__(1024) void rov_first_kernel(const RovStepArgs* __restrict__ pa) {
+  const RovStepArgs& a = *pa;
+  double* out = a.out;
+  __shared__ double s1[kRovMaxC][1024 / 64], s2[kRovMaxC][1024 / 64], sn[kRovMaxC][1024 / 64];
   const int G = (a.mode == kRovAmerican || a.pooled) ? 1 : a.C;
-  for (int c = 0; c < G; ++c) {
-    double x1 = 0.0, x2 = 0.0, n = 0.0;
-    for (int p = threadIdx.x; p < a.N; p += blockDim.x) {
-      const int g = (G == 1) ? 0 : a.controls[p];
-      if (g != c) continue;
-      const double v = a.disc * a.V[(size_t)a.N + p];
-      x1 += v;
-      x2 += v * v;
-      n += 1.0;
+  double x1[kRovMaxC], x2[kRovMaxC], n[kRovMaxC];
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) x1[c] = x2[c] = n[c] = 0.0;
+  for (int p = threadIdx.x; p < a.N; p += blockDim.x) {
+    const int g = (G == 1) ? 0 : a.controls[p];
+    const double v = a.disc * a.V[(size_t)a.N + p];
+#pragma unroll
+    for (int c = 0; c < kRovMaxC; ++c) {
+      const bool in = g == c;
+      x1[c] += in ? v : 0.0;
+      x2[c] += in ? v * v : 0.0;
+      n[c] += in ? 1.0 : 0.0;
     }
-    s1[c][threadIdx.x] = x1;
-    s2[c][threadIdx.x] = x2;
-    sn[c][threadIdx.x] = n;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      x1[c] += __shfl_xor(x1[c], off, 64);
+      x2[c] += __shfl_xor(x2[c], off, 64);
+      n[c] += __shfl_xor(n[c], off, 64);
+    }
+    if (lane == 0) {
+      s1[c][wave] = x1[c];
+      s2[c][wave] = x2[c];
+      sn[c][wave] = n[c];
+    }
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  const int nw = blockDim.x / 64;
   double mean[kRovMaxC], sd[kRovMaxC], cnt[kRovMaxC];
   for (int c = 0; c < G; ++c) {
-    double x1 = 0.0, x2 = 0.0, n = 0.0;
-    for (int i = 0; i < (int)blockDim.x; ++i) {
-      x1 += s1[c][i];
-      x2 += s2[c][i];
-      n += sn[c][i];
+    double y1 = 0.0, y2 = 0.0, m = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      y1 += s1[c][w];
+      y2 += s2[c][w];
+      m += sn[c][w];
     }
-    cnt[c] = n;
-    mean[c] = n > 0.0 ? x1 / n : 0.0;
-    sd[c] = n > 0.0 ? sqrt(fmax(x2 / n - mean[c] * mean[c], 0.0)) : 0.0;
+    cnt[c] = m;
+    mean[c] = m > 0.0 ? y1 / m : 0.0;
+    sd[c] = m > 0.0 ? sqrt(fmax(y2 / m - mean[c] * mean[c], 0.0)) : 0.0;
   }
   if (a.mode == kRovAmerican) {
     out[0] = mean[0];
@@ -495,9 +624,7 @@ hipError_t rov_set_lds_limits() {
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)rov_assemble_lds(kRovMaxK));
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rov_solve_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             kRovMaxC * (kRovMaxK * (kRovMaxK + 1) / 2 + kRovMaxK + 1) * 8);
+  return hipSuccess;
 }
 
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s) {
@@ -508,31 +635,47 @@ hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s) {
 
 hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double* scale,
                               hipStream_t s) {
-  hipLaunchKernelGGL(rov_moments_kernel, dim3(T1), dim3(256), 0, s, states, N, U, scale);
+  hipLaunchKernelGGL(rov_moments_kernel, dim3(T1 * U), dim3(1024), 0, s, states, N, U, scale);
   return hipGetLastError();
 }
 
-size_t rov_assemble_lds(int K) { return (size_t)K * 256 * 8 + 256 * 8 + 256 * 4; }
+size_t rov_assemble_lds(int K) {
+  return (size_t)kRovAsmPoints * (rov_kp(K) + 1) * sizeof(double) + kRovAsmPoints * sizeof(int);
+}
 
-hipError_t launch_rov_step(const RovStepArgs& a, int t, bool fit, bool terminal, int nblk,
-                           hipStream_t s) {
+hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, bool fit,
+                           bool terminal, hipStream_t s) {
   if (fit) {
-    hipLaunchKernelGGL(rov_assemble_kernel, dim3(nblk), dim3(256), rov_assemble_lds(a.K), s, a,
-                       t);
+    hipLaunchKernelGGL(rov_assemble_kernel, dim3(a.nblk), dim3(256), rov_assemble_lds(a.K), s,
+                       pa, t);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(rov_solve_kernel, dim3(1), dim3(256), (size_t)a.G * a.E * 8, s, a, t,
-                       nblk);
+    hipLaunchKernelGGL(rov_reduce_kernel, dim3(a.G * a.E), dim3(64), 0, s, pa);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t lds = (size_t)(2 * a.K * a.K + 2 * a.K) * sizeof(double);
+    switch (a.K) {
+#define ROV_SOLVE_CASE(KK)                                                                  \
+  case KK:                                                                                  \
+    hipLaunchKernelGGL(rov_solve_kernel<KK>, dim3(a.G), dim3(64), lds, s, pa, t);           \
+    break;
+      ROV_SOLVE_CASE(2) ROV_SOLVE_CASE(3) ROV_SOLVE_CASE(4) ROV_SOLVE_CASE(5) ROV_SOLVE_CASE(6)
+      ROV_SOLVE_CASE(7) ROV_SOLVE_CASE(8) ROV_SOLVE_CASE(9) ROV_SOLVE_CASE(10) ROV_SOLVE_CASE(15)
+      ROV_SOLVE_CASE(20) ROV_SOLVE_CASE(21) ROV_SOLVE_CASE(28) ROV_SOLVE_CASE(35)
+#undef ROV_SOLVE_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(rov_decide_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a, t,
+  hipLaunchKernelGGL(rov_decide_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, pa, t,
                      terminal ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t launch_rov_first(const RovStepArgs& a, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(rov_first_kernel, dim3(1), dim3(256), 0, s, a, out);
+hipError_t launch_rov_first(const RovStepArgs* pa, hipStream_t s) {
+  hipLaunchKernelGGL(rov_first_kernel, dim3(1), dim3(1024), 0, s, pa);
   return hipGetLastError();
 }
 

@@ -13,6 +13,10 @@ batched launch of 8 roads that all have P patches (2 species, 50 years,
 20,000 paths), the per-size workload used for counter profiles of the
 persistent MTE kernel.
 
+``python transportoptimiser_amd/profiling.py rov [--repeat R]`` runs the ROV
+valuation of 3 uncertainties (2 OU + 1 GBM), 2 controls, 50 steps, 100,000
+paths and a cubic basis R times, printing the host time of each call.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -59,6 +63,22 @@ def uniform(P: int, n_roads: int = 8) -> None:
         ms = b.kernel_ms()
     ctx.close()
     print(f"P={P}: {n_roads} roads, kernel {ms:.1f} ms, {flops / ms / 1e9:.1f} TFLOP/s", flush=True)
+
+
+def rov(repeat: int) -> None:
+    import os
+    import time
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd.api import Context
+    from transportoptimiser_amd.rov import config5, simulate_rov
+    ctx = Context(0)
+    pr = config5()
+    for i in range(repeat):
+        t = time.perf_counter()
+        r = simulate_rov(ctx, pr)
+        print(f"rov call {i}: {(time.perf_counter() - t) * 1e3:.2f} ms, value {r['value']:.4f}",
+              flush=True)
+    ctx.close()
 
 
 def summarize(paths: list[str]) -> None:
@@ -135,6 +155,8 @@ def main() -> int:
     c.add_argument("--bytes", type=int, default=1 << 30)
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
+    rv = sub.add_parser("rov")
+    rv.add_argument("--repeat", type=int, default=5)
     u = sub.add_parser("uniform")
     u.add_argument("--patches", type=int, required=True)
     u.add_argument("--roads", type=int, default=8)
@@ -153,6 +175,8 @@ def main() -> int:
     a = ap.parse_args()
     if a.cmd == "copy":
         copy(a.bytes)
+    elif a.cmd == "rov":
+        rov(a.repeat)
     elif a.cmd == "uniform":
         uniform(a.patches, a.roads)
     elif a.cmd == "dbstats":
