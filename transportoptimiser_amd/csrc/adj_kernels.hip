@@ -22,17 +22,46 @@
 
 namespace roadmc {
 
+// Lognormal year factors e = exp(s z - s^2/2) of the four paths of quad q
+// (paths 4q .. 4q+3) in patch `col`: one Philox call, as in the forward pass.
 template <class TR>
-__device__ __forceinline__ void adj_noise(const AdjJob& jb, uint32_t path, int year, int col,
-                                          typename TR::T* e, typename TR::T sig,
-                                          typename TR::T half_var) {
+__device__ __forceinline__ void adj_noise4(const AdjJob& jb, uint32_t quad, int year, int col,
+                                           typename TR::T e[4], typename TR::T sig,
+                                           typename TR::T half_var) {
   using T = typename TR::T;
   T z[4];
-  TR::normal4(path >> 2, (uint32_t)year, (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
-  if constexpr (sizeof(T) == 4)
-    *e = __builtin_amdgcn_exp2f(fmaf(sig, z[path & 3u], -half_var));
-  else
-    *e = exp(sig * z[path & 3u] - half_var);
+  TR::normal4(quad, (uint32_t)year, (uint32_t)col, jb.tag0, jb.key0, jb.key1, z);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if constexpr (sizeof(T) == 4)
+      e[s] = __builtin_amdgcn_exp2f(fmaf(sig, z[s], -half_var));
+    else
+      e[s] = exp(sig * z[s] - half_var);
+  }
+}
+
+// Row r of an accumulator and the quad layout: with kQuadItems (FP32), items
+// 4q .. 4q+3 of a lane are 4 consecutive paths; otherwise one Philox call per
+// item, whose element path & 3 is used.
+template <class TR>
+__device__ __forceinline__ void adj_noise_items(const AdjJob& jb, int p0, int trow, int lane,
+                                                int r4, int year, int col,
+                                                typename TR::T e[4], typename TR::T sig,
+                                                typename TR::T half_var) {
+  using T = typename TR::T;
+  constexpr int TM = TR::TM;
+  if constexpr (TR::kQuadItems) {
+    const int row0 = trow * TM + TR::acc_row(lane, r4);
+    adj_noise4<TR>(jb, (uint32_t)(p0 + row0) >> 2, year, col, e, sig, half_var);
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const uint32_t path = (uint32_t)(p0 + trow * TM + TR::acc_row(lane, r4 + s));
+      T ee[4];
+      adj_noise4<TR>(jb, path >> 2, year, col, ee, sig, half_var);
+      e[s] = ee[path & 3u];
+    }
+  }
 }
 
 template <class TR>
@@ -132,48 +161,77 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     }
     product(Mp);
     __syncthreads();
+    {
+      // Opaque copies: the epilogue's addresses are formed here, after the
+      // product, instead of being hoisted into registers live across it.
+      int lanee = lane, p0e = p0;
+      const T* colse = cols;
+      T* npe = np;
+      asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe));
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      if (t < my_n) {
-        const int col = tcol[t] * TN + TR::acc_col(lane);
-        const T Ki = capKinv[col], rr = rate[col];
+      for (int t = 0; t < NT; ++t) {
+        if (t < my_n) {
+          const int col = tcol[t] * TN + TR::acc_col(lanee);
+          const T Ki = colse[Pp + col], rr = colse[2 * Pp + col];
 #pragma unroll
-        for (int r = 0; r < NACC; ++r) {
-          const int row = trow[t] * TM + TR::acc_row(lane, r);
-          const uint32_t path = (uint32_t)(p0 + row);
-          T e;
-          adj_noise<TR>(jb, path, yr, col, &e, sig, half_var);
-          const T x = acc[t][r];
-          if ((int)path < N) np[i * plane + (size_t)path * Pp + col] = x;
-          A[row * LDA + TR::perm(col)] = x * ((T)1 + rr * ((T)1 - x * Ki)) * e;
+          for (int r4 = 0; r4 < NACC; r4 += 4) {
+            T e[4];
+            adj_noise_items<TR>(jb, p0e, trow[t], lanee, r4, yr, col, e, sig, half_var);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+              const int row = trow[t] * TM + TR::acc_row(lanee, r4 + s);
+              const uint32_t path = (uint32_t)(p0e + row);
+              const T x = acc[t][r4 + s];
+              if ((int)path < N) npe[i * plane + (size_t)path * Pp + col] = x;
+              A[row * LDA + TR::perm(col)] = x * ((T)1 + rr * ((T)1 - x * Ki)) * e[s];
+            }
+          }
         }
       }
     }
     __syncthreads();
   }
 
-  // Top of the segment: mu_{y1+1} from lambda_{y1+1}; one column per thread.
+  // Top of the segment: mu_{y1+1} from lambda_{y1+1}.  Thread w handles
+  // column w % Pp over row group w / Pp (S groups of BM / S rows, S a power
+  // of two that fits the threads and the grb scratch and keeps every group a
+  // whole number of path quads); the groups' growth-rate sums are then added
+  // in group order.
   {
     const int i = jb.y1 - jb.y0;
     const bool last = (jb.y1 == jb.T - 1);
-    for (int c = tid; c < Pp; c += nthreads) {
+    int S = 1;
+    while (S * 2 <= 8 && S * 2 * Pp <= nthreads && S * 2 <= nrb && BM % (S * 2 * 4) == 0) S *= 2;
+    const int rows = BM / S;  // a multiple of 4
+    for (int w = tid; w < Pp * S; w += nthreads) {
+      const int c = w % Pp, grp = w / Pp;
       const T Ki = capKinv[c], rr = rate[c];
       double gsum = 0.0;
-      for (int r = 0; r < BM; ++r) {
-        const uint32_t path = (uint32_t)(p0 + r);
-        T m = (T)0;
-        if ((int)path < N && c < jb.P) {
-          const T l = last ? (T)(jb.dobj / N) : lam[(size_t)path * Pp + c];
-          const T x = np[i * plane + (size_t)path * Pp + c];
-          T e;
-          adj_noise<TR>(jb, path, jb.y1, c, &e, sig, half_var);
-          m = l * ((T)1 + rr * ((T)1 - (T)2 * x * Ki)) * e;
-          gsum += (double)(l * x * ((T)1 - x * Ki) * e);
+      for (int r0 = grp * rows; r0 < (grp + 1) * rows; r0 += 4) {  // p0 is a multiple of 4
+        T e[4];
+        adj_noise4<TR>(jb, (uint32_t)(p0 + r0) >> 2, jb.y1, c, e, sig, half_var);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int r = r0 + s;
+          const uint32_t path = (uint32_t)(p0 + r);
+          T m = (T)0;
+          if ((int)path < N && c < jb.P) {
+            const T l = last ? (T)(jb.dobj / N) : lam[(size_t)path * Pp + c];
+            const T x = np[i * plane + (size_t)path * Pp + c];
+            m = l * ((T)1 + rr * ((T)1 - (T)2 * x * Ki)) * e[s];
+            gsum += (double)(l * x * ((T)1 - x * Ki) * e[s]);
+          }
+          if ((int)path < N) mu[i * plane + (size_t)path * Pp + c] = m;
+          A[r * LDA + TR::perm(c)] = m;
         }
-        if ((int)path < N) mu[i * plane + (size_t)path * Pp + c] = m;
-        A[r * LDA + TR::perm(c)] = m;
       }
-      gr[c] = gsum;
+      grb[grp * Pp + c] = gsum;
+    }
+    __syncthreads();
+    for (int c = tid; c < Pp; c += nthreads) {
+      double v = 0.0;
+      for (int g = 0; g < S; ++g) v += grb[g * Pp + c];
+      gr[c] = v;
     }
     __syncthreads();
   }
@@ -183,30 +241,40 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     product(Mtp);
     __syncthreads();
     const int i = yr - 1 - jb.y0;  // staging slot of n'_yr and mu_yr
+    int lanee = lane, p0e = p0;
+    const T* colse = cols;
+    T* npe = np;
+    T* mue = mu;
+    T* lame = lam;
+    asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe), "+s"(mue), "+s"(lame));
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (t < my_n) {
-        const int col = tcol[t] * TN + TR::acc_col(lane);
-        const T Ki = capKinv[col], rr = rate[col];
+        const int col = tcol[t] * TN + TR::acc_col(lanee);
+        const T Ki = colse[Pp + col], rr = colse[2 * Pp + col];
         double gsum = 0.0;
 #pragma unroll
-        for (int r = 0; r < NACC; ++r) {
-          const int row = trow[t] * TM + TR::acc_row(lane, r);
-          const uint32_t path = (uint32_t)(p0 + row);
-          const T l = acc[t][r];
-          if (yr == jb.y0) {
-            if ((int)path < N) lam[(size_t)path * Pp + col] = l;
-          } else {
-            T m = (T)0;
-            if ((int)path < N && col < jb.P) {
-              const T x = np[i * plane + (size_t)path * Pp + col];
-              T e;
-              adj_noise<TR>(jb, path, yr - 1, col, &e, sig, half_var);
-              m = l * ((T)1 + rr * ((T)1 - (T)2 * x * Ki)) * e;
-              gsum += (double)(l * x * ((T)1 - x * Ki) * e);
+        for (int r4 = 0; r4 < NACC; r4 += 4) {
+          T e[4] = {(T)0, (T)0, (T)0, (T)0};
+          if (yr != jb.y0) adj_noise_items<TR>(jb, p0e, trow[t], lanee, r4, yr - 1, col, e, sig,
+                                               half_var);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int row = trow[t] * TM + TR::acc_row(lanee, r4 + s);
+            const uint32_t path = (uint32_t)(p0e + row);
+            const T l = acc[t][r4 + s];
+            if (yr == jb.y0) {
+              if ((int)path < N) lame[(size_t)path * Pp + col] = l;
+            } else {
+              T m = (T)0;
+              if ((int)path < N && col < jb.P) {
+                const T x = npe[i * plane + (size_t)path * Pp + col];
+                m = l * ((T)1 + rr * ((T)1 - (T)2 * x * Ki)) * e[s];
+                gsum += (double)(l * x * ((T)1 - x * Ki) * e[s]);
+              }
+              if ((int)path < N) mue[i * plane + (size_t)path * Pp + col] = m;
+              A[row * LDA + TR::perm(col)] = m;
             }
-            if ((int)path < N) mu[i * plane + (size_t)path * Pp + col] = m;
-            A[row * LDA + TR::perm(col)] = m;
           }
         }
         if (yr != jb.y0) {
@@ -214,7 +282,7 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
           // per column writes the row block's sum.
 #pragma unroll
           for (int off = TN; off < 64; off <<= 1) gsum += __shfl_xor(gsum, off, 64);
-          if (lane < TN) grb[trow[t] * Pp + col] = gsum;
+          if (lanee < TN) grb[trow[t] * Pp + col] = gsum;
         }
       }
     }
@@ -231,20 +299,110 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   for (int c = tid; c < Pp; c += nthreads) jb.grad_part[(size_t)tile * Pp + c] = gr[c];
 }
 
-// grad_r[c] += sum over tiles of the segment's partials, in tile order.
-__global__ void adj_reduce_r_kernel(const double* __restrict__ part, int n_tiles, int Pp, int P,
-                                    double* __restrict__ grad_r) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= P) return;
+// grad_r[c] += sum over tiles of the segment's partials: one wave per
+// column, lane l adding tiles l, l + 64, ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void adj_reduce_r_kernel(const double* __restrict__ part,
+                                                          int n_tiles, int Pp, int P,
+                                                          double* __restrict__ grad_r) {
+  const int c = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
-  for (int t = 0; t < n_tiles; ++t) s += part[(size_t)t * Pp + c];
-  grad_r[c] += s;
+  for (int t = lane; t < n_tiles; t += 64) s += part[(size_t)t * Pp + c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) grad_r[c] += s;
 }
 
-// dJ/dM += sum_{i, paths} ns[i]^T mu[i] over the segment (K = n_year * N rows
-// of [Pp] each).  Grid (Pp/TB)^2 output blocks x nsplit chunks of K; each
-// workgroup (4 waves) forms one TB x TB block of one chunk on MFMA and writes
-// it to its partial slot; adj_reduce_kernel sums the slots in order (FP64).
+// dJ/dM += sum_{i, paths} ns[i]^T mu[i] over the segment: C = A^T B with
+// A, B of Ktot = n_year * N rows of Pp values.  Split-K: grid (Pp/128)^2
+// output blocks x nsplit chunks of K; a 256-thread workgroup forms one
+// 128 x 128 block of one chunk (each wave a 64 x 64 quarter), staging
+// 16-row slabs of A and B in LDS (double buffered, 16-byte loads) and
+// writing the block to its partial slot; adj_reduce_kernel sums the slots in
+// order (FP64).  FP32 only (the FP64 checker keeps adj_gradm_f64_kernel).
+constexpr int kGmBlock = 128, kGmSlab = 16;
+
+__global__ __launch_bounds__(256) void adj_gradm_f32_kernel(const float* __restrict__ A,
+                                                            const float* __restrict__ B,
+                                                            int Ktot, int Pp, int kchunk,
+                                                            double* __restrict__ partial) {
+  typedef float floatx4 __attribute__((ext_vector_type(4)));
+  typedef float floatx16 __attribute__((ext_vector_type(16)));
+  __shared__ __attribute__((aligned(16))) float sa[2][kGmSlab][kGmBlock];
+  __shared__ __attribute__((aligned(16))) float sb[2][kGmSlab][kGmBlock];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nb = Pp / kGmBlock;
+  const int ob = blockIdx.x % (nb * nb), split = blockIdx.x / (nb * nb);
+  const int i0 = (ob / nb) * kGmBlock, j0 = (ob % nb) * kGmBlock;
+  const int kb = split * kchunk, ke = min(Ktot, kb + kchunk);
+  const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;  // wave's quarter
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = floatx16{};
+  // Slab loads: 16 rows x 128 floats = 512 float4 per matrix, 2 per thread.
+  auto load_slab = [&](int k0, floatx4 (&ra)[2], floatx4 (&rb)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + h * 256;           // float4 index in the slab
+      const int r = v >> 5, c4 = (v & 31) * 4;
+      const int k = k0 + r;
+      const bool ok = k < ke;
+      ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + (size_t)k * Pp + i0 + c4) : floatx4{};
+      rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + (size_t)k * Pp + j0 + c4) : floatx4{};
+    }
+  };
+  auto store_slab = [&](int buf, const floatx4 (&ra)[2], const floatx4 (&rb)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + h * 256;
+      const int r = v >> 5, c4 = (v & 31) * 4;
+      *reinterpret_cast<floatx4*>(&sa[buf][r][c4]) = ra[h];
+      *reinterpret_cast<floatx4*>(&sb[buf][r][c4]) = rb[h];
+    }
+  };
+  floatx4 ra[2], rb[2];
+  load_slab(kb, ra, rb);
+  store_slab(0, ra, rb);
+  __syncthreads();
+  int buf = 0;
+  const int kl = lane >> 5, il = lane & 31;
+  for (int k0 = kb; k0 < ke; k0 += kGmSlab) {
+    const bool more = k0 + kGmSlab < ke;
+    if (more) load_slab(k0 + kGmSlab, ra, rb);  // in flight during the MFMAs
+#pragma unroll
+    for (int kk = 0; kk < kGmSlab; kk += 2) {
+      float a[2], b[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        a[x] = sa[buf][kk + kl][wi + x * 32 + il];
+        b[x] = sb[buf][kk + kl][wj + x * 32 + il];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+    if (more) store_slab(buf ^ 1, ra, rb);
+    __syncthreads();
+    buf ^= 1;
+  }
+  double* out = partial + (size_t)split * Pp * Pp;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = i0 + wi + x * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int col = j0 + wj + y * 32 + (lane & 31);
+        out[(size_t)row * Pp + col] = (double)acc[x][y][r];
+      }
+}
+
+// FP64 checker form: 64 x 64 output blocks, operands read directly.
 template <class TR>
 __global__ __launch_bounds__(256) void adj_gradm_kernel(const typename TR::T* __restrict__ A,
                                                         const typename TR::T* __restrict__ B,
@@ -323,24 +481,32 @@ hipError_t launch_adj_segment(bool fp64, const AdjJob& jb, int n_tiles, size_t l
                        jb);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(adj_reduce_r_kernel, dim3((jb.P + 255) / 256), dim3(256), 0, s,
-                     jb.grad_part, n_tiles, jb.Pp, jb.P, jb.grad_r);
+  hipLaunchKernelGGL(adj_reduce_r_kernel, dim3(jb.P), dim3(64), 0, s, jb.grad_part, n_tiles,
+                     jb.Pp, jb.P, jb.grad_r);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_gradm(bool fp64, const void* A, const void* B, int Ktot, int Pp,
                             int nsplit, double* partial, double* grad, hipStream_t s) {
-  const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
-  const int nb = Pp / 64;
-  const dim3 grid(nb * nb * nsplit);
-  if (fp64)
-    hipLaunchKernelGGL(adj_gradm_kernel<TraitsF64>, grid, dim3(256), 0, s,
+  if (fp64) {
+    const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
+    const int nb = Pp / 64;
+    hipLaunchKernelGGL(adj_gradm_kernel<TraitsF64>, dim3(nb * nb * nsplit), dim3(256), 0, s,
                        static_cast<const double*>(A), static_cast<const double*>(B), Ktot, Pp,
                        kchunk, partial);
-  else
-    hipLaunchKernelGGL(adj_gradm_kernel<TraitsF32>, grid, dim3(256), 0, s,
+  } else if (Pp % kGmBlock == 0) {
+    const int kchunk = ((Ktot + nsplit - 1) / nsplit + kGmSlab - 1) / kGmSlab * kGmSlab;
+    const int nb = Pp / kGmBlock;
+    hipLaunchKernelGGL(adj_gradm_f32_kernel, dim3(nb * nb * nsplit), dim3(256), 0, s,
                        static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
                        kchunk, partial);
+  } else {
+    const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
+    const int nb = Pp / 64;
+    hipLaunchKernelGGL(adj_gradm_kernel<TraitsF32>, dim3(nb * nb * nsplit), dim3(256), 0, s,
+                       static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
+                       kchunk, partial);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int n = Pp * Pp;

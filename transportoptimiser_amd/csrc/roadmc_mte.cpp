@@ -962,7 +962,7 @@ extern "C" int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
   const int Pp = j0.Pp, N = j0.N, L = adj->L, P = adj->P;
   const size_t es = adj->fp64 ? 8 : 4;
   const size_t plane = (size_t)N * Pp;
-  const int nsplit = 64;
+  const int nsplit = 128;
   int rc;
   if ((rc = rmc_buf_ensure(ctx, adj->ns, (size_t)L * plane * es)) ||
       (rc = rmc_buf_ensure(ctx, adj->np, (size_t)L * plane * es)) ||

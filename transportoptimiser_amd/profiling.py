@@ -17,6 +17,10 @@ persistent MTE kernel.
 valuation of 3 uncertainties (2 OU + 1 GBM), 2 controls, 50 steps, 100,000
 paths and a cubic basis R times, printing the host time of each call.
 
+``python transportoptimiser_amd/profiling.py adjoint [--repeat R]`` runs the
+adjoint of one species with 256 patches, 100 years, 50,000 paths and
+checkpoints every 10 years (forward and backward) R times.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -78,6 +82,26 @@ def rov(repeat: int) -> None:
         r = simulate_rov(ctx, pr)
         print(f"rov call {i}: {(time.perf_counter() - t) * 1e3:.2f} ms, value {r['value']:.4f}",
               flush=True)
+    ctx.close()
+
+
+def adjoint(repeat: int) -> None:
+    import os
+    import time
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import landscape
+    from transportoptimiser_amd.adjoint import mte_gradient
+    from transportoptimiser_amd.api import Context
+    ctx = Context(0)
+    pr = landscape.RoadProblem([landscape.make_species(np.random.default_rng(4), 256,
+                                                       mort_max=0.4)],
+                               n_years=100, n_paths=50000, seed=4)
+    for i in range(repeat):
+        t = time.perf_counter()
+        obj, gm, gr = mte_gradient(ctx, pr, 10)
+        print(f"adjoint call {i}: {(time.perf_counter() - t) * 1e3:.2f} ms, objective {obj:.6e}, "
+              f"|dJ/dm| {np.abs(gm).sum():.6e}", flush=True)
     ctx.close()
 
 
@@ -155,6 +179,8 @@ def main() -> int:
     c.add_argument("--bytes", type=int, default=1 << 30)
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
+    ad = sub.add_parser("adjoint")
+    ad.add_argument("--repeat", type=int, default=3)
     rv = sub.add_parser("rov")
     rv.add_argument("--repeat", type=int, default=5)
     u = sub.add_parser("uniform")
@@ -175,6 +201,8 @@ def main() -> int:
     a = ap.parse_args()
     if a.cmd == "copy":
         copy(a.bytes)
+    elif a.cmd == "adjoint":
+        adjoint(a.repeat)
     elif a.cmd == "rov":
         rov(a.repeat)
     elif a.cmd == "uniform":
