@@ -175,6 +175,55 @@ int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
                          uint32_t species, int32_t n_paths, int32_t n_years,
                          int32_t n_patches, float* out);
 
+/* ---- Road -> movement and survival matrices -------------------------------- */
+
+/* One species' patches and one road, as SpeciesRoadPatches reads them
+ * (SpeciesRoadPatches.cpp:269-451).  Transition weights use
+ * lambda = lambda_mean + lambda_sd * lambda_mult; the distance cut-off is
+ * max_dist = ln(0.05) / -(lambda_mean + lambda_sd * x), with x = lambda_mult
+ * for the crossing counts (roadCrossings, :327-330) and x = lambda for the
+ * transitions (computeTransitionProbabilities, :396-397), as there. */
+#define ROADMC_SEG_ROAD 0   /* RoadSegments::ROAD: counted */
+#define ROADMC_SEG_BRIDGE 1 /* not counted */
+#define ROADMC_SEG_TUNNEL 2 /* not counted */
+
+#define ROADMC_TRANS_REFERENCE 0 /* off-diagonal weight / exp(-lambda d), i.e. x exp(+lambda d)
+                                    (SpeciesRoadPatches.cpp:413-416) */
+#define ROADMC_TRANS_DECAY 1     /* off-diagonal weight x exp(-lambda d) */
+
+typedef struct {
+  int32_t n_patches;         /* P */
+  const double* x;           /* [P] patch centroids */
+  const double* y;           /* [P] */
+  const double* area;        /* [P] */
+  const double* pref_mean;   /* [P] habitat-preference mean of the patch's type */
+  const double* pref_sd;     /* [P] habitat-preference SD of the patch's type */
+  double pref_mult;          /* habitat-preference multiplier (scenario) */
+  double lambda_mean, lambda_sd, lambda_mult;
+  int32_t n_road_points;     /* road vertices; segment s joins s and s + 1 */
+  const double* road_x;      /* [n_road_points] */
+  const double* road_y;      /* [n_road_points] */
+  const int32_t* road_type;  /* [n_road_points - 1] ROADMC_SEG_*, or NULL: all road */
+  int32_t n_controls;        /* C flow rates */
+  const double* flow;        /* [C] vehicles per hour */
+  double vehicle_width;      /* sum of vehicle width x proportion */
+  double animal_length;      /* species length mean */
+  double animal_speed;       /* species speed mean */
+  int32_t transition_form;   /* ROADMC_TRANS_* */
+  int32_t drop_last_link;    /* 1: as roadCrossings, which passes all but the last
+                                in-range pair to the crossing count
+                                (SpeciesRoadPatches.cpp:369-374); 0: count all */
+} roadmc_road_species;
+
+/* SpeciesRoadPatches::roadCrossings + computeTransitionProbabilities +
+ * computeSurvivalProbabilities (SpeciesRoadPatches.cpp:296-451) with the
+ * segment test of Utility::lineSegmentIntersect (Utility.cpp:3-92; GPU form
+ * SimulateGPU.cu:454-493) in FP64 on the GPU.  Outputs are column-major P x P
+ * (Eigen's order), origin = row; any may be NULL:
+ *   crossings [P*P], trans [P*P], surv [C][P*P]. */
+int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_species* rs,
+                              int32_t* crossings, double* trans, double* surv);
+
 /* ---- Adjoint ---------------------------------------------------------------- */
 
 /* Gradient of J = d_objective * mean total end population (one species,

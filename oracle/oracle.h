@@ -20,6 +20,9 @@
  *   - the regression normal equations and the linear solve of multiQuadReg /
  *     solveLinearSystem (SimulateGPU.cu:6092-6207, 6297-6358), with correct
  *     partial pivoting;
+ *   - SpeciesRoadPatches::roadCrossings, computeTransitionProbabilities and
+ *     computeSurvivalProbabilities (include/road/SpeciesRoadPatches.cpp:
+ *     270-451) with Utility::lineSegmentIntersect (Utility.cpp:3-91);
  *   - the adjoint and MLP references used by the gradient tests.
  *
  * Random numbers follow the Philox4x32-10 stream contract documented in
@@ -133,6 +136,21 @@ int orc_mte_adjoint(const orc_species* sp, uint32_t road_id, int32_t n_years,
 /* Mean squared error over n samples and its gradient (grad may be NULL). */
 double orc_mlp_loss_grad(int n_in, int H, int n_out, const double* w,
                          const double* X, const double* Y, int n, double* grad);
+
+/* Road -> matrices (road_oracle.c); column-major P x P, row = origin.
+ * orc_road_crossings returns the number of in-range pairs, or -1 when out of
+ * memory. */
+int64_t orc_road_crossings(int32_t P, const double* x, const double* y, double lambda_mean,
+                           double lambda_sd, double lambda_mult, int32_t n_points,
+                           const double* road_x, const double* road_y, const int32_t* road_type,
+                           int32_t drop_last, int32_t* out);
+void orc_road_transitions(int32_t P, const double* x, const double* y, const double* area,
+                          const double* pref_mean, const double* pref_sd, double pref_mult,
+                          double lambda_mean, double lambda_sd, double lambda_mult,
+                          int32_t form, double* out);
+void orc_road_survival(int32_t P, const int32_t* crossings, int32_t n_controls,
+                       const double* flow, double vehicle_width, double animal_length,
+                       double animal_speed, double* out);
 
 #ifdef __cplusplus
 }

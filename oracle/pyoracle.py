@@ -299,3 +299,49 @@ def mlp_loss_grad(w, X, Y, n_in: int, n_out: int, H: int = 64):
     loss = h.orc_mlp_loss_grad(n_in, H, n_out, w.ctypes.data_as(_dp), X.ctypes.data_as(_dp),
                                Y.ctypes.data_as(_dp), len(X), g.ctypes.data_as(_dp))
     return loss, g
+
+
+# ---- Road -> matrices ---------------------------------------------------------
+
+def road_matrices(inp):
+    """Crossings (P, P) int32, transitions (P, P) and survival (C, P, P) of a
+    ``road.RoadSpeciesInput``, by the oracle's restatement."""
+    h = lib()
+    h.orc_road_crossings.argtypes = [C.c_int32, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                     C.c_int32, _dp, _dp, _ip, C.c_int32, _ip]
+    h.orc_road_crossings.restype = C.c_int64
+    h.orc_road_transitions.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_int32, _dp]
+    h.orc_road_transitions.restype = None
+    h.orc_road_survival.argtypes = [C.c_int32, _ip, C.c_int32, _dp, C.c_double, C.c_double,
+                                    C.c_double, _dp]
+    h.orc_road_survival.restype = None
+    d = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    P = inp.n_patches
+    x, y, area, pm, ps = d(inp.x), d(inp.y), d(inp.area), d(inp.pref_mean), d(inp.pref_sd)
+    road = d(inp.road).reshape(-1, 2)
+    rx, ry = d(road[:, 0]), d(road[:, 1])
+    rt = (np.ascontiguousarray(inp.road_type, dtype=np.int32)
+          if inp.road_type is not None else None)
+    flows = d(inp.flows).reshape(-1)
+    cr = np.zeros(P * P, dtype=np.int32)
+    n = h.orc_road_crossings(P, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), inp.lambda_mean,
+                             inp.lambda_sd, inp.lambda_mult, len(road), rx.ctypes.data_as(_dp),
+                             ry.ctypes.data_as(_dp),
+                             rt.ctypes.data_as(_ip) if rt is not None else None,
+                             1 if inp.drop_last_link else 0, cr.ctypes.data_as(_ip))
+    if n < 0:
+        raise MemoryError("orc_road_crossings")
+    tr = np.zeros(P * P)
+    h.orc_road_transitions(P, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                           area.ctypes.data_as(_dp), pm.ctypes.data_as(_dp),
+                           ps.ctypes.data_as(_dp), inp.pref_mult, inp.lambda_mean,
+                           inp.lambda_sd, inp.lambda_mult, int(inp.transition_form),
+                           tr.ctypes.data_as(_dp))
+    sv = np.zeros(len(flows) * P * P)
+    if len(flows):
+        h.orc_road_survival(P, cr.ctypes.data_as(_ip), len(flows), flows.ctypes.data_as(_dp),
+                            inp.vehicle_width, inp.animal_length, inp.animal_speed,
+                            sv.ctypes.data_as(_dp))
+    return (cr.reshape(P, P, order="F"), tr.reshape(P, P, order="F"),
+            sv.reshape(len(flows), P, P).transpose(0, 2, 1).copy())

@@ -1,0 +1,126 @@
+"""GPU road -> matrix builder against the oracle (roadmc_build_species_road).
+
+Crossing counts are compared bit-exactly.  Both sides evaluate the same FP64
+operations in the same order without contractions and differ only in the exp
+implementation (device vs libm, each within about 1 ulp) and in the order of
+the transition row sum (a fixed-order tree on the device, a sequential sum in
+the oracle).  Survival therefore agrees to 1e-13 relative, and transitions to
+1e-12: a sum of P positive terms in two orders differs by at most (P-1) eps,
+1.1e-13 at P = 1000.
+"""
+import numpy as np
+import pytest
+
+from transportoptimiser_amd import native as nat
+from transportoptimiser_amd import road as rd
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-13
+RTOL_TRANS = 1e-12
+
+
+def _check(ctx, oracle, inp):
+    print(f"road builder: P={inp.n_patches} points={len(inp.road)} "
+          f"form={inp.transition_form} drop={inp.drop_last_link}", flush=True)
+    m = rd.build_species_road(ctx, inp)
+    cr, tr, sv = oracle.road_matrices(inp)
+    np.testing.assert_array_equal(m.crossings, cr)
+    np.testing.assert_allclose(m.trans, tr, rtol=RTOL_TRANS, atol=0)
+    np.testing.assert_allclose(m.surv, sv, rtol=RTOL, atol=0)
+    return m
+
+
+@pytest.mark.parametrize("P,seed", [(2, 0), (7, 1), (64, 2), (300, 3), (1000, 4)])
+@pytest.mark.parametrize("form", [nat.TRANS_REFERENCE, nat.TRANS_DECAY])
+def test_matches_oracle(ctx, oracle, P, seed, form):
+    inp = rd.random_road_species(np.random.default_rng(seed), P)
+    inp.transition_form = form
+    m = _check(ctx, oracle, inp)
+    assert m.crossings.sum() > 0 or P < 10
+    np.testing.assert_allclose(m.trans.sum(1), 1.0, rtol=1e-13)
+
+
+def test_without_drop_and_without_types(ctx, oracle):
+    inp = rd.random_road_species(np.random.default_rng(5), 500, n_road_points=300)
+    inp.drop_last_link = False
+    inp.road_type = None
+    _check(ctx, oracle, inp)
+
+
+def test_degenerate_roads_and_controls(ctx, oracle):
+    rng = np.random.default_rng(6)
+    inp = rd.random_road_species(rng, 50)
+    inp.road = inp.road[:1]
+    inp.road_type = np.zeros(0, dtype=np.int32)
+    m = _check(ctx, oracle, inp)
+    assert m.crossings.sum() == 0
+    inp = rd.random_road_species(rng, 1)
+    _check(ctx, oracle, inp)
+    inp = rd.random_road_species(rng, 40, n_controls=0)
+    m = rd.build_species_road(ctx, inp)
+    assert m.surv.shape == (0, 40, 40)
+
+
+def test_more_segments_than_one_lds_chunk(ctx, oracle):
+    # 700 segments span three LDS chunks of 256.
+    inp = rd.random_road_species(np.random.default_rng(7), 400, n_road_points=701)
+    _check(ctx, oracle, inp)
+
+
+def test_partial_outputs(ctx, oracle):
+    inp = rd.random_road_species(np.random.default_rng(8), 100)
+    cr, tr, sv = oracle.road_matrices(inp)
+    m = rd.build_species_road(ctx, inp, crossings=False, trans=False, surv=True)
+    assert m.crossings is None and m.trans is None
+    np.testing.assert_allclose(m.surv, sv, rtol=RTOL, atol=0)
+    m = rd.build_species_road(ctx, inp, crossings=True, trans=False, surv=False)
+    np.testing.assert_array_equal(m.crossings, cr)
+
+
+def test_large_landscape_properties(ctx, oracle):
+    # 4096 patches: symmetric non-negative counts, survival a function of the
+    # counts, unit row sums, and the oracle's counts.
+    inp = rd.random_road_species(np.random.default_rng(9), 4096, n_road_points=120)
+    print("road builder: P=4096 properties", flush=True)
+    m = rd.build_species_road(ctx, inp)
+    cr = m.crossings
+    assert (cr == cr.T).all() and (cr >= 0).all() and (np.diag(cr) == 0).all()
+    np.testing.assert_allclose(m.trans.sum(1), 1.0, rtol=1e-12)
+    c = 0
+    k = (inp.vehicle_width + inp.animal_length) * inp.flows[c] / (inp.animal_speed * 3600)
+    np.testing.assert_allclose(m.surv[c], np.exp(-cr * k), rtol=1e-13)
+    ocr, _, _ = oracle.road_matrices(inp)
+    np.testing.assert_array_equal(cr, ocr)
+
+
+def test_invalid_arguments_raise(ctx):
+    inp = rd.random_road_species(np.random.default_rng(10), 20)
+    inp.animal_speed = 0.0
+    with pytest.raises(nat.RoadmcError) as e:
+        rd.build_species_road(ctx, inp)
+    assert e.value.code == nat.ERR_INVALID
+    inp = rd.random_road_species(np.random.default_rng(10), 20)
+    inp.transition_form = 7
+    with pytest.raises(nat.RoadmcError):
+        rd.build_species_road(ctx, inp)
+
+
+def test_road_to_mte_end_to_end(ctx, oracle):
+    """Matrices from the builder drive the MTE path; the oracle's MTE on the
+    oracle's matrices gives the same expected end population."""
+    from transportoptimiser_amd.landscape import RoadProblem
+    rng = np.random.default_rng(11)
+    inp = rd.random_road_species(rng, 96)
+    P = inp.n_patches
+    cap = rng.uniform(500, 5000, P)
+    srp = rd.SpeciesRoadPatches(ctx, inp)
+    sd_gpu = srp.to_species(1, 0.8 * cap, cap, np.full(P, 0.05), noise_sd=0.1)
+    _, tr, sv = oracle.road_matrices(inp)
+    from transportoptimiser_amd.landscape import SpeciesData
+    sd_orc = SpeciesData(trans=tr, surv=sv[1], init_pop=0.8 * cap, capacity=cap,
+                         growth=np.full(P, 0.05), noise_sd=0.1)
+    kw = dict(n_years=20, n_paths=2000, precision=nat.PREC_FP64, seed=5)
+    out = ctx.simulate_mte(RoadProblem(species=[sd_gpu], **kw))
+    ref = oracle.mte_problem(RoadProblem(species=[sd_orc], **kw), n_threads=8)
+    np.testing.assert_allclose(out.end_pops, ref["end_pops"], rtol=1e-9, atol=0)

@@ -41,6 +41,9 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } adj_bufs;
+  struct {
+    rmc_devbuf a, b, c, d, e, f;
+  } road_bufs;
 };
 
 int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);

@@ -1,0 +1,135 @@
+// C-ABI host code for the road -> matrix builder (road_kernels.hip).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "road_kernels.h"
+#include "roadmc.h"
+
+namespace {
+
+int validate_road(roadmc_ctx* ctx, const roadmc_road_species* r) {
+  if (!r) return rmc_fail(ctx, ROADMC_ERR_INVALID, "road species is NULL");
+  if (r->n_patches < 1 || r->n_patches > 16384)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_patches must be in [1, 16384]");
+  if (!r->x || !r->y || !r->area || !r->pref_mean || !r->pref_sd)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "patch arrays must not be NULL");
+  if (r->n_road_points < 0 || (r->n_road_points > 0 && (!r->road_x || !r->road_y)))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "road vertices invalid");
+  if (r->n_controls < 0 || (r->n_controls > 0 && !r->flow))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_controls/flow invalid");
+  if (r->transition_form != ROADMC_TRANS_REFERENCE && r->transition_form != ROADMC_TRANS_DECAY)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown transition_form");
+  const double sc[] = {r->pref_mult, r->lambda_mean, r->lambda_sd, r->lambda_mult,
+                       r->vehicle_width, r->animal_length, r->animal_speed};
+  for (double v : sc)
+    if (!std::isfinite(v)) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite parameter");
+  if (!(r->animal_speed > 0.0))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "animal_speed must be positive");
+  for (int i = 0; i < r->n_patches; ++i)
+    if (!std::isfinite(r->x[i]) || !std::isfinite(r->y[i]) || !std::isfinite(r->area[i]))
+      return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite patch data");
+  return ROADMC_OK;
+}
+
+}  // namespace
+
+extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_species* r,
+                                         int32_t* crossings, double* trans, double* surv) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  int rc = validate_road(ctx, r);
+  if (rc) return rc;
+  if (!crossings && !trans && !surv)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "no output requested");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const int P = r->n_patches, C = r->n_controls;
+  // Transition rate and the two distance cut-offs, as the reference forms
+  // them (SpeciesRoadPatches.cpp:327-330, 387-397).
+  const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
+  const double cut_cross = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * r->lambda_mult));
+  const double cut_trans = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * lambda));
+  std::vector<double> pw(P);
+  for (int j = 0; j < P; ++j)
+    pw[j] = r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
+  // Only open-road segments count (roadCrossings, SpeciesRoadPatches.cpp:335-344).
+  std::vector<double4> seg;
+  for (int s = 0; s + 1 < r->n_road_points; ++s)
+    if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD)
+      seg.push_back(make_double4(r->road_x[s], r->road_y[s], r->road_x[s + 1], r->road_y[s + 1]));
+  const int S = (int)seg.size();
+
+  const size_t n = (size_t)P * P;
+  auto& B = ctx->road_bufs;
+  if ((rc = rmc_buf_ensure(ctx, B.a, 3 * (size_t)P * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.b, std::max<size_t>(S, 1) * sizeof(double4))) ||
+      (rc = rmc_buf_ensure(ctx, B.c, n * sizeof(int32_t) + 64)) ||
+      (rc = rmc_buf_ensure(ctx, B.d, n * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.e, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))))
+    return rc;
+  hipStream_t s = ctx->stream;
+  double* d_xyw = static_cast<double*>(B.a.ptr);
+  std::vector<double> xyw(3 * (size_t)P);
+  std::memcpy(xyw.data(), r->x, P * sizeof(double));
+  std::memcpy(xyw.data() + P, r->y, P * sizeof(double));
+  std::memcpy(xyw.data() + 2 * P, pw.data(), P * sizeof(double));
+  RMC_HIP(ctx, hipMemcpyAsync(d_xyw, xyw.data(), xyw.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+  if (S > 0)
+    RMC_HIP(ctx, hipMemcpyAsync(B.b.ptr, seg.data(), S * sizeof(double4), hipMemcpyHostToDevice,
+                                s));
+  if (C > 0)
+    RMC_HIP(ctx, hipMemcpyAsync(B.f.ptr, r->flow, C * sizeof(double), hipMemcpyHostToDevice, s));
+  int* d_last = reinterpret_cast<int*>(static_cast<char*>(B.c.ptr) + n * sizeof(int32_t));
+  const int minus_one = -1;
+  RMC_HIP(ctx, hipMemcpyAsync(d_last, &minus_one, sizeof(int), hipMemcpyHostToDevice, s));
+
+  roadmc::RoadArgs a{};
+  a.P = P;
+  a.S = S;
+  a.C = C;
+  a.x = d_xyw;
+  a.y = d_xyw + P;
+  a.pw = d_xyw + 2 * P;
+  a.lambda = lambda;
+  a.cut_cross = cut_cross;
+  a.cut_trans = cut_trans;
+  a.seg = static_cast<const double4*>(B.b.ptr);
+  a.form = r->transition_form == ROADMC_TRANS_DECAY ? 1 : 0;
+  a.flow = static_cast<const double*>(B.f.ptr);
+  a.width_len = r->vehicle_width + r->animal_length;
+  a.speed_3600 = r->animal_speed * 3600.0;
+  a.cross = static_cast<int*>(B.c.ptr);
+  a.trans = static_cast<double*>(B.d.ptr);
+  a.surv = static_cast<double*>(B.e.ptr);
+  a.last = r->drop_last_link ? d_last : nullptr;
+  RMC_HIP(ctx, roadmc::launch_road_build(a, r->drop_last_link != 0, trans != nullptr,
+                                         surv != nullptr, s));
+  // Stage everything, then write the caller's buffers (nothing on failure).
+  std::vector<int32_t> hc;
+  std::vector<double> ht, hs;
+  if (crossings) {
+    hc.resize(n);
+    RMC_HIP(ctx, hipMemcpyAsync(hc.data(), a.cross, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  }
+  if (trans) {
+    ht.resize(n);
+    RMC_HIP(ctx, hipMemcpyAsync(ht.data(), a.trans, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (surv && C > 0) {
+    hs.resize((size_t)C * n);
+    RMC_HIP(ctx, hipMemcpyAsync(hs.data(), a.surv, hs.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  }
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  for (double v : ht)
+    if (!std::isfinite(v)) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
+  if (crossings) std::memcpy(crossings, hc.data(), n * sizeof(int32_t));
+  if (trans) std::memcpy(trans, ht.data(), n * sizeof(double));
+  if (surv && C > 0) std::memcpy(surv, hs.data(), hs.size() * sizeof(double));
+  return ROADMC_OK;
+}

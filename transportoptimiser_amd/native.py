@@ -134,6 +134,39 @@ class RovResult(C.Structure):
     ]
 
 
+SEG_ROAD = 0
+SEG_BRIDGE = 1
+SEG_TUNNEL = 2
+TRANS_REFERENCE = 0
+TRANS_DECAY = 1
+
+
+class RoadSpecies(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int32),
+        ("x", _dp),
+        ("y", _dp),
+        ("area", _dp),
+        ("pref_mean", _dp),
+        ("pref_sd", _dp),
+        ("pref_mult", C.c_double),
+        ("lambda_mean", C.c_double),
+        ("lambda_sd", C.c_double),
+        ("lambda_mult", C.c_double),
+        ("n_road_points", C.c_int32),
+        ("road_x", _dp),
+        ("road_y", _dp),
+        ("road_type", _ip),
+        ("n_controls", C.c_int32),
+        ("flow", _dp),
+        ("vehicle_width", C.c_double),
+        ("animal_length", C.c_double),
+        ("animal_speed", C.c_double),
+        ("transition_form", C.c_int32),
+        ("drop_last_link", C.c_int32),
+    ]
+
+
 # Every symbol include/roadmc.h declares: name -> (restype, argtypes).
 SIGNATURES: dict[str, tuple] = {
     "roadmc_device_count": (C.c_int, [_ip]),
@@ -159,6 +192,8 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_mte_penalty": (C.c_int, [C.c_int32, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]),
     "roadmc_patch_normals": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                        C.c_int32, C.c_int32, C.c_int32, _fp]),
+    "roadmc_build_species_road": (C.c_int, [C.c_void_p, C.POINTER(RoadSpecies), _ip, _dp,
+                                            _dp]),
     "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),

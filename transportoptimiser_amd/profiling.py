@@ -21,6 +21,10 @@ paths and a cubic basis R times, printing the host time of each call.
 adjoint of one species with 256 patches, 100 years, 50,000 paths and
 checkpoints every 10 years (forward and backward) R times.
 
+``python transportoptimiser_amd/profiling.py road [--patches P --points V]``
+builds the crossing, transition and survival matrices of a synthetic species
+of P patches and a road of V vertices (3 flow controls) R times.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -105,6 +109,27 @@ def adjoint(repeat: int) -> None:
     ctx.close()
 
 
+def road(patches: int, points: int, repeat: int) -> None:
+    """Road -> matrix builder on a synthetic landscape of ``patches`` patches
+    and a road of ``points`` vertices (kernel trace target)."""
+    import os
+    import time
+
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd.api import Context
+    from transportoptimiser_amd.road import build_species_road, random_road_species
+    ctx = Context(0)
+    inp = random_road_species(np.random.default_rng(1), patches, n_road_points=points)
+    for i in range(repeat):
+        t = time.perf_counter()
+        m = build_species_road(ctx, inp)
+        print(f"road call {i}: P={patches} segments={points - 1} "
+              f"{(time.perf_counter() - t) * 1e3:.2f} ms, crossings {int(m.crossings.sum())}",
+              flush=True)
+    ctx.close()
+
+
 def summarize(paths: list[str]) -> None:
     for p in paths:
         with open(p, newline="") as f:
@@ -186,6 +211,10 @@ def main() -> int:
     u = sub.add_parser("uniform")
     u.add_argument("--patches", type=int, required=True)
     u.add_argument("--roads", type=int, default=8)
+    rd = sub.add_parser("road")
+    rd.add_argument("--patches", type=int, default=1024)
+    rd.add_argument("--points", type=int, default=200)
+    rd.add_argument("--repeat", type=int, default=3)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -205,6 +234,8 @@ def main() -> int:
         adjoint(a.repeat)
     elif a.cmd == "rov":
         rov(a.repeat)
+    elif a.cmd == "road":
+        road(a.patches, a.points, a.repeat)
     elif a.cmd == "uniform":
         uniform(a.patches, a.roads)
     elif a.cmd == "dbstats":
