@@ -84,8 +84,9 @@ def read_traffic(kernel_ms: float):
 def run_extras(ctx) -> dict:
     """The other configurations of the hot path, each timed once after a
     warm-up: the large single road (3 x 256 patches, 100 years, 100k paths),
-    the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis) and
-    the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10)."""
+    the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis),
+    the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10) and
+    the road -> matrix builder (4096 patches)."""
     from transportoptimiser_amd import landscape
     from transportoptimiser_amd.adjoint import mte_gradient
     from transportoptimiser_amd.rov import config5, simulate_rov
@@ -133,6 +134,17 @@ def run_extras(ctx) -> dict:
                               "what": "256 patches, 100 years, 50,000 paths, checkpoint 10; "
                                       "forward + backward, FP32"}
     log(f"adjoint config4: {out['adjoint_config4']}")
+    from transportoptimiser_amd.road import build_species_road, random_road_species
+    rs = random_road_species(np.random.default_rng(1), 4096, n_road_points=200)
+    build_species_road(ctx, rs)
+    t = time.perf_counter()
+    build_species_road(ctx, rs)
+    ms = (time.perf_counter() - t) * 1e3
+    out["road_builder_p4096"] = {
+        "host_ms": round(ms, 2),
+        "what": "crossings, transitions and survival (3 controls) of 4096 patches and a "
+                "177-segment road, FP64; host call incl. 604 MB of outputs to pageable memory"}
+    log(f"road builder: {out['road_builder_p4096']}")
     return out
 
 

@@ -181,8 +181,8 @@ int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
  * (SpeciesRoadPatches.cpp:269-451).  Transition weights use
  * lambda = lambda_mean + lambda_sd * lambda_mult; the distance cut-off is
  * max_dist = ln(0.05) / -(lambda_mean + lambda_sd * x), with x = lambda_mult
- * for the crossing counts (roadCrossings, :327-330) and x = lambda for the
- * transitions (computeTransitionProbabilities, :396-397), as there. */
+ * for the crossing counts (roadCrossings, :328-332) and x = lambda for the
+ * transitions (computeTransitionProbabilities, :389-395), as there. */
 #define ROADMC_SEG_ROAD 0   /* RoadSegments::ROAD: counted */
 #define ROADMC_SEG_BRIDGE 1 /* not counted */
 #define ROADMC_SEG_TUNNEL 2 /* not counted */

@@ -344,4 +344,4 @@ def road_matrices(inp):
                             inp.vehicle_width, inp.animal_length, inp.animal_speed,
                             sv.ctypes.data_as(_dp))
     return (cr.reshape(P, P, order="F"), tr.reshape(P, P, order="F"),
-            sv.reshape(len(flows), P, P).transpose(0, 2, 1).copy())
+            sv.reshape(len(flows), P, P).transpose(0, 2, 1))

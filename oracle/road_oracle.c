@@ -43,7 +43,7 @@ int64_t orc_road_crossings(int32_t P, const double* x, const double* y, double l
                            double lambda_sd, double lambda_mult, int32_t n_points,
                            const double* road_x, const double* road_y, const int32_t* road_type,
                            int32_t drop_last, int32_t* out) {
-  /* Cut-off with the raw multiplier (SpeciesRoadPatches.cpp:327-330). */
+  /* Cut-off with the raw multiplier (SpeciesRoadPatches.cpp:328-332). */
   const double max_dist = log(0.05) / (-(lambda_mean + lambda_sd * lambda_mult));
   const int n_segs = n_points > 1 ? n_points - 1 : 0;
   double* vis = (double*)malloc(sizeof(double) * 4 * (n_segs > 0 ? n_segs : 1));
@@ -53,7 +53,7 @@ int64_t orc_road_crossings(int32_t P, const double* x, const double* y, double l
     free(pairs);
     return -1;
   }
-  /* Open-road segments only (:335-344). */
+  /* Open-road segments only (:337-347). */
   int n_vis = 0;
   for (int s = 0; s < n_segs; ++s)
     if (!road_type || road_type[s] == 0) {
@@ -63,7 +63,7 @@ int64_t orc_road_crossings(int32_t P, const double* x, const double* y, double l
       vis[4 * n_vis + 3] = road_y[s + 1];
       ++n_vis;
     }
-  /* Pairs i < j within the cut-off, in loop order (:351-366). */
+  /* Pairs i < j within the cut-off, in loop order (:352-366). */
   int64_t n_pairs = 0;
   for (int i = 0; i < P; ++i)
     for (int j = i + 1; j < P; ++j)

@@ -63,9 +63,52 @@ def test_degenerate_roads_and_controls(ctx, oracle):
 
 
 def test_more_segments_than_one_lds_chunk(ctx, oracle):
-    # 700 segments span three LDS chunks of 256.
-    inp = rd.random_road_species(np.random.default_rng(7), 400, n_road_points=701)
+    # 1300 segments span three LDS chunks of 512.
+    inp = rd.random_road_species(np.random.default_rng(7), 400, n_road_points=1301)
     _check(ctx, oracle, inp)
+
+
+def _boundary_input(xy, road):
+    P = len(xy)
+    return rd.RoadSpeciesInput(
+        x=xy[:, 0].copy(), y=xy[:, 1].copy(), area=np.ones(P), pref_mean=np.zeros(P),
+        pref_sd=np.zeros(P), road=road, lambda_mean=0.05, lambda_sd=0.0, lambda_mult=0.0,
+        drop_last_link=False)
+
+
+def test_boundary_geometry_matches_oracle(ctx, oracle):
+    """Road vertices on centroids, on links and an ulp off centroids, so that
+    segment parameters fall at and within ulps of 0 and 1: the device's
+    division-free decision must reproduce the division test exactly."""
+    rng = np.random.default_rng(13)
+    P = 60
+    xy = rng.uniform(0, 4, (P, 2))
+    pts = []
+    for _ in range(400):
+        i, j = rng.integers(0, P, 2)
+        kind = rng.integers(0, 4)
+        if kind == 0:
+            p = xy[i].copy()
+        elif kind == 1:
+            p = xy[i] + rng.choice([0.25, 0.5, 0.75]) * (xy[j] - xy[i])
+        elif kind == 2:
+            p = np.nextafter(xy[i], xy[i] + rng.choice([-1.0, 1.0], 2))
+        else:
+            p = rng.uniform(0, 4, 2)
+        pts.append(p)
+    m = _check(ctx, oracle, _boundary_input(xy, np.array(pts)))
+    assert m.crossings.sum() > 0
+
+
+def test_integer_grid_geometry_matches_oracle(ctx, oracle):
+    """Centroids and road on an integer grid: parallel, collinear and
+    touching segments (zero denominators and numerators) throughout."""
+    rng = np.random.default_rng(14)
+    xy = rng.integers(0, 6, (50, 2)).astype(np.float64)
+    xy = np.unique(xy, axis=0)
+    road = rng.integers(0, 6, (120, 2)).astype(np.float64)
+    m = _check(ctx, oracle, _boundary_input(xy, road))
+    assert m.crossings.sum() > 0
 
 
 def test_partial_outputs(ctx, oracle):

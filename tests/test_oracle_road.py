@@ -204,3 +204,55 @@ def test_single_patch(oracle):
     inp = _inp([[0, 0]], [[1, -1], [1, 1]])
     cr, tr, sv = oracle.road_matrices(inp)
     assert cr.tolist() == [[0]] and tr.tolist() == [[1.0]] and sv[0].tolist() == [[1.0]]
+
+
+def _ratio_class(n, de):
+    """numpy statement of the device's division-free verdict on RN(n / de)
+    against [0, 1] (road_kernels.hip, seg_hit): 0 outside, 1 inside,
+    2 undecided (the device divides)."""
+    ad = np.abs(de)
+    s = np.where(de < 0, -n, n)
+    hi = ad * (1.0 + 2.0 ** -51)
+    ok = (ad >= 2.0 ** -900) & (ad <= 2.0 ** 100)
+    inside = (s >= 0) & (s <= ad)
+    near = ((s < 0) & (s > -2.0 ** -900)) | ((s > ad) & (s <= hi))
+    cls = np.where(inside, 1, np.where(near, 2, 0)).astype(np.int8)
+    cls[~ok] = 2
+    return cls
+
+
+def test_division_free_classification_agrees_with_division():
+    """Every decided class equals the division's verdict, on quotients at and
+    within a few ulps of 0 and 1, subnormal and huge operands, both signs."""
+    rng = np.random.default_rng(12)
+    m = 400000
+    de = rng.uniform(0.5, 2.0, m) * 2.0 ** rng.integers(-1074, 1000, m).astype(np.float64)
+    de[: m // 8] = rng.uniform(-3, 3, m // 8)
+    de = np.where(rng.uniform(size=m) < 0.5, -de, de)
+    de = de[(de != 0) & np.isfinite(de)]
+    m = len(de)
+    kind = rng.integers(0, 5, m)
+    steps = rng.integers(-4, 5, m)
+    n = np.empty(m)
+    one = de.copy()                                        # q = 1
+    n[kind == 0] = one[kind == 0]
+    n[kind == 1] = (de * rng.uniform(-3, 3, m))[kind == 1]  # generic
+    n[kind == 2] = (de * 2.0 ** -rng.integers(1000, 1100, m))[kind == 2]  # q near 0
+    n[kind == 3] = -(de * 2.0 ** -rng.integers(1000, 1100, m))[kind == 3]
+    n[kind == 4] = 0.0
+    # walk some numerators a few ulps either way
+    for _ in range(4):
+        up = steps > 0
+        dn = steps < 0
+        n[up] = np.nextafter(n[up], np.inf)
+        n[dn] = np.nextafter(n[dn], -np.inf)
+        steps = steps - np.sign(steps)
+    with np.errstate(all="ignore"):
+        q = n / de
+    truth = ((q >= 0) & (q <= 1)).astype(np.int8)
+    cls = _ratio_class(n, de)
+    decided = cls != 2
+    np.testing.assert_array_equal(cls[decided], truth[decided])
+    # Generic quotients with moderate operands are all but always decided.
+    moderate = (kind == 1) & (np.abs(de) > 2.0 ** -800) & (np.abs(de) < 2.0 ** 90)
+    assert moderate.sum() > 1000 and decided[moderate].mean() > 0.999

@@ -31,6 +31,16 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{REPO / 'include'}", f"-I{CSRC}",
           "-Wall", "-Wno-unused-result"]
 
 
+# Per-source flags.  The road builder reproduces the reference's FP64
+# rounding (separate multiplies and adds), so it is compiled without
+# multiply-add contraction; hipcc's default for device code contracts across
+# statements and ignores `#pragma clang fp contract`.
+EXTRA_FLAGS = {
+    "road_kernels.hip": ["-ffp-contract=off"],
+    "roadmc_road.cpp": ["-ffp-contract=off"],
+}
+
+
 def _run(cmd: list[str]) -> None:
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
@@ -50,8 +60,8 @@ def build_lib(jobs: int = 8) -> pathlib.Path:
         obj = OBJ / (src.name + ".o")
         objs.append(obj)
         lang = ["-x", "hip"] if src.suffix == ".hip" else []
-        cmds.append([HIPCC, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *lang,
-                     "-c", str(src), "-o", str(obj)])
+        cmds.append([HIPCC, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS,
+                     *EXTRA_FLAGS.get(src.name, []), *lang, "-c", str(src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for f in [ex.submit(_run, c) for c in cmds]:
             f.result()

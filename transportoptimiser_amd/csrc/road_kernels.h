@@ -6,6 +6,12 @@
 
 namespace roadmc {
 
+// Device status word of one build, zeroed before the launch.
+struct RoadTail {
+  int bad;  // set when a normalised transition weight is not finite
+  int pad[3];
+};
+
 struct RoadArgs {
   int P, S, C;              // patches, visible road segments, controls
   const double* x;          // [P] patch centroids
@@ -16,17 +22,28 @@ struct RoadArgs {
   double cut_trans;         // distance cut-off of the transitions
   const double4* seg;       // [S] visible segments (x1, y1, x2, y2)
   int form;                 // 0: weight / exp(-lambda d); 1: weight * exp(-lambda d)
+  int drop_last;            // leave the last in-range pair uncounted
   const double* flow;       // [C]
   double width_len;         // vehicle width + animal length
   double speed_3600;        // animal speed * 3600
   int* cross;               // [P*P] column-major counts
   double* trans;            // [P*P] column-major
   double* surv;             // [C][P*P] column-major
-  int* last;                // [1] largest row-major index of an in-range pair, or null
+  int* blockmax;            // [road_cross_blocks(P)] largest in-range key per workgroup
+  double* part;             // [ceil(P / kRoadCols)][P] partial transition row sums
+  RoadTail* tail;
 };
 
-hipError_t launch_road_build(const RoadArgs& a, bool drop_last, bool want_trans,
-                             bool want_surv, hipStream_t s);
+constexpr int kRoadThreads = 256;
+constexpr int kRoadCols = 256;   // columns per transition workgroup
+constexpr int kRoadCand = 4096;  // matrix entries per crossing workgroup
+
+inline int road_cross_blocks(int P) {
+  return (int)(((long long)P * P + kRoadCand - 1) / kRoadCand);
+}
+inline size_t road_part_elems(int P) { return (size_t)((P + kRoadCols - 1) / kRoadCols) * P; }
+
+hipError_t launch_road_build(const RoadArgs& a, bool want_trans, bool want_surv, hipStream_t s);
 
 }  // namespace roadmc
 

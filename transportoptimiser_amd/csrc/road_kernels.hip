@@ -3,153 +3,293 @@
 // Reference: SpeciesRoadPatches::roadCrossings, computeTransitionProbabilities
 // and computeSurvivalProbabilities (include/road/SpeciesRoadPatches.cpp:
 // 296-451), whose crossing counts come from Utility::lineSegmentIntersect
-// (include/utilities/Utility.cpp:3-92; the reference's GPU form is
+// (include/utilities/Utility.cpp:3-91; the reference's GPU form is
 // pathAdjacencyKernel + roadCrossingsKernel, SimulateGPU.cu:454-493, in FP32
-// with one thread per (link, segment) and an HBM adjacency matrix).  Here one
-// thread owns a patch pair, the road segments are staged in LDS, the test is
-// FP64 with the CPU path's operation order and no fused multiply-adds, and
-// counts, weights and exponentials are written straight into the caller's
-// column-major layout.
+// with one thread per (link, segment) and an HBM adjacency matrix).
+//
+// Here, in FP64 with the CPU path's operations and order (no contractions):
+//   1. road_cross_kernel: a workgroup tests 4096 matrix entries against the
+//      crossing cut-off (column-major, so the zero fill coalesces), lists the
+//      in-range pairs i < j in LDS and runs one thread per listed pair
+//      against the road staged in LDS, so lanes stay busy however sparse the
+//      in-range pairs are.  The segment test decides the two parameter
+//      ranges from the signs and magnitudes of the numerators and the
+//      denominator, which is exact, and divides only when that cannot
+//      decide (a parameter within an ulp of 0 or 1, or a denominator that is
+//      zero, subnormal or huge);
+//   2. road_drop_kernel: the reference's uncounted last pair, from the
+//      per-workgroup maxima (no global atomics anywhere);
+//   3. road_trans_sum_kernel / road_trans_write_kernel: 64 rows x 256
+//      columns per workgroup, lanes on consecutive rows so that column-major
+//      stores coalesce; row sums are fixed-order sums of per-chunk partials
+//      and the weights are recomputed rather than re-read;
+//   4. road_surv_kernel: survival from a per-workgroup LDS table of
+//      exp(-k flow (width + length) / (speed 3600)) for small counts k,
+//      evaluated by the same expression as the direct form.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "road_kernels.h"
 
+// The segment test, the distances and the cut-off comparisons must round like
+// the reference's separate multiplies and adds, so this file is compiled with
+// -ffp-contract=off (the __d*_rn helpers are plain operators in HIP and would
+// otherwise fuse).
+
 namespace roadmc {
 
-constexpr int kRoadSegChunk = 256;
+constexpr int kRoadSegChunk = 512;  // segments staged per LDS pass (16 KiB)
+constexpr int kSurvTable = 64;       // tabulated crossing counts per control
+constexpr int kSurvMaxC = 16;        // controls the table covers
 
-// Utility::lineSegmentIntersect (Utility.cpp:53-77): link (x1,y1)-(x2,y2)
-// against segment (x3,y3)-(x4,y4); intersect when both parameters lie in
-// [0, 1].  Explicit round-to-nearest operations keep the CPU evaluation
-// order (no contraction into FMAs), so boundary cases match the CPU path.
-__device__ __forceinline__ int seg_hit(double x1, double y1, double x2, double y2, double x3,
-                                       double y3, double x4, double y4) {
-  const double x43 = __dsub_rn(x4, x3), y13 = __dsub_rn(y1, y3);
-  const double y43 = __dsub_rn(y4, y3), x13 = __dsub_rn(x1, x3);
-  const double x21 = __dsub_rn(x2, x1), y21 = __dsub_rn(y2, y1);
+__device__ __forceinline__ double pair_dist(const double* x, const double* y, int i, int j) {
+  const double dx = __dsub_rn(x[j], x[i]), dy = __dsub_rn(y[j], y[i]);
+  return __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
+}
+
+// Utility::lineSegmentIntersect (Utility.cpp:45-83) for the link from
+// (x1, y1) with direction (x21, y21) and the segment from (x3, y3) with
+// direction (x43, y43): 1 when RN(na / de) and RN(nb / de) both lie in
+// [0, 1].  The differences are the reference's, rounded the same way.
+//
+// For |de| in [2^-900, 2^100] the verdict is read off s = n * sign(de)
+// (an exact sign flip), so that q = s / |de|:
+//  - 0 <= s <= |de| (s = -0 included): 0 <= q <= 1, inside;
+//  - s <= -2^-900: q <= -2^-1000, which does not round to -0, outside;
+//  - s > RN(|de| (1 + 2^-51)): q > 1 + 2^-53, which rounds above 1, outside;
+//  - NaN or infinite s: outside, as the division would be.
+// The remaining bands (-2^-900 < s < 0 and |de| < s <= RN(|de| (1 + 2^-51)))
+// and every other |de| (0, subnormal, huge, non-finite) take the division.
+__device__ __forceinline__ int seg_hit(double x1, double y1, double x21, double y21, double x3,
+                                       double y3, double x43, double y43) {
+  const double y13 = __dsub_rn(y1, y3), x13 = __dsub_rn(x1, x3);
   const double na = __dsub_rn(__dmul_rn(x43, y13), __dmul_rn(y43, x13));
   const double nb = __dsub_rn(__dmul_rn(x21, y13), __dmul_rn(y21, x13));
   const double de = __dsub_rn(__dmul_rn(y43, x21), __dmul_rn(x43, y21));
+  const double ad = fabs(de);
+  // Bitwise '&' keeps the tests branch-free.
+  if ((ad >= 0x1p-900) & (ad <= 0x1p100)) {
+    const double sa = de < 0.0 ? -na : na, sb = de < 0.0 ? -nb : nb;
+    const double hi = __dmul_rn(ad, 1.0 + 0x1p-51);
+    // Outside unless both lie in [-2^-900, hi] (the common, cheap exit).
+    if (!((sa >= -0x1p-900) & (sa <= hi) & (sb >= -0x1p-900) & (sb <= hi))) return 0;
+    if ((sa >= 0.0) & (sa <= ad) & (sb >= 0.0) & (sb <= ad)) return 1;
+    // Otherwise a parameter lies in one of the bands next to 0 or 1.
+  }
   const double ua = __ddiv_rn(na, de), ub = __ddiv_rn(nb, de);
   return (ua >= 0.0 && ua <= 1.0 && ub >= 0.0 && ub <= 1.0) ? 1 : 0;
 }
 
-__device__ __forceinline__ double pair_dist(const RoadArgs& a, int i, int j) {
-  const double dx = __dsub_rn(a.x[i], a.x[j]), dy = __dsub_rn(a.y[i], a.y[j]);
-  return __dsqrt_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)));
-}
-
-// Crossing count of every pair i < j within the crossing cut-off; the
-// symmetric matrix is written both ways.  For the reference's dropped link,
-// the largest row-major index of an in-range pair is recorded.
-__global__ __launch_bounds__(256) void road_cross_kernel(RoadArgs a) {
+// Workgroup b owns chunk c = gridDim.x - 1 - b: matrix entries e in
+// [c * kRoadCand, (c + 1) * kRoadCand),
+// e = j * P + i (row i, column j; column-major, so the zero fill coalesces).
+// Entries on the diagonal or beyond the cut-off are zeroed; in-range pairs
+// with i < j are listed in LDS and counted against the road (staged in LDS
+// in chunks), and both (i, j) and (j, i) receive the count.  The workgroup's
+// largest in-range key i * P + j goes to blockmax[c] for the dropped link.
+__global__ __launch_bounds__(kRoadThreads) void road_cross_kernel(RoadArgs a) {
   __shared__ double4 seg[kRoadSegChunk];
-  const int P = a.P;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = (int)(idx / P), j = (int)(idx - (long long)i * P);
-  const bool mine = idx < (long long)P * P && i < j;
-  bool in_range = false;
-  double x1 = 0, y1 = 0, x2 = 0, y2 = 0;
-  if (mine) {
-    in_range = pair_dist(a, i, j) <= a.cut_cross;
-    x1 = a.x[i];
-    y1 = a.y[i];
-    x2 = a.x[j];
-    y2 = a.y[j];
+  __shared__ int list[kRoadCand];
+  __shared__ int n_list;
+  __shared__ int red[kRoadThreads / 64];
+  const int P = a.P, S = a.S;
+  const long long n = (long long)P * P;
+  // Chunks late in column-major order hold the most pairs i < j: dispatch
+  // them first so that the light ones fill the tail.
+  const int chunk = gridDim.x - 1 - blockIdx.x;
+  const long long e0 = (long long)chunk * kRoadCand;
+  if (threadIdx.x == 0) n_list = 0;
+  __syncthreads();
+  int kmax = -1;
+  for (int k = 0; k < kRoadCand / kRoadThreads; ++k) {
+    const long long e = e0 + k * kRoadThreads + threadIdx.x;
+    if (e >= n) break;
+    const int j = (int)(e / P), i = (int)(e - (long long)j * P);
+    const bool in_range = i != j && pair_dist(a.x, a.y, i, j) <= a.cut_cross;
+    if (!in_range) {
+      a.cross[e] = 0;
+    } else if (i < j) {
+      const int key = i * P + j;
+      list[atomicAdd(&n_list, 1)] = key;
+      kmax = max(kmax, key);
+    }
   }
-  int count = 0;
-  for (int s0 = 0; s0 < a.S; s0 += kRoadSegChunk) {
-    const int ns = min(kRoadSegChunk, a.S - s0);
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = kmax;
+  __syncthreads();
+  const int nl = n_list;
+  if (threadIdx.x == 0) {
+    int m = red[0];
+    for (int w = 1; w < kRoadThreads / 64; ++w) m = max(m, red[w]);
+    a.blockmax[chunk] = m;
+  }
+  constexpr int kPer = kRoadCand / kRoadThreads;
+  int count[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) count[r] = 0;
+  for (int s0 = 0; s0 < S; s0 += kRoadSegChunk) {
+    const int ns = min(kRoadSegChunk, S - s0);
     __syncthreads();
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) seg[s] = a.seg[s0 + s];
+    for (int q = threadIdx.x; q < ns; q += kRoadThreads) {
+      const double4 g = a.seg[s0 + q];  // (x3, y3, x4, y4) -> (x3, y3, x4 - x3, y4 - y3)
+      seg[q] = make_double4(g.x, g.y, __dsub_rn(g.z, g.x), __dsub_rn(g.w, g.y));
+    }
     __syncthreads();
-    if (in_range)
-      for (int s = 0; s < ns; ++s) {
-        const double4 q = seg[s];
-        count += seg_hit(x1, y1, x2, y2, q.x, q.y, q.z, q.w);
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int t = r * kRoadThreads + threadIdx.x;
+      if (t < nl) {
+        const int key = list[t];
+        const int i = key / P, j = key - i * P;
+        const double x1 = a.x[i], y1 = a.y[i];
+        const double x21 = __dsub_rn(a.x[j], x1), y21 = __dsub_rn(a.y[j], y1);
+        int c = 0;
+        for (int q = 0; q < ns; ++q) {
+          const double4 g = seg[q];
+          c += seg_hit(x1, y1, x21, y21, g.x, g.y, g.z, g.w);
+        }
+        count[r] += c;
       }
+    }
   }
-  if (mine) {
-    a.cross[(size_t)j * P + i] = count;  // (i, j), column-major
-    a.cross[(size_t)i * P + j] = count;  // (j, i)
-    if (in_range && a.last) atomicMax(a.last, (int)idx);
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int t = r * kRoadThreads + threadIdx.x;
+    if (t < nl) {
+      const int key = list[t];
+      const int i = key / P, j = key - i * P;
+      a.cross[(size_t)j * P + i] = count[r];  // (i, j)
+      a.cross[(size_t)i * P + j] = count[r];  // (j, i)
+    }
   }
-  if (idx < (long long)P * P && i == j) a.cross[(size_t)i * P + i] = 0;
 }
 
-// Zeroes the pair roadCrossings leaves out of its count (one thread).
-__global__ void road_drop_kernel(RoadArgs a) {
-  const int k = *a.last;
-  if (k < 0) return;
-  const int i = k / a.P, j = k - i * a.P;
+// roadCrossings leaves its last in-range pair (largest i * P + j) out of the
+// count (SpeciesRoadPatches.cpp:369-374): one workgroup finds it among the
+// per-workgroup maxima and zeroes it.
+__global__ __launch_bounds__(1024) void road_drop_kernel(RoadArgs a, int n_blocks) {
+  __shared__ int red[1024 / 64];
+  int m = -1;
+  for (int b = threadIdx.x; b < n_blocks; b += 1024) m = max(m, a.blockmax[b]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < 1024 / 64; ++w) m = max(m, red[w]);
+  m = max(m, red[0]);
+  if (m < 0) return;
+  const int i = m / a.P, j = m - i * a.P;
   a.cross[(size_t)j * a.P + i] = 0;
   a.cross[(size_t)i * a.P + j] = 0;
 }
 
-// Transition row i (computeTransitionProbabilities, SpeciesRoadPatches.cpp:
-// 401-425): weights of the patches within the transition cut-off, then the
-// row normalised by its sum (fixed-order reduction).
-__global__ __launch_bounds__(256) void road_trans_kernel(RoadArgs a) {
-  __shared__ double red[256];
-  const int i = blockIdx.x, P = a.P;
-  double part = 0.0;
-  for (int j = threadIdx.x; j < P; j += blockDim.x) {
-    const double d = pair_dist(a, i, j);
-    double w = 0.0;
-    if (d <= a.cut_trans) {
-      if (i == j)
-        w = a.pw[j];
-      else if (a.form == 0)
-        w = __ddiv_rn(a.pw[j], exp(__dmul_rn(-a.lambda, d)));
-      else
-        w = __dmul_rn(a.pw[j], exp(__dmul_rn(-a.lambda, d)));
+// Transition weight of (i, j) before normalisation
+// (SpeciesRoadPatches.cpp:403-419).
+__device__ __forceinline__ double trans_weight(const RoadArgs& a, int i, int j) {
+  const double d = pair_dist(a.x, a.y, i, j);
+  if (!(d <= a.cut_trans)) return 0.0;
+  if (i == j) return a.pw[j];
+  const double e = exp(__dmul_rn(-a.lambda, d));
+  return a.form == 0 ? __ddiv_rn(a.pw[j], e) : __dmul_rn(a.pw[j], e);
+}
+
+// Workgroup (rb, cb): rows rb*64 + lane, columns cb*256 + wave*64 .. +63.
+// Each thread sums its 64 columns in order; the four waves' sums are added
+// in wave order into part[cb][row].
+__global__ __launch_bounds__(kRoadThreads) void road_trans_sum_kernel(RoadArgs a) {
+  __shared__ double red[4][64];
+  const int P = a.P;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  const int j0 = blockIdx.y * kRoadCols + w * 64;
+  double s = 0.0;
+  if (i < P)
+    for (int jj = 0; jj < 64; ++jj) {
+      const int j = j0 + jj;
+      if (j >= P) break;
+      s = __dadd_rn(s, trans_weight(a, i, j));
     }
-    a.trans[(size_t)j * P + i] = w;
-    part += w;
-  }
-  red[threadIdx.x] = part;
+  red[w][lane] = s;
   __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+  if (w == 0 && i < P)
+    a.part[(size_t)blockIdx.y * P + i] =
+        __dadd_rn(__dadd_rn(__dadd_rn(red[0][lane], red[1][lane]), red[2][lane]), red[3][lane]);
+}
+
+// Row sums from the partials (fixed order), weights recomputed and written
+// normalised; a non-finite result raises tail->bad.
+__global__ __launch_bounds__(kRoadThreads) void road_trans_write_kernel(RoadArgs a) {
+  const int P = a.P;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  if (i >= P) return;
+  const int nchunk = (P + kRoadCols - 1) / kRoadCols;
+  double sum = 0.0;
+  for (int c = 0; c < nchunk; ++c) sum = __dadd_rn(sum, a.part[(size_t)c * P + i]);
+  const int j0 = blockIdx.y * kRoadCols + w * 64;
+  bool bad = false;
+  for (int jj = 0; jj < 64; ++jj) {
+    const int j = j0 + jj;
+    if (j >= P) break;
+    const double v = __ddiv_rn(trans_weight(a, i, j), sum);
+    bad |= !__builtin_isfinite(v);
+    a.trans[(size_t)j * P + i] = v;
+  }
+  if (bad) a.tail->bad = 1;
+}
+
+__device__ __forceinline__ double surv_value(double cr, double flow, const RoadArgs& a) {
+  double v = __dmul_rn(-cr, flow);  // SpeciesRoadPatches.cpp:447-449, in that order
+  v = __dmul_rn(v, a.width_len);
+  v = __ddiv_rn(v, a.speed_3600);
+  return exp(v);
+}
+
+// survProbs[c] = exp(-crossings * flow_c * (width + length) / (speed * 3600)).
+__global__ __launch_bounds__(kRoadThreads) void road_surv_kernel(RoadArgs a) {
+  __shared__ double table[kSurvMaxC][kSurvTable];
+  const bool tab = a.C <= kSurvMaxC;
+  if (tab) {
+    for (int t = threadIdx.x; t < a.C * kSurvTable; t += kRoadThreads) {
+      const int c = t / kSurvTable, k = t - c * kSurvTable;
+      table[c][k] = surv_value((double)k, a.flow[c], a);
+    }
     __syncthreads();
   }
-  const double sum = red[0];
-  for (int j = threadIdx.x; j < P; j += blockDim.x)
-    a.trans[(size_t)j * P + i] = __ddiv_rn(a.trans[(size_t)j * P + i], sum);
-}
-
-// survProbs[c] = exp(-crossings * flow_c * (width + length) / (speed * 3600))
-// (SpeciesRoadPatches.cpp:446-450), in that order of operations.
-__global__ void road_surv_kernel(RoadArgs a) {
   const size_t n = (size_t)a.P * a.P;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const double cr = (double)a.cross[e];
-  for (int c = 0; c < a.C; ++c) {
-    double v = __dmul_rn(-cr, a.flow[c]);
-    v = __dmul_rn(v, a.width_len);
-    v = __ddiv_rn(v, a.speed_3600);
-    a.surv[(size_t)c * n + e] = exp(v);
+  for (size_t e = (size_t)blockIdx.x * kRoadThreads + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * kRoadThreads) {
+    const int k = a.cross[e];
+    for (int c = 0; c < a.C; ++c)
+      a.surv[(size_t)c * n + e] =
+          (tab && k < kSurvTable) ? table[c][k] : surv_value((double)k, a.flow[c], a);
   }
 }
 
-hipError_t launch_road_build(const RoadArgs& a, bool drop_last, bool want_trans,
-                             bool want_surv, hipStream_t s) {
-  const long long pairs = (long long)a.P * a.P;
-  const int blocks = (int)((pairs + 255) / 256);
-  hipLaunchKernelGGL(road_cross_kernel, dim3(blocks), dim3(256), 0, s, a);
+hipError_t launch_road_build(const RoadArgs& a, bool want_trans, bool want_surv, hipStream_t s) {
+  const long long n = (long long)a.P * a.P;
+  const int blocks = (int)((n + kRoadThreads - 1) / kRoadThreads);
+  const int cblocks = road_cross_blocks(a.P);
+  hipLaunchKernelGGL(road_cross_kernel, dim3(cblocks), dim3(kRoadThreads), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (drop_last) {
-    hipLaunchKernelGGL(road_drop_kernel, dim3(1), dim3(1), 0, s, a);
+  if (a.drop_last) {
+    hipLaunchKernelGGL(road_drop_kernel, dim3(1), dim3(1024), 0, s, a, cblocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (want_trans) {
-    hipLaunchKernelGGL(road_trans_kernel, dim3(a.P), dim3(256), 0, s, a);
+    const dim3 g((a.P + 63) / 64, (a.P + kRoadCols - 1) / kRoadCols);
+    hipLaunchKernelGGL(road_trans_sum_kernel, g, dim3(kRoadThreads), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(road_trans_write_kernel, g, dim3(kRoadThreads), 0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (want_surv && a.C > 0) {
-    hipLaunchKernelGGL(road_surv_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(road_surv_kernel, dim3(std::min(blocks, 4096)), dim3(kRoadThreads), 0, s,
+                       a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

@@ -1,6 +1,7 @@
 // C-ABI host code for the road -> matrix builder (road_kernels.hip).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -48,14 +49,14 @@ extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_spec
   RMC_HIP(ctx, hipSetDevice(ctx->device));
   const int P = r->n_patches, C = r->n_controls;
   // Transition rate and the two distance cut-offs, as the reference forms
-  // them (SpeciesRoadPatches.cpp:327-330, 387-397).
+  // them (SpeciesRoadPatches.cpp:328-332, 387-395).
   const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
   const double cut_cross = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * r->lambda_mult));
   const double cut_trans = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * lambda));
   std::vector<double> pw(P);
   for (int j = 0; j < P; ++j)
     pw[j] = r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
-  // Only open-road segments count (roadCrossings, SpeciesRoadPatches.cpp:335-344).
+  // Only open-road segments count (roadCrossings, SpeciesRoadPatches.cpp:337-347).
   std::vector<double4> seg;
   for (int s = 0; s + 1 < r->n_road_points; ++s)
     if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD)
@@ -63,13 +64,16 @@ extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_spec
   const int S = (int)seg.size();
 
   const size_t n = (size_t)P * P;
+  const size_t tail_off = (n * sizeof(int32_t) + 255) / 256 * 256;
   auto& B = ctx->road_bufs;
   if ((rc = rmc_buf_ensure(ctx, B.a, 3 * (size_t)P * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.b, std::max<size_t>(S, 1) * sizeof(double4))) ||
-      (rc = rmc_buf_ensure(ctx, B.c, n * sizeof(int32_t) + 64)) ||
+      (rc = rmc_buf_ensure(ctx, B.c, tail_off + sizeof(roadmc::RoadTail))) ||
       (rc = rmc_buf_ensure(ctx, B.d, n * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.e, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))))
+      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.g, (size_t)roadmc::road_cross_blocks(P) * sizeof(int))) ||
+      (rc = rmc_buf_ensure(ctx, B.h, roadmc::road_part_elems(P) * sizeof(double))))
     return rc;
   hipStream_t s = ctx->stream;
   double* d_xyw = static_cast<double*>(B.a.ptr);
@@ -84,9 +88,9 @@ extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_spec
                                 s));
   if (C > 0)
     RMC_HIP(ctx, hipMemcpyAsync(B.f.ptr, r->flow, C * sizeof(double), hipMemcpyHostToDevice, s));
-  int* d_last = reinterpret_cast<int*>(static_cast<char*>(B.c.ptr) + n * sizeof(int32_t));
-  const int minus_one = -1;
-  RMC_HIP(ctx, hipMemcpyAsync(d_last, &minus_one, sizeof(int), hipMemcpyHostToDevice, s));
+  auto* d_tail = reinterpret_cast<roadmc::RoadTail*>(static_cast<char*>(B.c.ptr) + tail_off);
+  roadmc::RoadTail tail{0, {0, 0, 0}};
+  RMC_HIP(ctx, hipMemcpyAsync(d_tail, &tail, sizeof(tail), hipMemcpyHostToDevice, s));
 
   roadmc::RoadArgs a{};
   a.P = P;
@@ -100,36 +104,29 @@ extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_spec
   a.cut_trans = cut_trans;
   a.seg = static_cast<const double4*>(B.b.ptr);
   a.form = r->transition_form == ROADMC_TRANS_DECAY ? 1 : 0;
+  a.drop_last = r->drop_last_link ? 1 : 0;
   a.flow = static_cast<const double*>(B.f.ptr);
   a.width_len = r->vehicle_width + r->animal_length;
   a.speed_3600 = r->animal_speed * 3600.0;
   a.cross = static_cast<int*>(B.c.ptr);
   a.trans = static_cast<double*>(B.d.ptr);
   a.surv = static_cast<double*>(B.e.ptr);
-  a.last = r->drop_last_link ? d_last : nullptr;
-  RMC_HIP(ctx, roadmc::launch_road_build(a, r->drop_last_link != 0, trans != nullptr,
-                                         surv != nullptr, s));
-  // Stage everything, then write the caller's buffers (nothing on failure).
-  std::vector<int32_t> hc;
-  std::vector<double> ht, hs;
-  if (crossings) {
-    hc.resize(n);
-    RMC_HIP(ctx, hipMemcpyAsync(hc.data(), a.cross, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  }
-  if (trans) {
-    ht.resize(n);
-    RMC_HIP(ctx, hipMemcpyAsync(ht.data(), a.trans, n * sizeof(double), hipMemcpyDeviceToHost, s));
-  }
-  if (surv && C > 0) {
-    hs.resize((size_t)C * n);
-    RMC_HIP(ctx, hipMemcpyAsync(hs.data(), a.surv, hs.size() * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
-  }
+  a.blockmax = static_cast<int*>(B.g.ptr);
+  a.part = static_cast<double*>(B.h.ptr);
+  a.tail = d_tail;
+  RMC_HIP(ctx, roadmc::launch_road_build(a, trans != nullptr, surv != nullptr, s));
+  RMC_HIP(ctx, hipMemcpyAsync(&tail, d_tail, sizeof(tail), hipMemcpyDeviceToHost, s));
   RMC_HIP(ctx, hipStreamSynchronize(s));
-  for (double v : ht)
-    if (!std::isfinite(v)) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
-  if (crossings) std::memcpy(crossings, hc.data(), n * sizeof(int32_t));
-  if (trans) std::memcpy(trans, ht.data(), n * sizeof(double));
-  if (surv && C > 0) std::memcpy(surv, hs.data(), hs.size() * sizeof(double));
+  if (trans && tail.bad)
+    return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
+  // The caller's buffers are written only once the build has succeeded.
+  if (crossings)
+    RMC_HIP(ctx, hipMemcpyAsync(crossings, a.cross, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (trans)
+    RMC_HIP(ctx, hipMemcpyAsync(trans, a.trans, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (surv && C > 0)
+    RMC_HIP(ctx, hipMemcpyAsync(surv, a.surv, (size_t)C * n * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipStreamSynchronize(s));
   return ROADMC_OK;
 }

@@ -95,18 +95,19 @@ def build_species_road(ctx, inp: RoadSpeciesInput, *, crossings: bool = True,
     rs.animal_speed = float(inp.animal_speed)
     rs.transition_form = int(inp.transition_form)
     rs.drop_last_link = 1 if inp.drop_last_link else 0
-    cr = np.zeros(P * P, dtype=np.int32) if crossings else None
-    tr = np.zeros(P * P) if trans else None
-    sv = np.zeros(len(flows) * P * P) if surv else None
+    cr = np.empty(P * P, dtype=np.int32) if crossings else None
+    tr = np.empty(P * P) if trans else None
+    sv = np.empty(len(flows) * P * P) if surv else None
     ptr = lambda a, t: a.ctypes.data_as(t) if a is not None else None  # noqa: E731
     nat.check(nat.lib().roadmc_build_species_road(ctx.handle, C.byref(rs), ptr(cr, nat._ip),
                                                   ptr(tr, nat._dp), ptr(sv, nat._dp)),
               ctx.handle)
+    # Column-major views: no copies (surv[c] is Fortran-ordered, as the MTE
+    # path marshals it).
     return RoadMatrices(
         crossings=cr.reshape(P, P, order="F") if cr is not None else None,
         trans=tr.reshape(P, P, order="F") if tr is not None else None,
-        surv=(sv.reshape(len(flows), P, P).transpose(0, 2, 1).copy()
-              if sv is not None else None))
+        surv=sv.reshape(len(flows), P, P).transpose(0, 2, 1) if sv is not None else None)
 
 
 class SpeciesRoadPatches:
