@@ -362,6 +362,47 @@ int roadmc_mlp_train(roadmc_mlp* net, const float* x, const float* y,
 /* out [n_members][n][n_out]. */
 int roadmc_mlp_predict(roadmc_mlp* net, const float* x, int32_t n, float* out);
 
+/* ---- Grid surrogate: kNN local-linear regression -------------------------- */
+
+/* The reference's surrogate fit, SimulateGPU::buildSurrogateMTECUDA and
+ * buildSurrogateROVCUDA (SimulateGPU.cu:5533-5917) with createQueryPoints
+ * (:1152-1193), multiLocLinReg (:2934-3053) and rovCorrection (:3055-3073),
+ * evaluated by multilinear interpolation (Utility::interpolateSurrogate,
+ * Utility.cpp:253-363; interpolateMulti, SimulateGPU.cu:3077-3155).
+ *
+ * The surrogate vector holds the grid coordinates of each dimension, then
+ * the fitted values: surrogate[dd*dim_res + i] = x_min[dd] + i (x_max[dd] -
+ * x_min[dd]) / (dim_res - 1), and value[idx] at surrogate[n_dims*dim_res +
+ * idx] with idx = sum_dd i_dd dim_res^(n_dims-1-dd) (dimension 0 most
+ * significant), as the reference lays it out.  Each grid value is the
+ * intercept of a weighted linear fit over the k nearest samples (ties to the
+ * lower sample index), weights exp(-(d/h)^2/2)/sqrt(2 pi) with h the k-th
+ * neighbour's distance; solved by FP64 Cholesky.  When the fit is singular
+ * (neighbours that do not span the dimensions) the value is the weighted
+ * mean; with fewer than 3 samples every value is 0, as the reference. */
+#define ROADMC_KNN_MAX_DIMS 8
+#define ROADMC_KNN_MAX_K 64
+#define ROADMC_KNN_NORMALIZE 1   /* neighbours and fit in [0,1]-scaled coordinates
+                                    (buildSurrogateROVCUDA, :5807-5822) */
+#define ROADMC_KNN_NONPOSITIVE 2 /* clamp fitted values to <= 0 (rovCorrection) */
+#define ROADMC_KNN_RULE_MTE 0    /* neighbour count of buildSurrogateMTECUDA (:5545-5554) */
+#define ROADMC_KNN_RULE_ROV 1    /* neighbour count of buildSurrogateROVCUDA (:5711-5724) */
+
+/* The reference's neighbour count for n_samples samples (at most n_samples). */
+int roadmc_knn_k(int32_t n_samples, int32_t rule, int32_t* k);
+/* x: [n_samples][n_dims] row-major, y: [n_samples]; surrogate:
+ * [n_dims*dim_res + dim_res^n_dims].  dim_res >= 2, dim_res^n_dims <= 2^24,
+ * 1 <= k <= ROADMC_KNN_MAX_K. */
+int roadmc_knn_surrogate_fit(roadmc_ctx* ctx, int32_t n_samples, int32_t n_dims,
+                             const double* x, const double* y, int32_t dim_res, int32_t k,
+                             int32_t flags, double* surrogate);
+/* predictors: [n_points][n_dims]; out: [n_points].  Multilinear
+ * interpolation in the grid cell of each point (cells at the ends of each
+ * dimension extend linearly beyond the grid). */
+int roadmc_knn_surrogate_eval(roadmc_ctx* ctx, int32_t n_dims, int32_t dim_res,
+                              const double* surrogate, int32_t n_points,
+                              const double* predictors, double* out);
+
 #ifdef __cplusplus
 }
 #endif

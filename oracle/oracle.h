@@ -23,6 +23,9 @@
  *   - SpeciesRoadPatches::roadCrossings, computeTransitionProbabilities and
  *     computeSurvivalProbabilities (include/road/SpeciesRoadPatches.cpp:
  *     270-451) with Utility::lineSegmentIntersect (Utility.cpp:3-91);
+ *   - the kNN local-linear grid surrogate of buildSurrogateMTECUDA /
+ *     buildSurrogateROVCUDA (SimulateGPU.cu:5533-5917) and its multilinear
+ *     evaluation (Utility.cpp:253-363);
  *   - the adjoint and MLP references used by the gradient tests.
  *
  * Random numbers follow the Philox4x32-10 stream contract documented in
@@ -151,6 +154,14 @@ void orc_road_transitions(int32_t P, const double* x, const double* y, const dou
 void orc_road_survival(int32_t P, const int32_t* crossings, int32_t n_controls,
                        const double* flow, double vehicle_width, double animal_length,
                        double animal_speed, double* out);
+
+/* kNN local-linear grid surrogate (knn_oracle.c).  orc_knn_fit returns 0,
+ * or a negative value for bad arguments or when out of memory. */
+int32_t orc_knn_k(int32_t n, int32_t rule);
+int orc_knn_fit(int32_t n, int32_t d, const double* x, const double* y, int32_t dim_res,
+                int32_t k, int32_t flags, double* surrogate);
+void orc_knn_eval(int32_t d, int32_t dim_res, const double* surrogate, int32_t m,
+                  const double* pred, double* out);
 
 #ifdef __cplusplus
 }

@@ -345,3 +345,46 @@ def road_matrices(inp):
                             sv.ctypes.data_as(_dp))
     return (cr.reshape(P, P, order="F"), tr.reshape(P, P, order="F"),
             sv.reshape(len(flows), P, P).transpose(0, 2, 1))
+
+
+# ---- kNN grid surrogate -------------------------------------------------------
+
+def _knn_lib():
+    h = lib()
+    h.orc_knn_k.argtypes = [C.c_int32, C.c_int32]
+    h.orc_knn_k.restype = C.c_int32
+    h.orc_knn_fit.argtypes = [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, C.c_int32, C.c_int32,
+                              _dp]
+    h.orc_knn_fit.restype = C.c_int
+    h.orc_knn_eval.argtypes = [C.c_int32, C.c_int32, _dp, C.c_int32, _dp, _dp]
+    h.orc_knn_eval.restype = None
+    return h
+
+
+def knn_k(n: int, rule: int) -> int:
+    return _knn_lib().orc_knn_k(n, rule)
+
+
+def knn_fit(x, y, dim_res: int, k: int, flags: int = 0) -> np.ndarray:
+    h = _knn_lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n, d = x.shape
+    out = np.zeros(d * dim_res + dim_res ** d)
+    rc = h.orc_knn_fit(n, d, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), dim_res, k, flags,
+                       out.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError(f"orc_knn_fit failed: {rc}")
+    return out
+
+
+def knn_eval(surrogate, d: int, dim_res: int, pred) -> np.ndarray:
+    h = _knn_lib()
+    s = np.ascontiguousarray(surrogate, dtype=np.float64)
+    p = np.ascontiguousarray(pred, dtype=np.float64).reshape(-1, d)
+    out = np.zeros(len(p))
+    h.orc_knn_eval(d, dim_res, s.ctypes.data_as(_dp), len(p), p.ctypes.data_as(_dp),
+                   out.ctypes.data_as(_dp))
+    return out

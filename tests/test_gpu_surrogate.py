@@ -68,6 +68,24 @@ def test_surrogate_gain_over_full_monte_carlo(ctx):
           f"{err[:, 1].mean():.4f}", flush=True)
     rel_rov = err[kept, 1] / np.abs(Y_full[kept, 1])
     print(f"surrogate-answered ROV: mean relative error {rel_rov.mean():.4f}")
+    # The reference's own surrogate method on the same data: kNN local-linear
+    # grids over the initial AAR (MTE, 1-D) and over (AAR, unit cost) (ROV,
+    # 2-D, scaled coordinates, values <= 0), with the reference's neighbour
+    # counts (SimulateGPU.cu:5533-5917).
+    from transportoptimiser_amd import knn
+    from transportoptimiser_amd import native as nat
+    Xt = np.stack([r.features for r in test])
+    g_mte = knn.fit(ctx, X[:, 1], Y[:, 0], 50, knn.neighbour_count(len(X), nat.KNN_RULE_MTE))
+    g_rov = knn.fit(ctx, X[:, [1, 7]], Y[:, 1], 20,
+                    knn.neighbour_count(len(X), nat.KNN_RULE_ROV), normalize=True,
+                    nonpositive=True)
+    knn_err = np.abs(np.stack([g_mte.predict(ctx, Xt[:, 1]), g_rov.predict(ctx, Xt[:, [1, 7]])],
+                              1) - Y_full)
+    mlp_err = np.abs(sur.predict(Xt)[0] - Y_full)
+    print(f"reference kNN grid surrogate: mean |dMTE| {knn_err[:, 0].mean():.4f}, mean |dROV|/1000 "
+          f"{knn_err[:, 1].mean():.4f}; MLP (no fallback): mean |dMTE| {mlp_err[:, 0].mean():.4f}, "
+          f"mean |dROV|/1000 {mlp_err[:, 1].mean():.4f}", flush=True)
+    assert mlp_err[:, 0].mean() < knn_err[:, 0].mean()
     assert np.all(err[fall] == 0.0)  # fallback roads are the full MC values
     assert fall.mean() < 0.3
     assert err[kept, 0].mean() < 0.01

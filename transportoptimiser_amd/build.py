@@ -31,13 +31,15 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{REPO / 'include'}", f"-I{CSRC}",
           "-Wall", "-Wno-unused-result"]
 
 
-# Per-source flags.  The road builder reproduces the reference's FP64
-# rounding (separate multiplies and adds), so it is compiled without
-# multiply-add contraction; hipcc's default for device code contracts across
-# statements and ignores `#pragma clang fp contract`.
+# Per-source flags.  The road builder and the grid surrogate follow the CPU
+# statement's FP64 rounding (separate multiplies and adds), so they are
+# compiled without multiply-add contraction; hipcc's default for device code
+# contracts across statements and ignores `#pragma clang fp contract`.
 EXTRA_FLAGS = {
     "road_kernels.hip": ["-ffp-contract=off"],
     "roadmc_road.cpp": ["-ffp-contract=off"],
+    "knn_kernels.hip": ["-ffp-contract=off"],
+    "roadmc_knn.cpp": ["-ffp-contract=off"],
 }
 
 

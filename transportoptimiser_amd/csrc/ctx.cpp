@@ -100,7 +100,9 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->adj_bufs.e,     &ctx->adj_bufs.f,    &ctx->adj_bufs.g,
                         &ctx->adj_bufs.h,     &ctx->road_bufs.a,   &ctx->road_bufs.b,
                         &ctx->road_bufs.c,    &ctx->road_bufs.d,   &ctx->road_bufs.e,
-                        &ctx->road_bufs.f,    &ctx->road_bufs.g,   &ctx->road_bufs.h};
+                        &ctx->road_bufs.f,    &ctx->road_bufs.g,   &ctx->road_bufs.h,
+                        &ctx->knn_bufs.a,     &ctx->knn_bufs.b,    &ctx->knn_bufs.c,
+                        &ctx->knn_bufs.d};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
   if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
   rmc_buf_free(ctx->rov_graph.param);

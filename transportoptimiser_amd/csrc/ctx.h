@@ -44,6 +44,9 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } road_bufs;
+  struct {
+    rmc_devbuf a, b, c, d;
+  } knn_bufs;
 };
 
 int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);

@@ -1,0 +1,39 @@
+// Host-visible declarations of the grid-surrogate kernels (knn_kernels.hip).
+#ifndef ROADMC_KNN_KERNELS_H
+#define ROADMC_KNN_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+namespace roadmc {
+
+struct KnnFitArgs {
+  int n, d, R, k, clamp;  // samples, dimensions, grid points per dimension, neighbours
+  int cap;                // per-lane list capacity: min(k, ceil(n / 64))
+  long long G;            // grid points, R^d
+  const double* u;        // [n][d] sample coordinates of the search and the fit
+  const double* y;        // [n]
+  const double* qtab;     // [d][R] query coordinate of grid index i in dimension dd
+  double* val;            // [G]
+};
+
+struct KnnEvalArgs {
+  int d, R, m;
+  const double* surrogate;  // [d*R + R^d]
+  const double* pred;       // [m][d]
+  double* out;              // [m]
+};
+
+constexpr int kKnnThreads = 64;  // one wave per workgroup
+constexpr int kKnnMaxK = 128;
+
+// Dynamic LDS of the fit kernel for per-lane lists of cap entries.
+inline size_t knn_fit_lds(int cap) {
+  return (size_t)cap * kKnnThreads * (sizeof(double) + sizeof(int));
+}
+
+hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s);
+hipError_t launch_knn_eval(const KnnEvalArgs& a, hipStream_t s);
+
+}  // namespace roadmc
+
+#endif

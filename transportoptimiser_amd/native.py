@@ -139,6 +139,10 @@ SEG_BRIDGE = 1
 SEG_TUNNEL = 2
 TRANS_REFERENCE = 0
 TRANS_DECAY = 1
+KNN_NORMALIZE = 1
+KNN_NONPOSITIVE = 2
+KNN_RULE_MTE = 0
+KNN_RULE_ROV = 1
 
 
 class RoadSpecies(C.Structure):
@@ -194,6 +198,11 @@ SIGNATURES: dict[str, tuple] = {
                                        C.c_int32, C.c_int32, C.c_int32, _fp]),
     "roadmc_build_species_road": (C.c_int, [C.c_void_p, C.POINTER(RoadSpecies), _ip, _dp,
                                             _dp]),
+    "roadmc_knn_k": (C.c_int, [C.c_int32, C.c_int32, _ip]),
+    "roadmc_knn_surrogate_fit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, C.c_int32,
+                                           C.c_int32, C.c_int32, _dp]),
+    "roadmc_knn_surrogate_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32,
+                                            _dp, _dp]),
     "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),

@@ -25,6 +25,10 @@ checkpoints every 10 years (forward and backward) R times.
 builds the crossing, transition and survival matrices of a synthetic species
 of P patches and a road of V vertices (3 flow controls) R times.
 
+``python transportoptimiser_amd/profiling.py knn [--samples N --dims D --res R]``
+fits an R^D grid surrogate from N samples (scaled coordinates, values <= 0,
+the reference's neighbour count) and evaluates it at 10^6 points.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -130,6 +134,35 @@ def road(patches: int, points: int, repeat: int) -> None:
     ctx.close()
 
 
+def knn(n: int, dims: int, res: int, repeat: int) -> None:
+    """Grid-surrogate fit (ROV layout: scaled coordinates, values <= 0, the
+    reference's neighbour count for n samples) and an evaluation of 10^6
+    points."""
+    import os
+    import time
+
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import knn as kn
+    from transportoptimiser_amd import native as nat
+    from transportoptimiser_amd.api import Context
+    ctx = Context(0)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.0, 1.0, size=(n, dims))
+    y = -np.exp(x.sum(1)) + 0.05 * rng.normal(size=n)
+    k = kn.neighbour_count(n, nat.KNN_RULE_ROV)
+    pts = rng.uniform(0.0, 1.0, size=(1000000, dims))
+    for i in range(repeat):
+        t = time.perf_counter()
+        g = kn.fit(ctx, x, y, res, k, normalize=True, nonpositive=True)
+        t1 = time.perf_counter()
+        g.predict(ctx, pts)
+        t2 = time.perf_counter()
+        print(f"knn call {i}: fit {res}^{dims} grid from {n} samples (k={k}) "
+              f"{(t1 - t) * 1e3:.2f} ms, eval 1e6 points {(t2 - t1) * 1e3:.2f} ms", flush=True)
+    ctx.close()
+
+
 def summarize(paths: list[str]) -> None:
     for p in paths:
         with open(p, newline="") as f:
@@ -215,6 +248,11 @@ def main() -> int:
     rd.add_argument("--patches", type=int, default=1024)
     rd.add_argument("--points", type=int, default=200)
     rd.add_argument("--repeat", type=int, default=3)
+    kn = sub.add_parser("knn")
+    kn.add_argument("--samples", type=int, default=4096)
+    kn.add_argument("--dims", type=int, default=3)
+    kn.add_argument("--res", type=int, default=20)
+    kn.add_argument("--repeat", type=int, default=3)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -234,6 +272,8 @@ def main() -> int:
         adjoint(a.repeat)
     elif a.cmd == "rov":
         rov(a.repeat)
+    elif a.cmd == "knn":
+        knn(a.samples, a.dims, a.res, a.repeat)
     elif a.cmd == "road":
         road(a.patches, a.points, a.repeat)
     elif a.cmd == "uniform":
