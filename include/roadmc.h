@@ -335,6 +335,86 @@ int roadmc_exp_pv(roadmc_ctx* ctx, const roadmc_uncertainty* unc,
                   int32_t n_steps, double time_step, double rrr, double growth,
                   int64_t n_paths, uint64_t seed, double* mean, double* sd);
 
+/* ---- ROV with species states (the reference's ALGO4) --------------------- */
+
+/* One species of the species-state valuation.  Every species has the same
+ * patches (SimulateGPU.cu:3965-3975). */
+typedef struct {
+  const double* init_pop;  /* [P] */
+  const double* capacity;  /* [P] */
+  const double* trans;     /* [P*P] column-major, origin = row */
+  const double* surv;      /* [C][P*P] survival under each control, column-major */
+  double params[8];        /* the packing of SimulateGPU.cu:3990-4016: growth current,
+                              mean, noise SD, threshold, reversion, jump size, jump
+                              probability, local variability */
+} roadmc_rovs_species;
+
+/* SimulateGPU::simulateROVCUDA with ROVMethod ALGO4 (SimulateGPU.cu:3870-4998):
+ *  - forward (forwardPathKernel, :789-956): per path and year a control drawn
+ *    uniformly (randControls, :720-730); each species' patch populations moved
+ *    by trans o surv of that control, the surviving fraction recorded under
+ *    every control (the AARs); growth with the growth-rate SDE and
+ *    n = n'(1 + g (K - n') / K / 100), g = grMean + z local_variability; fuel
+ *    and commodity prices on the MRJD law; ore composition N(mean, sd) per
+ *    path, year and commodity;
+ *  - unit profit = unit_cost + sum_v fuel_cost[v] price[fuel_idx[v]]
+ *    - unit_revenue sum_c composition_c price[n_fuels + c];
+ *  - backward (:4401-4998, backwardInduction :2333-2784): state = (adjusted
+ *    population of each species under the highest flow, unit profit); per
+ *    step and control a kNN local-linear grid (k = 10 (ln n + 1), capped at
+ *    ROADMC_KNN_MAX_K) over the paths that drew that control and are in the
+ *    money (every adjusted population at its threshold) or drew control 0;
+ *    each path takes the valid control minimising Q_c unit profit plus the
+ *    discounted, interpolated continuation (<= 0), switching from the running
+ *    best only below hysteresis x best;
+ *  - time 0 (firstPeriodInduction, :2210-2330).
+ * P <= 512, n_species + 1 <= ROADMC_KNN_MAX_DIMS, n_controls <=
+ * ROADMC_MAX_CONTROLS, dim_res^(n_species+1) <= 2^24. */
+typedef struct {
+  int32_t n_patches;                      /* P */
+  int32_t n_species;                      /* S */
+  const roadmc_rovs_species* species;     /* [S] */
+  int32_t n_controls;                     /* C */
+  const double* flow;                     /* [C] Q_c; control C - 1 is the highest flow */
+  int32_t n_fuels;                        /* fuel prices: uncertainties 0 .. n_fuels - 1 */
+  int32_t n_commodities;                  /* ore prices: the next n_commodities */
+  const roadmc_uncertainty* uncertainties;/* [n_fuels + n_commodities] */
+  const double* ore_mean;                 /* [n_commodities] composition mean */
+  const double* ore_sd;                   /* [n_commodities] composition SD */
+  int32_t n_vehicles;
+  const double* fuel_cost;                /* [n_vehicles] fuel use per unit traffic */
+  const int32_t* fuel_idx;                /* [n_vehicles] fuel of each vehicle class */
+  double unit_cost, unit_revenue;
+  int32_t n_years;                        /* T */
+  double time_step;                       /* dt */
+  double rrr;                             /* discount 1 / (1 + rrr dt / 100) per step */
+  int64_t n_paths;                        /* N */
+  int32_t dim_res;                        /* grid points per state dimension */
+  double profit_scaling;                  /* unit-profit scale of the kNN search
+                                             (2 in SimulateGPU.cu:3900) */
+  double hysteresis;                      /* 0.975 in SimulateGPU.cu:2763 */
+  uint64_t seed;
+  uint32_t road_id;
+} roadmc_rovs_problem;
+
+typedef struct {
+  double* value;          /* [1] time-0 expected cost (<= 0 is profit) */
+  double* value_sd;       /* [1] SD over the paths of the chosen control */
+  int32_t* first_control; /* [1] */
+  double* cond_exp;       /* [T][N] value per step and path (time 0 included), or NULL */
+  int32_t* opt_control;   /* [T][N], or NULL */
+  double* adj_pops;       /* [T][S][N] highest-flow adjusted populations, or NULL */
+  double* unit_profit;    /* [T][N], or NULL */
+  int32_t* controls;      /* [T][N] drawn controls, or NULL */
+  double* total_pops;     /* [S][T+1][N] forward total populations, or NULL */
+  double* aars;           /* [S][T][C][N] surviving fraction per control, or NULL */
+  double* grids;          /* [T][C][(S+1) dim_res + dim_res^(S+1)] fitted grids
+                             (steps 1 .. T-2), or NULL */
+} roadmc_rovs_result;
+
+int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
+                    roadmc_rovs_result* result);
+
 /* ---- Learned surrogate --------------------------------------------------- */
 
 /* An ensemble of n_members MLPs n_in -> 64 -> 64 -> n_out (tanh), trained in

@@ -110,6 +110,36 @@ static double knn_local_linear(int d, int k, const knn_pair* nb, const double* u
   return z[0];
 }
 
+int orc_knn_fit_scaled(int32_t n, int32_t d, const double* u, const double* y, int32_t R,
+                       int32_t k, const double* qtab, double* val) {
+  size_t G = 1;
+  for (int dd = 0; dd < d; ++dd) G *= (size_t)R;
+  knn_pair* nb = (knn_pair*)malloc(sizeof(knn_pair) * (size_t)(n > 0 ? n : 1));
+  if (!nb) return -2;
+  if (k > n) k = n;
+  for (size_t g = 0; g < G; ++g) {
+    double q[8];
+    size_t rem = g;
+    for (int dd = d - 1; dd >= 0; --dd) {
+      q[dd] = qtab[dd * R + (int)(rem % (size_t)R)];
+      rem /= (size_t)R;
+    }
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int dd = 0; dd < d; ++dd) {
+        const double e = u[(size_t)j * d + dd] - q[dd];
+        s += e * e;
+      }
+      nb[j].d2 = s;
+      nb[j].j = j;
+    }
+    qsort(nb, (size_t)n, sizeof(knn_pair), knn_cmp);
+    val[g] = knn_local_linear(d, k, nb, u, y, q);
+  }
+  free(nb);
+  return 0;
+}
+
 int orc_knn_fit(int32_t n, int32_t d, const double* x, const double* y, int32_t R, int32_t k,
                 int32_t flags, double* surrogate) {
   if (d < 1 || d > 8 || R < 2 || k < 1) return -1;

@@ -26,6 +26,8 @@
  *   - the kNN local-linear grid surrogate of buildSurrogateMTECUDA /
  *     buildSurrogateROVCUDA (SimulateGPU.cu:5533-5917) and its multilinear
  *     evaluation (Utility.cpp:253-363);
+ *   - the species-state ROV of simulateROVCUDA's ALGO4 (SimulateGPU.cu:
+ *     789-1148, 2210-2784, 4401-4998);
  *   - the adjoint and MLP references used by the gradient tests.
  *
  * Random numbers follow the Philox4x32-10 stream contract documented in
@@ -162,6 +164,42 @@ int orc_knn_fit(int32_t n, int32_t d, const double* x, const double* y, int32_t 
                 int32_t k, int32_t flags, double* surrogate);
 void orc_knn_eval(int32_t d, int32_t dim_res, const double* surrogate, int32_t m,
                   const double* pred, double* out);
+/* Fit at the query coordinates qtab [d][dim_res] from samples already in the
+ * search coordinates u [n][d]; values into val [dim_res^d]. */
+int orc_knn_fit_scaled(int32_t n, int32_t d, const double* u, const double* y, int32_t dim_res,
+                       int32_t k, const double* qtab, double* val);
+
+/* Species-state ROV, the reference's ALGO4 (rovs_oracle.c). */
+typedef struct {
+  int32_t P, S, C, T, R;
+  int64_t N;
+  const double* init_pop;   /* [S][P] */
+  const double* capacity;   /* [S][P] */
+  const double* trans;      /* [S][P*P] column-major, origin = row */
+  const double* surv;       /* [S][C][P*P] */
+  const double* params;     /* [S][8] */
+  const double* flow;       /* [C] */
+  int32_t n_fuels, n_comm;
+  const double* unc;        /* [n_fuels + n_comm][7]: kind, x0, mean, sd, reversion, J, p */
+  const double* ore_mean;   /* [n_comm] */
+  const double* ore_sd;     /* [n_comm] */
+  int32_t n_veh;
+  const double* fuel_cost;  /* [n_veh] */
+  const int32_t* fuel_idx;  /* [n_veh] */
+  double unit_cost, unit_revenue, dt, rrr, scaling, hysteresis;
+  uint64_t seed;
+  uint32_t road_id;
+} orc_rovs_spec;
+
+/* Forward pass of paths [0, N): controls [T][N], total_pops [S][T+1][N],
+ * aars [S][T][C][N], profit [T][N]. */
+int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_pops, double* aars,
+                     double* profit);
+/* Backward induction on given forward arrays: V [T][N], optc [T][N],
+ * out = {value, SD, control}. */
+int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const double* total_pops,
+                      const double* aars, const double* profit, double* V, int32_t* optc,
+                      double out[3]);
 
 #ifdef __cplusplus
 }

@@ -388,3 +388,94 @@ def knn_eval(surrogate, d: int, dim_res: int, pred) -> np.ndarray:
     h.orc_knn_eval(d, dim_res, s.ctypes.data_as(_dp), len(p), p.ctypes.data_as(_dp),
                    out.ctypes.data_as(_dp))
     return out
+
+
+# ---- Species-state ROV --------------------------------------------------------
+
+class OrcRovsSpec(C.Structure):
+    _fields_ = [
+        ("P", C.c_int32), ("S", C.c_int32), ("C", C.c_int32), ("T", C.c_int32), ("R", C.c_int32),
+        ("N", C.c_int64),
+        ("init_pop", _dp), ("capacity", _dp), ("trans", _dp), ("surv", _dp), ("params", _dp),
+        ("flow", _dp), ("n_fuels", C.c_int32), ("n_comm", C.c_int32), ("unc", _dp),
+        ("ore_mean", _dp), ("ore_sd", _dp), ("n_veh", C.c_int32), ("fuel_cost", _dp),
+        ("fuel_idx", _ip), ("unit_cost", C.c_double), ("unit_revenue", C.c_double),
+        ("dt", C.c_double), ("rrr", C.c_double), ("scaling", C.c_double),
+        ("hysteresis", C.c_double), ("seed", C.c_uint64), ("road_id", C.c_uint32),
+    ]
+
+
+def _rovs_spec(pr, keep):
+    def arr(a, dt=np.float64):
+        b = np.ascontiguousarray(a, dtype=dt)
+        keep.append(b)
+        return b.ctypes.data_as(_ip if dt == np.int32 else _dp)
+
+    sp = pr.species
+    s = OrcRovsSpec()
+    s.P, s.S, s.C, s.T, s.R, s.N = (pr.n_patches, len(sp), len(pr.flow), pr.n_years, pr.dim_res,
+                                    pr.n_paths)
+    s.init_pop = arr(np.stack([x.init_pop for x in sp]))
+    s.capacity = arr(np.stack([x.capacity for x in sp]))
+    s.trans = arr(np.stack([np.asarray(x.trans).T for x in sp]))
+    s.surv = arr(np.stack([np.asarray(x.surv).transpose(0, 2, 1) for x in sp]))
+    s.params = arr(np.stack([x.params for x in sp]))
+    s.flow = arr(pr.flow)
+    prices = list(pr.fuels) + list(pr.commodities)
+    s.n_fuels, s.n_comm = len(pr.fuels), len(pr.commodities)
+    s.unc = arr(np.array([[q.kind, q.x0, q.mean, q.sd, q.reversion, q.jump_size, q.jump_prob]
+                          for q in prices]).reshape(-1))
+    s.ore_mean, s.ore_sd = arr(pr.ore_mean), arr(pr.ore_sd)
+    s.n_veh = len(pr.fuel_cost)
+    s.fuel_cost = arr(pr.fuel_cost)
+    s.fuel_idx = arr(pr.fuel_idx, np.int32)
+    s.unit_cost, s.unit_revenue, s.dt, s.rrr = pr.unit_cost, pr.unit_revenue, pr.time_step, pr.rrr
+    s.scaling, s.hysteresis, s.seed, s.road_id = (pr.profit_scaling, pr.hysteresis, pr.seed,
+                                                  pr.road_id)
+    return s
+
+
+def _rovs_lib():
+    h = lib()
+    h.orc_rovs_forward.argtypes = [C.POINTER(OrcRovsSpec), _ip, _dp, _dp, _dp]
+    h.orc_rovs_forward.restype = C.c_int
+    h.orc_rovs_backward.argtypes = [C.POINTER(OrcRovsSpec), _ip, _dp, _dp, _dp, _dp, _ip, _dp]
+    h.orc_rovs_backward.restype = C.c_int
+    return h
+
+
+def rovs_forward(pr):
+    """(controls [T][N], total_pops [S][T+1][N], aars [S][T][C][N], profit [T][N])."""
+    keep = []
+    s = _rovs_spec(pr, keep)
+    T, N, S, Cc = pr.n_years, pr.n_paths, len(pr.species), len(pr.flow)
+    ctl = np.zeros((T, N), dtype=np.int32)
+    tp = np.zeros((S, T + 1, N))
+    aa = np.zeros((S, T, Cc, N))
+    up = np.zeros((T, N))
+    rc = _rovs_lib().orc_rovs_forward(C.byref(s), ctl.ctypes.data_as(_ip), tp.ctypes.data_as(_dp),
+                                      aa.ctypes.data_as(_dp), up.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError(f"orc_rovs_forward failed: {rc}")
+    return ctl, tp, aa, up
+
+
+def rovs_backward(pr, controls, total_pops, aars, profit):
+    """(V [T][N], opt_control [T][N], {value, sd, control})."""
+    keep = []
+    s = _rovs_spec(pr, keep)
+    T, N = pr.n_years, pr.n_paths
+    ctl = np.ascontiguousarray(controls, dtype=np.int32)
+    tp = np.ascontiguousarray(total_pops, dtype=np.float64)
+    aa = np.ascontiguousarray(aars, dtype=np.float64)
+    up = np.ascontiguousarray(profit, dtype=np.float64)
+    V = np.zeros((T, N))
+    oc = np.zeros((T, N), dtype=np.int32)
+    out = np.zeros(3)
+    rc = _rovs_lib().orc_rovs_backward(C.byref(s), ctl.ctypes.data_as(_ip), tp.ctypes.data_as(_dp),
+                                       aa.ctypes.data_as(_dp), up.ctypes.data_as(_dp),
+                                       V.ctypes.data_as(_dp), oc.ctypes.data_as(_ip),
+                                       out.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError(f"orc_rovs_backward failed: {rc}")
+    return V, oc, {"value": out[0], "sd": out[1], "control": int(out[2])}

@@ -1,0 +1,373 @@
+/*
+ * CPU oracle: species-state ROV, the reference's ALGO4.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Restates, in FP64 and one path at
+ * a time:
+ *   - forwardPathKernel (include/gpuCode/SimulateGPU.cu:789-956) with
+ *     randControls (:720-730): each path's populations moved under its drawn
+ *     control, the surviving fraction under every control, the growth-rate
+ *     SDE (jump probability from its own field) and n = n'(1 + g (K - n') /
+ *     K / 100), g = grMean + z local_variability; prices on the laws of the
+ *     ROV oracle;
+ *   - computePathStates' unit profit (:985-1002) with the ore composition
+ *     drawn per path, year and commodity;
+ *   - the ALGO4 backward loop (:4401-4998): allocateXYRegressionData
+ *     (:1010-1068), the ranges of computeStateMinMax (:1073-1147), the scaled
+ *     kNN search (:4582-4607) with k = 10 (ln n + 1) (:4522-4525), the
+ *     local-linear grid fit (multiLocLinReg, as knn_oracle.c), and
+ *     backwardInduction's decision (:2333-2784) with multilinear
+ *     interpolation of the grids; firstPeriodInduction (:2210-2330).
+ * The library's decisions (documented with its API) are followed where the
+ * reference is inconsistent.
+ */
+#pragma GCC optimize("fp-contract=off")
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+static double rv_u01(uint32_t x) { return ((double)(x >> 9) + 0.5) / 8388608.0; }
+static const double kRvTwoPi = 6.283185307179586476925286766559;
+
+static void rv_words(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t tag,
+                     uint32_t out[4]) {
+  uint32_t ctr[4], key[2];
+  ctr[0] = c0;
+  ctr[1] = c1;
+  ctr[2] = c2;
+  ctr[3] = tag;
+  key[0] = (uint32_t)(seed & 0xffffffffu);
+  key[1] = (uint32_t)(seed >> 32);
+  orc_philox4x32_10(ctr, key, out);
+}
+
+static void rv_normal4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t tag,
+                       double z[4]) {
+  uint32_t x[4];
+  double r;
+  rv_words(seed, c0, c1, c2, tag, x);
+  r = sqrt(-2.0 * log(rv_u01(x[0])));
+  z[0] = r * cos(kRvTwoPi * rv_u01(x[1]));
+  z[1] = r * sin(kRvTwoPi * rv_u01(x[1]));
+  r = sqrt(-2.0 * log(rv_u01(x[2])));
+  z[2] = r * cos(kRvTwoPi * rv_u01(x[3]));
+  z[3] = r * sin(kRvTwoPi * rv_u01(x[3]));
+}
+
+static uint32_t rv_tag(uint32_t road, uint32_t species, uint32_t kind) {
+  return (road << 8) | ((species & 63u) << 2) | (kind & 3u);
+}
+
+int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_pops, double* aars,
+                     double* profit) {
+  const int P = s->P, S = s->S, C = s->C, T = s->T, U = s->n_fuels + s->n_comm;
+  const int64_t N = s->N;
+  const uint32_t tag_rov = rv_tag(s->road_id, 0, 2), tag_ore = rv_tag(s->road_id, 0, 3);
+  double* price = (double*)malloc(sizeof(double) * (size_t)(T + 1) * (U > 0 ? U : 1));
+  double* n = (double*)malloc(sizeof(double) * (size_t)P);
+  double* y = (double*)malloc(sizeof(double) * (size_t)P * C);
+  if (!price || !n || !y) {
+    free(price);
+    free(n);
+    free(y);
+    return -1;
+  }
+  for (int64_t p = 0; p < N; ++p) {
+    /* prices and controls (the ROV stream: blocks 0-1 diffusion normals, 2-3
+       jump-size normals, 4-5 jump uniforms, 6 the control uniform) */
+    double x[8];
+    for (int u = 0; u < U; ++u) {
+      x[u] = s->unc[u * 7 + 1];
+      price[u] = x[u];
+    }
+    for (int t = 0; t < T; ++t) {
+      double z[8], zj[8], uj[8];
+      for (int b = 0; 4 * b < U; ++b) {
+        double w[4];
+        uint32_t r[4];
+        rv_normal4(s->seed, (uint32_t)p, (uint32_t)t, (uint32_t)b, tag_rov, w);
+        for (int i = 0; i < 4 && 4 * b + i < U; ++i) z[4 * b + i] = w[i];
+        rv_normal4(s->seed, (uint32_t)p, (uint32_t)t, (uint32_t)(2 + b), tag_rov, w);
+        for (int i = 0; i < 4 && 4 * b + i < U; ++i) zj[4 * b + i] = w[i];
+        rv_words(s->seed, (uint32_t)p, (uint32_t)t, (uint32_t)(4 + b), tag_rov, r);
+        for (int i = 0; i < 4 && 4 * b + i < U; ++i) uj[4 * b + i] = rv_u01(r[i]);
+      }
+      for (int u = 0; u < U; ++u) {
+        const double* q = s->unc + u * 7;
+        const int kind = (int)q[0];
+        const double xv = x[u];
+        if (kind == 1) {
+          x[u] = xv + q[4] * (q[2] - xv) * s->dt + q[3] * sqrt(s->dt) * z[u];
+        } else if (kind == 2) {
+          x[u] = xv * exp((q[2] - 0.5 * q[3] * q[3]) * s->dt + q[3] * sqrt(s->dt) * z[u]);
+        } else {
+          const double jump = uj[u] < q[6] ? 1.0 : 0.0;
+          const double jsz = zj[u] * q[5] * q[5] - q[5] * q[5] / 2.0;
+          x[u] = xv + q[4] * (q[2] - xv) * s->dt + xv * q[3] * z[u] + (exp(jsz) - 1.0) * xv * jump;
+        }
+        price[(size_t)(t + 1) * U + u] = x[u];
+      }
+      uint32_t r[4];
+      rv_words(s->seed, (uint32_t)p, (uint32_t)t, 6u, tag_rov, r);
+      int c = (int)(rv_u01(r[0]) * C);
+      if (c >= C) c = C - 1;
+      controls[(size_t)t * N + p] = c;
+    }
+    /* unit profit */
+    for (int t = 0; t < T; ++t) {
+      double fuel = 0.0, ore = 0.0;
+      for (int v = 0; v < s->n_veh; ++v) fuel += s->fuel_cost[v] * price[(size_t)t * U + s->fuel_idx[v]];
+      for (int b = 0; 4 * b < s->n_comm; ++b) {
+        double w[4];
+        rv_normal4(s->seed, (uint32_t)p, (uint32_t)t, (uint32_t)b, tag_ore, w);
+        for (int i = 0; i < 4 && 4 * b + i < s->n_comm; ++i) {
+          const int c = 4 * b + i;
+          ore += (s->ore_mean[c] + s->ore_sd[c] * w[i]) * price[(size_t)t * U + s->n_fuels + c];
+        }
+      }
+      profit[(size_t)t * N + p] = s->unit_cost + fuel - s->unit_revenue * ore;
+    }
+    /* species */
+    for (int sp = 0; sp < S; ++sp) {
+      const double* prm = s->params + sp * 8;
+      const double* K = s->capacity + (size_t)sp * P;
+      const double* tr = s->trans + (size_t)sp * P * P;
+      const uint32_t tag0 = rv_tag(s->road_id, (uint32_t)sp, 0);
+      const uint32_t tag1 = rv_tag(s->road_id, (uint32_t)sp, 1);
+      double tot = 0.0, grm = prm[0];
+      for (int j = 0; j < P; ++j) {
+        n[j] = s->init_pop[(size_t)sp * P + j];
+        tot += n[j];
+      }
+      total_pops[((size_t)sp * (T + 1)) * N + p] = tot;
+      for (int t = 0; t < T; ++t) {
+        const int ctl = controls[(size_t)t * N + p];
+        for (int c = 0; c < C; ++c) {
+          const double* sv = s->surv + ((size_t)sp * C + c) * P * P;
+          double rs = 0.0;
+          for (int j = 0; j < P; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < P; ++k)
+              acc += n[k] * (tr[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+            y[(size_t)c * P + j] = acc;
+            rs += acc;
+          }
+          aars[(((size_t)sp * T + t) * C + c) * N + p] = rs / tot;
+        }
+        double sc[3];
+        orc_path_scalars(s->seed, tag1, (uint32_t)p, (uint32_t)t, sc);
+        const double jump = sc[2] < prm[6] ? 1.0 : 0.0;
+        const double jsz = sc[1] * prm[5] * prm[5] - prm[5] * prm[5] / 2.0;
+        grm = grm + prm[4] * (prm[1] - grm) * s->dt + grm * (sc[0] * prm[2]) +
+              (exp(jsz) - 1.0) * grm * jump;
+        tot = 0.0;
+        for (int j = 0; j < P; ++j) {
+          const double xv = y[(size_t)ctl * P + j];
+          const double z = orc_patch_normal(s->seed, tag0, (uint32_t)p, (uint32_t)t, (uint32_t)j, 0);
+          const double g = grm + z * prm[7];
+          n[j] = xv * (1.0 + g * (K[j] - xv) / K[j] / 100.0);
+          tot += n[j];
+        }
+        total_pops[((size_t)sp * (T + 1) + t + 1) * N + p] = tot;
+      }
+    }
+  }
+  free(price);
+  free(n);
+  free(y);
+  return 0;
+}
+
+static double rv_adj(const orc_rovs_spec* s, const double* tp, const double* aars, int sp, int t,
+                     int c, int64_t p) {
+  const int64_t N = s->N;
+  return tp[((size_t)sp * (s->T + 1) + t) * N + p] *
+         aars[(((size_t)sp * s->T + t) * s->C + c) * N + p];
+}
+
+int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const double* tp,
+                      const double* aars, const double* profit, double* V, int32_t* optc,
+                      double out[3]) {
+  const int S = s->S, C = s->C, T = s->T, D = s->S + 1, R = s->R;
+  const int64_t N = s->N;
+  size_t G = 1;
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  const double disc = 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  double* xs = (double*)malloc(sizeof(double) * (size_t)C * N * D);
+  double* us = (double*)malloc(sizeof(double) * (size_t)C * N * D);
+  double* ys = (double*)malloc(sizeof(double) * (size_t)C * N);
+  double* grid = (double*)malloc(sizeof(double) * (size_t)C * gsz);
+  double* qtab = (double*)malloc(sizeof(double) * (size_t)D * R);
+  int64_t cnt[8];
+  if (!xs || !us || !ys || !grid || !qtab) {
+    free(xs);
+    free(us);
+    free(ys);
+    free(grid);
+    free(qtab);
+    return -1;
+  }
+  /* terminal step */
+  for (int64_t p = 0; p < N; ++p) {
+    const int t = T - 1;
+    const double up = profit[(size_t)t * N + p];
+    double best = 0.0;
+    int bc = 0;
+    for (int c = 0; c < C; ++c) {
+      int valid = 1;
+      for (int sp = 0; sp < S && c > 0; ++sp)
+        if (rv_adj(s, tp, aars, sp, t, c, p) < s->params[sp * 8 + 3]) valid = 0;
+      const double pay = valid ? fmin(s->flow[c] * up, 0.0) : 0.0;
+      if (c == 0) {
+        best = pay;
+      } else if (valid && pay < best) {
+        best = pay;
+        bc = c;
+      }
+    }
+    V[(size_t)t * N + p] = best;
+    optc[(size_t)t * N + p] = bc;
+  }
+  for (int t = T - 2; t >= 1; --t) {
+    double lo[8], hi[8];
+    for (int c = 0; c < C; ++c) cnt[c] = 0;
+    for (int64_t p = 0; p < N; ++p) {
+      double x[8];
+      int valid = 1;
+      for (int sp = 0; sp < S; ++sp) {
+        x[sp] = rv_adj(s, tp, aars, sp, t, C - 1, p);
+        if (x[sp] < s->params[sp * 8 + 3]) valid = 0;
+      }
+      x[S] = profit[(size_t)t * N + p];
+      const int c = controls[(size_t)t * N + p];
+      if (!(valid || c == 0)) continue;
+      for (int d = 0; d < D; ++d) xs[((size_t)c * N + cnt[c]) * D + d] = x[d];
+      ys[(size_t)c * N + cnt[c]] = V[(size_t)(t + 1) * N + p];
+      ++cnt[c];
+    }
+    for (int d = 0; d < D; ++d) {
+      lo[d] = INFINITY;
+      hi[d] = -INFINITY;
+      for (int c = 0; c < C; ++c) {
+        if (c > 0 && cnt[c] <= D + 1) continue;
+        for (int64_t j = 0; j < cnt[c]; ++j) {
+          lo[d] = fmin(lo[d], xs[((size_t)c * N + j) * D + d]);
+          hi[d] = fmax(hi[d], xs[((size_t)c * N + j) * D + d]);
+        }
+      }
+      if (!(lo[d] <= hi[d])) lo[d] = hi[d] = 0.0;
+    }
+#define RV_SCALE(dd, v)                                                                    \
+  (!((hi[dd] - lo[dd]) > 0.0) ? 0.0                                                        \
+   : (dd) == D - 1             ? ((v) - lo[dd]) / (s->scaling * (hi[dd] - lo[dd]))         \
+                               : ((v) - lo[dd]) / (hi[dd] - lo[dd]))
+    for (int c = 0; c < C; ++c)
+      for (int64_t j = 0; j < cnt[c]; ++j)
+        for (int d = 0; d < D; ++d)
+          us[((size_t)c * N + j) * D + d] = RV_SCALE(d, xs[((size_t)c * N + j) * D + d]);
+    for (int d = 0; d < D; ++d)
+      for (int i = 0; i < R; ++i) {
+        const double q = (double)i * (hi[d] - lo[d]) / (double)(R - 1) + lo[d];
+        qtab[d * R + i] = RV_SCALE(d, q);
+        for (int c = 0; c < C; ++c) grid[(size_t)c * gsz + (size_t)d * R + i] = q;
+      }
+#undef RV_SCALE
+    for (int c = 0; c < C; ++c) {
+      const int64_t nc = cnt[c];
+      int k = nc > 3 ? (int)(10.0 * (log((double)nc) + 1.0)) : 0;
+      if (k > 128) k = 128;
+      if (k > nc) k = (int)nc;
+      double* val = grid + (size_t)c * gsz + (size_t)D * R;
+      if (nc < 3 || k < 1) {
+        for (size_t g = 0; g < G; ++g) val[g] = 0.0;
+      } else if (orc_knn_fit_scaled((int32_t)nc, D, us + (size_t)c * N * D, ys + (size_t)c * N, R,
+                                    k, qtab, val) != 0) {
+        free(xs);
+        free(us);
+        free(ys);
+        free(grid);
+        free(qtab);
+        return -2;
+      }
+    }
+    for (int64_t p = 0; p < N; ++p) {
+      double x[8];
+      for (int sp = 0; sp < S; ++sp) x[sp] = rv_adj(s, tp, aars, sp, t, C - 1, p);
+      x[S] = profit[(size_t)t * N + p];
+      double best = 0.0;
+      int bc = 0;
+      for (int c = 0; c < C; ++c) {
+        int valid = 1;
+        for (int sp = 0; sp < S && c > 0; ++sp)
+          if (rv_adj(s, tp, aars, sp, t, c, p) < s->params[sp * 8 + 3]) valid = 0;
+        double pay = 0.0;
+        if (valid) {
+          double cont;
+          orc_knn_eval(D, R, grid + (size_t)c * gsz, 1, x, &cont);
+          pay = s->flow[c] * x[S] + fmin(cont * disc, 0.0);
+        }
+        if (c == 0) {
+          best = pay;
+        } else if (valid && pay < s->hysteresis * best) {
+          best = pay;
+          bc = c;
+        }
+      }
+      V[(size_t)t * N + p] = best;
+      optc[(size_t)t * N + p] = bc;
+    }
+  }
+  /* time 0 */
+  {
+    double sums[8], ns[8], up = 0.0, best = 0.0;
+    int bc = -1;
+    for (int c = 0; c < C; ++c) sums[c] = ns[c] = 0.0;
+    for (int64_t p = 0; p < N; ++p) {
+      const int c = controls[p];
+      sums[c] += V[(size_t)N + p];
+      ns[c] += 1.0;
+      up += profit[p];
+    }
+    up /= (double)N;
+    for (int c = 0; c < C; ++c) {
+      if (ns[c] == 0.0) continue;
+      int valid = 1;
+      for (int sp = 0; sp < S && c > 0; ++sp)
+        if (rv_adj(s, tp, aars, sp, 0, c, 0) < s->params[sp * 8 + 3]) valid = 0;
+      if (!valid) continue;
+      const double pay = fmin(sums[c] / ns[c] * disc + s->flow[c] * up, 0.0);
+      if (bc < 0 || pay < best) {
+        best = pay;
+        bc = c;
+      }
+    }
+    if (bc < 0) bc = 0;
+    double m = 0.0, q = 0.0;
+    const double base = s->flow[bc] * up;
+    for (int64_t p = 0; p < N; ++p)
+      if (controls[p] == bc) m += V[(size_t)N + p] * disc + base;
+    const double n = ns[bc] > 0.0 ? ns[bc] : 1.0;
+    m /= n;
+    for (int64_t p = 0; p < N; ++p)
+      if (controls[p] == bc) {
+        const double d = V[(size_t)N + p] * disc + base - m;
+        q += d * d;
+      }
+    for (int64_t p = 0; p < N; ++p) {
+      V[p] = best;
+      optc[p] = bc;
+    }
+    out[0] = best;
+    out[1] = sqrt(q / n);
+    out[2] = (double)bc;
+  }
+  free(xs);
+  free(us);
+  free(ys);
+  free(grid);
+  free(qtab);
+  return 0;
+}

@@ -40,6 +40,7 @@ EXTRA_FLAGS = {
     "roadmc_road.cpp": ["-ffp-contract=off"],
     "knn_kernels.hip": ["-ffp-contract=off"],
     "roadmc_knn.cpp": ["-ffp-contract=off"],
+    "rovs_kernels.hip": ["-ffp-contract=off"],
 }
 
 

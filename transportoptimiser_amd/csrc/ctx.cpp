@@ -102,7 +102,13 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->road_bufs.c,    &ctx->road_bufs.d,   &ctx->road_bufs.e,
                         &ctx->road_bufs.f,    &ctx->road_bufs.g,   &ctx->road_bufs.h,
                         &ctx->knn_bufs.a,     &ctx->knn_bufs.b,    &ctx->knn_bufs.c,
-                        &ctx->knn_bufs.d};
+                        &ctx->knn_bufs.d,     &ctx->rovs_bufs.mpack, &ctx->rovs_bufs.cols,
+                        &ctx->rovs_bufs.prices, &ctx->rovs_bufs.controls, &ctx->rovs_bufs.tp,
+                        &ctx->rovs_bufs.aars, &ctx->rovs_bufs.profit, &ctx->rovs_bufs.V,
+                        &ctx->rovs_bufs.optc, &ctx->rovs_bufs.adj, &ctx->rovs_bufs.flag,
+                        &ctx->rovs_bufs.blk,  &ctx->rovs_bufs.xs,  &ctx->rovs_bufs.ys,
+                        &ctx->rovs_bufs.u,    &ctx->rovs_bufs.small, &ctx->rovs_bufs.grid,
+                        &ctx->rovs_bufs.args};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
   if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
   rmc_buf_free(ctx->rov_graph.param);

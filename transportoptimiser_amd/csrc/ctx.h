@@ -47,6 +47,10 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d;
   } knn_bufs;
+  struct {
+    rmc_devbuf mpack, cols, prices, controls, tp, aars, profit, V, optc, adj, flag, blk, xs, ys,
+        u, small, grid, args;
+  } rovs_bufs;
 };
 
 int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);

@@ -24,6 +24,8 @@
  *            (z_brownian, z_jumpsize) = box_muller(u(x0), u(x1)),
  *            u_jump = u(x2)
  *   kind 2   ROV streams, laid out in rov_kernels.hip.
+ *   kind 3   ore composition of the species-state ROV: z(path, year, c) =
+ *            normal4(philox({path, year, c >> 2, tag}))[c & 3].
  *   u(x)     = ((x >> 9) + 0.5) * 2^-23, exact in binary32, in (0, 1).
  *   normal4  = two Box-Muller pairs: (u0, u1) -> (r cos 2 pi u1, r sin 2 pi u1),
  *              r = sqrt(-2 ln u0); (u2, u3) likewise.
@@ -54,6 +56,7 @@
 #define RMC_KIND_PATCH 0u
 #define RMC_KIND_PATH 1u
 #define RMC_KIND_ROV 2u
+#define RMC_KIND_ORE 3u
 
 typedef struct {
   uint32_t x, y, z, w;

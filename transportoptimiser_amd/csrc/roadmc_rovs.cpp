@@ -1,0 +1,339 @@
+// C-ABI host code for the species-state ROV (rovs_kernels.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+#include "knn_kernels.h"
+#include "mte_kernels.h"
+#include "rng.h"
+#include "roadmc.h"
+#include "rov_kernels.h"
+#include "rovs_kernels.h"
+
+namespace {
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int validate(roadmc_ctx* ctx, const roadmc_rovs_problem* p, const roadmc_rovs_result* r) {
+  auto bad = [&](const char* m) { return rmc_fail(ctx, ROADMC_ERR_INVALID, m); };
+  if (!p || !r) return bad("problem or result is NULL");
+  if (!r->value) return bad("result.value is NULL");
+  if (p->n_patches < 1 || p->n_patches > roadmc::kRovsMaxPatches) return bad("n_patches must be in [1, 512]");
+  if (p->n_species < 1 || p->n_species + 1 > ROADMC_KNN_MAX_DIMS || !p->species)
+    return bad("n_species must be in [1, 7]");
+  if (p->n_controls < 1 || p->n_controls > ROADMC_MAX_CONTROLS || !p->flow)
+    return bad("n_controls must be in [1, 8] with flow rates");
+  if (p->n_fuels < 0 || p->n_commodities < 0 || p->n_fuels + p->n_commodities > ROADMC_MAX_UNCERTAINTIES)
+    return bad("n_fuels + n_commodities must be <= 8");
+  if (p->n_fuels + p->n_commodities > 0 && !p->uncertainties) return bad("uncertainties is NULL");
+  if (p->n_commodities > 0 && (!p->ore_mean || !p->ore_sd)) return bad("ore composition is NULL");
+  if (p->n_vehicles < 0 || p->n_vehicles > roadmc::kRovsMaxFuel ||
+      (p->n_vehicles > 0 && (!p->fuel_cost || !p->fuel_idx)))
+    return bad("n_vehicles must be in [0, 8] with fuel costs and indices");
+  for (int v = 0; v < p->n_vehicles; ++v)
+    if (p->fuel_idx[v] < 0 || p->fuel_idx[v] >= std::max(p->n_fuels, 1) ||
+        (p->n_fuels == 0))
+      return bad("fuel_idx out of range");
+  if (p->n_years < 3) return bad("n_years must be >= 3");
+  if (p->n_paths < 1 || p->n_paths > (1 << 24)) return bad("n_paths must be in [1, 2^24]");
+  if (p->dim_res < 2) return bad("dim_res must be >= 2");
+  long long G = 1;
+  for (int d = 0; d <= p->n_species; ++d) {
+    G *= p->dim_res;
+    if (G > (1LL << 24)) return bad("dim_res^(n_species+1) must be <= 2^24");
+  }
+  if (!(p->time_step > 0) || !(p->profit_scaling > 0) || !std::isfinite(p->rrr) ||
+      !std::isfinite(p->hysteresis) || !std::isfinite(p->unit_cost) ||
+      !std::isfinite(p->unit_revenue))
+    return bad("time_step, profit_scaling > 0 and finite scalars required");
+  for (int s = 0; s < p->n_species; ++s) {
+    const roadmc_rovs_species& sp = p->species[s];
+    if (!sp.init_pop || !sp.capacity || !sp.trans || !sp.surv) return bad("species array is NULL");
+    for (int i = 0; i < p->n_patches; ++i)
+      if (!(sp.capacity[i] > 0) || !std::isfinite(sp.init_pop[i]))
+        return bad("capacities must be positive, initial populations finite");
+  }
+  return ROADMC_OK;
+}
+
+}  // namespace
+
+extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
+                               roadmc_rovs_result* res) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  int rc = validate(ctx, pr, res);
+  if (rc) return rc;
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const int P = pr->n_patches, S = pr->n_species, C = pr->n_controls, T = pr->n_years;
+  const int N = (int)pr->n_paths, U = pr->n_fuels + pr->n_commodities, D = S + 1, R = pr->dim_res;
+  const int Pp = round_up(P, 32), Pk = round_up(P, 16);
+  size_t G = 1;
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  const int nblk = (N + roadmc::kRovsScanBlock - 1) / roadmc::kRovsScanBlock;
+  const uint32_t k0 = (uint32_t)(pr->seed & 0xffffffffu), k1 = (uint32_t)(pr->seed >> 32);
+
+  // Device workspace.
+  auto& B = ctx->rovs_bufs;
+  const size_t Uc = std::max(U, 1);
+  struct Need {
+    rmc_devbuf* b;
+    size_t bytes;
+  } need[] = {
+      {&B.mpack, (size_t)S * C * Pp * Pk * sizeof(float)},
+      {&B.cols, (size_t)S * 3 * Pp * sizeof(float)},
+      {&B.prices, (size_t)(T + 1) * Uc * N * sizeof(double)},
+      {&B.controls, (size_t)T * N * sizeof(int)},
+      {&B.tp, (size_t)S * (T + 1) * N * sizeof(float)},
+      {&B.aars, (size_t)S * T * C * N * sizeof(float)},
+      {&B.profit, (size_t)T * N * sizeof(double)},
+      {&B.V, (size_t)T * N * sizeof(double)},
+      {&B.optc, (size_t)T * N * sizeof(int)},
+      {&B.adj, res->adj_pops ? (size_t)T * S * N * sizeof(double) : 8},
+      {&B.flag, (size_t)N * sizeof(int)},
+      {&B.blk, (size_t)2 * nblk * C * sizeof(int) + 64},
+      {&B.xs, (size_t)C * N * D * sizeof(double)},
+      {&B.ys, (size_t)C * N * sizeof(double)},
+      {&B.u, (size_t)C * N * D * sizeof(double)},
+      {&B.small, (size_t)(64 * D * 2 + D * 2 + D * R + 8) * sizeof(double)},
+      {&B.grid, (size_t)C * gsz * sizeof(double)},
+      {&B.args, sizeof(roadmc::RovsStepArgs)},
+  };
+  for (auto& n : need)
+    if ((rc = rmc_buf_ensure(ctx, *n.b, n.bytes))) return rc;
+  hipStream_t s = ctx->stream;
+
+  // Packed movement matrices trans o surv_c (FP32, MFMA B order) and columns.
+  std::vector<float> mp((size_t)S * C * Pp * Pk, 0.f), cols((size_t)S * 3 * Pp);
+  std::vector<float> init_total(S);
+  for (int sp = 0; sp < S; ++sp) {
+    const roadmc_rovs_species& sc = pr->species[sp];
+    for (int c = 0; c < C; ++c) {
+      float* m = mp.data() + ((size_t)sp * C + c) * Pp * Pk;
+      const double* sv = sc.surv + (size_t)c * P * P;
+      for (int k = 0; k < P; ++k)
+        for (int j = 0; j < P; ++j)
+          m[roadmc::mte_pack_index(false, Pk, k, j)] =
+              (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+    }
+    float* cl = cols.data() + (size_t)sp * 3 * Pp;
+    double tot = 0.0;
+    for (int j = 0; j < Pp; ++j) {
+      const bool in = j < P;
+      cl[j] = in ? (float)sc.capacity[j] : 1.f;
+      cl[Pp + j] = in ? (float)(1.0 / sc.capacity[j]) : 1.f;
+      cl[2 * Pp + j] = in ? (float)sc.init_pop[j] : 0.f;
+      if (in) tot += (double)(float)sc.init_pop[j];
+    }
+    init_total[sp] = (float)tot;
+  }
+  RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float), hipMemcpyHostToDevice, s));
+
+  // Prices and drawn controls (the ROV forward pass, independent prices).
+  roadmc::RovForwardArgs fa{};
+  fa.N = N;
+  fa.T = T;
+  fa.U = U;
+  fa.C = C;
+  fa.dt = pr->time_step;
+  for (int u = 0; u < U; ++u) {
+    const roadmc_uncertainty& q = pr->uncertainties[u];
+    double* pp = fa.par + u * 7;
+    pp[0] = q.kind;
+    pp[1] = q.x0;
+    pp[2] = q.mean;
+    pp[3] = q.sd;
+    pp[4] = q.reversion;
+    pp[5] = q.jump_size;
+    pp[6] = q.jump_prob;
+    fa.chol[u * roadmc::kRovMaxU + u] = 1.0;
+  }
+  fa.tag = rmc_tag(pr->road_id, 0, RMC_KIND_ROV);
+  fa.k0 = k0;
+  fa.k1 = k1;
+  fa.states = static_cast<double*>(B.prices.ptr);
+  fa.controls = static_cast<int*>(B.controls.ptr);
+  RMC_HIP(ctx, roadmc::launch_rov_forward(fa, s));
+
+  // Species populations under the drawn controls.
+  for (int sp = 0; sp < S; ++sp) {
+    roadmc::RovsForwardArgs a{};
+    a.N = N;
+    a.T = T;
+    a.P = P;
+    a.Pp = Pp;
+    a.Pk = Pk;
+    a.C = C;
+    a.mpack = static_cast<const float*>(B.mpack.ptr) + (size_t)sp * C * Pp * Pk;
+    a.cols = static_cast<const float*>(B.cols.ptr) + (size_t)sp * 3 * Pp;
+    for (int i = 0; i < 8; ++i) a.params[i] = pr->species[sp].params[i];
+    a.dt = pr->time_step;
+    a.init_total = init_total[sp];
+    a.tag0 = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATCH);
+    a.tag1 = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATH);
+    a.key0 = k0;
+    a.key1 = k1;
+    a.controls = static_cast<const int*>(B.controls.ptr);
+    a.total_pops = static_cast<float*>(B.tp.ptr) + (size_t)sp * (T + 1) * N;
+    a.aars = static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N;
+    RMC_HIP(ctx, roadmc::launch_rovs_forward(a, s));
+  }
+
+  // Unit profit per path and year.
+  roadmc::RovsProfitArgs pa{};
+  pa.N = N;
+  pa.T = T;
+  pa.U = U;
+  pa.n_vehicles = pr->n_vehicles;
+  pa.n_fuels = pr->n_fuels;
+  pa.n_comm = pr->n_commodities;
+  pa.unit_cost = pr->unit_cost;
+  pa.unit_revenue = pr->unit_revenue;
+  for (int v = 0; v < pr->n_vehicles; ++v) {
+    pa.fuel_cost[v] = pr->fuel_cost[v];
+    pa.fuel_idx[v] = pr->fuel_idx[v];
+  }
+  for (int c = 0; c < pr->n_commodities; ++c) {
+    pa.ore_mean[c] = pr->ore_mean[c];
+    pa.ore_sd[c] = pr->ore_sd[c];
+  }
+  pa.tag = rmc_tag(pr->road_id, 0, RMC_KIND_ORE);
+  pa.key0 = k0;
+  pa.key1 = k1;
+  pa.prices = static_cast<const double*>(B.prices.ptr);
+  pa.profit = static_cast<double*>(B.profit.ptr);
+  RMC_HIP(ctx, roadmc::launch_rovs_profit(pa, s));
+
+  // Backward induction.
+  roadmc::RovsStepArgs h{};
+  h.N = N;
+  h.T = T;
+  h.S = S;
+  h.C = C;
+  h.D = D;
+  h.R = R;
+  h.disc = 1.0 / (1.0 + pr->rrr * pr->time_step / 100.0);
+  h.hysteresis = pr->hysteresis;
+  h.scaling = pr->profit_scaling;
+  for (int sp = 0; sp < S; ++sp) h.thr[sp] = pr->species[sp].params[3];
+  for (int c = 0; c < C; ++c) h.flow[c] = pr->flow[c];
+  h.total_pops = static_cast<const float*>(B.tp.ptr);
+  h.aars = static_cast<const float*>(B.aars.ptr);
+  h.profit = static_cast<const double*>(B.profit.ptr);
+  h.controls = static_cast<const int*>(B.controls.ptr);
+  h.V = static_cast<double*>(B.V.ptr);
+  h.optc = static_cast<int*>(B.optc.ptr);
+  h.adj_pops = res->adj_pops ? static_cast<double*>(B.adj.ptr) : nullptr;
+  h.flag = static_cast<int*>(B.flag.ptr);
+  h.blk_count = static_cast<int*>(B.blk.ptr);
+  h.blk_off = h.blk_count + (size_t)nblk * C;
+  h.count = h.blk_off + (size_t)nblk * C;
+  h.xs = static_cast<double*>(B.xs.ptr);
+  h.ys = static_cast<double*>(B.ys.ptr);
+  h.u = static_cast<double*>(B.u.ptr);
+  h.mm = static_cast<double*>(B.small.ptr);
+  h.range = h.mm + 64 * D * 2;
+  h.qtab = h.range + D * 2;
+  double* d_out = h.qtab + (size_t)D * R;
+  h.grid = static_cast<double*>(B.grid.ptr);
+  auto* d_args = static_cast<roadmc::RovsStepArgs*>(B.args.ptr);
+  RMC_HIP(ctx, hipMemcpyAsync(d_args, &h, sizeof(h), hipMemcpyHostToDevice, s));
+
+  std::vector<double> grids_host;
+  if (res->grids) grids_host.assign((size_t)T * C * gsz, 0.0);
+  RMC_HIP(ctx, roadmc::launch_rovs_terminal(d_args, N, s));
+  std::vector<int> counts(C);
+  for (int t = T - 2; t >= 1; --t) {
+    RMC_HIP(ctx, roadmc::launch_rovs_collect(d_args, N, t, s));
+    RMC_HIP(ctx, hipMemcpyAsync(counts.data(), h.count, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RMC_HIP(ctx, hipStreamSynchronize(s));
+    RMC_HIP(ctx, roadmc::launch_rovs_ranges(d_args, N, D, s));
+    RMC_HIP(ctx, roadmc::launch_rovs_normalise(d_args, N, D, R, s));
+    for (int c = 0; c < C; ++c) {
+      const int n = counts[c];
+      // k = 10 (ln n + 1) for more than 3 samples (SimulateGPU.cu:4522-4525).
+      int k = n > 3 ? (int)(10.0 * (std::log((double)n) + 1.0)) : 0;
+      k = std::min({k, roadmc::kKnnMaxK, n});
+      double* vals = h.grid + (size_t)c * gsz + (size_t)D * R;
+      if (n < 3 || k < 1) {
+        RMC_HIP(ctx, hipMemsetAsync(vals, 0, G * sizeof(double), s));
+        continue;
+      }
+      roadmc::KnnFitArgs ka{};
+      ka.n = n;
+      ka.d = D;
+      ka.R = R;
+      ka.k = k;
+      ka.clamp = 0;
+      ka.cap = std::min(k, (n + roadmc::kKnnThreads - 1) / roadmc::kKnnThreads);
+      ka.G = (long long)G;
+      ka.u = h.u + (size_t)c * N * D;
+      ka.y = h.ys + (size_t)c * N;
+      ka.qtab = h.qtab;
+      ka.val = vals;
+      RMC_HIP(ctx, roadmc::launch_knn_fit(ka, s));
+    }
+    RMC_HIP(ctx, roadmc::launch_rovs_decide(d_args, N, D, t, s));
+    if (res->grids)
+      RMC_HIP(ctx, hipMemcpyAsync(grids_host.data() + (size_t)t * C * gsz, h.grid,
+                                  (size_t)C * gsz * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  RMC_HIP(ctx, roadmc::launch_rovs_first(d_args, d_out, s));
+  double out3[3];
+  RMC_HIP(ctx, hipMemcpyAsync(out3, d_out, sizeof(out3), hipMemcpyDeviceToHost, s));
+
+  // Optional outputs, staged, then written.
+  std::vector<double> ce, adj, up, tp, aa;
+  std::vector<int32_t> oc, ctl;
+  std::vector<float> tpf, aaf;
+  if (res->cond_exp) {
+    ce.resize((size_t)T * N);
+    RMC_HIP(ctx, hipMemcpyAsync(ce.data(), h.V, ce.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (res->opt_control) {
+    oc.resize((size_t)T * N);
+    RMC_HIP(ctx, hipMemcpyAsync(oc.data(), h.optc, oc.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (res->adj_pops) {
+    adj.resize((size_t)T * S * N);
+    RMC_HIP(ctx, hipMemcpyAsync(adj.data(), h.adj_pops, adj.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (res->unit_profit) {
+    up.resize((size_t)T * N);
+    RMC_HIP(ctx, hipMemcpyAsync(up.data(), h.profit, up.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  if (res->controls) {
+    ctl.resize((size_t)T * N);
+    RMC_HIP(ctx, hipMemcpyAsync(ctl.data(), h.controls, ctl.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (res->total_pops) {
+    tpf.resize((size_t)S * (T + 1) * N);
+    RMC_HIP(ctx, hipMemcpyAsync(tpf.data(), h.total_pops, tpf.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  if (res->aars) {
+    aaf.resize((size_t)S * T * C * N);
+    RMC_HIP(ctx, hipMemcpyAsync(aaf.data(), h.aars, aaf.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  if (!std::isfinite(out3[0]) || !std::isfinite(out3[1]))
+    return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite value");
+  *res->value = out3[0];
+  if (res->value_sd) *res->value_sd = out3[1];
+  if (res->first_control) *res->first_control = (int32_t)out3[2];
+  if (res->cond_exp) std::memcpy(res->cond_exp, ce.data(), ce.size() * sizeof(double));
+  if (res->opt_control) std::memcpy(res->opt_control, oc.data(), oc.size() * sizeof(int32_t));
+  if (res->adj_pops) std::memcpy(res->adj_pops, adj.data(), adj.size() * sizeof(double));
+  if (res->unit_profit) std::memcpy(res->unit_profit, up.data(), up.size() * sizeof(double));
+  if (res->controls) std::memcpy(res->controls, ctl.data(), ctl.size() * sizeof(int32_t));
+  if (res->total_pops)
+    for (size_t i = 0; i < tpf.size(); ++i) res->total_pops[i] = tpf[i];
+  if (res->aars)
+    for (size_t i = 0; i < aaf.size(); ++i) res->aars[i] = aaf[i];
+  if (res->grids) std::memcpy(res->grids, grids_host.data(), grids_host.size() * sizeof(double));
+  return ROADMC_OK;
+}

@@ -1,0 +1,92 @@
+// Host-visible declarations of the species-state ROV kernels (rovs_kernels.hip).
+#ifndef ROADMC_ROVS_KERNELS_H
+#define ROADMC_ROVS_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace roadmc {
+
+constexpr int kRovsRows = 32;          // paths per forward workgroup
+constexpr int kRovsNT = 2;            // 32-column tiles per forward wave
+constexpr int kRovsMaxThreads = 512;  // 8 waves: 16 column tiles, P <= 512
+constexpr int kRovsMaxPatches = 512;
+constexpr int kRovsMaxC = 8;
+constexpr int kRovsMaxS = 7;       // S + 1 state dimensions <= 8
+constexpr int kRovsMaxFuel = 8;
+constexpr int kRovsMaxComm = 8;
+constexpr int kRovsScanBlock = 1024;
+
+// One species' forward pass: populations under the randomly drawn control of
+// each path and year, the surviving fraction under every control.
+struct RovsForwardArgs {
+  int N, T, P, Pp, Pk, C;
+  const float* mpack;   // [C][Pp * Pk] trans o surv_c in MFMA B order
+  const float* cols;    // [3][Pp]: K, 1/K, initial population (K = 1 on padding)
+  double params[8];     // growth current, mean, noise SD, threshold, reversion,
+                        // jump size, jump probability, local variability
+  double dt;
+  float init_total;
+  uint32_t tag0, tag1, key0, key1;
+  const int* controls;  // [T][N]
+  float* total_pops;    // [T+1][N]
+  float* aars;          // [T][C][N]
+};
+
+size_t rovs_forward_lds(int Pp, int C);
+hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s);
+
+// Unit profit per path and year: unit_cost + sum_v fuel_cost_v price_{fuel_idx_v}
+// - unit_revenue sum_c composition_c price_{n_fuels + c}.
+struct RovsProfitArgs {
+  int N, T, U, n_vehicles, n_fuels, n_comm;
+  double unit_cost, unit_revenue;
+  double fuel_cost[kRovsMaxFuel];
+  int fuel_idx[kRovsMaxFuel];
+  double ore_mean[kRovsMaxComm], ore_sd[kRovsMaxComm];
+  uint32_t tag, key0, key1;
+  const double* prices;  // [(T+1)][U][N]
+  double* profit;        // [T][N]
+  double* composition;   // [T][n_comm][N] or nullptr
+};
+hipError_t launch_rovs_profit(const RovsProfitArgs& a, hipStream_t s);
+
+// Per-step arguments of the backward induction (device copy, read by pointer).
+struct RovsStepArgs {
+  int N, T, S, C, D, R;
+  double disc, hysteresis, scaling;
+  double thr[kRovsMaxS];
+  double flow[kRovsMaxC];
+  const float* total_pops;  // [S][T+1][N]
+  const float* aars;        // [S][T][C][N]
+  const double* profit;     // [T][N]
+  const int* controls;      // [T][N]
+  double* V;                // [T][N]
+  int* optc;                // [T][N]
+  double* adj_pops;         // [T][S][N] (highest-flow adjusted populations)
+  // per-step scratch
+  int* flag;                // [N] control of an included sample, or -1
+  int* blk_count;           // [nblk][C]
+  int* blk_off;             // [nblk][C]
+  int* count;               // [C]
+  int* pos;                 // [N] position of the sample within its control
+  double* xs;               // [C][N][D] included samples (state order), compacted
+  double* ys;               // [C][N]
+  double* mm;               // [nblk][D][2] block min / max
+  double* range;            // [D][2] global min / max
+  double* u;                // [C][N][D] normalised samples
+  double* qtab;             // [D][R] normalised query coordinates
+  double* grid;             // [C][D*R + R^D] grid coordinates then fitted values
+};
+
+hipError_t launch_rovs_terminal(const RovsStepArgs* a, int N, hipStream_t s);
+hipError_t launch_rovs_collect(const RovsStepArgs* a, int N, int t, hipStream_t s);
+hipError_t launch_rovs_ranges(const RovsStepArgs* a, int N, int D, hipStream_t s);
+hipError_t launch_rovs_normalise(const RovsStepArgs* a, int N, int D, int R, hipStream_t s);
+hipError_t launch_rovs_decide(const RovsStepArgs* a, int N, int D, int t, hipStream_t s);
+// out (device): {value, SD, control}.
+hipError_t launch_rovs_first(const RovsStepArgs* a, double* out, hipStream_t s);
+
+}  // namespace roadmc
+
+#endif

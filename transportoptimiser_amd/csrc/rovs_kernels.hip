@@ -1,0 +1,663 @@
+// Species-state ROV (the reference's ALGO4) for gfx950.
+//
+// Reference: forwardPathKernel (include/gpuCode/SimulateGPU.cu:789-956)
+// moves each path's species populations under a randomly drawn control per
+// year and records the surviving fraction (AAR) under every control;
+// computePathStates / allocateXYRegressionData / computeStateMinMax
+// (:959-1148) form the state (adjusted populations under the highest flow,
+// unit profit) and the per-control regression data; the ALGO4 loop
+// (:4401-4998) fits a kNN local-linear grid per control and step and
+// backwardInduction (:2333-2784) decides by interpolating it;
+// firstPeriodInduction (:2210-2330) values time 0.  There the forward pass is
+// one thread per path walking CSC matrices (every control's product computed
+// serially), and every step makes two <<<1,1>>> kernels and a host kNN round
+// trip.
+//
+// Here:
+//   - rovs_forward_kernel: 32 paths x all patches per workgroup (one wave
+//     per two 32-column tiles, at least four; P <= 512), the state
+//     tile in LDS; per year, one FP32 MFMA product per control
+//     (v_mfma_f32_32x32x2_f32 on the packed trans o surv_c), row sums of each
+//     product for the AARs (lane shuffles, then a fixed-order sum over the
+//     waves), each path's own control kept by a register select, then the
+//     growth-rate SDE and the logistic growth with the reference's /100 in
+//     the epilogue, noise from Philox in registers;
+//   - per backward step: one kernel forms the states and counts each
+//     control's included paths per block, one scans the counts, one
+//     scatters the samples in path order (so that kNN ties resolve as in a
+//     serial pass), one reduces the state ranges, one scales them; the grid
+//     fits are the knn_fit_kernel of the grid surrogate, and one kernel
+//     decides every path by multilinear interpolation of the grids.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mte_common.h"
+#include "rng.h"
+#include "rovs_kernels.h"
+
+namespace roadmc {
+
+using TF = TraitsF32;
+
+// Waves of a forward workgroup: at most kRovsNT column tiles each, at least 4.
+static int rovs_waves(int Pp) {
+  const int ntc = Pp / TF::TN;
+  return std::max(4, (ntc + kRovsNT - 1) / kRovsNT);
+}
+
+size_t rovs_forward_lds(int Pp, int C) {
+  const int LDA = Pp + TF::PAD;
+  return (size_t)kRovsRows * LDA * sizeof(float)      // state tile
+         + kRovsRows * sizeof(float)                  // growth-rate means
+         + (size_t)rovs_waves(Pp) * kRovsRows * sizeof(float)  // per-wave row sums
+         + kRovsRows * sizeof(float)                  // row totals
+         + (size_t)C * kRovsRows * sizeof(float)      // AARs
+         + kRovsRows * sizeof(int);                   // controls
+}
+
+// Sum over the 32 columns held by the 32 lanes of a half-wave.
+__device__ __forceinline__ float half_sum(float v) {
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NT>
+__device__ __forceinline__ void rovs_product(floatx16* acc, const PackedB& B, const float* A,
+                                             const int* tcol, int LDA, int arow, int akoff,
+                                             int lane) {
+  floatx4 b0[NT];
+  int trow[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    trow[t] = 0;
+    b0[t] = load_b<floatx4>(B, b_voff(B, tcol[t], lane), 0);
+  }
+  year_product<TF, NT, true>(acc, b0, B, A, trow, tcol, LDA, arow, akoff, lane);
+}
+
+__global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwardArgs a) {
+  extern __shared__ char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nthreads = blockDim.x, NW = nthreads >> 6;
+  const int Pp = a.Pp, LDA = Pp + TF::PAD, C = a.C;
+  const int p0 = blockIdx.x * kRovsRows;
+  float* A = reinterpret_cast<float*>(smem);
+  float* grm = A + kRovsRows * LDA;
+  float* rsum = grm + kRovsRows;    // [NW][32]
+  float* tot = rsum + NW * kRovsRows;
+  float* aar = tot + kRovsRows;     // [C][32]
+  int* ctl = reinterpret_cast<int*>(aar + C * kRovsRows);
+  const float* K = a.cols;
+  const float* Kinv = a.cols + Pp;
+  const float* init = a.cols + 2 * Pp;
+  const float mu = (float)a.params[1], sd = (float)a.params[2], rev = (float)a.params[4];
+  const float jmp = (float)a.params[5], jprob = (float)a.params[6], lv = (float)a.params[7];
+  const float dt = (float)a.dt;
+
+  for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
+    const int r = idx / Pp, c = idx - r * Pp;
+    A[r * LDA + TF::perm(c)] = init[c];
+  }
+  if (tid < kRovsRows) {
+    grm[tid] = (float)a.params[0];
+    tot[tid] = a.init_total;
+    if (p0 + tid < a.N) a.total_pops[p0 + tid] = a.init_total;
+  }
+  // Column tiles of this wave: a contiguous run of at most kRovsNT tiles.
+  const int ntc = Pp / TF::TN;
+  const int base = ntc / NW, extra = ntc % NW;
+  const int my_n = base + (wave < extra ? 1 : 0);
+  const int my_t0 = wave * base + (wave < extra ? wave : extra);
+  int tcol[kRovsNT];
+#pragma unroll
+  for (int t = 0; t < kRovsNT; ++t) tcol[t] = my_t0 + t;
+  const int arow = TF::a_row(lane), akoff = TF::KV * TF::k_lane(lane);
+  const int nkg = a.Pk >> 3;
+  const size_t mbytes = (size_t)Pp * a.Pk * sizeof(float);
+  __syncthreads();
+
+  for (int year = 0; year < a.T; ++year) {
+    // Opaque per-year copies of loop-invariant scalars: the addresses derived
+    // from them are recomputed each year instead of being hoisted out of the
+    // year loop into registers that would spill.
+    int LDAy = LDA, p0y = p0, laney = lane;
+    asm volatile("" : "+s"(LDAy), "+s"(p0y), "+v"(laney));
+    if (tid < kRovsRows)
+      ctl[tid] = (p0 + tid < a.N) ? a.controls[(size_t)year * a.N + p0 + tid] : 0;
+    __syncthreads();
+    floatx16 sel[kRovsNT], acc[kRovsNT];
+#pragma unroll
+    for (int t = 0; t < kRovsNT; ++t) sel[t] = floatx16{};
+    for (int c = 0; c < C; ++c) {
+      const PackedB B = packed_b(a.mpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
+      if (my_n == 1)
+        rovs_product<1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+      else if (my_n == 2)
+        rovs_product<2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+      // Row sums of this control's product over the wave's columns, and the
+      // select of each path's own control.
+#pragma unroll
+      for (int r = 0; r < TF::NACC; ++r) {
+        float v = 0.f;
+        const int row = TF::acc_row(laney, r);
+        const bool mine = ctl[row] == c;
+#pragma unroll
+        for (int t = 0; t < kRovsNT; ++t)
+          if (t < my_n) {
+            v += acc[t][r];
+            if (mine) sel[t][r] = acc[t][r];
+          }
+        v = half_sum(v);
+        if ((laney & 31) == 0) rsum[wave * kRovsRows + row] = v;
+      }
+      __syncthreads();
+      if (tid < kRovsRows) {
+        float s = 0.f;
+        for (int w = 0; w < NW; ++w) s += rsum[w * kRovsRows + tid];
+        aar[c * kRovsRows + tid] = s / tot[tid];
+      }
+      __syncthreads();
+    }
+    // Growth-rate SDE per path (SimulateGPU.cu:839-851), jump probability
+    // from its own field.
+    if (tid < kRovsRows) {
+      float zb, zj, uj;
+      path_scalars<float>((uint32_t)(p0 + tid), (uint32_t)year, a.tag1, a.key0, a.key1, &zb,
+                          &zj, &uj);
+      const float g = grm[tid];
+      const float jump = (uj < jprob) ? 1.f : 0.f;
+      const float jsz = zj * jmp * jmp - jmp * jmp / 2.f;
+      grm[tid] = g + rev * (mu - g) * dt + g * (zb * sd) + (__expf(jsz) - 1.f) * g * jump;
+    }
+    __syncthreads();
+    // Growth (SimulateGPU.cu:902-916): g = grMean + z lv per patch,
+    // n = n' (1 + g (K - n') / K / 100), written back to the tile.
+    int LDAe = LDA, p0e = p0, lanee = lane;
+    asm volatile("" : "+s"(LDAe), "+s"(p0e), "+v"(lanee));
+    float rowv[TF::NACC];
+#pragma unroll
+    for (int r = 0; r < TF::NACC; ++r) rowv[r] = 0.f;
+#pragma unroll
+    for (int t = 0; t < kRovsNT; ++t) {
+      if (t >= my_n) break;
+      const int col = tcol[t] * TF::TN + TF::acc_col(lanee);
+      const int pcol = TF::perm(col);
+      const float Kc = K[col], Ki = Kinv[col];
+#pragma unroll
+      for (int r4 = 0; r4 < TF::NACC; r4 += 4) {
+        float z[4];
+        const int row0 = TF::acc_row(lanee, r4);
+        TF::normal4((uint32_t)(p0e + row0) >> 2, (uint32_t)year, (uint32_t)col, a.tag0, a.key0,
+                    a.key1, z);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int row = TF::acc_row(lanee, r4 + s);
+          const float x = sel[t][r4 + s];
+          const float g = grm[row] + z[s] * lv;
+          const float n = x * (1.f + g * (Kc - x) * Ki * 0.01f);
+          A[row * LDAe + pcol] = n;
+          rowv[r4 + s] += n;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < TF::NACC; ++r) {
+      const float v = half_sum(rowv[r]);
+      if ((lanee & 31) == 0) rsum[wave * kRovsRows + TF::acc_row(lanee, r)] = v;
+    }
+    __syncthreads();
+    if (tid < kRovsRows) {
+      float s = 0.f;
+      for (int w = 0; w < NW; ++w) s += rsum[w * kRovsRows + tid];
+      tot[tid] = s;
+      const int p = p0 + tid;
+      if (p < a.N) {
+        a.total_pops[(size_t)(year + 1) * a.N + p] = s;
+        for (int c = 0; c < C; ++c)
+          a.aars[((size_t)year * C + c) * a.N + p] = aar[c * kRovsRows + tid];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s) {
+  const size_t lds = rovs_forward_lds(a.Pp, a.C);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_forward_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
+  hipLaunchKernelGGL(rovs_forward_kernel, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s, a);
+  return hipGetLastError();
+}
+
+// Unit profit (computePathStates, SimulateGPU.cu:985-1002) with the ore
+// composition drawn per path, year and commodity (kind 3 of the stream
+// contract): N(ore_mean, ore_sd).
+__global__ __launch_bounds__(256) void rovs_profit_kernel(RovsProfitArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.N) return;
+  for (int t = 0; t < a.T; ++t) {
+    const double* x = a.prices + (size_t)t * a.U * a.N;
+    double fuel = 0.0;
+    for (int v = 0; v < a.n_vehicles; ++v) fuel += a.fuel_cost[v] * x[(size_t)a.fuel_idx[v] * a.N + p];
+    double ore = 0.0;
+    for (int b = 0; 4 * b < a.n_comm; ++b) {
+      double z[4];
+      rmc_normal4_d((uint32_t)p, (uint32_t)t, (uint32_t)b, a.tag, a.key0, a.key1, z);
+      for (int i = 0; i < 4 && 4 * b + i < a.n_comm; ++i) {
+        const int c = 4 * b + i;
+        const double comp = a.ore_mean[c] + a.ore_sd[c] * z[i];
+        if (a.composition) a.composition[((size_t)t * a.n_comm + c) * a.N + p] = comp;
+        ore += comp * x[(size_t)(a.n_fuels + c) * a.N + p];
+      }
+    }
+    a.profit[(size_t)t * a.N + p] = a.unit_cost + fuel - a.unit_revenue * ore;
+  }
+}
+
+hipError_t launch_rovs_profit(const RovsProfitArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(rovs_profit_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// Adjusted population of species sp at step t under control c.
+__device__ __forceinline__ double rovs_adj(const RovsStepArgs* a, int sp, int t, int c, int p) {
+  const int N = a->N, T = a->T, C = a->C;
+  return (double)a->total_pops[((size_t)sp * (T + 1) + t) * N + p] *
+         (double)a->aars[(((size_t)sp * T + t) * C + c) * N + p];
+}
+
+// Last step (backwardInduction with start = T - 1, SimulateGPU.cu:2375-2452):
+// payoff Q_c x unit profit (<= 0) for the valid controls, the minimum wins.
+__global__ __launch_bounds__(256) void rovs_terminal_kernel(const RovsStepArgs* __restrict__ a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a->N) return;
+  const int t = a->T - 1, C = a->C, S = a->S, N = a->N;
+  const double up = a->profit[(size_t)t * N + p];
+  double best = 0.0;
+  int bc = 0;
+  for (int c = 0; c < C; ++c) {
+    bool valid = true;
+    for (int sp = 0; sp < S && c > 0; ++sp) valid = valid && !(rovs_adj(a, sp, t, c, p) < a->thr[sp]);
+    double pay = 0.0;
+    if (valid) pay = fmin(a->flow[c] * up, 0.0);
+    if (c == 0) {
+      best = pay;
+    } else if (valid && pay < best) {
+      best = pay;
+      bc = c;
+    }
+  }
+  a->V[(size_t)t * N + p] = best;
+  a->optc[(size_t)t * N + p] = bc;
+  if (a->adj_pops)
+    for (int sp = 0; sp < S; ++sp)
+      a->adj_pops[((size_t)t * S + sp) * N + p] = rovs_adj(a, sp, t, C - 1, p);
+}
+
+hipError_t launch_rovs_terminal(const RovsStepArgs* a, int N, hipStream_t s) {
+  hipLaunchKernelGGL(rovs_terminal_kernel, dim3((N + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// Regression sample of path p at step t (allocateXYRegressionData,
+// SimulateGPU.cu:1010-1068): included when every highest-flow adjusted
+// population reaches its threshold, or when the drawn control is 0.
+__device__ __forceinline__ int rovs_include(const RovsStepArgs* a, int t, int p, double* x) {
+  const int S = a->S, C = a->C;
+  bool valid = true;
+  for (int sp = 0; sp < S; ++sp) {
+    x[sp] = rovs_adj(a, sp, t, C - 1, p);
+    valid = valid && !(x[sp] < a->thr[sp]);
+  }
+  x[S] = a->profit[(size_t)t * a->N + p];
+  const int c = a->controls[(size_t)t * a->N + p];
+  return (valid || c == 0) ? c : -1;
+}
+
+// Per block of kRovsScanBlock paths: each control's included count.
+__global__ __launch_bounds__(kRovsScanBlock) void rovs_count_kernel(const RovsStepArgs* __restrict__ a,
+                                                                    int t) {
+  __shared__ int cnt[kRovsMaxC];
+  const int p = blockIdx.x * kRovsScanBlock + threadIdx.x;
+  if (threadIdx.x < kRovsMaxC) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int c = -1;
+  if (p < a->N) {
+    double x[kRovsMaxS + 1];
+    c = rovs_include(a, t, p, x);
+    a->flag[p] = c;
+  }
+  if (c >= 0) atomicAdd(&cnt[c], 1);
+  __syncthreads();
+  if (threadIdx.x < a->C) a->blk_count[blockIdx.x * a->C + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of the block counts per control (one workgroup).
+__global__ __launch_bounds__(256) void rovs_scan_kernel(const RovsStepArgs* __restrict__ a, int nblk) {
+  const int c = threadIdx.x;
+  if (c >= a->C) return;
+  int run = 0;
+  for (int b = 0; b < nblk; ++b) {
+    a->blk_off[b * a->C + c] = run;
+    run += a->blk_count[b * a->C + c];
+  }
+  a->count[c] = run;
+}
+
+// Scatter in path order: a path's position is its block's offset plus the
+// number of earlier paths of its block with the same control.
+__global__ __launch_bounds__(kRovsScanBlock) void rovs_scatter_kernel(const RovsStepArgs* __restrict__ a,
+                                                                      int t) {
+  constexpr int NW = kRovsScanBlock / 64;
+  __shared__ int wcnt[kRovsMaxC][NW];
+  const int p = blockIdx.x * kRovsScanBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = p < a->N ? a->flag[p] : -1;
+  int rank = 0;
+  for (int cc = 0; cc < a->C; ++cc) {
+    const unsigned long long m = __ballot(c == cc);
+    if (lane == 0) wcnt[cc][w] = __popcll(m);
+    if (c == cc)
+      rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+  }
+  __syncthreads();
+  if (c < 0) return;
+  for (int ww = 0; ww < w; ++ww) rank += wcnt[c][ww];
+  const int pos = a->blk_off[blockIdx.x * a->C + c] + rank;
+  double x[kRovsMaxS + 1];
+  rovs_include(a, t, p, x);
+  const int D = a->D, N = a->N;
+  for (int d = 0; d < D; ++d) a->xs[((size_t)c * N + pos) * D + d] = x[d];
+  a->ys[(size_t)c * N + pos] = a->V[(size_t)(t + 1) * N + p];
+}
+
+hipError_t launch_rovs_collect(const RovsStepArgs* a, int N, int t, hipStream_t s) {
+  const int nblk = (N + kRovsScanBlock - 1) / kRovsScanBlock;
+  hipLaunchKernelGGL(rovs_count_kernel, dim3(nblk), dim3(kRovsScanBlock), 0, s, a, t);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rovs_scan_kernel, dim3(1), dim3(256), 0, s, a, nblk);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(rovs_scatter_kernel, dim3(nblk), dim3(kRovsScanBlock), 0, s, a, t);
+  return hipGetLastError();
+}
+
+// State ranges over the samples of the controls that computeStateMinMax
+// (SimulateGPU.cu:1073-1147) takes its own range from: control 0 and every
+// control with more than D + 1 samples.  Block partials, then one block.
+__global__ __launch_bounds__(256) void rovs_minmax_kernel(const RovsStepArgs* __restrict__ a) {
+  __shared__ double smin[256], smax[256];
+  const int d = blockIdx.y, D = a->D, N = a->N;
+  double lo = __builtin_inf(), hi = -__builtin_inf();
+  for (int c = 0; c < a->C; ++c) {
+    const int n = a->count[c];
+    if (c > 0 && n <= D + 1) continue;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+      const double v = a->xs[((size_t)c * N + j) * D + d];
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + off]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a->mm[((size_t)blockIdx.x * D + d) * 2 + 0] = smin[0];
+    a->mm[((size_t)blockIdx.x * D + d) * 2 + 1] = smax[0];
+  }
+}
+
+__global__ void rovs_minmax_final_kernel(const RovsStepArgs* __restrict__ a, int nb) {
+  const int d = threadIdx.x;
+  if (d >= a->D) return;
+  double lo = __builtin_inf(), hi = -__builtin_inf();
+  for (int b = 0; b < nb; ++b) {
+    lo = fmin(lo, a->mm[((size_t)b * a->D + d) * 2 + 0]);
+    hi = fmax(hi, a->mm[((size_t)b * a->D + d) * 2 + 1]);
+  }
+  if (!(lo <= hi)) lo = hi = 0.0;  // no samples at all
+  a->range[d * 2 + 0] = lo;
+  a->range[d * 2 + 1] = hi;
+}
+
+constexpr int kRovsMinMaxBlocks = 64;
+
+hipError_t launch_rovs_ranges(const RovsStepArgs* a, int N, int D, hipStream_t s) {
+  (void)N;
+  hipLaunchKernelGGL(rovs_minmax_kernel, dim3(kRovsMinMaxBlocks, D), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rovs_minmax_final_kernel, dim3(1), dim3(64), 0, s, a, kRovsMinMaxBlocks);
+  return hipGetLastError();
+}
+
+// Scaled coordinates of the kNN search and fit (SimulateGPU.cu:4582-4607):
+// species dimensions (x - min) / (max - min), unit profit
+// (x - min) / (scaling (max - min)); grid coordinates x_min + i (max - min) /
+// (R - 1) (createQueryPoints, :1176-1190), the query points scaled the same
+// way.
+__device__ __forceinline__ double rovs_scale(const RovsStepArgs* a, int d, double v) {
+  const double lo = a->range[2 * d], span = a->range[2 * d + 1] - a->range[2 * d];
+  if (!(span > 0.0)) return 0.0;
+  return d == a->D - 1 ? (v - lo) / (a->scaling * span) : (v - lo) / span;
+}
+
+__global__ __launch_bounds__(256) void rovs_normalise_kernel(const RovsStepArgs* __restrict__ a) {
+  const int D = a->D, N = a->N, R = a->R, C = a->C;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < (size_t)C * N) {
+    const int c = (int)(i / N), j = (int)(i - (size_t)c * N);
+    if (j < a->count[c])
+      for (int d = 0; d < D; ++d)
+        a->u[((size_t)c * N + j) * D + d] = rovs_scale(a, d, a->xs[((size_t)c * N + j) * D + d]);
+  }
+  if (i < (size_t)D * R) {
+    const int d = (int)(i / R), k = (int)(i - (size_t)d * R);
+    const double lo = a->range[2 * d], span = a->range[2 * d + 1] - a->range[2 * d];
+    const double q = (double)k * span / (double)(R - 1) + lo;
+    a->qtab[i] = rovs_scale(a, d, q);
+    size_t G = 1;
+    for (int dd = 0; dd < D; ++dd) G *= (size_t)R;
+    for (int c = 0; c < C; ++c) a->grid[(size_t)c * ((size_t)D * R + G) + i] = q;
+  }
+}
+
+hipError_t launch_rovs_normalise(const RovsStepArgs* a, int N, int D, int R, hipStream_t s) {
+  const size_t n = std::max((size_t)kRovsMaxC * N, (size_t)D * R);
+  hipLaunchKernelGGL(rovs_normalise_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// Multilinear interpolation of one control's grid at state x.
+template <int D>
+__device__ __forceinline__ double rovs_interp(const double* grid, int R, const double* x) {
+  const double* val = grid + (size_t)D * R;
+  int li[D];
+  double xd[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double lo = grid[d * R], hi = grid[d * R + R - 1];
+    const double c = hi != lo ? (R - 1) * (x[d] - lo) / (hi - lo) : 0.0;
+    const int i = c >= (double)(R - 2) ? R - 2 : (c > 0.0 ? (int)c : 0);
+    li[d] = i;
+    const double x0 = grid[d * R + i], x1 = grid[d * R + i + 1];
+    xd[d] = (fabs(x1 - x0) < 2.220446049250313e-16 || x0 == x1) ? 0.0 : (x[d] - x0) / (x1 - x0);
+  }
+  double s = 0.0;
+  for (int c = 0; c < (1 << D); ++c) {
+    double w = 1.0;
+    size_t idx = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int bit = (c >> (D - 1 - d)) & 1;
+      w *= bit ? xd[d] : 1.0 - xd[d];
+      idx = idx * (size_t)R + (size_t)(li[d] + bit);
+    }
+    s += val[idx] * w;
+  }
+  return s;
+}
+
+// backwardInduction at an interior step (SimulateGPU.cu:2453-2771): for each
+// valid control, Q_c x unit profit plus the discounted continuation
+// interpolated from that control's grid at the highest-flow state (clamped to
+// <= 0); switch from the running best only below hysteresis x best.
+template <int D>
+__global__ __launch_bounds__(256) void rovs_decide_kernel(const RovsStepArgs* __restrict__ a, int t) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a->N) return;
+  const int C = a->C, S = a->S, N = a->N, R = a->R;
+  double x[D];
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d) x[d] = rovs_adj(a, d, t, C - 1, p);
+  x[D - 1] = a->profit[(size_t)t * N + p];
+  size_t G = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  double best = 0.0;
+  int bc = 0;
+  for (int c = 0; c < C; ++c) {
+    bool valid = true;
+    for (int sp = 0; sp < S && c > 0; ++sp) valid = valid && !(rovs_adj(a, sp, t, c, p) < a->thr[sp]);
+    double pay = 0.0;
+    if (valid) {
+      const double cont = fmin(rovs_interp<D>(a->grid + (size_t)c * gsz, R, x) * a->disc, 0.0);
+      pay = a->flow[c] * x[D - 1] + cont;
+    }
+    if (c == 0) {
+      best = pay;
+    } else if (valid && pay < a->hysteresis * best) {
+      best = pay;
+      bc = c;
+    }
+  }
+  a->V[(size_t)t * N + p] = best;
+  a->optc[(size_t)t * N + p] = bc;
+  if (a->adj_pops)
+    for (int sp = 0; sp < S; ++sp) a->adj_pops[((size_t)t * S + sp) * N + p] = x[sp];
+}
+
+hipError_t launch_rovs_decide(const RovsStepArgs* a, int N, int D, int t, hipStream_t s) {
+  const dim3 g((N + 255) / 256), b(256);
+  switch (D) {
+    case 2: hipLaunchKernelGGL(rovs_decide_kernel<2>, g, b, 0, s, a, t); break;
+    case 3: hipLaunchKernelGGL(rovs_decide_kernel<3>, g, b, 0, s, a, t); break;
+    case 4: hipLaunchKernelGGL(rovs_decide_kernel<4>, g, b, 0, s, a, t); break;
+    case 5: hipLaunchKernelGGL(rovs_decide_kernel<5>, g, b, 0, s, a, t); break;
+    case 6: hipLaunchKernelGGL(rovs_decide_kernel<6>, g, b, 0, s, a, t); break;
+    case 7: hipLaunchKernelGGL(rovs_decide_kernel<7>, g, b, 0, s, a, t); break;
+    case 8: hipLaunchKernelGGL(rovs_decide_kernel<8>, g, b, 0, s, a, t); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Time 0 (firstPeriodInduction, SimulateGPU.cu:2210-2330): per control, the
+// discounted mean of the step-1 values of the paths that drew it, plus
+// Q_c x the mean time-0 unit profit when valid (<= 0); the minimum over the
+// controls with paths wins.  Every path gets that value and control at time
+// 0; the SD is over the paths that drew the chosen control.  One workgroup,
+// fixed-order reductions.  out = {value, SD, control}.
+__global__ __launch_bounds__(1024) void rovs_first_kernel(const RovsStepArgs* __restrict__ a,
+                                                          double* out) {
+  __shared__ double red[1024];
+  __shared__ double sums[kRovsMaxC + 1], cnts[kRovsMaxC];
+  __shared__ double res[3];
+  const int N = a->N, C = a->C, S = a->S, tid = threadIdx.x;
+  const double* V1 = a->V + (size_t)N;
+  const int* c0 = a->controls;
+  auto block_sum = [&](double v) -> double {
+    red[tid] = v;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+      if (tid < off) red[tid] += red[tid + off];
+      __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  };
+  for (int c = 0; c < C; ++c) {
+    double s = 0.0, n = 0.0;
+    for (int p = tid; p < N; p += 1024)
+      if (c0[p] == c) {
+        s += V1[p];
+        n += 1.0;
+      }
+    s = block_sum(s);
+    n = block_sum(n);
+    if (tid == 0) {
+      sums[c] = s;
+      cnts[c] = n;
+    }
+  }
+  double up = 0.0;
+  for (int p = tid; p < N; p += 1024) up += a->profit[p];
+  up = block_sum(up) / N;
+  if (tid == 0) {
+    double best = 0.0;
+    int bc = -1;
+    for (int c = 0; c < C; ++c) {
+      if (cnts[c] == 0.0) continue;  // no path drew it (the reference breaks here)
+      bool valid = true;
+      for (int sp = 0; sp < S && c > 0; ++sp) valid = valid && !(rovs_adj(a, sp, 0, c, 0) < a->thr[sp]);
+      if (!valid) continue;
+      const double pay = fmin(sums[c] / cnts[c] * a->disc + a->flow[c] * up, 0.0);
+      if (bc < 0 || pay < best) {
+        best = pay;
+        bc = c;
+      }
+    }
+    if (bc < 0) bc = 0;
+    res[0] = best;
+    res[2] = (double)bc;
+    sums[kRovsMaxC] = a->flow[bc] * up;
+  }
+  __syncthreads();
+  const int bc = (int)res[2];
+  const double base = sums[kRovsMaxC];
+  double m = 0.0, q = 0.0;
+  for (int p = tid; p < N; p += 1024)
+    if (c0[p] == bc) m += V1[p] * a->disc + base;
+  m = block_sum(m);
+  const double n = cnts[bc] > 0.0 ? cnts[bc] : 1.0;
+  const double mean = m / n;
+  for (int p = tid; p < N; p += 1024)
+    if (c0[p] == bc) {
+      const double d = V1[p] * a->disc + base - mean;
+      q += d * d;
+    }
+  q = block_sum(q);
+  for (int p = tid; p < N; p += 1024) {
+    a->V[p] = res[0];
+    a->optc[p] = bc;
+  }
+  if (tid == 0) {
+    out[0] = res[0];
+    out[1] = sqrt(q / n);
+    out[2] = (double)bc;
+  }
+  if (a->adj_pops)
+    for (int p = tid; p < N; p += 1024)
+      for (int sp = 0; sp < S; ++sp) a->adj_pops[(size_t)sp * N + p] = rovs_adj(a, sp, 0, C - 1, p);
+}
+
+hipError_t launch_rovs_first(const RovsStepArgs* a, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(rovs_first_kernel, dim3(1), dim3(1024), 0, s, a, out);
+  return hipGetLastError();
+}
+
+}  // namespace roadmc

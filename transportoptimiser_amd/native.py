@@ -171,6 +171,61 @@ class RoadSpecies(C.Structure):
     ]
 
 
+class RovsSpecies(C.Structure):
+    _fields_ = [
+        ("init_pop", _dp),
+        ("capacity", _dp),
+        ("trans", _dp),
+        ("surv", _dp),
+        ("params", C.c_double * 8),
+    ]
+
+
+class RovsProblem(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int32),
+        ("n_species", C.c_int32),
+        ("species", C.POINTER(RovsSpecies)),
+        ("n_controls", C.c_int32),
+        ("flow", _dp),
+        ("n_fuels", C.c_int32),
+        ("n_commodities", C.c_int32),
+        ("uncertainties", C.POINTER(Uncertainty)),
+        ("ore_mean", _dp),
+        ("ore_sd", _dp),
+        ("n_vehicles", C.c_int32),
+        ("fuel_cost", _dp),
+        ("fuel_idx", _ip),
+        ("unit_cost", C.c_double),
+        ("unit_revenue", C.c_double),
+        ("n_years", C.c_int32),
+        ("time_step", C.c_double),
+        ("rrr", C.c_double),
+        ("n_paths", C.c_int64),
+        ("dim_res", C.c_int32),
+        ("profit_scaling", C.c_double),
+        ("hysteresis", C.c_double),
+        ("seed", C.c_uint64),
+        ("road_id", C.c_uint32),
+    ]
+
+
+class RovsResult(C.Structure):
+    _fields_ = [
+        ("value", _dp),
+        ("value_sd", _dp),
+        ("first_control", _ip),
+        ("cond_exp", _dp),
+        ("opt_control", _ip),
+        ("adj_pops", _dp),
+        ("unit_profit", _dp),
+        ("controls", _ip),
+        ("total_pops", _dp),
+        ("aars", _dp),
+        ("grids", _dp),
+    ]
+
+
 # Every symbol include/roadmc.h declares: name -> (restype, argtypes).
 SIGNATURES: dict[str, tuple] = {
     "roadmc_device_count": (C.c_int, [_ip]),
@@ -203,6 +258,7 @@ SIGNATURES: dict[str, tuple] = {
                                            C.c_int32, C.c_int32, _dp]),
     "roadmc_knn_surrogate_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32,
                                             _dp, _dp]),
+    "roadmc_rovs_run": (C.c_int, [C.c_void_p, C.POINTER(RovsProblem), C.POINTER(RovsResult)]),
     "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
