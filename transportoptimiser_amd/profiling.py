@@ -29,6 +29,10 @@ of P patches and a road of V vertices (3 flow controls) R times.
 fits an R^D grid surrogate from N samples (scaled coordinates, values <= 0,
 the reference's neighbour count) and evaluates it at 10^6 points.
 
+``python transportoptimiser_amd/profiling.py rovs [--patches P --species S
+--controls C --paths N --years T --res R]`` runs the species-state ROV
+(the reference's ALGO4) on the synthetic landscape R times.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -163,6 +167,27 @@ def knn(n: int, dims: int, res: int, repeat: int) -> None:
     ctx.close()
 
 
+def rovs(patches: int, species: int, controls: int, paths: int, years: int, res: int,
+         repeat: int) -> None:
+    """Species-state ROV (ALGO4) on the synthetic landscape."""
+    import os
+    import time
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import rov_species as rs
+    from transportoptimiser_amd.api import Context
+    ctx = Context(0)
+    pr = rs.synthetic_problem(seed=5, n_patches=patches, n_species=species, n_controls=controls,
+                              n_paths=paths, n_years=years, dim_res=res)
+    for i in range(repeat):
+        t = time.perf_counter()
+        r = rs.simulate_rov_species(ctx, pr)
+        print(f"rovs call {i}: P={patches} S={species} C={controls} N={paths} T={years} "
+              f"R={res}: {(time.perf_counter() - t) * 1e3:.2f} ms, value {r['value']:.4f}, "
+              f"control {r['first_control']}", flush=True)
+    ctx.close()
+
+
 def summarize(paths: list[str]) -> None:
     for p in paths:
         with open(p, newline="") as f:
@@ -253,6 +278,14 @@ def main() -> int:
     kn.add_argument("--dims", type=int, default=3)
     kn.add_argument("--res", type=int, default=20)
     kn.add_argument("--repeat", type=int, default=3)
+    rv2 = sub.add_parser("rovs")
+    rv2.add_argument("--patches", type=int, default=256)
+    rv2.add_argument("--species", type=int, default=1)
+    rv2.add_argument("--controls", type=int, default=3)
+    rv2.add_argument("--paths", type=int, default=100000)
+    rv2.add_argument("--years", type=int, default=50)
+    rv2.add_argument("--res", type=int, default=20)
+    rv2.add_argument("--repeat", type=int, default=3)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -272,6 +305,8 @@ def main() -> int:
         adjoint(a.repeat)
     elif a.cmd == "rov":
         rov(a.repeat)
+    elif a.cmd == "rovs":
+        rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat)
     elif a.cmd == "knn":
         knn(a.samples, a.dims, a.res, a.repeat)
     elif a.cmd == "road":
