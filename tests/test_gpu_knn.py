@@ -1,9 +1,10 @@
 """GPU grid surrogate against the oracle (roadmc_knn_surrogate_fit / _eval).
 
 Both sides rank neighbours by the same FP64 squared distances (no
-contraction, ties to the lower index), so the neighbour sets agree exactly;
-the fitted values then differ only through exp (device vs libm, about an ulp)
-and agree to 1e-9 relative.  Evaluation agrees to 1e-12.
+contraction, ties to the lower index) and sum them in that order, so the
+neighbour sets agree exactly; the fitted values then differ only through exp
+(device vs libm, about an ulp) and agree to 1e-9 relative.  Evaluation agrees
+to 1e-12.
 """
 import numpy as np
 import pytest
@@ -31,6 +32,9 @@ def _data(rng, n, d, integer=False):
     (3, 300, 10, 128, 0),
     (4, 800, 6, 20, nat.KNN_NORMALIZE),
     (8, 400, 3, 30, nat.KNN_NORMALIZE),
+    # more than 2048 samples: the radix-select kernel
+    (2, 20000, 10, 114, nat.KNN_NORMALIZE),
+    (3, 9000, 6, 128, nat.KNN_NORMALIZE | nat.KNN_NONPOSITIVE),
 ])
 def test_fit_matches_oracle(ctx, oracle, d, n, R, k, flags):
     print(f"knn fit: d={d} n={n} R={R} k={k} flags={flags}", flush=True)
@@ -43,10 +47,12 @@ def test_fit_matches_oracle(ctx, oracle, d, n, R, k, flags):
     np.testing.assert_allclose(g.surrogate[d * R:], ref[d * R:], rtol=1e-9, atol=1e-12)
 
 
-def test_fit_with_tied_distances_matches_oracle(ctx, oracle):
-    # Integer sample points: many equal distances, broken by sample index.
+@pytest.mark.parametrize("n", [300, 5000])
+def test_fit_with_tied_distances_matches_oracle(ctx, oracle, n):
+    # Integer sample points: many equal distances, broken by sample index
+    # (300: lane lists and merge; 5000: radix select, equal keys in index order).
     rng = np.random.default_rng(5)
-    x, y = _data(rng, 300, 2, integer=True)
+    x, y = _data(rng, n, 2, integer=True)
     g = knn.fit(ctx, x, y, 9, 12)
     np.testing.assert_allclose(g.surrogate, oracle.knn_fit(x, y, 9, 12, 0), rtol=1e-9,
                                atol=1e-12)

@@ -26,9 +26,12 @@ struct KnnEvalArgs {
 constexpr int kKnnThreads = 64;  // one wave per workgroup
 constexpr int kKnnMaxK = 128;
 
-// Dynamic LDS of the fit kernel for per-lane lists of cap entries.
-inline size_t knn_fit_lds(int cap) {
-  return (size_t)cap * kKnnThreads * (sizeof(double) + sizeof(int));
+constexpr int kKnnSelectMin = 2048;  // above this many samples: the radix-select kernel
+
+// Dynamic LDS of the wave-per-point fit kernel: per-lane lists of cap
+// entries, then the k merged neighbours.
+inline size_t knn_fit_lds(int cap, int k) {
+  return ((size_t)cap * kKnnThreads + (size_t)k) * (sizeof(double) + sizeof(int));
 }
 
 hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s);
