@@ -85,8 +85,9 @@ def run_extras(ctx) -> dict:
     """The other configurations of the hot path, each timed once after a
     warm-up: the large single road (3 x 256 patches, 100 years, 100k paths),
     the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis),
-    the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10) and
-    the road -> matrix builder (4096 patches)."""
+    the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10),
+    the road -> matrix builder (4096 patches), the species-state ROV (ALGO4)
+    and the grid-surrogate fit."""
     from transportoptimiser_amd import landscape
     from transportoptimiser_amd.adjoint import mte_gradient
     from transportoptimiser_amd.rov import config5, simulate_rov
@@ -145,6 +146,30 @@ def run_extras(ctx) -> dict:
         "what": "crossings, transitions and survival (3 controls) of 4096 patches and a "
                 "177-segment road, FP64; host call incl. 604 MB of outputs to pageable memory"}
     log(f"road builder: {out['road_builder_p4096']}")
+    from transportoptimiser_amd import rov_species as rsp
+    pr5 = rsp.synthetic_problem(seed=5, n_patches=256, n_controls=3, n_paths=100000, n_years=50,
+                                dim_res=20, threshold_ratio=0.7)
+    rsp.simulate_rov_species(ctx, pr5)
+    t = time.perf_counter()
+    r5 = rsp.simulate_rov_species(ctx, pr5)
+    ms = (time.perf_counter() - t) * 1e3
+    out["rov_species_algo4"] = {
+        "ms": round(ms, 2), "value": round(r5["value"], 4), "first_control": r5["first_control"],
+        "what": "species-state ROV (ALGO4): 1 species x 256 patches, 3 flow controls, "
+                "100,000 paths, 50 years, 20 x 20 kNN grids; host call"}
+    log(f"rov species: {out['rov_species_algo4']}")
+    from transportoptimiser_amd import knn as kn
+    rng = np.random.default_rng(1)
+    xk = rng.uniform(0.0, 1.0, size=(4096, 3))
+    yk = -np.exp(xk.sum(1))
+    kk = kn.neighbour_count(4096, 1)
+    kn.fit(ctx, xk, yk, 20, kk, normalize=True, nonpositive=True)
+    t = time.perf_counter()
+    kn.fit(ctx, xk, yk, 20, kk, normalize=True, nonpositive=True)
+    ms = (time.perf_counter() - t) * 1e3
+    out["knn_grid_fit"] = {"ms": round(ms, 2),
+                           "what": "kNN local-linear 20^3 grid from 4,096 samples, k = 97; host call"}
+    log(f"knn grid fit: {out['knn_grid_fit']}")
     return out
 
 

@@ -169,7 +169,9 @@ def knn(n: int, dims: int, res: int, repeat: int) -> None:
 
 def rovs(patches: int, species: int, controls: int, paths: int, years: int, res: int,
          repeat: int) -> None:
-    """Species-state ROV (ALGO4) on the synthetic landscape."""
+    """Species-state ROV (ALGO4) on the synthetic landscape, thresholds at 0.7
+    of the initial total (every control has in-the-money paths for the first
+    half of the horizon at P = 256)."""
     import os
     import time
 
@@ -178,7 +180,7 @@ def rovs(patches: int, species: int, controls: int, paths: int, years: int, res:
     from transportoptimiser_amd.api import Context
     ctx = Context(0)
     pr = rs.synthetic_problem(seed=5, n_patches=patches, n_species=species, n_controls=controls,
-                              n_paths=paths, n_years=years, dim_res=res)
+                              n_paths=paths, n_years=years, dim_res=res, threshold_ratio=0.7)
     for i in range(repeat):
         t = time.perf_counter()
         r = rs.simulate_rov_species(ctx, pr)

@@ -149,11 +149,12 @@ def simulate_rov_species(ctx, pr: RovsProblem, *, paths: bool = False,
 
 def synthetic_problem(seed: int = 5, n_patches: int = 64, n_species: int = 1,
                       n_controls: int = 3, n_paths: int = 10000, n_years: int = 30,
-                      dim_res: int = 12) -> RovsProblem:
+                      dim_res: int = 12, threshold_ratio: float = 0.965) -> RovsProblem:
     """A seeded landscape and road: exp(-d / 2 km) transitions, crossing
     mortality on the links a random road cuts, rising with the flow level;
     growth rates in percent per year as the reference's species law takes
-    them; one MRJD fuel price and one ore price; flows 0, ..., 100."""
+    them; thresholds threshold_ratio x the initial total; one MRJD fuel price
+    and one ore price; flows 0, ..., 100."""
     rng = np.random.default_rng(seed)
     P, C_ = n_patches, n_controls
     flow = np.linspace(0.0, 100.0, C_)
@@ -168,7 +169,7 @@ def synthetic_problem(seed: int = 5, n_patches: int = 64, n_species: int = 1,
         surv = np.stack([1.0 - mort * (q / flow[-1]) for q in flow])
         cap = rng.uniform(500.0, 5000.0, size=P)
         init = 0.8 * cap
-        params = np.array([5.0, 5.0, 0.1, 0.965 * init.sum(), 0.5, 0.1, 0.05, 1.0])
+        params = np.array([5.0, 5.0, 0.1, threshold_ratio * init.sum(), 0.5, 0.1, 0.05, 1.0])
         species.append(RovsSpecies(init, cap, trans, surv, params))
     return RovsProblem(
         species=species, flow=flow,
