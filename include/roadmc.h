@@ -415,6 +415,62 @@ typedef struct {
 int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
                     roadmc_rovs_result* result);
 
+/* ---- On-disk formats ------------------------------------------------------- */
+
+/* The surrogate training data file of a scenario ("surrogate_data"):
+ * RoadGA::writeOutSurrogateData (RoadGA.cpp:2977-3070) and
+ * RoadGA::getExistingSurrogateData (RoadGA.cpp:4553-4611).  Four header lines
+ * (three banner lines, one column line), then one sample per line. */
+#define ROADMC_SURR_MTE 0        /* per species: initial AAR, mean end population, its SD */
+#define ROADMC_SURR_CONTROLLED 1 /* initial AAR per species, initial unit profit, mean
+                                    value, value SD */
+
+/* Appends n_rows samples (rows: [n_rows][3 n_species] for MTE, [n_rows][n_species + 3]
+ * for CONTROLLED), creating the file with the reference's header when it does not
+ * exist.  CONTROLLED samples are kept only when |value| and |SD| <= |max_benefit|
+ * (and max_benefit is finite), as the reference filters them.  Numbers are written
+ * as the reference's stream writes them: width 20, 6 significant digits. */
+int roadmc_surrogate_data_write(const char* path, int32_t kind, int32_t n_species,
+                                int32_t n_rows, const double* rows, double max_benefit,
+                                int32_t* n_written);
+/* Reads up to max_rows samples into rows (same layout); *n_rows receives the
+ * number of samples in the file. */
+int roadmc_surrogate_data_read(const char* path, int32_t kind, int32_t n_species,
+                               int32_t max_rows, double* rows, int32_t* n_rows);
+
+/* Input files of the reference's data directory, read by key ("Key : value"
+ * lines; RoadGA::initialiseFromTextInput, RoadGA.cpp:3464-4550, reads the same
+ * fields by position), so that Monte Carlo inputs can be built from them. */
+#define ROADMC_NAME_LEN 64
+#define ROADMC_MAX_HABITAT_TYPES 8
+
+typedef struct { /* Commodities/<name> */
+  char name[ROADMC_NAME_LEN];
+  double current, mean, sd, reversion, jump_prob, poisson_jump;
+  double ore_mean, ore_sd; /* contribution to the transported ore: mean, SD */
+} roadmc_commodity_file;
+
+typedef struct { /* Vehicles/<name> */
+  char name[ROADMC_NAME_LEN];
+  double width, length, proportion, load;
+  int32_t fuel; /* index into the fuels of the main input file */
+  double a, agr, av, avsq, travel_cost;
+} roadmc_vehicle_file;
+
+typedef struct { /* Species/<name>; fields absent from the file are NaN */
+  char name[ROADMC_NAME_LEN];
+  int32_t n_habitat_types;
+  char habitat_name[ROADMC_MAX_HABITAT_TYPES][16];
+  double max_pop[ROADMC_MAX_HABITAT_TYPES], cost_per_m2[ROADMC_MAX_HABITAT_TYPES];
+  double pref_mean[ROADMC_MAX_HABITAT_TYPES], pref_sd[ROADMC_MAX_HABITAT_TYPES];
+  double growth_current, growth_mean, growth_sd, growth_reversion, growth_jump_prob,
+      growth_poisson_jump;
+} roadmc_species_file;
+
+int roadmc_read_commodity(const char* path, roadmc_commodity_file* out);
+int roadmc_read_vehicle(const char* path, roadmc_vehicle_file* out);
+int roadmc_read_species(const char* path, roadmc_species_file* out);
+
 /* ---- Learned surrogate --------------------------------------------------- */
 
 /* An ensemble of n_members MLPs n_in -> 64 -> 64 -> n_out (tanh), trained in

@@ -226,6 +226,36 @@ class RovsResult(C.Structure):
     ]
 
 
+SURR_MTE = 0
+SURR_CONTROLLED = 1
+NAME_LEN = 64
+MAX_HABITAT_TYPES = 8
+
+
+class CommodityFile(C.Structure):
+    _fields_ = [("name", C.c_char * NAME_LEN)] + [
+        (k, C.c_double) for k in ("current", "mean", "sd", "reversion", "jump_prob",
+                                  "poisson_jump", "ore_mean", "ore_sd")]
+
+
+class VehicleFile(C.Structure):
+    _fields_ = [("name", C.c_char * NAME_LEN), ("width", C.c_double), ("length", C.c_double),
+                ("proportion", C.c_double), ("load", C.c_double), ("fuel", C.c_int32),
+                ("a", C.c_double), ("agr", C.c_double), ("av", C.c_double),
+                ("avsq", C.c_double), ("travel_cost", C.c_double)]
+
+
+class SpeciesFile(C.Structure):
+    _fields_ = [("name", C.c_char * NAME_LEN), ("n_habitat_types", C.c_int32),
+                ("habitat_name", (C.c_char * 16) * MAX_HABITAT_TYPES),
+                ("max_pop", C.c_double * MAX_HABITAT_TYPES),
+                ("cost_per_m2", C.c_double * MAX_HABITAT_TYPES),
+                ("pref_mean", C.c_double * MAX_HABITAT_TYPES),
+                ("pref_sd", C.c_double * MAX_HABITAT_TYPES)] + [
+        (k, C.c_double) for k in ("growth_current", "growth_mean", "growth_sd",
+                                  "growth_reversion", "growth_jump_prob", "growth_poisson_jump")]
+
+
 # Every symbol include/roadmc.h declares: name -> (restype, argtypes).
 SIGNATURES: dict[str, tuple] = {
     "roadmc_device_count": (C.c_int, [_ip]),
@@ -259,6 +289,13 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_knn_surrogate_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32,
                                             _dp, _dp]),
     "roadmc_rovs_run": (C.c_int, [C.c_void_p, C.POINTER(RovsProblem), C.POINTER(RovsResult)]),
+    "roadmc_surrogate_data_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp,
+                                              C.c_double, _ip]),
+    "roadmc_surrogate_data_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp,
+                                             _ip]),
+    "roadmc_read_commodity": (C.c_int, [C.c_char_p, C.POINTER(CommodityFile)]),
+    "roadmc_read_vehicle": (C.c_int, [C.c_char_p, C.POINTER(VehicleFile)]),
+    "roadmc_read_species": (C.c_int, [C.c_char_p, C.POINTER(SpeciesFile)]),
     "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
