@@ -23,6 +23,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -81,13 +82,13 @@ def read_traffic(kernel_ms: float):
         return None
 
 
-def run_extras(ctx) -> dict:
+def run_extras(ctx, roads, prep_ms) -> dict:
     """The other configurations of the hot path, each timed once after a
     warm-up: the large single road (3 x 256 patches, 100 years, 100k paths),
     the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis),
     the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10),
-    the road -> matrix builder (4096 patches), the species-state ROV (ALGO4)
-    and the grid-surrogate fit."""
+    the road -> matrix builder (4096 patches), the generation prepared from
+    road geometry, the species-state ROV (ALGO4) and the grid-surrogate fit."""
     from transportoptimiser_amd import landscape
     from transportoptimiser_amd.adjoint import mte_gradient
     from transportoptimiser_amd.rov import config5, simulate_rov
@@ -146,6 +147,36 @@ def run_extras(ctx) -> dict:
         "what": "crossings, transitions and survival (3 controls) of 4096 patches and a "
                 "177-segment road, FP64; host call incl. 604 MB of outputs to pageable memory"}
     log(f"road builder: {out['road_builder_p4096']}")
+    # The same generation shapes with every species given by its road: the
+    # batch builds and packs all matrices on the device at prepare time.
+    from transportoptimiser_amd.road import SpeciesRoadPatches
+    rng = np.random.default_rng(9)
+    groads = []
+    for r in roads:
+        sps = []
+        for sd in r.species:
+            P = sd.n_patches
+            srp = SpeciesRoadPatches(ctx, random_road_species(rng, P))
+            sps.append(srp.device_species(1, sd.init_pop, sd.capacity, sd.growth,
+                                          params=sd.params, noise_sd=sd.noise_sd,
+                                          threshold_ratio=sd.threshold_ratio))
+        groads.append(dataclasses.replace(r, species=sps))
+    ctx.prepare_batch(groads[:4]).close()
+    t = time.perf_counter()
+    gb = ctx.prepare_batch(groads)
+    gms = (time.perf_counter() - t) * 1e3
+    gb.launch()
+    gb.sync()
+    kms = gb.kernel_ms()
+    gb.close()
+    out["generation_from_roads"] = {
+        "prepare_ms": round(gms, 1), "prepare_ms_host_matrices": round(prep_ms, 1),
+        "graph_ms": round(kms, 1),
+        "what": f"{len(groads)} roads x {len(groads[0].species)} species of the headline shapes, "
+                "each species given by patches and a 40-point road: transitions and the "
+                "control's survival built on the device and packed into the MFMA operand at "
+                "prepare; compared with preparing the same shapes from host matrices"}
+    log(f"generation from roads: {out['generation_from_roads']}")
     from transportoptimiser_amd import rov_species as rsp
     pr5 = rsp.synthetic_problem(seed=5, n_patches=256, n_controls=3, n_paths=100000, n_years=50,
                                 dim_res=20, threshold_ratio=0.7)
@@ -216,7 +247,10 @@ def main() -> int:
         log(f"autotuned tile rows: {ctx.autotune(n_paths=args.paths, n_years=5)}")
     tiles = ctx.tiles()
 
+    t = time.perf_counter()
     batch = ctx.prepare_batch(roads)
+    prep_ms = (time.perf_counter() - t) * 1e3
+    log(f"prepared the generation from host matrices in {prep_ms:.1f} ms")
     for _ in range(args.warmup):
         batch.launch()
         batch.sync()
@@ -309,7 +343,7 @@ def main() -> int:
                          f"512 roads as a + b P^2"}
         log(f"cpu baseline: {cpu}")
 
-    extras = None if args.no_extras else run_extras(ctx)
+    extras = None if args.no_extras else run_extras(ctx, roads, prep_ms)
 
     achieved = flops / (kernel_ms / 1e3) / 1e12
     traffic = read_traffic(kernel_ms)

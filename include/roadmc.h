@@ -48,6 +48,8 @@ extern "C" {
 
 typedef struct roadmc_ctx roadmc_ctx;
 
+struct roadmc_road_species; /* declared with roadmc_build_species_road below */
+
 /* One species on one road: the arrays SpeciesRoadPatches holds
  * (include/road/SpeciesRoadPatches.h:294-306) and the packed parameters of
  * SimulateGPU.cu:3690-3714. */
@@ -67,6 +69,14 @@ typedef struct {
                                6 jump probability, 7 local variability */
   double noise_sd;          /* LOGNORMAL year noise sigma */
   double threshold_ratio;   /* MTE: P(total_T >= ratio * total_0) */
+  const struct roadmc_road_species* road; /* with trans == NULL (roadmc_mte_run,
+                               roadmc_mte_run_batch, roadmc_batch_prepare): trans
+                               and the survival of road_control are built on the
+                               device from this species' patches and road (as
+                               roadmc_build_species_road builds them) and never
+                               leave it; NULL otherwise */
+  int32_t road_control;     /* that flow control; negative: the last (the
+                               reference's survProbs[last]) */
 } roadmc_species;
 
 /* One road.  The scalars the reference reads from the optimiser object
@@ -191,7 +201,7 @@ int roadmc_patch_normals(roadmc_ctx* ctx, uint64_t seed, uint32_t road_id,
                                     (SpeciesRoadPatches.cpp:413-416) */
 #define ROADMC_TRANS_DECAY 1     /* off-diagonal weight x exp(-lambda d) */
 
-typedef struct {
+typedef struct roadmc_road_species {
   int32_t n_patches;         /* P */
   const double* x;           /* [P] patch centroids */
   const double* y;           /* [P] */

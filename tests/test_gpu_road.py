@@ -167,3 +167,65 @@ def test_road_to_mte_end_to_end(ctx, oracle):
     out = ctx.simulate_mte(RoadProblem(species=[sd_gpu], **kw))
     ref = oracle.mte_problem(RoadProblem(species=[sd_orc], **kw), n_threads=8)
     np.testing.assert_allclose(out.end_pops, ref["end_pops"], rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("precision", [nat.PREC_FP32, nat.PREC_FP64])
+def test_batch_builds_matrices_on_device(ctx, precision):
+    """Species that carry their road (trans None) get trans o surv built and
+    packed on the device inside the batch; the results equal, bit for bit,
+    the batch run on the host copies of the same builder's matrices: the
+    packed operand is the same FP64 product, narrowed the same way."""
+    from transportoptimiser_amd.landscape import RoadProblem
+    rng = np.random.default_rng(21)
+    dev, host = [], []
+    for r, P in enumerate([40, 129, 300, 17]):
+        inp = rd.random_road_species(rng, P, n_controls=3)
+        srp = rd.SpeciesRoadPatches(ctx, inp)
+        cap = rng.uniform(500, 5000, P)
+        control = [0, 2, -1, 1][r]
+        kw = dict(noise_sd=0.1, threshold_ratio=0.4)
+        args = (0.8 * cap, cap, np.full(P, 0.05))
+        d = srp.device_species(control, *args, **kw)
+        h = srp.to_species(control if control >= 0 else 2, *args, **kw)
+        # Road 2 mixes a road species with a host-matrix one.
+        other = None
+        if r == 2:
+            sp = rd.random_road_species(rng, 50, n_controls=2)
+            o = rd.SpeciesRoadPatches(ctx, sp)
+            c2 = rng.uniform(500, 5000, 50)
+            other = o.to_species(1, 0.5 * c2, c2, np.full(50, 0.02), noise_sd=0.1)
+        pk = dict(n_years=15, n_paths=1500, precision=precision, seed=7, road_id=r,
+                  drop_tol=1e-6 if r % 2 else 0.0)
+        dev.append(RoadProblem(species=[d] + ([other] if other else []), **pk))
+        host.append(RoadProblem(species=[h] + ([other] if other else []), **pk))
+    print(f"batch road->device pack precision={precision}", flush=True)
+    a = ctx.simulate_mte_batch(dev, end_pops=True)
+    b = ctx.simulate_mte_batch(host, end_pops=True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.end_pops, y.end_pops)
+        np.testing.assert_array_equal(x.mte_prob, y.mte_prob)
+        for pm, pn in zip(x.patch_mean, y.patch_mean):
+            np.testing.assert_array_equal(pm, pn)
+    # A prepared batch (the per-generation replay path) builds them once.
+    with ctx.prepare_batch(dev) as bt:
+        bt.launch()
+        bt.sync()
+        c = bt.fetch(end_pops=True)
+    for x, y in zip(c, b):
+        np.testing.assert_array_equal(x.end_pops, y.end_pops)
+
+
+def test_device_road_species_rejected_where_unsupported(ctx):
+    from transportoptimiser_amd.landscape import RoadProblem
+    rng = np.random.default_rng(22)
+    inp = rd.random_road_species(rng, 30, n_controls=2)
+    srp = rd.SpeciesRoadPatches(ctx, inp)
+    cap = rng.uniform(500, 5000, 30)
+    d = srp.device_species(5, 0.8 * cap, cap, np.full(30, 0.05))
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte_batch([RoadProblem(species=[d], n_years=5, n_paths=100)])
+    assert e.value.code == nat.ERR_INVALID
+    d.road_control = 0
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte(RoadProblem(species=[d], n_years=5, n_paths=100), plain=True)
+    assert e.value.code == nat.ERR_UNSUPPORTED

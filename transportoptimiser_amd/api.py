@@ -71,6 +71,11 @@ class _Marshal:
         # Column-major (Eigen) storage with (i, j) = origin i -> destination j.
         s.trans = self._arr(sd.trans, "F")
         s.surv = self._arr(sd.surv, "F")
+        if sd.trans is None and sd.road is not None:
+            from .road import road_struct
+            rs = road_struct(sd.road, self.keep)
+            s.road = C.pointer(rs)
+            s.road_control = int(sd.road_control)
         for k in range(8):
             s.params[k] = float(sd.params[k])
         s.noise_sd = float(sd.noise_sd)

@@ -40,8 +40,21 @@ constexpr int kMteRowsF64 = 16;
 // split into k-quarters (0 when the job does not split).
 size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split);
 // Packed index of movement entry (k = origin, j = destination) in a matrix
-// packed with Kp / 8 k-groups per column block (Kp: the contraction extent).
-size_t mte_pack_index(bool fp64, int Kp, int k, int j);
+// packed with Kp / 8 k-groups per column block (Kp: the contraction extent):
+// element (k, j) lands where lane l of the wave handling column tile j / TN
+// reads it as B operand s of k-group k / 8.
+__host__ __device__ inline size_t mte_pack_index(bool fp64, int Kp, int k, int j) {
+  const int kg = k >> 3, kk = k & 7;
+  const int nkg = Kp >> 3;
+  if (!fp64) {
+    const int s = kk >> 1, h = kk & 1;
+    const int lane = h * 32 + (j & 31), cb = j >> 5;
+    return (((size_t)cb * nkg + kg) * 64 + lane) * 4 + s;
+  }
+  const int s = kk >> 2, h = kk & 3;
+  const int lane = h * 16 + (j & 15), cb = j >> 4;
+  return (((size_t)cb * nkg + kg) * 64 + lane) * 2 + s;
+}
 
 hipError_t mte_set_lds_limit(bool fp64, size_t bytes);
 hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,

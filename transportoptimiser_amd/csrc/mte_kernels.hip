@@ -39,21 +39,6 @@ size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split) {
               : lds_bytes_t<TraitsF32>(BM, Pp, n_rem_split);
 }
 
-size_t mte_pack_index(bool fp64, int Kp, int k, int j) {
-  // Element (k, j) lands where lane l of the wave handling column tile
-  // j / TN reads it as B operand s of k-group k / 8.
-  const int kg = k >> 3, kk = k & 7;
-  const int nkg = Kp >> 3;
-  if (!fp64) {
-    const int s = kk >> 1, h = kk & 1;
-    const int lane = h * 32 + (j & 31), cb = j >> 5;
-    return (((size_t)cb * nkg + kg) * 64 + lane) * 4 + s;
-  }
-  const int s = kk >> 2, h = kk & 3;
-  const int lane = h * 16 + (j & 15), cb = j >> 4;
-  return (((size_t)cb * nkg + kg) * 64 + lane) * 2 + s;
-}
-
 
 // Simulates one tile (BM paths of one job) for all years and writes the
 // tile's tallies.  Every thread of the workgroup calls it.  The growth law is

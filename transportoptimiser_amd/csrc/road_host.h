@@ -1,0 +1,35 @@
+// Host side of the road -> matrix builder shared by roadmc_build_species_road
+// and the MTE runs that build their matrices on the device (internal).
+#ifndef ROADMC_ROAD_HOST_H
+#define ROADMC_ROAD_HOST_H
+
+#include <vector>
+
+#include "ctx.h"
+#include "road_kernels.h"
+
+namespace roadmc {
+
+// Host copies a build's uploads read from; kept alive until the stream has
+// passed them.
+struct RoadStaging {
+  std::vector<double> xyw;
+  std::vector<double4> seg;
+  RoadTail tail;
+};
+
+int road_validate(roadmc_ctx* ctx, const roadmc_road_species* r);
+int road_visible_segments(const roadmc_road_species* r);
+// Device scratch for builds of up to P patches, S segments and C controls.
+int road_reserve(roadmc_ctx* ctx, int P, int S, int C);
+// Uploads one species' patches and road and launches the build on ctx's
+// stream (no synchronisation).  With reset_tail the status word is cleared
+// first; a non-finite transition weight sets tail.bad.  The scratch must
+// already be reserved for this build.
+int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, bool want_surv,
+                bool reset_tail, RoadStaging& st, RoadArgs* out);
+RoadTail* road_tail(roadmc_ctx* ctx);
+
+}  // namespace roadmc
+
+#endif

@@ -45,6 +45,12 @@ inline size_t road_part_elems(int P) { return (size_t)((P + kRoadCols - 1) / kRo
 
 hipError_t launch_road_build(const RoadArgs& a, bool want_trans, bool want_surv, hipStream_t s);
 
+// Packs trans o surv (column-major P x P, FP64; surv may be null) into the
+// MTE kernels' B-operand order (Pp columns, Pk contraction rows), entries
+// whose binary32 value is not above tol dropped when tol > 0.
+hipError_t launch_road_pack(bool fp64, const double* trans, const double* surv, int P, int Pp,
+                            int Pk, double tol, void* out, hipStream_t s);
+
 }  // namespace roadmc
 
 #endif

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 
+#include "mte_kernels.h"
 #include "road_kernels.h"
 
 // The segment test, the distances and the cut-off comparisons must round like
@@ -267,6 +268,36 @@ __global__ __launch_bounds__(kRoadThreads) void road_surv_kernel(RoadArgs a) {
       a.surv[(size_t)c * n + e] =
           (tab && k < kSurvTable) ? table[c][k] : surv_value((double)k, a.flow[c], a);
   }
+}
+
+// Movement-and-mortality matrix in MFMA B order, as the host packing forms
+// it (trans o surv, then dense2Sparse's drop rule on the binary32 value).
+__global__ __launch_bounds__(256) void road_pack_kernel(const double* __restrict__ trans,
+                                                       const double* __restrict__ surv, int P,
+                                                       int Pp, int Pk, double tol, int fp64,
+                                                       void* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)Pk * Pp) return;
+  const int j = (int)(e / Pk), k = (int)(e - (size_t)j * Pk);
+  double v = 0.0;
+  if (k < P && j < P) {
+    const size_t cm = (size_t)j * P + k;
+    v = __dmul_rn(trans[cm], surv ? surv[cm] : 1.0);
+    if (tol > 0.0 && !((float)v > (float)tol)) v = 0.0;
+  }
+  const size_t idx = mte_pack_index(fp64 != 0, Pk, k, j);
+  if (fp64)
+    static_cast<double*>(out)[idx] = v;
+  else
+    static_cast<float*>(out)[idx] = (float)v;
+}
+
+hipError_t launch_road_pack(bool fp64, const double* trans, const double* surv, int P, int Pp,
+                            int Pk, double tol, void* out, hipStream_t s) {
+  const size_t n = (size_t)Pk * Pp;
+  hipLaunchKernelGGL(road_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, trans,
+                     surv, P, Pp, Pk, tol, fp64 ? 1 : 0, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_road_build(const RoadArgs& a, bool want_trans, bool want_surv, hipStream_t s) {

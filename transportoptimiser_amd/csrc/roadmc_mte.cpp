@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "road_host.h"
 #include "mte_kernels.h"
 #include "rng.h"
 #include "roadmc.h"
@@ -61,14 +62,24 @@ int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr) {
     if (P < 1 || P > ROADMC_MAX_PATCHES)
       return rmc_fail(ctx, ROADMC_ERR_INVALID,
                       "n_patches must be in [1, ROADMC_MAX_PATCHES]");
-    if (!sp.init_pop || !sp.capacity || !sp.trans)
+    if (!sp.init_pop || !sp.capacity || (!sp.trans && !sp.road))
       return rmc_fail(ctx, ROADMC_ERR_INVALID,
-                      "init_pop, capacity and trans are required");
+                      "init_pop, capacity and trans (or a road) are required");
+    if (!sp.trans) {
+      // Matrices built on the device from the species' road.
+      int rc = roadmc::road_validate(ctx, sp.road);
+      if (rc) return rc;
+      if (sp.road->n_patches != P)
+        return rmc_fail(ctx, ROADMC_ERR_INVALID, "road n_patches differs from the species'");
+      if (sp.road_control >= sp.road->n_controls)
+        return rmc_fail(ctx, ROADMC_ERR_INVALID, "road_control out of range");
+    }
     for (int j = 0; j < P; ++j)
       if (!(sp.capacity[j] > 0.0) || !std::isfinite(sp.capacity[j]))
         return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "capacity must be finite and > 0");
-    if (!finite_array(sp.init_pop, P) || !finite_array(sp.trans, (size_t)P * P) ||
-        (sp.surv && !finite_array(sp.surv, (size_t)P * P)) ||
+    if (!finite_array(sp.init_pop, P) ||
+        (sp.trans && !finite_array(sp.trans, (size_t)P * P)) ||
+        (sp.trans && sp.surv && !finite_array(sp.surv, (size_t)P * P)) ||
         (sp.growth && !finite_array(sp.growth, P)) || !finite_array(sp.params, 8) ||
         !std::isfinite(sp.noise_sd) || !std::isfinite(sp.threshold_ratio))
       return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite input");
@@ -93,9 +104,10 @@ void pack_species(const roadmc_species& sp, int Pp, int Kp, double tol, bool fp6
                   T* cols) {
   const int P = sp.n_patches;
   std::fill(mp, mp + (size_t)Pp * Kp, (T)0);
-  for (int k = 0; k < P; ++k)
-    for (int j = 0; j < P; ++j)
-      mp[roadmc::mte_pack_index(fp64, Kp, k, j)] = (T)movement(sp, P, k, j, tol);
+  if (sp.trans)  // otherwise packed on the device from the species' road
+    for (int k = 0; k < P; ++k)
+      for (int j = 0; j < P; ++j)
+        mp[roadmc::mte_pack_index(fp64, Kp, k, j)] = (T)movement(sp, P, k, j, tol);
   for (int j = 0; j < Pp; ++j) {
     const bool in = j < P;
     cols[j] = in ? (T)sp.capacity[j] : (T)1;
@@ -262,8 +274,46 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
   // Queue boundaries live after the 16 queue counters in the counter buffer.
   RMC_HIP(ctx, hipMemcpyAsync(static_cast<int*>(b.counter.ptr) + 16, js.qbeg.data(),
                               js.qbeg.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  // Species given by a road: build trans and surv on the device and pack
+  // them into their job's slot (after the zero-filled upload above).
+  int maxP = 0, maxS = 0, maxC = 0;
+  for (size_t j = 0; j < js.jobs.size(); ++j) {
+    const roadmc_species& sp = roads[js.road_of[j]].species[js.species_of[j]];
+    if (sp.trans) continue;
+    maxP = std::max(maxP, sp.road->n_patches);
+    maxS = std::max(maxS, roadmc::road_visible_segments(sp.road));
+    maxC = std::max(maxC, sp.road->n_controls);
+  }
+  std::vector<roadmc::RoadStaging> staging;
+  if (maxP > 0) {
+    if ((rc = roadmc::road_reserve(ctx, maxP, maxS, maxC))) return rc;
+    staging.reserve(js.jobs.size());
+    bool first = true;
+    for (size_t j = 0; j < js.jobs.size(); ++j) {
+      const roadmc_mte_problem& pr = roads[js.road_of[j]];
+      const roadmc_species& sp = pr.species[js.species_of[j]];
+      if (sp.trans) continue;
+      const roadmc_road_species* rd = sp.road;
+      const int control = sp.road_control < 0 ? rd->n_controls - 1 : sp.road_control;
+      staging.emplace_back();
+      roadmc::RoadArgs a;
+      if ((rc = roadmc::road_launch(ctx, rd, true, control >= 0, first, staging.back(), &a)))
+        return rc;
+      first = false;
+      const double* surv = control >= 0 ? a.surv + (size_t)control * rd->n_patches * rd->n_patches
+                                        : nullptr;
+      T* dst = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];
+      RMC_HIP(ctx, roadmc::launch_road_pack(js.fp64, a.trans, surv, rd->n_patches, js.jobs[j].Pp,
+                                            js.jobs[j].Pk, pr.drop_tol, dst, s));
+    }
+  }
+  roadmc::RoadTail tail{0, {0, 0, 0}};
+  if (maxP > 0)
+    RMC_HIP(ctx, hipMemcpyAsync(&tail, roadmc::road_tail(ctx), sizeof(tail),
+                                hipMemcpyDeviceToHost, s));
   // The host vectors die at return: wait for the copies.
   RMC_HIP(ctx, hipStreamSynchronize(s));
+  if (tail.bad) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
   return ROADMC_OK;
 }
 
@@ -733,6 +783,9 @@ extern "C" int roadmc_mte_run_plain(roadmc_ctx* ctx, const roadmc_mte_problem* p
   if (!result) return rmc_fail(ctx, ROADMC_ERR_INVALID, "result is NULL");
   int rc = validate_problem(ctx, problem);
   if (rc) return rc;
+  for (int sp = 0; sp < problem->n_species; ++sp)
+    if (!problem->species[sp].trans)
+      return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the plain path needs host matrices");
   RMC_HIP(ctx, hipSetDevice(ctx->device));
   const int S = problem->n_species;
   const int64_t N = problem->n_paths;
@@ -857,6 +910,8 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   *out = nullptr;
   int rc = validate_problem(ctx, pr);
   if (rc) return rc;
+  if (!pr->species[0].trans)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the adjoint needs host matrices");
   if (pr->n_species != 1 || pr->growth_law != ROADMC_LAW_LOGNORMAL ||
       pr->endpoint != ROADMC_END_POST_GROWTH || pr->drop_tol != 0.0)
     return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,

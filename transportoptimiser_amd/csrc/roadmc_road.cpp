@@ -8,12 +8,13 @@
 #include <vector>
 
 #include "ctx.h"
+#include "road_host.h"
 #include "road_kernels.h"
 #include "roadmc.h"
 
-namespace {
+namespace roadmc {
 
-int validate_road(roadmc_ctx* ctx, const roadmc_road_species* r) {
+int road_validate(roadmc_ctx* ctx, const roadmc_road_species* r) {
   if (!r) return rmc_fail(ctx, ROADMC_ERR_INVALID, "road species is NULL");
   if (r->n_patches < 1 || r->n_patches > 16384)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_patches must be in [1, 16384]");
@@ -37,62 +38,68 @@ int validate_road(roadmc_ctx* ctx, const roadmc_road_species* r) {
   return ROADMC_OK;
 }
 
-}  // namespace
+// Only open-road segments count (roadCrossings, SpeciesRoadPatches.cpp:337-347).
+int road_visible_segments(const roadmc_road_species* r) {
+  int S = 0;
+  for (int s = 0; s + 1 < r->n_road_points; ++s)
+    if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD) ++S;
+  return S;
+}
 
-extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_species* r,
-                                         int32_t* crossings, double* trans, double* surv) {
-  if (!ctx) return ROADMC_ERR_INVALID;
-  int rc = validate_road(ctx, r);
-  if (rc) return rc;
-  if (!crossings && !trans && !surv)
-    return rmc_fail(ctx, ROADMC_ERR_INVALID, "no output requested");
-  RMC_HIP(ctx, hipSetDevice(ctx->device));
+int road_reserve(roadmc_ctx* ctx, int P, int S, int C) {
+  const size_t n = (size_t)P * P;
+  auto& B = ctx->road_bufs;
+  int rc;
+  if ((rc = rmc_buf_ensure(ctx, B.a, 3 * (size_t)P * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.b, std::max<size_t>(S, 1) * sizeof(double4))) ||
+      (rc = rmc_buf_ensure(ctx, B.c, n * sizeof(int32_t))) ||
+      (rc = rmc_buf_ensure(ctx, B.d, n * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.e, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.g, (size_t)road_cross_blocks(P) * sizeof(int))) ||
+      (rc = rmc_buf_ensure(ctx, B.h, road_part_elems(P) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.tail, sizeof(RoadTail))))
+    return rc;
+  return ROADMC_OK;
+}
+
+RoadTail* road_tail(roadmc_ctx* ctx) { return static_cast<RoadTail*>(ctx->road_bufs.tail.ptr); }
+
+int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, bool want_surv,
+                bool reset_tail, RoadStaging& st, RoadArgs* out) {
   const int P = r->n_patches, C = r->n_controls;
   // Transition rate and the two distance cut-offs, as the reference forms
   // them (SpeciesRoadPatches.cpp:328-332, 387-395).
   const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
   const double cut_cross = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * r->lambda_mult));
   const double cut_trans = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * lambda));
-  std::vector<double> pw(P);
+  st.xyw.resize(3 * (size_t)P);
+  std::memcpy(st.xyw.data(), r->x, P * sizeof(double));
+  std::memcpy(st.xyw.data() + P, r->y, P * sizeof(double));
   for (int j = 0; j < P; ++j)
-    pw[j] = r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
-  // Only open-road segments count (roadCrossings, SpeciesRoadPatches.cpp:337-347).
-  std::vector<double4> seg;
+    st.xyw[2 * (size_t)P + j] =
+        r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
+  st.seg.clear();
   for (int s = 0; s + 1 < r->n_road_points; ++s)
     if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD)
-      seg.push_back(make_double4(r->road_x[s], r->road_y[s], r->road_x[s + 1], r->road_y[s + 1]));
-  const int S = (int)seg.size();
-
-  const size_t n = (size_t)P * P;
-  const size_t tail_off = (n * sizeof(int32_t) + 255) / 256 * 256;
+      st.seg.push_back(make_double4(r->road_x[s], r->road_y[s], r->road_x[s + 1], r->road_y[s + 1]));
+  const int S = (int)st.seg.size();
   auto& B = ctx->road_bufs;
-  if ((rc = rmc_buf_ensure(ctx, B.a, 3 * (size_t)P * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.b, std::max<size_t>(S, 1) * sizeof(double4))) ||
-      (rc = rmc_buf_ensure(ctx, B.c, tail_off + sizeof(roadmc::RoadTail))) ||
-      (rc = rmc_buf_ensure(ctx, B.d, n * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.e, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.g, (size_t)roadmc::road_cross_blocks(P) * sizeof(int))) ||
-      (rc = rmc_buf_ensure(ctx, B.h, roadmc::road_part_elems(P) * sizeof(double))))
-    return rc;
   hipStream_t s = ctx->stream;
   double* d_xyw = static_cast<double*>(B.a.ptr);
-  std::vector<double> xyw(3 * (size_t)P);
-  std::memcpy(xyw.data(), r->x, P * sizeof(double));
-  std::memcpy(xyw.data() + P, r->y, P * sizeof(double));
-  std::memcpy(xyw.data() + 2 * P, pw.data(), P * sizeof(double));
-  RMC_HIP(ctx, hipMemcpyAsync(d_xyw, xyw.data(), xyw.size() * sizeof(double),
+  RMC_HIP(ctx, hipMemcpyAsync(d_xyw, st.xyw.data(), st.xyw.size() * sizeof(double),
                               hipMemcpyHostToDevice, s));
   if (S > 0)
-    RMC_HIP(ctx, hipMemcpyAsync(B.b.ptr, seg.data(), S * sizeof(double4), hipMemcpyHostToDevice,
-                                s));
+    RMC_HIP(ctx, hipMemcpyAsync(B.b.ptr, st.seg.data(), S * sizeof(double4),
+                                hipMemcpyHostToDevice, s));
   if (C > 0)
     RMC_HIP(ctx, hipMemcpyAsync(B.f.ptr, r->flow, C * sizeof(double), hipMemcpyHostToDevice, s));
-  auto* d_tail = reinterpret_cast<roadmc::RoadTail*>(static_cast<char*>(B.c.ptr) + tail_off);
-  roadmc::RoadTail tail{0, {0, 0, 0}};
-  RMC_HIP(ctx, hipMemcpyAsync(d_tail, &tail, sizeof(tail), hipMemcpyHostToDevice, s));
-
-  roadmc::RoadArgs a{};
+  if (reset_tail) {
+    st.tail = RoadTail{0, {0, 0, 0}};
+    RMC_HIP(ctx, hipMemcpyAsync(road_tail(ctx), &st.tail, sizeof(RoadTail),
+                                hipMemcpyHostToDevice, s));
+  }
+  RoadArgs a{};
   a.P = P;
   a.S = S;
   a.C = C;
@@ -113,13 +120,36 @@ extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_spec
   a.surv = static_cast<double*>(B.e.ptr);
   a.blockmax = static_cast<int*>(B.g.ptr);
   a.part = static_cast<double*>(B.h.ptr);
-  a.tail = d_tail;
-  RMC_HIP(ctx, roadmc::launch_road_build(a, trans != nullptr, surv != nullptr, s));
-  RMC_HIP(ctx, hipMemcpyAsync(&tail, d_tail, sizeof(tail), hipMemcpyDeviceToHost, s));
+  a.tail = road_tail(ctx);
+  RMC_HIP(ctx, launch_road_build(a, want_trans, want_surv, s));
+  *out = a;
+  return ROADMC_OK;
+}
+
+}  // namespace roadmc
+
+extern "C" int roadmc_build_species_road(roadmc_ctx* ctx, const roadmc_road_species* r,
+                                         int32_t* crossings, double* trans, double* surv) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  int rc = roadmc::road_validate(ctx, r);
+  if (rc) return rc;
+  if (!crossings && !trans && !surv)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "no output requested");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  const int P = r->n_patches, C = r->n_controls;
+  if ((rc = roadmc::road_reserve(ctx, P, roadmc::road_visible_segments(r), C))) return rc;
+  roadmc::RoadStaging st;
+  roadmc::RoadArgs a;
+  if ((rc = roadmc::road_launch(ctx, r, trans != nullptr, surv != nullptr, true, st, &a)))
+    return rc;
+  hipStream_t s = ctx->stream;
+  roadmc::RoadTail tail;
+  RMC_HIP(ctx, hipMemcpyAsync(&tail, roadmc::road_tail(ctx), sizeof(tail), hipMemcpyDeviceToHost, s));
   RMC_HIP(ctx, hipStreamSynchronize(s));
   if (trans && tail.bad)
     return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
   // The caller's buffers are written only once the build has succeeded.
+  const size_t n = (size_t)P * P;
   if (crossings)
     RMC_HIP(ctx, hipMemcpyAsync(crossings, a.cross, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   if (trans)

@@ -27,7 +27,7 @@ from . import native as nat
 class SpeciesData:
     """One species on one road: the data SpeciesRoadPatches exposes."""
 
-    trans: np.ndarray            # (P, P) row = origin, rows sum to 1
+    trans: np.ndarray | None     # (P, P) row = origin, rows sum to 1
     surv: np.ndarray | None      # (P, P) survival for the control in force
     init_pop: np.ndarray         # (P,)
     capacity: np.ndarray         # (P,)
@@ -35,6 +35,11 @@ class SpeciesData:
     params: np.ndarray = field(default_factory=lambda: np.zeros(8))
     noise_sd: float = 0.1
     threshold_ratio: float = 0.5
+    # With trans None: the road (a road.RoadSpeciesInput) from which the
+    # library builds trans and the survival of road_control on the device
+    # (-1: the last control).
+    road: object = None
+    road_control: int = -1
 
     @property
     def n_patches(self) -> int:

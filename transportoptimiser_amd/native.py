@@ -32,17 +32,7 @@ _ip = C.POINTER(C.c_int32)
 
 
 class Species(C.Structure):
-    _fields_ = [
-        ("n_patches", C.c_int32),
-        ("init_pop", _dp),
-        ("capacity", _dp),
-        ("growth", _dp),
-        ("trans", _dp),
-        ("surv", _dp),
-        ("params", C.c_double * 8),
-        ("noise_sd", C.c_double),
-        ("threshold_ratio", C.c_double),
-    ]
+    pass  # fields set below, after RoadSpecies
 
 
 class MteProblem(C.Structure):
@@ -254,6 +244,21 @@ class SpeciesFile(C.Structure):
                 ("pref_sd", C.c_double * MAX_HABITAT_TYPES)] + [
         (k, C.c_double) for k in ("growth_current", "growth_mean", "growth_sd",
                                   "growth_reversion", "growth_jump_prob", "growth_poisson_jump")]
+
+
+Species._fields_ = [
+    ("n_patches", C.c_int32),
+    ("init_pop", _dp),
+    ("capacity", _dp),
+    ("growth", _dp),
+    ("trans", _dp),
+    ("surv", _dp),
+    ("params", C.c_double * 8),
+    ("noise_sd", C.c_double),
+    ("threshold_ratio", C.c_double),
+    ("road", C.POINTER(RoadSpecies)),
+    ("road_control", C.c_int32),
+]
 
 
 # Every symbol include/roadmc.h declares: name -> (restype, argtypes).

@@ -61,12 +61,12 @@ def _d(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def build_species_road(ctx, inp: RoadSpeciesInput, *, crossings: bool = True,
-                       trans: bool = True, surv: bool = True) -> RoadMatrices:
-    """One call of ``roadmc_build_species_road`` on ``ctx``'s device."""
+def road_struct(inp: RoadSpeciesInput, keep: list) -> "nat.RoadSpecies":
+    """The ``roadmc_road_species`` struct for ``inp``; the arrays it points
+    into are appended to ``keep``, which must outlive every call using it."""
     P = inp.n_patches
-    keep = [_d(inp.x), _d(inp.y), _d(inp.area), _d(inp.pref_mean), _d(inp.pref_sd)]
-    for a in keep:
+    pa = [_d(inp.x), _d(inp.y), _d(inp.area), _d(inp.pref_mean), _d(inp.pref_sd)]
+    for a in pa:
         if a.shape != (P,):
             raise ValueError("patch arrays must all have shape (P,)")
     road = _d(inp.road).reshape(-1, 2)
@@ -77,9 +77,10 @@ def build_species_road(ctx, inp: RoadSpeciesInput, *, crossings: bool = True,
         rt = np.ascontiguousarray(inp.road_type, dtype=np.int32)
         if rt.shape != (max(len(road) - 1, 0),):
             raise ValueError("road_type needs one entry per segment")
+    keep.extend(pa + [rx, ry, flows] + ([rt] if rt is not None else []))
     rs = nat.RoadSpecies()
     rs.n_patches = P
-    rs.x, rs.y, rs.area, rs.pref_mean, rs.pref_sd = (a.ctypes.data_as(nat._dp) for a in keep)
+    rs.x, rs.y, rs.area, rs.pref_mean, rs.pref_sd = (a.ctypes.data_as(nat._dp) for a in pa)
     rs.pref_mult = float(inp.pref_mult)
     rs.lambda_mean = float(inp.lambda_mean)
     rs.lambda_sd = float(inp.lambda_sd)
@@ -95,6 +96,17 @@ def build_species_road(ctx, inp: RoadSpeciesInput, *, crossings: bool = True,
     rs.animal_speed = float(inp.animal_speed)
     rs.transition_form = int(inp.transition_form)
     rs.drop_last_link = 1 if inp.drop_last_link else 0
+    keep.append(rs)
+    return rs
+
+
+def build_species_road(ctx, inp: RoadSpeciesInput, *, crossings: bool = True,
+                       trans: bool = True, surv: bool = True) -> RoadMatrices:
+    """One call of ``roadmc_build_species_road`` on ``ctx``'s device."""
+    P = inp.n_patches
+    keep: list = []
+    rs = road_struct(inp, keep)
+    flows = keep[7]
     cr = np.empty(P * P, dtype=np.int32) if crossings else None
     tr = np.empty(P * P) if trans else None
     sv = np.empty(len(flows) * P * P) if surv else None
@@ -138,6 +150,17 @@ class SpeciesRoadPatches:
         if self.trans_probs is None:
             self.create_species_model()
         return SpeciesData(trans=self.trans_probs, surv=self.surv_probs[control],
+                           init_pop=np.asarray(init_pop, dtype=np.float64),
+                           capacity=np.asarray(capacity, dtype=np.float64),
+                           growth=np.asarray(growth, dtype=np.float64), **kw)
+
+    def device_species(self, control: int, init_pop: np.ndarray, capacity: np.ndarray,
+                       growth: np.ndarray, **kw) -> SpeciesData:
+        """SpeciesData that carries the road instead of matrices: the batched
+        Monte Carlo path builds trans and the survival of ``control`` on the
+        device and packs them straight into its movement operand, so no
+        matrix crosses PCIe (``control`` -1: the last)."""
+        return SpeciesData(trans=None, surv=None, road=self.inp, road_control=int(control),
                            init_pop=np.asarray(init_pop, dtype=np.float64),
                            capacity=np.asarray(capacity, dtype=np.float64),
                            growth=np.asarray(growth, dtype=np.float64), **kw)
