@@ -189,6 +189,21 @@ def run_extras(ctx, roads, prep_ms) -> dict:
         "what": "species-state ROV (ALGO4): 1 species x 256 patches, 3 flow controls, "
                 "100,000 paths, 50 years, 20 x 20 kNN grids; host call"}
     log(f"rov species: {out['rov_species_algo4']}")
+    pr5.method = 6
+    rsp.simulate_rov_species(ctx, pr5)
+    t = time.perf_counter()
+    r6 = rsp.simulate_rov_species(ctx, pr5)
+    ms = (time.perf_counter() - t) * 1e3
+    # Recomputation work: every step t re-simulates the years t .. T-2 of
+    # every path under C products of 2 N P^2 FLOP.
+    T5 = pr5.n_years
+    years6 = sum(T5 - 1 - t for t in range(1, T5 - 1))
+    fl6 = 2.0 * pr5.n_paths * pr5.n_patches ** 2 * len(pr5.flow) * years6
+    out["rov_species_algo6"] = {
+        "ms": round(ms, 2), "value": round(r6["value"], 4), "first_control": r6["first_control"],
+        "recompute_tflop": round(fl6 / 1e12, 2), "tflops_whole_call": round(fl6 / ms / 1e9, 1),
+        "what": "species-state ROV with forward recomputation (ALGO6), same problem; host call"}
+    log(f"rov species ALGO6: {out['rov_species_algo6']}")
     from transportoptimiser_amd import knn as kn
     rng = np.random.default_rng(1)
     xk = rng.uniform(0.0, 1.0, size=(4096, 3))

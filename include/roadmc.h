@@ -359,6 +359,22 @@ typedef struct {
                               probability, local variability */
 } roadmc_rovs_species;
 
+/* The valuation methods (Optimiser::ROVType, Optimiser.h:74-82).  ALGO6
+ * (SimulateGPU.cu:5011-5444, optimalForwardPaths :1199-2206): after each
+ * step's grids are fitted, every path is recomputed from that step to the
+ * horizon, re-simulated to the step under its drawn controls and then under
+ * the fitted policy (each year's decision on the recomputed state with that
+ * year's grids, the terminal rule in the last year), and its value at the step
+ * is the realized cash flows, Q_c x unit profit of each year's chosen control
+ * discounted by 1 / (1 + rrr dt / 100) per year; those values are the next
+ * step's regression responses.  The reference's kernel accumulates that value
+ * but leaves its store commented out (:2191) and migrates with the stored
+ * rather than the recomputed decisions; here the value is kept and the policy
+ * followed.  S 32 x (P rounded up to 32) + 4 float tiles must fit the 160 KiB
+ * of LDS of one workgroup (S <= 2 at P = 512, S <= 4 at P = 256). */
+#define ROADMC_ROVS_ALGO4 4
+#define ROADMC_ROVS_ALGO6 6
+
 /* SimulateGPU::simulateROVCUDA with ROVMethod ALGO4 (SimulateGPU.cu:3870-4998):
  *  - forward (forwardPathKernel, :789-956): per path and year a control drawn
  *    uniformly (randControls, :720-730); each species' patch populations moved
@@ -405,6 +421,12 @@ typedef struct {
   double hysteresis;                      /* 0.975 in SimulateGPU.cu:2763 */
   uint64_t seed;
   uint32_t road_id;
+  int32_t method;                         /* ROADMC_ROVS_ALGO4 (0 means the same) or
+                                             ROADMC_ROVS_ALGO6 */
+  int32_t checkpoint_years;               /* ALGO6: forward populations kept every this
+                                             many years, the recomputation replaying from
+                                             the last one (0: every year while they fit
+                                             in 16 GiB) */
 } roadmc_rovs_problem;
 
 typedef struct {

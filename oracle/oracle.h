@@ -200,6 +200,13 @@ int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_po
 int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const double* total_pops,
                       const double* aars, const double* profit, double* V, int32_t* optc,
                       double out[3]);
+/* The same with the method named: 4 as above; 6 values every path at each
+ * step by recomputing it forward from that step under the fitted policy
+ * (realized cash flows, V [t] the realized value).  grids [T][C][gsz] (steps
+ * 1 .. T-2 filled) may be NULL. */
+int orc_rovs_backward_method(const orc_rovs_spec* s, int32_t method, const int32_t* controls,
+                             const double* total_pops, const double* aars, const double* profit,
+                             double* V, int32_t* optc, double* grids, double out[3]);
 
 #ifdef __cplusplus
 }

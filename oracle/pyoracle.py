@@ -441,6 +441,9 @@ def _rovs_lib():
     h.orc_rovs_forward.restype = C.c_int
     h.orc_rovs_backward.argtypes = [C.POINTER(OrcRovsSpec), _ip, _dp, _dp, _dp, _dp, _ip, _dp]
     h.orc_rovs_backward.restype = C.c_int
+    h.orc_rovs_backward_method.argtypes = [C.POINTER(OrcRovsSpec), C.c_int32, _ip, _dp, _dp, _dp,
+                                           _dp, _ip, _dp, _dp]
+    h.orc_rovs_backward_method.restype = C.c_int
     return h
 
 
@@ -460,11 +463,13 @@ def rovs_forward(pr):
     return ctl, tp, aa, up
 
 
-def rovs_backward(pr, controls, total_pops, aars, profit):
-    """(V [T][N], opt_control [T][N], {value, sd, control})."""
+def rovs_backward(pr, controls, total_pops, aars, profit, method=4, grids=False):
+    """(V [T][N], opt_control [T][N], {value, sd, control}) for method 4 or 6
+    (the recomputation); with grids=True also the fitted grids [T][C][gsz]."""
     keep = []
     s = _rovs_spec(pr, keep)
     T, N = pr.n_years, pr.n_paths
+    D, R, Cc = len(pr.species) + 1, pr.dim_res, len(pr.flow)
     ctl = np.ascontiguousarray(controls, dtype=np.int32)
     tp = np.ascontiguousarray(total_pops, dtype=np.float64)
     aa = np.ascontiguousarray(aars, dtype=np.float64)
@@ -472,10 +477,13 @@ def rovs_backward(pr, controls, total_pops, aars, profit):
     V = np.zeros((T, N))
     oc = np.zeros((T, N), dtype=np.int32)
     out = np.zeros(3)
-    rc = _rovs_lib().orc_rovs_backward(C.byref(s), ctl.ctypes.data_as(_ip), tp.ctypes.data_as(_dp),
-                                       aa.ctypes.data_as(_dp), up.ctypes.data_as(_dp),
-                                       V.ctypes.data_as(_dp), oc.ctypes.data_as(_ip),
-                                       out.ctypes.data_as(_dp))
+    g = np.zeros((T, Cc, D * R + R ** D)) if grids else None
+    rc = _rovs_lib().orc_rovs_backward_method(
+        C.byref(s), int(method), ctl.ctypes.data_as(_ip), tp.ctypes.data_as(_dp),
+        aa.ctypes.data_as(_dp), up.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+        oc.ctypes.data_as(_ip), g.ctypes.data_as(_dp) if g is not None else None,
+        out.ctypes.data_as(_dp))
     if rc != 0:
-        raise RuntimeError(f"orc_rovs_backward failed: {rc}")
-    return V, oc, {"value": out[0], "sd": out[1], "control": int(out[2])}
+        raise RuntimeError(f"orc_rovs_backward_method failed: {rc}")
+    res = (V, oc, {"value": out[0], "sd": out[1], "control": int(out[2])})
+    return res + (g,) if grids else res

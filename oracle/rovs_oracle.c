@@ -60,6 +60,52 @@ static uint32_t rv_tag(uint32_t road, uint32_t species, uint32_t kind) {
   return (road << 8) | ((species & 63u) << 2) | (kind & 3u);
 }
 
+/* Every control's product y_c = n M_c (M_c = trans o surv_c, y[c][j] =
+ * sum_k n_k M_c(k, j)) and its total rs[c]. */
+static void rv_products(const orc_rovs_spec* s, int sp, const double* n, double* y, double* rs) {
+  const int P = s->P, C = s->C;
+  const double* tr = s->trans + (size_t)sp * P * P;
+  for (int c = 0; c < C; ++c) {
+    const double* sv = s->surv + ((size_t)sp * s->C + c) * P * P;
+    double r = 0.0;
+    for (int j = 0; j < P; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < P; ++k) acc += n[k] * (tr[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+      y[(size_t)c * P + j] = acc;
+      r += acc;
+    }
+    rs[c] = r;
+  }
+}
+
+/* Year t of species sp on path p after migration to yc: the growth-rate SDE
+ * step (jump probability from its own field), then
+ * n = y (1 + g (K - y) / K / 100) with g = grMean + z local_variability.
+ * Returns the new total. */
+static double rv_grow(const orc_rovs_spec* s, int sp, int64_t p, int t, const double* yc,
+                      double* n, double* grm) {
+  const int P = s->P;
+  const double* prm = s->params + sp * 8;
+  const double* K = s->capacity + (size_t)sp * P;
+  const uint32_t tag0 = rv_tag(s->road_id, (uint32_t)sp, 0);
+  const uint32_t tag1 = rv_tag(s->road_id, (uint32_t)sp, 1);
+  double sc[3];
+  orc_path_scalars(s->seed, tag1, (uint32_t)p, (uint32_t)t, sc);
+  const double jump = sc[2] < prm[6] ? 1.0 : 0.0;
+  const double jsz = sc[1] * prm[5] * prm[5] - prm[5] * prm[5] / 2.0;
+  const double g0 = *grm;
+  *grm = g0 + prm[4] * (prm[1] - g0) * s->dt + g0 * (sc[0] * prm[2]) + (exp(jsz) - 1.0) * g0 * jump;
+  double tot = 0.0;
+  for (int j = 0; j < P; ++j) {
+    const double xv = yc[j];
+    const double z = orc_patch_normal(s->seed, tag0, (uint32_t)p, (uint32_t)t, (uint32_t)j, 0);
+    const double g = *grm + z * prm[7];
+    n[j] = xv * (1.0 + g * (K[j] - xv) / K[j] / 100.0);
+    tot += n[j];
+  }
+  return tot;
+}
+
 int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_pops, double* aars,
                      double* profit) {
   const int P = s->P, S = s->S, C = s->C, T = s->T, U = s->n_fuels + s->n_comm;
@@ -131,12 +177,7 @@ int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_po
     }
     /* species */
     for (int sp = 0; sp < S; ++sp) {
-      const double* prm = s->params + sp * 8;
-      const double* K = s->capacity + (size_t)sp * P;
-      const double* tr = s->trans + (size_t)sp * P * P;
-      const uint32_t tag0 = rv_tag(s->road_id, (uint32_t)sp, 0);
-      const uint32_t tag1 = rv_tag(s->road_id, (uint32_t)sp, 1);
-      double tot = 0.0, grm = prm[0];
+      double tot = 0.0, grm = s->params[sp * 8 + 0], rs[8];
       for (int j = 0; j < P; ++j) {
         n[j] = s->init_pop[(size_t)sp * P + j];
         tot += n[j];
@@ -144,32 +185,9 @@ int orc_rovs_forward(const orc_rovs_spec* s, int32_t* controls, double* total_po
       total_pops[((size_t)sp * (T + 1)) * N + p] = tot;
       for (int t = 0; t < T; ++t) {
         const int ctl = controls[(size_t)t * N + p];
-        for (int c = 0; c < C; ++c) {
-          const double* sv = s->surv + ((size_t)sp * C + c) * P * P;
-          double rs = 0.0;
-          for (int j = 0; j < P; ++j) {
-            double acc = 0.0;
-            for (int k = 0; k < P; ++k)
-              acc += n[k] * (tr[(size_t)j * P + k] * sv[(size_t)j * P + k]);
-            y[(size_t)c * P + j] = acc;
-            rs += acc;
-          }
-          aars[(((size_t)sp * T + t) * C + c) * N + p] = rs / tot;
-        }
-        double sc[3];
-        orc_path_scalars(s->seed, tag1, (uint32_t)p, (uint32_t)t, sc);
-        const double jump = sc[2] < prm[6] ? 1.0 : 0.0;
-        const double jsz = sc[1] * prm[5] * prm[5] - prm[5] * prm[5] / 2.0;
-        grm = grm + prm[4] * (prm[1] - grm) * s->dt + grm * (sc[0] * prm[2]) +
-              (exp(jsz) - 1.0) * grm * jump;
-        tot = 0.0;
-        for (int j = 0; j < P; ++j) {
-          const double xv = y[(size_t)ctl * P + j];
-          const double z = orc_patch_normal(s->seed, tag0, (uint32_t)p, (uint32_t)t, (uint32_t)j, 0);
-          const double g = grm + z * prm[7];
-          n[j] = xv * (1.0 + g * (K[j] - xv) / K[j] / 100.0);
-          tot += n[j];
-        }
+        rv_products(s, sp, n, y, rs);
+        for (int c = 0; c < C; ++c) aars[(((size_t)sp * T + t) * C + c) * N + p] = rs[c] / tot;
+        tot = rv_grow(s, sp, p, t, y + (size_t)ctl * P, n, &grm);
         total_pops[((size_t)sp * (T + 1) + t + 1) * N + p] = tot;
       }
     }
@@ -187,139 +205,243 @@ static double rv_adj(const orc_rovs_spec* s, const double* tp, const double* aar
          aars[(((size_t)sp * s->T + t) * s->C + c) * N + p];
 }
 
-int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const double* tp,
-                      const double* aars, const double* profit, double* V, int32_t* optc,
-                      double out[3]) {
-  const int S = s->S, C = s->C, T = s->T, D = s->S + 1, R = s->R;
+/* The decision of one path at one step: each valid control's payoff
+ * (control 0 always valid; control c > 0 needs every species' adjusted
+ * population adj[sp][c] at its threshold) is Q_c x unit profit plus, with a
+ * grid, the discounted continuation interpolated at the highest-flow state x
+ * and clamped to <= 0 (backwardInduction, SimulateGPU.cu:2453-2771), without
+ * one the terminal payoff min(Q_c x unit profit, 0) (:2375-2452); a control
+ * replaces the running best only below hysteresis x best (1 at the terminal
+ * step).  Returns the control; *best is its payoff. */
+static int rv_decide(const orc_rovs_spec* s, const double* grid, const double* x,
+                     const double* adj, double* best_out) {
+  const int S = s->S, C = s->C, D = S + 1, R = s->R;
+  size_t G = 1;
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  const double disc = 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  const double hyst = grid ? s->hysteresis : 1.0;
+  double best = 0.0;
+  int bc = 0;
+  for (int c = 0; c < C; ++c) {
+    int valid = 1;
+    for (int sp = 0; sp < S && c > 0; ++sp)
+      if (adj[sp * C + c] < s->params[sp * 8 + 3]) valid = 0;
+    double pay = 0.0;
+    if (valid) {
+      if (grid) {
+        double cont;
+        orc_knn_eval(D, R, grid + (size_t)c * gsz, 1, x, &cont);
+        pay = s->flow[c] * x[S] + fmin(cont * disc, 0.0);
+      } else {
+        pay = fmin(s->flow[c] * x[S], 0.0);
+      }
+    }
+    if (c == 0) {
+      best = pay;
+    } else if (valid && pay < hyst * best) {
+      best = pay;
+      bc = c;
+    }
+  }
+  *best_out = best;
+  return bc;
+}
+
+/* The samples of step t (allocateXYRegressionData, :1010-1068), their
+ * ranges (computeStateMinMax, :1073-1147), the scaled kNN coordinates
+ * (:4582-4607) and one local-linear grid per control into grid [C][gsz]. */
+static int rv_fit_step(const orc_rovs_spec* s, int t, const int32_t* controls, const double* tp,
+                       const double* aars, const double* profit, const double* V, double* xs,
+                       double* us, double* ys, double* qtab, double* grid) {
+  const int S = s->S, C = s->C, D = s->S + 1, R = s->R;
+  const int64_t N = s->N;
+  size_t G = 1;
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  int64_t cnt[8];
+  double lo[8], hi[8];
+  for (int c = 0; c < C; ++c) cnt[c] = 0;
+  for (int64_t p = 0; p < N; ++p) {
+    double x[8];
+    int valid = 1;
+    for (int sp = 0; sp < S; ++sp) {
+      x[sp] = rv_adj(s, tp, aars, sp, t, C - 1, p);
+      if (x[sp] < s->params[sp * 8 + 3]) valid = 0;
+    }
+    x[S] = profit[(size_t)t * N + p];
+    const int c = controls[(size_t)t * N + p];
+    if (!(valid || c == 0)) continue;
+    for (int d = 0; d < D; ++d) xs[((size_t)c * N + cnt[c]) * D + d] = x[d];
+    ys[(size_t)c * N + cnt[c]] = V[(size_t)(t + 1) * N + p];
+    ++cnt[c];
+  }
+  for (int d = 0; d < D; ++d) {
+    lo[d] = INFINITY;
+    hi[d] = -INFINITY;
+    for (int c = 0; c < C; ++c) {
+      if (c > 0 && cnt[c] <= D + 1) continue;
+      for (int64_t j = 0; j < cnt[c]; ++j) {
+        lo[d] = fmin(lo[d], xs[((size_t)c * N + j) * D + d]);
+        hi[d] = fmax(hi[d], xs[((size_t)c * N + j) * D + d]);
+      }
+    }
+    if (!(lo[d] <= hi[d])) lo[d] = hi[d] = 0.0;
+  }
+#define RV_SCALE(dd, v)                                                                    \
+  (!((hi[dd] - lo[dd]) > 0.0) ? 0.0                                                        \
+   : (dd) == D - 1             ? ((v) - lo[dd]) / (s->scaling * (hi[dd] - lo[dd]))         \
+                               : ((v) - lo[dd]) / (hi[dd] - lo[dd]))
+  for (int c = 0; c < C; ++c)
+    for (int64_t j = 0; j < cnt[c]; ++j)
+      for (int d = 0; d < D; ++d)
+        us[((size_t)c * N + j) * D + d] = RV_SCALE(d, xs[((size_t)c * N + j) * D + d]);
+  for (int d = 0; d < D; ++d)
+    for (int i = 0; i < R; ++i) {
+      const double q = (double)i * (hi[d] - lo[d]) / (double)(R - 1) + lo[d];
+      qtab[d * R + i] = RV_SCALE(d, q);
+      for (int c = 0; c < C; ++c) grid[(size_t)c * gsz + (size_t)d * R + i] = q;
+    }
+#undef RV_SCALE
+  for (int c = 0; c < C; ++c) {
+    const int64_t nc = cnt[c];
+    int k = nc > 3 ? (int)(10.0 * (log((double)nc) + 1.0)) : 0;
+    if (k > 128) k = 128;
+    if (k > nc) k = (int)nc;
+    double* val = grid + (size_t)c * gsz + (size_t)D * R;
+    if (nc < 3 || k < 1) {
+      for (size_t g = 0; g < G; ++g) val[g] = 0.0;
+    } else if (orc_knn_fit_scaled((int32_t)nc, D, us + (size_t)c * N * D, ys + (size_t)c * N, R, k,
+                                  qtab, val) != 0) {
+      return -2;
+    }
+  }
+  return 0;
+}
+
+/* The recomputation of path p from step t (optimalForwardPaths,
+ * SimulateGPU.cu:1199-2206, the ALGO6 loop :5011-5444): the populations at t
+ * re-simulated under the drawn controls, then per year u = t .. T-1 the
+ * decision on the recomputed state with the grid of step u (the terminal rule
+ * at T-1), the chosen control's cash flow Q_c x unit profit (the terminal
+ * payoff at T-1) discounted to t, and the migration and growth under that
+ * control.  Returns the realized value; *c_t is the decision at t and x_t the
+ * highest-flow state at t. */
+static double rv_recompute_path(const orc_rovs_spec* s, int t, int64_t p,
+                                const int32_t* controls, const double* profit,
+                                const double* grids, double* n /*[S][P]*/, double* y /*[C][P]*/,
+                                int* c_t, double* x_t) {
+  const int S = s->S, C = s->C, T = s->T, P = s->P, D = S + 1, R = s->R;
   const int64_t N = s->N;
   size_t G = 1;
   for (int d = 0; d < D; ++d) G *= (size_t)R;
   const size_t gsz = (size_t)D * R + G;
   const double disc = 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  double grm[8], rs[8], adj[64], x[8];
+  for (int sp = 0; sp < S; ++sp) {
+    grm[sp] = s->params[sp * 8 + 0];
+    for (int j = 0; j < P; ++j) n[(size_t)sp * P + j] = s->init_pop[(size_t)sp * P + j];
+    for (int u = 0; u < t; ++u) {
+      rv_products(s, sp, n + (size_t)sp * P, y, rs);
+      rv_grow(s, sp, p, u, y + (size_t)controls[(size_t)u * N + p] * P, n + (size_t)sp * P,
+              &grm[sp]);
+    }
+  }
+  double value = 0.0, f = 1.0;
+  for (int u = t; u < T; ++u) {
+    /* states: every control's total after one migration */
+    for (int sp = 0; sp < S; ++sp) {
+      rv_products(s, sp, n + (size_t)sp * P, y, rs);
+      for (int c = 0; c < C; ++c) adj[sp * C + c] = rs[c];
+      x[sp] = rs[C - 1];
+    }
+    x[S] = profit[(size_t)u * N + p];
+    double best;
+    const int last = u == T - 1;
+    const int bc = rv_decide(s, last ? NULL : grids + (size_t)u * C * gsz, x, adj, &best);
+    value += f * (last ? best : s->flow[bc] * x[S]);
+    f *= disc;
+    if (u == t) {
+      *c_t = bc;
+      for (int d = 0; d < S; ++d) x_t[d] = x[d];
+    }
+    if (last) break;
+    for (int sp = 0; sp < S; ++sp) {
+      rv_products(s, sp, n + (size_t)sp * P, y, rs);
+      rv_grow(s, sp, p, u, y + (size_t)bc * P, n + (size_t)sp * P, &grm[sp]);
+    }
+  }
+  return value;
+}
+
+static int rv_backward(const orc_rovs_spec* s, int method, const int32_t* controls,
+                       const double* tp, const double* aars, const double* profit, double* V,
+                       int32_t* optc, double* grids_out, double out[3]) {
+  const int S = s->S, C = s->C, T = s->T, D = s->S + 1, R = s->R, P = s->P;
+  const int64_t N = s->N;
+  size_t G = 1;
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  const double disc = 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  const int recompute = method == 6;
   double* xs = (double*)malloc(sizeof(double) * (size_t)C * N * D);
   double* us = (double*)malloc(sizeof(double) * (size_t)C * N * D);
   double* ys = (double*)malloc(sizeof(double) * (size_t)C * N);
-  double* grid = (double*)malloc(sizeof(double) * (size_t)C * gsz);
+  double* grids = (double*)calloc((size_t)T * C * gsz, sizeof(double));
   double* qtab = (double*)malloc(sizeof(double) * (size_t)D * R);
-  int64_t cnt[8];
-  if (!xs || !us || !ys || !grid || !qtab) {
+  double* n = (double*)malloc(sizeof(double) * (size_t)S * P);
+  double* y = (double*)malloc(sizeof(double) * (size_t)C * P);
+  if (!xs || !us || !ys || !grids || !qtab || !n || !y) {
     free(xs);
     free(us);
     free(ys);
-    free(grid);
+    free(grids);
     free(qtab);
+    free(n);
+    free(y);
     return -1;
   }
   /* terminal step */
   for (int64_t p = 0; p < N; ++p) {
     const int t = T - 1;
-    const double up = profit[(size_t)t * N + p];
-    double best = 0.0;
-    int bc = 0;
-    for (int c = 0; c < C; ++c) {
-      int valid = 1;
-      for (int sp = 0; sp < S && c > 0; ++sp)
-        if (rv_adj(s, tp, aars, sp, t, c, p) < s->params[sp * 8 + 3]) valid = 0;
-      const double pay = valid ? fmin(s->flow[c] * up, 0.0) : 0.0;
-      if (c == 0) {
-        best = pay;
-      } else if (valid && pay < best) {
-        best = pay;
-        bc = c;
-      }
-    }
+    double x[8], adj[64], best;
+    for (int sp = 0; sp < S; ++sp)
+      for (int c = 0; c < C; ++c) adj[sp * C + c] = rv_adj(s, tp, aars, sp, t, c, p);
+    x[S] = profit[(size_t)t * N + p];
+    optc[(size_t)t * N + p] = rv_decide(s, NULL, x, adj, &best);
     V[(size_t)t * N + p] = best;
-    optc[(size_t)t * N + p] = bc;
   }
   for (int t = T - 2; t >= 1; --t) {
-    double lo[8], hi[8];
-    for (int c = 0; c < C; ++c) cnt[c] = 0;
+    double* grid = grids + (size_t)t * C * gsz;
+    if (rv_fit_step(s, t, controls, tp, aars, profit, V, xs, us, ys, qtab, grid) != 0) {
+      free(xs);
+      free(us);
+      free(ys);
+      free(grids);
+      free(qtab);
+      free(n);
+      free(y);
+      return -2;
+    }
     for (int64_t p = 0; p < N; ++p) {
-      double x[8];
-      int valid = 1;
+      if (recompute) {
+        int bc;
+        double xt[8];
+        V[(size_t)t * N + p] = rv_recompute_path(s, t, p, controls, profit, grids, n, y, &bc, xt);
+        optc[(size_t)t * N + p] = bc;
+        continue;
+      }
+      double x[8], adj[64], best;
       for (int sp = 0; sp < S; ++sp) {
         x[sp] = rv_adj(s, tp, aars, sp, t, C - 1, p);
-        if (x[sp] < s->params[sp * 8 + 3]) valid = 0;
+        for (int c = 0; c < C; ++c) adj[sp * C + c] = rv_adj(s, tp, aars, sp, t, c, p);
       }
       x[S] = profit[(size_t)t * N + p];
-      const int c = controls[(size_t)t * N + p];
-      if (!(valid || c == 0)) continue;
-      for (int d = 0; d < D; ++d) xs[((size_t)c * N + cnt[c]) * D + d] = x[d];
-      ys[(size_t)c * N + cnt[c]] = V[(size_t)(t + 1) * N + p];
-      ++cnt[c];
-    }
-    for (int d = 0; d < D; ++d) {
-      lo[d] = INFINITY;
-      hi[d] = -INFINITY;
-      for (int c = 0; c < C; ++c) {
-        if (c > 0 && cnt[c] <= D + 1) continue;
-        for (int64_t j = 0; j < cnt[c]; ++j) {
-          lo[d] = fmin(lo[d], xs[((size_t)c * N + j) * D + d]);
-          hi[d] = fmax(hi[d], xs[((size_t)c * N + j) * D + d]);
-        }
-      }
-      if (!(lo[d] <= hi[d])) lo[d] = hi[d] = 0.0;
-    }
-#define RV_SCALE(dd, v)                                                                    \
-  (!((hi[dd] - lo[dd]) > 0.0) ? 0.0                                                        \
-   : (dd) == D - 1             ? ((v) - lo[dd]) / (s->scaling * (hi[dd] - lo[dd]))         \
-                               : ((v) - lo[dd]) / (hi[dd] - lo[dd]))
-    for (int c = 0; c < C; ++c)
-      for (int64_t j = 0; j < cnt[c]; ++j)
-        for (int d = 0; d < D; ++d)
-          us[((size_t)c * N + j) * D + d] = RV_SCALE(d, xs[((size_t)c * N + j) * D + d]);
-    for (int d = 0; d < D; ++d)
-      for (int i = 0; i < R; ++i) {
-        const double q = (double)i * (hi[d] - lo[d]) / (double)(R - 1) + lo[d];
-        qtab[d * R + i] = RV_SCALE(d, q);
-        for (int c = 0; c < C; ++c) grid[(size_t)c * gsz + (size_t)d * R + i] = q;
-      }
-#undef RV_SCALE
-    for (int c = 0; c < C; ++c) {
-      const int64_t nc = cnt[c];
-      int k = nc > 3 ? (int)(10.0 * (log((double)nc) + 1.0)) : 0;
-      if (k > 128) k = 128;
-      if (k > nc) k = (int)nc;
-      double* val = grid + (size_t)c * gsz + (size_t)D * R;
-      if (nc < 3 || k < 1) {
-        for (size_t g = 0; g < G; ++g) val[g] = 0.0;
-      } else if (orc_knn_fit_scaled((int32_t)nc, D, us + (size_t)c * N * D, ys + (size_t)c * N, R,
-                                    k, qtab, val) != 0) {
-        free(xs);
-        free(us);
-        free(ys);
-        free(grid);
-        free(qtab);
-        return -2;
-      }
-    }
-    for (int64_t p = 0; p < N; ++p) {
-      double x[8];
-      for (int sp = 0; sp < S; ++sp) x[sp] = rv_adj(s, tp, aars, sp, t, C - 1, p);
-      x[S] = profit[(size_t)t * N + p];
-      double best = 0.0;
-      int bc = 0;
-      for (int c = 0; c < C; ++c) {
-        int valid = 1;
-        for (int sp = 0; sp < S && c > 0; ++sp)
-          if (rv_adj(s, tp, aars, sp, t, c, p) < s->params[sp * 8 + 3]) valid = 0;
-        double pay = 0.0;
-        if (valid) {
-          double cont;
-          orc_knn_eval(D, R, grid + (size_t)c * gsz, 1, x, &cont);
-          pay = s->flow[c] * x[S] + fmin(cont * disc, 0.0);
-        }
-        if (c == 0) {
-          best = pay;
-        } else if (valid && pay < s->hysteresis * best) {
-          best = pay;
-          bc = c;
-        }
-      }
+      optc[(size_t)t * N + p] = rv_decide(s, grid, x, adj, &best);
       V[(size_t)t * N + p] = best;
-      optc[(size_t)t * N + p] = bc;
     }
   }
+  if (grids_out) memcpy(grids_out, grids, sizeof(double) * (size_t)T * C * gsz);
   /* time 0 */
   {
     double sums[8], ns[8], up = 0.0, best = 0.0;
@@ -367,7 +489,22 @@ int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const dou
   free(xs);
   free(us);
   free(ys);
-  free(grid);
+  free(grids);
   free(qtab);
+  free(n);
+  free(y);
   return 0;
+}
+
+int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const double* tp,
+                      const double* aars, const double* profit, double* V, int32_t* optc,
+                      double out[3]) {
+  return rv_backward(s, 4, controls, tp, aars, profit, V, optc, NULL, out);
+}
+
+int orc_rovs_backward_method(const orc_rovs_spec* s, int32_t method, const int32_t* controls,
+                             const double* tp, const double* aars, const double* profit,
+                             double* V, int32_t* optc, double* grids, double out[3]) {
+  if (method != 4 && method != 6) return -3;
+  return rv_backward(s, method, controls, tp, aars, profit, V, optc, grids, out);
 }

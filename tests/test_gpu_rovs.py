@@ -87,3 +87,98 @@ def test_invalid_arguments(ctx):
     pr = rs.synthetic_problem(n_patches=16, n_paths=100, n_years=2)
     with pytest.raises(nat.RoadmcError):
         rs.simulate_rov_species(ctx, pr)
+
+
+# ---- ALGO6: the recomputation ------------------------------------------------
+#
+# The GPU recomputes each path in FP32 (the forward kernel's products and
+# growth, states as n . r_c); the oracle in FP64.  Realized values are sums of
+# Q_c x unit profit over the decisions taken, so they agree to rounding
+# wherever the decisions agree; decisions can differ only where two payoffs or
+# a state and a threshold are within FP32 rounding of each other.
+
+@pytest.mark.parametrize("P,S,C,N,T,R,ratio,ck", [
+    (24, 1, 3, 2000, 12, 8, 0.98, 0),
+    (40, 2, 2, 1500, 10, 6, 0.9, 3),
+    (64, 1, 4, 3001, 15, 10, 0.93, 4),
+])
+def test_algo6_matches_oracle(ctx, oracle, P, S, C, N, T, R, ratio, ck):
+    pr = rs.synthetic_problem(seed=P + S, n_patches=P, n_species=S, n_controls=C, n_paths=N,
+                              n_years=T, dim_res=R, threshold_ratio=ratio)
+    pr.method, pr.checkpoint_years = 6, ck
+    g = _run(ctx, pr)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    np.testing.assert_array_equal(g["controls"], ctl)
+    np.testing.assert_allclose(g["total_pops"], tp, rtol=1e-4)
+    V, oc, out = oracle.rovs_backward(pr, g["controls"], g["total_pops"], g["aars"],
+                                      g["unit_profit"], method=6)
+    assert len(np.unique(oc[1:T - 1])) > 1
+    scale = np.abs(V).max()
+    close = np.abs(g["cond_exp"] - V) <= 1e-9 * scale
+    print(f"algo6: decisions equal {np.mean(g['opt_control'] == oc):.6f}, "
+          f"values equal {close.mean():.6f}", flush=True)
+    assert (g["opt_control"] == oc).mean() > 0.999
+    assert close.mean() > 0.999
+    assert g["value"] == pytest.approx(out["value"], rel=1e-4)
+    assert g["first_control"] == out["control"]
+
+
+def test_algo6_realized_recursion_without_mortality(ctx):
+    from tests.test_oracle_rovs import _stochastic_problem
+    T = 7
+    pr = _stochastic_problem(T=T, N=1000)
+    pr.method = 6
+    g = _run(ctx, pr)
+    V, oc, up = g["cond_exp"], g["opt_control"], g["unit_profit"]
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    Q = np.asarray(pr.flow)
+    assert len(np.unique(oc[1:T - 1])) > 1
+    for t in range(1, T - 1):
+        np.testing.assert_allclose(V[t], Q[oc[t]] * up[t] + d * V[t + 1], rtol=1e-10, atol=1e-12)
+
+
+def test_algo6_closed_forms(ctx):
+    T, u = 6, -0.5
+    pr = _flat_problem(T=T, u=u, N=777)
+    pr.method = 6
+    g = _run(ctx, pr)
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    Q = pr.flow[-1]
+    assert g["value"] == pytest.approx(Q * u * sum(d ** i for i in range(T)), rel=1e-9)
+    for t in range(1, T):
+        np.testing.assert_allclose(g["cond_exp"][t], Q * u * sum(d ** i for i in range(T - t)),
+                                   rtol=1e-9)
+    pr = _flat_problem(thr=1e12)
+    pr.method = 6
+    g = _run(ctx, pr)
+    assert g["value"] == 0.0 and (g["opt_control"] == 0).all()
+
+
+def test_algo6_checkpoint_interval_does_not_change_results(ctx):
+    # Replaying from a checkpoint reproduces the forward populations, so the
+    # result is the same for every checkpoint interval.
+    pr = rs.synthetic_problem(seed=5, n_patches=48, n_species=1, n_controls=3, n_paths=1200,
+                              n_years=11, dim_res=6, threshold_ratio=0.98)
+    pr.method = 6
+    ref = _run(ctx, pr)
+    for ck in (2, 5, 11):
+        pr.checkpoint_years = ck
+        g = _run(ctx, pr)
+        np.testing.assert_array_equal(g["cond_exp"], ref["cond_exp"])
+        np.testing.assert_array_equal(g["opt_control"], ref["opt_control"])
+
+
+def test_algo6_limits(ctx):
+    from transportoptimiser_amd import native as nat
+    # Three species of 512 patches need 3 x 66 KiB of tiles: more than the LDS.
+    pr = rs.synthetic_problem(seed=3, n_patches=512, n_species=3, n_paths=64, n_years=4,
+                              dim_res=3)
+    pr.method = 6
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED
+    pr = rs.synthetic_problem(n_patches=16, n_paths=100, n_years=5)
+    pr.method = 5
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_INVALID

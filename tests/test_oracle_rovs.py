@@ -131,3 +131,133 @@ def test_backward_no_profit_means_zero_flow(oracle):
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     V, oc, out = oracle.rovs_backward(pr, ctl, tp, aa, up)
     assert np.all(V == 0.0) and (oc == 0).all()
+
+
+# ---- ALGO6: the recomputation ------------------------------------------------
+
+def test_algo6_closed_form_with_constant_profit(oracle):
+    # As for ALGO4: the highest flow wins every year, and the realized value
+    # is Q u (1 + d + ... + d^(T-1-t)).
+    T, u = 6, -0.5
+    pr = _flat_problem(T=T, u=u)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    V, oc, out = oracle.rovs_backward(pr, ctl, tp, aa, up, method=6)
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    Q = pr.flow[-1]
+    for t in range(1, T):
+        np.testing.assert_allclose(V[t], Q * u * sum(d ** i for i in range(T - t)), rtol=1e-9)
+        assert (oc[t] == len(pr.flow) - 1).all()
+    assert out["value"] == pytest.approx(Q * u * sum(d ** i for i in range(T)), rel=1e-9)
+
+
+def test_algo6_unreachable_thresholds_keep_zero_flow(oracle):
+    pr = _flat_problem(thr=1e12)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    V, oc, out = oracle.rovs_backward(pr, ctl, tp, aa, up, method=6)
+    assert (oc == 0).all() and np.all(V == 0.0)
+
+
+def _stochastic_problem(T=7, N=500, C=3, surv=None, thr=0.0, P=5):
+    pr = _flat_problem(P=P, C=C, N=N, T=T, thr=thr, surv=surv)
+    pr.fuels = [rs.Price(1.5, 1.5, 0.25, 0.5)]
+    pr.commodities = [rs.Price(1.0, 1.0, 0.2, 0.5)]
+    pr.ore_sd = np.array([0.1])
+    return pr
+
+
+def test_algo6_realized_recursion_without_mortality(oracle):
+    # With no mortality every control moves the populations alike, so the
+    # recomputed states are the forward ones and the policy's later decisions
+    # are the ones taken at their own steps: the realized value obeys
+    # V_t = Q_{c_t} UP_t + d V_{t+1}, the terminal payoff at T - 1.
+    T = 7
+    pr = _stochastic_problem(T=T)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    V, oc, _ = oracle.rovs_backward(pr, ctl, tp, aa, up, method=6)
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    Q = np.asarray(pr.flow)
+    assert len(np.unique(oc[1:T - 1])) > 1  # the decisions vary
+    for t in range(1, T - 1):
+        np.testing.assert_allclose(V[t], Q[oc[t]] * up[t] + d * V[t + 1], rtol=1e-10, atol=1e-12)
+    # ALGO4's values on the same paths are regressed, not realized.
+    V4, _, _ = oracle.rovs_backward(pr, ctl, tp, aa, up, method=4)
+    assert np.max(np.abs(V4[1] - V[1])) > 1e-3
+
+
+def _py_recompute(oracle, pr, ctl, up, grids, t, p):
+    """The recomputation of path p from step t restated in Python from the
+    definition: replay to t under the drawn controls, then each year the
+    decision on the recomputed state with that year's grid (terminal rule in
+    the last year), the chosen control's cash flow, its migration and
+    growth."""
+    S, C, T, P = len(pr.species), len(pr.flow), pr.n_years, pr.n_patches
+    D, R = S + 1, pr.dim_res
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    M = [[np.asarray(sp.trans) * np.asarray(sp.surv)[c] for c in range(C)] for sp in pr.species]
+    n = [np.array(sp.init_pop, dtype=float) for sp in pr.species]
+    grm = [float(sp.params[0]) for sp in pr.species]
+
+    def grow(si, u, y):
+        prm, K = pr.species[si].params, np.asarray(pr.species[si].capacity)
+        zb, zj, uj = oracle.path_scalars(pr.seed, oracle.tag(pr.road_id, si, 1), p, u)
+        jump = 1.0 if uj < prm[6] else 0.0
+        jsz = zj * prm[5] ** 2 - prm[5] ** 2 / 2.0
+        g0 = grm[si]
+        grm[si] = g0 + prm[4] * (prm[1] - g0) * pr.time_step + g0 * (zb * prm[2]) + \
+            (np.exp(jsz) - 1.0) * g0 * jump
+        z = np.array([oracle.patch_normal(pr.seed, oracle.tag(pr.road_id, si, 0), p, u, j)
+                      for j in range(P)])
+        g = grm[si] + z * prm[7]
+        return y * (1.0 + g * (K - y) / K / 100.0)
+
+    for si in range(S):
+        for u in range(t):
+            n[si] = grow(si, u, n[si] @ M[si][ctl[u, p]])
+    value, f, c_t = 0.0, 1.0, None
+    for u in range(t, T):
+        adj = [[float((n[si] @ M[si][c]).sum()) for c in range(C)] for si in range(S)]
+        x = [adj[si][C - 1] for si in range(S)] + [up[u, p]]
+        last = u == T - 1
+        best, bc = 0.0, 0
+        for c in range(C):
+            valid = c == 0 or all(not adj[si][c] < pr.species[si].params[3] for si in range(S))
+            pay = 0.0
+            if valid:
+                if last:
+                    pay = min(pr.flow[c] * x[S], 0.0)
+                else:
+                    cont = oracle.knn_eval(grids[u, c], D, R, [x])[0]
+                    pay = pr.flow[c] * x[S] + min(cont * d, 0.0)
+            if c == 0:
+                best = pay
+            elif valid and pay < (1.0 if last else pr.hysteresis) * best:
+                best, bc = pay, c
+        value += f * (best if last else pr.flow[bc] * x[S])
+        f *= d
+        if u == t:
+            c_t = bc
+        if last:
+            break
+        for si in range(S):
+            n[si] = grow(si, u, n[si] @ M[si][bc])
+    return value, c_t
+
+
+@pytest.mark.parametrize("S", [1, 2])
+def test_algo6_matches_python_restatement(oracle, S):
+    # Mortality that differs by control, thresholds that bind on some paths,
+    # stochastic prices, growth noise and local variability: every path's
+    # realized value and decision at every step against the restatement.
+    pr = rs.synthetic_problem(seed=11, n_patches=6, n_species=S, n_controls=3, n_paths=90,
+                              n_years=6, dim_res=4, threshold_ratio=0.98)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    adj = tp[:, :-1, None, :] * aa
+    thr = np.array([sp.params[3] for sp in pr.species])[:, None, None, None]
+    assert 0.05 < (adj < thr).any(axis=0)[:, 1:].mean() < 0.5  # thresholds bind
+    V, oc, _, grids = oracle.rovs_backward(pr, ctl, tp, aa, up, method=6, grids=True)
+    assert np.bincount(oc[1:pr.n_years - 1].ravel(), minlength=3).min() > 20
+    for t in range(1, pr.n_years - 1):
+        for p in range(pr.n_paths):
+            v, c = _py_recompute(oracle, pr, ctl, up, grids, t, p)
+            assert c == oc[t, p], (t, p)
+            assert V[t, p] == pytest.approx(v, rel=1e-9, abs=1e-9), (t, p)

@@ -109,7 +109,8 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->rovs_bufs.optc, &ctx->rovs_bufs.adj, &ctx->rovs_bufs.flag,
                         &ctx->rovs_bufs.blk,  &ctx->rovs_bufs.xs,  &ctx->rovs_bufs.ys,
                         &ctx->rovs_bufs.u,    &ctx->rovs_bufs.small, &ctx->rovs_bufs.grid,
-                        &ctx->rovs_bufs.args};
+                        &ctx->rovs_bufs.args, &ctx->rovs_bufs.ckpt, &ctx->rovs_bufs.grm,
+                        &ctx->rovs_bufs.rvec, &ctx->rovs_bufs.grids};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
   if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
   rmc_buf_free(ctx->rov_graph.param);

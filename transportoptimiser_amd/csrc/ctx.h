@@ -49,7 +49,7 @@ struct roadmc_ctx {
   } knn_bufs;
   struct {
     rmc_devbuf mpack, cols, prices, controls, tp, aars, profit, V, optc, adj, flag, blk, xs, ys,
-        u, small, grid, args;
+        u, small, grid, args, ckpt, grm, rvec, grids;
   } rovs_bufs;
 };
 

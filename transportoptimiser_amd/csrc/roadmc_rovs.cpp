@@ -46,6 +46,9 @@ int validate(roadmc_ctx* ctx, const roadmc_rovs_problem* p, const roadmc_rovs_re
     G *= p->dim_res;
     if (G > (1LL << 24)) return bad("dim_res^(n_species+1) must be <= 2^24");
   }
+  if (p->method != 0 && p->method != ROADMC_ROVS_ALGO4 && p->method != ROADMC_ROVS_ALGO6)
+    return bad("method must be ROADMC_ROVS_ALGO4 or ROADMC_ROVS_ALGO6");
+  if (p->checkpoint_years < 0) return bad("checkpoint_years must be >= 0");
   if (!(p->time_step > 0) || !(p->profit_scaling > 0) || !std::isfinite(p->rrr) ||
       !std::isfinite(p->hysteresis) || !std::isfinite(p->unit_cost) ||
       !std::isfinite(p->unit_revenue))
@@ -76,6 +79,20 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   const size_t gsz = (size_t)D * R + G;
   const int nblk = (N + roadmc::kRovsScanBlock - 1) / roadmc::kRovsScanBlock;
   const uint32_t k0 = (uint32_t)(pr->seed & 0xffffffffu), k1 = (uint32_t)(pr->seed >> 32);
+  const bool algo6 = pr->method == ROADMC_ROVS_ALGO6;
+  // The recomputation keeps every species' tile of 32 paths in LDS, starts
+  // from a checkpoint of the forward populations every ck years (every year
+  // while the checkpoints fit in 16 GiB) and reads every step's grids.
+  constexpr size_t kLdsMax = 160 * 1024, kCkptBudget = (size_t)16 << 30;
+  if (algo6 && roadmc::rovs_recompute_lds(Pp, C, S) > kLdsMax)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "ALGO6: the species' population tiles exceed the LDS of a workgroup");
+  const size_t ck_all = (size_t)S * T * N * Pp * sizeof(float);
+  const int ck = !algo6                     ? 1
+                 : pr->checkpoint_years > 0 ? std::min(pr->checkpoint_years, T)
+                                            : (int)std::max<size_t>(1, (ck_all + kCkptBudget - 1) /
+                                                                           kCkptBudget);
+  const int nck = (T + ck - 1) / ck;
 
   // Device workspace.
   auto& B = ctx->rovs_bufs;
@@ -102,6 +119,10 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
       {&B.small, (size_t)(64 * D * 2 + D * 2 + D * R + 8) * sizeof(double)},
       {&B.grid, (size_t)C * gsz * sizeof(double)},
       {&B.args, sizeof(roadmc::RovsStepArgs)},
+      {&B.ckpt, algo6 ? (size_t)S * nck * N * Pp * sizeof(float) : 8},
+      {&B.grm, algo6 ? (size_t)S * T * N * sizeof(float) : 8},
+      {&B.rvec, algo6 ? (size_t)S * C * Pp * sizeof(float) : 8},
+      {&B.grids, algo6 ? (size_t)T * C * gsz * sizeof(double) : 8},
   };
   for (auto& n : need)
     if ((rc = rmc_buf_ensure(ctx, *n.b, n.bytes))) return rc;
@@ -109,16 +130,23 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
 
   // Packed movement matrices trans o surv_c (FP32, MFMA B order) and columns.
   std::vector<float> mp((size_t)S * C * Pp * Pk, 0.f), cols((size_t)S * 3 * Pp);
+  std::vector<float> rvec(algo6 ? (size_t)S * C * Pp : 0, 0.f);
   std::vector<float> init_total(S);
   for (int sp = 0; sp < S; ++sp) {
     const roadmc_rovs_species& sc = pr->species[sp];
     for (int c = 0; c < C; ++c) {
       float* m = mp.data() + ((size_t)sp * C + c) * Pp * Pk;
       const double* sv = sc.surv + (size_t)c * P * P;
-      for (int k = 0; k < P; ++k)
-        for (int j = 0; j < P; ++j)
-          m[roadmc::mte_pack_index(false, Pk, k, j)] =
-              (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+      for (int k = 0; k < P; ++k) {
+        double rs = 0.0;
+        for (int j = 0; j < P; ++j) {
+          const float v = (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+          m[roadmc::mte_pack_index(false, Pk, k, j)] = v;
+          rs += v;
+        }
+        // Row sum of origin k, at k's column of the population tile.
+        if (algo6) rvec[((size_t)sp * C + c) * Pp + roadmc::rovs_tile_col(k)] = (float)rs;
+      }
     }
     float* cl = cols.data() + (size_t)sp * 3 * Pp;
     double tot = 0.0;
@@ -133,6 +161,9 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   }
   RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  if (algo6)
+    RMC_HIP(ctx, hipMemcpyAsync(B.rvec.ptr, rvec.data(), rvec.size() * sizeof(float),
+                                hipMemcpyHostToDevice, s));
 
   // Prices and drawn controls (the ROV forward pass, independent prices).
   roadmc::RovForwardArgs fa{};
@@ -181,6 +212,12 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     a.controls = static_cast<const int*>(B.controls.ptr);
     a.total_pops = static_cast<float*>(B.tp.ptr) + (size_t)sp * (T + 1) * N;
     a.aars = static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N;
+    a.ck = 1;
+    if (algo6) {
+      a.ckpt = static_cast<float*>(B.ckpt.ptr) + (size_t)sp * nck * N * Pp;
+      a.ck = ck;
+      a.grm_out = static_cast<float*>(B.grm.ptr) + (size_t)sp * T * N;
+    }
     RMC_HIP(ctx, roadmc::launch_rovs_forward(a, s));
   }
 
@@ -246,6 +283,41 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
 
   std::vector<double> grids_host;
   if (res->grids) grids_host.assign((size_t)T * C * gsz, 0.0);
+  roadmc::RovsRecomputeArgs ra{};
+  if (algo6) {
+    ra.N = N;
+    ra.T = T;
+    ra.P = P;
+    ra.Pp = Pp;
+    ra.Pk = Pk;
+    ra.C = C;
+    ra.S = S;
+    ra.D = D;
+    ra.R = R;
+    ra.ck = ck;
+    ra.mpack = static_cast<const float*>(B.mpack.ptr);
+    ra.cols = static_cast<const float*>(B.cols.ptr);
+    ra.rvec = static_cast<const float*>(B.rvec.ptr);
+    ra.ckpt = static_cast<const float*>(B.ckpt.ptr);
+    ra.ckpt_species = (size_t)nck * N * Pp;
+    ra.grm = static_cast<const float*>(B.grm.ptr);
+    ra.controls = static_cast<const int*>(B.controls.ptr);
+    ra.profit = static_cast<const double*>(B.profit.ptr);
+    ra.grids = static_cast<const double*>(B.grids.ptr);
+    for (int sp = 0; sp < S; ++sp) {
+      ra.lv[sp] = (float)pr->species[sp].params[7];
+      ra.thr[sp] = pr->species[sp].params[3];
+      ra.tag0[sp] = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATCH);
+    }
+    for (int c = 0; c < C; ++c) ra.flow[c] = pr->flow[c];
+    ra.disc = h.disc;
+    ra.hysteresis = h.hysteresis;
+    ra.key0 = k0;
+    ra.key1 = k1;
+    ra.V = h.V;
+    ra.optc = h.optc;
+    ra.adj_pops = h.adj_pops;
+  }
   RMC_HIP(ctx, roadmc::launch_rovs_terminal(d_args, N, s));
   std::vector<int> counts(C);
   for (int t = T - 2; t >= 1; --t) {
@@ -278,7 +350,16 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
       ka.val = vals;
       RMC_HIP(ctx, roadmc::launch_knn_fit(ka, s));
     }
-    RMC_HIP(ctx, roadmc::launch_rovs_decide(d_args, N, D, t, s));
+    if (algo6) {
+      double* gt = static_cast<double*>(B.grids.ptr) + (size_t)t * C * gsz;
+      RMC_HIP(ctx, hipMemcpyAsync(gt, h.grid, (size_t)C * gsz * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+      ra.t = t;
+      ra.y0 = t / ck * ck;
+      RMC_HIP(ctx, roadmc::launch_rovs_recompute(ra, s));
+    } else {
+      RMC_HIP(ctx, roadmc::launch_rovs_decide(d_args, N, D, t, s));
+    }
     if (res->grids)
       RMC_HIP(ctx, hipMemcpyAsync(grids_host.data() + (size_t)t * C * gsz, h.grid,
                                   (size_t)C * gsz * sizeof(double), hipMemcpyDeviceToHost, s));

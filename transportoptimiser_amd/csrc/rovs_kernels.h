@@ -31,6 +31,12 @@ struct RovsForwardArgs {
   const int* controls;  // [T][N]
   float* total_pops;    // [T+1][N]
   float* aars;          // [T][C][N]
+  // For the recomputation (nullptr otherwise): the population tile at the
+  // start of every ck-th year, [T / ck rounded up][N][Pp] in the tile's
+  // column order, and the growth-rate mean of each year's growth, [T][N].
+  float* ckpt;
+  int ck;
+  float* grm_out;
 };
 
 size_t rovs_forward_lds(int Pp, int C);
@@ -84,6 +90,38 @@ hipError_t launch_rovs_collect(const RovsStepArgs* a, int N, int t, hipStream_t 
 hipError_t launch_rovs_ranges(const RovsStepArgs* a, int N, int D, hipStream_t s);
 hipError_t launch_rovs_normalise(const RovsStepArgs* a, int N, int D, int R, hipStream_t s);
 hipError_t launch_rovs_decide(const RovsStepArgs* a, int N, int D, int t, hipStream_t s);
+
+// One backward step of the recomputation (ALGO6): every path re-simulated
+// from the checkpoint year y0 <= t under its drawn controls up to t, then
+// from t to T - 1 under the fitted policy (the decision of each year on the
+// recomputed state with that year's grids, the terminal rule at T - 1);
+// writes V[t] (the realized value), optc[t] and adj_pops[t].
+struct RovsRecomputeArgs {
+  int N, T, P, Pp, Pk, C, S, D, R;
+  int t, y0, ck;
+  const float* mpack;    // [S][C][Pp * Pk]
+  const float* cols;     // [S][3][Pp]
+  const float* rvec;     // [S][C][Pp] row sums of trans o surv_c, tile column order
+  const float* ckpt;     // [S][nck][N][Pp]
+  size_t ckpt_species;   // floats per species in ckpt
+  const float* grm;      // [S][T][N]
+  const int* controls;   // [T][N]
+  const double* profit;  // [T][N]
+  const double* grids;   // [T][C][D R + R^D]
+  float lv[kRovsMaxS];
+  double thr[kRovsMaxS];
+  double flow[kRovsMaxC];
+  double disc, hysteresis;
+  uint32_t tag0[kRovsMaxS];
+  uint32_t key0, key1;
+  double* V;
+  int* optc;
+  double* adj_pops;      // [T][S][N] or nullptr
+};
+size_t rovs_recompute_lds(int Pp, int C, int S);
+// Position of patch k within a row of the population tile.
+int rovs_tile_col(int k);
+hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s);
 // out (device): {value, SD, control}.
 hipError_t launch_rovs_first(const RovsStepArgs* a, double* out, hipStream_t s);
 

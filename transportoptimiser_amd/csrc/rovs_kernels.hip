@@ -127,6 +127,15 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
     asm volatile("" : "+s"(LDAy), "+s"(p0y), "+v"(laney));
     if (tid < kRovsRows)
       ctl[tid] = (p0 + tid < a.N) ? a.controls[(size_t)year * a.N + p0 + tid] : 0;
+    // Checkpoint of the tile at the start of the year (read-only here; the
+    // epilogue's writes come after the barriers of the control loop).
+    if (a.ckpt && year % a.ck == 0) {
+      float* dst = a.ckpt + (size_t)(year / a.ck) * a.N * Pp;
+      for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
+        const int r = idx / Pp, c = idx - r * Pp;
+        if (p0y + r < a.N) dst[(size_t)(p0y + r) * Pp + c] = A[r * LDAy + c];
+      }
+    }
     __syncthreads();
     floatx16 sel[kRovsNT], acc[kRovsNT];
 #pragma unroll
@@ -171,6 +180,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
       const float jump = (uj < jprob) ? 1.f : 0.f;
       const float jsz = zj * jmp * jmp - jmp * jmp / 2.f;
       grm[tid] = g + rev * (mu - g) * dt + g * (zb * sd) + (__expf(jsz) - 1.f) * g * jump;
+      if (a.grm_out && p0 + tid < a.N) a.grm_out[(size_t)year * a.N + p0 + tid] = grm[tid];
     }
     __syncthreads();
     // Growth (SimulateGPU.cu:902-916): g = grMean + z lv per patch,
@@ -562,6 +572,242 @@ hipError_t launch_rovs_decide(const RovsStepArgs* a, int N, int D, int t, hipStr
     case 8: hipLaunchKernelGGL(rovs_decide_kernel<8>, g, b, 0, s, a, t); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// ---- The recomputation (ALGO6) ---------------------------------------------
+
+int rovs_tile_col(int k) { return TF::perm(k); }
+
+size_t rovs_recompute_lds(int Pp, int C, int S) {
+  const int LDA = Pp + TF::PAD;
+  const int nq = rovs_waves(Pp) * 64 / kRovsRows;
+  return (size_t)S * kRovsRows * LDA * sizeof(float)  // the species' population tiles
+         + (size_t)nq * C * kRovsRows * sizeof(float) // partial state sums
+         + (size_t)S * C * kRovsRows * sizeof(float)  // states per species and control
+         + kRovsRows * sizeof(float)                  // growth-rate means of the year
+         + kRovsRows * sizeof(int);                   // controls of the year
+}
+
+// One path's decision at year u on the recomputed states st [S][C][32]
+// (backwardInduction's rule, the terminal rule at T - 1) and its cash flow:
+// the chosen control's Q_c x unit profit, or the terminal payoff.
+template <int D>
+__device__ double rovs_policy_step(const RovsRecomputeArgs& a, const float* st, int row, int p,
+                                   int u, int* bc_out) {
+  constexpr int S = D - 1;
+  const int C = a.C, N = a.N, R = a.R;
+  double x[D];
+#pragma unroll
+  for (int sp = 0; sp < S; ++sp) x[sp] = (double)st[(sp * C + C - 1) * kRovsRows + row];
+  x[S] = a.profit[(size_t)u * N + p];
+  size_t G = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) G *= (size_t)R;
+  const size_t gsz = (size_t)D * R + G;
+  const bool last = u == a.T - 1;
+  const double hyst = last ? 1.0 : a.hysteresis;
+  double best = 0.0;
+  int bc = 0;
+  for (int c = 0; c < C; ++c) {
+    bool valid = true;
+#pragma unroll
+    for (int sp = 0; sp < S; ++sp)
+      valid = valid && (c == 0 || !((double)st[(sp * C + c) * kRovsRows + row] < a.thr[sp]));
+    double pay = 0.0;
+    if (valid) {
+      if (last) {
+        pay = fmin(a.flow[c] * x[S], 0.0);
+      } else {
+        const double* g = a.grids + ((size_t)u * C + c) * gsz;
+        pay = a.flow[c] * x[S] + fmin(rovs_interp<D>(g, R, x) * a.disc, 0.0);
+      }
+    }
+    if (c == 0) {
+      best = pay;
+    } else if (valid && pay < hyst * best) {
+      best = pay;
+      bc = c;
+    }
+  }
+  if (u == a.t) {
+    a.optc[(size_t)u * N + p] = bc;
+    if (a.adj_pops)
+#pragma unroll
+      for (int sp = 0; sp < S; ++sp) a.adj_pops[((size_t)u * S + sp) * N + p] = x[sp];
+  }
+  *bc_out = bc;
+  return last ? best : a.flow[bc] * x[S];
+}
+
+// Every path of a 32-path tile from the checkpoint year y0: the drawn
+// controls up to t, then the fitted policy from t to T - 1 (optimalForwardPaths,
+// SimulateGPU.cu:1199-2206, with the value of the path the realized, discounted
+// cash flows).  The migration and growth are the forward kernel's: one MFMA
+// product per control on the species' tile, each path's own control kept by a
+// register select, the epilogue with the stored growth-rate mean of the year
+// and the same Philox noise, so the replay reproduces the forward pass.  The
+// states of the decision, every control's total after one migration, are
+// n . r_c with r_c the row sums of trans o surv_c (fixed-order sums).
+__global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRecomputeArgs a) {
+  extern __shared__ char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nthreads = blockDim.x, NW = nthreads >> 6, nq = nthreads / kRovsRows;
+  const int Pp = a.Pp, LDA = Pp + TF::PAD, C = a.C, S = a.S, N = a.N;
+  const int p0 = blockIdx.x * kRovsRows;
+  float* A0 = reinterpret_cast<float*>(smem);
+  float* part = A0 + (size_t)S * kRovsRows * LDA;  // [nq][C][32]
+  float* st = part + nq * C * kRovsRows;           // [S][C][32]
+  float* gm = st + S * C * kRovsRows;              // [32]
+  int* ctl = reinterpret_cast<int*>(gm + kRovsRows);
+
+  for (int sp = 0; sp < S; ++sp) {
+    const float* src = a.ckpt + sp * a.ckpt_species + (size_t)(a.y0 / a.ck) * N * Pp;
+    float* A = A0 + (size_t)sp * kRovsRows * LDA;
+    for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
+      const int r = idx / Pp, c = idx - r * Pp;
+      A[r * LDA + c] = (p0 + r < N) ? src[(size_t)(p0 + r) * Pp + c] : 0.f;
+    }
+  }
+  const int ntc = Pp / TF::TN;
+  const int base = ntc / NW, extra = ntc % NW;
+  const int my_n = base + (wave < extra ? 1 : 0);
+  const int my_t0 = wave * base + (wave < extra ? wave : extra);
+  int tcol[kRovsNT];
+#pragma unroll
+  for (int t = 0; t < kRovsNT; ++t) tcol[t] = my_t0 + t;
+  const int arow = TF::a_row(lane), akoff = TF::KV * TF::k_lane(lane);
+  const int nkg = a.Pk >> 3;
+  const size_t mbytes = (size_t)Pp * a.Pk * sizeof(float);
+  double vacc = 0.0, fac = 1.0;  // the realized value, threads tid < 32
+  __syncthreads();
+
+  for (int u = a.y0; u < a.T; ++u) {
+    if (u >= a.t) {
+      // States: partial sums over a stride of columns, then a fixed-order
+      // sum over the strides.
+      const int r = tid & (kRovsRows - 1), q = tid / kRovsRows;
+      for (int sp = 0; sp < S; ++sp) {
+        const float* A = A0 + (size_t)sp * kRovsRows * LDA;
+        const float* rv = a.rvec + (size_t)sp * C * Pp;
+        float accs[kRovsMaxC];
+#pragma unroll
+        for (int c = 0; c < kRovsMaxC; ++c) accs[c] = 0.f;
+        for (int i = q; i < Pp; i += nq) {
+          const float v = A[r * LDA + i];
+#pragma unroll
+          for (int c = 0; c < kRovsMaxC; ++c)
+            if (c < C) accs[c] += v * rv[(size_t)c * Pp + i];
+        }
+#pragma unroll
+        for (int c = 0; c < kRovsMaxC; ++c)
+          if (c < C) part[(q * C + c) * kRovsRows + r] = accs[c];
+        __syncthreads();
+        if (tid < C * kRovsRows) {
+          const int c = tid / kRovsRows, rr = tid - c * kRovsRows;
+          float sum = 0.f;
+          for (int qq = 0; qq < nq; ++qq) sum += part[(qq * C + c) * kRovsRows + rr];
+          st[(sp * C + c) * kRovsRows + rr] = sum;
+        }
+        __syncthreads();
+      }
+      if (tid < kRovsRows) {
+        const int p = p0 + tid;
+        int bc = 0;
+        if (p < N) {
+          double cash = 0.0;
+          switch (a.D) {
+            case 2: cash = rovs_policy_step<2>(a, st, tid, p, u, &bc); break;
+            case 3: cash = rovs_policy_step<3>(a, st, tid, p, u, &bc); break;
+            case 4: cash = rovs_policy_step<4>(a, st, tid, p, u, &bc); break;
+            case 5: cash = rovs_policy_step<5>(a, st, tid, p, u, &bc); break;
+            case 6: cash = rovs_policy_step<6>(a, st, tid, p, u, &bc); break;
+            case 7: cash = rovs_policy_step<7>(a, st, tid, p, u, &bc); break;
+            default: cash = rovs_policy_step<8>(a, st, tid, p, u, &bc); break;
+          }
+          vacc += fac * cash;
+          fac *= a.disc;
+        }
+        ctl[tid] = bc;
+      }
+      __syncthreads();
+      if (u == a.T - 1) break;
+    } else {
+      if (tid < kRovsRows) ctl[tid] = (p0 + tid < N) ? a.controls[(size_t)u * N + p0 + tid] : 0;
+      __syncthreads();
+    }
+    // Migration under each path's control and growth, species by species.
+    for (int sp = 0; sp < S; ++sp) {
+      float* A = A0 + (size_t)sp * kRovsRows * LDA;
+      if (tid < kRovsRows)
+        gm[tid] = (p0 + tid < N) ? a.grm[((size_t)sp * a.T + u) * N + p0 + tid] : 0.f;
+      int LDAy = LDA, laney = lane;
+      asm volatile("" : "+s"(LDAy), "+v"(laney));
+      floatx16 sel[kRovsNT], acc[kRovsNT];
+#pragma unroll
+      for (int t = 0; t < kRovsNT; ++t) sel[t] = floatx16{};
+      for (int c = 0; c < C; ++c) {
+        const PackedB B = packed_b(a.mpack + ((size_t)sp * C + c) * Pp * a.Pk, nkg, mbytes);
+        if (my_n == 1)
+          rovs_product<1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+        else if (my_n == 2)
+          rovs_product<2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+#pragma unroll
+        for (int r = 0; r < TF::NACC; ++r) {
+          const int row = TF::acc_row(laney, r);
+          if (ctl[row] == c) {
+#pragma unroll
+            for (int t = 0; t < kRovsNT; ++t)
+              if (t < my_n) sel[t][r] = acc[t][r];
+          }
+        }
+      }
+      // Every wave has read the tile (all columns feed every product) before
+      // the epilogue overwrites it.
+      __syncthreads();
+      int LDAe = LDA, p0e = p0, lanee = lane;
+      asm volatile("" : "+s"(LDAe), "+s"(p0e), "+v"(lanee));
+      const float* K = a.cols + (size_t)sp * 3 * Pp;
+      const float* Kinv = K + Pp;
+      const float lv = a.lv[sp];
+#pragma unroll
+      for (int t = 0; t < kRovsNT; ++t) {
+        if (t >= my_n) break;
+        const int col = tcol[t] * TF::TN + TF::acc_col(lanee);
+        const int pcol = TF::perm(col);
+        const float Kc = K[col], Ki = Kinv[col];
+#pragma unroll
+        for (int r4 = 0; r4 < TF::NACC; r4 += 4) {
+          float z[4];
+          const int row0 = TF::acc_row(lanee, r4);
+          TF::normal4((uint32_t)(p0e + row0) >> 2, (uint32_t)u, (uint32_t)col, a.tag0[sp], a.key0,
+                      a.key1, z);
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            const int row = TF::acc_row(lanee, r4 + s4);
+            const float x = sel[t][r4 + s4];
+            const float g = gm[row] + z[s4] * lv;
+            A[row * LDAe + pcol] = x * (1.f + g * (Kc - x) * Ki * 0.01f);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid < kRovsRows && p0 + tid < N) a.V[(size_t)a.t * N + p0 + tid] = vacc;
+}
+
+hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s) {
+  if (a.D < 2 || a.D > 8 || a.S != a.D - 1 || a.C < 1 || a.C > kRovsMaxC || a.Pp % TF::TN ||
+      a.Pp > kRovsMaxPatches || a.t < 1 || a.t > a.T - 2 || a.y0 > a.t || a.y0 % a.ck)
+    return hipErrorInvalidValue;
+  const size_t lds = rovs_recompute_lds(a.Pp, a.C, a.S);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_recompute_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
+  hipLaunchKernelGGL(rovs_recompute_kernel, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s, a);
   return hipGetLastError();
 }
 

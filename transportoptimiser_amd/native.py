@@ -17,6 +17,9 @@ ERR_OOM = 3
 ERR_NUMERIC = 4
 ERR_UNSUPPORTED = 5
 
+ROVS_ALGO4 = 4
+ROVS_ALGO6 = 6
+
 LAW_LOGNORMAL = 0
 LAW_REF_CPU = 1
 LAW_REF_GPU = 2
@@ -197,6 +200,8 @@ class RovsProblem(C.Structure):
         ("hysteresis", C.c_double),
         ("seed", C.c_uint64),
         ("road_id", C.c_uint32),
+        ("method", C.c_int32),
+        ("checkpoint_years", C.c_int32),
     ]
 
 

@@ -30,8 +30,9 @@ fits an R^D grid surrogate from N samples (scaled coordinates, values <= 0,
 the reference's neighbour count) and evaluates it at 10^6 points.
 
 ``python transportoptimiser_amd/profiling.py rovs [--patches P --species S
---controls C --paths N --years T --res R]`` runs the species-state ROV
-(the reference's ALGO4) on the synthetic landscape R times.
+--controls C --paths N --years T --res R --method 4|6]`` runs the
+species-state ROV (the reference's ALGO4, or ALGO6 with the forward
+recomputation) on the synthetic landscape R times.
 
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
@@ -168,8 +169,8 @@ def knn(n: int, dims: int, res: int, repeat: int) -> None:
 
 
 def rovs(patches: int, species: int, controls: int, paths: int, years: int, res: int,
-         repeat: int) -> None:
-    """Species-state ROV (ALGO4) on the synthetic landscape, thresholds at 0.7
+         repeat: int, method: int = 4) -> None:
+    """Species-state ROV (ALGO4 or ALGO6) on the synthetic landscape, thresholds at 0.7
     of the initial total (every control has in-the-money paths for the first
     half of the horizon at P = 256)."""
     import os
@@ -181,10 +182,11 @@ def rovs(patches: int, species: int, controls: int, paths: int, years: int, res:
     ctx = Context(0)
     pr = rs.synthetic_problem(seed=5, n_patches=patches, n_species=species, n_controls=controls,
                               n_paths=paths, n_years=years, dim_res=res, threshold_ratio=0.7)
+    pr.method = method
     for i in range(repeat):
         t = time.perf_counter()
         r = rs.simulate_rov_species(ctx, pr)
-        print(f"rovs call {i}: P={patches} S={species} C={controls} N={paths} T={years} "
+        print(f"rovs call {i}: ALGO{method} P={patches} S={species} C={controls} N={paths} T={years} "
               f"R={res}: {(time.perf_counter() - t) * 1e3:.2f} ms, value {r['value']:.4f}, "
               f"control {r['first_control']}", flush=True)
     ctx.close()
@@ -288,6 +290,7 @@ def main() -> int:
     rv2.add_argument("--years", type=int, default=50)
     rv2.add_argument("--res", type=int, default=20)
     rv2.add_argument("--repeat", type=int, default=3)
+    rv2.add_argument("--method", type=int, default=4, choices=(4, 6))
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -308,7 +311,7 @@ def main() -> int:
     elif a.cmd == "rov":
         rov(a.repeat)
     elif a.cmd == "rovs":
-        rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat)
+        rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat, a.method)
     elif a.cmd == "knn":
         knn(a.samples, a.dims, a.res, a.repeat)
     elif a.cmd == "road":

@@ -61,6 +61,8 @@ class RovsProblem:
     hysteresis: float = 0.975          # SimulateGPU.cu:2763
     seed: int = 0
     road_id: int = 0
+    method: int = 4                    # 4: ALGO4; 6: ALGO6 (forward recomputation)
+    checkpoint_years: int = 0          # ALGO6: forward populations kept every k years (0: auto)
 
     @property
     def n_patches(self) -> int:
@@ -109,7 +111,8 @@ class _Marshal:
         p.n_years, p.time_step, p.rrr = pr.n_years, pr.time_step, pr.rrr
         p.n_paths, p.dim_res = pr.n_paths, pr.dim_res
         p.profit_scaling, p.hysteresis = pr.profit_scaling, pr.hysteresis
-        p.seed, p.road_id = pr.seed, pr.road_id
+        p.seed, p.road_id, p.method = pr.seed, pr.road_id, pr.method
+        p.checkpoint_years = pr.checkpoint_years
         self.problem = p
 
     def _arr(self, a):
