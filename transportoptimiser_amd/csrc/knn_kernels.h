@@ -27,6 +27,7 @@ constexpr int kKnnThreads = 64;  // one wave per workgroup
 constexpr int kKnnMaxK = 128;
 
 constexpr int kKnnSelectMin = 2048;  // above this many samples: the radix-select kernel
+constexpr int kKnnSelectCap = 2048;  // samples of the k-th binade and below ranked in LDS
 
 // Dynamic LDS of the wave-per-point fit kernel: per-lane lists of cap
 // entries, then the k merged neighbours.

@@ -16,8 +16,9 @@
 // neighbours, in the same order, as a stable sort of all samples; the
 // (d+1) x (d+1) weighted normal equations are summed by one lane in that
 // order and Cholesky-solved in FP64 registers (templated on d so that nothing
-// spills).  For more samples, one workgroup per grid point radix-selects the
-// k-th neighbour exactly (knn_select_kernel).
+// spills).  For more samples, one workgroup per grid point selects the k
+// nearest by radix passes over the squared distances' bits, finishing in LDS
+// once the k-th key's bucket is small (knn_select_kernel).
 // Compiled without multiply-add contraction, so that the neighbour ranking
 // and the sums follow the same roundings as the CPU statement.
 #include <hip/hip_runtime.h>
@@ -173,19 +174,28 @@ __global__ __launch_bounds__(kKnnThreads) void knn_fit_kernel(KnnFitArgs a) {
   a.val[blockIdx.x] = v;
 }
 
-// Many samples per grid point: one 256-thread workgroup per point finds the
-// k-th smallest (squared distance, index) key exactly by an 8-pass radix
-// select over the bits of the non-negative FP64 squared distance
-// (distances recomputed each pass, no per-sample storage), collects the
-// strictly smaller keys and, in index order, the equal ones it needs, ranks
-// the k neighbours into (distance, index) order, and fits as above.
+// Many samples per grid point: one 256-thread workgroup per point.  The keys
+// are the bits of the non-negative FP64 squared distances, recomputed on
+// every pass (no per-sample storage), ordered with the sample index as the
+// tie-break.  Radix passes of 11 bits from the top, over all samples, narrow
+// down the bucket of the k-th key until it holds at most kKnnSelectCap
+// samples; one more pass gathers the keys below that bucket (fewer than k)
+// and the bucket itself into LDS, where the bucket is bitonic-sorted by
+// (key, index) and its first entries complete the k neighbours.  If every
+// bit is resolved and the bucket is still larger (that many samples at one
+// distance), the equal keys are taken in index order by chunked ballots.
+// Either way the k neighbours are ranked into (distance, index) order and
+// fitted as above.
 template <int D>
 __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitArgs a) {
-  __shared__ unsigned int hist[256];
+  constexpr int kBins = 2048;
+  __shared__ unsigned int hist[kBins];
+  __shared__ unsigned int part[256];
+  __shared__ unsigned long long ekey[kKnnSelectCap];
+  __shared__ int eidx[kKnnSelectCap];
   __shared__ double cd[kKnnMaxK], nbd[kKnnMaxK];
   __shared__ int ci[kKnnMaxK], nbi[kKnnMaxK];
-  __shared__ unsigned long long s_prefix;
-  __shared__ int s_kk, s_cnt;
+  __shared__ int s_b, s_cum, s_hb, s_cnt, s_ecnt;
   __shared__ int wcnt[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = a.n, k = a.k, R = a.R;
@@ -208,76 +218,153 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitArgs a) {
     }
     return s;
   };
-  // Radix select: after the passes, prefix is the k-th smallest squared
-  // distance v and kk the number of samples at exactly v to take.
-  unsigned long long prefix = 0ull;
-  int kk = k;
-  for (int pass = 0; pass < 8; ++pass) {
-    const int shift = 56 - 8 * pass;
-    const unsigned long long hmask = pass == 0 ? 0ull : (~0ull << (64 - 8 * pass));
-    hist[tid] = 0u;
+  auto keyof = [&](int j) -> unsigned long long {
+    return (unsigned long long)__double_as_longlong(dist2(j));
+  };
+  // 1. Radix passes: prefix holds the resolved high bits of the k-th key,
+  //    kk its rank among the keys that share them, hb their number.
+  unsigned long long prefix = 0ull, hmask = 0ull;
+  int kk = k, hb = n;
+  for (int pass = 0; pass < 6; ++pass) {
+    const int shift = pass < 5 ? 53 - 11 * pass : 0;
+    const unsigned int dmask = pass < 5 ? kBins - 1 : 511u;
+    for (int i = tid; i < kBins; i += 256) hist[i] = 0u;
     __syncthreads();
     for (int j = tid; j < n; j += 256) {
-      const unsigned long long key = (unsigned long long)__double_as_longlong(dist2(j));
-      if ((key & hmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      const unsigned long long key = keyof(j);
+      if ((key & hmask) == prefix) atomicAdd(&hist[(unsigned)(key >> shift) & dmask], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
-      int cum = 0, b = 0;
-      for (; b < 255; ++b) {
-        if (cum + (int)hist[b] >= kk) break;
-        cum += (int)hist[b];
+    {
+      unsigned int sum = 0u;
+#pragma unroll
+      for (int i = 0; i < kBins / 256; ++i) sum += hist[tid * (kBins / 256) + i];
+      part[tid] = sum;
+    }
+    __syncthreads();
+    if (w == 0) {
+      // First bin whose cumulative count reaches kk: an inclusive scan over
+      // the lanes' four partial sums, then within the lane's 32 bins.
+      const int s4 = (int)(part[4 * lane] + part[4 * lane + 1] + part[4 * lane + 2] +
+                           part[4 * lane + 3]);
+      int inc = s4;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(inc, off);
+        if (lane >= off) inc += o;
       }
-      s_prefix = prefix | ((unsigned long long)b << shift);
-      s_kk = kk - cum;
+      const unsigned long long reach = __ballot(inc >= kk);  // hb >= kk keys counted
+      if (lane == __ffsll((long long)reach) - 1) {
+        int cum = inc - s4, c = 4 * lane;
+        while (cum + (int)part[c] < kk) cum += (int)part[c++];
+        int b = c * (kBins / 256);
+        while (cum + (int)hist[b] < kk) cum += (int)hist[b++];
+        s_b = b;
+        s_cum = cum;
+        s_hb = (int)hist[b];
+      }
     }
     __syncthreads();
-    prefix = s_prefix;
-    kk = s_kk;
+    prefix |= (unsigned long long)s_b << shift;
+    hmask |= (unsigned long long)dmask << shift;
+    kk -= s_cum;
+    hb = s_hb;
     __syncthreads();
+    if (hb <= kKnnSelectCap) break;
   }
-  // Strictly smaller keys (any order; ranked below).
-  if (tid == 0) s_cnt = 0;
+  // 2. Keys below the prefix (exactly k - kk of them) and the bucket.
+  if (tid == 0) {
+    s_cnt = 0;
+    s_ecnt = 0;
+  }
   __syncthreads();
+  const bool in_lds = hb <= kKnnSelectCap;
   for (int j = tid; j < n; j += 256) {
-    const double d2 = dist2(j);
-    if ((unsigned long long)__double_as_longlong(d2) < prefix) {
+    const unsigned long long key = keyof(j);
+    if (key < prefix) {
       const int pos = atomicAdd(&s_cnt, 1);
-      cd[pos] = d2;
-      ci[pos] = j;
+      if (pos < kKnnMaxK) {
+        cd[pos] = __longlong_as_double((long long)key);
+        ci[pos] = j;
+      }
+    } else if (in_lds && (key & hmask) == prefix) {
+      const int pos = atomicAdd(&s_ecnt, 1);
+      if (pos < kKnnSelectCap) {
+        ekey[pos] = key;
+        eidx[pos] = j;
+      }
     }
   }
   __syncthreads();
-  const int m = s_cnt;  // == k - kk
-  // Keys equal to v, lowest indices first: chunks of 256 consecutive samples.
-  const double vd = __longlong_as_double((long long)prefix);
-  int taken = 0;
-  for (int base = 0; base < n && taken < kk; base += 256) {
-    const int j = base + tid;
-    const bool eq = j < n && dist2(j) == vd;
-    const unsigned long long msk = __ballot(eq);
-    if (lane == 0) wcnt[w] = __popcll(msk);
-    __syncthreads();
-    int off = 0;
-    for (int ww = 0; ww < w; ++ww) off += wcnt[ww];
-    const int rank = taken + off +
-                     __builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
-                                               __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0));
-    if (eq && rank < kk) {
-      cd[m + rank] = vd;
-      ci[m + rank] = j;
+  const int nl = k - kk;  // == s_cnt
+  if (in_lds) {
+    int S2 = 1;
+    while (S2 < hb) S2 <<= 1;
+    for (int i = hb + tid; i < S2; i += 256) {
+      ekey[i] = ~0ull;
+      eidx[i] = 0x7fffffff;
     }
-    taken += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
     __syncthreads();
-  }
-  // Rank into (distance, index) order.
-  if (tid < k) {
-    const double d = cd[tid];
-    const int i = ci[tid];
-    int r = 0;
-    for (int j = 0; j < k; ++j) r += (cd[j] < d || (cd[j] == d && ci[j] < i)) ? 1 : 0;
-    nbd[r] = d;
-    nbi[r] = i;
+    for (int size = 2; size <= S2; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = tid; i < S2 / 2; i += 256) {
+          const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+          const unsigned long long kl = ekey[lo], kh = ekey[hi];
+          const int il = eidx[lo], ih = eidx[hi];
+          const bool gt = kl > kh || (kl == kh && il > ih);
+          if (gt == ((lo & size) == 0)) {
+            ekey[lo] = kh;
+            ekey[hi] = kl;
+            eidx[lo] = ih;
+            eidx[hi] = il;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (tid < nl) {
+      const double d = cd[tid];
+      const int i = ci[tid];
+      int r = 0;
+      for (int j = 0; j < nl; ++j) r += (cd[j] < d || (cd[j] == d && ci[j] < i)) ? 1 : 0;
+      nbd[r] = d;
+      nbi[r] = i;
+    }
+    if (tid < kk) {
+      nbd[nl + tid] = __longlong_as_double((long long)ekey[tid]);
+      nbi[nl + tid] = eidx[tid];
+    }
+  } else {
+    // Every bit resolved: hb samples at exactly the k-th key; the kk lowest
+    // indices, in chunks of 256 consecutive samples.
+    const double vd = __longlong_as_double((long long)prefix);
+    int taken = 0;
+    for (int base = 0; base < n && taken < kk; base += 256) {
+      const int j = base + tid;
+      const bool eq = j < n && dist2(j) == vd;
+      const unsigned long long msk = __ballot(eq);
+      if (lane == 0) wcnt[w] = __popcll(msk);
+      __syncthreads();
+      int off = 0;
+      for (int ww = 0; ww < w; ++ww) off += wcnt[ww];
+      const int rank = taken + off +
+                       __builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0));
+      if (eq && rank < kk) {
+        cd[nl + rank] = vd;
+        ci[nl + rank] = j;
+      }
+      taken += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      __syncthreads();
+    }
+    if (tid < k) {
+      const double d = cd[tid];
+      const int i = ci[tid];
+      int r = 0;
+      for (int j = 0; j < k; ++j) r += (cd[j] < d || (cd[j] == d && ci[j] < i)) ? 1 : 0;
+      nbd[r] = d;
+      nbi[r] = i;
+    }
   }
   __syncthreads();
   if (tid != 0) return;
