@@ -27,7 +27,8 @@
  *     buildSurrogateROVCUDA (SimulateGPU.cu:5533-5917) and its multilinear
  *     evaluation (Utility.cpp:253-363);
  *   - the species-state ROV of simulateROVCUDA's ALGO4 (SimulateGPU.cu:
- *     789-1148, 2210-2784, 4401-4998);
+ *     789-1148, 2210-2784, 4401-4998) and ALGO6's forward recomputation
+ *     (optimalForwardPaths, :1199-2206, the loop :5011-5444);
  *   - the adjoint and MLP references used by the gradient tests.
  *
  * Random numbers follow the Philox4x32-10 stream contract documented in

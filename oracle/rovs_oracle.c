@@ -16,7 +16,10 @@
  *     kNN search (:4582-4607) with k = 10 (ln n + 1) (:4522-4525), the
  *     local-linear grid fit (multiLocLinReg, as knn_oracle.c), and
  *     backwardInduction's decision (:2333-2784) with multilinear
- *     interpolation of the grids; firstPeriodInduction (:2210-2330).
+ *     interpolation of the grids; firstPeriodInduction (:2210-2330);
+ *   - ALGO6 (:5011-5444, optimalForwardPaths :1199-2206): after each step's
+ *     fit, every path re-simulated from that step under the fitted policy,
+ *     its value the realized discounted cash flows.
  * The library's decisions (documented with its API) are followed where the
  * reference is inconsistent.
  */

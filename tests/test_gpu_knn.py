@@ -58,14 +58,17 @@ def test_fit_with_tied_distances_matches_oracle(ctx, oracle, n):
                                atol=1e-12)
 
 
-def test_fit_with_crowded_binade_matches_oracle(ctx, oracle):
-    # Two clusters of 2500 coincident samples: the k-th neighbour's binade
-    # holds more samples than the select kernel ranks in LDS, so it takes the
-    # radix-select path; the neighbours are the lowest indices of the nearer
+@pytest.mark.parametrize("offset", [0.0, 1e-3])
+def test_fit_with_crowded_distances_matches_oracle(ctx, oracle, offset):
+    # Coincident samples: two clusters of 2500 (offset 0), so that more
+    # samples share the k-th distance than the select kernel sorts in LDS and
+    # the equal keys are taken in index order over all samples; or four of
+    # 1250 (every other sample moved by the offset), so that 1250 equal keys
+    # are sorted in LDS.  The neighbours are the lowest indices of the nearest
     # cluster.
     rng = np.random.default_rng(8)
     x = np.repeat(np.array([[0.0, 0.0], [1.0, 1.0]]), 2500, axis=0)
-    x += np.array([0.0, 1e-3]) * (np.arange(5000) % 2)[:, None]
+    x += np.array([0.0, offset]) * (np.arange(5000) % 2)[:, None]
     y = rng.normal(size=5000)
     g = knn.fit(ctx, x, y, 5, 12)
     np.testing.assert_allclose(g.surrogate, oracle.knn_fit(x, y, 5, 12, 0), rtol=1e-9,
