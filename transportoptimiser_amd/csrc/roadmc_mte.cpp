@@ -7,7 +7,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ctx.h"
@@ -105,8 +108,8 @@ void pack_species(const roadmc_species& sp, int Pp, int Kp, double tol, bool fp6
   const int P = sp.n_patches;
   std::fill(mp, mp + (size_t)Pp * Kp, (T)0);
   if (sp.trans)  // otherwise packed on the device from the species' road
-    for (int k = 0; k < P; ++k)
-      for (int j = 0; j < P; ++j)
+    for (int j = 0; j < P; ++j)    // destination: one contiguous column of trans and surv
+      for (int k = 0; k < P; ++k)
         mp[roadmc::mte_pack_index(fp64, Kp, k, j)] = (T)movement(sp, P, k, j, tol);
   for (int j = 0; j < Pp; ++j) {
     const bool in = j < P;
@@ -247,11 +250,40 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
       (rc = grow(b.off, js.jobs.size() * sizeof(int64_t))) ||
       (rc = grow(b.counter, 256)))
     return rc;
-  std::vector<T> mp(js.mpack_elems), cols(js.cols_elems);
-  for (size_t j = 0; j < js.jobs.size(); ++j) {
-    const roadmc_mte_problem& pr = roads[js.road_of[j]];
-    pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, js.jobs[j].Pk, pr.drop_tol,
-                    js.fp64, mp.data() + js.mpack_off[j], cols.data() + js.cols_off[j]);
+  // Host packing, the jobs split over threads by matrix size; every job
+  // writes its own contiguous regions, which tile both arrays.
+  std::unique_ptr<T[]> mp(new (std::nothrow) T[std::max<size_t>(js.mpack_elems, 1)]);
+  std::unique_ptr<T[]> cols(new (std::nothrow) T[std::max<size_t>(js.cols_elems, 1)]);
+  if (!mp || !cols) return rmc_fail(ctx, ROADMC_ERR_OOM, "host staging allocation failed");
+  const size_t nj = js.jobs.size();
+  auto pack_range = [&](size_t j0, size_t j1) {
+    for (size_t j = j0; j < j1; ++j) {
+      const roadmc_mte_problem& pr = roads[js.road_of[j]];
+      pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, js.jobs[j].Pk, pr.drop_tol,
+                      js.fp64, mp.get() + js.mpack_off[j], cols.get() + js.cols_off[j]);
+    }
+  };
+  const int nth = (int)std::min<size_t>(
+      nj, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  std::vector<size_t> bounds(nth + 1, nj);
+  bounds[0] = 0;
+  for (size_t j = 0, acc = 0, t = 1; j < nj && t < (size_t)nth; ++j) {
+    acc += (size_t)js.jobs[j].Pp * js.jobs[j].Pk + 4096;
+    if (acc * nth >= (js.mpack_elems + 4096 * nj) * t) bounds[t++] = j + 1;
+  }
+  {
+    std::vector<std::thread> pool;
+    try {
+      for (int t = 1; t < nth; ++t) pool.emplace_back(pack_range, bounds[t], bounds[t + 1]);
+      pack_range(bounds[0], bounds[1]);
+    } catch (...) {
+      for (auto& th : pool) th.join();
+      pool.clear();
+      pack_range(0, nj);  // no threads to be had: pack serially
+    }
+    for (auto& th : pool) th.join();
+  }
+  for (size_t j = 0; j < nj; ++j) {
     MteJob& jb = js.jobs[j];
     jb.mpack = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];
     jb.cols = static_cast<T*>(b.cols.ptr) + js.cols_off[j];
@@ -261,9 +293,9 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
                       : nullptr;
   }
   hipStream_t s = ctx->stream;
-  RMC_HIP(ctx, hipMemcpyAsync(b.mpack.ptr, mp.data(), mp.size() * es,
+  RMC_HIP(ctx, hipMemcpyAsync(b.mpack.ptr, mp.get(), js.mpack_elems * es,
                               hipMemcpyHostToDevice, s));
-  RMC_HIP(ctx, hipMemcpyAsync(b.cols.ptr, cols.data(), cols.size() * es,
+  RMC_HIP(ctx, hipMemcpyAsync(b.cols.ptr, cols.get(), js.cols_elems * es,
                               hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(b.jobs.ptr, js.jobs.data(), js.jobs.size() * sizeof(MteJob),
                               hipMemcpyHostToDevice, s));
