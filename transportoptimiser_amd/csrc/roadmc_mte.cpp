@@ -964,7 +964,10 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   jb.split = 0;
   jb.N = (int32_t)pr->n_paths;
   jb.T = pr->n_years;
-  jb.BM = rows_for(ctx, adj->fp64, jb.Pp, pr->n_paths);
+  // FP32: 32-row tiles.  The segment kernels hold one tile per workgroup, and
+  // finer tiles even out the last round of workgroups: 38.9 ms against
+  // 41.7 ms with 128-row tiles at 256 patches, 100 years, 50,000 paths.
+  jb.BM = adj->fp64 ? rows_for(ctx, true, jb.Pp, pr->n_paths) : roadmc::kMteRowsF32;
   jb.law = ROADMC_LAW_LOGNORMAL;
   jb.endpoint = ROADMC_END_POST_GROWTH;
   jb.n_tiles = (int)((pr->n_paths + jb.BM - 1) / jb.BM);
