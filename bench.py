@@ -239,9 +239,9 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
-              "multi-GPU path (roads shard across GPUs with no communication; see "
-              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single MI355X and has no "
+              "multi-GPU path (roads would shard across GPUs with no communication, one "
+              "process per GPU).  Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from transportoptimiser_amd import landscape
