@@ -83,8 +83,9 @@ def read_traffic(kernel_ms: float):
 
 
 def run_extras(ctx, roads, prep_ms) -> dict:
-    """The other configurations of the hot path, each timed once after a
-    warm-up: the large single road (3 x 256 patches, 100 years, 100k paths),
+    """The other configurations of the hot path, each timed after a warm-up:
+    the small single road (32 patches, 10k paths) on the GPU and the CPU
+    oracle, the large single road (3 x 256 patches, 100 years, 100k paths),
     the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis),
     the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10),
     the road -> matrix builder (4096 patches), the generation prepared from
@@ -94,6 +95,24 @@ def run_extras(ctx, roads, prep_ms) -> dict:
     from transportoptimiser_amd.rov import config5, simulate_rov
 
     out = {}
+    # Config 1, the single small road (1 species, 32 patches, 50 years, 10,000
+    # paths): tiled and plain GPU calls (the CPU baseline times the same road).
+    p1 = landscape.config1()
+    ctx.simulate_mte(p1, end_pops=False, patch_mean=False)
+    t = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        ctx.simulate_mte(p1, end_pops=False, patch_mean=False)
+    tiled = (time.perf_counter() - t) / reps
+    ctx.simulate_mte(p1, plain=True, end_pops=False, patch_mean=False)
+    t = time.perf_counter()
+    ctx.simulate_mte(p1, plain=True, end_pops=False, patch_mean=False)
+    plain1 = time.perf_counter() - t
+    out["config1_road"] = {
+        "tiled_ms": round(tiled * 1e3, 3), "plain_ms": round(plain1 * 1e3, 2),
+        "what": "1 species x 32 patches, 50 years, 10,000 paths; roadmc_mte_run host call "
+                "(tiled) and roadmc_mte_run_plain host call"}
+    log(f"config1 road: {out['config1_road']}")
     pr = landscape.config2()
     fl = road_flops(pr)
     ctx.simulate_mte(pr, end_pops=False, patch_mean=False)
@@ -356,6 +375,15 @@ def main() -> int:
                          f"generation (P={ps}), {n_sample} of {args.paths} paths each, both "
                          f"species; per-road time scaled by paths and extrapolated over the "
                          f"512 roads as a + b P^2"}
+        # The CPU oracle on the whole config-1 road (1 species, 32 patches,
+        # 50 years, 10,000 paths), for the single-road comparison.
+        p1 = landscape.config1()
+        sd1 = p1.species[0]
+        t = time.perf_counter()
+        pyoracle.mte_species(sd1, 0, p1.road_id, p1.n_years, p1.time_step, p1.growth_law,
+                             p1.endpoint, p1.drop_tol, p1.seed, 0, p1.n_paths,
+                             n_threads=threads)
+        cpu["config1_road_ms"] = round((time.perf_counter() - t) * 1e3, 1)
         log(f"cpu baseline: {cpu}")
 
     extras = None if args.no_extras else run_extras(ctx, roads, prep_ms)
