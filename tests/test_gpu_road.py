@@ -229,3 +229,44 @@ def test_device_road_species_rejected_where_unsupported(ctx):
     with pytest.raises(nat.RoadmcError) as e:
         ctx.simulate_mte(RoadProblem(species=[d], n_years=5, n_paths=100), plain=True)
     assert e.value.code == nat.ERR_UNSUPPORTED
+
+
+def test_batch_road_species_edge_cases(ctx):
+    """No flow controls (transitions only, no mortality), a road with no
+    vertices, and a one-patch species, each against the host matrices of the
+    same builder; a road whose patch count differs from the species' is an
+    error."""
+    from transportoptimiser_amd.landscape import RoadProblem
+    rng = np.random.default_rng(23)
+    cases = []
+    a = rd.random_road_species(rng, 33, n_controls=0)
+    b = rd.random_road_species(rng, 20)
+    b.road = np.zeros((0, 2))
+    b.road_type = None
+    c = rd.random_road_species(rng, 1)
+    for inp, control in ((a, -1), (b, 1), (c, 0)):
+        srp = rd.SpeciesRoadPatches(ctx, inp)
+        P = inp.n_patches
+        cap = rng.uniform(500, 5000, P)
+        args = (0.8 * cap, cap, np.full(P, 0.05))
+        d = srp.device_species(control, *args, noise_sd=0.1)
+        m = rd.build_species_road(ctx, inp)
+        surv = None if len(inp.flows) == 0 else m.surv[control]
+        from transportoptimiser_amd.landscape import SpeciesData
+        h = SpeciesData(trans=m.trans, surv=surv, init_pop=args[0], capacity=args[1],
+                        growth=args[2], noise_sd=0.1)
+        cases.append((d, h))
+    kw = dict(n_years=8, n_paths=700, seed=3)
+    print("batch road species edge cases", flush=True)
+    out_d = ctx.simulate_mte_batch([RoadProblem(species=[d], road_id=i, **kw)
+                                    for i, (d, _) in enumerate(cases)], end_pops=True)
+    out_h = ctx.simulate_mte_batch([RoadProblem(species=[h], road_id=i, **kw)
+                                    for i, (_, h) in enumerate(cases)], end_pops=True)
+    for x, y in zip(out_d, out_h):
+        np.testing.assert_array_equal(x.end_pops, y.end_pops)
+        np.testing.assert_array_equal(x.mte_prob, y.mte_prob)
+    bad = cases[0][0]
+    bad.init_pop, bad.capacity, bad.growth = bad.init_pop[:-1], bad.capacity[:-1], bad.growth[:-1]
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte_batch([RoadProblem(species=[bad], **kw)])
+    assert e.value.code == nat.ERR_INVALID
