@@ -503,6 +503,72 @@ int roadmc_read_commodity(const char* path, roadmc_commodity_file* out);
 int roadmc_read_vehicle(const char* path, roadmc_vehicle_file* out);
 int roadmc_read_species(const char* path, roadmc_species_file* out);
 
+/* The Monte Carlo subset of the main input file (INPUT_FILE, parsed by
+ * position in RoadGA::initialiseFromTextInput, RoadGA.cpp:3464-3890): the
+ * economic horizon, path count and grid resolutions, the method codes, the
+ * variable-parameter multiplier lists of the scenario grid, the traffic
+ * programs and the commodity, vehicle and species file lists.  Absent fields
+ * read as NaN, -1 or empty lists. */
+#define ROADMC_MAX_LIST 16     /* values of one multiplier list */
+#define ROADMC_MAX_PROGRAMS 8  /* traffic programs */
+#define ROADMC_MAX_FILES 16    /* commodity, vehicle or species files */
+#define ROADMC_PATH_LEN 256
+
+typedef struct {
+  int32_t n;
+  double v[ROADMC_MAX_LIST];
+} roadmc_list;
+
+typedef struct {
+  double confidence_interval;  /* "Confidence Interval": quantile of the MTE penalty */
+  int32_t use_gpu;             /* "Use GPU": 1 TRUE, 0 FALSE */
+  int32_t surrogate_dim_res;   /* "Surrogate Dimension Resolution" */
+  int32_t interpolation;       /* "Interpolation Routine": 0 cubic spline, 1 multiple
+                                  local linear regression (the kNN grid) */
+  double traffic_growth;       /* "Traffic Growth Rate (p.a.)" */
+  double rrr;                  /* "Required Rate of Return (per cen per annum)", per cent */
+  int32_t n_years;             /* "Number of Years" */
+  double time_step;            /* "Step Size (Years)" */
+  int32_t n_paths;             /* "Number of Simulation Paths" */
+  int32_t rov_dim_res;         /* "ROV Dimension Resolution" */
+  int32_t optimisation_method; /* "Optimisation Method" as written.  The file's legend
+                                  counts from 1; the reference casts the value to
+                                  Optimiser::Type, which counts from 0 (RoadGA.cpp:
+                                  3797-3799, Optimiser.h:67-72) */
+  int32_t rov_method;          /* "ROV Method" as written, with the same mismatch: 4
+                                  ("State & Control" in the legend) selects ALGO5, an
+                                  empty case, in the reference (RoadGA.cpp:3803-3805,
+                                  Optimiser.h:74-82, SimulateGPU.cu:5005-5009) */
+  /* "VARIABLE PARAMETERS" (the scenario grid, VariableParameters) */
+  roadmc_list animal_bridges, hab_pref_sd, movement_sd, ranging_sd, growth_mean,
+      growth_sd, price_mean, price_sd, ore_sd, pop_level, comparison_cost;
+  /* "Program k : flows : switching costs [: bridge]" ("Programs" gives the
+     count where present).  The bridge flag is read from the fourth field; the
+     reference compares the flow field instead (RoadGA.cpp:3918), so its flag
+     is always false */
+  int32_t n_programs;
+  int32_t n_flows[ROADMC_MAX_PROGRAMS];
+  double flows[ROADMC_MAX_PROGRAMS][ROADMC_MAX_CONTROLS];
+  int32_t n_switching[ROADMC_MAX_PROGRAMS];
+  double switching[ROADMC_MAX_PROGRAMS][1 << ROADMC_MAX_CONTROLS];
+  int32_t bridge[ROADMC_MAX_PROGRAMS];
+  /* "Number of Fuels" and "Number of Transported Commodities" (-1 when absent;
+     older files give only "Number of Commodities", read into n_transported);
+     the "Commodity k" files list the fuels first.  "Vehicle k", "Species k":
+     paths as written, relative to the program's root directory */
+  int32_t n_fuels, n_transported;
+  int32_t n_commodities, n_vehicles, n_species;
+  char commodity[ROADMC_MAX_FILES][ROADMC_PATH_LEN];
+  char vehicle[ROADMC_MAX_FILES][ROADMC_PATH_LEN];
+  char species[ROADMC_MAX_FILES][ROADMC_PATH_LEN];
+  /* "EXISTING ROAD DATA" (the CONTROLLED method's comparison road) */
+  int32_t road_exists;         /* 1 TRUE, 0 FALSE, -1 absent */
+  double travel_time, road_length, accident_cost;
+  roadmc_list fuel_per_journey;
+} roadmc_run_settings;
+
+int roadmc_read_run_settings(const char* path, roadmc_run_settings* out);
+
 /* ---- Learned surrogate --------------------------------------------------- */
 
 /* An ensemble of n_members MLPs n_in -> 64 -> 64 -> n_out (tanh), trained in

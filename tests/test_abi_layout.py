@@ -30,6 +30,8 @@ PAIRS = [
     ("roadmc_commodity_file", nat.CommodityFile),
     ("roadmc_vehicle_file", nat.VehicleFile),
     ("roadmc_species_file", nat.SpeciesFile),
+    ("roadmc_list", nat.RoadmcList),
+    ("roadmc_run_settings", nat.RunSettings),
 ]
 
 

@@ -141,3 +141,89 @@ def test_valuation_inputs_from_reference_files(oracle):
     _, _, _, up = oracle.rovs_forward(pr)
     fuel = sum(fc * (petrol.current, diesel.current)[i] for fc, i in zip(fuel_cost, fuel_idx))
     np.testing.assert_allclose(up[0], 3.0 + fuel - 0.2 * ore.ore_mean * ore.current, rtol=1e-12)
+
+
+def test_run_settings_of_the_reference_test_input_file():
+    # The older layout of tests/Input_Data/INPUT_FILE: "Number of Commodities",
+    # a legend counting methods from 1, programs without a count or bridge flag.
+    r = fm.read_run_settings(GOLD / "INPUT_FILE")
+    assert r.confidence_interval == 0.95 and r.use_gpu == 1
+    assert r.surrogate_dim_res == 30 and r.interpolation == 1
+    assert r.traffic_growth == 0.0 and r.rrr == 7.0
+    assert r.n_years == 50 and r.time_step == 1.0
+    assert r.n_paths == 50000 and r.rov_dim_res == 40
+    assert r.optimisation_method == 3 and r.rov_method == 4
+    # The reference casts the written 4 to ROVType ALGO5 (an empty case).
+    assert r.rov_type() == "ALGO5"
+    assert r.lists["growth_mean"] == [0.5, 1.0, 1.5]
+    assert r.lists["growth_sd"] == [1.0, 2.0, 3.0]
+    assert r.lists["price_mean"] == [1.0]
+    assert r.lists["price_sd"] == [1.0, 2.0, 3.0]
+    assert r.lists["ore_sd"] == [1.0, 2.0, 3.0]
+    assert r.lists["pop_level"] == [0.8, 0.9, 1.0]
+    assert r.lists["animal_bridges"] == [0.0] and r.lists["comparison_cost"] == []
+    assert r.programs == [([0.0, 75.0, 150.0], [0.0] * 8, False)]
+    assert r.n_fuels == -1 and r.n_transported == 3
+    assert r.commodities == ["Input_Data/Commodities/ore", "Input_Data/Commodities/petrol",
+                             "Input_Data/Commodities/diesel"]
+    assert r.vehicles == ["Input_Data/Vehicles/small", "Input_Data/Vehicles/medium",
+                          "Input_Data/Vehicles/large"]
+    assert r.species == ["Input_data/Species/species1"]
+    assert r.road_exists == -1 and np.isnan(r.travel_time)
+    args = r.simulation_args()
+    assert (args["n_years"], args["time_step"], args["rrr"], args["n_paths"],
+            args["dim_res"]) == (50, 1.0, 7.0, 50000, 40)
+    np.testing.assert_array_equal(args["flow"], [0.0, 75.0, 150.0])
+
+
+def test_run_settings_current_layout(tmp_path):
+    # The layout RoadGA::initialiseFromTextInput reads (RoadGA.cpp:3896-4500):
+    # a program count, a bridge flag, fuels and transported commodities
+    # counted apart, and the existing-road section.
+    text = """###
+## ECONOMIC PARAMETERS
+Required Rate of Return (per cen per annum)     : 5.5
+Number of Years                                 : 30
+Step Size (Years)                               : 0.5
+Number of Simulation Paths                      : 100000
+ROV Dimension Resolution                        : 40
+Optimisation Method                             : 3
+ROV Method                                      : 5
+Population Growth Rate Mean Multipliers         : 1
+Minimum Population Levels (proportion of min)   : 0.9
+Comparison Road Variable Cost Multipliers       : 0.5,1
+Programs                                        : 2
+Program 1   : 0,200,400 : 0,0,0,0,0,0,0,0 : FALSE
+Program 2   : 0,100 : 1,2,3,4 : TRUE
+Number of Fuels                                 : 2
+Number of Transported Commodities               : 0
+Commodity 1                                     : Experiments/Input_Data/Commodities/diesel
+Commodity 2                                     : Experiments/Input_Data/Commodities/petrol
+Number of Species                               : 1
+Species 1                                       : Experiments/Input_Data/Species/Species_3/Species_3
+Road Exists                                     : TRUE
+Travel Time (hours)                             : 0.5
+Length (m)                                      : 15000
+Fuel Consumption Per Journey                    : 2,3.3,6.7
+Accident Cost per Unit Traffic                  : 1000
+"""
+    p = tmp_path / "INPUT_FILE"
+    p.write_text(text)
+    r = fm.read_run_settings(p)
+    assert (r.rrr, r.n_years, r.time_step, r.n_paths) == (5.5, 30, 0.5, 100000)
+    assert r.rov_method == 5 and r.rov_type() == "ALGO6"
+    assert r.programs == [([0.0, 200.0, 400.0], [0.0] * 8, False),
+                          ([0.0, 100.0], [1.0, 2.0, 3.0, 4.0], True)]
+    assert r.lists["comparison_cost"] == [0.5, 1.0]
+    assert (r.n_fuels, r.n_transported) == (2, 0)
+    assert r.commodities[1] == "Experiments/Input_Data/Commodities/petrol"
+    assert r.road_exists == 1 and r.road_length == 15000.0 and r.accident_cost == 1000.0
+    assert r.fuel_per_journey == [2.0, 3.3, 6.7]
+    assert np.isnan(r.confidence_interval) and r.use_gpu == -1
+    # A program count that disagrees with the program lines is an error.
+    p.write_text(text.replace("Programs                                        : 2",
+                              "Programs                                        : 3"))
+    with pytest.raises(nat.RoadmcError):
+        fm.read_run_settings(p)
+    with pytest.raises(nat.RoadmcError):
+        fm.read_run_settings(tmp_path / "missing")

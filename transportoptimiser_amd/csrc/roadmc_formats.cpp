@@ -239,3 +239,129 @@ extern "C" int roadmc_read_species(const char* path, roadmc_species_file* o) {
                   num(kv, "Jump Poisson Parameter", &o->growth_poisson_jump);
   return ok ? ROADMC_OK : ROADMC_ERR_INVALID;
 }
+
+namespace {
+
+// A comma-separated list of numbers (the multiplier and flow lists).
+int parse_list(const std::string& s, double* out, int cap) {
+  int n = 0;
+  for (const std::string& f : fields(s, ',')) {
+    if (f.empty()) continue;
+    char* end = nullptr;
+    const double v = std::strtod(f.c_str(), &end);
+    if (end == f.c_str()) return -1;
+    if (n >= cap) return -1;
+    out[n++] = v;
+  }
+  return n;
+}
+
+int int_or(const std::map<std::string, std::string>& kv, const char* key, int dflt) {
+  double v;
+  return num(kv, key, &v) ? (int)v : dflt;
+}
+
+double num_or_nan(const std::map<std::string, std::string>& kv, const char* key) {
+  double v;
+  return num(kv, key, &v) ? v : std::numeric_limits<double>::quiet_NaN();
+}
+
+int flag_or(const std::map<std::string, std::string>& kv, const char* key) {
+  auto it = kv.find(key);
+  if (it == kv.end()) return -1;
+  return it->second == "TRUE" ? 1 : 0;
+}
+
+bool list_of(const std::map<std::string, std::string>& kv, const char* key, roadmc_list* l) {
+  l->n = 0;
+  auto it = kv.find(key);
+  if (it == kv.end()) return true;
+  const int n = parse_list(it->second, l->v, ROADMC_MAX_LIST);
+  if (n < 0) return false;
+  l->n = n;
+  return true;
+}
+
+// "<prefix> 1", "<prefix> 2", ... in order, up to the first gap.
+int file_list(const std::map<std::string, std::string>& kv, const char* prefix,
+              char (*out)[ROADMC_PATH_LEN]) {
+  int n = 0;
+  while (n < ROADMC_MAX_FILES) {
+    auto it = kv.find(std::string(prefix) + " " + std::to_string(n + 1));
+    if (it == kv.end()) break;
+    copy_name(out[n], it->second, ROADMC_PATH_LEN);
+    ++n;
+  }
+  return n;
+}
+
+}  // namespace
+
+// The simulation subset of the main input file, read by key where the
+// reference reads by line position (RoadGA::initialiseFromTextInput,
+// RoadGA.cpp:3464-4500), so that both the older layout of tests/Input_Data
+// and the current one of the experiment directories are accepted.
+extern "C" int roadmc_read_run_settings(const char* path, roadmc_run_settings* o) {
+  if (!path || !o) return ROADMC_ERR_INVALID;
+  std::map<std::string, std::string> kv;
+  std::vector<std::vector<std::string>> tables;
+  if (!read_dict(path, kv, &tables)) return ROADMC_ERR_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  o->confidence_interval = num_or_nan(kv, "Confidence Interval");
+  o->use_gpu = flag_or(kv, "Use GPU");
+  o->surrogate_dim_res = int_or(kv, "Surrogate Dimension Resolution", -1);
+  o->interpolation = int_or(kv, "Interpolation Routine", -1);
+  o->traffic_growth = num_or_nan(kv, "Traffic Growth Rate (p.a.)");
+  o->rrr = num_or_nan(kv, "Required Rate of Return (per cen per annum)");
+  o->n_years = int_or(kv, "Number of Years", -1);
+  o->time_step = num_or_nan(kv, "Step Size (Years)");
+  o->n_paths = int_or(kv, "Number of Simulation Paths", -1);
+  o->rov_dim_res = int_or(kv, "ROV Dimension Resolution", -1);
+  o->optimisation_method = int_or(kv, "Optimisation Method", -1);
+  o->rov_method = int_or(kv, "ROV Method", -1);
+  const struct {
+    const char* key;
+    roadmc_list* l;
+  } lists[] = {
+      {"Animal Bridges to Build", &o->animal_bridges},
+      {"Habitat Preference SD Multipliers", &o->hab_pref_sd},
+      {"Movement Propensity SD Multipliers", &o->movement_sd},
+      {"Ranging Coefficient SD Multipliers", &o->ranging_sd},
+      {"Population Growth Rate Mean Multipliers", &o->growth_mean},
+      {"Population Growth Rate SD Multipliers", &o->growth_sd},
+      {"Commodity Price Mean Multipliers", &o->price_mean},
+      {"Commodity Price SD Multipliers", &o->price_sd},
+      {"Ore Composition SD Multipliers", &o->ore_sd},
+      {"Minimum Population Levels (proportion of min)", &o->pop_level},
+      {"Comparison Road Variable Cost Multipliers", &o->comparison_cost},
+      {"Fuel Consumption Per Journey", &o->fuel_per_journey},
+  };
+  for (const auto& l : lists)
+    if (!list_of(kv, l.key, l.l)) return ROADMC_ERR_NUMERIC;
+  // Traffic programs: "Program k : flows : switching [: bridge]" in file order.
+  for (const std::vector<std::string>& t : tables) {
+    if (t[0].rfind("Program ", 0) != 0) continue;
+    if (o->n_programs >= ROADMC_MAX_PROGRAMS) return ROADMC_ERR_INVALID;
+    const int k = o->n_programs;
+    const int nf = parse_list(t[1], o->flows[k], ROADMC_MAX_CONTROLS);
+    const int ns = parse_list(t[2], o->switching[k], 1 << ROADMC_MAX_CONTROLS);
+    if (nf < 0 || ns < 0) return ROADMC_ERR_NUMERIC;
+    o->n_flows[k] = nf;
+    o->n_switching[k] = ns;
+    o->bridge[k] = t.size() > 3 && t[3] == "TRUE" ? 1 : 0;
+    ++o->n_programs;
+  }
+  const int declared = int_or(kv, "Programs", -1);
+  if (declared >= 0 && declared != o->n_programs) return ROADMC_ERR_INVALID;
+  o->n_fuels = int_or(kv, "Number of Fuels", -1);
+  o->n_transported = int_or(kv, "Number of Transported Commodities",
+                            int_or(kv, "Number of Commodities", -1));
+  o->n_commodities = file_list(kv, "Commodity", o->commodity);
+  o->n_vehicles = file_list(kv, "Vehicle", o->vehicle);
+  o->n_species = file_list(kv, "Species", o->species);
+  o->road_exists = flag_or(kv, "Road Exists");
+  o->travel_time = num_or_nan(kv, "Travel Time (hours)");
+  o->road_length = num_or_nan(kv, "Length (m)");
+  o->accident_cost = num_or_nan(kv, "Accident Cost per Unit Traffic");
+  return ROADMC_OK;
+}

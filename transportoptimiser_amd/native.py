@@ -251,6 +251,43 @@ class SpeciesFile(C.Structure):
                                   "growth_reversion", "growth_jump_prob", "growth_poisson_jump")]
 
 
+MAX_LIST, MAX_PROGRAMS, MAX_FILES, PATH_LEN, MAX_CONTROLS = 16, 8, 16, 256, 8
+
+
+class RoadmcList(C.Structure):
+    _fields_ = [("n", C.c_int32), ("v", C.c_double * MAX_LIST)]
+
+    def values(self) -> list:
+        return [self.v[i] for i in range(self.n)]
+
+
+RUN_LISTS = ("animal_bridges", "hab_pref_sd", "movement_sd", "ranging_sd", "growth_mean",
+             "growth_sd", "price_mean", "price_sd", "ore_sd", "pop_level", "comparison_cost")
+
+
+class RunSettings(C.Structure):
+    _fields_ = [
+        ("confidence_interval", C.c_double), ("use_gpu", C.c_int32),
+        ("surrogate_dim_res", C.c_int32), ("interpolation", C.c_int32),
+        ("traffic_growth", C.c_double), ("rrr", C.c_double), ("n_years", C.c_int32),
+        ("time_step", C.c_double), ("n_paths", C.c_int32), ("rov_dim_res", C.c_int32),
+        ("optimisation_method", C.c_int32), ("rov_method", C.c_int32),
+    ] + [(k, RoadmcList) for k in RUN_LISTS] + [
+        ("n_programs", C.c_int32), ("n_flows", C.c_int32 * MAX_PROGRAMS),
+        ("flows", (C.c_double * MAX_CONTROLS) * MAX_PROGRAMS),
+        ("n_switching", C.c_int32 * MAX_PROGRAMS),
+        ("switching", (C.c_double * (1 << MAX_CONTROLS)) * MAX_PROGRAMS),
+        ("bridge", C.c_int32 * MAX_PROGRAMS),
+        ("n_fuels", C.c_int32), ("n_transported", C.c_int32),
+        ("n_commodities", C.c_int32), ("n_vehicles", C.c_int32), ("n_species", C.c_int32),
+        ("commodity", (C.c_char * PATH_LEN) * MAX_FILES),
+        ("vehicle", (C.c_char * PATH_LEN) * MAX_FILES),
+        ("species", (C.c_char * PATH_LEN) * MAX_FILES),
+        ("road_exists", C.c_int32), ("travel_time", C.c_double), ("road_length", C.c_double),
+        ("accident_cost", C.c_double), ("fuel_per_journey", RoadmcList),
+    ]
+
+
 Species._fields_ = [
     ("n_patches", C.c_int32),
     ("init_pop", _dp),
@@ -306,6 +343,7 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_read_commodity": (C.c_int, [C.c_char_p, C.POINTER(CommodityFile)]),
     "roadmc_read_vehicle": (C.c_int, [C.c_char_p, C.POINTER(VehicleFile)]),
     "roadmc_read_species": (C.c_int, [C.c_char_p, C.POINTER(SpeciesFile)]),
+    "roadmc_read_run_settings": (C.c_int, [C.c_char_p, C.POINTER(RunSettings)]),
     "roadmc_mte_adjoint_fwd": (C.c_int, [C.c_void_p, C.POINTER(MteProblem), C.c_int32,
                                          C.POINTER(C.c_void_p), _dp]),
     "roadmc_mte_adjoint_bwd": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
