@@ -27,7 +27,15 @@ constexpr int kKnnThreads = 64;  // one wave per workgroup
 constexpr int kKnnMaxK = 128;
 
 constexpr int kKnnSelectMin = 2048;  // above this many samples: the radix-select kernel
-constexpr int kKnnSelectCap = 2048;  // samples of the k-th binade and below ranked in LDS
+constexpr int kKnnSelectCap = 2048;  // largest bucket of the k-th key sorted in LDS
+constexpr int kKnnMaxBatch = 8;      // fits of one launch (blockIdx.y)
+
+// Fits launched together, one per blockIdx.y; grid points beyond a fit's G
+// exit at once.
+struct KnnFitBatch {
+  int nfits;
+  KnnFitArgs f[kKnnMaxBatch];
+};
 
 // Dynamic LDS of the wave-per-point fit kernel: per-lane lists of cap
 // entries, then the k merged neighbours.
@@ -36,6 +44,9 @@ inline size_t knn_fit_lds(int cap, int k) {
 }
 
 hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s);
+// Up to kKnnMaxBatch fits of the same dimension d in at most two launches
+// (one per fit kernel), so that small fits fill the machine together.
+hipError_t launch_knn_fit_batch(const KnnFitArgs* fits, int nfits, hipStream_t s);
 hipError_t launch_knn_eval(const KnnEvalArgs& a, hipStream_t s);
 
 }  // namespace roadmc

@@ -23,6 +23,8 @@
 // and the sums follow the same roundings as the CPU statement.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "knn_kernels.h"
 
 namespace roadmc {
@@ -100,7 +102,9 @@ __device__ double knn_regress(const double* nbd, const int* nbi, int k, const do
 }
 
 template <int D>
-__global__ __launch_bounds__(kKnnThreads) void knn_fit_kernel(KnnFitArgs a) {
+__global__ __launch_bounds__(kKnnThreads) void knn_fit_kernel(KnnFitBatch batch) {
+  const KnnFitArgs& a = batch.f[blockIdx.y];
+  if ((long long)blockIdx.x >= a.G) return;
   extern __shared__ double knn_lds[];
   const int cap = a.cap, k = a.k, R = a.R, n = a.n;
   double* topd = knn_lds;                                              // [cap][64]
@@ -187,7 +191,9 @@ __global__ __launch_bounds__(kKnnThreads) void knn_fit_kernel(KnnFitArgs a) {
 // Either way the k neighbours are ranked into (distance, index) order and
 // fitted as above.
 template <int D>
-__global__ __launch_bounds__(256) void knn_select_kernel(KnnFitArgs a) {
+__global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
+  const KnnFitArgs& a = batch.f[blockIdx.y];
+  if ((long long)blockIdx.x >= a.G) return;
   constexpr int kBins = 2048;
   __shared__ unsigned int hist[kBins];
   __shared__ unsigned int part[256];
@@ -409,17 +415,36 @@ __global__ __launch_bounds__(256) void knn_eval_kernel(KnnEvalArgs a) {
 }
 
 template <int D>
-static hipError_t fit_d(const KnnFitArgs& a, hipStream_t s) {
-  if (a.n > kKnnSelectMin) {
-    hipLaunchKernelGGL(knn_select_kernel<D>, dim3((unsigned)a.G), dim3(256), 0, s, a);
+static hipError_t fit_d(const KnnFitArgs* fits, int nfits, hipStream_t s) {
+  KnnFitBatch sel{}, wave{};
+  long long gsel = 0, gwave = 0;
+  size_t lds = 0;
+  for (int i = 0; i < nfits; ++i) {
+    const KnnFitArgs& a = fits[i];
+    if (a.n > kKnnSelectMin) {
+      sel.f[sel.nfits++] = a;
+      gsel = std::max(gsel, a.G);
+    } else {
+      wave.f[wave.nfits++] = a;
+      gwave = std::max(gwave, a.G);
+      lds = std::max(lds, knn_fit_lds(a.cap, a.k));
+    }
+  }
+  if (sel.nfits) {
+    hipLaunchKernelGGL(knn_select_kernel<D>, dim3((unsigned)gsel, sel.nfits), dim3(256), 0, s,
+                       sel);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (wave.nfits) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&knn_fit_kernel<D>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(knn_fit_kernel<D>, dim3((unsigned)gwave, wave.nfits), dim3(kKnnThreads),
+                       lds, s, wave);
     return hipGetLastError();
   }
-  const size_t lds = knn_fit_lds(a.cap, a.k);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&knn_fit_kernel<D>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(knn_fit_kernel<D>, dim3((unsigned)a.G), dim3(kKnnThreads), lds, s, a);
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 template <int D>
@@ -428,18 +453,27 @@ static hipError_t eval_d(const KnnEvalArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s) {
-  switch (a.d) {
-    case 1: return fit_d<1>(a, s);
-    case 2: return fit_d<2>(a, s);
-    case 3: return fit_d<3>(a, s);
-    case 4: return fit_d<4>(a, s);
-    case 5: return fit_d<5>(a, s);
-    case 6: return fit_d<6>(a, s);
-    case 7: return fit_d<7>(a, s);
-    case 8: return fit_d<8>(a, s);
+hipError_t launch_knn_fit_batch(const KnnFitArgs* fits, int nfits, hipStream_t s) {
+  if (nfits < 1) return hipSuccess;
+  if (nfits > kKnnMaxBatch) return hipErrorInvalidValue;
+  const int d = fits[0].d;
+  for (int i = 1; i < nfits; ++i)
+    if (fits[i].d != d) return hipErrorInvalidValue;
+  switch (d) {
+    case 1: return fit_d<1>(fits, nfits, s);
+    case 2: return fit_d<2>(fits, nfits, s);
+    case 3: return fit_d<3>(fits, nfits, s);
+    case 4: return fit_d<4>(fits, nfits, s);
+    case 5: return fit_d<5>(fits, nfits, s);
+    case 6: return fit_d<6>(fits, nfits, s);
+    case 7: return fit_d<7>(fits, nfits, s);
+    case 8: return fit_d<8>(fits, nfits, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s) {
+  return launch_knn_fit_batch(&a, 1, s);
 }
 
 hipError_t launch_knn_eval(const KnnEvalArgs& a, hipStream_t s) {

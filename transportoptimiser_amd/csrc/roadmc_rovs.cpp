@@ -326,6 +326,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     RMC_HIP(ctx, hipStreamSynchronize(s));
     RMC_HIP(ctx, roadmc::launch_rovs_ranges(d_args, N, D, s));
     RMC_HIP(ctx, roadmc::launch_rovs_normalise(d_args, N, D, R, s));
+    std::vector<roadmc::KnnFitArgs> fits;
     for (int c = 0; c < C; ++c) {
       const int n = counts[c];
       // k = 10 (ln n + 1) for more than 3 samples (SimulateGPU.cu:4522-4525).
@@ -348,8 +349,10 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
       ka.y = h.ys + (size_t)c * N;
       ka.qtab = h.qtab;
       ka.val = vals;
-      RMC_HIP(ctx, roadmc::launch_knn_fit(ka, s));
+      fits.push_back(ka);
     }
+    // The step's fits in one launch per fit kernel.
+    RMC_HIP(ctx, roadmc::launch_knn_fit_batch(fits.data(), (int)fits.size(), s));
     if (algo6) {
       double* gt = static_cast<double*>(B.grids.ptr) + (size_t)t * C * gsz;
       RMC_HIP(ctx, hipMemcpyAsync(gt, h.grid, (size_t)C * gsz * sizeof(double),
