@@ -227,14 +227,15 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
     // for every interior step, then time 0.
     RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    if (american) {
-      e = roadmc::launch_rov_step(a, pa, T, false, true, s);
-      for (int t = T - 1; t >= 1 && e == hipSuccess; --t)
-        e = roadmc::launch_rov_step(a, pa, t, true, false, s);
+    // Each step's decision runs inside the next step's assembly (the
+    // terminal one inside the first fit); the decision of step 1 alone.
+    const int t_term = american ? T : T - 1;
+    if (t_term - 1 >= 1) {
+      for (int t = t_term - 1; t >= 1 && e == hipSuccess; --t)
+        e = roadmc::launch_rov_fit(a, pa, t, t + 1 == t_term ? 2 : 1, s);
+      if (e == hipSuccess) e = roadmc::launch_rov_decide(a, pa, 1, false, s);
     } else {
-      e = roadmc::launch_rov_step(a, pa, T - 1, false, true, s);
-      for (int t = T - 2; t >= 1 && e == hipSuccess; --t)
-        e = roadmc::launch_rov_step(a, pa, t, true, false, s);
+      e = roadmc::launch_rov_decide(a, pa, t_term, true, s);
     }
     if (e == hipSuccess) e = roadmc::launch_rov_first(pa, s);
     hipGraph_t graph = nullptr;

@@ -54,8 +54,13 @@ hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double
 // The step kernels read their parameters from the device copy `pa` of `a`
 // (a supplies the launch shapes), so a captured graph of them stays valid
 // for any problem of the same shape.
-hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, bool fit,
-                           bool terminal, hipStream_t s);
+// Fit of step t: normal equations, their reduction and solve.  With
+// decide_next >= 0 the assembly first takes the decision of step t + 1
+// (decide_next == 2: the terminal rule) for every path.
+hipError_t launch_rov_fit(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
+                          hipStream_t s);
+hipError_t launch_rov_decide(const RovStepArgs& a, const RovStepArgs* pa, int t, bool terminal,
+                             hipStream_t s);
 hipError_t launch_rov_first(const RovStepArgs* pa, hipStream_t s);
 hipError_t launch_exp_pv(const ExpPvArgs& a, hipStream_t s);
 

@@ -213,8 +213,95 @@ constexpr int kRovAsmPoints = 256;
 
 __host__ __device__ inline int rov_kp(int K) { return (K + 1 + 15) / 16 * 16; }
 
+__device__ double rov_decide_path(const RovStepArgs& a, int t, int terminal, int p) {
+  const double disc = a.disc;
+  double* V = a.V;
+  if (a.mode == kRovAmerican) {
+    const double e = fmax(a.strike - a.states[((size_t)t * a.U) * a.N + p], 0.0);
+    if (terminal) {
+      V[(size_t)t * a.N + p] = e;
+      a.optc[(size_t)(t - 1) * a.N + p] = e > 0.0 ? 1 : 0;
+      return e;
+    }
+    const double later = disc * V[(size_t)(t + 1) * a.N + p];
+    int ex = 0;
+    if (e > 0.0 || !a.itm_only) {
+      double x[kRovMaxU];
+      rov_features(a, t, p, x);
+      const double* b = a.beta + (size_t)t * a.G * a.K;
+      double cont = 0.0;
+      rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { cont = fma(v, b[k], cont); });
+      cont *= disc;
+      ex = e > cont ? 1 : 0;
+    }
+    double v;
+    if (a.value_mode == kValueRealized) {
+      v = ex ? e : later;
+    } else {
+      v = ex ? e : later;
+    }
+    V[(size_t)t * a.N + p] = v;
+    a.optc[(size_t)(t - 1) * a.N + p] = ex;
+    return v;
+  }
+  // Flow control (backwardInduction, SimulateGPU.cu:2333-2784).
+  const double up = rov_unit_profit(a, t, p);
+  const double m = rov_flow_scale(a, t, p);
+  // Continuation value of every fit group, one pass over the basis.
+  double cont[kRovMaxC];
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) cont[c] = 0.0;
+  if (!terminal) {
+    double x[kRovMaxU];
+    rov_features(a, t, p, x);
+    const double* b = a.beta + (size_t)t * a.G * a.K;
+    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) {
+#pragma unroll
+      for (int c = 0; c < kRovMaxC; ++c)
+        if (c < a.G) cont[c] = fma(v, b[(size_t)c * a.K + k], cont[c]);
+    });
+  }
+  double best = 0.0, best_cur = 0.0, best_pay = 0.0;
+  int bc = 0;
+#pragma unroll
+  for (int c = 0; c < kRovMaxC; ++c) {
+    if (c >= a.C) break;
+    const double cur = a.flow[c] * m * up;
+    double pay;
+    if (terminal) {
+      // Terminal step: payoff clamped at zero (SimulateGPU.cu:2385-2416).
+      pay = fmin(cur, 0.0);
+    } else {
+      double cv = cont[a.pooled ? 0 : c] * disc;  // SimulateGPU.cu:2703
+      if (cv > 0.0) cv = 0.0;                      // SimulateGPU.cu:2712-2714
+      pay = cur + cv;
+    }
+    if (c == 0 || pay < a.hysteresis * best) {  // SimulateGPU.cu:2757-2771
+      best = pay;
+      bc = c;
+      best_cur = cur;
+      best_pay = pay;
+    }
+  }
+  double v = best;
+  if (a.value_mode == kValueRealized)
+    v = terminal ? best_pay : best_cur + disc * V[(size_t)(t + 1) * a.N + p];
+  V[(size_t)t * a.N + p] = v;
+  a.optc[(size_t)t * a.N + p] = bc;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void rov_decide_kernel(const RovStepArgs* __restrict__ pa, int t,
+                                                         int terminal) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < pa->N) rov_decide_path(*pa, t, terminal, p);
+}
+
+// With decide >= 0 the kernel first takes the decision of step t + 1 for its
+// paths (rov_decide_path, terminal when decide == 2) and fits on the values
+// it just wrote, so that step's separate decide launch is saved.
 __global__ __launch_bounds__(256) void rov_assemble_kernel(const RovStepArgs* __restrict__ pa,
-                                                           int t) {
+                                                           int t, int decide) {
   typedef double doublex4 __attribute__((ext_vector_type(4)));
   const RovStepArgs& a = *pa;
   extern __shared__ double sm[];
@@ -227,11 +314,13 @@ __global__ __launch_bounds__(256) void rov_assemble_kernel(const RovStepArgs* __
   int g = -1;
   for (int k = 0; k < KP; ++k) row[k] = 0.0;
   if (p < a.N) {
+    const double vnext = decide >= 0 ? rov_decide_path(a, t + 1, decide == 2 ? 1 : 0, p)
+                                     : a.V[(size_t)(t + 1) * a.N + p];
     g = rov_fit_group(a, t, p);
     double x[kRovMaxU];
     rov_features(a, t, p, x);
     rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { row[k] = v; });
-    row[K] = a.V[(size_t)(t + 1) * a.N + p];
+    row[K] = vnext;
   }
   sg[q] = g;
   __syncthreads();
@@ -436,87 +525,6 @@ __global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __rest
 }
 
 // Decision of step t for every path.
-__global__ __launch_bounds__(256) void rov_decide_kernel(const RovStepArgs* __restrict__ pa, int t,
-                                                         int terminal) {
-  const RovStepArgs& a = *pa;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.N) return;
-  const double disc = a.disc;
-  double* V = a.V;
-  if (a.mode == kRovAmerican) {
-    const double e = fmax(a.strike - a.states[((size_t)t * a.U) * a.N + p], 0.0);
-    if (terminal) {
-      V[(size_t)t * a.N + p] = e;
-      a.optc[(size_t)(t - 1) * a.N + p] = e > 0.0 ? 1 : 0;
-      return;
-    }
-    const double later = disc * V[(size_t)(t + 1) * a.N + p];
-    int ex = 0;
-    if (e > 0.0 || !a.itm_only) {
-      double x[kRovMaxU];
-      rov_features(a, t, p, x);
-      const double* b = a.beta + (size_t)t * a.G * a.K;
-      double cont = 0.0;
-      rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { cont = fma(v, b[k], cont); });
-      cont *= disc;
-      ex = e > cont ? 1 : 0;
-    }
-    double v;
-    if (a.value_mode == kValueRealized) {
-      v = ex ? e : later;
-    } else {
-      v = ex ? e : later;
-    }
-    V[(size_t)t * a.N + p] = v;
-    a.optc[(size_t)(t - 1) * a.N + p] = ex;
-    return;
-  }
-  // Flow control (backwardInduction, SimulateGPU.cu:2333-2784).
-  const double up = rov_unit_profit(a, t, p);
-  const double m = rov_flow_scale(a, t, p);
-  // Continuation value of every fit group, one pass over the basis.
-  double cont[kRovMaxC];
-#pragma unroll
-  for (int c = 0; c < kRovMaxC; ++c) cont[c] = 0.0;
-  if (!terminal) {
-    double x[kRovMaxU];
-    rov_features(a, t, p, x);
-    const double* b = a.beta + (size_t)t * a.G * a.K;
-    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) {
-#pragma unroll
-      for (int c = 0; c < kRovMaxC; ++c)
-        if (c < a.G) cont[c] = fma(v, b[(size_t)c * a.K + k], cont[c]);
-    });
-  }
-  double best = 0.0, best_cur = 0.0, best_pay = 0.0;
-  int bc = 0;
-#pragma unroll
-  for (int c = 0; c < kRovMaxC; ++c) {
-    if (c >= a.C) break;
-    const double cur = a.flow[c] * m * up;
-    double pay;
-    if (terminal) {
-      // Terminal step: payoff clamped at zero (SimulateGPU.cu:2385-2416).
-      pay = fmin(cur, 0.0);
-    } else {
-      double cv = cont[a.pooled ? 0 : c] * disc;  // SimulateGPU.cu:2703
-      if (cv > 0.0) cv = 0.0;                      // SimulateGPU.cu:2712-2714
-      pay = cur + cv;
-    }
-    if (c == 0 || pay < a.hysteresis * best) {  // SimulateGPU.cu:2757-2771
-      best = pay;
-      bc = c;
-      best_cur = cur;
-      best_pay = pay;
-    }
-  }
-  double v = best;
-  if (a.value_mode == kValueRealized)
-    v = terminal ? best_pay : best_cur + disc * V[(size_t)(t + 1) * a.N + p];
-  V[(size_t)t * a.N + p] = v;
-  a.optc[(size_t)t * a.N + p] = bc;
-}
-
 // Time 0 (firstPeriodInduction, SimulateGPU.cu:2210-2330): per initial
 // control, the mean discounted value of the paths that took it, plus the
 // deterministic current payoff; out = {value, control, SD}.  One 1024-thread
@@ -643,11 +651,11 @@ size_t rov_assemble_lds(int K) {
   return (size_t)kRovAsmPoints * (rov_kp(K) + 1) * sizeof(double) + kRovAsmPoints * sizeof(int);
 }
 
-hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, bool fit,
-                           bool terminal, hipStream_t s) {
-  if (fit) {
+hipError_t launch_rov_fit(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
+                          hipStream_t s) {
+  {
     hipLaunchKernelGGL(rov_assemble_kernel, dim3(a.nblk), dim3(256), rov_assemble_lds(a.K), s,
-                       pa, t);
+                       pa, t, decide_next);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rov_reduce_kernel, dim3(a.G * a.E), dim3(64), 0, s, pa);
@@ -666,9 +674,12 @@ hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, b
       default:
         return hipErrorInvalidValue;
     }
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
   }
+}
+
+hipError_t launch_rov_decide(const RovStepArgs& a, const RovStepArgs* pa, int t, bool terminal,
+                             hipStream_t s) {
   hipLaunchKernelGGL(rov_decide_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, pa, t,
                      terminal ? 1 : 0);
   return hipGetLastError();
