@@ -182,3 +182,23 @@ def test_algo6_limits(ctx):
     with pytest.raises(nat.RoadmcError) as e:
         rs.simulate_rov_species(ctx, pr)
     assert e.value.code == nat.ERR_INVALID
+
+
+@pytest.mark.parametrize("P,S", [(512, 2), (256, 4), (33, 7)])
+def test_algo6_realized_recursion_at_the_tile_limits(ctx, P, S):
+    # The largest species tiles one workgroup holds (2 x 512, 4 x 256 patches)
+    # and the most species (7, at a ragged patch count): without mortality
+    # the realized recursion holds path by path.
+    from tests.test_oracle_rovs import _stochastic_problem
+    T = 6
+    pr = _stochastic_problem(T=T, N=333, P=P)
+    pr.species = [pr.species[0]] * S
+    pr.method = 6
+    pr.dim_res = 3
+    g = _run(ctx, pr)
+    V, oc, up = g["cond_exp"], g["opt_control"], g["unit_profit"]
+    d = 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+    Q = np.asarray(pr.flow)
+    assert len(np.unique(oc[1:T - 1])) > 1
+    for t in range(1, T - 1):
+        np.testing.assert_allclose(V[t], Q[oc[t]] * up[t] + d * V[t + 1], rtol=1e-10, atol=1e-12)
