@@ -101,7 +101,7 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->adj_bufs.h,     &ctx->road_bufs.a,   &ctx->road_bufs.b,
                         &ctx->road_bufs.c,    &ctx->road_bufs.d,   &ctx->road_bufs.e,
                         &ctx->road_bufs.f,    &ctx->road_bufs.g,   &ctx->road_bufs.h,
-                        &ctx->road_bufs.tail,
+                        &ctx->road_bufs.tail, &ctx->road_bufs.in,
                         &ctx->knn_bufs.a,     &ctx->knn_bufs.b,    &ctx->knn_bufs.c,
                         &ctx->knn_bufs.d,     &ctx->rovs_bufs.mpack, &ctx->rovs_bufs.cols,
                         &ctx->rovs_bufs.prices, &ctx->rovs_bufs.controls, &ctx->rovs_bufs.tp,

@@ -42,7 +42,7 @@ struct roadmc_ctx {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } adj_bufs;
   struct {
-    rmc_devbuf a, b, c, d, e, f, g, h, tail;
+    rmc_devbuf a, b, c, d, e, f, g, h, tail, in;
   } road_bufs;
   struct {
     rmc_devbuf a, b, c, d;

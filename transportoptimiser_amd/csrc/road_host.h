@@ -13,8 +13,7 @@ namespace roadmc {
 // Host copies a build's uploads read from; kept alive until the stream has
 // passed them.
 struct RoadStaging {
-  std::vector<double> xyw;
-  std::vector<double4> seg;
+  std::vector<double> in;  // road_stage blocks
   RoadTail tail;
 };
 
@@ -28,6 +27,13 @@ int road_reserve(roadmc_ctx* ctx, int P, int S, int C);
 // already be reserved for this build.
 int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, bool want_surv,
                 bool reset_tail, RoadStaging& st, RoadArgs* out);
+// Appends one species' build inputs to a host block (patch x, y and weights,
+// the open-road segments, the flows; 32-byte aligned parts) and returns the
+// block's offset in doubles, so that many builds share one upload.
+size_t road_stage(const roadmc_road_species* r, std::vector<double>& in);
+// Launches the build of r whose staged block is at d_in on the device.
+int road_launch_staged(roadmc_ctx* ctx, const roadmc_road_species* r, const double* d_in,
+                       bool want_trans, bool want_surv, RoadArgs* out);
 RoadTail* road_tail(roadmc_ctx* ctx);
 
 }  // namespace roadmc

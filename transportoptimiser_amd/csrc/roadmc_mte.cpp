@@ -106,11 +106,14 @@ template <typename T>
 void pack_species(const roadmc_species& sp, int Pp, int Kp, double tol, bool fp64, T* mp,
                   T* cols) {
   const int P = sp.n_patches;
-  std::fill(mp, mp + (size_t)Pp * Kp, (T)0);
-  if (sp.trans)  // otherwise packed on the device from the species' road
+  // A species given by its road is packed on the device over the whole
+  // region (padding included); the host leaves its region untouched.
+  if (sp.trans) {
+    std::fill(mp, mp + (size_t)Pp * Kp, (T)0);
     for (int j = 0; j < P; ++j)    // destination: one contiguous column of trans and surv
       for (int k = 0; k < P; ++k)
         mp[roadmc::mte_pack_index(fp64, Kp, k, j)] = (T)movement(sp, P, k, j, tol);
+  }
   for (int j = 0; j < Pp; ++j) {
     const bool in = j < P;
     cols[j] = in ? (T)sp.capacity[j] : (T)1;
@@ -307,7 +310,9 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
   RMC_HIP(ctx, hipMemcpyAsync(static_cast<int*>(b.counter.ptr) + 16, js.qbeg.data(),
                               js.qbeg.size() * sizeof(int), hipMemcpyHostToDevice, s));
   // Species given by a road: build trans and surv on the device and pack
-  // them into their job's slot (after the zero-filled upload above).
+  // them into their job's slot (after the upload above).  Every road's
+  // inputs go up in one copy; the builds then run one after the other in
+  // the shared scratch.
   int maxP = 0, maxS = 0, maxC = 0;
   for (size_t j = 0; j < js.jobs.size(); ++j) {
     const roadmc_species& sp = roads[js.road_of[j]].species[js.species_of[j]];
@@ -316,22 +321,30 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
     maxS = std::max(maxS, roadmc::road_visible_segments(sp.road));
     maxC = std::max(maxC, sp.road->n_controls);
   }
-  std::vector<roadmc::RoadStaging> staging;
+  std::vector<double> road_in;
+  std::vector<size_t> road_off(js.jobs.size(), 0);
+  roadmc::RoadTail tail0{0, {0, 0, 0}};
   if (maxP > 0) {
     if ((rc = roadmc::road_reserve(ctx, maxP, maxS, maxC))) return rc;
-    staging.reserve(js.jobs.size());
-    bool first = true;
+    for (size_t j = 0; j < js.jobs.size(); ++j) {
+      const roadmc_species& sp = roads[js.road_of[j]].species[js.species_of[j]];
+      if (!sp.trans) road_off[j] = roadmc::road_stage(sp.road, road_in);
+    }
+    if ((rc = rmc_buf_ensure(ctx, ctx->road_bufs.in, road_in.size() * sizeof(double)))) return rc;
+    const double* d_in = static_cast<const double*>(ctx->road_bufs.in.ptr);
+    RMC_HIP(ctx, hipMemcpyAsync(ctx->road_bufs.in.ptr, road_in.data(),
+                                road_in.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    RMC_HIP(ctx, hipMemcpyAsync(roadmc::road_tail(ctx), &tail0, sizeof(tail0),
+                                hipMemcpyHostToDevice, s));
     for (size_t j = 0; j < js.jobs.size(); ++j) {
       const roadmc_mte_problem& pr = roads[js.road_of[j]];
       const roadmc_species& sp = pr.species[js.species_of[j]];
       if (sp.trans) continue;
       const roadmc_road_species* rd = sp.road;
       const int control = sp.road_control < 0 ? rd->n_controls - 1 : sp.road_control;
-      staging.emplace_back();
       roadmc::RoadArgs a;
-      if ((rc = roadmc::road_launch(ctx, rd, true, control >= 0, first, staging.back(), &a)))
+      if ((rc = roadmc::road_launch_staged(ctx, rd, d_in + road_off[j], true, control >= 0, &a)))
         return rc;
-      first = false;
       const double* surv = control >= 0 ? a.surv + (size_t)control * rd->n_patches * rd->n_patches
                                         : nullptr;
       T* dst = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];

@@ -46,16 +46,24 @@ int road_visible_segments(const roadmc_road_species* r) {
   return S;
 }
 
+namespace {
+
+inline size_t up4(size_t n) { return (n + 3) & ~(size_t)3; }
+
+// Doubles of one staged block: x, y, weights (padded to 4), the segments as
+// (x3, y3, x4, y4), the flows.
+size_t staged_len(int P, int S, int C) { return up4(3 * (size_t)P) + 4 * (size_t)S + up4(C); }
+
+}  // namespace
+
 int road_reserve(roadmc_ctx* ctx, int P, int S, int C) {
   const size_t n = (size_t)P * P;
   auto& B = ctx->road_bufs;
   int rc;
-  if ((rc = rmc_buf_ensure(ctx, B.a, 3 * (size_t)P * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.b, std::max<size_t>(S, 1) * sizeof(double4))) ||
+  if ((rc = rmc_buf_ensure(ctx, B.a, staged_len(P, S, C) * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.c, n * sizeof(int32_t))) ||
       (rc = rmc_buf_ensure(ctx, B.d, n * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.e, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, std::max<size_t>(C, 1) * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.g, (size_t)road_cross_blocks(P) * sizeof(int))) ||
       (rc = rmc_buf_ensure(ctx, B.h, road_part_elems(P) * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.tail, sizeof(RoadTail))))
@@ -65,54 +73,54 @@ int road_reserve(roadmc_ctx* ctx, int P, int S, int C) {
 
 RoadTail* road_tail(roadmc_ctx* ctx) { return static_cast<RoadTail*>(ctx->road_bufs.tail.ptr); }
 
-int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, bool want_surv,
-                bool reset_tail, RoadStaging& st, RoadArgs* out) {
-  const int P = r->n_patches, C = r->n_controls;
-  // Transition rate and the two distance cut-offs, as the reference forms
-  // them (SpeciesRoadPatches.cpp:328-332, 387-395).
+size_t road_stage(const roadmc_road_species* r, std::vector<double>& in) {
+  const int P = r->n_patches, C = r->n_controls, S = road_visible_segments(r);
+  const size_t off = in.size();
+  in.resize(off + staged_len(P, S, C), 0.0);
+  double* b = in.data() + off;
+  // Transition rate (SpeciesRoadPatches.cpp:328-332) and the patch weights
+  // A_j lambda e^(h_j) of the transition entries (:402-419).
+  const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
+  std::memcpy(b, r->x, P * sizeof(double));
+  std::memcpy(b + P, r->y, P * sizeof(double));
+  for (int j = 0; j < P; ++j)
+    b[2 * (size_t)P + j] = r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
+  double* seg = b + up4(3 * (size_t)P);
+  for (int s = 0, q = 0; s + 1 < r->n_road_points; ++s)
+    if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD) {
+      seg[4 * q + 0] = r->road_x[s];
+      seg[4 * q + 1] = r->road_y[s];
+      seg[4 * q + 2] = r->road_x[s + 1];
+      seg[4 * q + 3] = r->road_y[s + 1];
+      ++q;
+    }
+  if (C > 0) std::memcpy(seg + 4 * (size_t)S, r->flow, C * sizeof(double));
+  return off;
+}
+
+int road_launch_staged(roadmc_ctx* ctx, const roadmc_road_species* r, const double* d_in,
+                       bool want_trans, bool want_surv, RoadArgs* out) {
+  const int P = r->n_patches, C = r->n_controls, S = road_visible_segments(r);
+  // The two distance cut-offs, as the reference forms them
+  // (SpeciesRoadPatches.cpp:328-332, 387-395).
   const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
   const double cut_cross = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * r->lambda_mult));
   const double cut_trans = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * lambda));
-  st.xyw.resize(3 * (size_t)P);
-  std::memcpy(st.xyw.data(), r->x, P * sizeof(double));
-  std::memcpy(st.xyw.data() + P, r->y, P * sizeof(double));
-  for (int j = 0; j < P; ++j)
-    st.xyw[2 * (size_t)P + j] =
-        r->area[j] * lambda * std::exp(r->pref_mean[j] + r->pref_sd[j] * r->pref_mult);
-  st.seg.clear();
-  for (int s = 0; s + 1 < r->n_road_points; ++s)
-    if (!r->road_type || r->road_type[s] == ROADMC_SEG_ROAD)
-      st.seg.push_back(make_double4(r->road_x[s], r->road_y[s], r->road_x[s + 1], r->road_y[s + 1]));
-  const int S = (int)st.seg.size();
   auto& B = ctx->road_bufs;
-  hipStream_t s = ctx->stream;
-  double* d_xyw = static_cast<double*>(B.a.ptr);
-  RMC_HIP(ctx, hipMemcpyAsync(d_xyw, st.xyw.data(), st.xyw.size() * sizeof(double),
-                              hipMemcpyHostToDevice, s));
-  if (S > 0)
-    RMC_HIP(ctx, hipMemcpyAsync(B.b.ptr, st.seg.data(), S * sizeof(double4),
-                                hipMemcpyHostToDevice, s));
-  if (C > 0)
-    RMC_HIP(ctx, hipMemcpyAsync(B.f.ptr, r->flow, C * sizeof(double), hipMemcpyHostToDevice, s));
-  if (reset_tail) {
-    st.tail = RoadTail{0, {0, 0, 0}};
-    RMC_HIP(ctx, hipMemcpyAsync(road_tail(ctx), &st.tail, sizeof(RoadTail),
-                                hipMemcpyHostToDevice, s));
-  }
   RoadArgs a{};
   a.P = P;
   a.S = S;
   a.C = C;
-  a.x = d_xyw;
-  a.y = d_xyw + P;
-  a.pw = d_xyw + 2 * P;
+  a.x = d_in;
+  a.y = d_in + P;
+  a.pw = d_in + 2 * P;
   a.lambda = lambda;
   a.cut_cross = cut_cross;
   a.cut_trans = cut_trans;
-  a.seg = static_cast<const double4*>(B.b.ptr);
+  a.seg = reinterpret_cast<const double4*>(d_in + up4(3 * (size_t)P));
   a.form = r->transition_form == ROADMC_TRANS_DECAY ? 1 : 0;
   a.drop_last = r->drop_last_link ? 1 : 0;
-  a.flow = static_cast<const double*>(B.f.ptr);
+  a.flow = d_in + up4(3 * (size_t)P) + 4 * (size_t)S;
   a.width_len = r->vehicle_width + r->animal_length;
   a.speed_3600 = r->animal_speed * 3600.0;
   a.cross = static_cast<int*>(B.c.ptr);
@@ -121,9 +129,25 @@ int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, 
   a.blockmax = static_cast<int*>(B.g.ptr);
   a.part = static_cast<double*>(B.h.ptr);
   a.tail = road_tail(ctx);
-  RMC_HIP(ctx, launch_road_build(a, want_trans, want_surv, s));
+  RMC_HIP(ctx, launch_road_build(a, want_trans, want_surv, ctx->stream));
   *out = a;
   return ROADMC_OK;
+}
+
+int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, bool want_surv,
+                bool reset_tail, RoadStaging& st, RoadArgs* out) {
+  st.in.clear();
+  road_stage(r, st.in);
+  hipStream_t s = ctx->stream;
+  RMC_HIP(ctx, hipMemcpyAsync(ctx->road_bufs.a.ptr, st.in.data(), st.in.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+  if (reset_tail) {
+    st.tail = RoadTail{0, {0, 0, 0}};
+    RMC_HIP(ctx, hipMemcpyAsync(road_tail(ctx), &st.tail, sizeof(RoadTail),
+                                hipMemcpyHostToDevice, s));
+  }
+  return road_launch_staged(ctx, r, static_cast<const double*>(ctx->road_bufs.a.ptr), want_trans,
+                            want_surv, out);
 }
 
 }  // namespace roadmc
