@@ -333,7 +333,7 @@ def main() -> int:
 
     plain = None
     if not args.no_plain:
-        idx = pick_sample(roads, [0.0, 0.12, 0.25])
+        idx = pick_sample(roads, [0.0, 0.25, 0.5])
         ts, ps = [], []
         for i in idx:
             t = time.perf_counter()
