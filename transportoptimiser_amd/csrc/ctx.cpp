@@ -112,6 +112,12 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->rovs_bufs.args, &ctx->rovs_bufs.ckpt, &ctx->rovs_bufs.grm,
                         &ctx->rovs_bufs.rvec, &ctx->rovs_bufs.grids};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
+  for (auto& l : ctx->road_lanes) {
+    for (rmc_devbuf* b : {&l.cross, &l.trans, &l.surv, &l.blockmax, &l.part}) rmc_buf_free(*b);
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+  }
+  if (ctx->road_ready) (void)hipEventDestroy(ctx->road_ready);
   if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
   rmc_buf_free(ctx->rov_graph.param);
   (void)hipStreamDestroy(ctx->stream);

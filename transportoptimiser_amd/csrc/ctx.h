@@ -44,6 +44,15 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h, tail, in;
   } road_bufs;
+  // Side streams (default priority) on which the road builds of one batch
+  // overlap, each with its own scratch; created on first use.
+  static constexpr int kRoadLanes = 4;
+  struct {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    rmc_devbuf cross, trans, surv, blockmax, part;
+  } road_lanes[kRoadLanes];
+  hipEvent_t road_ready = nullptr;
   struct {
     rmc_devbuf a, b, c, d;
   } knn_bufs;

@@ -31,9 +31,25 @@ int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, 
 // the open-road segments, the flows; 32-byte aligned parts) and returns the
 // block's offset in doubles, so that many builds share one upload.
 size_t road_stage(const roadmc_road_species* r, std::vector<double>& in);
-// Launches the build of r whose staged block is at d_in on the device.
+// Device scratch of one build: crossings, transitions, survival, the
+// per-workgroup maxima and the row-sum partials.
+struct RoadScratch {
+  int* cross;
+  double* trans;
+  double* surv;
+  int* blockmax;
+  double* part;
+};
+// Launches the build of r whose staged block is at d_in on the device, in
+// the given scratch on stream s.
 int road_launch_staged(roadmc_ctx* ctx, const roadmc_road_species* r, const double* d_in,
-                       bool want_trans, bool want_surv, RoadArgs* out);
+                       bool want_trans, bool want_surv, const RoadScratch& sc, hipStream_t s,
+                       RoadArgs* out);
+// The scratch of ctx's own stream (road_reserve).
+RoadScratch road_scratch(roadmc_ctx* ctx);
+// Side-stream lanes for overlapping builds: their streams and events, and
+// scratch for builds of up to P patches and C controls.
+int road_reserve_lanes(roadmc_ctx* ctx, int P, int C);
 RoadTail* road_tail(roadmc_ctx* ctx);
 
 }  // namespace roadmc

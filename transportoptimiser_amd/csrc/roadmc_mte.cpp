@@ -313,19 +313,18 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
   // them into their job's slot (after the upload above).  Every road's
   // inputs go up in one copy; the builds then run one after the other in
   // the shared scratch.
-  int maxP = 0, maxS = 0, maxC = 0;
+  int maxP = 0, maxC = 0;
   for (size_t j = 0; j < js.jobs.size(); ++j) {
     const roadmc_species& sp = roads[js.road_of[j]].species[js.species_of[j]];
     if (sp.trans) continue;
     maxP = std::max(maxP, sp.road->n_patches);
-    maxS = std::max(maxS, roadmc::road_visible_segments(sp.road));
     maxC = std::max(maxC, sp.road->n_controls);
   }
   std::vector<double> road_in;
   std::vector<size_t> road_off(js.jobs.size(), 0);
   roadmc::RoadTail tail0{0, {0, 0, 0}};
   if (maxP > 0) {
-    if ((rc = roadmc::road_reserve(ctx, maxP, maxS, maxC))) return rc;
+    if ((rc = rmc_buf_ensure(ctx, ctx->road_bufs.tail, sizeof(roadmc::RoadTail)))) return rc;
     for (size_t j = 0; j < js.jobs.size(); ++j) {
       const roadmc_species& sp = roads[js.road_of[j]].species[js.species_of[j]];
       if (!sp.trans) road_off[j] = roadmc::road_stage(sp.road, road_in);
@@ -336,20 +335,37 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
                                 road_in.size() * sizeof(double), hipMemcpyHostToDevice, s));
     RMC_HIP(ctx, hipMemcpyAsync(roadmc::road_tail(ctx), &tail0, sizeof(tail0),
                                 hipMemcpyHostToDevice, s));
+    // The builds run on the side lanes, round robin, each lane in its own
+    // scratch and after the uploads above; ctx's stream waits for all of
+    // them before anything reads the packed operands.
+    if ((rc = roadmc::road_reserve_lanes(ctx, maxP, maxC))) return rc;
+    RMC_HIP(ctx, hipEventRecord(ctx->road_ready, s));
+    for (auto& l : ctx->road_lanes) RMC_HIP(ctx, hipStreamWaitEvent(l.stream, ctx->road_ready, 0));
+    int lane = 0;
     for (size_t j = 0; j < js.jobs.size(); ++j) {
       const roadmc_mte_problem& pr = roads[js.road_of[j]];
       const roadmc_species& sp = pr.species[js.species_of[j]];
       if (sp.trans) continue;
       const roadmc_road_species* rd = sp.road;
       const int control = sp.road_control < 0 ? rd->n_controls - 1 : sp.road_control;
+      auto& L = ctx->road_lanes[lane];
+      lane = (lane + 1) % roadmc_ctx::kRoadLanes;
+      const roadmc::RoadScratch sc{static_cast<int*>(L.cross.ptr), static_cast<double*>(L.trans.ptr),
+                                   static_cast<double*>(L.surv.ptr), static_cast<int*>(L.blockmax.ptr),
+                                   static_cast<double*>(L.part.ptr)};
       roadmc::RoadArgs a;
-      if ((rc = roadmc::road_launch_staged(ctx, rd, d_in + road_off[j], true, control >= 0, &a)))
+      if ((rc = roadmc::road_launch_staged(ctx, rd, d_in + road_off[j], true, control >= 0, sc,
+                                           L.stream, &a)))
         return rc;
       const double* surv = control >= 0 ? a.surv + (size_t)control * rd->n_patches * rd->n_patches
                                         : nullptr;
       T* dst = static_cast<T*>(b.mpack.ptr) + js.mpack_off[j];
       RMC_HIP(ctx, roadmc::launch_road_pack(js.fp64, a.trans, surv, rd->n_patches, js.jobs[j].Pp,
-                                            js.jobs[j].Pk, pr.drop_tol, dst, s));
+                                            js.jobs[j].Pk, pr.drop_tol, dst, L.stream));
+    }
+    for (auto& l : ctx->road_lanes) {
+      RMC_HIP(ctx, hipEventRecord(l.done, l.stream));
+      RMC_HIP(ctx, hipStreamWaitEvent(s, l.done, 0));
     }
   }
   roadmc::RoadTail tail{0, {0, 0, 0}};

@@ -98,15 +98,40 @@ size_t road_stage(const roadmc_road_species* r, std::vector<double>& in) {
   return off;
 }
 
+RoadScratch road_scratch(roadmc_ctx* ctx) {
+  auto& B = ctx->road_bufs;
+  return RoadScratch{static_cast<int*>(B.c.ptr), static_cast<double*>(B.d.ptr),
+                     static_cast<double*>(B.e.ptr), static_cast<int*>(B.g.ptr),
+                     static_cast<double*>(B.h.ptr)};
+}
+
+int road_reserve_lanes(roadmc_ctx* ctx, int P, int C) {
+  const size_t n = (size_t)P * P;
+  int rc;
+  if (!ctx->road_ready)
+    RMC_HIP(ctx, hipEventCreateWithFlags(&ctx->road_ready, hipEventDisableTiming));
+  for (auto& l : ctx->road_lanes) {
+    if (!l.stream) RMC_HIP(ctx, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+    if (!l.done) RMC_HIP(ctx, hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    if ((rc = rmc_buf_ensure(ctx, l.cross, n * sizeof(int32_t))) ||
+        (rc = rmc_buf_ensure(ctx, l.trans, n * sizeof(double))) ||
+        (rc = rmc_buf_ensure(ctx, l.surv, std::max<size_t>((size_t)C * n, 1) * sizeof(double))) ||
+        (rc = rmc_buf_ensure(ctx, l.blockmax, (size_t)road_cross_blocks(P) * sizeof(int))) ||
+        (rc = rmc_buf_ensure(ctx, l.part, road_part_elems(P) * sizeof(double))))
+      return rc;
+  }
+  return ROADMC_OK;
+}
+
 int road_launch_staged(roadmc_ctx* ctx, const roadmc_road_species* r, const double* d_in,
-                       bool want_trans, bool want_surv, RoadArgs* out) {
+                       bool want_trans, bool want_surv, const RoadScratch& sc, hipStream_t s,
+                       RoadArgs* out) {
   const int P = r->n_patches, C = r->n_controls, S = road_visible_segments(r);
   // The two distance cut-offs, as the reference forms them
   // (SpeciesRoadPatches.cpp:328-332, 387-395).
   const double lambda = r->lambda_mult * r->lambda_sd + r->lambda_mean;
   const double cut_cross = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * r->lambda_mult));
   const double cut_trans = std::log(0.05) / (-(r->lambda_mean + r->lambda_sd * lambda));
-  auto& B = ctx->road_bufs;
   RoadArgs a{};
   a.P = P;
   a.S = S;
@@ -123,13 +148,13 @@ int road_launch_staged(roadmc_ctx* ctx, const roadmc_road_species* r, const doub
   a.flow = d_in + up4(3 * (size_t)P) + 4 * (size_t)S;
   a.width_len = r->vehicle_width + r->animal_length;
   a.speed_3600 = r->animal_speed * 3600.0;
-  a.cross = static_cast<int*>(B.c.ptr);
-  a.trans = static_cast<double*>(B.d.ptr);
-  a.surv = static_cast<double*>(B.e.ptr);
-  a.blockmax = static_cast<int*>(B.g.ptr);
-  a.part = static_cast<double*>(B.h.ptr);
+  a.cross = sc.cross;
+  a.trans = sc.trans;
+  a.surv = sc.surv;
+  a.blockmax = sc.blockmax;
+  a.part = sc.part;
   a.tail = road_tail(ctx);
-  RMC_HIP(ctx, launch_road_build(a, want_trans, want_surv, ctx->stream));
+  RMC_HIP(ctx, launch_road_build(a, want_trans, want_surv, s));
   *out = a;
   return ROADMC_OK;
 }
@@ -147,7 +172,7 @@ int road_launch(roadmc_ctx* ctx, const roadmc_road_species* r, bool want_trans, 
                                 hipMemcpyHostToDevice, s));
   }
   return road_launch_staged(ctx, r, static_cast<const double*>(ctx->road_bufs.a.ptr), want_trans,
-                            want_surv, out);
+                            want_surv, road_scratch(ctx), s, out);
 }
 
 }  // namespace roadmc
