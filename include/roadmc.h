@@ -113,8 +113,8 @@ int roadmc_device_count(int* count);
 int roadmc_ctx_create(int device, roadmc_ctx** out);
 void roadmc_ctx_destroy(roadmc_ctx* ctx);
 const char* roadmc_last_error(const roadmc_ctx* ctx);
-/* Test hook: the next n device allocations of this context fail with
- * ROADMC_ERR_OOM. */
+/* Test hook: the next n device (or pinned host) allocations of this context
+ * fail with ROADMC_ERR_OOM. */
 int roadmc_debug_fail_allocs(roadmc_ctx* ctx, int n);
 
 /* ---- MTE ---------------------------------------------------------------- */

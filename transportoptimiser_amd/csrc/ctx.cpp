@@ -118,6 +118,7 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
     if (l.stream) (void)hipStreamDestroy(l.stream);
   }
   if (ctx->road_ready) (void)hipEventDestroy(ctx->road_ready);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->rov_graph.exec) (void)hipGraphExecDestroy(ctx->rov_graph.exec);
   rmc_buf_free(ctx->rov_graph.param);
   (void)hipStreamDestroy(ctx->stream);
@@ -200,5 +201,24 @@ extern "C" int roadmc_mte_penalty(int32_t n_species, const double* threshold,
       total += (threshold[s] - level) * initial_population[s] * cost_per_animal[s];
   }
   *penalty = total;
+  return ROADMC_OK;
+}
+
+int rmc_pinned_ensure(roadmc_ctx* ctx, size_t bytes) {
+  if (ctx->pinned_bytes >= bytes) return ROADMC_OK;
+  if (ctx->pinned) {
+    (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+  }
+  if (ctx->fail_allocs > 0) {
+    --ctx->fail_allocs;
+    return rmc_fail(ctx, ROADMC_ERR_OOM, "pinned host allocation failed (injected)");
+  }
+  if (hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault) != hipSuccess) {
+    ctx->pinned = nullptr;
+    return rmc_fail(ctx, ROADMC_ERR_OOM, "pinned host allocation failed");
+  }
+  ctx->pinned_bytes = bytes;
   return ROADMC_OK;
 }

@@ -53,6 +53,9 @@ struct roadmc_ctx {
     rmc_devbuf cross, trans, surv, blockmax, part;
   } road_lanes[kRoadLanes];
   hipEvent_t road_ready = nullptr;
+  // Pinned host staging of the packed operands (grown on demand, kept).
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
   struct {
     rmc_devbuf a, b, c, d;
   } knn_bufs;
@@ -65,6 +68,8 @@ struct roadmc_ctx {
 int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);
 int rmc_malloc(roadmc_ctx* ctx, void** p, size_t bytes);
 int rmc_buf_ensure(roadmc_ctx* ctx, rmc_devbuf& b, size_t bytes);
+// ctx->pinned holds at least bytes (hipHostMalloc); the old contents are lost.
+int rmc_pinned_ensure(roadmc_ctx* ctx, size_t bytes);
 int rmc_buf_alloc_exact(roadmc_ctx* ctx, rmc_devbuf& b, size_t bytes);
 void rmc_buf_free(rmc_devbuf& b);
 

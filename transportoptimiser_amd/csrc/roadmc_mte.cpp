@@ -253,17 +253,19 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
       (rc = grow(b.off, js.jobs.size() * sizeof(int64_t))) ||
       (rc = grow(b.counter, 256)))
     return rc;
-  // Host packing, the jobs split over threads by matrix size; every job
+  // Host packing into the context's pinned staging (so that the upload runs
+  // at DMA speed), the jobs split over threads by matrix size; every job
   // writes its own contiguous regions, which tile both arrays.
-  std::unique_ptr<T[]> mp(new (std::nothrow) T[std::max<size_t>(js.mpack_elems, 1)]);
-  std::unique_ptr<T[]> cols(new (std::nothrow) T[std::max<size_t>(js.cols_elems, 1)]);
-  if (!mp || !cols) return rmc_fail(ctx, ROADMC_ERR_OOM, "host staging allocation failed");
+  const size_t mp_bytes = (js.mpack_elems * es + 255) / 256 * 256;
+  if ((rc = rmc_pinned_ensure(ctx, mp_bytes + std::max<size_t>(js.cols_elems, 1) * es))) return rc;
+  T* const mp_h = static_cast<T*>(ctx->pinned);
+  T* const cols_h = reinterpret_cast<T*>(static_cast<char*>(ctx->pinned) + mp_bytes);
   const size_t nj = js.jobs.size();
   auto pack_range = [&](size_t j0, size_t j1) {
     for (size_t j = j0; j < j1; ++j) {
       const roadmc_mte_problem& pr = roads[js.road_of[j]];
       pack_species<T>(pr.species[js.species_of[j]], js.jobs[j].Pp, js.jobs[j].Pk, pr.drop_tol,
-                      js.fp64, mp.get() + js.mpack_off[j], cols.get() + js.cols_off[j]);
+                      js.fp64, mp_h + js.mpack_off[j], cols_h + js.cols_off[j]);
     }
   };
   const int nth = (int)std::min<size_t>(
@@ -296,9 +298,9 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
                       : nullptr;
   }
   hipStream_t s = ctx->stream;
-  RMC_HIP(ctx, hipMemcpyAsync(b.mpack.ptr, mp.get(), js.mpack_elems * es,
+  RMC_HIP(ctx, hipMemcpyAsync(b.mpack.ptr, mp_h, js.mpack_elems * es,
                               hipMemcpyHostToDevice, s));
-  RMC_HIP(ctx, hipMemcpyAsync(b.cols.ptr, cols.get(), js.cols_elems * es,
+  RMC_HIP(ctx, hipMemcpyAsync(b.cols.ptr, cols_h, js.cols_elems * es,
                               hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(b.jobs.ptr, js.jobs.data(), js.jobs.size() * sizeof(MteJob),
                               hipMemcpyHostToDevice, s));
