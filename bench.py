@@ -3,7 +3,9 @@
 
 Workload (one "step"): a generation of 512 candidate roads, patch counts
 log-uniform in [64, 1024], 2 species, 50 years, 20,000 Monte Carlo paths per
-road, FP32 arithmetic (FP32-accumulating MFMA), synthetic landscapes.  The
+road, synthetic landscapes.  The products run as split-bf16 on the bf16
+matrix cores (x = hi + lo, x.m = hi.m_hi + hi.m_lo + lo.m_hi, FP32
+accumulation; --precision fp32 selects FP32 MFMA products instead).  The
 batched path evaluates the whole generation in one replay of a captured HIP
 graph around a persistent kernel; inputs are resident in HBM when the timed
 region starts.
@@ -13,9 +15,12 @@ Reported beside the headline value:
            allocation, random streams pre-generated into HBM, one thread per
            path over CSC), timed on a sample of roads and extrapolated by P^2;
   single   the tiled kernel called once per road through roadmc_mte_run;
-  accuracy max |dMTE| of the FP32 generation against the FP64 kernels;
-  roofline FP32 MFMA flops 2 N P^2 T S of the persistent kernel over its
-           event-timed duration;
+  accuracy max |dMTE| of the measured generation against the FP64 kernels;
+  roofline algorithmic flops 2 N P^2 T S of the persistent kernel over its
+           event-timed duration, against the peak of the matrix cores used
+           (bf16 dense 2.5 PFLOP/s; each algorithmic product is three bf16
+           products; FP32 157.3 TFLOP/s with --precision fp32);
+  fp32_products the same generation with FP32 MFMA products;
   cpu_baseline the CPU oracle (FP64, OpenMP over paths) on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -36,6 +41,7 @@ sys.path.insert(0, REPO)
 
 METRIC = "road MTE evals/sec, batched vs plain GPU, 1 MI355X at |ΔMTE|≤2e-3"
 FP32_MFMA_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 
 
 def log(msg: str) -> None:
@@ -63,16 +69,19 @@ def fit_p2(sizes, times):
     return a, b
 
 
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "mte_persistent_generation512_traffic.txt")
+TRAFFIC_FILES = {
+    "fp32": os.path.join(REPO, "profiles", "mte_persistent_generation512_traffic.txt"),
+    "bf16x3": os.path.join(REPO, "profiles", "mte_persistent_bf16x3_generation512_traffic.txt"),
+}
 
 
-def read_traffic(kernel_ms: float):
+def read_traffic(precision: str):
     """HBM bytes per launch of the persistent kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a calibrated copy);
     None when that summary is absent."""
     try:
         vals = {}
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[precision]) as f:
             for line in f:
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
@@ -248,6 +257,10 @@ def main() -> int:
     ap.add_argument("--years", type=int, default=50)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--autotune", action="store_true", help="tune tile rows first")
+    ap.add_argument("--precision", choices=["bf16x3", "fp32"], default="bf16x3",
+                    help="products of the measured generation")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-products comparison generation")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -281,6 +294,9 @@ def main() -> int:
         log(f"autotuned tile rows: {ctx.autotune(n_paths=args.paths, n_years=5)}")
     tiles = ctx.tiles()
 
+    prec = nat.PREC_BF16X3 if args.precision == "bf16x3" else nat.PREC_FP32
+    for r in roads:
+        r.precision = prec
     t = time.perf_counter()
     batch = ctx.prepare_batch(roads)
     prep_ms = (time.perf_counter() - t) * 1e3
@@ -311,7 +327,7 @@ def main() -> int:
             r.precision = nat.PREC_FP64
         out64 = ctx.simulate_mte_batch(roads, end_pops=False, patch_mean=False)
         for r in roads:
-            r.precision = nat.PREC_FP32
+            r.precision = prec
         d = np.array([np.max(np.abs(a.mte_prob - b.mte_prob)) for a, b in zip(out32, out64)])
         dm = np.array([np.max(np.abs(a.end_mean - b.end_mean) / b.end_mean)
                        for a, b in zip(out32, out64)])
@@ -337,7 +353,8 @@ def main() -> int:
         ts, ps = [], []
         for i in idx:
             t = time.perf_counter()
-            ctx.simulate_mte(roads[i], plain=True, end_pops=False, patch_mean=False)
+            ctx.simulate_mte(dataclasses.replace(roads[i], precision=nat.PREC_FP32), plain=True,
+                             end_pops=False, patch_mean=False)
             ts.append(time.perf_counter() - t)
             ps.append(sizes[i])
             log(f"plain road P={sizes[i]}: {ts[-1]:.2f}s")
@@ -386,10 +403,33 @@ def main() -> int:
         cpu["config1_road_ms"] = round((time.perf_counter() - t) * 1e3, 1)
         log(f"cpu baseline: {cpu}")
 
+    fp32 = None
+    if not args.no_fp32 and prec != nat.PREC_FP32:
+        r32 = [dataclasses.replace(r, precision=nat.PREC_FP32) for r in roads]
+        with ctx.prepare_batch(r32) as b32:
+            b32.launch()
+            b32.sync()
+            b32.kernel_ms()
+            t = time.perf_counter()
+            for _ in range(2):
+                b32.launch()
+            b32.sync()
+            ms32 = (time.perf_counter() - t) * 1e3 / 2
+            k32 = b32.kernel_ms()
+            o32 = b32.fetch(end_pops=False, patch_mean=False)
+        d32 = max(float(np.max(np.abs(a.mte_prob - c.mte_prob))) for a, c in zip(out32, o32))
+        fp32 = {"ms_per_generation": round(ms32, 1), "evals_per_s": round(len(roads) / ms32 * 1e3, 2),
+                "tflops": round(flops / k32 / 1e9, 1),
+                "frac_fp32_mfma": round(flops / k32 / 1e9 / FP32_MFMA_PEAK_TFLOPS, 4),
+                "max_abs_dMTE_vs_measured": d32,
+                "what": "the same generation with FP32 MFMA products (v_mfma_f32_32x32x2_f32)"}
+        log(f"fp32 products: {fp32}")
+
     extras = None if args.no_extras else run_extras(ctx, roads, prep_ms)
 
     achieved = flops / (kernel_ms / 1e3) / 1e12
-    traffic = read_traffic(kernel_ms)
+    traffic = read_traffic(args.precision)
+    peak = BF16_MFMA_PEAK_TFLOPS if prec == nat.PREC_BF16X3 else FP32_MFMA_PEAK_TFLOPS
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -401,7 +441,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "bf16x3" if prec == nat.PREC_BF16X3 else "fp32",
         "data": "synthetic",
         "config": {"workload": f"GA generation: {len(roads)} roads, P log-uniform [64,1024], "
                                f"{len(roads[0].species)} species, {args.years} years, "
@@ -409,15 +449,18 @@ def main() -> int:
                    "n_roads": len(roads), "paths": args.paths, "years": args.years,
                    "species": len(roads[0].species), "tile_rows": tiles},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
-                     "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "mte_persistent_kernel<TraitsF32>",
+                     "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": "mte_persistent_kernel<%s>" % (
+                         "TraitsB3" if prec == nat.PREC_BF16X3 else "TraitsF32"),
                      "kernel_ms": round(kernel_ms, 3),
-                     "flops_per_launch": flops},
+                     "flops_per_launch": flops,
+                     "mfma_products_per_flop": 3 if prec == nat.PREC_BF16X3 else 1},
         "cpu_baseline": cpu,
         "plain": plain,
         "single": single,
         "accuracy": accuracy,
+        "fp32_products": fp32,
         "extras": extras,
     }
     print(json.dumps(line), flush=True)

@@ -41,6 +41,10 @@ extern "C" {
 
 #define ROADMC_PREC_FP32 0
 #define ROADMC_PREC_FP64 1
+/* Tiled MTE kernels only: FP32 state and accumulation, products on bf16
+ * matrix cores as hi.m_hi + hi.m_lo + lo.m_hi of the two-term bf16 splits
+ * x = hi + lo (about 17 significant bits of state and movement matrix). */
+#define ROADMC_PREC_BF16X3 2
 
 /* Largest patch count the tiled kernels take (the population tile of one
  * workgroup must fit in LDS). */

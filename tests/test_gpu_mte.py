@@ -1,7 +1,8 @@
 """MTE on the GPU against the CPU oracle (runs on an MI355X).
 
 Bars: FP32 kernels give per-path end populations within 1e-4 relative of the
-FP64 oracle on the same Philox streams; FP64 kernels within 1e-10; the MTE
+FP64 oracle on the same Philox streams (split-bf16 products, PREC_BF16X3,
+too); FP64 kernels within 1e-10; the MTE
 probability may differ only through paths that sit within that tolerance of
 the threshold.  Batched and single-road launches are bit-identical.
 """
@@ -66,6 +67,54 @@ def test_tiled_fp64_matches_oracle(ctx, oracle, P, N, T, S, law, endpoint, tol):
     out = ctx.simulate_mte(pr)
     ref = oracle.mte_problem(pr)
     _check_against_oracle(out, ref, pr, rtol=1e-10)
+
+
+# Split-bf16 products: the state carries about 17 significant bits and a
+# product drops only lo.m_lo; per-path end populations stay within 1e-4
+# relative of the FP64 oracle, the FP32 kernels' bar.
+B3_RTOL = 1e-4
+
+
+@pytest.mark.parametrize("P,N,T,S,law,endpoint,tol", CASES)
+def test_tiled_bf16x3_matches_oracle(ctx, oracle, P, N, T, S, law, endpoint, tol):
+    pr = _problem(P, N, T, S, law, endpoint, drop_tol=tol, prec=nat.PREC_BF16X3)
+    print(f"bf16x3 case P={P} N={N} T={T} S={S} law={law} end={endpoint}", flush=True)
+    out = ctx.simulate_mte(pr)
+    ref = oracle.mte_problem(pr)
+    err = np.max(np.abs(out.end_pops - ref["end_pops"]) / np.abs(ref["end_pops"]))
+    print(f"  max rel err of end populations {err:.2e}", flush=True)
+    _check_against_oracle(out, ref, pr, rtol=B3_RTOL)
+
+
+def test_batch_bf16x3_generation(ctx):
+    # Against the FP64 kernels on a generation's shapes, and batched launches
+    # bit-identical to single-road runs.
+    Ps = [64, 96, 128, 256, 512, 700, 1000, 1024]
+    r3 = [_problem(P, 2000, 20, 2, seed=8, road_id=r, prec=nat.PREC_BF16X3)
+          for r, P in enumerate(Ps)]
+    r64 = [_problem(P, 2000, 20, 2, seed=8, road_id=r, prec=nat.PREC_FP64)
+           for r, P in enumerate(Ps)]
+    a = ctx.simulate_mte_batch(r3, end_pops=True)
+    b = ctx.simulate_mte_batch(r64, end_pops=True)
+    for x, y in zip(a, b):
+        print(f"  dMTE {np.max(np.abs(x.mte_prob - y.mte_prob)):.1e} "
+              f"rel {np.max(np.abs(x.end_pops - y.end_pops) / y.end_pops):.1e}", flush=True)
+        assert np.max(np.abs(x.mte_prob - y.mte_prob)) <= 2e-3
+        np.testing.assert_allclose(x.end_pops, y.end_pops, rtol=B3_RTOL)
+    for r, x in zip(r3[:3], a[:3]):
+        s = ctx.simulate_mte(r, end_pops=True)
+        assert np.array_equal(s.end_pops, x.end_pops)
+
+
+def test_bf16x3_is_for_the_tiled_kernels_only(ctx):
+    from transportoptimiser_amd import adjoint
+    pr = _problem(40, 100, 5, 1, prec=nat.PREC_BF16X3)
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte(pr, plain=True)
+    assert e.value.code == nat.ERR_UNSUPPORTED
+    with pytest.raises(nat.RoadmcError) as e:
+        adjoint.Adjoint(ctx, pr, checkpoint_every=2)
+    assert e.value.code == nat.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("prec,rtol", [(nat.PREC_FP32, 1e-4), (nat.PREC_FP64, 1e-10)])

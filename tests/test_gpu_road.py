@@ -169,7 +169,7 @@ def test_road_to_mte_end_to_end(ctx, oracle):
     np.testing.assert_allclose(out.end_pops, ref["end_pops"], rtol=1e-9, atol=0)
 
 
-@pytest.mark.parametrize("precision", [nat.PREC_FP32, nat.PREC_FP64])
+@pytest.mark.parametrize("precision", [nat.PREC_FP32, nat.PREC_FP64, nat.PREC_BF16X3])
 def test_batch_builds_matrices_on_device(ctx, precision):
     """Species that carry their road (trans None) get trans o surv built and
     packed on the device inside the batch; the results equal, bit for bit,

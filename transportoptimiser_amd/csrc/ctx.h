@@ -24,7 +24,7 @@ struct roadmc_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int fail_allocs = 0;  // test hook: next n allocations fail
-  size_t lds_limit_f32 = 0, lds_limit_f64 = 0;
+  size_t lds_limit_f32 = 0, lds_limit_f64 = 0, lds_limit_b3 = 0;
   int tile_rows[4] = {256, 128, 64, 32};  // FP32 rows per bucket P<=128..1024
   rmc_job_buffers mte_bufs;
   struct {

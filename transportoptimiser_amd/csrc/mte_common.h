@@ -30,6 +30,7 @@ struct TraitsF32 {
   using AccT = floatx16;
   static constexpr int TM = 32, TN = 32, KV = 4, NACC = 16, NT = 4, PAD = 4;
   static constexpr bool kQuadItems = true;  // items 4q..4q+3 are 4 paths
+  static constexpr bool kSplit = false;
   __device__ static AccT mfma(float a, float b, AccT c) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
   }
@@ -59,6 +60,7 @@ struct TraitsF64 {
   using AccT = doublex4;
   static constexpr int TM = 16, TN = 16, KV = 2, NACC = 4, NT = 8, PAD = 2;
   static constexpr bool kQuadItems = false;
+  static constexpr bool kSplit = false;
   __device__ static AccT mfma(double a, double b, AccT c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -75,6 +77,51 @@ struct TraitsF64 {
                                  uint32_t tag, uint32_t k0, uint32_t k1,
                                  double z[4]) {
     rmc_normal4_d(q, y, p, tag, k0, k1, z);
+  }
+};
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Split-bf16 products on v_mfma_f32_32x32x16_bf16 (FP32 accumulate).  A value
+// x is held as the pair hi = bf16(x), lo = bf16(x - hi), which carries about
+// 17 significant bits; a product x.m is hi.m_hi + hi.m_lo + lo.m_hi (the
+// dropped lo.m_lo is below 2^-16 of it).  The population tile lives in LDS as
+// two bf16 arrays (hi, then lo), k in natural order; a lane's operand is the
+// 8 consecutive k of its row and lane half: A lane l = A[l%32][8(l/32) + i],
+// B lane l = B[8(l/32) + i][l%32], i = 0..7 (one 16-byte vector).  The
+// accumulator layout is the one of v_mfma_f32_32x32x2_f32.
+struct TraitsB3 {
+  using T = float;
+  using VecT = floatx4;  // 16 bytes: 8 bf16 operands
+  using AccT = floatx16;
+  // Each bf16 row has Pp + PAD slots (16 bytes of padding), so the hi and lo
+  // arrays together take BM (Pp + PAD) 4-byte units, as lds_bytes_t counts.
+  static constexpr int TM = 32, TN = 32, KV = 1, NACC = 16, NT = 4, PAD = 8;
+  static constexpr bool kQuadItems = true;
+  static constexpr bool kSplit = true;
+  __device__ static AccT mfma(floatx4 a, floatx4 b, AccT c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  }
+  __device__ static int acc_row(int lane, int r) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+  }
+  __device__ static int acc_col(int lane) { return lane & 31; }
+  __device__ static int a_row(int lane) { return lane & 31; }
+  __device__ static int k_lane(int lane) { return lane >> 5; }
+  __host__ __device__ static int perm(int k) { return k; }
+  __device__ static float expo(float x) { return __expf(x); }
+  __device__ static void normal4(uint32_t q, uint32_t y, uint32_t p,
+                                 uint32_t tag, uint32_t k0, uint32_t k1,
+                                 float z[4]) {
+    rmc_normal4_f(q, y, p, tag, k0, k1, z);
+  }
+  // The split pair of x, hi in the low half.
+  __device__ static uint32_t split(float x) {
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
+    return (uint32_t)__builtin_bit_cast(uint16_t, h) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
   }
 };
 
@@ -282,6 +329,124 @@ __device__ __forceinline__ void range_product(typename TR::AccT& acc, const Pack
 #pragma unroll
     for (int s = 0; s < KV; ++s) acc = TR::mfma(a0[s], b0[s], acc);
   }
+}
+
+// Split-bf16 form of year_product (TraitsB3).  A k-chunk is 16 k: the A
+// operands are 8 bf16 of the hi and of the lo array (Ah and Ah + lo_off), the
+// B operands packed k-groups 2c (hi) and 2c + 1 (lo).  Each chunk issues the
+// three products of every tile, one product kind at a time across the tiles,
+// so that consecutive MFMAs are independent.  Stages ping-pong over the chunks
+// (the chunk count is even).  Unlike year_product, no operands stay in
+// registers between years: the epilogue needs them.
+template <int NTILE, bool SHARED_A = false>
+__device__ __forceinline__ void year_product_b3(floatx16* acc, const PackedB& B,
+                                                const uint16_t* Ah,
+                                                int lo_off, const int* trow, const int* tcol,
+                                                int LDA, int arow, int akoff, int lane) {
+  constexpr int NA = SHARED_A ? 1 : NTILE;
+  const int nch = B.nkg >> 1;
+  uint32_t boff[NTILE];
+  const uint16_t* ap[NA];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) boff[t] = b_voff(B, tcol[t], lane);
+#pragma unroll
+  for (int t = 0; t < NA; ++t) ap[t] = Ah + (trow[t] * 32 + arow) * LDA + akoff;
+  auto ldb = [&](int t, int kg) -> floatx4 { return load_b<floatx4>(B, boff[t], kg); };
+  auto lda = [&](int t, int off) -> floatx4 {
+    return *reinterpret_cast<const floatx4*>(ap[t] + off);
+  };
+  auto ia = [](int t) { return SHARED_A ? 0 : t; };
+  floatx4 bh0[NTILE], bl0[NTILE], bh1[NTILE], bl1[NTILE], ah0[NA], al0[NA], ah1[NA], al1[NA];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) {
+    bh0[t] = ldb(t, 0);
+    bl0[t] = ldb(t, 1);
+  }
+#pragma unroll
+  for (int t = 0; t < NA; ++t) {
+    ah0[t] = lda(t, 0);
+    al0[t] = lda(t, lo_off);
+  }
+  auto products = [&](const floatx4* ah, const floatx4* al, const floatx4* bh,
+                      const floatx4* bl, auto first) {
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      if constexpr (decltype(first)::value)
+        acc[t] = TraitsB3::mfma(ah[ia(t)], bh[t], floatx16{});
+      else
+        acc[t] = TraitsB3::mfma(ah[ia(t)], bh[t], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) acc[t] = TraitsB3::mfma(ah[ia(t)], bl[t], acc[t]);
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) acc[t] = TraitsB3::mfma(al[ia(t)], bh[t], acc[t]);
+  };
+  auto step = [&](const int c, auto first) {
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      bh1[t] = ldb(t, 2 * c + 2);
+      bl1[t] = ldb(t, 2 * c + 3);
+    }
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      ah1[t] = lda(t, 16);
+      al1[t] = lda(t, lo_off + 16);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    products(ah0, al0, bh0, bl0, first);
+    // The last step's stage-0 loads read chunk 0 again (B) and past the
+    // row's last chunk (A: inside the LDS allocation); both are never used.
+    const int c2 = (c + 2 < nch) ? c + 2 : 0;
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      bh0[t] = ldb(t, 2 * c2);
+      bl0[t] = ldb(t, 2 * c2 + 1);
+    }
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      ah0[t] = lda(t, 32);
+      al0[t] = lda(t, lo_off + 32);
+      ap[t] += 32;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    products(ah1, al1, bh1, bl1, std::false_type{});
+  };
+  step(0, std::true_type{});
+  for (int c = 2; c < nch; c += 2) step(c, std::false_type{});
+}
+
+// acc += the split-bf16 product over k-chunks c0 .. c1 (c1 > c0) of one tile,
+// with range_product's ping-pong and odd tail.
+__device__ __forceinline__ void range_product_b3(floatx16& acc, const PackedB& B,
+                                                 const uint16_t* Ah, int lo_off, int trow,
+                                                 int tcol, int c0, int c1, int LDA, int arow,
+                                                 int akoff, int lane) {
+  const uint32_t boff = b_voff(B, tcol, lane);
+  const uint16_t* ap = Ah + (trow * 32 + arow) * LDA + akoff;
+  auto lda = [&](int off) -> floatx4 { return *reinterpret_cast<const floatx4*>(ap + off); };
+  floatx4 bh0 = load_b<floatx4>(B, boff, 2 * c0), bl0 = load_b<floatx4>(B, boff, 2 * c0 + 1);
+  floatx4 ah0 = lda(16 * c0), al0 = lda(lo_off + 16 * c0);
+  auto products = [&](floatx4 ah, floatx4 al, floatx4 bh, floatx4 bl) {
+    acc = TraitsB3::mfma(ah, bh, acc);
+    acc = TraitsB3::mfma(ah, bl, acc);
+    acc = TraitsB3::mfma(al, bh, acc);
+  };
+  int c = c0;
+  for (; c + 1 < c1; c += 2) {
+    const floatx4 bh1 = load_b<floatx4>(B, boff, 2 * c + 2);
+    const floatx4 bl1 = load_b<floatx4>(B, boff, 2 * c + 3);
+    const floatx4 ah1 = lda(16 * (c + 1)), al1 = lda(lo_off + 16 * (c + 1));
+    __builtin_amdgcn_sched_barrier(0);
+    products(ah0, al0, bh0, bl0);
+    const int k2 = min(c + 2, c1 - 1);
+    bh0 = load_b<floatx4>(B, boff, 2 * k2);
+    bl0 = load_b<floatx4>(B, boff, 2 * k2 + 1);
+    ah0 = lda(16 * k2);
+    al0 = lda(lo_off + 16 * k2);
+    __builtin_amdgcn_sched_barrier(0);
+    products(ah1, al1, bh1, bl1);
+  }
+  if (c < c1) products(ah0, al0, bh0, bl0);
 }
 
 }  // namespace roadmc

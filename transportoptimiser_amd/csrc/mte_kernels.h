@@ -7,6 +7,11 @@
 
 namespace roadmc {
 
+// Arithmetic of a launch (the values of ROADMC_PREC_*): FP32, FP64, or FP32
+// accumulation of split-bf16 products (state and movement matrix held as
+// bf16 hi + lo pairs).
+enum { kPrecF32 = 0, kPrecF64 = 1, kPrecB3 = 2 };
+
 // One (road, species) pair of a launch.  Device arrays are of the launch's
 // element type (float for FP32, double for FP64).
 struct MteJob {
@@ -22,7 +27,8 @@ struct MteJob {
   double dt;           // time step
   double thr_total;    // threshold_ratio * initial total
   int32_t P, Pp, N, T, BM, law, endpoint, n_tiles;
-  int32_t Pk;          // contraction extent: P rounded up to 16 (<= Pp)
+  int32_t Pk;          // contraction extent: P rounded up to 16 (<= Pp); Pp
+                       // in the split-bf16 form (an even count of 16-k chunks)
   int32_t split;       // split the ntiles % 4 remainder tiles into k-quarters
   uint32_t tag0, tag1, key0, key1;
 };
@@ -38,7 +44,7 @@ constexpr int kMteRowsF64 = 16;
 
 // LDS bytes of one workgroup; n_rem_split is the number of remainder tiles
 // split into k-quarters (0 when the job does not split).
-size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split);
+size_t mte_lds_bytes(int prec, int BM, int Pp, int n_rem_split);
 // Packed index of movement entry (k = origin, j = destination) in a matrix
 // packed with Kp / 8 k-groups per column block (Kp: the contraction extent):
 // element (k, j) lands where lane l of the wave handling column tile j / TN
@@ -56,13 +62,16 @@ __host__ __device__ inline size_t mte_pack_index(bool fp64, int Kp, int k, int j
   return (((size_t)cb * nkg + kg) * 64 + lane) * 2 + s;
 }
 
-hipError_t mte_set_lds_limit(bool fp64, size_t bytes);
-hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
+hipError_t mte_set_lds_limit(int prec, size_t bytes);
+hipError_t launch_mte_direct(int prec, const MteJob* jobs, const int2* work,
                              int n_work, size_t lds, hipStream_t s);
+// Converts FP32-packed matrices (n_floats, a multiple of 512, each matrix
+// with an even k-group count) in place to the split-bf16 operand order.
+hipError_t launch_mte_split_b3(float* m, size_t n_floats, hipStream_t s);
 // Work queues: items work[qbeg[q] .. qbeg[q+1]) form queue q; counters[q]
 // must be zero at launch.
 constexpr int kMteQueues = 8;  // one per XCD of an MI355X
-hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
+hipError_t launch_mte_persistent(int prec, const MteJob* jobs,
                                  const int2* work, const int* qbeg, int nq,
                                  int* counters, int grid, size_t lds,
                                  hipStream_t s);

@@ -34,9 +34,12 @@
 
 namespace roadmc {
 
-size_t mte_lds_bytes(bool fp64, int BM, int Pp, int n_rem_split) {
-  return fp64 ? lds_bytes_t<TraitsF64>(BM, Pp, n_rem_split)
-              : lds_bytes_t<TraitsF32>(BM, Pp, n_rem_split);
+size_t mte_lds_bytes(int prec, int BM, int Pp, int n_rem_split) {
+  switch (prec) {
+    case kPrecF64: return lds_bytes_t<TraitsF64>(BM, Pp, n_rem_split);
+    case kPrecB3: return lds_bytes_t<TraitsB3>(BM, Pp, n_rem_split);
+    default: return lds_bytes_t<TraitsF32>(BM, Pp, n_rem_split);
+  }
 }
 
 
@@ -60,6 +63,10 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   const int p0 = tile * BM;
 
   T* A = reinterpret_cast<T*>(smem);
+  // Split-bf16 state (TraitsB3): the hi array, then the lo array, LDA bf16
+  // slots per row.
+  uint16_t* const Ah = reinterpret_cast<uint16_t*>(smem);
+  const int lo_off = BM * LDA;
   T* grm = reinterpret_cast<T*>(smem + align16((size_t)BM * LDA * sizeof(T)));
   double* red = reinterpret_cast<double*>(
       smem + align16((size_t)BM * LDA * sizeof(T)) +
@@ -80,7 +87,13 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
 
   for (int idx = tid; idx < BM * Pp; idx += nthreads) {
     const int r = idx / Pp, c = idx - r * Pp;
-    A[r * LDA + TR::perm(c)] = init[c];
+    if constexpr (TR::kSplit) {
+      const uint32_t hl = TR::split(init[c]);
+      Ah[r * LDA + c] = (uint16_t)hl;
+      Ah[lo_off + r * LDA + c] = (uint16_t)(hl >> 16);
+    } else {
+      A[r * LDA + TR::perm(c)] = init[c];
+    }
   }
   for (int r = tid; r < BM; r += nthreads) grm[r] = (T)jb.params[0];
   __syncthreads();
@@ -125,7 +138,9 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   // Pk / 8 k-groups); output columns run over Pp = P rounded up to 32.
   const int nkg = jb.Pk >> 3;
   const PackedB Bm = packed_b(jb.mpack, nkg, (size_t)Pp * jb.Pk * sizeof(T));
-  const int arow = TR::a_row(lane), akoff = KV * TR::k_lane(lane);
+  // A operand offset of the lane half: KV floats (8 bf16 in the split form).
+  const int arow = TR::a_row(lane), akoff = (TR::kSplit ? 8 : KV) * TR::k_lane(lane);
+  const int nch = nkg >> 1;  // split form: 16-k chunks
 
   // All of this wave's tiles in one row block (contiguous runs usually are):
   // they can share one A fragment per k-group.
@@ -133,9 +148,11 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
 
   AccT acc[NT];
   VecT b0[NT];
+  if constexpr (!TR::kSplit) {
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
-    if (t < my_n) b0[t] = load_b<VecT>(Bm, b_voff(Bm, tcol[t], lane), 0);
+    for (int t = 0; t < NT; ++t)
+      if (t < my_n) b0[t] = load_b<VecT>(Bm, b_voff(Bm, tcol[t], lane), 0);
+  }
   for (int year = 0; year < jb.T; ++year) {
     // Opaque per-year copies of loop-invariant scalars.  Addresses derived
     // from them (the 64 LDS write addresses and 16 Philox counters of a lane,
@@ -145,7 +162,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     int LDAy = LDA, p0y = p0;
     const T* colsy = cols;
     asm volatile("" : "+s"(LDAy), "+s"(p0y), "+s"(colsy));
-    if (jb.ckpt && year % jb.ckpt_every == 0) {
+    if (!TR::kSplit && jb.ckpt && year % jb.ckpt_every == 0) {
       // Adjoint checkpoint: the state entering this year, natural column order.
       T* ck = static_cast<T*>(jb.ckpt) + ((size_t)(year / jb.ckpt_every) * jb.N) * Pp;
       for (int idx = tid; idx < BM * Pp; idx += nthreads) {
@@ -153,40 +170,74 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
         if (p0 + r < jb.N) ck[(size_t)(p0 + r) * Pp + c] = A[r * LDA + TR::perm(c)];
       }
     }
-    // k-quarter pieces of the remainder tiles first (their partial products
-    // go to LDS): the wave with pieces then overlaps its whole tiles with the
-    // other wave on its SIMD instead of finishing alone on a one-tile chain.
-    for (int i = 0; i < my_np; ++i) {
-      const int ti = 4 * q4 + i;
-      const int tr = ti / ntc, tc = ti - (ti / ntc) * ntc;
-      AccT pacc = AccT{};
-      range_product<TR>(pacc, Bm, A, tr, tc, (simd * nkg) >> 2, ((simd + 1) * nkg) >> 2,
-                        LDAy, arow, akoff, lane);
-      T* slot = qpart + (size_t)((i * 4 + simd) * NACC) * 64 + lane;
+    if constexpr (TR::kSplit) {
+      // The same schedule in the split form: quarters of the 16-k chunks,
+      // then the whole tiles.
+      for (int i = 0; i < my_np; ++i) {
+        const int ti = 4 * q4 + i;
+        const int tr = ti / ntc, tc = ti - (ti / ntc) * ntc;
+        AccT pacc = AccT{};
+        range_product_b3(pacc, Bm, Ah, lo_off, tr, tc, (simd * nch) >> 2,
+                         ((simd + 1) * nch) >> 2, LDAy, arow, akoff, lane);
+        T* slot = qpart + (size_t)((i * 4 + simd) * NACC) * 64 + lane;
 #pragma unroll
-      for (int r = 0; r < NACC; ++r) slot[r * 64] = pacc[r];
-    }
-    // One branch per year on the (wave-uniform) tile count and row sharing;
-    // the product loops themselves are straight-line code.
-    switch (my_n * 2 + (shared_a ? 1 : 0)) {
-      case 2: case 3: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 4: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 5: year_product<TR, 2, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 6: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 7: year_product<TR, 3, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 8: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      case 9: year_product<TR, 4, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-      default:
-        if constexpr (NT > 4) {
-          switch (my_n) {
-            case 5: year_product<TR, 5>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-            case 6: year_product<TR, 6>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-            case 7: year_product<TR, 7>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-            case 8: year_product<TR, 8>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-            default: break;
+        for (int r = 0; r < NACC; ++r) slot[r * 64] = pacc[r];
+      }
+#define RMC_YP3(n, sh, i) \
+  year_product_b3<n, sh>(acc + (i), Bm, Ah, lo_off, trow + (i), tcol + (i), LDAy, arow, akoff, lane)
+      // Three or four tiles over two row blocks would need their A operands
+      // in registers at once (spills): they run as passes of one or two.
+      auto pair = [&](const int i) {
+        if (trow[i] == trow[i + 1]) RMC_YP3(2, true, i);
+        else RMC_YP3(2, false, i);
+      };
+      switch (my_n * 2 + (shared_a ? 1 : 0)) {
+        case 2: case 3: RMC_YP3(1, false, 0); break;
+        case 4: RMC_YP3(2, false, 0); break;
+        case 5: RMC_YP3(2, true, 0); break;
+        case 6: pair(0); RMC_YP3(1, false, 2); break;
+        case 7: RMC_YP3(3, true, 0); break;
+        case 8: pair(0); pair(2); break;
+        case 9: RMC_YP3(4, true, 0); break;
+        default: break;
+      }
+#undef RMC_YP3
+    } else {
+      // k-quarter pieces of the remainder tiles first (their partial products
+      // go to LDS): the wave with pieces then overlaps its whole tiles with the
+      // other wave on its SIMD instead of finishing alone on a one-tile chain.
+      for (int i = 0; i < my_np; ++i) {
+        const int ti = 4 * q4 + i;
+        const int tr = ti / ntc, tc = ti - (ti / ntc) * ntc;
+        AccT pacc = AccT{};
+        range_product<TR>(pacc, Bm, A, tr, tc, (simd * nkg) >> 2, ((simd + 1) * nkg) >> 2,
+                          LDAy, arow, akoff, lane);
+        T* slot = qpart + (size_t)((i * 4 + simd) * NACC) * 64 + lane;
+  #pragma unroll
+        for (int r = 0; r < NACC; ++r) slot[r * 64] = pacc[r];
+      }
+      // One branch per year on the (wave-uniform) tile count and row sharing;
+      // the product loops themselves are straight-line code.
+      switch (my_n * 2 + (shared_a ? 1 : 0)) {
+        case 2: case 3: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 4: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 5: year_product<TR, 2, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 6: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 7: year_product<TR, 3, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 8: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        case 9: year_product<TR, 4, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+        default:
+          if constexpr (NT > 4) {
+            switch (my_n) {
+              case 5: year_product<TR, 5>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+              case 6: year_product<TR, 6>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+              case 7: year_product<TR, 7>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+              case 8: year_product<TR, 8>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+              default: break;
+            }
           }
-        }
-        break;
+          break;
+      }
     }
     __syncthreads();  // every wave has finished reading this year's state
 
@@ -219,6 +270,15 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     const T sig_y = (kFastLn && keep_pre) ? (T)0 : sig;
     const T hv_y = (kFastLn && keep_pre) ? (T)0 : half_var;
     const T rscale = (kFastLn && keep_pre) ? (T)0 : (T)1;
+    auto store = [&](const int row, const int pc, const T v) {
+      if constexpr (TR::kSplit) {
+        const uint32_t hl = TR::split(v);
+        Ah[row * LDAe + pc] = (uint16_t)hl;
+        Ah[lo_off + row * LDAe + pc] = (uint16_t)(hl >> 16);
+      } else {
+        A[row * LDAe + pc] = v;
+      }
+    };
     // Noise, growth and write-back of one output tile.
     auto epilogue = [&](const AccT& a, const int tr, const int tc) {
       const int col = tc * TN + TR::acc_col(lanee);
@@ -248,13 +308,13 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
           const T x = a[r4 + s];
           if constexpr (kFastLn) {
             const T e = __builtin_amdgcn_exp2f(fmaf(sig_y, zz, -hv_y));
-            A[row * LDAe + pcol] = x * fmaf(-gb, x, ga) * e;
+            store(row, pcol, x * fmaf(-gb, x, ga) * e);
           } else {
             T g = (T)0;
             if constexpr (LAW == kLawRefGpu) g = grm[row];
             const T n = grow<T, LAW>(x, zz, K, Ki, rr, sig, half_var, dt, mu,
                                      sd, g, lv);
-            A[row * LDAe + pcol] = keep_pre ? x : n;
+            store(row, pcol, keep_pre ? x : n);
           }
         }
       }
@@ -277,12 +337,22 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   }
 
   // Tallies in FP64: per-path totals, exceedance count, sum, sum of squares.
+  // State entry (r, stored column c), hi + lo in the split form.
+  auto state_at = [&](const int r, const int c) -> double {
+    if constexpr (TR::kSplit) {
+      const uint16_t h = Ah[r * LDA + c], l = Ah[lo_off + r * LDA + c];
+      return (double)__builtin_bit_cast(float, (uint32_t)h << 16) +
+             (double)__builtin_bit_cast(float, (uint32_t)l << 16);
+    } else {
+      return (double)A[r * LDA + c];
+    }
+  };
   double cnt = 0.0, sum = 0.0, sq = 0.0;
   for (int r = tid; r < BM; r += nthreads) {
     const int path = p0 + r;
     if (path < jb.N) {
       double tot = 0.0;
-      for (int c = 0; c < Pp; ++c) tot += (double)A[r * LDA + c];
+      for (int c = 0; c < Pp; ++c) tot += state_at(r, c);
       if (jb.end_pops) jb.end_pops[path] = tot;
       cnt += (tot >= jb.thr_total) ? 1.0 : 0.0;
       sum += tot;
@@ -308,7 +378,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   for (int c = tid; c < Pp; c += nthreads) {
     const int pc = TR::perm(c);
     double s = 0.0;
-    for (int r = 0; r < nvalid; ++r) s += (double)A[r * LDA + pc];
+    for (int r = 0; r < nvalid; ++r) s += state_at(r, pc);
     part[3 + c] = s;
   }
   __syncthreads();  // LDS is reused by the next tile
@@ -381,51 +451,96 @@ __global__ void fill_i32_kernel(int* p, int v, int n) {
   if (i < n) p[i] = v;
 }
 
-hipError_t mte_set_lds_limit(bool fp64, size_t bytes) {
-  hipError_t e;
-  if (fp64) {
-    e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&mte_direct_kernel<TraitsF64>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&mte_persistent_kernel<TraitsF64>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  }
-  e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&mte_direct_kernel<TraitsF32>),
+template <class TR>
+static hipError_t set_lds_limit_t(size_t bytes) {
+  const hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&mte_direct_kernel<TR>),
       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&mte_persistent_kernel<TraitsF32>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&mte_persistent_kernel<TR>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_mte_direct(bool fp64, const MteJob* jobs, const int2* work,
+hipError_t mte_set_lds_limit(int prec, size_t bytes) {
+  switch (prec) {
+    case kPrecF64: return set_lds_limit_t<TraitsF64>(bytes);
+    case kPrecB3: return set_lds_limit_t<TraitsB3>(bytes);
+    default: return set_lds_limit_t<TraitsF32>(bytes);
+  }
+}
+
+hipError_t launch_mte_direct(int prec, const MteJob* jobs, const int2* work,
                              int n_work, size_t lds, hipStream_t s) {
   if (n_work <= 0) return hipSuccess;
-  if (fp64)
-    hipLaunchKernelGGL(mte_direct_kernel<TraitsF64>, dim3(n_work),
-                       dim3(kMteThreads), lds, s, jobs, work);
-  else
-    hipLaunchKernelGGL(mte_direct_kernel<TraitsF32>, dim3(n_work),
-                       dim3(kMteThreads), lds, s, jobs, work);
+  switch (prec) {
+    case kPrecF64:
+      hipLaunchKernelGGL(mte_direct_kernel<TraitsF64>, dim3(n_work), dim3(kMteThreads), lds, s,
+                         jobs, work);
+      break;
+    case kPrecB3:
+      hipLaunchKernelGGL(mte_direct_kernel<TraitsB3>, dim3(n_work), dim3(kMteThreads), lds, s,
+                         jobs, work);
+      break;
+    default:
+      hipLaunchKernelGGL(mte_direct_kernel<TraitsF32>, dim3(n_work), dim3(kMteThreads), lds, s,
+                         jobs, work);
+      break;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_mte_persistent(bool fp64, const MteJob* jobs,
+hipError_t launch_mte_persistent(int prec, const MteJob* jobs,
                                  const int2* work, const int* qbeg, int nq,
                                  int* counters, int grid, size_t lds,
                                  hipStream_t s) {
   if (nq <= 0 || grid <= 0) return hipSuccess;
-  if (fp64)
-    hipLaunchKernelGGL(mte_persistent_kernel<TraitsF64>, dim3(grid),
-                       dim3(kMteThreads), lds, s, jobs, work, qbeg, nq,
-                       counters);
-  else
-    hipLaunchKernelGGL(mte_persistent_kernel<TraitsF32>, dim3(grid),
-                       dim3(kMteThreads), lds, s, jobs, work, qbeg, nq,
-                       counters);
+  switch (prec) {
+    case kPrecF64:
+      hipLaunchKernelGGL(mte_persistent_kernel<TraitsF64>, dim3(grid), dim3(kMteThreads), lds,
+                         s, jobs, work, qbeg, nq, counters);
+      break;
+    case kPrecB3:
+      hipLaunchKernelGGL(mte_persistent_kernel<TraitsB3>, dim3(grid), dim3(kMteThreads), lds,
+                         s, jobs, work, qbeg, nq, counters);
+      break;
+    default:
+      hipLaunchKernelGGL(mte_persistent_kernel<TraitsF32>, dim3(grid), dim3(kMteThreads), lds,
+                         s, jobs, work, qbeg, nq, counters);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// In-place conversion of FP32-packed movement matrices to the split-bf16
+// operand order.  Each 512-float chunk (one column block, k-groups 2c and
+// 2c + 1: 16 k x 32 columns) is converted on its own.  In the FP32 order,
+// k = 16c + 8g + 2s + h of column n sits at float (64 g + 32 h + n) * 4 + s
+// of the chunk; in the split order lane l = n + 32g holds k = 16c + 8g + i,
+// i = 0..7, as 8 bf16 hi values at bytes 16 l and 8 bf16 lo values at bytes
+// 1024 + 16 l.  One wave per chunk reads all of it before writing.
+__global__ __launch_bounds__(64) void mte_split_b3_kernel(float* __restrict__ m) {
+  float* c = m + (size_t)blockIdx.x * 512;
+  const int l = threadIdx.x, n = l & 31, g = l >> 5;
+  float v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = c[(64 * g + 32 * (i & 1) + n) * 4 + (i >> 1)];
+  __syncthreads();
+  uint32_t hi[4], lo[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t a = TraitsB3::split(v[2 * j]), b = TraitsB3::split(v[2 * j + 1]);
+    hi[j] = (a & 0xffffu) | (b << 16);
+    lo[j] = (a >> 16) | (b & 0xffff0000u);
+  }
+  uint4* out = reinterpret_cast<uint4*>(c);
+  out[l] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+  out[64 + l] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+}
+
+hipError_t launch_mte_split_b3(float* m, size_t n_floats, hipStream_t s) {
+  if (n_floats % 512 != 0) return hipErrorInvalidValue;
+  if (n_floats == 0) return hipSuccess;
+  hipLaunchKernelGGL(mte_split_b3_kernel, dim3((unsigned)(n_floats / 512)), dim3(64), 0, s, m);
   return hipGetLastError();
 }
 

@@ -53,7 +53,8 @@ int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr) {
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown growth_law");
   if (pr->endpoint != ROADMC_END_POST_GROWTH && pr->endpoint != ROADMC_END_PRE_GROWTH)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown endpoint");
-  if (pr->precision != ROADMC_PREC_FP32 && pr->precision != ROADMC_PREC_FP64)
+  if (pr->precision != ROADMC_PREC_FP32 && pr->precision != ROADMC_PREC_FP64 &&
+      pr->precision != ROADMC_PREC_BF16X3)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown precision");
   if (!(pr->drop_tol >= 0.0) || !std::isfinite(pr->time_step))
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "drop_tol/time_step invalid");
@@ -140,6 +141,7 @@ int rows_for(const roadmc_ctx* ctx, bool fp64, int Pp, int64_t N) {
 // Host-side description of a set of (road, species) jobs, shared by the
 // single-road and batched entry points.
 struct JobSet {
+  int prec = roadmc::kPrecF32;
   bool fp64 = false;
   std::vector<MteJob> jobs;
   std::vector<int> road_of, species_of;
@@ -153,7 +155,8 @@ struct JobSet {
 
 void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roads,
                bool want_end_pops, JobSet& js) {
-  js.fp64 = roads[0].precision == ROADMC_PREC_FP64;
+  js.prec = roads[0].precision;
+  js.fp64 = js.prec == ROADMC_PREC_FP64;
   for (int r = 0; r < n_roads; ++r) {
     const roadmc_mte_problem& pr = roads[r];
     for (int s = 0; s < pr.n_species; ++s) {
@@ -161,7 +164,8 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       MteJob jb{};
       jb.P = sp.n_patches;
       jb.Pp = round_up(jb.P, 32);
-      jb.Pk = round_up(jb.P, 16);
+      // The split-bf16 products take 16-k chunks two at a time.
+      jb.Pk = js.prec == roadmc::kPrecB3 ? jb.Pp : round_up(jb.P, 16);
       jb.N = (int32_t)pr.n_paths;
       jb.T = pr.n_years;
       jb.BM = rows_for(ctx, js.fp64, jb.Pp, pr.n_paths);
@@ -195,9 +199,9 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       const int rem = ntiles % 4;
       jb.split = 0;
       if (!js.fp64 && rem && (jb.Pk >> 3) >= 8 &&
-          roadmc::mte_lds_bytes(false, jb.BM, jb.Pp, rem) <= kLdsPerCu)
+          roadmc::mte_lds_bytes(js.prec, jb.BM, jb.Pp, rem) <= kLdsPerCu)
         jb.split = 1;
-      js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.fp64, jb.BM, jb.Pp,
+      js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.prec, jb.BM, jb.Pp,
                                                       jb.split ? rem : 0));
       js.jobs.push_back(jb);
       js.road_of.push_back(r);
@@ -370,6 +374,9 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
       RMC_HIP(ctx, hipStreamWaitEvent(s, l.done, 0));
     }
   }
+  // Split-bf16 operands are converted in place from the FP32 packing.
+  if (js.prec == roadmc::kPrecB3)
+    RMC_HIP(ctx, roadmc::launch_mte_split_b3(static_cast<float*>(b.mpack.ptr), js.mpack_elems, s));
   roadmc::RoadTail tail{0, {0, 0, 0}};
   if (maxP > 0)
     RMC_HIP(ctx, hipMemcpyAsync(&tail, roadmc::road_tail(ctx), sizeof(tail),
@@ -380,10 +387,12 @@ int upload_jobs(roadmc_ctx* ctx, const roadmc_mte_problem* roads, JobSet& js,
   return ROADMC_OK;
 }
 
-int ensure_lds_limit(roadmc_ctx* ctx, bool fp64, size_t lds) {
-  size_t& cur = fp64 ? ctx->lds_limit_f64 : ctx->lds_limit_f32;
+int ensure_lds_limit(roadmc_ctx* ctx, int prec, size_t lds) {
+  size_t& cur = prec == roadmc::kPrecF64  ? ctx->lds_limit_f64
+                : prec == roadmc::kPrecB3 ? ctx->lds_limit_b3
+                                          : ctx->lds_limit_f32;
   if (lds > cur) {
-    RMC_HIP(ctx, roadmc::mte_set_lds_limit(fp64, lds));
+    RMC_HIP(ctx, roadmc::mte_set_lds_limit(prec, lds));
     cur = lds;
   }
   return ROADMC_OK;
@@ -465,8 +474,8 @@ extern "C" int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem
   rc = js.fp64 ? upload_jobs<double>(ctx, problem, js, b, false)
                : upload_jobs<float>(ctx, problem, js, b, false);
   if (rc) return rc;
-  if ((rc = ensure_lds_limit(ctx, js.fp64, js.lds))) return rc;
-  RMC_HIP(ctx, roadmc::launch_mte_direct(js.fp64, static_cast<MteJob*>(b.jobs.ptr),
+  if ((rc = ensure_lds_limit(ctx, js.prec, js.lds))) return rc;
+  RMC_HIP(ctx, roadmc::launch_mte_direct(js.prec, static_cast<MteJob*>(b.jobs.ptr),
                                          static_cast<int2*>(b.work.ptr), (int)js.work.size(),
                                          js.lds, ctx->stream));
   RMC_HIP(ctx, roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
@@ -502,7 +511,7 @@ int capture_batch_graph(roadmc_batch* bt) {
   hipError_t e = roadmc::launch_fill_i32(static_cast<int*>(b.counter.ptr), 0, 16, s);
   if (e == hipSuccess)
     e = roadmc::launch_mte_persistent(
-        js.fp64, static_cast<MteJob*>(b.jobs.ptr), static_cast<int2*>(b.work.ptr),
+        js.prec, static_cast<MteJob*>(b.jobs.ptr), static_cast<int2*>(b.work.ptr),
         static_cast<int*>(b.counter.ptr) + 16, (int)js.qbeg.size() - 1,
         static_cast<int*>(b.counter.ptr), bt->grid, js.lds, s);
   if (e == hipSuccess)
@@ -541,7 +550,7 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
   plan_jobs(ctx, roads, n_roads, true, bt->js);
   int rc = bt->js.fp64 ? upload_jobs<double>(ctx, roads, bt->js, bt->bufs, true)
                        : upload_jobs<float>(ctx, roads, bt->js, bt->bufs, true);
-  if (!rc) rc = ensure_lds_limit(ctx, bt->js.fp64, bt->js.lds);
+  if (!rc) rc = ensure_lds_limit(ctx, bt->js.prec, bt->js.lds);
   if (!rc) {
     // One workgroup per CU fits the LDS of the largest tile; small tiles
     // leave room for a second one.
@@ -849,6 +858,8 @@ extern "C" int roadmc_mte_run_plain(roadmc_ctx* ctx, const roadmc_mte_problem* p
   for (int sp = 0; sp < problem->n_species; ++sp)
     if (!problem->species[sp].trans)
       return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the plain path needs host matrices");
+  if (problem->precision == ROADMC_PREC_BF16X3)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the plain path runs FP32 or FP64");
   RMC_HIP(ctx, hipSetDevice(ctx->device));
   const int S = problem->n_species;
   const int64_t N = problem->n_paths;
@@ -975,6 +986,8 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   if (rc) return rc;
   if (!pr->species[0].trans)
     return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the adjoint needs host matrices");
+  if (pr->precision == ROADMC_PREC_BF16X3)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the adjoint runs FP32 or FP64");
   if (pr->n_species != 1 || pr->growth_law != ROADMC_LAW_LOGNORMAL ||
       pr->endpoint != ROADMC_END_POST_GROWTH || pr->drop_tol != 0.0)
     return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
@@ -1035,13 +1048,13 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   std::vector<int2> work(jb.n_tiles);
   for (int t = 0; t < jb.n_tiles; ++t) work[t] = make_int2(0, t);
   const int64_t off0 = 0;
-  const size_t lds = roadmc::mte_lds_bytes(adj->fp64, jb.BM, jb.Pp, 0);
+  const size_t lds = roadmc::mte_lds_bytes(adj->fp64 ? roadmc::kPrecF64 : roadmc::kPrecF32, jb.BM, jb.Pp, 0);
   hipStream_t s = ctx->stream;
   auto fail = [&](int code) {
     roadmc_adjoint_destroy(adj);
     return code;
   };
-  if ((rc = ensure_lds_limit(ctx, adj->fp64, lds))) return fail(rc);
+  if ((rc = ensure_lds_limit(ctx, adj->fp64 ? roadmc::kPrecF64 : roadmc::kPrecF32, lds))) return fail(rc);
   hipError_t e = hipMemcpyAsync(adj->bufs.jobs.ptr, &jb, sizeof(MteJob), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(adj->bufs.work.ptr, work.data(), work.size() * sizeof(int2),
@@ -1049,7 +1062,7 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   if (e == hipSuccess)
     e = hipMemcpyAsync(adj->bufs.off.ptr, &off0, sizeof(off0), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = roadmc::launch_mte_direct(adj->fp64, static_cast<MteJob*>(adj->bufs.jobs.ptr),
+    e = roadmc::launch_mte_direct(adj->fp64 ? roadmc::kPrecF64 : roadmc::kPrecF32, static_cast<MteJob*>(adj->bufs.jobs.ptr),
                                   static_cast<int2*>(adj->bufs.work.ptr), jb.n_tiles, lds, s);
   if (e == hipSuccess)
     e = roadmc::launch_mte_reduce(static_cast<MteJob*>(adj->bufs.jobs.ptr), 1,

@@ -27,6 +27,7 @@ END_POST_GROWTH = 0
 END_PRE_GROWTH = 1
 PREC_FP32 = 0
 PREC_FP64 = 1
+PREC_BF16X3 = 2  # split-bf16 products, tiled MTE kernels only
 MAX_PATCHES = 1024
 
 _dp = C.POINTER(C.c_double)
