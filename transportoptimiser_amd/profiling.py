@@ -11,7 +11,7 @@ prints the kernel-trace statistics rows of the named kernels.
 ``python transportoptimiser_amd/profiling.py uniform --patches P`` runs one
 batched launch of 8 roads that all have P patches (2 species, 50 years,
 20,000 paths), the per-size workload used for counter profiles of the
-persistent MTE kernel.
+persistent MTE kernel; ``--precision bf16x3`` runs the split-bf16 products.
 
 ``python transportoptimiser_amd/profiling.py rov [--repeat R]`` runs the ROV
 valuation of 3 uncertainties (2 OU + 1 GBM), 2 controls, 50 steps, 100,000
@@ -63,15 +63,18 @@ def copy(nbytes: int) -> None:
     print(f"copied {n * 4} bytes twice (second copy is the measured one)")
 
 
-def uniform(P: int, n_roads: int = 8) -> None:
+def uniform(P: int, n_roads: int = 8, precision: str = "fp32") -> None:
     import os
     import numpy as np
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     from transportoptimiser_amd import landscape as ls
+    from transportoptimiser_amd import native as nat
     from transportoptimiser_amd.api import Context
+    prec = nat.PREC_BF16X3 if precision == "bf16x3" else nat.PREC_FP32
     rng = np.random.default_rng(P)
     roads = [ls.RoadProblem([ls.make_species(rng, P) for _ in range(2)], n_years=50,
-                            n_paths=20000, seed=1, road_id=r) for r in range(n_roads)]
+                            n_paths=20000, seed=1, road_id=r, precision=prec)
+             for r in range(n_roads)]
     flops = sum(2.0 * r.n_paths * P * P * r.n_years * len(r.species) for r in roads)
     ctx = Context(0)
     with ctx.prepare_batch(roads) as b:
@@ -79,7 +82,8 @@ def uniform(P: int, n_roads: int = 8) -> None:
         b.sync()
         ms = b.kernel_ms()
     ctx.close()
-    print(f"P={P}: {n_roads} roads, kernel {ms:.1f} ms, {flops / ms / 1e9:.1f} TFLOP/s", flush=True)
+    print(f"P={P} {precision}: {n_roads} roads, kernel {ms:.1f} ms, "
+          f"{flops / ms / 1e9:.1f} TFLOP/s", flush=True)
 
 
 def rov(repeat: int) -> None:
@@ -273,6 +277,7 @@ def main() -> int:
     u = sub.add_parser("uniform")
     u.add_argument("--patches", type=int, required=True)
     u.add_argument("--roads", type=int, default=8)
+    u.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
     rd = sub.add_parser("road")
     rd.add_argument("--patches", type=int, default=1024)
     rd.add_argument("--points", type=int, default=200)
@@ -317,7 +322,7 @@ def main() -> int:
     elif a.cmd == "road":
         road(a.patches, a.points, a.repeat)
     elif a.cmd == "uniform":
-        uniform(a.patches, a.roads)
+        uniform(a.patches, a.roads, a.precision)
     elif a.cmd == "dbstats":
         dbstats(a.path)
     elif a.cmd == "counters":
