@@ -232,6 +232,23 @@ def run_extras(ctx, roads, prep_ms) -> dict:
         "recompute_tflop": round(fl6 / 1e12, 2), "tflops_whole_call": round(fl6 / ms / 1e9, 1),
         "what": "species-state ROV with forward recomputation (ALGO6), same problem; host call"}
     log(f"rov species ALGO6: {out['rov_species_algo6']}")
+    # The same two valuations with split-bf16 products in the forward pass and
+    # the recomputation.
+    from transportoptimiser_amd import native as nat
+    for method in (4, 6):
+        pr5.method, pr5.precision = method, nat.PREC_BF16X3
+        rsp.simulate_rov_species(ctx, pr5)
+        t = time.perf_counter()
+        rb = rsp.simulate_rov_species(ctx, pr5)
+        ms = (time.perf_counter() - t) * 1e3
+        key = f"rov_species_algo{method}_bf16x3"
+        out[key] = {"ms": round(ms, 2), "value": round(rb["value"], 4),
+                    "first_control": rb["first_control"],
+                    "what": f"ALGO{method}, same problem, split-bf16 products; host call"}
+        if method == 6:
+            out[key]["tflops_whole_call"] = round(fl6 / ms / 1e9, 1)
+        log(f"rov species ALGO{method} bf16x3: {out[key]}")
+    pr5.precision = nat.PREC_FP32
     from transportoptimiser_amd import knn as kn
     rng = np.random.default_rng(1)
     xk = rng.uniform(0.0, 1.0, size=(4096, 3))

@@ -431,6 +431,9 @@ typedef struct {
                                              many years, the recomputation replaying from
                                              the last one (0: every year while they fit
                                              in 16 GiB) */
+  int32_t precision;                      /* products of the forward pass and the
+                                             recomputation: ROADMC_PREC_FP32 or
+                                             ROADMC_PREC_BF16X3 */
 } roadmc_rovs_problem;
 
 typedef struct {

@@ -1,6 +1,7 @@
 """GPU species-state ROV (roadmc_rovs_run) against the oracle.
 
-The forward pass runs in FP32 on MFMA on the GPU and in FP64 in the oracle:
+The forward pass runs in FP32 (or split-bf16) on MFMA on the GPU and in FP64
+in the oracle:
 totals and surviving fractions agree to 1e-4 relative, and the drawn
 controls exactly (the same Philox stream).  The backward induction is FP64 on
 both sides; it is compared on identical inputs (the oracle's backward runs on
@@ -9,6 +10,7 @@ the GPU's forward arrays): values to 1e-8 relative, decisions path by path.
 import numpy as np
 import pytest
 
+from transportoptimiser_amd import native as nat
 from transportoptimiser_amd import rov_species as rs
 from tests.test_oracle_rovs import _flat_problem
 
@@ -21,14 +23,20 @@ def _run(ctx, pr):
     return rs.simulate_rov_species(ctx, pr, paths=True, grids=True)
 
 
+PRECISIONS = [nat.PREC_FP32, nat.PREC_BF16X3]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("P,S,C,N,T,R", [
     (24, 1, 3, 2000, 12, 8),
     (40, 2, 2, 1500, 10, 6),
     (64, 1, 4, 3001, 15, 10),
 ])
-def test_matches_oracle(ctx, oracle, P, S, C, N, T, R):
+def test_matches_oracle(ctx, oracle, P, S, C, N, T, R, prec):
+    # Split-bf16 products (PREC_BF16X3) hold the forward within the FP32 bar.
     pr = rs.synthetic_problem(seed=P + S, n_patches=P, n_species=S, n_controls=C, n_paths=N,
                               n_years=T, dim_res=R)
+    pr.precision = prec
     g = _run(ctx, pr)
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     np.testing.assert_array_equal(g["controls"], ctl)
@@ -68,10 +76,12 @@ def test_unreachable_thresholds_keep_zero_flow(ctx):
     assert (g["opt_control"] == 0).all()
 
 
-def test_largest_landscape(ctx, oracle):
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_largest_landscape(ctx, oracle, prec):
     # P = 512 (16 column tiles, 8 waves per workgroup), N not a multiple of 32.
     pr = rs.synthetic_problem(seed=11, n_patches=512, n_controls=2, n_paths=333, n_years=6,
                               dim_res=6)
+    pr.precision = prec
     g = _run(ctx, pr)
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     np.testing.assert_allclose(g["total_pops"], tp, rtol=1e-4)
@@ -97,15 +107,16 @@ def test_invalid_arguments(ctx):
 # wherever the decisions agree; decisions can differ only where two payoffs or
 # a state and a threshold are within FP32 rounding of each other.
 
+@pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("P,S,C,N,T,R,ratio,ck", [
     (24, 1, 3, 2000, 12, 8, 0.98, 0),
     (40, 2, 2, 1500, 10, 6, 0.9, 3),
     (64, 1, 4, 3001, 15, 10, 0.93, 4),
 ])
-def test_algo6_matches_oracle(ctx, oracle, P, S, C, N, T, R, ratio, ck):
+def test_algo6_matches_oracle(ctx, oracle, P, S, C, N, T, R, ratio, ck, prec):
     pr = rs.synthetic_problem(seed=P + S, n_patches=P, n_species=S, n_controls=C, n_paths=N,
                               n_years=T, dim_res=R, threshold_ratio=ratio)
-    pr.method, pr.checkpoint_years = 6, ck
+    pr.method, pr.checkpoint_years, pr.precision = 6, ck, prec
     g = _run(ctx, pr)
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     np.testing.assert_array_equal(g["controls"], ctl)
@@ -154,12 +165,14 @@ def test_algo6_closed_forms(ctx):
     assert g["value"] == 0.0 and (g["opt_control"] == 0).all()
 
 
-def test_algo6_checkpoint_interval_does_not_change_results(ctx):
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_algo6_checkpoint_interval_does_not_change_results(ctx, prec):
     # Replaying from a checkpoint reproduces the forward populations, so the
-    # result is the same for every checkpoint interval.
+    # result is the same for every checkpoint interval (split-bf16 checkpoints
+    # keep each entry's hi and lo halves).
     pr = rs.synthetic_problem(seed=5, n_patches=48, n_species=1, n_controls=3, n_paths=1200,
                               n_years=11, dim_res=6, threshold_ratio=0.98)
-    pr.method = 6
+    pr.method, pr.precision = 6, prec
     ref = _run(ctx, pr)
     for ck in (2, 5, 11):
         pr.checkpoint_years = ck
@@ -182,6 +195,11 @@ def test_algo6_limits(ctx):
     with pytest.raises(nat.RoadmcError) as e:
         rs.simulate_rov_species(ctx, pr)
     assert e.value.code == nat.ERR_INVALID
+    pr = rs.synthetic_problem(n_patches=16, n_paths=100, n_years=5)
+    pr.precision = nat.PREC_FP64
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("P,S", [(512, 2), (256, 4), (33, 7)])

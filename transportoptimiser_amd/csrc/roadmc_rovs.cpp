@@ -48,6 +48,11 @@ int validate(roadmc_ctx* ctx, const roadmc_rovs_problem* p, const roadmc_rovs_re
   }
   if (p->method != 0 && p->method != ROADMC_ROVS_ALGO4 && p->method != ROADMC_ROVS_ALGO6)
     return bad("method must be ROADMC_ROVS_ALGO4 or ROADMC_ROVS_ALGO6");
+  if (p->precision == ROADMC_PREC_FP64)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "the species-state forward runs FP32 or split-bf16 products");
+  if (p->precision != ROADMC_PREC_FP32 && p->precision != ROADMC_PREC_BF16X3)
+    return bad("unknown precision");
   if (p->checkpoint_years < 0) return bad("checkpoint_years must be >= 0");
   if (!(p->time_step > 0) || !(p->profit_scaling > 0) || !std::isfinite(p->rrr) ||
       !std::isfinite(p->hysteresis) || !std::isfinite(p->unit_cost) ||
@@ -73,7 +78,9 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   RMC_HIP(ctx, hipSetDevice(ctx->device));
   const int P = pr->n_patches, S = pr->n_species, C = pr->n_controls, T = pr->n_years;
   const int N = (int)pr->n_paths, U = pr->n_fuels + pr->n_commodities, D = S + 1, R = pr->dim_res;
-  const int Pp = round_up(P, 32), Pk = round_up(P, 16);
+  // Split-bf16 products contract over Pp (an even count of 16-k chunks).
+  const bool split = pr->precision == ROADMC_PREC_BF16X3;
+  const int Pp = round_up(P, 32), Pk = split ? Pp : round_up(P, 16);
   size_t G = 1;
   for (int d = 0; d < D; ++d) G *= (size_t)R;
   const size_t gsz = (size_t)D * R + G;
@@ -84,7 +91,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   // from a checkpoint of the forward populations every ck years (every year
   // while the checkpoints fit in 16 GiB) and reads every step's grids.
   constexpr size_t kLdsMax = 160 * 1024, kCkptBudget = (size_t)16 << 30;
-  if (algo6 && roadmc::rovs_recompute_lds(Pp, C, S) > kLdsMax)
+  if (algo6 && roadmc::rovs_recompute_lds(Pp, C, S, split) > kLdsMax)
     return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
                     "ALGO6: the species' population tiles exceed the LDS of a workgroup");
   const size_t ck_all = (size_t)S * T * N * Pp * sizeof(float);
@@ -145,7 +152,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
           rs += v;
         }
         // Row sum of origin k, at k's column of the population tile.
-        if (algo6) rvec[((size_t)sp * C + c) * Pp + roadmc::rovs_tile_col(k)] = (float)rs;
+        if (algo6) rvec[((size_t)sp * C + c) * Pp + roadmc::rovs_tile_col(k, split)] = (float)rs;
       }
     }
     float* cl = cols.data() + (size_t)sp * 3 * Pp;
@@ -160,6 +167,8 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     init_total[sp] = (float)tot;
   }
   RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  if (split)
+    RMC_HIP(ctx, roadmc::launch_mte_split_b3(static_cast<float*>(B.mpack.ptr), mp.size(), s));
   RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float), hipMemcpyHostToDevice, s));
   if (algo6)
     RMC_HIP(ctx, hipMemcpyAsync(B.rvec.ptr, rvec.data(), rvec.size() * sizeof(float),
@@ -213,6 +222,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     a.total_pops = static_cast<float*>(B.tp.ptr) + (size_t)sp * (T + 1) * N;
     a.aars = static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N;
     a.ck = 1;
+    a.split = split;
     if (algo6) {
       a.ckpt = static_cast<float*>(B.ckpt.ptr) + (size_t)sp * nck * N * Pp;
       a.ck = ck;
@@ -295,6 +305,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     ra.D = D;
     ra.R = R;
     ra.ck = ck;
+    ra.split = split;
     ra.mpack = static_cast<const float*>(B.mpack.ptr);
     ra.cols = static_cast<const float*>(B.cols.ptr);
     ra.rvec = static_cast<const float*>(B.rvec.ptr);

@@ -37,9 +37,12 @@ struct RovsForwardArgs {
   float* ckpt;
   int ck;
   float* grm_out;
+  // Split-bf16 products (mpack in that order, Pk == Pp); the checkpoints then
+  // hold each entry's hi | lo << 16 bits.
+  int split;
 };
 
-size_t rovs_forward_lds(int Pp, int C);
+size_t rovs_forward_lds(int Pp, int C, bool split);
 hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s);
 
 // Unit profit per path and year: unit_cost + sum_v fuel_cost_v price_{fuel_idx_v}
@@ -117,10 +120,11 @@ struct RovsRecomputeArgs {
   double* V;
   int* optc;
   double* adj_pops;      // [T][S][N] or nullptr
+  int split;             // split-bf16 products, as the forward pass ran them
 };
-size_t rovs_recompute_lds(int Pp, int C, int S);
+size_t rovs_recompute_lds(int Pp, int C, int S, bool split);
 // Position of patch k within a row of the population tile.
-int rovs_tile_col(int k);
+int rovs_tile_col(int k, bool split);
 hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s);
 // out (device): {value, SD, control}.
 hipError_t launch_rovs_first(const RovsStepArgs* a, double* out, hipStream_t s);

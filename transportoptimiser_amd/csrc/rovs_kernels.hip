@@ -46,8 +46,54 @@ static int rovs_waves(int Pp) {
   return std::max(4, (ntc + kRovsNT - 1) / kRovsNT);
 }
 
-size_t rovs_forward_lds(int Pp, int C) {
-  const int LDA = Pp + TF::PAD;
+// Population tile of 32 rows in LDS, through the arithmetic of the products:
+// FP32 (TraitsF32: floats, k permuted within groups of 8) or split bf16
+// (TraitsB3: a bf16 hi array then a bf16 lo array, natural k order).  Both
+// take 4 bytes per entry; LDA is Pp + PAD entries.  Positions c are stored
+// column positions (TR::perm of the patch).
+template <class TR>
+struct RovsTile {
+  float* A;
+  int LDA;
+  __device__ uint16_t* hi() const { return reinterpret_cast<uint16_t*>(A); }
+  __device__ void put(int r, int c, float v) const {
+    if constexpr (TR::kSplit) {
+      const uint32_t hl = TR::split(v);
+      hi()[r * LDA + c] = (uint16_t)hl;
+      hi()[kRovsRows * LDA + r * LDA + c] = (uint16_t)(hl >> 16);
+    } else {
+      A[r * LDA + c] = v;
+    }
+  }
+  __device__ float get(int r, int c) const {
+    if constexpr (TR::kSplit) {
+      return __builtin_bit_cast(float, (uint32_t)hi()[r * LDA + c] << 16) +
+             __builtin_bit_cast(float, (uint32_t)hi()[kRovsRows * LDA + r * LDA + c] << 16);
+    } else {
+      return A[r * LDA + c];
+    }
+  }
+  // The stored bits of an entry (checkpoints): the float, or hi | lo << 16.
+  __device__ uint32_t raw(int r, int c) const {
+    if constexpr (TR::kSplit)
+      return (uint32_t)hi()[r * LDA + c] | ((uint32_t)hi()[kRovsRows * LDA + r * LDA + c] << 16);
+    else
+      return __builtin_bit_cast(uint32_t, A[r * LDA + c]);
+  }
+  __device__ void put_raw(int r, int c, uint32_t u) const {
+    if constexpr (TR::kSplit) {
+      hi()[r * LDA + c] = (uint16_t)u;
+      hi()[kRovsRows * LDA + r * LDA + c] = (uint16_t)(u >> 16);
+    } else {
+      A[r * LDA + c] = __builtin_bit_cast(float, u);
+    }
+  }
+};
+
+static int rovs_pad(bool split) { return split ? TraitsB3::PAD : TF::PAD; }
+
+size_t rovs_forward_lds(int Pp, int C, bool split) {
+  const int LDA = Pp + rovs_pad(split);
   return (size_t)kRovsRows * LDA * sizeof(float)      // state tile
          + kRovsRows * sizeof(float)                  // growth-rate means
          + (size_t)rovs_waves(Pp) * kRovsRows * sizeof(float)  // per-wave row sums
@@ -63,26 +109,31 @@ __device__ __forceinline__ float half_sum(float v) {
   return v;
 }
 
-template <int NT>
+template <class TR, int NT>
 __device__ __forceinline__ void rovs_product(floatx16* acc, const PackedB& B, const float* A,
                                              const int* tcol, int LDA, int arow, int akoff,
                                              int lane) {
-  floatx4 b0[NT];
   int trow[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    trow[t] = 0;
-    b0[t] = load_b<floatx4>(B, b_voff(B, tcol[t], lane), 0);
+  for (int t = 0; t < NT; ++t) trow[t] = 0;
+  if constexpr (TR::kSplit) {
+    year_product_b3<NT, true>(acc, B, reinterpret_cast<const uint16_t*>(A), kRovsRows * LDA,
+                              trow, tcol, LDA, arow, akoff, lane);
+  } else {
+    floatx4 b0[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b0[t] = load_b<floatx4>(B, b_voff(B, tcol[t], lane), 0);
+    year_product<TF, NT, true>(acc, b0, B, A, trow, tcol, LDA, arow, akoff, lane);
   }
-  year_product<TF, NT, true>(acc, b0, B, A, trow, tcol, LDA, arow, akoff, lane);
 }
 
+template <class TR>
 __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwardArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nthreads = blockDim.x, NW = nthreads >> 6;
-  const int Pp = a.Pp, LDA = Pp + TF::PAD, C = a.C;
+  const int Pp = a.Pp, LDA = Pp + TR::PAD, C = a.C;
   const int p0 = blockIdx.x * kRovsRows;
   float* A = reinterpret_cast<float*>(smem);
   float* grm = A + kRovsRows * LDA;
@@ -97,9 +148,10 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
   const float jmp = (float)a.params[5], jprob = (float)a.params[6], lv = (float)a.params[7];
   const float dt = (float)a.dt;
 
+  const RovsTile<TR> tile{A, LDA};
   for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
     const int r = idx / Pp, c = idx - r * Pp;
-    A[r * LDA + TF::perm(c)] = init[c];
+    tile.put(r, TR::perm(c), init[c]);
   }
   if (tid < kRovsRows) {
     grm[tid] = (float)a.params[0];
@@ -114,7 +166,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
   int tcol[kRovsNT];
 #pragma unroll
   for (int t = 0; t < kRovsNT; ++t) tcol[t] = my_t0 + t;
-  const int arow = TF::a_row(lane), akoff = TF::KV * TF::k_lane(lane);
+  const int arow = TR::a_row(lane), akoff = (TR::kSplit ? 8 : TR::KV) * TR::k_lane(lane);
   const int nkg = a.Pk >> 3;
   const size_t mbytes = (size_t)Pp * a.Pk * sizeof(float);
   __syncthreads();
@@ -130,10 +182,11 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
     // Checkpoint of the tile at the start of the year (read-only here; the
     // epilogue's writes come after the barriers of the control loop).
     if (a.ckpt && year % a.ck == 0) {
-      float* dst = a.ckpt + (size_t)(year / a.ck) * a.N * Pp;
+      uint32_t* dst = reinterpret_cast<uint32_t*>(a.ckpt) + (size_t)(year / a.ck) * a.N * Pp;
+      const RovsTile<TR> ty{A, LDAy};
       for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
         const int r = idx / Pp, c = idx - r * Pp;
-        if (p0y + r < a.N) dst[(size_t)(p0y + r) * Pp + c] = A[r * LDAy + c];
+        if (p0y + r < a.N) dst[(size_t)(p0y + r) * Pp + c] = ty.raw(r, c);
       }
     }
     __syncthreads();
@@ -143,9 +196,9 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
     for (int c = 0; c < C; ++c) {
       const PackedB B = packed_b(a.mpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
       if (my_n == 1)
-        rovs_product<1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+        rovs_product<TR, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
       else if (my_n == 2)
-        rovs_product<2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+        rovs_product<TR, 2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
       // Row sums of this control's product over the wave's columns, and the
       // select of each path's own control.
 #pragma unroll
@@ -187,6 +240,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
     // n = n' (1 + g (K - n') / K / 100), written back to the tile.
     int LDAe = LDA, p0e = p0, lanee = lane;
     asm volatile("" : "+s"(LDAe), "+s"(p0e), "+v"(lanee));
+    const RovsTile<TR> te{A, LDAe};
     float rowv[TF::NACC];
 #pragma unroll
     for (int r = 0; r < TF::NACC; ++r) rowv[r] = 0.f;
@@ -194,7 +248,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
     for (int t = 0; t < kRovsNT; ++t) {
       if (t >= my_n) break;
       const int col = tcol[t] * TF::TN + TF::acc_col(lanee);
-      const int pcol = TF::perm(col);
+      const int pcol = TR::perm(col);
       const float Kc = K[col], Ki = Kinv[col];
 #pragma unroll
       for (int r4 = 0; r4 < TF::NACC; r4 += 4) {
@@ -208,7 +262,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
           const float x = sel[t][r4 + s];
           const float g = grm[row] + z[s] * lv;
           const float n = x * (1.f + g * (Kc - x) * Ki * 0.01f);
-          A[row * LDAe + pcol] = n;
+          te.put(row, pcol, n);
           rowv[r4 + s] += n;
         }
       }
@@ -234,14 +288,22 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwa
   }
 }
 
-hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s) {
-  const size_t lds = rovs_forward_lds(a.Pp, a.C);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_forward_kernel),
+template <class TR>
+static hipError_t launch_rovs_forward_t(const RovsForwardArgs& a, hipStream_t s) {
+  const size_t lds = rovs_forward_lds(a.Pp, a.C, TR::kSplit);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_forward_kernel<TR>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
-  hipLaunchKernelGGL(rovs_forward_kernel, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s, a);
+  hipLaunchKernelGGL(rovs_forward_kernel<TR>, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s,
+                     a);
   return hipGetLastError();
+}
+
+hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s) {
+  // Split-bf16 products contract over Pp (an even count of 16-k chunks).
+  if (a.split && a.Pk != a.Pp) return hipErrorInvalidValue;
+  return a.split ? launch_rovs_forward_t<TraitsB3>(a, s) : launch_rovs_forward_t<TF>(a, s);
 }
 
 // Unit profit (computePathStates, SimulateGPU.cu:985-1002) with the ore
@@ -577,10 +639,10 @@ hipError_t launch_rovs_decide(const RovsStepArgs* a, int N, int D, int t, hipStr
 
 // ---- The recomputation (ALGO6) ---------------------------------------------
 
-int rovs_tile_col(int k) { return TF::perm(k); }
+int rovs_tile_col(int k, bool split) { return split ? TraitsB3::perm(k) : TF::perm(k); }
 
-size_t rovs_recompute_lds(int Pp, int C, int S) {
-  const int LDA = Pp + TF::PAD;
+size_t rovs_recompute_lds(int Pp, int C, int S, bool split) {
+  const int LDA = Pp + rovs_pad(split);
   const int nq = rovs_waves(Pp) * 64 / kRovsRows;
   return (size_t)S * kRovsRows * LDA * sizeof(float)  // the species' population tiles
          + (size_t)nq * C * kRovsRows * sizeof(float) // partial state sums
@@ -649,12 +711,13 @@ __device__ double rovs_policy_step(const RovsRecomputeArgs& a, const float* st, 
 // and the same Philox noise, so the replay reproduces the forward pass.  The
 // states of the decision, every control's total after one migration, are
 // n . r_c with r_c the row sums of trans o surv_c (fixed-order sums).
+template <class TR>
 __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRecomputeArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nthreads = blockDim.x, NW = nthreads >> 6, nq = nthreads / kRovsRows;
-  const int Pp = a.Pp, LDA = Pp + TF::PAD, C = a.C, S = a.S, N = a.N;
+  const int Pp = a.Pp, LDA = Pp + TR::PAD, C = a.C, S = a.S, N = a.N;
   const int p0 = blockIdx.x * kRovsRows;
   float* A0 = reinterpret_cast<float*>(smem);
   float* part = A0 + (size_t)S * kRovsRows * LDA;  // [nq][C][32]
@@ -663,11 +726,12 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
   int* ctl = reinterpret_cast<int*>(gm + kRovsRows);
 
   for (int sp = 0; sp < S; ++sp) {
-    const float* src = a.ckpt + sp * a.ckpt_species + (size_t)(a.y0 / a.ck) * N * Pp;
-    float* A = A0 + (size_t)sp * kRovsRows * LDA;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ckpt) + sp * a.ckpt_species +
+                          (size_t)(a.y0 / a.ck) * N * Pp;
+    const RovsTile<TR> tile{A0 + (size_t)sp * kRovsRows * LDA, LDA};
     for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
       const int r = idx / Pp, c = idx - r * Pp;
-      A[r * LDA + c] = (p0 + r < N) ? src[(size_t)(p0 + r) * Pp + c] : 0.f;
+      tile.put_raw(r, c, (p0 + r < N) ? src[(size_t)(p0 + r) * Pp + c] : 0u);
     }
   }
   const int ntc = Pp / TF::TN;
@@ -677,7 +741,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
   int tcol[kRovsNT];
 #pragma unroll
   for (int t = 0; t < kRovsNT; ++t) tcol[t] = my_t0 + t;
-  const int arow = TF::a_row(lane), akoff = TF::KV * TF::k_lane(lane);
+  const int arow = TR::a_row(lane), akoff = (TR::kSplit ? 8 : TR::KV) * TR::k_lane(lane);
   const int nkg = a.Pk >> 3;
   const size_t mbytes = (size_t)Pp * a.Pk * sizeof(float);
   double vacc = 0.0, fac = 1.0;  // the realized value, threads tid < 32
@@ -689,13 +753,13 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
       // sum over the strides.
       const int r = tid & (kRovsRows - 1), q = tid / kRovsRows;
       for (int sp = 0; sp < S; ++sp) {
-        const float* A = A0 + (size_t)sp * kRovsRows * LDA;
+        const RovsTile<TR> tile{A0 + (size_t)sp * kRovsRows * LDA, LDA};
         const float* rv = a.rvec + (size_t)sp * C * Pp;
         float accs[kRovsMaxC];
 #pragma unroll
         for (int c = 0; c < kRovsMaxC; ++c) accs[c] = 0.f;
         for (int i = q; i < Pp; i += nq) {
-          const float v = A[r * LDA + i];
+          const float v = tile.get(r, i);
 #pragma unroll
           for (int c = 0; c < kRovsMaxC; ++c)
             if (c < C) accs[c] += v * rv[(size_t)c * Pp + i];
@@ -750,9 +814,9 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
       for (int c = 0; c < C; ++c) {
         const PackedB B = packed_b(a.mpack + ((size_t)sp * C + c) * Pp * a.Pk, nkg, mbytes);
         if (my_n == 1)
-          rovs_product<1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+          rovs_product<TR, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
         else if (my_n == 2)
-          rovs_product<2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+          rovs_product<TR, 2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
 #pragma unroll
         for (int r = 0; r < TF::NACC; ++r) {
           const int row = TF::acc_row(laney, r);
@@ -771,11 +835,12 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
       const float* K = a.cols + (size_t)sp * 3 * Pp;
       const float* Kinv = K + Pp;
       const float lv = a.lv[sp];
+      const RovsTile<TR> te{A, LDAe};
 #pragma unroll
       for (int t = 0; t < kRovsNT; ++t) {
         if (t >= my_n) break;
         const int col = tcol[t] * TF::TN + TF::acc_col(lanee);
-        const int pcol = TF::perm(col);
+        const int pcol = TR::perm(col);
         const float Kc = K[col], Ki = Kinv[col];
 #pragma unroll
         for (int r4 = 0; r4 < TF::NACC; r4 += 4) {
@@ -788,7 +853,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
             const int row = TF::acc_row(lanee, r4 + s4);
             const float x = sel[t][r4 + s4];
             const float g = gm[row] + z[s4] * lv;
-            A[row * LDAe + pcol] = x * (1.f + g * (Kc - x) * Ki * 0.01f);
+            te.put(row, pcol, x * (1.f + g * (Kc - x) * Ki * 0.01f));
           }
         }
       }
@@ -798,17 +863,24 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRec
   if (tid < kRovsRows && p0 + tid < N) a.V[(size_t)a.t * N + p0 + tid] = vacc;
 }
 
-hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s) {
-  if (a.D < 2 || a.D > 8 || a.S != a.D - 1 || a.C < 1 || a.C > kRovsMaxC || a.Pp % TF::TN ||
-      a.Pp > kRovsMaxPatches || a.t < 1 || a.t > a.T - 2 || a.y0 > a.t || a.y0 % a.ck)
-    return hipErrorInvalidValue;
-  const size_t lds = rovs_recompute_lds(a.Pp, a.C, a.S);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_recompute_kernel),
+template <class TR>
+static hipError_t launch_rovs_recompute_t(const RovsRecomputeArgs& a, hipStream_t s) {
+  const size_t lds = rovs_recompute_lds(a.Pp, a.C, a.S, TR::kSplit);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_recompute_kernel<TR>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
-  hipLaunchKernelGGL(rovs_recompute_kernel, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s, a);
+  hipLaunchKernelGGL(rovs_recompute_kernel<TR>, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds,
+                     s, a);
   return hipGetLastError();
+}
+
+hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s) {
+  if (a.D < 2 || a.D > 8 || a.S != a.D - 1 || a.C < 1 || a.C > kRovsMaxC || a.Pp % TF::TN ||
+      a.Pp > kRovsMaxPatches || a.t < 1 || a.t > a.T - 2 || a.y0 > a.t || a.y0 % a.ck ||
+      (a.split && a.Pk != a.Pp))
+    return hipErrorInvalidValue;
+  return a.split ? launch_rovs_recompute_t<TraitsB3>(a, s) : launch_rovs_recompute_t<TF>(a, s);
 }
 
 // Time 0 (firstPeriodInduction, SimulateGPU.cu:2210-2330): per control, the

@@ -203,6 +203,7 @@ class RovsProblem(C.Structure):
         ("road_id", C.c_uint32),
         ("method", C.c_int32),
         ("checkpoint_years", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 

@@ -173,7 +173,7 @@ def knn(n: int, dims: int, res: int, repeat: int) -> None:
 
 
 def rovs(patches: int, species: int, controls: int, paths: int, years: int, res: int,
-         repeat: int, method: int = 4) -> None:
+         repeat: int, method: int = 4, precision: str = "fp32") -> None:
     """Species-state ROV (ALGO4 or ALGO6) on the synthetic landscape, thresholds at 0.7
     of the initial total (every control has in-the-money paths for the first
     half of the horizon at P = 256)."""
@@ -181,16 +181,18 @@ def rovs(patches: int, species: int, controls: int, paths: int, years: int, res:
     import time
 
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import native as nat
     from transportoptimiser_amd import rov_species as rs
     from transportoptimiser_amd.api import Context
     ctx = Context(0)
     pr = rs.synthetic_problem(seed=5, n_patches=patches, n_species=species, n_controls=controls,
                               n_paths=paths, n_years=years, dim_res=res, threshold_ratio=0.7)
     pr.method = method
+    pr.precision = nat.PREC_BF16X3 if precision == "bf16x3" else nat.PREC_FP32
     for i in range(repeat):
         t = time.perf_counter()
         r = rs.simulate_rov_species(ctx, pr)
-        print(f"rovs call {i}: ALGO{method} P={patches} S={species} C={controls} N={paths} T={years} "
+        print(f"rovs call {i}: ALGO{method} {precision} P={patches} S={species} C={controls} N={paths} T={years} "
               f"R={res}: {(time.perf_counter() - t) * 1e3:.2f} ms, value {r['value']:.4f}, "
               f"control {r['first_control']}", flush=True)
     ctx.close()
@@ -296,6 +298,7 @@ def main() -> int:
     rv2.add_argument("--res", type=int, default=20)
     rv2.add_argument("--repeat", type=int, default=3)
     rv2.add_argument("--method", type=int, default=4, choices=(4, 6))
+    rv2.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -316,7 +319,8 @@ def main() -> int:
     elif a.cmd == "rov":
         rov(a.repeat)
     elif a.cmd == "rovs":
-        rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat, a.method)
+        rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat, a.method,
+             a.precision)
     elif a.cmd == "knn":
         knn(a.samples, a.dims, a.res, a.repeat)
     elif a.cmd == "road":

@@ -63,6 +63,7 @@ class RovsProblem:
     road_id: int = 0
     method: int = 4                    # 4: ALGO4; 6: ALGO6 (forward recomputation)
     checkpoint_years: int = 0          # ALGO6: forward populations kept every k years (0: auto)
+    precision: int = 0                 # products: PREC_FP32 (0) or PREC_BF16X3 (2)
 
     @property
     def n_patches(self) -> int:
@@ -113,6 +114,7 @@ class _Marshal:
         p.profit_scaling, p.hysteresis = pr.profit_scaling, pr.hysteresis
         p.seed, p.road_id, p.method = pr.seed, pr.road_id, pr.method
         p.checkpoint_years = pr.checkpoint_years
+        p.precision = pr.precision
         self.problem = p
 
     def _arr(self, a):
