@@ -3,9 +3,7 @@
 
 Workload (one "step"): a generation of 512 candidate roads, patch counts
 log-uniform in [64, 1024], 2 species, 50 years, 20,000 Monte Carlo paths per
-road, synthetic landscapes.  The products run as split-bf16 on the bf16
-matrix cores (x = hi + lo, x.m = hi.m_hi + hi.m_lo + lo.m_hi, FP32
-accumulation; --precision fp32 selects FP32 MFMA products instead).  The
+road, synthetic landscapes, FP32 arithmetic (FP32 MFMA products).  The
 batched path evaluates the whole generation in one replay of a captured HIP
 graph around a persistent kernel; inputs are resident in HBM when the timed
 region starts.
@@ -17,10 +15,12 @@ Reported beside the headline value:
   single   the tiled kernel called once per road through roadmc_mte_run;
   accuracy max |dMTE| of the measured generation against the FP64 kernels;
   roofline algorithmic flops 2 N P^2 T S of the persistent kernel over its
-           event-timed duration, against the peak of the matrix cores used
-           (bf16 dense 2.5 PFLOP/s; each algorithmic product is three bf16
-           products; FP32 157.3 TFLOP/s with --precision fp32);
-  fp32_products the same generation with FP32 MFMA products;
+           event-timed duration, against the FP32 MFMA peak (157.3 TFLOP/s;
+           with --precision bf16x3 against the dense bf16 peak, 2.5 PFLOP/s,
+           each algorithmic product being three bf16 products);
+  bf16x3_products the same generation with split-bf16 products (x = hi + lo,
+           x.m = hi.m_hi + hi.m_lo + lo.m_hi on the bf16 matrix cores, FP32
+           accumulation) and its error against the FP64 kernels;
   cpu_baseline the CPU oracle (FP64, OpenMP over paths) on a bounded sample.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -274,10 +274,10 @@ def main() -> int:
     ap.add_argument("--years", type=int, default=50)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--autotune", action="store_true", help="tune tile rows first")
-    ap.add_argument("--precision", choices=["bf16x3", "fp32"], default="bf16x3",
+    ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32",
                     help="products of the measured generation")
-    ap.add_argument("--no-fp32", action="store_true",
-                    help="skip the FP32-products comparison generation")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the generation with the other products")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -338,6 +338,7 @@ def main() -> int:
     batch.close()
 
     accuracy = None
+    out64 = None
     if not args.no_fp64:
         t = time.perf_counter()
         for r in roads:
@@ -420,27 +421,45 @@ def main() -> int:
         cpu["config1_road_ms"] = round((time.perf_counter() - t) * 1e3, 1)
         log(f"cpu baseline: {cpu}")
 
-    fp32 = None
-    if not args.no_fp32 and prec != nat.PREC_FP32:
-        r32 = [dataclasses.replace(r, precision=nat.PREC_FP32) for r in roads]
-        with ctx.prepare_batch(r32) as b32:
-            b32.launch()
-            b32.sync()
-            b32.kernel_ms()
+    # The same generation with the other products: split bf16 beside the FP32
+    # headline (or FP32 beside a split-bf16 run), with its error against the
+    # FP64 kernels.
+    other = None
+    if not args.no_other:
+        oprec = nat.PREC_FP32 if prec == nat.PREC_BF16X3 else nat.PREC_BF16X3
+        ro = [dataclasses.replace(r, precision=oprec) for r in roads]
+        with ctx.prepare_batch(ro) as bo:
+            bo.launch()
+            bo.sync()
+            bo.kernel_ms()
             t = time.perf_counter()
             for _ in range(2):
-                b32.launch()
-            b32.sync()
-            ms32 = (time.perf_counter() - t) * 1e3 / 2
-            k32 = b32.kernel_ms()
-            o32 = b32.fetch(end_pops=False, patch_mean=False)
-        d32 = max(float(np.max(np.abs(a.mte_prob - c.mte_prob))) for a, c in zip(out32, o32))
-        fp32 = {"ms_per_generation": round(ms32, 1), "evals_per_s": round(len(roads) / ms32 * 1e3, 2),
-                "tflops": round(flops / k32 / 1e9, 1),
-                "frac_fp32_mfma": round(flops / k32 / 1e9 / FP32_MFMA_PEAK_TFLOPS, 4),
-                "max_abs_dMTE_vs_measured": d32,
-                "what": "the same generation with FP32 MFMA products (v_mfma_f32_32x32x2_f32)"}
-        log(f"fp32 products: {fp32}")
+                bo.launch()
+            bo.sync()
+            mso = (time.perf_counter() - t) * 1e3 / 2
+            ko = bo.kernel_ms()
+            oo = bo.fetch(end_pops=False, patch_mean=False)
+        other = {"ms_per_generation": round(mso, 1),
+                 "evals_per_s": round(len(roads) / mso * 1e3, 2),
+                 "tflops_algorithmic": round(flops / ko / 1e9, 1),
+                 "max_abs_dMTE_vs_measured": max(float(np.max(np.abs(a.mte_prob - c.mte_prob)))
+                                                 for a, c in zip(out32, oo))}
+        if out64 is not None:
+            do = np.array([np.max(np.abs(a.mte_prob - b.mte_prob)) for a, b in zip(oo, out64)])
+            dmo = np.array([np.max(np.abs(a.end_mean - b.end_mean) / b.end_mean)
+                            for a, b in zip(oo, out64)])
+            other["max_abs_dMTE_vs_fp64"] = float(do.max())
+            other["max_rel_dmean_vs_fp64"] = float(dmo.max())
+        if oprec == nat.PREC_BF16X3:
+            other["frac_bf16_mfma_executed"] = round(3 * flops / ko / 1e9 / BF16_MFMA_PEAK_TFLOPS, 4)
+            other["what"] = ("the same generation with split-bf16 products (x = hi + lo, "
+                             "hi.m_hi + hi.m_lo + lo.m_hi on v_mfma_f32_32x32x16_bf16, FP32 "
+                             "accumulation); per-path end populations within 1e-4 of the FP64 "
+                             "oracle in the GPU tests")
+        else:
+            other["frac_fp32_mfma"] = round(flops / ko / 1e9 / FP32_MFMA_PEAK_TFLOPS, 4)
+            other["what"] = "the same generation with FP32 MFMA products (v_mfma_f32_32x32x2_f32)"
+        log(f"other products: {other}")
 
     extras = None if args.no_extras else run_extras(ctx, roads, prep_ms)
 
@@ -477,7 +496,7 @@ def main() -> int:
         "plain": plain,
         "single": single,
         "accuracy": accuracy,
-        "fp32_products": fp32,
+        "bf16x3_products" if prec == nat.PREC_FP32 else "fp32_products": other,
         "extras": extras,
     }
     print(json.dumps(line), flush=True)
