@@ -202,8 +202,9 @@ def test_algo6_limits(ctx):
     assert e.value.code == nat.ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("P,S", [(512, 2), (256, 4), (33, 7)])
-def test_algo6_realized_recursion_at_the_tile_limits(ctx, P, S):
+def test_algo6_realized_recursion_at_the_tile_limits(ctx, P, S, prec):
     # The largest species tiles one workgroup holds (2 x 512, 4 x 256 patches)
     # and the most species (7, at a ragged patch count): without mortality
     # the realized recursion holds path by path.
@@ -211,7 +212,7 @@ def test_algo6_realized_recursion_at_the_tile_limits(ctx, P, S):
     T = 6
     pr = _stochastic_problem(T=T, N=333, P=P)
     pr.species = [pr.species[0]] * S
-    pr.method = 6
+    pr.method, pr.precision = 6, prec
     pr.dim_res = 3
     g = _run(ctx, pr)
     V, oc, up = g["cond_exp"], g["opt_control"], g["unit_profit"]
