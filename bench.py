@@ -232,10 +232,10 @@ def run_extras(ctx, roads, prep_ms) -> dict:
         "recompute_tflop": round(fl6 / 1e12, 2), "tflops_whole_call": round(fl6 / ms / 1e9, 1),
         "what": "species-state ROV with forward recomputation (ALGO6), same problem; host call"}
     log(f"rov species ALGO6: {out['rov_species_algo6']}")
-    # The same two valuations with split-bf16 products in the forward pass and
-    # the recomputation.
+    # ALGO6 with split-bf16 products in the forward pass and the recomputation
+    # (ALGO4 runs FP32 products only).
     from transportoptimiser_amd import native as nat
-    for method in (4, 6):
+    for method in (6,):
         pr5.method, pr5.precision = method, nat.PREC_BF16X3
         rsp.simulate_rov_species(ctx, pr5)
         t = time.perf_counter()

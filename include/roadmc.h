@@ -267,7 +267,9 @@ void roadmc_adjoint_destroy(roadmc_adjoint* adj);
 #define ROADMC_ROV_SWITCHING 0 /* flow control per step (backwardInduction,
                                   SimulateGPU.cu:2333-2784) */
 #define ROADMC_ROV_AMERICAN 1  /* optimal stopping of a put on uncertainty 0
-                                  (Longstaff and Schwartz 2001) */
+                                  (Longstaff and Schwartz 2001); realized cash
+                                  flows only: ROADMC_VALUE_REGRESSED returns
+                                  ROADMC_ERR_UNSUPPORTED */
 
 #define ROADMC_VALUE_REGRESSED 0 /* carry the regressed value back
                                     (SimulateGPU.cu:2757-2771) */
@@ -432,8 +434,9 @@ typedef struct {
                                              the last one (0: every year while they fit
                                              in 16 GiB) */
   int32_t precision;                      /* products of the forward pass and the
-                                             recomputation: ROADMC_PREC_FP32 or
-                                             ROADMC_PREC_BF16X3 */
+                                             recomputation: ROADMC_PREC_FP32, or
+                                             ROADMC_PREC_BF16X3 with ALGO6 only
+                                             (ALGO4 returns ROADMC_ERR_UNSUPPORTED) */
 } roadmc_rovs_problem;
 
 typedef struct {
@@ -622,14 +625,17 @@ int roadmc_mlp_predict(roadmc_mlp* net, const float* x, int32_t n, float* out);
  * (neighbours that do not span the dimensions) the value is the weighted
  * mean; with fewer than 3 samples every value is 0, as the reference. */
 #define ROADMC_KNN_MAX_DIMS 8
-#define ROADMC_KNN_MAX_K 64
+#define ROADMC_KNN_MAX_K 128
 #define ROADMC_KNN_NORMALIZE 1   /* neighbours and fit in [0,1]-scaled coordinates
                                     (buildSurrogateROVCUDA, :5807-5822) */
 #define ROADMC_KNN_NONPOSITIVE 2 /* clamp fitted values to <= 0 (rovCorrection) */
 #define ROADMC_KNN_RULE_MTE 0    /* neighbour count of buildSurrogateMTECUDA (:5545-5554) */
 #define ROADMC_KNN_RULE_ROV 1    /* neighbour count of buildSurrogateROVCUDA (:5711-5724) */
 
-/* The reference's neighbour count for n_samples samples (at most n_samples). */
+/* The reference's neighbour count for n_samples samples, at most n_samples and
+ * at most ROADMC_KNN_MAX_K.  The reference has no cap: its ROV rule
+ * (25 + ceil(0.02 (n - 500))) passes 128 above 5 650 samples, where this
+ * returns 128. */
 int roadmc_knn_k(int32_t n_samples, int32_t rule, int32_t* k);
 /* x: [n_samples][n_dims] row-major, y: [n_samples]; surrogate:
  * [n_dims*dim_res + dim_res^n_dims].  dim_res >= 2, dim_res^n_dims <= 2^24,

@@ -110,6 +110,11 @@ int orc_rov_forward(int32_t n_unc, const double* par, const double* chol,
                     int32_t n_controls, int32_t n_steps, double dt,
                     int64_t n_paths, uint64_t seed, double* states,
                     int32_t* controls);
+/* Expected PV of an uncontrolled MRJD price (SimulateGPU::expPV,
+ * Uncertainty::singlePathValue); values [n_paths] (may be NULL: then sd is
+ * NaN). */
+int orc_exp_pv(const double* par, int32_t n_steps, double dt, double rrr, double growth,
+               int64_t n_paths, uint64_t seed, double* values, double* mean, double* sd);
 
 typedef struct {
   int32_t n_unc, n_steps, n_controls, deg, mode, value_mode, pooled, itm_only,

@@ -182,6 +182,9 @@ def _rov_lib():
         h.orc_rov_backward.restype = C.c_int
         h.orc_solve.argtypes = [C.c_int, _dp, _dp, _dp]
         h.orc_solve.restype = C.c_int
+        h.orc_exp_pv.argtypes = [_dp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                 C.c_uint64, _dp, _dp, _dp]
+        h.orc_exp_pv.restype = C.c_int
         h._rov_ready = True
     return h
 
@@ -218,6 +221,20 @@ def rov_forward(pr):
     if rc:
         raise RuntimeError("orc_rov_forward failed")
     return states, ctrl
+
+
+def exp_pv(u, n_steps, time_step, rrr, growth, n_paths, seed):
+    """Expected PV of one uncontrolled MRJD price on the expPV stream:
+    returns (per-path values (N,), mean, SD with 1/N)."""
+    par = np.array([u.kind, u.x0, u.mean, u.sd, u.reversion, u.jump_size, u.jump_prob],
+                   dtype=np.float64)
+    vals = np.zeros(n_paths)
+    m, sd = C.c_double(), C.c_double()
+    rc = _rov_lib().orc_exp_pv(par.ctypes.data_as(_dp), n_steps, time_step, rrr, growth,
+                               n_paths, seed, vals.ctypes.data_as(_dp), C.byref(m), C.byref(sd))
+    if rc:
+        raise RuntimeError("orc_exp_pv failed")
+    return vals, m.value, sd.value
 
 
 def rov_backward(pr, states, controls):
@@ -358,7 +375,26 @@ def _knn_lib():
     h.orc_knn_fit.restype = C.c_int
     h.orc_knn_eval.argtypes = [C.c_int32, C.c_int32, _dp, C.c_int32, _dp, _dp]
     h.orc_knn_eval.restype = None
+    h.orc_knn_fit_scaled.argtypes = [C.c_int32, C.c_int32, _dp, _dp, C.c_int32, C.c_int32, _dp,
+                                     _dp]
+    h.orc_knn_fit_scaled.restype = C.c_int
     return h
+
+
+def knn_fit_scaled(u, y, dim_res: int, k: int, qtab) -> np.ndarray:
+    """Grid values [dim_res^d] fitted at the query coordinates qtab [d][dim_res]
+    from samples u [n][d] already in the search coordinates."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    q = np.ascontiguousarray(qtab, dtype=np.float64)
+    n, d = u.shape
+    out = np.zeros(dim_res ** d)
+    rc = _knn_lib().orc_knn_fit_scaled(n, d, u.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                       dim_res, k, q.ctypes.data_as(_dp),
+                                       out.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError(f"orc_knn_fit_scaled failed: {rc}")
+    return out
 
 
 def knn_k(n: int, rule: int) -> int:

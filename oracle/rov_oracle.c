@@ -115,6 +115,52 @@ int orc_rov_forward(int32_t n_unc, const double* par /* [U][7] */,
   return 0;
 }
 
+/* Expected present value of one uncontrolled MRJD price: SimulateGPU::expPV
+ * with its kernel expPVPath (SimulateGPU.cu:256-279, 3163-3259) and the CPU
+ * form Uncertainty::singlePathValue (Uncertainty.cpp:83-112).  Per path p and
+ * step t: x += k(m - x)dt + x b + (e^J - 1) x 1[u < prob], b ~ N(0, sd dt)
+ * (the Brownian SD the GPU driver passes, :3222-3225), J ~ N(-J0^2/2, J0^4),
+ * then value += (1 + growth)^t x / (1 + rrr)^t.  The draws are indexed per
+ * (path, step) (the reference's kernel reads [idx + ii], so its paths share
+ * draws) on the Philox stream {p, t, 0, tag} (normals 0, 1) and {p, t, 4, tag}
+ * (the jump uniform, word 0), tag = (1 << 8) | 2.  The SD is over paths with
+ * 1/N (the reference omits the 1/N, Uncertainty.cpp:75, SimulateGPU.cu:3248).
+ * par = {kind, x0, mean, sd, reversion, jump size, jump prob}. */
+int orc_exp_pv(const double* par, int32_t n_steps, double dt, double rrr, double growth,
+               int64_t n_paths, uint64_t seed, double* values, double* mean, double* sd) {
+  const uint32_t tag = (1u << 8) | (0u << 2) | 2u;
+  const double J = par[5];
+  int64_t p;
+  double s1 = 0.0, s2 = 0.0, m;
+  if (n_steps < 1 || n_paths < 1) return 1;
+  for (p = 0; p < n_paths; ++p) {
+    double curr = par[1], value = 0.0;
+    int t;
+    for (t = 0; t < n_steps; ++t) {
+      double z[4];
+      uint32_t r[4];
+      normal4(seed, (uint32_t)p, (uint32_t)t, 0u, tag, z);
+      words(seed, (uint32_t)p, (uint32_t)t, 4u, tag, r);
+      {
+        const double brown = z[0] * par[3] * dt;
+        const double jsz = z[1] * J * J - J * J / 2.0;
+        const double jumped = u01(r[0]) < par[6] ? 1.0 : 0.0;
+        curr += par[4] * (par[2] - curr) * dt + curr * brown + (exp(jsz) - 1.0) * curr * jumped;
+      }
+      value += pow(1.0 + growth, t) * curr / pow(1.0 + rrr, t);
+    }
+    if (values) values[p] = value;
+    s1 += value;
+  }
+  m = s1 / (double)n_paths;
+  if (values) {
+    for (p = 0; p < n_paths; ++p) s2 += (values[p] - m) * (values[p] - m);
+  }
+  *mean = m;
+  *sd = values ? sqrt(s2 / (double)n_paths) : NAN;
+  return 0;
+}
+
 static int basis(const double* x, int U, int deg, double* phi) {
   int k = 0, i, j, l;
   phi[k++] = 1.0;

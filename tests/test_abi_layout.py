@@ -70,3 +70,28 @@ def test_struct_layouts_match_header(tmp_path):
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is not installed")
 def test_oracle_struct_layouts_match_header(tmp_path):
     _check(tmp_path, "oracle.h", REPO / "oracle", _oracle_pairs())
+
+
+def test_header_constants_match_python():
+    """Every integer #define ROADMC_<NAME> of the header that the Python mirror
+    also names (as <NAME>) has the same value there."""
+    import re
+    text = (REPO / "include" / "roadmc.h").read_text()
+    seen = 0
+    for name, val in re.findall(r"^#define ROADMC_(\w+)\s+(-?\d+)\b", text, re.M):
+        if hasattr(nat, name):
+            assert getattr(nat, name) == int(val), name
+            seen += 1
+    assert seen >= 20
+
+
+def test_neighbour_count_is_capped_at_the_header_limit():
+    """roadmc_knn_k never returns more than ROADMC_KNN_MAX_K, the limit
+    roadmc_knn_surrogate_fit enforces (no GPU needed for the count)."""
+    import re
+    text = (REPO / "include" / "roadmc.h").read_text()
+    cap = int(re.search(r"^#define ROADMC_KNN_MAX_K\s+(\d+)", text, re.M).group(1))
+    from transportoptimiser_amd import knn
+    assert knn.neighbour_count(10000, nat.KNN_RULE_ROV) == cap
+    assert knn.neighbour_count(5650, nat.KNN_RULE_ROV) == 128 == cap
+    assert knn.neighbour_count(4096, nat.KNN_RULE_ROV) == 97

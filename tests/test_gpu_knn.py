@@ -104,9 +104,28 @@ def test_eval_matches_oracle(ctx, oracle, d):
 
 
 def test_neighbour_rules_match_oracle(oracle):
-    for n in list(range(0, 700, 7)) + [4096, 5650, 10000]:
+    # The library caps the reference's rule at ROADMC_KNN_MAX_K (128); the ROV
+    # rule passes it above 5 650 samples (5 651 -> 25 + ceil(103.02) = 129).
+    for n in list(range(0, 700, 7)) + [4096, 5650, 5651, 10000]:
         for rule in (nat.KNN_RULE_MTE, nat.KNN_RULE_ROV):
-            assert knn.neighbour_count(n, rule) == oracle.knn_k(n, rule)
+            assert knn.neighbour_count(n, rule) == min(oracle.knn_k(n, rule), nat.KNN_MAX_K)
+    assert oracle.knn_k(5651, nat.KNN_RULE_ROV) == 129
+    assert knn.neighbour_count(5651, nat.KNN_RULE_ROV) == 128
+
+
+def test_rov_surrogate_beyond_the_neighbour_cap(ctx, oracle):
+    # 6 000 samples: the reference's ROV rule asks for 136 neighbours, the
+    # library fits with 128 (the radix-select kernel, n > 2 048).
+    rng = np.random.default_rng(81)
+    n = 6000
+    iars = rng.uniform(0.0, 0.3, size=(n, 2))
+    cost = rng.uniform(0, 0.4, n)
+    vals = -1000 * cost * (1 + iars.sum(1)) + 5 * rng.normal(size=n)
+    assert oracle.knn_k(n, nat.KNN_RULE_ROV) == 136
+    mean, _ = knn.build_surrogate_rov(ctx, iars, cost, vals, np.abs(vals) * 0.1, dim_res=8)
+    x = np.column_stack([iars, cost])
+    ref = oracle.knn_fit(x, vals, 8, nat.KNN_MAX_K, nat.KNN_NORMALIZE | nat.KNN_NONPOSITIVE)
+    np.testing.assert_allclose(mean.surrogate, ref, rtol=1e-9, atol=1e-12)
 
 
 def test_invalid_arguments(ctx):

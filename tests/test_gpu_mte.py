@@ -61,7 +61,7 @@ def test_tiled_fp32_matches_oracle(ctx, oracle, P, N, T, S, law, endpoint, tol):
     _check_against_oracle(out, ref, pr, rtol=1e-4)
 
 
-@pytest.mark.parametrize("P,N,T,S,law,endpoint,tol", CASES[:7])
+@pytest.mark.parametrize("P,N,T,S,law,endpoint,tol", CASES)
 def test_tiled_fp64_matches_oracle(ctx, oracle, P, N, T, S, law, endpoint, tol):
     pr = _problem(P, N, T, S, law, endpoint, drop_tol=tol, prec=nat.PREC_FP64)
     out = ctx.simulate_mte(pr)
@@ -209,3 +209,81 @@ def test_injected_allocation_failure(ctx):
         # The context stays usable after the failure.
         ok = fresh.simulate_mte(pr)
         assert np.all(np.isfinite(ok.end_pops))
+
+
+# ---- thresholds that give the MTE probability something to decide ----------
+
+def _informative_thresholds(pr, ref, q=0.5):
+    """Sets each species' threshold ratio to the q-quantile of the oracle's
+    end totals over the initial total, so that the MTE probability sits
+    near q instead of at 0 or 1."""
+    for s, sd in enumerate(pr.species):
+        sd.threshold_ratio = float(np.quantile(ref["end_pops"][:, s], q) / sd.init_pop.sum())
+        ref["mte_prob"][s] = np.mean(ref["end_pops"][:, s] >=
+                                     sd.threshold_ratio * sd.init_pop.sum())
+
+
+@pytest.mark.parametrize("prec,rtol", [(nat.PREC_FP32, 1e-4), (nat.PREC_BF16X3, 1e-4),
+                                       (nat.PREC_FP64, 1e-10)])
+def test_batch_of_a_generation_matches_oracle(ctx, oracle, prec, rtol):
+    # Eight roads spanning the generation's patch counts in one persistent
+    # launch, against the oracle directly; thresholds at the median end total
+    # of each road and species, so that every MTE probability is near 0.5.
+    Ps = [64, 100, 160, 256, 400, 640, 800, 1024]
+    roads = [_problem(P, 600, 10, 2, seed=31, road_id=r, prec=prec) for r, P in enumerate(Ps)]
+    refs = [oracle.mte_problem(r) for r in roads]
+    for r, ref in zip(roads, refs):
+        _informative_thresholds(r, ref, 0.5)
+    print(f"batch of {len(roads)} roads P={Ps} prec={prec}", flush=True)
+    outs = ctx.simulate_mte_batch(roads, end_pops=True, patch_mean=True)
+    for pr, out, ref in zip(roads, outs, refs):
+        assert (0.2 < ref["mte_prob"]).all() and (ref["mte_prob"] < 0.8).all()
+        _check_against_oracle(out, ref, pr, rtol=rtol)
+        assert np.max(np.abs(out.mte_prob - ref["mte_prob"])) <= 2e-3
+
+
+# ---- the large single road: 3 species x 256 patches, 100 years -------------
+
+@pytest.mark.parametrize("prec", [nat.PREC_FP32, nat.PREC_BF16X3])
+def test_config2_shape_matches_oracle(ctx, oracle, prec):
+    # The per-species loop of simulateMTECUDA (SimulateGPU.cu:3680-3843) at the
+    # large road's shape (l = 1.5 km, mortality U(0, 0.4), theta = 0.3) on
+    # 1 500 paths; then again with thresholds at the median end total.
+    from transportoptimiser_amd.landscape import config2
+    pr = config2(n_paths=1500)
+    pr.precision = prec
+    print(f"config2 shape, 1500 paths, prec={prec}", flush=True)
+    out = ctx.simulate_mte(pr)
+    ref = oracle.mte_problem(pr)
+    _check_against_oracle(out, ref, pr, rtol=1e-4)
+    assert np.max(np.abs(out.mte_prob - ref["mte_prob"])) <= 2e-3
+    _informative_thresholds(pr, ref, 0.5)
+    out = ctx.simulate_mte(pr)
+    assert (0.2 < ref["mte_prob"]).all() and (ref["mte_prob"] < 0.8).all()
+    _check_against_oracle(out, ref, pr, rtol=1e-4)
+    assert np.max(np.abs(out.mte_prob - ref["mte_prob"])) <= 2e-3
+
+
+def test_config2_full_size(ctx):
+    # 100 000 paths: FP32 and split-bf16 products against the FP64 kernels
+    # (themselves oracle-checked above), at theta = 0.3 and at the median.
+    from transportoptimiser_amd.landscape import config2
+    pr = config2()
+    res = {}
+    for prec in (nat.PREC_FP64, nat.PREC_FP32, nat.PREC_BF16X3):
+        pr.precision = prec
+        print(f"config2 full size prec={prec}", flush=True)
+        res[prec] = ctx.simulate_mte(pr, end_pops=True)
+    ref = res[nat.PREC_FP64]
+    assert np.isfinite(ref.end_pops).all()
+    for prec in (nat.PREC_FP32, nat.PREC_BF16X3):
+        o = res[prec]
+        assert np.isfinite(o.end_pops).all() and ((0 <= o.mte_prob) & (o.mte_prob <= 1)).all()
+        err = np.max(np.abs(o.end_pops - ref.end_pops) / ref.end_pops)
+        print(f"  prec={prec}: max rel err {err:.2e}, MTE {o.mte_prob}", flush=True)
+        assert err <= 1e-4
+        assert np.max(np.abs(o.mte_prob - ref.mte_prob)) <= 2e-3
+        for s, sd in enumerate(pr.species):
+            thr = np.median(ref.end_pops[:, s])
+            d = abs(np.mean(o.end_pops[:, s] >= thr) - np.mean(ref.end_pops[:, s] >= thr))
+            assert d <= 2e-3

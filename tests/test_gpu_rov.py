@@ -77,15 +77,25 @@ def test_backward_induction_matches_oracle(ctx, oracle, kw):
     assert same.mean() > 0.999
 
 
-def test_config5_full_size(ctx):
+def test_config5_full_size(ctx, oracle):
+    # The full configuration (100 000 paths, 50 steps, cubic basis in 3
+    # variables) against the oracle on the same streams.
     pr = config5()
     t = time.perf_counter()
-    r = simulate_rov(ctx, pr)
+    r = simulate_rov(ctx, pr, want_paths=True)
     dt = time.perf_counter() - t
     print(f"config5: value {r['value']:.3f} sd {r['value_sd']:.3f} control {r['first_control']}"
           f" in {dt * 1e3:.1f} ms", flush=True)
     assert np.isfinite(r["value"]) and r["value"] <= 0.0
     assert r["coeffs"].shape == (50, 2, 20)
+    st, ctrl = oracle.rov_forward(pr)
+    np.testing.assert_allclose(r["states"], st, rtol=1e-10, atol=1e-12)
+    ref = oracle.rov_backward(pr, st, ctrl)
+    assert r["value"] == pytest.approx(ref["value"], rel=1e-8)
+    assert r["first_control"] == ref["first_control"]
+    assert r["value_sd"] == pytest.approx(ref["value_sd"], rel=1e-6)
+    np.testing.assert_allclose(r["coeffs"][1:], ref["beta"][1:], rtol=1e-6, atol=1e-8)
+    assert np.isclose(r["cond_exp"][1:], ref["V"][1:], rtol=1e-8, atol=1e-10).mean() > 0.999
 
 
 def test_exp_pv_matches_moment_recursion(ctx):
@@ -100,6 +110,28 @@ def test_exp_pv_matches_moment_recursion(ctx):
         e = e + 0.25 * (1.4 - e) * dt + e * 0.1 * (ej - 1)
         pv += (1 + g) ** t * e / (1 + r_) ** t
     assert mean == pytest.approx(pv, abs=4 * sd / np.sqrt(200000))
+
+
+@pytest.mark.parametrize("u,T,dt,r_,g", [
+    (Uncertainty(nat.SDE_MRJD, 1.0, 1.4, 0.2, 0.25, 0.3, 0.1), 30, 1.0, 0.07, 0.01),
+    (Uncertainty(nat.SDE_MRJD, 2.5, 1.0, 0.1, 0.6, 0.5, 0.3), 12, 0.5, 0.04, 0.0),
+])
+def test_exp_pv_matches_oracle(ctx, oracle, u, T, dt, r_, g):
+    # The same Philox draws in FP64 on both sides: mean and SD (1/N) to 1e-10.
+    N = 100000
+    mean, sd = exp_pv(ctx, u, T, dt, r_, g, N, 21)
+    _, m_ref, sd_ref = oracle.exp_pv(u, T, dt, r_, g, N, 21)
+    assert mean == pytest.approx(m_ref, rel=1e-10)
+    assert sd == pytest.approx(sd_ref, rel=1e-10)
+
+
+def test_american_refuses_regressed_values(ctx):
+    pr = RovProblem([Uncertainty(nat.SDE_GBM, 1.0, 0.0, 0.2)], n_steps=3, n_paths=100,
+                    mode=nat.ROV_AMERICAN, value_mode=nat.VALUE_REGRESSED, strike=1.1,
+                    basis_degree=2)
+    with pytest.raises(nat.RoadmcError) as e:
+        simulate_rov(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED
 
 
 def test_rov_invalid_inputs(ctx):

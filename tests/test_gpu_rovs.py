@@ -26,17 +26,15 @@ def _run(ctx, pr):
 PRECISIONS = [nat.PREC_FP32, nat.PREC_BF16X3]
 
 
-@pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("P,S,C,N,T,R", [
     (24, 1, 3, 2000, 12, 8),
     (40, 2, 2, 1500, 10, 6),
     (64, 1, 4, 3001, 15, 10),
 ])
-def test_matches_oracle(ctx, oracle, P, S, C, N, T, R, prec):
-    # Split-bf16 products (PREC_BF16X3) hold the forward within the FP32 bar.
+def test_matches_oracle(ctx, oracle, P, S, C, N, T, R):
+    # ALGO4 runs FP32 forward products (split-bf16 is refused below).
     pr = rs.synthetic_problem(seed=P + S, n_patches=P, n_species=S, n_controls=C, n_paths=N,
                               n_years=T, dim_res=R)
-    pr.precision = prec
     g = _run(ctx, pr)
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     np.testing.assert_array_equal(g["controls"], ctl)
@@ -76,12 +74,23 @@ def test_unreachable_thresholds_keep_zero_flow(ctx):
     assert (g["opt_control"] == 0).all()
 
 
+def test_algo4_refuses_split_bf16(ctx):
+    pr = rs.synthetic_problem(seed=3, n_patches=24, n_paths=200, n_years=5, dim_res=4)
+    pr.precision = nat.PREC_BF16X3
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED
+
+
 @pytest.mark.parametrize("prec", PRECISIONS)
 def test_largest_landscape(ctx, oracle, prec):
-    # P = 512 (16 column tiles, 8 waves per workgroup), N not a multiple of 32.
+    # P = 512 (16 column tiles, 8 waves per workgroup), N not a multiple of 32;
+    # split-bf16 products through ALGO6, the method that takes them.
     pr = rs.synthetic_problem(seed=11, n_patches=512, n_controls=2, n_paths=333, n_years=6,
                               dim_res=6)
     pr.precision = prec
+    if prec == nat.PREC_BF16X3:
+        pr.method = 6
     g = _run(ctx, pr)
     ctl, tp, aa, up = oracle.rovs_forward(pr)
     np.testing.assert_allclose(g["total_pops"], tp, rtol=1e-4)

@@ -121,3 +121,145 @@ def test_switching_decisions_are_consistent(oracle):
     up = pr.unit_cost + np.tensordot(pr.profit_weight, st[1], axes=(0, 0))
     want = np.minimum(150.0 * st[1, 2] * up, 0.0)
     np.testing.assert_allclose(r["V"][1], want, rtol=1e-12, atol=1e-12)
+
+
+# ---- switching mode over several steps -------------------------------------
+#
+# Unit profit u_t, flows Q_c, discount d = 1 / (1 + rrr dt / 100).  The
+# decision rule is backwardInduction's (SimulateGPU.cu:2333-2784): the value at
+# step t is min_c (Q_c u_t + min(d E[V_{t+1} | x_t], 0)) with the running best
+# replaced only below h x best; time 0 adds the mean of d V_1 over the paths of
+# each drawn control (firstPeriodInduction, :2210-2330).
+
+def _disc(pr):
+    return 1.0 / (1.0 + pr.rrr * pr.time_step / 100.0)
+
+
+def test_switching_constant_profit_closed_form(oracle):
+    # An OU price that the unit profit does not load on: V_{t+1} is the same
+    # constant on every path, which the regression (cubic, standardised)
+    # reproduces exactly, so V_t = Q_max u sum_{i=0}^{T-1-t} d^i.
+    T, u = 7, -0.5
+    pr = RovProblem([Uncertainty(nat.SDE_OU, 1.0, 1.2, 0.2, 0.5)], n_steps=T, n_paths=3000,
+                    flow=[0.0, 1.0, 2.0], unit_cost=u, profit_weight=[0.0], seed=5)
+    st, ctrl = oracle.rov_forward(pr)
+    r = oracle.rov_backward(pr, st, ctrl)
+    d = _disc(pr)
+    for t in range(1, T):
+        want = 2.0 * u * sum(d ** i for i in range(T - t))
+        np.testing.assert_allclose(r["V"][t], want, rtol=1e-10)
+    assert r["value"] == pytest.approx(2.0 * u * sum(d ** i for i in range(T)), rel=1e-10)
+    assert r["first_control"] == 2
+    assert r["value_sd"] == pytest.approx(0.0, abs=1e-5)  # one-pass variance
+
+
+@pytest.mark.parametrize("value_mode", [nat.VALUE_REGRESSED, nat.VALUE_REALIZED])
+@pytest.mark.parametrize("pooled", [0, 1])
+def test_switching_deterministic_paths_by_hand(oracle, value_mode, pooled):
+    # Deterministic price paths x_t(p) = a_p + 0.1 t, a_p in [0.01, 0.04], unit
+    # profit u_t = -0.6 + x_t: negative (profitable) up to t = 5 on every path,
+    # positive from t = 6.  V_{t+1} is then linear in a_p, hence in x_t, so the
+    # regression is exact and the value is the hand-computed sum
+    # V_t = sum_{i=0}^{T-1-t} d^i min(u_{t+i}, 0) (flows 0 and 1).
+    T, N = 9, 64
+    rng = np.random.default_rng(2)
+    a = rng.uniform(0.01, 0.04, N)
+    x = a[None, :] + 0.1 * np.arange(T + 1)[:, None]          # (T+1, N)
+    states = x[:, None, :].copy()
+    controls = rng.integers(0, 2, size=(T, N)).astype(np.int32)
+    pr = RovProblem([Uncertainty(nat.SDE_OU, 0.0, 0.0, 0.0)], n_steps=T, n_paths=N,
+                    flow=[0.0, 1.0], unit_cost=-0.6, profit_weight=[1.0], basis_degree=2,
+                    pooled=pooled, value_mode=value_mode)
+    r = oracle.rov_backward(pr, states, controls)
+    d = _disc(pr)
+    u = -0.6 + x
+    assert (u[:6] < 0).all() and (u[6:] > 0).all()
+    V = np.zeros((T + 1, N))
+    for t in range(1, T):
+        V[t] = sum(d ** i * np.minimum(u[t + i], 0.0) for i in range(T - t))
+    np.testing.assert_allclose(r["V"][1:T], V[1:T], rtol=1e-9, atol=1e-12)
+    # time 0: the state of path 0, the mean of d V_1 over each drawn control
+    G = 1 if pooled else 2
+    best, bc = 0.0, None
+    for c in range(2):
+        sel = np.ones(N, bool) if G == 1 else controls[0] == c
+        pay = min(np.mean(d * V[1][sel]) + [0.0, 1.0][c] * u[0, 0], 0.0)
+        if bc is None or pay < best:
+            best, bc = pay, c
+    assert r["value"] == pytest.approx(best, rel=1e-10)
+    assert r["first_control"] == bc
+
+
+def test_switching_hysteresis_flips_the_decision(oracle):
+    # Constant unit profit u = -1, flows 1.0 and 0.99.  With h = 1 control 0
+    # (the larger profit) is kept: V_t = -sum d^i.  With the reference's
+    # h = 0.975 the test is pay_1 < 0.975 pay_0, which for negative payoffs
+    # (-0.99 - X < -0.975 - 0.975 X for every X >= 0) switches to control 1
+    # at every step: V_t = -0.99 sum d^i.
+    T = 6
+    pr = RovProblem([Uncertainty(nat.SDE_OU, 1.0, 1.0, 0.3, 0.5)], n_steps=T, n_paths=2000,
+                    flow=[1.0, 0.99], unit_cost=-1.0, profit_weight=[0.0], seed=8)
+    st, ctrl = oracle.rov_forward(pr)
+    d = _disc(pr)
+    for h, q in ((1.0, 1.0), (0.975, 0.99)):
+        pr.hysteresis = h
+        r = oracle.rov_backward(pr, st, ctrl)
+        for t in range(1, T):
+            np.testing.assert_allclose(r["V"][t], -q * sum(d ** i for i in range(T - t)),
+                                       rtol=1e-10)
+        # time 0 takes the plain minimum: control 0
+        assert r["first_control"] == 0
+        assert r["value"] == pytest.approx(-1.0 + d * np.mean(r["V"][1]), rel=1e-12)
+
+
+# ---- expected PV (SimulateGPU::expPV, Uncertainty::singlePathValue) ---------
+
+def _pv_moments(u, T, dt, r, g):
+    """Exact mean and variance of PV = sum_t w_t x_{t+1}, w_t = (1+g)^t/(1+r)^t,
+    for x_{t+1} = A_t x_t + c, A_t = 1 - k dt + s dt z + (e^J - 1) 1[v < p]
+    iid, c = k m dt, J ~ N(-J0^2/2, J0^4): first and second moments of A,
+    then E[x_s x_t] by the recursion E[x_s x_t] = a E[x_{s-1} x_t] + c E[x_t]."""
+    k, m, s, J0, p = u.reversion, u.mean, u.sd, u.jump_size, u.jump_prob
+    muJ, sJ = -J0 ** 2 / 2, J0 ** 2
+    e1 = np.exp(muJ + sJ ** 2 / 2)
+    e2 = np.exp(2 * muJ + 2 * sJ ** 2)
+    b = 1 - k * dt
+    a = b + p * (e1 - 1)
+    a2 = b * b + 2 * b * p * (e1 - 1) + (s * dt) ** 2 + p * (e2 - 2 * e1 + 1)
+    c = k * m * dt
+    Ex = np.zeros(T + 1)
+    Ex2 = np.zeros(T + 1)
+    Ex[0], Ex2[0] = u.x0, u.x0 ** 2
+    for t in range(T):
+        Ex[t + 1] = a * Ex[t] + c
+        Ex2[t + 1] = a2 * Ex2[t] + 2 * a * c * Ex[t] + c * c
+    M = np.zeros((T + 1, T + 1))
+    for t in range(T + 1):
+        M[t, t] = Ex2[t]
+        for s_ in range(t + 1, T + 1):
+            M[s_, t] = M[t, s_] = a * M[s_ - 1, t] + c * Ex[t]
+    cov = M - np.outer(Ex, Ex)
+    w = np.array([(1 + g) ** t / (1 + r) ** t for t in range(T)])
+    return float(w @ Ex[1:]), float(w @ cov[1:, 1:] @ w)
+
+
+def test_exp_pv_deterministic_closed_form(oracle):
+    u = Uncertainty(nat.SDE_MRJD, 2.0, 1.2, 0.0, 0.3)
+    T, dt, r, g = 25, 0.5, 0.06, 0.02
+    vals, mean, sd = oracle.exp_pv(u, T, dt, r, g, 100, 4)
+    want = sum((1 + g) ** t / (1 + r) ** t * (1.2 + 0.8 * (1 - 0.3 * dt) ** (t + 1))
+               for t in range(T))
+    np.testing.assert_allclose(vals, want, rtol=1e-13)
+    assert mean == pytest.approx(want, rel=1e-13) and sd == pytest.approx(0.0, abs=1e-12)
+
+
+def test_exp_pv_moments(oracle):
+    # Mean to 4 standard errors and the SD (with 1/N) to 3% of the exact SD of
+    # the PV: the reference's SD without 1/N would be sqrt(N) times larger.
+    u = Uncertainty(nat.SDE_MRJD, 1.0, 1.4, 0.2, 0.25, 0.3, 0.1)
+    T, dt, r, g, N = 30, 1.0, 0.07, 0.01, 100000
+    vals, mean, sd = oracle.exp_pv(u, T, dt, r, g, N, 9)
+    m_ex, v_ex = _pv_moments(u, T, dt, r, g)
+    assert mean == pytest.approx(m_ex, abs=4 * np.sqrt(v_ex / N))
+    assert sd == pytest.approx(np.sqrt(v_ex), rel=0.03)
+    assert mean == pytest.approx(vals.mean(), rel=1e-13)

@@ -261,3 +261,76 @@ def test_algo6_matches_python_restatement(oracle, S):
             v, c = _py_recompute(oracle, pr, ctl, up, grids, t, p)
             assert c == oc[t, p], (t, p)
             assert V[t, p] == pytest.approx(v, rel=1e-9, abs=1e-9), (t, p)
+
+
+def _py_allocate_xy(pr, t, controls, tp, aars, up, V):
+    """allocateXYRegressionData (SimulateGPU.cu:1010-1069) restated: in path
+    order, a path joins the samples of its drawn control when every species'
+    highest-flow adjusted population is at its threshold, or when it drew
+    control 0; its predictors are those populations and the unit profit, its
+    response V[t+1]."""
+    S, C, N = len(pr.species), len(pr.flow), pr.n_paths
+    xs = [[] for _ in range(C)]
+    ys = [[] for _ in range(C)]
+    for p in range(N):
+        x = [tp[sp, t, p] * aars[sp, t, C - 1, p] for sp in range(S)] + [up[t, p]]
+        valid = all(x[sp] >= pr.species[sp].params[3] for sp in range(S))
+        c = int(controls[t, p])
+        if valid or c == 0:
+            xs[c].append(x)
+            ys[c].append(V[t + 1, p])
+    return [np.array(a).reshape(-1, S + 1) for a in xs], [np.array(b) for b in ys]
+
+
+def _py_step_grids(pr, t, controls, tp, aars, up, V, scaling):
+    """One step's grids from the samples above: ranges over control 0 and the
+    controls with more than D + 1 samples (computeStateMinMax, :1073-1147,
+    taking min and max independently), species coordinates scaled to [0, 1],
+    the unit profit to [0, 1 / scaling] (:4582-4607), k = 10 (ln n + 1)
+    neighbours (:4543-4545; capped at 128)."""
+    S, C, R = len(pr.species), len(pr.flow), pr.dim_res
+    D = S + 1
+    xs, ys = _py_allocate_xy(pr, t, controls, tp, aars, up, V)
+    use = [c for c in range(C) if c == 0 or len(xs[c]) > D + 1]
+    allx = np.concatenate([xs[c] for c in use])
+    lo, hi = allx.min(0), allx.max(0)
+    div = (hi - lo) * np.array([1.0] * S + [scaling])
+    out = []
+    q = lo[:, None] + np.arange(R)[None, :] * (hi - lo)[:, None] / (R - 1)   # (D, R)
+    qtab = (q - lo[:, None]) / div[:, None]
+    for c in range(C):
+        n = len(xs[c])
+        k = min(int(10 * (np.log(n) + 1)), 128, n) if n > 3 else 0
+        if n < 3 or k < 1:
+            vals = np.zeros(R ** D)
+        else:
+            vals = oracle_knn_fit_scaled((xs[c] - lo) / div, ys[c], R, k, qtab)
+        out.append(np.concatenate([q.ravel(), vals]))
+    return np.array(out), [len(a) for a in xs]
+
+
+oracle_knn_fit_scaled = None
+
+
+@pytest.mark.parametrize("S", [1, 2])
+def test_algo4_samples_and_scaling_match_python_restatement(oracle, S):
+    # Thresholds that bind on part of the paths: the grids of every step
+    # (axes and fitted values) against the Python selection and scaling,
+    # fitted by the same pinned local-linear routine.
+    global oracle_knn_fit_scaled
+    oracle_knn_fit_scaled = oracle.knn_fit_scaled
+    pr = rs.synthetic_problem(seed=21, n_patches=8, n_species=S, n_controls=3, n_paths=400,
+                              n_years=6, dim_res=5, threshold_ratio=0.985 if S == 1 else 0.95)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    V, oc, _, grids = oracle.rovs_backward(pr, ctl, tp, aa, up, grids=True)
+    C = len(pr.flow)
+    dropped = 0
+    for t in range(1, pr.n_years - 1):
+        want, counts = _py_step_grids(pr, t, ctl, tp, aa, up, V, pr.profit_scaling)
+        dropped += int((ctl[t] != 0).sum()) - sum(counts[1:])
+        np.testing.assert_allclose(grids[t], want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+        assert min(counts) > 3 * (S + 2)
+        # the profit scaling matters: scaling 1 gives other neighbours
+        other, _ = _py_step_grids(pr, t, ctl, tp, aa, up, V, 1.0)
+        assert not np.allclose(other[:, (S + 1) * pr.dim_res:], want[:, (S + 1) * pr.dim_res:])
+    assert dropped > 50  # out-of-the-money paths of controls 1, 2 were left out

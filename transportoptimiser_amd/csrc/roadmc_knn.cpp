@@ -10,6 +10,11 @@
 #include "knn_kernels.h"
 #include "roadmc.h"
 
+static_assert(ROADMC_KNN_MAX_K == roadmc::kKnnMaxK,
+              "the header's neighbour limit must be the one the kernels enforce");
+
+// The reference's rules, capped at the kernels' limit: its ROV rule grows
+// without bound (25 + ceil(0.02 (n - 500)) passes 128 above 5 650 samples).
 extern "C" int roadmc_knn_k(int32_t n, int32_t rule, int32_t* k_out) {
   if (!k_out || n < 0 || (rule != ROADMC_KNN_RULE_MTE && rule != ROADMC_KNN_RULE_ROV))
     return ROADMC_ERR_INVALID;
@@ -33,7 +38,7 @@ extern "C" int roadmc_knn_k(int32_t n, int32_t rule, int32_t* k_out) {
     else
       k = (int)(25 + std::ceil((n - 500) * 0.02));
   }
-  *k_out = std::min(k, n);
+  *k_out = std::min({k, n, roadmc::kKnnMaxK});
   return ROADMC_OK;
 }
 
@@ -61,7 +66,7 @@ extern "C" int roadmc_knn_surrogate_fit(roadmc_ctx* ctx, int32_t n, int32_t d, c
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "dim_res must be >= 2 with dim_res^n_dims <= 2^24");
   if (n < 0 || n > (1 << 24)) return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_samples out of range");
   if (k < 1 || k > roadmc::kKnnMaxK)
-    return rmc_fail(ctx, ROADMC_ERR_INVALID, "k must be in [1, 128]");
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "k must be in [1, ROADMC_KNN_MAX_K]");
   if (flags & ~(ROADMC_KNN_NORMALIZE | ROADMC_KNN_NONPOSITIVE))
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown flags");
   if (!surrogate || (n > 0 && (!x || !y)))

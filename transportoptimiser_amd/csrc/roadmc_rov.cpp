@@ -46,6 +46,10 @@ int validate_rov(roadmc_ctx* ctx, const roadmc_rov_problem* p) {
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown ROV mode");
   if (p->value_mode != ROADMC_VALUE_REGRESSED && p->value_mode != ROADMC_VALUE_REALIZED)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "unknown value mode");
+  if (p->mode == ROADMC_ROV_AMERICAN && p->value_mode != ROADMC_VALUE_REALIZED)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "ROADMC_ROV_AMERICAN carries realized cash flows back: "
+                    "value_mode must be ROADMC_VALUE_REALIZED");
   if (p->mode == ROADMC_ROV_SWITCHING) {
     if (p->n_controls < 1 || p->n_controls > ROADMC_MAX_CONTROLS || !p->flow)
       return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_controls/flow invalid");

@@ -234,12 +234,8 @@ __device__ double rov_decide_path(const RovStepArgs& a, int t, int terminal, int
       cont *= disc;
       ex = e > cont ? 1 : 0;
     }
-    double v;
-    if (a.value_mode == kValueRealized) {
-      v = ex ? e : later;
-    } else {
-      v = ex ? e : later;
-    }
+    // Realized cash flows only: the host refuses ROADMC_VALUE_REGRESSED here.
+    const double v = ex ? e : later;
     V[(size_t)t * a.N + p] = v;
     a.optc[(size_t)(t - 1) * a.N + p] = ex;
     return v;

@@ -198,6 +198,8 @@ def config3(seed: int = 3, n_roads: int = 512, n_paths: int = 20000, n_years: in
     args = [(seed, r, int(P), n_paths, n_years, n_species) for r, P in enumerate(sizes)]
     if workers > 1:
         import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
+        # spawned, not forked: the caller may already hold an initialised
+        # HIP runtime, which a forked child must not inherit.
+        with mp.get_context("spawn").Pool(workers) as pool:
             return pool.map(_road, args, chunksize=4)
     return [_road(a) for a in args]

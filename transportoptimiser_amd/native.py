@@ -137,6 +137,7 @@ KNN_NORMALIZE = 1
 KNN_NONPOSITIVE = 2
 KNN_RULE_MTE = 0
 KNN_RULE_ROV = 1
+KNN_MAX_K = 128
 
 
 class RoadSpecies(C.Structure):

@@ -155,7 +155,9 @@ def make_roads(n: int, seed: int, n_paths: int = 8000, n_years: int = 50,
     args = [(seed, r, n_paths, n_years) for r in range(n)]
     if workers > 1:
         import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
+        # spawned, not forked: the caller may already hold an initialised
+        # HIP runtime, which a forked child must not inherit.
+        with mp.get_context("spawn").Pool(workers) as pool:
             return pool.map(_make_road, args, chunksize=16)
     return [_make_road(a) for a in args]
 
