@@ -199,7 +199,8 @@ def config3(seed: int = 3, n_roads: int = 512, n_paths: int = 20000, n_years: in
     if workers > 1:
         import multiprocessing as mp
         # spawned, not forked: the caller may already hold an initialised
-        # HIP runtime, which a forked child must not inherit.
+        # HIP runtime, which a forked child must not inherit.  A calling
+        # script therefore needs the usual `if __name__ == "__main__":` guard.
         with mp.get_context("spawn").Pool(workers) as pool:
             return pool.map(_road, args, chunksize=4)
     return [_road(a) for a in args]

@@ -156,7 +156,8 @@ def make_roads(n: int, seed: int, n_paths: int = 8000, n_years: int = 50,
     if workers > 1:
         import multiprocessing as mp
         # spawned, not forked: the caller may already hold an initialised
-        # HIP runtime, which a forked child must not inherit.
+        # HIP runtime, which a forked child must not inherit.  A calling
+        # script therefore needs the usual `if __name__ == "__main__":` guard.
         with mp.get_context("spawn").Pool(workers) as pool:
             return pool.map(_make_road, args, chunksize=16)
     return [_make_road(a) for a in args]
