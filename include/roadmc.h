@@ -351,6 +351,19 @@ int roadmc_exp_pv(roadmc_ctx* ctx, const roadmc_uncertainty* unc,
                   int32_t n_steps, double time_step, double rrr, double growth,
                   int64_t n_paths, uint64_t seed, double* mean, double* sd);
 
+/* roadmc_exp_pv without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream
+ * capture, serving loops).  The per-path values and the statistics live in a
+ * fixed device array of the library (8 MiB per device, held until the process
+ * ends), so n_paths <= ROADMC_NOALLOC_MAX_PATHS; larger requests return
+ * ROADMC_ERR_INVALID.  Calls on one device share that array: calls that
+ * overlap on one device race.  Same streams and results as roadmc_exp_pv to
+ * FP64 rounding (mean and SD are reduced on the device). */
+#define ROADMC_NOALLOC_MAX_PATHS 1048576
+int roadmc_exp_pv_noalloc(roadmc_ctx* ctx, const roadmc_uncertainty* unc,
+                          int32_t n_steps, double time_step, double rrr, double growth,
+                          int64_t n_paths, uint64_t seed, double* mean, double* sd);
+
 /* ---- ROV with species states (the reference's ALGO4) --------------------- */
 
 /* One species of the species-state valuation.  Every species has the same

@@ -138,6 +138,7 @@ KNN_NONPOSITIVE = 2
 KNN_RULE_MTE = 0
 KNN_RULE_ROV = 1
 KNN_MAX_K = 128
+NOALLOC_MAX_PATHS = 1 << 20
 
 
 class RoadSpecies(C.Structure):
@@ -364,6 +365,9 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_rov_run": (C.c_int, [C.c_void_p, C.POINTER(RovProblem), C.POINTER(RovResult)]),
     "roadmc_exp_pv": (C.c_int, [C.c_void_p, C.POINTER(Uncertainty), C.c_int32, C.c_double,
                                 C.c_double, C.c_double, C.c_int64, C.c_uint64, _dp, _dp]),
+    "roadmc_exp_pv_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Uncertainty), C.c_int32,
+                                        C.c_double, C.c_double, C.c_double, C.c_int64,
+                                        C.c_uint64, _dp, _dp]),
 }
 
 _LIB = None
