@@ -99,7 +99,11 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   const int K = basis_size(U, pr->basis_degree);
   const int E = K * (K + 1) / 2 + K + 1;
   const int G = (american || pr->pooled) ? 1 : C;
-  const int nblk = (N + 255) / 256;
+  const int nblk = (N + roadmc::kRovStepPaths - 1) / roadmc::kRovStepPaths;
+  const int nblk256 = (N + 255) / 256;
+  const int t_term = american ? T : T - 1;
+  const int n_fit = t_term - 1;  // steps 1 .. t_term - 1 are fitted
+  const size_t GE = (size_t)G * E, GK = (size_t)G * K;
 
   double L[roadmc::kRovMaxU * roadmc::kRovMaxU] = {0};
   if (pr->correlation) {
@@ -119,18 +123,23 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
       (rc = rmc_buf_ensure(ctx, B.c, (size_t)(T + 1) * N * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.d, (size_t)T * N * sizeof(int))) ||
       (rc = rmc_buf_ensure(ctx, B.e, (size_t)(T + 1) * U * 2 * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, ((size_t)nblk + 1) * G * E * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.f, ((size_t)std::max(n_fit, 1) * nblk256 * GE +
+                                       (size_t)T * GK * K + 2 * (size_t)nblk * GK +
+                                       (size_t)nblk256 * 3 * roadmc::kRovMaxC) *
+                                          sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.g, (size_t)T * G * K * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.h, ((size_t)T * G + 8) * sizeof(int) + 64)))
+      (rc = rmc_buf_ensure(ctx, B.h, ((size_t)T * G + 1) * sizeof(int) + 128)))
     return rc;
   double* d_states = static_cast<double*>(B.a.ptr);
   int* d_ctrl = static_cast<int*>(B.b.ptr);
   double* d_V = static_cast<double*>(B.c.ptr);
   int* d_optc = static_cast<int*>(B.d.ptr);
   double* d_scale = static_cast<double*>(B.e.ptr);
+  // B.h: status [T][G] ints, out [3] doubles (16-byte aligned), tickets.
   int* d_status = static_cast<int*>(B.h.ptr);
-  double* d_out = reinterpret_cast<double*>(static_cast<char*>(B.h.ptr) +
-                                            (((size_t)T * G * sizeof(int) + 15) & ~(size_t)15));
+  const size_t out_off = ((size_t)T * G * sizeof(int) + 15) & ~(size_t)15;
+  double* d_out = reinterpret_cast<double*>(static_cast<char*>(B.h.ptr) + out_off);
+  int* d_ticket = reinterpret_cast<int*>(static_cast<char*>(B.h.ptr) + out_off + 32);
   hipStream_t s = ctx->stream;
 
   const uint32_t k0 = (uint32_t)(pr->seed & 0xffffffffu), k1 = (uint32_t)(pr->seed >> 32);
@@ -201,15 +210,36 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   a.controls = d_ctrl;
   a.V = d_V;
   a.optc = d_optc;
-  a.partial = static_cast<double*>(B.f.ptr);
-  a.sums = a.partial + (size_t)nblk * G * E;
+  {
+    // Basis columns in the order of rov_basis_visit: indices into (1, x).
+    int k = 0;
+    a.mono[k++] = 0;
+    if (pr->basis_degree >= 1)
+      for (int i = 0; i < U; ++i) a.mono[k++] = 1 + i;
+    if (pr->basis_degree >= 2)
+      for (int i = 0; i < U; ++i)
+        for (int j = i; j < U; ++j) a.mono[k++] = (1 + i) | ((1 + j) << 8);
+    if (pr->basis_degree >= 3)
+      for (int i = 0; i < U; ++i)
+        for (int j = i; j < U; ++j)
+          for (int l = j; l < U; ++l) a.mono[k++] = (1 + i) | ((1 + j) << 8) | ((1 + l) << 16);
+    if (k != K) return rmc_fail(ctx, ROADMC_ERR_INVALID, "basis enumeration mismatch");
+  }
   a.nblk = nblk;
+  a.partial = static_cast<double*>(B.f.ptr);
+  a.ainv = a.partial + (size_t)std::max(n_fit, 1) * nblk256 * GE;
+  a.bpart = a.ainv + (size_t)T * GK * K;
+  a.fpart = a.bpart + 2 * (size_t)nblk * GK;
+  a.ticket = d_ticket;
   a.beta = static_cast<double*>(B.g.ptr);
   a.status = d_status;
 
-  // Zero the coefficients so steps without a fit read zeros.
+  // Zero the coefficients so steps without a fit read zeros, and the arrival
+  // ticket (each launch leaves it zero again; cleared here in case the buffer
+  // was regrown).  Both before, not inside, the captured graph.
   RMC_HIP(ctx, hipMemsetAsync(a.beta, 0, (size_t)T * G * K * sizeof(double), s));
   RMC_HIP(ctx, hipMemsetAsync(d_status, 0, (size_t)T * G * sizeof(int), s));
+  RMC_HIP(ctx, hipMemsetAsync(d_ticket, 0, sizeof(int), s));
 
   RMC_HIP(ctx, roadmc::rov_set_lds_limits());
   a.out = d_out;
@@ -220,28 +250,22 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   RMC_HIP(ctx, hipMemcpyAsync(RG.param.ptr, &a, sizeof(a), hipMemcpyHostToDevice, s));
   // The graph depends only on the launch shapes and the parameter block's
   // address.
-  const long long key[8] = {N, T, K, G * 1000LL + E, C, american ? 1 : 0,
+  const long long key[8] = {N, T, K, G * 1000LL + E, C * 100LL + U, american ? 1 : 0,
                             (long long)(uintptr_t)pa, a.nblk};
   if (!RG.exec || std::memcmp(key, RG.key, sizeof(key)) != 0) {
     if (RG.exec) {
       (void)hipGraphExecDestroy(RG.exec);
       RG.exec = nullptr;
     }
-    // Backward induction as one graph: terminal decision, then fit + decide
-    // for every interior step, then time 0.
+    // Backward induction as one graph: the matrices of every fitted step and
+    // their inverses, then one launch per step (the coefficients of the
+    // step after it, its decisions, the right-hand sides of this step), then
+    // step 1 and time 0.
     RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    // Each step's decision runs inside the next step's assembly (the
-    // terminal one inside the first fit); the decision of step 1 alone.
-    const int t_term = american ? T : T - 1;
-    if (t_term - 1 >= 1) {
-      for (int t = t_term - 1; t >= 1 && e == hipSuccess; --t)
-        e = roadmc::launch_rov_fit(a, pa, t, t + 1 == t_term ? 2 : 1, s);
-      if (e == hipSuccess) e = roadmc::launch_rov_decide(a, pa, 1, false, s);
-    } else {
-      e = roadmc::launch_rov_decide(a, pa, t_term, true, s);
-    }
-    if (e == hipSuccess) e = roadmc::launch_rov_first(pa, s);
+    hipError_t e = roadmc::launch_rov_gram(a, pa, 1, n_fit, s);
+    for (int t = t_term - 1; t >= 1 && e == hipSuccess; --t)
+      e = roadmc::launch_rov_step(a, pa, t, t + 1 == t_term ? 2 : 1, s);
+    if (e == hipSuccess) e = roadmc::launch_rov_final(a, pa, t_term == 1, s);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(s, &graph);
     if (e != hipSuccess || ec != hipSuccess) {

@@ -21,19 +21,29 @@ struct RovForwardArgs {
   int* controls;   // [T][N] or nullptr
 };
 
+constexpr int kRovStepPaths = 512;  // paths per block of the step kernel (two per thread)
+constexpr int kRovMaxCol = 48;      // basis columns + response, rounded up to 16
+
 struct RovStepArgs {
   int N, T, U, C, G, K, E, deg, mode, value_mode, pooled, itm_only, traffic;
   double disc, hysteresis, unit_cost, strike;
   double weight[kRovMaxU];
   double flow[kRovMaxC];
+  // Basis column k as three indices into (1, x_0, .., x_{U-1}), packed
+  // i0 | i1 << 8 | i2 << 16: phi_k = (v[i0] v[i1]) v[i2], in the graded
+  // order of the basis (1, x_i, x_i x_j (i <= j), x_i x_j x_l (i <= j <= l)).
+  int mono[kRovMaxCol];
   const double* states;  // [(T+1)][U][N]
   const double* scale;   // [(T+1)][U][2] mean, 1/sd, or nullptr
   const int* controls;   // [T][N]
   double* V;             // [(T+1)][N]
   int* optc;             // [T][N]
-  double* partial;       // [G][E][nblk] block partial sums of one step
-  double* sums;          // [G][E] their fixed-order totals
-  int nblk;              // assembly blocks
+  int nblk;              // blocks of the step kernel, ceil(N / kRovStepPaths)
+  double* partial;       // [steps][ceil(N / 256)][G][E] block sums of the matrices A
+  double* ainv;          // [T][G][K][K] inverses of A
+  double* bpart;         // [2][nblk][G][K] block sums of the right-hand sides, by step parity
+  double* fpart;         // [ceil(N / 256)][3][kRovMaxC] time-0 block sums
+  int* ticket;           // [1] arrival counter of the time-0 kernel, zero between launches
   double* beta;          // [T][G][K]
   int* status;           // [T][G]
   double* out;           // [3] value, control, SD of time 0
@@ -46,22 +56,26 @@ struct ExpPvArgs {
   double* out;  // [N]
 };
 
-size_t rov_assemble_lds(int K);
 hipError_t rov_set_lds_limits();
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s);
 hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double* scale,
                               hipStream_t s);
-// The step kernels read their parameters from the device copy `pa` of `a`
+// The kernels below read their parameters from the device copy `pa` of `a`
 // (a supplies the launch shapes), so a captured graph of them stays valid
 // for any problem of the same shape.
-// Fit of step t: normal equations, their reduction and solve.  With
-// decide_next >= 0 the assembly first takes the decision of step t + 1
-// (decide_next == 2: the terminal rule) for every path.
-hipError_t launch_rov_fit(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
-                          hipStream_t s);
-hipError_t launch_rov_decide(const RovStepArgs& a, const RovStepArgs* pa, int t, bool terminal,
-                             hipStream_t s);
-hipError_t launch_rov_first(const RovStepArgs* pa, hipStream_t s);
+// The matrices A and their inverses of steps t_lo .. t_lo + n_steps - 1, all
+// at once (two launches).
+hipError_t launch_rov_gram(const RovStepArgs& a, const RovStepArgs* pa, int t_lo, int n_steps,
+                           hipStream_t s);
+// Step t of the backward induction, one launch: with decide_next == 1 the
+// coefficients of step t + 1 and its decisions, with decide_next == 2 the
+// terminal decisions of step t + 1; then step t's partial right-hand sides.
+hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
+                           hipStream_t s);
+// The coefficients and decisions of step 1 (terminal when `terminal`), then
+// time 0 (firstPeriodInduction) in the last block to arrive.
+hipError_t launch_rov_final(const RovStepArgs& a, const RovStepArgs* pa, bool terminal,
+                            hipStream_t s);
 hipError_t launch_exp_pv(const ExpPvArgs& a, hipStream_t s);
 
 }  // namespace roadmc

@@ -132,55 +132,6 @@ __global__ __launch_bounds__(1024) void rov_moments_kernel(const double* __restr
   }
 }
 
-// All monomials of degree <= deg in U variables, in graded order:
-// 1, x_i, x_i x_j (i <= j), x_i x_j x_k (i <= j <= k); f(k, value) is called
-// for each in turn.  The loops run over the compile-time maximum with guards,
-// so x stays in registers.
-template <class F>
-__device__ __forceinline__ int rov_basis_visit(const double (&x)[kRovMaxU], int U, int deg,
-                                               F&& f) {
-  int k = 0;
-  f(k++, 1.0);
-  if (deg >= 1) {
-#pragma unroll
-    for (int i = 0; i < kRovMaxU; ++i)
-      if (i < U) f(k++, x[i]);
-  }
-  if (deg >= 2) {
-#pragma unroll
-    for (int i = 0; i < kRovMaxU; ++i)
-#pragma unroll
-      for (int j = i; j < kRovMaxU; ++j)
-        if (j < U) f(k++, x[i] * x[j]);
-  }
-  if (deg >= 3) {
-#pragma unroll
-    for (int i = 0; i < kRovMaxU; ++i)
-#pragma unroll
-      for (int j = i; j < kRovMaxU; ++j)
-#pragma unroll
-        for (int l = j; l < kRovMaxU; ++l)
-          if (l < U) f(k++, x[i] * x[j] * x[l]);
-  }
-  return k;
-}
-
-__device__ __forceinline__ void rov_features(const RovStepArgs& a, int t, int p,
-                                             double (&x)[kRovMaxU]) {
-#pragma unroll
-  for (int u = 0; u < kRovMaxU; ++u) {
-    double v = 0.0;
-    if (u < a.U) {
-      v = a.states[((size_t)t * a.U + u) * a.N + p];
-      if (a.scale) {
-        const double* sc = a.scale + ((size_t)t * a.U + u) * 2;
-        v = (v - sc[0]) * sc[1];
-      }
-    }
-    x[u] = v;
-  }
-}
-
 __device__ __forceinline__ double rov_unit_profit(const RovStepArgs& a, int t, int p) {
   double up = a.unit_cost;
   for (int u = 0; u < a.U; ++u) up += a.weight[u] * a.states[((size_t)t * a.U + u) * a.N + p];
@@ -191,71 +142,160 @@ __device__ __forceinline__ double rov_flow_scale(const RovStepArgs& a, int t, in
   return a.traffic >= 0 ? a.states[((size_t)t * a.U + a.traffic) * a.N + p] : 1.0;
 }
 
-// Which fit a path of step t joins (-1: none).
-__device__ __forceinline__ int rov_fit_group(const RovStepArgs& a, int t, int p) {
-  if (a.mode == kRovAmerican) {
-    const double e = a.strike - a.states[((size_t)t * a.U) * a.N + p];
-    return (e > 0.0 || !a.itm_only) ? 0 : -1;
+// The polynomial basis of the regression, fixed at compile time: all
+// monomials of degree <= DEG in U variables in graded order 1, x_i,
+// x_i x_j (i <= j), x_i x_j x_l (i <= j <= l).  Monomial k is
+// (v[i0] v[i1]) v[i2] with v = (1, x_0, .., x_{U-1}), so pairs and triples
+// round exactly as x_i x_j and (x_i x_j) x_l.
+template <int U, int DEG>
+struct RovBasis {
+  static constexpr int count() {
+    int k = 1;
+    if (DEG >= 1) k += U;
+    if (DEG >= 2) k += U * (U + 1) / 2;
+    if (DEG >= 3) k += U * (U + 1) * (U + 2) / 6;
+    return k;
   }
-  if (a.itm_only && rov_unit_profit(a, t, p) > 0.0) return -1;
-  return a.pooled ? 0 : a.controls[(size_t)t * a.N + p];
+  static constexpr int K = count();
+  static constexpr int NT = (K + 15) / 16;  // 16-column tiles of the basis
+  static constexpr int NP = NT * (NT + 1) / 2;
+  static constexpr int E = K * (K + 1) / 2 + K + 1;  // triangle, b, count
+  struct Table {
+    int i0[K], i1[K], i2[K];
+  };
+  static constexpr Table make() {
+    Table tb{};
+    int k = 0;
+    tb.i0[k] = tb.i1[k] = tb.i2[k] = 0;
+    ++k;
+    if (DEG >= 1)
+      for (int i = 0; i < U; ++i) {
+        tb.i0[k] = 1 + i;
+        tb.i1[k] = tb.i2[k] = 0;
+        ++k;
+      }
+    if (DEG >= 2)
+      for (int i = 0; i < U; ++i)
+        for (int j = i; j < U; ++j) {
+          tb.i0[k] = 1 + i;
+          tb.i1[k] = 1 + j;
+          tb.i2[k] = 0;
+          ++k;
+        }
+    if (DEG >= 3)
+      for (int i = 0; i < U; ++i)
+        for (int j = i; j < U; ++j)
+          for (int l = j; l < U; ++l) {
+            tb.i0[k] = 1 + i;
+            tb.i1[k] = 1 + j;
+            tb.i2[k] = 1 + l;
+            ++k;
+          }
+    return tb;
+  }
+  static constexpr Table tab = make();
+};
+
+// Raw states of step t of path p (all U loads issued together).
+template <int U>
+__device__ __forceinline__ void rov_load_states(const RovStepArgs& a, int t, int p,
+                                                double (&x)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) x[u] = a.states[((size_t)t * U + u) * a.N + p];
 }
 
-// Per-block partial sums of the normal equations of step t for every fit
-// group: A (upper triangle, row-major), b, and the point count.
-//
-// The block's 256 points form Phi = [phi_0 .. phi_{K-1}, y] (256 x KP, KP =
-// K + 1 rounded up to 16, zero padded) in LDS; for group g the sums are the
-// Gram matrix Phi_g^T Phi with Phi_g the rows of the group's points, built
-// from 16 x 16 tiles on v_mfma_f64_16x16x4_f64 (4 points per MFMA, points in
-// a fixed order).  Column K of the Gram matrix is b, entry (0, 0) the count.
-constexpr int kRovAsmPoints = 256;
+// v = (1, features): the raw states standardised with step t's moments.
+template <int U>
+__device__ __forceinline__ void rov_features(const RovStepArgs& a, int t, const double (&xr)[U],
+                                             double (&v)[U + 1]) {
+  v[0] = 1.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    double x = xr[u];
+    if (a.scale) {
+      const double* sc = a.scale + ((size_t)t * U + u) * 2;
+      x = (x - sc[0]) * sc[1];
+    }
+    v[1 + u] = x;
+  }
+}
 
-__host__ __device__ inline int rov_kp(int K) { return (K + 1 + 15) / 16 * 16; }
+template <int U>
+__device__ __forceinline__ double rov_unit_profit_of(const RovStepArgs& a, const double (&xr)[U]) {
+  double up = a.unit_cost;
+#pragma unroll
+  for (int u = 0; u < U; ++u) up += a.weight[u] * xr[u];
+  return up;
+}
 
-__device__ double rov_decide_path(const RovStepArgs& a, int t, int terminal, int p) {
+// The fit a path of step t joins (-1: none): in the money (keepValidPoints,
+// SimulateGPU.cu:5999-6043) and, per control, the control it drew.
+template <int U>
+__device__ __forceinline__ int rov_group_of(const RovStepArgs& a, const double (&xr)[U], int ctl) {
+  if (a.mode == kRovAmerican) return (a.strike - xr[0] > 0.0 || !a.itm_only) ? 0 : -1;
+  if (a.itm_only && rov_unit_profit_of<U>(a, xr) > 0.0) return -1;
+  return a.pooled ? 0 : ctl;
+}
+
+// Whether the decision of step t reads V[t + 1] (realized values, American
+// continuation).
+__device__ __forceinline__ bool rov_decide_reads_later(const RovStepArgs& a, int terminal) {
+  if (terminal) return false;
+  return a.mode == kRovAmerican || a.value_mode == kValueRealized;
+}
+
+// The decision of step t for one path from its raw states xr at t, the
+// coefficients beta_t [G][K] of step t (unused at the terminal step) and,
+// where rov_decide_reads_later, later = V[t + 1] of the path; returns the
+// value and the control (American: 1 = exercise) in *bc.
+template <int U, int DEG>
+__device__ __forceinline__ double rov_decide(const RovStepArgs& a, int t, int terminal,
+                                             const double (&xr)[U], const double* beta_t,
+                                             double later, int* bc_out) {
+  using B = RovBasis<U, DEG>;
+  constexpr int K = B::K;
   const double disc = a.disc;
-  double* V = a.V;
   if (a.mode == kRovAmerican) {
-    const double e = fmax(a.strike - a.states[((size_t)t * a.U) * a.N + p], 0.0);
+    const double e = fmax(a.strike - xr[0], 0.0);
     if (terminal) {
-      V[(size_t)t * a.N + p] = e;
-      a.optc[(size_t)(t - 1) * a.N + p] = e > 0.0 ? 1 : 0;
+      *bc_out = e > 0.0 ? 1 : 0;
       return e;
     }
-    const double later = disc * V[(size_t)(t + 1) * a.N + p];
     int ex = 0;
     if (e > 0.0 || !a.itm_only) {
-      double x[kRovMaxU];
-      rov_features(a, t, p, x);
-      const double* b = a.beta + (size_t)t * a.G * a.K;
+      double v[U + 1];
+      rov_features<U>(a, t, xr, v);
       double cont = 0.0;
-      rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { cont = fma(v, b[k], cont); });
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        cont = fma((v[B::tab.i0[k]] * v[B::tab.i1[k]]) * v[B::tab.i2[k]], beta_t[k], cont);
       cont *= disc;
       ex = e > cont ? 1 : 0;
     }
     // Realized cash flows only: the host refuses ROADMC_VALUE_REGRESSED here.
-    const double v = ex ? e : later;
-    V[(size_t)t * a.N + p] = v;
-    a.optc[(size_t)(t - 1) * a.N + p] = ex;
-    return v;
+    *bc_out = ex;
+    return ex ? e : disc * later;
   }
   // Flow control (backwardInduction, SimulateGPU.cu:2333-2784).
-  const double up = rov_unit_profit(a, t, p);
-  const double m = rov_flow_scale(a, t, p);
+  const double up = rov_unit_profit_of<U>(a, xr);
+  double m = 1.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (u == a.traffic) m = xr[u];
   // Continuation value of every fit group, one pass over the basis.
   double cont[kRovMaxC];
 #pragma unroll
   for (int c = 0; c < kRovMaxC; ++c) cont[c] = 0.0;
   if (!terminal) {
-    double x[kRovMaxU];
-    rov_features(a, t, p, x);
-    const double* b = a.beta + (size_t)t * a.G * a.K;
-    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) {
+    double v[U + 1];
+    rov_features<U>(a, t, xr, v);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double phi = (v[B::tab.i0[k]] * v[B::tab.i1[k]]) * v[B::tab.i2[k]];
 #pragma unroll
       for (int c = 0; c < kRovMaxC; ++c)
-        if (c < a.G) cont[c] = fma(v, b[(size_t)c * a.K + k], cont[c]);
-    });
+        if (c < a.G) cont[c] = fma(phi, beta_t[c * K + k], cont[c]);
+    }
   }
   double best = 0.0, best_cur = 0.0, best_pay = 0.0;
   int bc = 0;
@@ -279,95 +319,9 @@ __device__ double rov_decide_path(const RovStepArgs& a, int t, int terminal, int
       best_pay = pay;
     }
   }
-  double v = best;
-  if (a.value_mode == kValueRealized)
-    v = terminal ? best_pay : best_cur + disc * V[(size_t)(t + 1) * a.N + p];
-  V[(size_t)t * a.N + p] = v;
-  a.optc[(size_t)t * a.N + p] = bc;
-  return v;
-}
-
-__global__ __launch_bounds__(256) void rov_decide_kernel(const RovStepArgs* __restrict__ pa, int t,
-                                                         int terminal) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < pa->N) rov_decide_path(*pa, t, terminal, p);
-}
-
-// With decide >= 0 the kernel first takes the decision of step t + 1 for its
-// paths (rov_decide_path, terminal when decide == 2) and fits on the values
-// it just wrote, so that step's separate decide launch is saved.
-__global__ __launch_bounds__(256) void rov_assemble_kernel(const RovStepArgs* __restrict__ pa,
-                                                           int t, int decide) {
-  typedef double doublex4 __attribute__((ext_vector_type(4)));
-  const RovStepArgs& a = *pa;
-  extern __shared__ double sm[];
-  const int K = a.K, E = a.E, KP = rov_kp(K), LD = KP + 1;  // +1: rotate banks per row
-  double* phi = sm;  // [256][LD]
-  int* sg = reinterpret_cast<int*>(phi + (size_t)kRovAsmPoints * LD);
-  const int q = threadIdx.x;
-  const int p = blockIdx.x * kRovAsmPoints + q;
-  double* row = phi + (size_t)q * LD;
-  int g = -1;
-  for (int k = 0; k < KP; ++k) row[k] = 0.0;
-  if (p < a.N) {
-    const double vnext = decide >= 0 ? rov_decide_path(a, t + 1, decide == 2 ? 1 : 0, p)
-                                     : a.V[(size_t)(t + 1) * a.N + p];
-    g = rov_fit_group(a, t, p);
-    double x[kRovMaxU];
-    rov_features(a, t, p, x);
-    rov_basis_visit(x, a.U, a.deg, [&](int k, double v) { row[k] = v; });
-    row[K] = vnext;
-  }
-  sg[q] = g;
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nt = KP / 16, ntile = nt * (nt + 1) / 2, njobs = a.G * ntile;
-  const int n_tri = K * (K + 1) / 2;
-  const int kq = lane >> 4, li = lane & 15;
-  for (int job = wave; job < njobs; job += 4) {
-    const int grp = job / ntile;
-    int I = 0, rem = job - grp * ntile;
-    while (rem >= nt - I) {
-      rem -= nt - I;
-      ++I;
-    }
-    const int J = I + rem;
-    doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int q0 = 0; q0 < kRovAsmPoints; q0 += 4) {
-      const double* r = phi + (size_t)(q0 + kq) * LD;
-      const double av = sg[q0 + kq] == grp ? r[I * 16 + li] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, r[J * 16 + li], acc, 0, 0, 0);
-    }
-    // Item r of lane l is Gram entry (I*16 + l/16 + 4r, J*16 + l%16).
-    const int j = J * 16 + li;
-#pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const int i = I * 16 + kq + 4 * rr;
-      int e = -1;
-      if (i < K && j < K && i <= j)
-        e = i * K - i * (i - 1) / 2 + (j - i);
-      else if (i < K && j == K)
-        e = n_tri + i;
-      double* dst = a.partial + (size_t)grp * E * a.nblk + blockIdx.x;
-      if (e >= 0) dst[(size_t)e * a.nblk] = acc[rr];
-      if (i == 0 && j == 0) dst[(size_t)(n_tri + K) * a.nblk] = acc[rr];
-    }
-  }
-}
-
-// sums[e] = sum over blocks of partial[e][.], one wave per entry: lane l adds
-// blocks l, l + 64, ... in order, then a fixed butterfly across lanes, so
-// the totals do not depend on timing.
-__global__ __launch_bounds__(64) void rov_reduce_kernel(const RovStepArgs* __restrict__ pa) {
-  const RovStepArgs& a = *pa;
-  const int e = blockIdx.x, lane = threadIdx.x;
-  const double* src = a.partial + (size_t)e * a.nblk;
-  double s = 0.0;
-  for (int b = lane; b < a.nblk; b += 64) s += src[b];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) a.sums[e] = s;
+  *bc_out = bc;
+  if (a.value_mode == kValueRealized) return terminal ? best_pay : best_cur + disc * later;
+  return best;
 }
 
 // Value of v in lane `src` (a compile-time constant in the unrolled loops
@@ -379,25 +333,194 @@ __device__ __forceinline__ double bcast_lane(double v, int src) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Solves each group's K x K normal equations in FP64, one wave per group:
-// Cholesky with the rows in registers (lane i owns row i; K is a template
-// parameter, one instance per basis size the problem validation admits, so
-// the loops are straight-line code), or, when the matrix is not numerically
-// positive definite, Gaussian elimination with partial pivoting on
-// |M[i][col]| (the pivot search the reference intends,
-// SimulateGPU.cu:6307-6316).  Groups with fewer points than K keep zero
-// coefficients (multiQuadReg skips them, SimulateGPU.cu:6101).
-template <int K>
-__global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __restrict__ pa, int t) {
-  const RovStepArgs& a = *pa;
-  extern __shared__ double sm[];
-  const int E = a.E, grp = blockIdx.x, lane = threadIdx.x;
-  double* M = sm;            // [K][K]
-  double* L = M + K * K;     // [K][K]
-  double* r = L + K * K;     // [K]
-  double* y = r + K;         // [K]
-  const double* S = a.sums + (size_t)grp * E;
+// Completes this wave's LDS accesses before its lanes read each other's
+// values (the factorisation below runs on single waves).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// sum_{m < n} src[m * stride] in order m = 0, 1, ..., with the loads of
+// each run of 16 issued together.
+__device__ __forceinline__ double rov_sum_rows(const double* src, size_t stride, int n) {
+  double s = 0.0;
+  for (int m0 = 0; m0 < n; m0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = m0 + i < n ? src[(size_t)(m0 + i) * stride] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (m0 + i < n) s += v[i];
+  }
+  return s;
+}
+
+// ---- the regression of the backward induction ------------------------------
+//
+// Step t's fit (multiQuadReg, SimulateGPU.cu:6092-6207) solves A_{t,g} beta =
+// b_{t,g} for each fit group g, A = sum phi phi^T over the group's paths and
+// b = sum phi V_{t+1}.  Group membership (in the money, drawn control) and the
+// basis depend only on the simulated states, so every A is formed and
+// inverted before the induction starts, all steps at once (rov_gram_kernel,
+// rov_invert_kernel); the induction's serial chain is then one launch per
+// step (rov_step_kernel): the reduction of b of the step before, beta =
+// A^{-1} b, the decisions, and the partial sums of the next b.
+
+// LDS doubles of rov_gram_kernel: features [256][U+1], the waves' tiles
+// [4][NP][4][64]; then ints: 256 groups, [4][64] path lists.
+__host__ __device__ inline size_t rov_gram_lds_doubles(int K, int U) {
+  const int nt = (K + 15) / 16, np = nt * (nt + 1) / 2;
+  return (size_t)256 * (U + 1) + (size_t)4 * np * 4 * 64;
+}
+
+size_t rov_gram_lds(int K, int U) {
+  return rov_gram_lds_doubles(K, U) * sizeof(double) + (size_t)(256 + 4 * 64) * sizeof(int);
+}
+
+// Entry e of a group's sums -> Gram coordinates (i, j), i <= j: the upper
+// triangle row-major, then column K (b), then (0, 0) again for the count.
+__device__ __forceinline__ void rov_entry_ij(int e, int K, int* i_out, int* j_out) {
   const int n_tri = K * (K + 1) / 2;
+  if (e < n_tri) {
+    int i = 0, rem = e;
+    while (rem >= K - i) {
+      rem -= K - i;
+      ++i;
+    }
+    *i_out = i;
+    *j_out = i + rem;
+  } else if (e < n_tri + K) {
+    *i_out = e - n_tri;
+    *j_out = K;
+  } else {
+    *i_out = 0;
+    *j_out = 0;
+  }
+}
+
+// The matrices A of steps t_lo + blockIdx.y: block partial sums over paths
+// [256 blockIdx.x, +256) into partial [step][blk][G][E] (the b entries zero).
+// Per group, each wave lists its 64 paths of the group (ballot, in path
+// order) and takes them four at a time on v_mfma_f64_16x16x4_f64 with both
+// operands in registers: lane l computes basis columns 16 I + (l & 15) of path
+// number l >> 4 of the four (columns >= K zero), so tile (I, J) accumulates
+// sum phi_{16I+i} phi_{16J+j}; item r of lane l of tile (I, J) is entry
+// (16 I + (l >> 4) + 4 r, 16 J + (l & 15)).  Only the group's own paths
+// enter, padded to a multiple of four; the four waves' tiles are summed in
+// wave order.
+template <int U, int DEG>
+__global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __restrict__ pa,
+                                                       int t_lo) {
+  typedef double doublex4 __attribute__((ext_vector_type(4)));
+  using B = RovBasis<U, DEG>;
+  constexpr int K = B::K, NT = B::NT, NP = B::NP, E = B::E, U1 = U + 1;
+  const RovStepArgs& a = *pa;
+  const int G = a.G, GE = G * E;
+  const int t = t_lo + (int)blockIdx.y;
+  extern __shared__ double sm[];
+  double* xs = sm;                       // [256][U1], column 0 = 1
+  double* tiles = xs + 256 * U1;         // [4][NP][4][64]
+  int* gs = reinterpret_cast<int*>(sm + rov_gram_lds_doubles(K, U));  // [256]
+  int* lst = gs + 256;                   // [4][64]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  {
+    const int p = blockIdx.x * 256 + tid;
+    double* row = xs + (size_t)tid * U1;
+    int g = -1;
+    double v[U1];
+    if (p < a.N) {
+      double xr[U];
+      rov_load_states<U>(a, t, p, xr);
+      g = rov_group_of<U>(a, xr, a.pooled ? 0 : a.controls[(size_t)t * a.N + p]);
+      rov_features<U>(a, t, xr, v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U1; ++u) v[u] = 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U1; ++u) row[u] = v[u];
+    gs[tid] = g;
+  }
+  // Per-lane monomials of columns 16 I + (lane & 15).
+  const int c_lo = lane & 15, kq = lane >> 4;
+  int mi0[NT], mi1[NT], mi2[NT];
+#pragma unroll
+  for (int I = 0; I < NT; ++I) {
+    const int col = I * 16 + c_lo;
+    const int m = col < K ? a.mono[col] : 0;
+    mi0[I] = m & 255;
+    mi1[I] = (m >> 8) & 255;
+    mi2[I] = m >> 16;
+  }
+  __syncthreads();
+  double* part = a.partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * GE;
+  for (int g = 0; g < G; ++g) {
+    const bool in = gs[w * 64 + lane] == g;
+    const uint64_t mask = __ballot(in);
+    const int cnt = __popcll(mask);
+    if (in) lst[w * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = lane;
+    wave_lds_sync();
+    doublex4 acc[NP];
+#pragma unroll
+    for (int pi = 0; pi < NP; ++pi) acc[pi] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 0; r0 < cnt; r0 += 4) {
+      const int idx = r0 + kq;
+      const int pt = w * 64 + lst[w * 64 + (idx < cnt ? idx : 0)];
+      const double* xr = xs + (size_t)pt * U1;
+      double v[NT];
+#pragma unroll
+      for (int I = 0; I < NT; ++I) {
+        const double val = (xr[mi0[I]] * xr[mi1[I]]) * xr[mi2[I]];
+        v[I] = (idx < cnt && I * 16 + c_lo < K) ? val : 0.0;
+      }
+      int pi = 0;
+#pragma unroll
+      for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = I; J < NT; ++J) {
+          acc[pi] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[pi], 0, 0, 0);
+          ++pi;
+        }
+    }
+#pragma unroll
+    for (int pi = 0; pi < NP; ++pi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tiles[(((size_t)w * NP + pi) * 4 + r) * 64 + lane] = acc[pi][r];
+    __syncthreads();
+    for (int e = tid; e < E; e += 256) {
+      int i, j;
+      rov_entry_ij(e, K, &i, &j);
+      double s = 0.0;
+      if (j < K) {
+        const int I = i >> 4, J = j >> 4, ii = i & 15, jj = j & 15;
+        const int pi = I * NT - I * (I - 1) / 2 + (J - I);
+        const int l = ((ii & 3) << 4) | jj, r = ii >> 2;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) s += tiles[(((size_t)ww * NP + pi) * 4 + r) * 64 + l];
+      }
+      part[(size_t)g * E + e] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// A^{-1} of one group from its sums S (upper triangle, b, count), on one
+// wave: Cholesky with the rows in registers (lane i owns row i; K is a
+// template parameter, one instance per basis size the problem validation
+// admits, so the loops are straight-line code), then X = L^{-1} column by
+// column (lane c solves L x = e_c) and A^{-1} = X^T X; when the matrix is not
+// numerically positive definite, Gauss-Jordan elimination with partial
+// pivoting on |M[i][col]| (the pivot search the reference intends,
+// SimulateGPU.cu:6307-6316).  status: 0 Cholesky, 1 fewer points than K (A^{-1}
+// = 0: the coefficients stay zero, as multiQuadReg skips such groups,
+// SimulateGPU.cu:6101), 2 pivoted elimination, 3 singular (A^{-1} = 0).  work
+// holds 2 K^2 + 2 K doubles of LDS.
+template <int K>
+__device__ void rov_invert_wave(const double* S, double* work, double* inv, int* status,
+                                int lane) {
+  double* M = work;       // [K][K]
+  double* L = M + K * K;  // [K][K]
+  constexpr int n_tri = K * (K + 1) / 2;
   for (int e = lane; e < n_tri; e += 64) {
     int i = 0, rem = e;
     while (rem >= K - i) {
@@ -408,21 +531,15 @@ __global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __rest
     M[i * K + j] = S[e];
     M[j * K + i] = S[e];
   }
-  for (int i = lane; i < K; i += 64) r[i] = S[n_tri + i];
-  __syncthreads();
-  double* coef = a.beta + ((size_t)t * a.G + grp) * K;
+  wave_lds_sync();
   const double count = S[n_tri + K];
   if (count < (double)K) {
-    for (int i = lane; i < K; i += 64) coef[i] = 0.0;
-    if (lane == 0) a.status[(size_t)t * a.G + grp] = 1;
+    for (int e = lane; e < K * K; e += 64) inv[e] = 0.0;
+    if (lane == 0) *status = 1;
     return;
   }
   double dmax = 0.0;
   for (int i = 0; i < K; ++i) dmax = fmax(dmax, fabs(M[i * K + i]));
-  // Cholesky in registers: lane i holds row i of M (m) and of L (l).  Every
-  // loop is unrolled over the compile-time bound, so the rows stay in
-  // registers and each cross-lane value is a v_readlane broadcast from a
-  // constant lane (no LDS round trip on the critical path).
   double m[K], l[K], invd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -436,9 +553,9 @@ __global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __rest
     if (spd) {
       const double djj = bcast_lane(m[j], j);
       if (djj > 1e-13 * dmax) {
-        const double inv = 1.0 / sqrt(djj);
-        invd[j] = inv;
-        const double lij = m[j] * inv;  // L[lane][j] (lane j: L[j][j])
+        const double iv = 1.0 / sqrt(djj);
+        invd[j] = iv;
+        const double lij = m[j] * iv;  // L[lane][j] (lane j: L[j][j])
         l[j] = lij;
 #pragma unroll
         for (int k = j + 1; k < K; ++k)
@@ -449,36 +566,47 @@ __global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __rest
     }
   }
   if (spd) {
-    // L y = r: y_j from lane j, removed from the lanes below (1/L_jj = invd).
-    double rv = lane < K ? r[lane] : 0.0, yv = 0.0;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      const double yj = bcast_lane(rv, j) * invd[j];
-      if (lane == j) yv = yj;
-      if (lane > j) rv = fma(-l[j], yj, rv);
-    }
-    // L^T coef = y: coef_j from lane j, removed from the lanes above; lane i
-    // needs L[j][i], a column of L: stage the rows of L in LDS.
     if (lane < K) {
 #pragma unroll
       for (int k = 0; k < K; ++k) L[lane * K + k] = l[k];
     }
-    __syncthreads();
-    double cv = 0.0;
+    wave_lds_sync();
+    // Column `lane` of X = L^{-1} by forward substitution (zero above the
+    // diagonal), into M (no longer needed) as X[i][lane].
+    double x[K];
 #pragma unroll
-    for (int j = K - 1; j >= 0; --j) {
-      const double cj = bcast_lane(yv, j) * invd[j];
-      if (lane == j) cv = cj;
-      if (lane < j) yv = fma(-L[j * K + lane], cj, yv);
+    for (int i = 0; i < K; ++i) {
+      double s = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s = fma(-L[i * K + k], x[k], s);
+      x[i] = s * invd[i];
     }
-    if (lane < K) coef[lane] = cv;
-    if (lane == 0) a.status[(size_t)t * a.G + grp] = 0;
+    wave_lds_sync();
+    if (lane < K) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) M[i * K + lane] = x[i];
+    }
+    wave_lds_sync();
+    // A^{-1}[i][lane] = sum_k X[k][i] X[k][lane].
+    if (lane < K) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) s = fma(M[k * K + i], x[k], s);
+        inv[i * K + lane] = s;
+      }
+    }
+    if (lane == 0) *status = 0;
     return;
   }
   if (lane != 0) return;
-  // Rare fallback: pivoted elimination by one lane.
-  int piv_ok = 1;
-  for (int col = 0; col < K; ++col) {
+  // Rare fallback, one lane: Gauss-Jordan on [M | I] with partial pivoting;
+  // L holds the right-hand block.
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) L[i * K + j] = i == j ? 1.0 : 0.0;
+  int ok = 1;
+  for (int col = 0; col < K && ok; ++col) {
     int best = col;
     double bv = fabs(M[col * K + col]);
     for (int i = col + 1; i < K; ++i)
@@ -487,64 +615,264 @@ __global__ __launch_bounds__(64) void rov_solve_kernel(const RovStepArgs* __rest
         best = i;
       }
     if (!(bv > 1e-300)) {
-      piv_ok = 0;
+      ok = 0;
       break;
     }
-    if (best != col) {
+    if (best != col)
       for (int j = 0; j < K; ++j) {
-        const double tmp = M[col * K + j];
+        double tmp = M[col * K + j];
         M[col * K + j] = M[best * K + j];
         M[best * K + j] = tmp;
+        tmp = L[col * K + j];
+        L[col * K + j] = L[best * K + j];
+        L[best * K + j] = tmp;
       }
-      const double tr = r[col];
-      r[col] = r[best];
-      r[best] = tr;
-    }
-    for (int i = col + 1; i < K; ++i) {
-      const double f = M[i * K + col] / M[col * K + col];
-      for (int j = col; j < K; ++j) M[i * K + j] -= f * M[col * K + j];
-      r[i] -= f * r[col];
+    const double pv = M[col * K + col];
+    for (int i = 0; i < K; ++i) {
+      if (i == col) continue;
+      const double f = M[i * K + col] / pv;
+      for (int j = 0; j < K; ++j) {
+        M[i * K + j] -= f * M[col * K + j];
+        L[i * K + j] -= f * L[col * K + j];
+      }
     }
   }
-  if (piv_ok) {
-    for (int i = K - 1; i >= 0; --i) {
-      double s = r[i];
-      for (int j = i + 1; j < K; ++j) s -= M[i * K + j] * y[j];
-      y[i] = s / M[i * K + i];
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) inv[i * K + j] = ok ? L[i * K + j] / M[i * K + i] : 0.0;
+  *status = ok ? 2 : 3;
+}
+
+// Block (step t_lo + blockIdx.x, group blockIdx.y): the fixed-order sum of
+// the step's block partials (thread (e, h) adds its 1/H of the blocks in
+// order, then the H pieces are added in order), then A^{-1} on wave 0 into
+// ainv [step][G][K][K] and status [step][G].
+template <int K>
+__global__ __launch_bounds__(1024) void rov_invert_kernel(const RovStepArgs* __restrict__ pa,
+                                                          int t_lo, int nblk_gram) {
+  constexpr int E = K * (K + 1) / 2 + K + 1;  // triangle, b, count
+  const RovStepArgs& a = *pa;
+  const int G = a.G, GE = G * E, g = blockIdx.y, t = t_lo + (int)blockIdx.x;
+  extern __shared__ double sm[];
+  constexpr int H = 1024 / E > 0 ? (1024 / E < 8 ? 1024 / E : 8) : 1;
+  double* piece = sm;                 // [H][E]
+  double* S = piece + (size_t)H * E;  // [E]
+  double* work = S + E;               // [WORK]
+  const double* src = a.partial + (size_t)blockIdx.x * nblk_gram * GE + (size_t)g * E;
+  for (int q = threadIdx.x; q < H * E; q += 1024) {
+    const int h = q / E, e = q % E;
+    const int lo = (int)((long long)nblk_gram * h / H), hi = (int)((long long)nblk_gram * (h + 1) / H);
+    piece[(size_t)h * E + e] = rov_sum_rows(src + (size_t)lo * GE + e, GE, hi - lo);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < E; e += 1024) {
+    double s = 0.0;
+    for (int h = 0; h < H; ++h) s += piece[(size_t)h * E + e];
+    S[e] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64)
+    rov_invert_wave<K>(S, work, a.ainv + ((size_t)t * G + g) * K * K, a.status + (size_t)t * G + g,
+                       threadIdx.x);
+}
+
+// The right-hand sides b of step t (sum over the step's paths of group g of
+// phi V[t+1]) from the step kernels' block partials bpart [nblk][G][K], in a
+// fixed order, then beta_{t,g} = A^{-1}_{t,g} b_{t,g} into beta_l [G][K] (LDS);
+// block 0 also stores it in beta [t].  seg holds 256 doubles of LDS.
+__device__ void rov_solve_step(const RovStepArgs& a, int t, const double* bpart, int nblk,
+                               double* seg, double* bsum, double* beta_l) {
+  const int GK = a.G * a.K, tid = threadIdx.x;
+  const int H = GK <= 256 ? 256 / GK : 1;
+  if (tid < GK * H) {
+    const int e = tid % GK, h = tid / GK;
+    const int lo = (int)((long long)nblk * h / H), hi = (int)((long long)nblk * (h + 1) / H);
+    seg[tid] = rov_sum_rows(bpart + (size_t)lo * GK + e, GK, hi - lo);
+  }
+  for (int e = 256 + tid; e < GK && H == 1; e += 256) bsum[e] = rov_sum_rows(bpart + e, GK, nblk);
+  __syncthreads();
+  if (H > 1 && tid < GK) {
+    double s = 0.0;
+    for (int h = 0; h < H; ++h) s += seg[h * GK + tid];
+    bsum[tid] = s;
+  } else if (H == 1 && tid < GK) {
+    bsum[tid] = seg[tid];
+  }
+  __syncthreads();
+  for (int q = tid; q < GK; q += 256) {
+    const int g = q / a.K, i = q % a.K;
+    const int st = a.status[(size_t)t * a.G + g];
+    double s = 0.0;
+    if (st == 0 || st == 2) {
+      const double* row = a.ainv + (((size_t)t * a.G + g) * a.K + i) * a.K;
+      const double* b = bsum + g * a.K;
+      for (int k = 0; k < a.K; ++k) s = fma(row[k], b[k], s);
     }
-    for (int i = 0; i < K; ++i) coef[i] = y[i];
-    a.status[(size_t)t * a.G + grp] = 2;
-  } else {
-    for (int i = 0; i < K; ++i) coef[i] = 0.0;
-    a.status[(size_t)t * a.G + grp] = 3;
+    beta_l[q] = s;
+    if (blockIdx.x == 0) a.beta[(size_t)t * GK + q] = s;
+  }
+  __syncthreads();
+}
+
+// One step of the backward induction (t >= 1), paths [512 b, 512 b + 512) on
+// block b, two per thread.  With decide == 1: beta_{t+1} from the previous
+// launch's partials and the decisions of step t + 1; with decide == 2: the
+// terminal decisions of step t + 1.  Then the partial right-hand sides of
+// step t for every group at once on v_mfma_f64_16x16x4_f64: operand A holds
+// basis columns 16 I + (l & 15) of path l >> 4 of four paths, operand B the
+// path's value V[t+1] in the column of its group (zero elsewhere), so column g
+// of tile I accumulates sum_{paths of g} phi_{16I+i} V[t+1]; written to
+// bpart [t & 1][b][G][K].
+template <int U, int DEG>
+__global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __restrict__ pa, int t,
+                                                       int decide) {
+  typedef double doublex4 __attribute__((ext_vector_type(4)));
+  using B = RovBasis<U, DEG>;
+  constexpr int K = B::K, NT = B::NT, U1 = U + 1;
+  const RovStepArgs& a = *pa;
+  const int G = a.G, GK = G * K, N = a.N;
+  __shared__ double xs[kRovStepPaths * U1];
+  __shared__ double ys[kRovStepPaths];
+  __shared__ double seg[256];
+  __shared__ double bsum[kRovMaxC * K];
+  __shared__ double beta_l[kRovMaxC * K];
+  __shared__ double tiles[4][NT][4][64];
+  __shared__ int gs[kRovStepPaths];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // Loads of both paths first.
+  const int p0 = blockIdx.x * kRovStepPaths + tid;
+  double xn[2][U], xc[2][U], later[2] = {0.0, 0.0};
+  int ctl[2] = {0, 0};
+  const int term = decide == 2 ? 1 : 0;
+  const bool need_later = rov_decide_reads_later(a, term);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = p0 + h * 256;
+    if (p < N) {
+      rov_load_states<U>(a, t + 1, p, xn[h]);
+      rov_load_states<U>(a, t, p, xc[h]);
+      ctl[h] = a.pooled ? 0 : a.controls[(size_t)t * N + p];
+      if (need_later) later[h] = a.V[(size_t)(t + 2) * N + p];
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) xn[h][u] = xc[h][u] = 0.0;
+    }
+  }
+  if (!term)
+    rov_solve_step(a, t + 1, a.bpart + (size_t)((t + 1) & 1) * a.nblk * GK, a.nblk, seg, bsum,
+                   beta_l);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = p0 + h * 256, q = tid + h * 256;
+    int g = -1;
+    double y = 0.0, v[U1];
+    if (p < N) {
+      int bc;
+      y = rov_decide<U, DEG>(a, t + 1, term, xn[h], beta_l, later[h], &bc);
+      a.V[(size_t)(t + 1) * N + p] = y;
+      a.optc[(size_t)(a.mode == kRovAmerican ? t : t + 1) * N + p] = bc;
+      g = rov_group_of<U>(a, xc[h], ctl[h]);
+      rov_features<U>(a, t, xc[h], v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U1; ++u) v[u] = 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U1; ++u) xs[(size_t)q * U1 + u] = v[u];
+    ys[q] = y;
+    gs[q] = g;
+  }
+  // Per-lane monomials of columns 16 I + (lane & 15).
+  const int c_lo = lane & 15, kq = lane >> 4;
+  int mi0[NT], mi1[NT], mi2[NT];
+#pragma unroll
+  for (int I = 0; I < NT; ++I) {
+    const int col = I * 16 + c_lo;
+    const int m = col < K ? a.mono[col] : 0;
+    mi0[I] = m & 255;
+    mi1[I] = (m >> 8) & 255;
+    mi2[I] = m >> 16;
+  }
+  __syncthreads();
+  // Wave w takes paths [64 w, +64) and [256 + 64 w, +64) of the block.
+  doublex4 acc[NT];
+#pragma unroll
+  for (int I = 0; I < NT; ++I) acc[I] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll 4
+    for (int r = 0; r < 16; ++r) {
+      const int pt = h * 256 + w * 64 + 4 * r + kq;
+      const double bv = gs[pt] == c_lo ? ys[pt] : 0.0;
+      const double* xr = xs + (size_t)pt * U1;
+#pragma unroll
+      for (int I = 0; I < NT; ++I) {
+        const double val = (xr[mi0[I]] * xr[mi1[I]]) * xr[mi2[I]];
+        acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(I * 16 + c_lo < K ? val : 0.0, bv, acc[I],
+                                                     0, 0, 0);
+      }
+    }
+  }
+  // Item r of lane l of tile I: row 16 I + (l >> 4) + 4 r, group l & 15.
+#pragma unroll
+  for (int I = 0; I < NT; ++I)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tiles[w][I][r][lane] = acc[I][r];
+  __syncthreads();
+  double* out = a.bpart + (size_t)(t & 1) * a.nblk * GK + (size_t)blockIdx.x * GK;
+  for (int q = tid; q < GK; q += 256) {
+    const int g = q / K, k = q % K;
+    const int I = k >> 4, ii = k & 15, r = ii >> 2, l = ((ii & 3) << 4) | g;
+    out[q] = ((tiles[0][I][r][l] + tiles[1][I][r][l]) + tiles[2][I][r][l]) + tiles[3][I][r][l];
   }
 }
 
-// Decision of step t for every path.
-// Time 0 (firstPeriodInduction, SimulateGPU.cu:2210-2330): per initial
-// control, the mean discounted value of the paths that took it, plus the
-// deterministic current payoff; out = {value, control, SD}.  One 1024-thread
-// block, one pass over the paths, fixed-order tree reduction.
-__global__ __launch_bounds__(1024) void rov_first_kernel(const RovStepArgs* __restrict__ pa) {
+// beta_1 (from the last step's partials, unless step 1 is terminal), the
+// decision of step 1 for every path, then time 0 (firstPeriodInduction,
+// SimulateGPU.cu:2210-2330): per initial control, the mean discounted value of
+// the paths that took it plus the deterministic current payoff;
+// out = {value, control, SD}.  Each block sums its paths per control (count,
+// sum, sum of squares) into fpart in a fixed order; the last block to arrive
+// (an arrival ticket, thread 0 fencing at device scope around it) sums the
+// blocks in block order and decides.
+template <int U, int DEG>
+__global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __restrict__ pa,
+                                                        int terminal) {
+  constexpr int K = RovBasis<U, DEG>::K;
   const RovStepArgs& a = *pa;
-  double* out = a.out;
-  __shared__ double s1[kRovMaxC][1024 / 64], s2[kRovMaxC][1024 / 64], sn[kRovMaxC][1024 / 64];
   const int G = (a.mode == kRovAmerican || a.pooled) ? 1 : a.C;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ double seg[256];
+  __shared__ double bsum[kRovMaxC * K];
+  __shared__ double beta_l[kRovMaxC * K];
+  __shared__ double red[3 * kRovMaxC][4];
+  __shared__ int s_last;
+  const int p = blockIdx.x * 256 + tid;
+  double xr[U], later = 0.0;
+  if (p < a.N) {
+    rov_load_states<U>(a, 1, p, xr);
+    if (rov_decide_reads_later(a, terminal)) later = a.V[(size_t)2 * a.N + p];
+  }
+  if (!terminal)
+    rov_solve_step(a, 1, a.bpart + (size_t)(1 & 1) * a.nblk * a.G * K, a.nblk, seg, bsum,
+                   beta_l);
   double x1[kRovMaxC], x2[kRovMaxC], n[kRovMaxC];
 #pragma unroll
   for (int c = 0; c < kRovMaxC; ++c) x1[c] = x2[c] = n[c] = 0.0;
-  for (int p = threadIdx.x; p < a.N; p += blockDim.x) {
+  if (p < a.N) {
+    int bc;
+    const double v1 = rov_decide<U, DEG>(a, 1, terminal, xr, beta_l, later, &bc);
+    a.V[(size_t)a.N + p] = v1;
+    a.optc[(size_t)(a.mode == kRovAmerican ? 0 : 1) * a.N + p] = bc;
     const int g = (G == 1) ? 0 : a.controls[p];
-    const double v = a.disc * a.V[(size_t)a.N + p];
+    const double v = a.disc * v1;
 #pragma unroll
     for (int c = 0; c < kRovMaxC; ++c) {
       const bool in = g == c;
-      x1[c] += in ? v : 0.0;
-      x2[c] += in ? v * v : 0.0;
-      n[c] += in ? 1.0 : 0.0;
+      x1[c] = in ? v : 0.0;
+      x2[c] = in ? v * v : 0.0;
+      n[c] = in ? 1.0 : 0.0;
     }
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < kRovMaxC; ++c) {
 #pragma unroll
@@ -554,22 +882,45 @@ __global__ __launch_bounds__(1024) void rov_first_kernel(const RovStepArgs* __re
       n[c] += __shfl_xor(n[c], off, 64);
     }
     if (lane == 0) {
-      s1[c][wave] = x1[c];
-      s2[c][wave] = x2[c];
-      sn[c][wave] = n[c];
+      red[c][w] = x1[c];
+      red[kRovMaxC + c][w] = x2[c];
+      red[2 * kRovMaxC + c][w] = n[c];
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const int nw = blockDim.x / 64;
+  const int F = 3 * kRovMaxC;
+  if (tid < F) a.fpart[(size_t)blockIdx.x * F + tid] = ((red[tid][0] + red[tid][1]) + red[tid][2]) +
+                                                       red[tid][3];
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    s_last = atomicAdd(&a.ticket[0], 1) == (int)gridDim.x - 1 ? 1 : 0;
+    if (s_last) __threadfence();
+  }
+  __syncthreads();
+  if (!s_last) return;
+  // Block sums in block order: thread (f, h) adds its tenth of the blocks of
+  // quantity f in order, then the ten pieces are added in order.
+  __shared__ double part10[3 * kRovMaxC][10];
+  __shared__ double tot[3 * kRovMaxC];
+  if (tid < F * 10) {
+    const int f = tid / 10, h = tid % 10, nb = (int)gridDim.x;
+    const int lo = (int)((long long)nb * h / 10), hi = (int)((long long)nb * (h + 1) / 10);
+    part10[f][h] = rov_sum_rows(a.fpart + (size_t)lo * F + f, F, hi - lo);
+  }
+  __syncthreads();
+  if (tid < F) {
+    double s = 0.0;
+    for (int h = 0; h < 10; ++h) s += part10[tid][h];
+    tot[tid] = s;
+  }
+  if (tid == 0) atomicExch(&a.ticket[0], 0);
+  __syncthreads();
+  if (tid != 0) return;
+  double* out = a.out;
   double mean[kRovMaxC], sd[kRovMaxC], cnt[kRovMaxC];
   for (int c = 0; c < G; ++c) {
-    double y1 = 0.0, y2 = 0.0, m = 0.0;
-    for (int w = 0; w < nw; ++w) {
-      y1 += s1[c][w];
-      y2 += s2[c][w];
-      m += sn[c][w];
-    }
+    const double y1 = tot[c], y2 = tot[kRovMaxC + c], m = tot[2 * kRovMaxC + c];
     cnt[c] = m;
     mean[c] = m > 0.0 ? y1 / m : 0.0;
     sd[c] = m > 0.0 ? sqrt(fmax(y2 / m - mean[c] * mean[c], 0.0)) : 0.0;
@@ -623,11 +974,41 @@ __global__ void exp_pv_kernel(ExpPvArgs a) {
   a.out[p] = value;
 }
 
+// Basis shapes the problem validation admits (K <= 35): degree 1 in 1..8
+// variables, degree 2 in 1..6, degree 3 in 1..4.
+#define ROV_UD_CASES(X)                                                                     \
+  X(1, 1) X(2, 1) X(3, 1) X(4, 1) X(5, 1) X(6, 1) X(7, 1) X(8, 1) X(1, 2) X(2, 2) X(3, 2)   \
+  X(4, 2) X(5, 2) X(6, 2) X(1, 3) X(2, 3) X(3, 3) X(4, 3)
+#define ROV_K_CASES(X)                                                                     \
+  X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(15) X(20) X(21) X(28) X(35)
+
+size_t rov_invert_lds(int K) {
+  const int E = K * (K + 1) / 2 + K + 1;
+  int H = 1024 / E;
+  H = H < 1 ? 1 : (H > 8 ? 8 : H);
+  return ((size_t)H * E + E + 2 * K * K + 2 * K) * sizeof(double);
+}
+
 hipError_t rov_set_lds_limits() {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rov_assemble_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)rov_assemble_lds(kRovMaxK));
-  if (e != hipSuccess) return e;
+#define ROV_LDS_UD(UU, DD)                                                                  \
+  {                                                                                         \
+    hipError_t e = hipFuncSetAttribute(                                                     \
+        reinterpret_cast<const void*>(&rov_gram_kernel<UU, DD>),                            \
+        hipFuncAttributeMaxDynamicSharedMemorySize,                                         \
+        (int)rov_gram_lds(RovBasis<UU, DD>::K, UU));                                         \
+    if (e != hipSuccess) return e;                                                          \
+  }
+  ROV_UD_CASES(ROV_LDS_UD)
+#undef ROV_LDS_UD
+#define ROV_LDS_K(KK)                                                                       \
+  {                                                                                         \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rov_invert_kernel<KK>), \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,            \
+                                       (int)rov_invert_lds(KK));                              \
+    if (e != hipSuccess) return e;                                                          \
+  }
+  ROV_K_CASES(ROV_LDS_K)
+#undef ROV_LDS_K
   return hipSuccess;
 }
 
@@ -643,47 +1024,63 @@ hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double
   return hipGetLastError();
 }
 
-size_t rov_assemble_lds(int K) {
-  return (size_t)kRovAsmPoints * (rov_kp(K) + 1) * sizeof(double) + kRovAsmPoints * sizeof(int);
-}
-
-hipError_t launch_rov_fit(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
-                          hipStream_t s) {
-  {
-    hipLaunchKernelGGL(rov_assemble_kernel, dim3(a.nblk), dim3(256), rov_assemble_lds(a.K), s,
-                       pa, t, decide_next);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(rov_reduce_kernel, dim3(a.G * a.E), dim3(64), 0, s, pa);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const size_t lds = (size_t)(2 * a.K * a.K + 2 * a.K) * sizeof(double);
-    switch (a.K) {
-#define ROV_SOLVE_CASE(KK)                                                                  \
-  case KK:                                                                                  \
-    hipLaunchKernelGGL(rov_solve_kernel<KK>, dim3(a.G), dim3(64), lds, s, pa, t);           \
-    break;
-      ROV_SOLVE_CASE(2) ROV_SOLVE_CASE(3) ROV_SOLVE_CASE(4) ROV_SOLVE_CASE(5) ROV_SOLVE_CASE(6)
-      ROV_SOLVE_CASE(7) ROV_SOLVE_CASE(8) ROV_SOLVE_CASE(9) ROV_SOLVE_CASE(10) ROV_SOLVE_CASE(15)
-      ROV_SOLVE_CASE(20) ROV_SOLVE_CASE(21) ROV_SOLVE_CASE(28) ROV_SOLVE_CASE(35)
-#undef ROV_SOLVE_CASE
-      default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+hipError_t launch_rov_gram(const RovStepArgs& a, const RovStepArgs* pa, int t_lo, int n_steps,
+                           hipStream_t s) {
+  if (n_steps <= 0) return hipSuccess;
+  const dim3 grid((a.N + 255) / 256, n_steps), inv_grid(n_steps, a.G);
+  bool found = false;
+#define ROV_GRAM_UD(UU, DD)                                                                 \
+  if (a.U == UU && a.deg == DD) {                                                           \
+    found = true;                                                                           \
+    hipLaunchKernelGGL((rov_gram_kernel<UU, DD>), grid, dim3(256),                          \
+                       rov_gram_lds(RovBasis<UU, DD>::K, UU), s, pa, t_lo);                 \
   }
-}
-
-hipError_t launch_rov_decide(const RovStepArgs& a, const RovStepArgs* pa, int t, bool terminal,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(rov_decide_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, pa, t,
-                     terminal ? 1 : 0);
+  ROV_UD_CASES(ROV_GRAM_UD)
+#undef ROV_GRAM_UD
+  if (!found) return hipErrorInvalidValue;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  switch (a.K) {
+#define ROV_INV_K(KK)                                                                       \
+  case KK:                                                                                  \
+    hipLaunchKernelGGL(rov_invert_kernel<KK>, inv_grid, dim3(1024), rov_invert_lds(KK), s,   \
+                       pa, t_lo, (int)grid.x);                                              \
+    break;
+    ROV_K_CASES(ROV_INV_K)
+#undef ROV_INV_K
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_rov_first(const RovStepArgs* pa, hipStream_t s) {
-  hipLaunchKernelGGL(rov_first_kernel, dim3(1), dim3(1024), 0, s, pa);
-  return hipGetLastError();
+hipError_t launch_rov_step(const RovStepArgs& a, const RovStepArgs* pa, int t, int decide_next,
+                           hipStream_t s) {
+  bool found = false;
+#define ROV_STEP_UD(UU, DD)                                                                 \
+  if (a.U == UU && a.deg == DD) {                                                           \
+    found = true;                                                                           \
+    hipLaunchKernelGGL((rov_step_kernel<UU, DD>), dim3(a.nblk), dim3(256), 0, s, pa, t,     \
+                       decide_next);                                                        \
+  }
+  ROV_UD_CASES(ROV_STEP_UD)
+#undef ROV_STEP_UD
+  return found ? hipGetLastError() : hipErrorInvalidValue;
+}
+
+hipError_t launch_rov_final(const RovStepArgs& a, const RovStepArgs* pa, bool terminal,
+                            hipStream_t s) {
+  const dim3 grid((a.N + 255) / 256);
+  bool found = false;
+#define ROV_FINAL_UD(UU, DD)                                                                \
+  if (a.U == UU && a.deg == DD) {                                                           \
+    found = true;                                                                           \
+    hipLaunchKernelGGL((rov_final_kernel<UU, DD>), grid, dim3(256), 0, s, pa,               \
+                       terminal ? 1 : 0);                                                   \
+  }
+  ROV_UD_CASES(ROV_FINAL_UD)
+#undef ROV_FINAL_UD
+  return found ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 hipError_t launch_exp_pv(const ExpPvArgs& a, hipStream_t s) {
