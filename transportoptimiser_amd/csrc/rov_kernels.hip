@@ -12,6 +12,7 @@
 // the whole induction.
 #include <hip/hip_runtime.h>
 
+#include "gmem.h"
 #include "rng.h"
 #include "rov_kernels.h"
 
@@ -30,64 +31,75 @@ __device__ __forceinline__ void rov_normals(uint32_t p, uint32_t t, uint32_t blk
   rmc_normal4_d(p, t, blk, tag, k0, k1, z);
 }
 
+// One step of uncertainty u's price law (kind uniform across the grid, so
+// each law is a real branch rather than computed for every path and
+// selected).
+__device__ __forceinline__ double rov_price_step(const double* q, double xv, double zc, double zj,
+                                                 double uj, double dt, double sqdt) {
+  const int kind = (int)q[0];  // q: kind, x0, mean, sd, reversion, J, p
+  if (kind == kSdeOu) return xv + q[4] * (q[2] - xv) * dt + q[3] * sqdt * zc;
+  if (kind == kSdeGbm) return xv * exp((q[2] - 0.5 * q[3] * q[3]) * dt + q[3] * sqdt * zc);
+  // SimulateGPU.cu:930-952: Brownian term x * sd * z, jump size
+  // J' = J^2 z' - J^2 / 2 applied when the uniform is below p.
+  const double jump = (uj < q[6]) ? 1.0 : 0.0;
+  const double jsz = zj * q[5] * q[5] - q[5] * q[5] / 2.0;
+  return xv + q[4] * (q[2] - xv) * dt + xv * q[3] * zc + (exp(jsz) - 1.0) * xv * jump;
+}
+
+// Price paths of U uncertainties, one path per thread.  The jump draws
+// (blocks 2-5 of the stream) are generated only when some uncertainty follows
+// the jump law; the stream is counter-based, so skipping them changes no other
+// draw.
+template <int U>
 __global__ __launch_bounds__(256) void rov_forward_kernel(RovForwardArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.N) return;
-  const int U = a.U;
-  // Per-uncertainty arrays are indexed only by unrolled loop counters, so
-  // they stay in registers.
-  double x[kRovMaxU];
+  constexpr int NB = (U + 3) / 4;  // Philox blocks of four normals
+  bool jumps = false;
 #pragma unroll
-  for (int u = 0; u < kRovMaxU; ++u) {
-    x[u] = 0.0;
-    if (u < U) {
-      x[u] = a.par[u * 7 + 1];
-      a.states[(size_t)u * a.N + p] = x[u];
-    }
+  for (int u = 0; u < U; ++u) jumps = jumps || (int)a.par[u * 7] == kSdeMrjd;
+  double x[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    x[u] = a.par[u * 7 + 1];
+    a.states[(size_t)u * a.N + p] = x[u];
   }
   const double dt = a.dt, sqdt = sqrt(a.dt);
   for (int t = 0; t < a.T; ++t) {
-    double z[kRovMaxU], zj[kRovMaxU], uj[kRovMaxU];
+    double z[4 * NB], zj[4 * NB], uj[4 * NB];
 #pragma unroll
-    for (int b = 0; b < kRovMaxU / 4; ++b) {
-      if (4 * b >= U) break;
+    for (int b = 0; b < NB; ++b) {
       double w[4];
       rov_normals((uint32_t)p, (uint32_t)t, (uint32_t)b, a.tag, a.k0, a.k1, w);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) z[4 * b + i] = w[i];
-      rov_normals((uint32_t)p, (uint32_t)t, (uint32_t)(2 + b), a.tag, a.k0, a.k1, w);
+      for (int i = 0; i < 4; ++i) {
+        z[4 * b + i] = w[i];
+        zj[4 * b + i] = 0.0;
+        uj[4 * b + i] = 1.0;
+      }
+    }
+    if (jumps) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) zj[4 * b + i] = w[i];
-      const rmc_u4 r = rmc_philox((uint32_t)p, (uint32_t)t, (uint32_t)(4 + b), a.tag, a.k0, a.k1);
-      uj[4 * b + 0] = rmc_u01d(r.x);
-      uj[4 * b + 1] = rmc_u01d(r.y);
-      uj[4 * b + 2] = rmc_u01d(r.z);
-      uj[4 * b + 3] = rmc_u01d(r.w);
+      for (int b = 0; b < NB; ++b) {
+        double w[4];
+        rov_normals((uint32_t)p, (uint32_t)t, (uint32_t)(2 + b), a.tag, a.k0, a.k1, w);
+        const rmc_u4 r = rmc_philox((uint32_t)p, (uint32_t)t, (uint32_t)(4 + b), a.tag, a.k0, a.k1);
+        const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          zj[4 * b + i] = w[i];
+          uj[4 * b + i] = rmc_u01d(rw[i]);
+        }
+      }
     }
 #pragma unroll
-    for (int u = 0; u < kRovMaxU; ++u) {
-      if (u >= U) break;
+    for (int u = 0; u < U; ++u) {
       // Correlated diffusion normal: row u of L z, L the Cholesky factor.
       double zc = 0.0;
 #pragma unroll
       for (int v = 0; v <= u; ++v) zc += a.chol[u * kRovMaxU + v] * z[v];
-      const double* q = a.par + u * 7;  // kind, x0, mean, sd, reversion, J, p
-      const int kind = (int)q[0];
-      const double xv = x[u];
-      double nx;
-      if (kind == kSdeOu) {
-        nx = xv + q[4] * (q[2] - xv) * dt + q[3] * sqdt * zc;
-      } else if (kind == kSdeGbm) {
-        nx = xv * exp((q[2] - 0.5 * q[3] * q[3]) * dt + q[3] * sqdt * zc);
-      } else {
-        // SimulateGPU.cu:930-952: Brownian term x * sd * z, jump size
-        // J' = J^2 z' - J^2 / 2 applied when the uniform is below p.
-        const double jump = (uj[u] < q[6]) ? 1.0 : 0.0;
-        const double jsz = zj[u] * q[5] * q[5] - q[5] * q[5] / 2.0;
-        nx = xv + q[4] * (q[2] - xv) * dt + xv * q[3] * zc + (exp(jsz) - 1.0) * xv * jump;
-      }
-      x[u] = nx;
-      a.states[((size_t)(t + 1) * U + u) * a.N + p] = nx;
+      x[u] = rov_price_step(a.par + u * 7, x[u], zc, zj[u], uj[u], dt, sqdt);
+      a.states[((size_t)(t + 1) * U + u) * a.N + p] = x[u];
     }
     if (a.controls) {
       // randControls (SimulateGPU.cu:720-730): floor(u C), clamped.
@@ -134,12 +146,12 @@ __global__ __launch_bounds__(1024) void rov_moments_kernel(const double* __restr
 
 __device__ __forceinline__ double rov_unit_profit(const RovStepArgs& a, int t, int p) {
   double up = a.unit_cost;
-  for (int u = 0; u < a.U; ++u) up += a.weight[u] * a.states[((size_t)t * a.U + u) * a.N + p];
+  for (int u = 0; u < a.U; ++u) up += a.weight[u] * gmem(a.states)[((size_t)t * a.U + u) * a.N + p];
   return up;
 }
 
 __device__ __forceinline__ double rov_flow_scale(const RovStepArgs& a, int t, int p) {
-  return a.traffic >= 0 ? a.states[((size_t)t * a.U + a.traffic) * a.N + p] : 1.0;
+  return a.traffic >= 0 ? gmem(a.states)[((size_t)t * a.U + a.traffic) * a.N + p] : 1.0;
 }
 
 // The polynomial basis of the regression, fixed at compile time: all
@@ -194,30 +206,69 @@ struct RovBasis {
     return tb;
   }
   static constexpr Table tab = make();
+  // Per 16-column tile I, the indices of columns 16 I + c packed 4 bits each
+  // (column c at bits 4c), so that a lane finds its column's monomial with
+  // shifts instead of a table load: lanes[w][I] for index i_w.
+  struct Lanes {
+    uint64_t v[3][NT];
+  };
+  static constexpr Lanes make_lanes() {
+    Lanes ln{};
+    for (int w = 0; w < 3; ++w)
+      for (int I = 0; I < NT; ++I) {
+        uint64_t x = 0;
+        for (int c = 0; c < 16; ++c) {
+          const int col = 16 * I + c;
+          const int idx = col < K ? (w == 0 ? tab.i0[col] : w == 1 ? tab.i1[col] : tab.i2[col]) : 0;
+          x |= (uint64_t)idx << (4 * c);
+        }
+        ln.v[w][I] = x;
+      }
+    return ln;
+  }
+  static constexpr Lanes lanes = make_lanes();
 };
+
+// Monomial indices of basis column 16 I + c_lo (zero past K) for every tile I.
+template <int U, int DEG>
+__device__ __forceinline__ void rov_lane_monomials(int c_lo, int (&m0)[RovBasis<U, DEG>::NT],
+                                                   int (&m1)[RovBasis<U, DEG>::NT],
+                                                   int (&m2)[RovBasis<U, DEG>::NT]) {
+  using B = RovBasis<U, DEG>;
+#pragma unroll
+  for (int I = 0; I < B::NT; ++I) {
+    m0[I] = (int)((B::lanes.v[0][I] >> (4 * c_lo)) & 15u);
+    m1[I] = (int)((B::lanes.v[1][I] >> (4 * c_lo)) & 15u);
+    m2[I] = (int)((B::lanes.v[2][I] >> (4 * c_lo)) & 15u);
+  }
+}
+
+// Step t's feature scaling (mean, 1/SD per uncertainty), loaded once.
+template <int U>
+__device__ __forceinline__ void rov_load_scale(const RovStepArgs& a, int t, double (&sc)[U][2]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    sc[u][0] = a.scale ? gmem(a.scale)[((size_t)t * U + u) * 2] : 0.0;
+    sc[u][1] = a.scale ? gmem(a.scale)[((size_t)t * U + u) * 2 + 1] : 1.0;
+  }
+}
 
 // Raw states of step t of path p (all U loads issued together).
 template <int U>
 __device__ __forceinline__ void rov_load_states(const RovStepArgs& a, int t, int p,
                                                 double (&x)[U]) {
 #pragma unroll
-  for (int u = 0; u < U; ++u) x[u] = a.states[((size_t)t * U + u) * a.N + p];
+  for (int u = 0; u < U; ++u) x[u] = gmem(a.states)[((size_t)t * U + u) * a.N + p];
 }
 
-// v = (1, features): the raw states standardised with step t's moments.
+// v = (1, features): the raw states standardised with step t's moments
+// (sc from rov_load_scale; mean 0 and 1/SD 1 without standardisation).
 template <int U>
-__device__ __forceinline__ void rov_features(const RovStepArgs& a, int t, const double (&xr)[U],
+__device__ __forceinline__ void rov_features(const double (&sc)[U][2], const double (&xr)[U],
                                              double (&v)[U + 1]) {
   v[0] = 1.0;
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    double x = xr[u];
-    if (a.scale) {
-      const double* sc = a.scale + ((size_t)t * U + u) * 2;
-      x = (x - sc[0]) * sc[1];
-    }
-    v[1 + u] = x;
-  }
+  for (int u = 0; u < U; ++u) v[1 + u] = (xr[u] - sc[u][0]) * sc[u][1];
 }
 
 template <int U>
@@ -244,14 +295,15 @@ __device__ __forceinline__ bool rov_decide_reads_later(const RovStepArgs& a, int
   return a.mode == kRovAmerican || a.value_mode == kValueRealized;
 }
 
-// The decision of step t for one path from its raw states xr at t, the
-// coefficients beta_t [G][K] of step t (unused at the terminal step) and,
+// The decision of step t for one path from its raw states xr at t, step t's
+// feature scaling sc, the coefficients beta_t [G][K] of step t (unused at the
+// terminal step) and,
 // where rov_decide_reads_later, later = V[t + 1] of the path; returns the
 // value and the control (American: 1 = exercise) in *bc.
 template <int U, int DEG>
-__device__ __forceinline__ double rov_decide(const RovStepArgs& a, int t, int terminal,
-                                             const double (&xr)[U], const double* beta_t,
-                                             double later, int* bc_out) {
+__device__ __forceinline__ double rov_decide(const RovStepArgs& a, int terminal,
+                                             const double (&xr)[U], const double (&sc)[U][2],
+                                             const double* beta_t, double later, int* bc_out) {
   using B = RovBasis<U, DEG>;
   constexpr int K = B::K;
   const double disc = a.disc;
@@ -264,7 +316,7 @@ __device__ __forceinline__ double rov_decide(const RovStepArgs& a, int t, int te
     int ex = 0;
     if (e > 0.0 || !a.itm_only) {
       double v[U + 1];
-      rov_features<U>(a, t, xr, v);
+      rov_features<U>(sc, xr, v);
       double cont = 0.0;
 #pragma unroll
       for (int k = 0; k < K; ++k)
@@ -282,34 +334,30 @@ __device__ __forceinline__ double rov_decide(const RovStepArgs& a, int t, int te
 #pragma unroll
   for (int u = 0; u < U; ++u)
     if (u == a.traffic) m = xr[u];
-  // Continuation value of every fit group, one pass over the basis.
-  double cont[kRovMaxC];
-#pragma unroll
-  for (int c = 0; c < kRovMaxC; ++c) cont[c] = 0.0;
+  double phi[K];
   if (!terminal) {
     double v[U + 1];
-    rov_features<U>(a, t, xr, v);
+    rov_features<U>(sc, xr, v);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double phi = (v[B::tab.i0[k]] * v[B::tab.i1[k]]) * v[B::tab.i2[k]];
-#pragma unroll
-      for (int c = 0; c < kRovMaxC; ++c)
-        if (c < a.G) cont[c] = fma(phi, beta_t[c * K + k], cont[c]);
-    }
+    for (int k = 0; k < K; ++k) phi[k] = (v[B::tab.i0[k]] * v[B::tab.i1[k]]) * v[B::tab.i2[k]];
   }
+  // One pass over the controls (a run-time loop: each control costs K
+  // multiply-adds of its fit group's continuation, not kRovMaxC of them).
   double best = 0.0, best_cur = 0.0, best_pay = 0.0;
   int bc = 0;
-#pragma unroll
-  for (int c = 0; c < kRovMaxC; ++c) {
-    if (c >= a.C) break;
+  for (int c = 0; c < a.C; ++c) {
     const double cur = a.flow[c] * m * up;
     double pay;
     if (terminal) {
       // Terminal step: payoff clamped at zero (SimulateGPU.cu:2385-2416).
       pay = fmin(cur, 0.0);
     } else {
-      double cv = cont[a.pooled ? 0 : c] * disc;  // SimulateGPU.cu:2703
-      if (cv > 0.0) cv = 0.0;                      // SimulateGPU.cu:2712-2714
+      const double* b = beta_t + (a.pooled ? 0 : c) * K;
+      double cont = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) cont = fma(phi[k], b[k], cont);
+      double cv = cont * disc;  // SimulateGPU.cu:2703
+      if (cv > 0.0) cv = 0.0;   // SimulateGPU.cu:2712-2714
       pay = cur + cv;
     }
     if (c == 0 || pay < a.hysteresis * best) {  // SimulateGPU.cu:2757-2771
@@ -340,16 +388,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Workgroup barrier ordering LDS only: unlike __syncthreads it does not wait
+// for this thread's global stores, which no other thread of the block reads.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // sum_{m < n} src[m * stride] in order m = 0, 1, ..., with the loads of
-// each run of 16 issued together.
-__device__ __forceinline__ double rov_sum_rows(const double* src, size_t stride, int n) {
+// each run of BATCH issued together (BATCH doubles of registers).
+template <int BATCH>
+__device__ __forceinline__ double rov_sum_rows(const RMC_GLOBAL double* src, size_t stride, int n) {
   double s = 0.0;
-  for (int m0 = 0; m0 < n; m0 += 16) {
-    double v[16];
+  for (int m0 = 0; m0 < n; m0 += BATCH) {
+    double v[BATCH];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = m0 + i < n ? src[(size_t)(m0 + i) * stride] : 0.0;
+    for (int i = 0; i < BATCH; ++i) v[i] = m0 + i < n ? src[(size_t)(m0 + i) * stride] : 0.0;
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
+    for (int i = 0; i < BATCH; ++i)
       if (m0 + i < n) s += v[i];
   }
   return s;
@@ -398,109 +455,118 @@ __device__ __forceinline__ void rov_entry_ij(int e, int K, int* i_out, int* j_ou
   }
 }
 
-// The matrices A of steps t_lo + blockIdx.y: block partial sums over paths
-// [256 blockIdx.x, +256) into partial [step][blk][G][E] (the b entries zero).
-// Per group, each wave lists its 64 paths of the group (ballot, in path
-// order) and takes them four at a time on v_mfma_f64_16x16x4_f64 with both
-// operands in registers: lane l computes basis columns 16 I + (l & 15) of path
-// number l >> 4 of the four (columns >= K zero), so tile (I, J) accumulates
+// The matrices A of steps t_lo + kRovGramSteps blockIdx.y + j (j <
+// kRovGramSteps, up to t_hi): block partial sums over paths [256 blockIdx.x,
+// +256) into partial [step - t_lo][blk][G][E] (the b entries zero).  The
+// block's loads for all its steps are issued first.  Per step and group,
+// each wave lists its 64 paths of the group (ballot, in path order) and takes
+// them four at a time on v_mfma_f64_16x16x4_f64 with both operands in
+// registers: lane l computes basis columns 16 I + (l & 15) of path number
+// l >> 4 of the four (columns >= K zero), so tile (I, J) accumulates
 // sum phi_{16I+i} phi_{16J+j}; item r of lane l of tile (I, J) is entry
 // (16 I + (l >> 4) + 4 r, 16 J + (l & 15)).  Only the group's own paths
 // enter, padded to a multiple of four; the four waves' tiles are summed in
 // wave order.
+constexpr int kRovGramSteps = 4;
+
 template <int U, int DEG>
 __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __restrict__ pa,
-                                                       int t_lo) {
+                                                       int t_lo, int t_hi) {
   typedef double doublex4 __attribute__((ext_vector_type(4)));
   using B = RovBasis<U, DEG>;
   constexpr int K = B::K, NT = B::NT, NP = B::NP, E = B::E, U1 = U + 1;
+  constexpr int S = kRovGramSteps;
   const RovStepArgs& a = *pa;
   const int G = a.G, GE = G * E;
-  const int t = t_lo + (int)blockIdx.y;
   extern __shared__ double sm[];
   double* xs = sm;                       // [256][U1], column 0 = 1
   double* tiles = xs + 256 * U1;         // [4][NP][4][64]
   int* gs = reinterpret_cast<int*>(sm + rov_gram_lds_doubles(K, U));  // [256]
   int* lst = gs + 256;                   // [4][64]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  {
-    const int p = blockIdx.x * 256 + tid;
-    double* row = xs + (size_t)tid * U1;
-    int g = -1;
-    double v[U1];
+  const int p = blockIdx.x * 256 + tid;
+  const int t0 = t_lo + (int)blockIdx.y * S;
+  // Loads of every step first.
+  double xr[S][U], sc[S][U][2];
+  int ctl[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int t = t0 + j <= t_hi ? t0 + j : t_hi;
+    rov_load_scale<U>(a, t, sc[j]);
+    ctl[j] = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) xr[j][u] = 0.0;
     if (p < a.N) {
-      double xr[U];
-      rov_load_states<U>(a, t, p, xr);
-      g = rov_group_of<U>(a, xr, a.pooled ? 0 : a.controls[(size_t)t * a.N + p]);
-      rov_features<U>(a, t, xr, v);
-    } else {
-#pragma unroll
-      for (int u = 0; u < U1; ++u) v[u] = 0.0;
+      rov_load_states<U>(a, t, p, xr[j]);
+      ctl[j] = a.pooled ? 0 : gmem(a.controls)[(size_t)t * a.N + p];
     }
-#pragma unroll
-    for (int u = 0; u < U1; ++u) row[u] = v[u];
-    gs[tid] = g;
   }
-  // Per-lane monomials of columns 16 I + (lane & 15).
   const int c_lo = lane & 15, kq = lane >> 4;
   int mi0[NT], mi1[NT], mi2[NT];
+  rov_lane_monomials<U, DEG>(c_lo, mi0, mi1, mi2);
 #pragma unroll
-  for (int I = 0; I < NT; ++I) {
-    const int col = I * 16 + c_lo;
-    const int m = col < K ? a.mono[col] : 0;
-    mi0[I] = m & 255;
-    mi1[I] = (m >> 8) & 255;
-    mi2[I] = m >> 16;
-  }
-  __syncthreads();
-  double* part = a.partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * GE;
-  for (int g = 0; g < G; ++g) {
-    const bool in = gs[w * 64 + lane] == g;
-    const uint64_t mask = __ballot(in);
-    const int cnt = __popcll(mask);
-    if (in) lst[w * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = lane;
-    wave_lds_sync();
-    doublex4 acc[NP];
+  for (int j = 0; j < S; ++j) {
+    const int t = t0 + j;
+    if (t > t_hi) break;
+    {
+      double v[U1];
+      rov_features<U>(sc[j], xr[j], v);
+      double* row = xs + (size_t)tid * U1;
 #pragma unroll
-    for (int pi = 0; pi < NP; ++pi) acc[pi] = doublex4{0.0, 0.0, 0.0, 0.0};
-    for (int r0 = 0; r0 < cnt; r0 += 4) {
-      const int idx = r0 + kq;
-      const int pt = w * 64 + lst[w * 64 + (idx < cnt ? idx : 0)];
-      const double* xr = xs + (size_t)pt * U1;
-      double v[NT];
+      for (int u = 0; u < U1; ++u) row[u] = p < a.N ? v[u] : 0.0;
+      gs[tid] = p < a.N ? rov_group_of<U>(a, xr[j], ctl[j]) : -1;
+    }
+    lds_barrier();
+    RMC_GLOBAL double* part = gmem(a.partial) + ((size_t)(t - t_lo) * gridDim.x + blockIdx.x) * GE;
+    for (int g = 0; g < G; ++g) {
+      const bool in = gs[w * 64 + lane] == g;
+      const uint64_t mask = __ballot(in);
+      const int cnt = __popcll(mask);
+      if (in) lst[w * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = lane;
+      wave_lds_sync();
+      doublex4 acc[NP];
 #pragma unroll
-      for (int I = 0; I < NT; ++I) {
-        const double val = (xr[mi0[I]] * xr[mi1[I]]) * xr[mi2[I]];
-        v[I] = (idx < cnt && I * 16 + c_lo < K) ? val : 0.0;
-      }
-      int pi = 0;
+      for (int pi = 0; pi < NP; ++pi) acc[pi] = doublex4{0.0, 0.0, 0.0, 0.0};
+      for (int r0 = 0; r0 < cnt; r0 += 4) {
+        const int idx = r0 + kq;
+        const int pt = w * 64 + lst[w * 64 + (idx < cnt ? idx : 0)];
+        const double* xp = xs + (size_t)pt * U1;
+        double v[NT];
 #pragma unroll
-      for (int I = 0; I < NT; ++I)
-#pragma unroll
-        for (int J = I; J < NT; ++J) {
-          acc[pi] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[pi], 0, 0, 0);
-          ++pi;
+        for (int I = 0; I < NT; ++I) {
+          const double val = (xp[mi0[I]] * xp[mi1[I]]) * xp[mi2[I]];
+          v[I] = (idx < cnt && I * 16 + c_lo < K) ? val : 0.0;
         }
-    }
+        int pi = 0;
 #pragma unroll
-    for (int pi = 0; pi < NP; ++pi)
+        for (int I = 0; I < NT; ++I)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tiles[(((size_t)w * NP + pi) * 4 + r) * 64 + lane] = acc[pi][r];
-    __syncthreads();
-    for (int e = tid; e < E; e += 256) {
-      int i, j;
-      rov_entry_ij(e, K, &i, &j);
-      double s = 0.0;
-      if (j < K) {
-        const int I = i >> 4, J = j >> 4, ii = i & 15, jj = j & 15;
-        const int pi = I * NT - I * (I - 1) / 2 + (J - I);
-        const int l = ((ii & 3) << 4) | jj, r = ii >> 2;
-#pragma unroll
-        for (int ww = 0; ww < 4; ++ww) s += tiles[(((size_t)ww * NP + pi) * 4 + r) * 64 + l];
+          for (int J = I; J < NT; ++J) {
+            acc[pi] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[pi], 0, 0, 0);
+            ++pi;
+          }
       }
-      part[(size_t)g * E + e] = s;
+#pragma unroll
+      for (int pi = 0; pi < NP; ++pi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          tiles[(((size_t)w * NP + pi) * 4 + r) * 64 + lane] = acc[pi][r];
+      lds_barrier();
+      for (int e = tid; e < E; e += 256) {
+        int i, jj;
+        rov_entry_ij(e, K, &i, &jj);
+        double s = 0.0;
+        if (jj < K) {
+          const int I = i >> 4, J = jj >> 4, ii = i & 15, c = jj & 15;
+          const int pi = I * NT - I * (I - 1) / 2 + (J - I);
+          const int l = ((ii & 3) << 4) | c, r = ii >> 2;
+#pragma unroll
+          for (int ww = 0; ww < 4; ++ww) s += tiles[(((size_t)ww * NP + pi) * 4 + r) * 64 + l];
+        }
+        part[(size_t)g * E + e] = s;
+      }
+      lds_barrier();
     }
-    __syncthreads();
   }
 }
 
@@ -516,7 +582,7 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
 // SimulateGPU.cu:6101), 2 pivoted elimination, 3 singular (A^{-1} = 0).  work
 // holds 2 K^2 + 2 K doubles of LDS.
 template <int K>
-__device__ void rov_invert_wave(const double* S, double* work, double* inv, int* status,
+__device__ void rov_invert_wave(const double* S, double* work, RMC_GLOBAL double* inv, RMC_GLOBAL int* status,
                                 int lane) {
   double* M = work;       // [K][K]
   double* L = M + K * K;  // [K][K]
@@ -657,60 +723,88 @@ __global__ __launch_bounds__(1024) void rov_invert_kernel(const RovStepArgs* __r
   double* piece = sm;                 // [H][E]
   double* S = piece + (size_t)H * E;  // [E]
   double* work = S + E;               // [WORK]
-  const double* src = a.partial + (size_t)blockIdx.x * nblk_gram * GE + (size_t)g * E;
+  const RMC_GLOBAL double* src = gmem(a.partial) + (size_t)blockIdx.x * nblk_gram * GE + (size_t)g * E;
   for (int q = threadIdx.x; q < H * E; q += 1024) {
     const int h = q / E, e = q % E;
     const int lo = (int)((long long)nblk_gram * h / H), hi = (int)((long long)nblk_gram * (h + 1) / H);
-    piece[(size_t)h * E + e] = rov_sum_rows(src + (size_t)lo * GE + e, GE, hi - lo);
+    piece[(size_t)h * E + e] = rov_sum_rows<16>(src + (size_t)lo * GE + e, GE, hi - lo);
   }
-  __syncthreads();
+  lds_barrier();
   for (int e = threadIdx.x; e < E; e += 1024) {
     double s = 0.0;
     for (int h = 0; h < H; ++h) s += piece[(size_t)h * E + e];
     S[e] = s;
   }
-  __syncthreads();
+  lds_barrier();
   if (threadIdx.x < 64)
-    rov_invert_wave<K>(S, work, a.ainv + ((size_t)t * G + g) * K * K, a.status + (size_t)t * G + g,
+    rov_invert_wave<K>(S, work, gmem(a.ainv) + ((size_t)t * G + g) * K * K, gmem(a.status) + (size_t)t * G + g,
                        threadIdx.x);
+}
+
+// The inverse rows of step t that thread tid applies in rov_solve_step
+// (entry q = tid of [G][K]; threads past G K hold zeros), and the groups'
+// solve status; loaded at kernel start so that their latency overlaps.
+template <int K>
+__device__ __forceinline__ void rov_load_inverse_row(const RovStepArgs& a, int t, double (&row)[K],
+                                                     int* st) {
+  const int q = threadIdx.x, GK = a.G * K;
+  *st = 1;
+#pragma unroll
+  for (int k = 0; k < K; ++k) row[k] = 0.0;
+  if (q < GK) {
+    const int g = q / K, i = q % K;
+    *st = gmem(a.status)[(size_t)t * a.G + g];
+    const RMC_GLOBAL double* r = gmem(a.ainv) + (((size_t)t * a.G + g) * K + i) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) row[k] = r[k];
+  }
 }
 
 // The right-hand sides b of step t (sum over the step's paths of group g of
 // phi V[t+1]) from the step kernels' block partials bpart [nblk][G][K], in a
-// fixed order, then beta_{t,g} = A^{-1}_{t,g} b_{t,g} into beta_l [G][K] (LDS);
-// block 0 also stores it in beta [t].  seg holds 256 doubles of LDS.
-__device__ void rov_solve_step(const RovStepArgs& a, int t, const double* bpart, int nblk,
-                               double* seg, double* bsum, double* beta_l) {
-  const int GK = a.G * a.K, tid = threadIdx.x;
+// fixed order (thread (e, h) adds its 1/H of the blocks in block order, then
+// the H pieces are added in order), then beta_{t,g} = A^{-1}_{t,g} b_{t,g}
+// into beta_l [G][K] (LDS); block 0 also stores it in beta [t].  row and st
+// come from rov_load_inverse_row; seg holds 256 doubles of LDS.
+template <int K>
+__device__ void rov_solve_step(const RovStepArgs& a, int t, const RMC_GLOBAL double* bpart, int nblk,
+                               double* seg, double* bsum, double* beta_l, const double (&row)[K],
+                               int st) {
+  const int GK = a.G * K, tid = threadIdx.x;
   const int H = GK <= 256 ? 256 / GK : 1;
   if (tid < GK * H) {
     const int e = tid % GK, h = tid / GK;
     const int lo = (int)((long long)nblk * h / H), hi = (int)((long long)nblk * (h + 1) / H);
-    seg[tid] = rov_sum_rows(bpart + (size_t)lo * GK + e, GK, hi - lo);
+    seg[tid] = rov_sum_rows<40>(bpart + (size_t)lo * GK + e, GK, hi - lo);
   }
-  for (int e = 256 + tid; e < GK && H == 1; e += 256) bsum[e] = rov_sum_rows(bpart + e, GK, nblk);
-  __syncthreads();
-  if (H > 1 && tid < GK) {
+  for (int e = 256 + tid; e < GK; e += 256) bsum[e] = rov_sum_rows<16>(bpart + e, GK, nblk);
+  lds_barrier();
+  if (tid < GK && tid < 256) {
     double s = 0.0;
     for (int h = 0; h < H; ++h) s += seg[h * GK + tid];
     bsum[tid] = s;
-  } else if (H == 1 && tid < GK) {
-    bsum[tid] = seg[tid];
   }
-  __syncthreads();
+  lds_barrier();
   for (int q = tid; q < GK; q += 256) {
-    const int g = q / a.K, i = q % a.K;
-    const int st = a.status[(size_t)t * a.G + g];
+    const int g = q / K;
     double s = 0.0;
-    if (st == 0 || st == 2) {
-      const double* row = a.ainv + (((size_t)t * a.G + g) * a.K + i) * a.K;
-      const double* b = bsum + g * a.K;
-      for (int k = 0; k < a.K; ++k) s = fma(row[k], b[k], s);
+    if (q == tid) {
+      if (st == 0 || st == 2) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) s = fma(row[k], bsum[g * K + k], s);
+      }
+    } else {  // G K > 256: rows past the first 256 read here
+      const int i = q % K;
+      const int sq = gmem(a.status)[(size_t)t * a.G + g];
+      if (sq == 0 || sq == 2) {
+        const RMC_GLOBAL double* r = gmem(a.ainv) + (((size_t)t * a.G + g) * K + i) * K;
+        for (int k = 0; k < K; ++k) s = fma(r[k], bsum[g * K + k], s);
+      }
     }
     beta_l[q] = s;
-    if (blockIdx.x == 0) a.beta[(size_t)t * GK + q] = s;
+    if (blockIdx.x == 0) gmem(a.beta)[(size_t)t * GK + q] = s;
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // One step of the backward induction (t >= 1), paths [512 b, 512 b + 512) on
@@ -728,8 +822,16 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
   typedef double doublex4 __attribute__((ext_vector_type(4)));
   using B = RovBasis<U, DEG>;
   constexpr int K = B::K, NT = B::NT, U1 = U + 1;
-  const RovStepArgs& a = *pa;
-  const int G = a.G, GK = G * K, N = a.N;
+  // The parameter block is copied to LDS with the first loads, so that the
+  // decisions read it there instead of stalling on scalar-cache misses.
+  __shared__ RovStepArgs sa;
+  {
+    const RMC_GLOBAL uint32_t* src = gmem(reinterpret_cast<const uint32_t*>(pa));
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&sa);
+    for (int i = threadIdx.x; i < (int)(sizeof(RovStepArgs) / 4); i += 256) dst[i] = src[i];
+  }
+  const RovStepArgs& ga = *pa;
+  const int G = ga.G, GK = G * K, N = ga.N;
   __shared__ double xs[kRovStepPaths * U1];
   __shared__ double ys[kRovStepPaths];
   __shared__ double seg[256];
@@ -740,26 +842,33 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // Loads of both paths first.
   const int p0 = blockIdx.x * kRovStepPaths + tid;
-  double xn[2][U], xc[2][U], later[2] = {0.0, 0.0};
+  double xn[2][U], xc[2][U], later[2] = {0.0, 0.0}, sc_n[U][2], sc_c[U][2];
   int ctl[2] = {0, 0};
   const int term = decide == 2 ? 1 : 0;
-  const bool need_later = rov_decide_reads_later(a, term);
+  const bool need_later = rov_decide_reads_later(ga, term);
+  rov_load_scale<U>(ga, t + 1, sc_n);
+  rov_load_scale<U>(ga, t, sc_c);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int p = p0 + h * 256;
     if (p < N) {
-      rov_load_states<U>(a, t + 1, p, xn[h]);
-      rov_load_states<U>(a, t, p, xc[h]);
-      ctl[h] = a.pooled ? 0 : a.controls[(size_t)t * N + p];
-      if (need_later) later[h] = a.V[(size_t)(t + 2) * N + p];
+      rov_load_states<U>(ga, t + 1, p, xn[h]);
+      rov_load_states<U>(ga, t, p, xc[h]);
+      ctl[h] = ga.pooled ? 0 : gmem(ga.controls)[(size_t)t * N + p];
+      if (need_later) later[h] = gmem(ga.V)[(size_t)(t + 2) * N + p];
     } else {
 #pragma unroll
       for (int u = 0; u < U; ++u) xn[h][u] = xc[h][u] = 0.0;
     }
   }
+  double arow[K];
+  int ast = 1;
+  if (!term) rov_load_inverse_row<K>(ga, t + 1, arow, &ast);
+  lds_barrier();
+  const RovStepArgs& a = sa;
   if (!term)
-    rov_solve_step(a, t + 1, a.bpart + (size_t)((t + 1) & 1) * a.nblk * GK, a.nblk, seg, bsum,
-                   beta_l);
+    rov_solve_step<K>(a, t + 1, gmem(a.bpart) + (size_t)((t + 1) & 1) * a.nblk * GK, a.nblk, seg,
+                      bsum, beta_l, arow, ast);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int p = p0 + h * 256, q = tid + h * 256;
@@ -767,11 +876,11 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
     double y = 0.0, v[U1];
     if (p < N) {
       int bc;
-      y = rov_decide<U, DEG>(a, t + 1, term, xn[h], beta_l, later[h], &bc);
-      a.V[(size_t)(t + 1) * N + p] = y;
-      a.optc[(size_t)(a.mode == kRovAmerican ? t : t + 1) * N + p] = bc;
+      y = rov_decide<U, DEG>(a, term, xn[h], sc_n, beta_l, later[h], &bc);
+      gmem(a.V)[(size_t)(t + 1) * N + p] = y;
+      gmem(a.optc)[(size_t)(a.mode == kRovAmerican ? t : t + 1) * N + p] = bc;
       g = rov_group_of<U>(a, xc[h], ctl[h]);
-      rov_features<U>(a, t, xc[h], v);
+      rov_features<U>(sc_c, xc[h], v);
     } else {
 #pragma unroll
       for (int u = 0; u < U1; ++u) v[u] = 0.0;
@@ -781,34 +890,30 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
     ys[q] = y;
     gs[q] = g;
   }
-  // Per-lane monomials of columns 16 I + (lane & 15).
+  // Monomials of this lane's columns 16 I + (lane & 15).
   const int c_lo = lane & 15, kq = lane >> 4;
   int mi0[NT], mi1[NT], mi2[NT];
+  rov_lane_monomials<U, DEG>(c_lo, mi0, mi1, mi2);
+  lds_barrier();
+  // Wave w takes paths [64 w, +64) (into acc[0]) and [256 + 64 w, +64) (into
+  // acc[1]): two independent accumulation chains, added at the end.
+  doublex4 acc[2][NT];
 #pragma unroll
-  for (int I = 0; I < NT; ++I) {
-    const int col = I * 16 + c_lo;
-    const int m = col < K ? a.mono[col] : 0;
-    mi0[I] = m & 255;
-    mi1[I] = (m >> 8) & 255;
-    mi2[I] = m >> 16;
-  }
-  __syncthreads();
-  // Wave w takes paths [64 w, +64) and [256 + 64 w, +64) of the block.
-  doublex4 acc[NT];
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-  for (int I = 0; I < NT; ++I) acc[I] = doublex4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
+    for (int I = 0; I < NT; ++I) acc[h][I] = doublex4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
-    for (int r = 0; r < 16; ++r) {
+  for (int r = 0; r < 16; ++r) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
       const int pt = h * 256 + w * 64 + 4 * r + kq;
       const double bv = gs[pt] == c_lo ? ys[pt] : 0.0;
       const double* xr = xs + (size_t)pt * U1;
 #pragma unroll
       for (int I = 0; I < NT; ++I) {
         const double val = (xr[mi0[I]] * xr[mi1[I]]) * xr[mi2[I]];
-        acc[I] = __builtin_amdgcn_mfma_f64_16x16x4f64(I * 16 + c_lo < K ? val : 0.0, bv, acc[I],
-                                                     0, 0, 0);
+        acc[h][I] = __builtin_amdgcn_mfma_f64_16x16x4f64(I * 16 + c_lo < K ? val : 0.0, bv,
+                                                        acc[h][I], 0, 0, 0);
       }
     }
   }
@@ -816,9 +921,9 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
 #pragma unroll
   for (int I = 0; I < NT; ++I)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) tiles[w][I][r][lane] = acc[I][r];
-  __syncthreads();
-  double* out = a.bpart + (size_t)(t & 1) * a.nblk * GK + (size_t)blockIdx.x * GK;
+    for (int r = 0; r < 4; ++r) tiles[w][I][r][lane] = acc[0][I][r] + acc[1][I][r];
+  lds_barrier();
+  RMC_GLOBAL double* out = gmem(a.bpart) + (size_t)(t & 1) * a.nblk * GK + (size_t)blockIdx.x * GK;
   for (int q = tid; q < GK; q += 256) {
     const int g = q / K, k = q % K;
     const int I = k >> 4, ii = k & 15, r = ii >> 2, l = ((ii & 3) << 4) | g;
@@ -847,23 +952,30 @@ __global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __res
   __shared__ double red[3 * kRovMaxC][4];
   __shared__ int s_last;
   const int p = blockIdx.x * 256 + tid;
-  double xr[U], later = 0.0;
+  double xr[U], later = 0.0, sc1[U][2];
+  rov_load_scale<U>(a, 1, sc1);
+#pragma unroll
+  for (int u = 0; u < U; ++u) xr[u] = 0.0;
   if (p < a.N) {
     rov_load_states<U>(a, 1, p, xr);
-    if (rov_decide_reads_later(a, terminal)) later = a.V[(size_t)2 * a.N + p];
+    if (rov_decide_reads_later(a, terminal)) later = gmem(a.V)[(size_t)2 * a.N + p];
   }
-  if (!terminal)
-    rov_solve_step(a, 1, a.bpart + (size_t)(1 & 1) * a.nblk * a.G * K, a.nblk, seg, bsum,
-                   beta_l);
+  if (!terminal) {
+    double arow[K];
+    int ast;
+    rov_load_inverse_row<K>(a, 1, arow, &ast);
+    rov_solve_step<K>(a, 1, gmem(a.bpart) + (size_t)(1 & 1) * a.nblk * a.G * K, a.nblk, seg, bsum,
+                      beta_l, arow, ast);
+  }
   double x1[kRovMaxC], x2[kRovMaxC], n[kRovMaxC];
 #pragma unroll
   for (int c = 0; c < kRovMaxC; ++c) x1[c] = x2[c] = n[c] = 0.0;
   if (p < a.N) {
     int bc;
-    const double v1 = rov_decide<U, DEG>(a, 1, terminal, xr, beta_l, later, &bc);
-    a.V[(size_t)a.N + p] = v1;
-    a.optc[(size_t)(a.mode == kRovAmerican ? 0 : 1) * a.N + p] = bc;
-    const int g = (G == 1) ? 0 : a.controls[p];
+    const double v1 = rov_decide<U, DEG>(a, terminal, xr, sc1, beta_l, later, &bc);
+    gmem(a.V)[(size_t)a.N + p] = v1;
+    gmem(a.optc)[(size_t)(a.mode == kRovAmerican ? 0 : 1) * a.N + p] = bc;
+    const int g = (G == 1) ? 0 : gmem(a.controls)[p];
     const double v = a.disc * v1;
 #pragma unroll
     for (int c = 0; c < kRovMaxC; ++c) {
@@ -889,7 +1001,7 @@ __global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __res
   }
   __syncthreads();
   const int F = 3 * kRovMaxC;
-  if (tid < F) a.fpart[(size_t)blockIdx.x * F + tid] = ((red[tid][0] + red[tid][1]) + red[tid][2]) +
+  if (tid < F) gmem(a.fpart)[(size_t)blockIdx.x * F + tid] = ((red[tid][0] + red[tid][1]) + red[tid][2]) +
                                                        red[tid][3];
   __syncthreads();
   if (tid == 0) {
@@ -906,7 +1018,7 @@ __global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __res
   if (tid < F * 10) {
     const int f = tid / 10, h = tid % 10, nb = (int)gridDim.x;
     const int lo = (int)((long long)nb * h / 10), hi = (int)((long long)nb * (h + 1) / 10);
-    part10[f][h] = rov_sum_rows(a.fpart + (size_t)lo * F + f, F, hi - lo);
+    part10[f][h] = rov_sum_rows<16>(gmem(a.fpart) + (size_t)lo * F + f, F, hi - lo);
   }
   __syncthreads();
   if (tid < F) {
@@ -917,7 +1029,7 @@ __global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __res
   if (tid == 0) atomicExch(&a.ticket[0], 0);
   __syncthreads();
   if (tid != 0) return;
-  double* out = a.out;
+  RMC_GLOBAL double* out = gmem(a.out);
   double mean[kRovMaxC], sd[kRovMaxC], cnt[kRovMaxC];
   for (int c = 0; c < G; ++c) {
     const double y1 = tot[c], y2 = tot[kRovMaxC + c], m = tot[2 * kRovMaxC + c];
@@ -1014,7 +1126,18 @@ hipError_t rov_set_lds_limits() {
 
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s) {
   if (a.N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(rov_forward_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
+  const dim3 grid((a.N + 255) / 256);
+  switch (a.U) {
+#define ROV_FWD_U(UU)                                                                       \
+  case UU:                                                                                  \
+    hipLaunchKernelGGL(rov_forward_kernel<UU>, grid, dim3(256), 0, s, a);                   \
+    break;
+    ROV_FWD_U(1) ROV_FWD_U(2) ROV_FWD_U(3) ROV_FWD_U(4) ROV_FWD_U(5) ROV_FWD_U(6) ROV_FWD_U(7)
+    ROV_FWD_U(8)
+#undef ROV_FWD_U
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -1027,13 +1150,15 @@ hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double
 hipError_t launch_rov_gram(const RovStepArgs& a, const RovStepArgs* pa, int t_lo, int n_steps,
                            hipStream_t s) {
   if (n_steps <= 0) return hipSuccess;
-  const dim3 grid((a.N + 255) / 256, n_steps), inv_grid(n_steps, a.G);
+  const dim3 grid((a.N + 255) / 256, (n_steps + kRovGramSteps - 1) / kRovGramSteps),
+      inv_grid(n_steps, a.G);
   bool found = false;
 #define ROV_GRAM_UD(UU, DD)                                                                 \
   if (a.U == UU && a.deg == DD) {                                                           \
     found = true;                                                                           \
     hipLaunchKernelGGL((rov_gram_kernel<UU, DD>), grid, dim3(256),                          \
-                       rov_gram_lds(RovBasis<UU, DD>::K, UU), s, pa, t_lo);                 \
+                       rov_gram_lds(RovBasis<UU, DD>::K, UU), s, pa, t_lo,                  \
+                       t_lo + n_steps - 1);                                                 \
   }
   ROV_UD_CASES(ROV_GRAM_UD)
 #undef ROV_GRAM_UD
