@@ -122,7 +122,7 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
       (rc = rmc_buf_ensure(ctx, B.b, (size_t)T * N * sizeof(int))) ||
       (rc = rmc_buf_ensure(ctx, B.c, (size_t)(T + 1) * N * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.d, (size_t)T * N * sizeof(int))) ||
-      (rc = rmc_buf_ensure(ctx, B.e, (size_t)(T + 1) * U * 2 * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, B.e, (size_t)(T + 1) * U * 18 * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.f, ((size_t)std::max(n_fit, 1) * nblk256 * GE +
                                        (size_t)T * GK * K + 2 * (size_t)nblk * GK +
                                        (size_t)nblk256 * 3 * roadmc::kRovMaxC) *
@@ -257,10 +257,10 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
       (void)hipGraphExecDestroy(RG.exec);
       RG.exec = nullptr;
     }
-    // Backward induction as one graph: the matrices of every fitted step and
-    // their inverses, then one launch per step (the coefficients of the
-    // step after it, its decisions, the right-hand sides of this step), then
-    // step 1 and time 0.
+    // Backward induction as one graph: the regression matrices of every
+    // fitted step (t_term - 1 .. 1) and their inverses, then one launch per
+    // step (the coefficients of the step after it, its decisions, the
+    // right-hand sides of this step), then step 1 and time 0.
     RMC_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t e = roadmc::launch_rov_gram(a, pa, 1, n_fit, s);
     for (int t = t_term - 1; t >= 1 && e == hipSuccess; --t)

@@ -39,7 +39,7 @@ struct RovStepArgs {
   double* V;             // [(T+1)][N]
   int* optc;             // [T][N]
   int nblk;              // blocks of the step kernel, ceil(N / kRovStepPaths)
-  double* partial;       // [steps][ceil(N / 256)][G][E] block sums of the matrices A
+  double* partial;       // [step - 1][ceil(N / 256)][G][E] block sums of the matrices A
   double* ainv;          // [T][G][K][K] inverses of A
   double* bpart;         // [2][nblk][G][K] block sums of the right-hand sides, by step parity
   double* fpart;         // [ceil(N / 256)][3][kRovMaxC] time-0 block sums
@@ -58,6 +58,7 @@ struct ExpPvArgs {
 
 hipError_t rov_set_lds_limits();
 hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s);
+// scale holds T1 U (mean, 1/SD) pairs followed by 16 T1 U doubles of scratch.
 hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double* scale,
                               hipStream_t s);
 // The kernels below read their parameters from the device copy `pa` of `a`

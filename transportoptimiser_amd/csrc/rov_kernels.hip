@@ -111,37 +111,62 @@ __global__ __launch_bounds__(256) void rov_forward_kernel(RovForwardArgs a) {
   }
 }
 
-// Mean and 1/SD of every state dimension at every step (feature scaling):
-// one 1024-thread block per (step, dimension), fixed-order tree reduction.
-__global__ __launch_bounds__(1024) void rov_moments_kernel(const double* __restrict__ states,
-                                                           int N, int U,
-                                                           double* __restrict__ scale) {
-  const int tu = blockIdx.x;  // t * U + u
-  __shared__ double s1[1024], s2[1024];
+// Mean and 1/SD of every state dimension at every step (feature scaling),
+// in two launches.  rov_moments_kernel: block (tu, c), tu = t U + u, sums
+// chunk c of the paths (each thread its paths in order, loads issued four at
+// a time, then a fixed-shape tree) into part [tu][kRovMomentChunks][2];
+// rov_moments_final_kernel adds the chunks in order.
+constexpr int kRovMomentChunks = 8;
+
+__global__ __launch_bounds__(256) void rov_moments_kernel(const double* __restrict__ states,
+                                                          int N, double* __restrict__ part) {
+  const int tu = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  __shared__ double s1[256], s2[256];
+  const int lo = (int)((long long)N * c / kRovMomentChunks);
+  const int hi = (int)((long long)N * (c + 1) / kRovMomentChunks);
   const double* x = states + (size_t)tu * N;
   double a = 0.0, b = 0.0;
-  for (int p = threadIdx.x; p < N; p += blockDim.x) {
-    const double v = x[p];
-    a += v;
-    b += v * v;
+  for (int p0 = lo + tid; p0 < hi; p0 += 4 * 256) {
+    double v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = p0 + i * 256 < hi ? x[p0 + i * 256] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a += v[i];
+      b += v[i] * v[i];
+    }
   }
-  s1[threadIdx.x] = a;
-  s2[threadIdx.x] = b;
+  s1[tid] = a;
+  s2[tid] = b;
   __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      s1[threadIdx.x] += s1[threadIdx.x + off];
-      s2[threadIdx.x] += s2[threadIdx.x + off];
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) {
+      s1[tid] += s1[tid + off];
+      s2[tid] += s2[tid + off];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    const double m = s1[0] / N;
-    const double v = s2[0] / N - m * m;
-    const double sd = v > 1e-300 ? sqrt(v) : 1.0;
-    scale[(size_t)tu * 2 + 0] = m;
-    scale[(size_t)tu * 2 + 1] = (sd > 1e-12 * (fabs(m) + 1.0)) ? 1.0 / sd : 1.0;
+  if (tid == 0) {
+    part[((size_t)tu * kRovMomentChunks + c) * 2] = s1[0];
+    part[((size_t)tu * kRovMomentChunks + c) * 2 + 1] = s2[0];
   }
+}
+
+__global__ __launch_bounds__(64) void rov_moments_final_kernel(const double* __restrict__ part,
+                                                               int N, int n_tu,
+                                                               double* __restrict__ scale) {
+  const int tu = blockIdx.x * 64 + threadIdx.x;
+  if (tu >= n_tu) return;
+  double a = 0.0, b = 0.0;
+  for (int c = 0; c < kRovMomentChunks; ++c) {
+    a += part[((size_t)tu * kRovMomentChunks + c) * 2];
+    b += part[((size_t)tu * kRovMomentChunks + c) * 2 + 1];
+  }
+  const double m = a / N;
+  const double v = b / N - m * m;
+  const double sd = v > 1e-300 ? sqrt(v) : 1.0;
+  scale[(size_t)tu * 2 + 0] = m;
+  scale[(size_t)tu * 2 + 1] = (sd > 1e-12 * (fabs(m) + 1.0)) ? 1.0 / sd : 1.0;
 }
 
 __device__ __forceinline__ double rov_unit_profit(const RovStepArgs& a, int t, int p) {
@@ -457,7 +482,7 @@ __device__ __forceinline__ void rov_entry_ij(int e, int K, int* i_out, int* j_ou
 
 // The matrices A of steps t_lo + kRovGramSteps blockIdx.y + j (j <
 // kRovGramSteps, up to t_hi): block partial sums over paths [256 blockIdx.x,
-// +256) into partial [step - t_lo][blk][G][E] (the b entries zero).  The
+// +256) into partial [step - 1][blk][G][E] (the b entries zero).  The
 // block's loads for all its steps are issued first.  Per step and group,
 // each wave lists its 64 paths of the group (ballot, in path order) and takes
 // them four at a time on v_mfma_f64_16x16x4_f64 with both operands in
@@ -504,6 +529,24 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
   const int c_lo = lane & 15, kq = lane >> 4;
   int mi0[NT], mi1[NT], mi2[NT];
   rov_lane_monomials<U, DEG>(c_lo, mi0, mi1, mi2);
+  // The tile slot of each entry this thread writes (entries e = tid + 256 m,
+  // m < ceil(E / 256)); -1: a b entry, which stays zero.
+  constexpr int ME = (E + 255) / 256;
+  int slot[ME];
+#pragma unroll
+  for (int m = 0; m < ME; ++m) {
+    const int e = tid + 256 * m;
+    slot[m] = -1;
+    if (e < E) {
+      int i, jj;
+      rov_entry_ij(e, K, &i, &jj);
+      if (jj < K) {
+        const int I = i >> 4, J = jj >> 4, ii = i & 15, c = jj & 15;
+        const int pi = I * NT - I * (I - 1) / 2 + (J - I);
+        slot[m] = (pi * 4 + (ii >> 2)) * 64 + (((ii & 3) << 4) | c);
+      }
+    }
+  }
 #pragma unroll
   for (int j = 0; j < S; ++j) {
     const int t = t0 + j;
@@ -517,7 +560,7 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
       gs[tid] = p < a.N ? rov_group_of<U>(a, xr[j], ctl[j]) : -1;
     }
     lds_barrier();
-    RMC_GLOBAL double* part = gmem(a.partial) + ((size_t)(t - t_lo) * gridDim.x + blockIdx.x) * GE;
+    RMC_GLOBAL double* part = gmem(a.partial) + ((size_t)(t - 1) * gridDim.x + blockIdx.x) * GE;
     for (int g = 0; g < G; ++g) {
       const bool in = gs[w * 64 + lane] == g;
       const uint64_t mask = __ballot(in);
@@ -552,18 +595,17 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
         for (int r = 0; r < 4; ++r)
           tiles[(((size_t)w * NP + pi) * 4 + r) * 64 + lane] = acc[pi][r];
       lds_barrier();
-      for (int e = tid; e < E; e += 256) {
-        int i, jj;
-        rov_entry_ij(e, K, &i, &jj);
-        double s = 0.0;
-        if (jj < K) {
-          const int I = i >> 4, J = jj >> 4, ii = i & 15, c = jj & 15;
-          const int pi = I * NT - I * (I - 1) / 2 + (J - I);
-          const int l = ((ii & 3) << 4) | c, r = ii >> 2;
 #pragma unroll
-          for (int ww = 0; ww < 4; ++ww) s += tiles[(((size_t)ww * NP + pi) * 4 + r) * 64 + l];
+      for (int m = 0; m < ME; ++m) {
+        const int e = tid + 256 * m;
+        if (e < E) {
+          double s = 0.0;
+          if (slot[m] >= 0) {
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) s += tiles[(size_t)ww * NP * 256 + slot[m]];
+          }
+          part[(size_t)g * E + e] = s;
         }
-        part[(size_t)g * E + e] = s;
       }
       lds_barrier();
     }
@@ -708,7 +750,7 @@ __device__ void rov_invert_wave(const double* S, double* work, RMC_GLOBAL double
   *status = ok ? 2 : 3;
 }
 
-// Block (step t_lo + blockIdx.x, group blockIdx.y): the fixed-order sum of
+// Block (step t = t_lo + blockIdx.x, group blockIdx.y): the fixed-order sum of
 // the step's block partials (thread (e, h) adds its 1/H of the blocks in
 // order, then the H pieces are added in order), then A^{-1} on wave 0 into
 // ainv [step][G][K][K] and status [step][G].
@@ -723,7 +765,7 @@ __global__ __launch_bounds__(1024) void rov_invert_kernel(const RovStepArgs* __r
   double* piece = sm;                 // [H][E]
   double* S = piece + (size_t)H * E;  // [E]
   double* work = S + E;               // [WORK]
-  const RMC_GLOBAL double* src = gmem(a.partial) + (size_t)blockIdx.x * nblk_gram * GE + (size_t)g * E;
+  const RMC_GLOBAL double* src = gmem(a.partial) + (size_t)(t - 1) * nblk_gram * GE + (size_t)g * E;
   for (int q = threadIdx.x; q < H * E; q += 1024) {
     const int h = q / E, e = q % E;
     const int lo = (int)((long long)nblk_gram * h / H), hi = (int)((long long)nblk_gram * (h + 1) / H);
@@ -1143,7 +1185,14 @@ hipError_t launch_rov_forward(const RovForwardArgs& a, hipStream_t s) {
 
 hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double* scale,
                               hipStream_t s) {
-  hipLaunchKernelGGL(rov_moments_kernel, dim3(T1 * U), dim3(1024), 0, s, states, N, U, scale);
+  // The chunk sums go after the T1 U (mean, 1/SD) pairs of scale.
+  double* part = scale + (size_t)T1 * U * 2;
+  hipLaunchKernelGGL(rov_moments_kernel, dim3(T1 * U, kRovMomentChunks), dim3(256), 0, s, states,
+                     N, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(rov_moments_final_kernel, dim3((T1 * U + 63) / 64), dim3(64), 0, s, part, N,
+                     T1 * U, scale);
   return hipGetLastError();
 }
 
