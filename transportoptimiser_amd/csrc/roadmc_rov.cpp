@@ -101,6 +101,7 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   const int G = (american || pr->pooled) ? 1 : C;
   const int nblk = (N + roadmc::kRovStepPaths - 1) / roadmc::kRovStepPaths;
   const int nblk256 = (N + 255) / 256;
+  const int nblk_gram = (N + roadmc::kRovGramPaths - 1) / roadmc::kRovGramPaths;
   const int t_term = american ? T : T - 1;
   const int n_fit = t_term - 1;  // steps 1 .. t_term - 1 are fitted
   const size_t GE = (size_t)G * E, GK = (size_t)G * K;
@@ -123,7 +124,7 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
       (rc = rmc_buf_ensure(ctx, B.c, (size_t)(T + 1) * N * sizeof(double))) ||
       (rc = rmc_buf_ensure(ctx, B.d, (size_t)T * N * sizeof(int))) ||
       (rc = rmc_buf_ensure(ctx, B.e, (size_t)(T + 1) * U * 18 * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, B.f, ((size_t)std::max(n_fit, 1) * nblk256 * GE +
+      (rc = rmc_buf_ensure(ctx, B.f, ((size_t)std::max(n_fit, 1) * nblk_gram * GE +
                                        (size_t)T * GK * K + 2 * (size_t)nblk * GK +
                                        (size_t)nblk256 * 3 * roadmc::kRovMaxC) *
                                           sizeof(double))) ||
@@ -210,24 +211,9 @@ extern "C" int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* pr,
   a.controls = d_ctrl;
   a.V = d_V;
   a.optc = d_optc;
-  {
-    // Basis columns in the order of rov_basis_visit: indices into (1, x).
-    int k = 0;
-    a.mono[k++] = 0;
-    if (pr->basis_degree >= 1)
-      for (int i = 0; i < U; ++i) a.mono[k++] = 1 + i;
-    if (pr->basis_degree >= 2)
-      for (int i = 0; i < U; ++i)
-        for (int j = i; j < U; ++j) a.mono[k++] = (1 + i) | ((1 + j) << 8);
-    if (pr->basis_degree >= 3)
-      for (int i = 0; i < U; ++i)
-        for (int j = i; j < U; ++j)
-          for (int l = j; l < U; ++l) a.mono[k++] = (1 + i) | ((1 + j) << 8) | ((1 + l) << 16);
-    if (k != K) return rmc_fail(ctx, ROADMC_ERR_INVALID, "basis enumeration mismatch");
-  }
   a.nblk = nblk;
   a.partial = static_cast<double*>(B.f.ptr);
-  a.ainv = a.partial + (size_t)std::max(n_fit, 1) * nblk256 * GE;
+  a.ainv = a.partial + (size_t)std::max(n_fit, 1) * nblk_gram * GE;
   a.bpart = a.ainv + (size_t)T * GK * K;
   a.fpart = a.bpart + 2 * (size_t)nblk * GK;
   a.ticket = d_ticket;

@@ -21,25 +21,21 @@ struct RovForwardArgs {
   int* controls;   // [T][N] or nullptr
 };
 
-constexpr int kRovStepPaths = 512;  // paths per block of the step kernel (two per thread)
-constexpr int kRovMaxCol = 48;      // basis columns + response, rounded up to 16
+constexpr int kRovStepPaths = 512;   // paths per block of the step kernel (two per thread)
+constexpr int kRovGramPaths = 1024;  // paths per block of the Gram kernel (four per thread)
 
 struct RovStepArgs {
   int N, T, U, C, G, K, E, deg, mode, value_mode, pooled, itm_only, traffic;
   double disc, hysteresis, unit_cost, strike;
   double weight[kRovMaxU];
   double flow[kRovMaxC];
-  // Basis column k as three indices into (1, x_0, .., x_{U-1}), packed
-  // i0 | i1 << 8 | i2 << 16: phi_k = (v[i0] v[i1]) v[i2], in the graded
-  // order of the basis (1, x_i, x_i x_j (i <= j), x_i x_j x_l (i <= j <= l)).
-  int mono[kRovMaxCol];
   const double* states;  // [(T+1)][U][N]
   const double* scale;   // [(T+1)][U][2] mean, 1/sd, or nullptr
   const int* controls;   // [T][N]
   double* V;             // [(T+1)][N]
   int* optc;             // [T][N]
   int nblk;              // blocks of the step kernel, ceil(N / kRovStepPaths)
-  double* partial;       // [step - 1][ceil(N / 256)][G][E] block sums of the matrices A
+  double* partial;       // [step - 1][ceil(N / kRovGramPaths)][G][E] block sums of the matrices A
   double* ainv;          // [T][G][K][K] inverses of A
   double* bpart;         // [2][nblk][G][K] block sums of the right-hand sides, by step parity
   double* fpart;         // [ceil(N / 256)][3][kRovMaxC] time-0 block sums

@@ -12,6 +12,8 @@
 // the whole induction.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "gmem.h"
 #include "rng.h"
 #include "rov_kernels.h"
@@ -194,8 +196,6 @@ struct RovBasis {
     return k;
   }
   static constexpr int K = count();
-  static constexpr int NT = (K + 15) / 16;  // 16-column tiles of the basis
-  static constexpr int NP = NT * (NT + 1) / 2;
   static constexpr int E = K * (K + 1) / 2 + K + 1;  // triangle, b, count
   struct Table {
     int i0[K], i1[K], i2[K];
@@ -231,40 +231,131 @@ struct RovBasis {
     return tb;
   }
   static constexpr Table tab = make();
-  // Per 16-column tile I, the indices of columns 16 I + c packed 4 bits each
-  // (column c at bits 4c), so that a lane finds its column's monomial with
-  // shifts instead of a table load: lanes[w][I] for index i_w.
-  struct Lanes {
-    uint64_t v[3][NT];
+  // Columns in blocks of four: lane c (0..3) of a quad of lanes computes
+  // columns 4 F + c, F < NF, of one path, whose values v = (1, x_0, ..) are
+  // spread over the quad (lane c holds v[c + 4 r] in register r < NR).
+  // idx[w][F][c] is factor w of column 4 F + c (0, i.e. v[0] = 1, past K).
+  static constexpr int NF = (K + 3) / 4;
+  static constexpr int NQ = NF * (NF + 1) / 2;  // block pairs I <= J of the Gram matrix
+  static constexpr int NR = (U + 4) / 4;        // ceil((U + 1) / 4)
+  struct Quad {
+    int idx[3][NF][4];
   };
-  static constexpr Lanes make_lanes() {
-    Lanes ln{};
-    for (int w = 0; w < 3; ++w)
-      for (int I = 0; I < NT; ++I) {
-        uint64_t x = 0;
-        for (int c = 0; c < 16; ++c) {
-          const int col = 16 * I + c;
-          const int idx = col < K ? (w == 0 ? tab.i0[col] : w == 1 ? tab.i1[col] : tab.i2[col]) : 0;
-          x |= (uint64_t)idx << (4 * c);
-        }
-        ln.v[w][I] = x;
+  static constexpr Quad make_quad() {
+    Quad q{};
+    for (int F = 0; F < NF; ++F)
+      for (int c = 0; c < 4; ++c) {
+        const int col = 4 * F + c;
+        q.idx[0][F][c] = col < K ? tab.i0[col] : 0;
+        q.idx[1][F][c] = col < K ? tab.i1[col] : 0;
+        q.idx[2][F][c] = col < K ? tab.i2[col] : 0;
       }
-    return ln;
+    return q;
   }
-  static constexpr Lanes lanes = make_lanes();
+  static constexpr Quad quad = make_quad();
 };
 
-// Monomial indices of basis column 16 I + c_lo (zero past K) for every tile I.
-template <int U, int DEG>
-__device__ __forceinline__ void rov_lane_monomials(int c_lo, int (&m0)[RovBasis<U, DEG>::NT],
-                                                   int (&m1)[RovBasis<U, DEG>::NT],
-                                                   int (&m2)[RovBasis<U, DEG>::NT]) {
+// x of quad lane CTRL-selected (DPP quad_perm: lane c reads lane (CTRL >> 2c) & 3
+// of its quad).  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_quad(double x) {
+  if constexpr (CTRL == 0xE4) {
+    return x;
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// The sum over the four lanes j + 4 b (b < 4) of x, within each row of 16
+// lanes: x plus x of the lane 4 positions along, then the same with 8 (each
+// lane ends with all four; the callers read lanes b = 0).
+__device__ __forceinline__ double dpp_row_sum4(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0x124, 0xf, 0xf, false);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0x124, 0xf, 0xf, false);
+  const double y = x + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+  const uint64_t v = __builtin_bit_cast(uint64_t, y);
+  lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, 0x128, 0xf, 0xf, false);
+  hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), 0x128, 0xf, 0xf, false);
+  return y + __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Register R's share of factor m (per quad lane): lanes whose index lives in
+// register R take it from the quad lane that holds it.
+template <int R, int NR, int M0, int M1, int M2, int M3>
+__device__ __forceinline__ void rov_quad_pick(const double (&vq)[NR], int c, double& r) {
+  constexpr bool u0 = (M0 >> 2) == R, u1 = (M1 >> 2) == R, u2 = (M2 >> 2) == R,
+                 u3 = (M3 >> 2) == R;
+  if constexpr (R < NR && (u0 || u1 || u2 || u3)) {
+    constexpr int ctrl = (u0 ? (M0 & 3) : 0) | ((u1 ? (M1 & 3) : 1) << 2) |
+                         ((u2 ? (M2 & 3) : 2) << 4) | ((u3 ? (M3 & 3) : 3) << 6);
+    const double x = dpp_quad<ctrl>(vq[R]);
+    if constexpr (u0 && u1 && u2 && u3) {
+      r = x;
+    } else {
+      constexpr int msk = (u0 ? 1 : 0) | (u1 ? 2 : 0) | (u2 ? 4 : 0) | (u3 ? 8 : 0);
+      r = ((msk >> c) & 1) ? x : r;
+    }
+  }
+}
+
+// Factor W of column 4 F + c.
+template <int U, int DEG, int F, int W>
+__device__ __forceinline__ double rov_quad_factor(const double (&vq)[RovBasis<U, DEG>::NR], int c) {
   using B = RovBasis<U, DEG>;
+  constexpr int m0 = B::quad.idx[W][F][0], m1 = B::quad.idx[W][F][1],
+                m2 = B::quad.idx[W][F][2], m3 = B::quad.idx[W][F][3];
+  if constexpr (m0 == 0 && m1 == 0 && m2 == 0 && m3 == 0) {
+    return 1.0;
+  } else {
+    constexpr int NR = B::NR;
+    double r = 0.0;
+    rov_quad_pick<0, NR, m0, m1, m2, m3>(vq, c, r);
+    rov_quad_pick<1, NR, m0, m1, m2, m3>(vq, c, r);
+    rov_quad_pick<2, NR, m0, m1, m2, m3>(vq, c, r);
+    return r;
+  }
+}
+
+// Column 4 F + c of the quad's path: (v[i0] v[i1]) v[i2] (zero past K).
+template <int U, int DEG, int F>
+__device__ __forceinline__ double rov_quad_column(const double (&vq)[RovBasis<U, DEG>::NR], int c) {
+  using B = RovBasis<U, DEG>;
+  const double f0 = rov_quad_factor<U, DEG, F, 0>(vq, c);
+  const double f1 = rov_quad_factor<U, DEG, F, 1>(vq, c);
+  const double f2 = rov_quad_factor<U, DEG, F, 2>(vq, c);
+  double f = (f0 * f1) * f2;
+  if constexpr (4 * F + 3 >= B::K) f = c < B::K - 4 * F ? f : 0.0;
+  return f;
+}
+
+template <int U, int DEG, int... Fs>
+__device__ __forceinline__ void rov_quad_columns_impl(const double (&vq)[RovBasis<U, DEG>::NR],
+                                                      int c, double (&f)[RovBasis<U, DEG>::NF],
+                                                      std::integer_sequence<int, Fs...>) {
+  ((f[Fs] = rov_quad_column<U, DEG, Fs>(vq, c)), ...);
+}
+
+// f[F] = column 4 F + c of the path whose values the quad holds in vq.
+template <int U, int DEG>
+__device__ __forceinline__ void rov_quad_columns(const double (&vq)[RovBasis<U, DEG>::NR], int c,
+                                                 double (&f)[RovBasis<U, DEG>::NF]) {
+  rov_quad_columns_impl<U, DEG>(vq, c, f, std::make_integer_sequence<int, RovBasis<U, DEG>::NF>{});
+}
+
+// This lane's values of path row `row` of xs ([.][U1], column 0 = 1) for
+// rov_quad_columns: v[c + 4 r] (zero past U).
+template <int U, int NR>
+__device__ __forceinline__ void rov_quad_load(const double* xs, int row, int c, double (&vq)[NR]) {
+  constexpr int U1 = U + 1;
 #pragma unroll
-  for (int I = 0; I < B::NT; ++I) {
-    m0[I] = (int)((B::lanes.v[0][I] >> (4 * c_lo)) & 15u);
-    m1[I] = (int)((B::lanes.v[1][I] >> (4 * c_lo)) & 15u);
-    m2[I] = (int)((B::lanes.v[2][I] >> (4 * c_lo)) & 15u);
+  for (int r = 0; r < NR; ++r) {
+    const int m = c + 4 * r;
+    vq[r] = m < U1 ? xs[(size_t)row * U1 + (m < U1 ? m : 0)] : 0.0;
   }
 }
 
@@ -448,15 +539,16 @@ __device__ __forceinline__ double rov_sum_rows(const RMC_GLOBAL double* src, siz
 // step (rov_step_kernel): the reduction of b of the step before, beta =
 // A^{-1} b, the decisions, and the partial sums of the next b.
 
-// LDS doubles of rov_gram_kernel: features [256][U+1], the waves' tiles
-// [4][NP][4][64]; then ints: 256 groups, [4][64] path lists.
+// LDS of rov_gram_kernel: values [kRovGramPaths][U+1], the waves' sums
+// [4][NQ][16]; then ints: groups and the group's path list, kRovGramPaths
+// each, and 4 wave counts.
 __host__ __device__ inline size_t rov_gram_lds_doubles(int K, int U) {
-  const int nt = (K + 15) / 16, np = nt * (nt + 1) / 2;
-  return (size_t)256 * (U + 1) + (size_t)4 * np * 4 * 64;
+  const int nf = (K + 3) / 4, nq = nf * (nf + 1) / 2;
+  return (size_t)kRovGramPaths * (U + 1) + (size_t)4 * nq * 16;
 }
 
 size_t rov_gram_lds(int K, int U) {
-  return rov_gram_lds_doubles(K, U) * sizeof(double) + (size_t)(256 + 4 * 64) * sizeof(int);
+  return rov_gram_lds_doubles(K, U) * sizeof(double) + (size_t)(2 * kRovGramPaths + 4) * sizeof(int);
 }
 
 // Entry e of a group's sums -> Gram coordinates (i, j), i <= j: the upper
@@ -480,55 +572,58 @@ __device__ __forceinline__ void rov_entry_ij(int e, int K, int* i_out, int* j_ou
   }
 }
 
-// The matrices A of steps t_lo + kRovGramSteps blockIdx.y + j (j <
-// kRovGramSteps, up to t_hi): block partial sums over paths [256 blockIdx.x,
-// +256) into partial [step - 1][blk][G][E] (the b entries zero).  The
-// block's loads for all its steps are issued first.  Per step and group,
-// each wave lists its 64 paths of the group (ballot, in path order) and takes
-// them four at a time on v_mfma_f64_16x16x4_f64 with both operands in
-// registers: lane l computes basis columns 16 I + (l & 15) of path number
-// l >> 4 of the four (columns >= K zero), so tile (I, J) accumulates
-// sum phi_{16I+i} phi_{16J+j}; item r of lane l of tile (I, J) is entry
-// (16 I + (l >> 4) + 4 r, 16 J + (l & 15)).  Only the group's own paths
-// enter, padded to a multiple of four; the four waves' tiles are summed in
-// wave order.
-constexpr int kRovGramSteps = 4;
-
+// The matrix A of step t = t_lo + blockIdx.y over paths [kRovGramPaths
+// blockIdx.x, +kRovGramPaths): block partial sums into partial [t - 1][blk]
+// [G][E] (the b entries zero).  Per group, the block lists its paths of the
+// group (in path order) and the waves take them sixteen at a time, round
+// robin, on v_mfma_f64_4x4x4f64: lane l = c + 4 b + 16 k holds, for block b
+// of the instruction, path k of that block's four paths and basis columns
+// 4 F + c (rov_quad_columns), and the NQ products of column blocks I <= J
+// accumulate sum phi_{4I+i} phi_{4J+j} in lane j + 4 b + 16 i.  The four
+// instruction blocks (different paths) are summed across lanes, then the
+// waves in wave order.
 template <int U, int DEG>
 __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __restrict__ pa,
-                                                       int t_lo, int t_hi) {
-  typedef double doublex4 __attribute__((ext_vector_type(4)));
+                                                       int t_lo) {
   using B = RovBasis<U, DEG>;
-  constexpr int K = B::K, NT = B::NT, NP = B::NP, E = B::E, U1 = U + 1;
-  constexpr int S = kRovGramSteps;
+  constexpr int K = B::K, NF = B::NF, NQ = B::NQ, NR = B::NR, E = B::E, U1 = U + 1;
+  constexpr int PB = kRovGramPaths, H = PB / 256;
   const RovStepArgs& a = *pa;
-  const int G = a.G, GE = G * E;
+  const int G = a.G, GE = G * E, N = a.N;
   extern __shared__ double sm[];
-  double* xs = sm;                       // [256][U1], column 0 = 1
-  double* tiles = xs + 256 * U1;         // [4][NP][4][64]
-  int* gs = reinterpret_cast<int*>(sm + rov_gram_lds_doubles(K, U));  // [256]
-  int* lst = gs + 256;                   // [4][64]
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int p = blockIdx.x * 256 + tid;
-  const int t0 = t_lo + (int)blockIdx.y * S;
-  // Loads of every step first.
-  double xr[S][U], sc[S][U][2];
-  int ctl[S];
+  double* xs = sm;                       // [PB][U1], column 0 = 1
+  double* tiles = xs + PB * U1;          // [4][NQ][16]
+  int* gs = reinterpret_cast<int*>(sm + rov_gram_lds_doubles(K, U));  // [PB]
+  int* lst = gs + PB;                    // [PB]
+  int* wcnt = lst + PB;                  // [4]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 3;
+  const int t = t_lo + (int)blockIdx.y;
+  const int p0 = blockIdx.x * PB + tid;
+  {
+    double xr[H][U], sc[U][2];
+    int ctl[H];
+    rov_load_scale<U>(a, t, sc);
 #pragma unroll
-  for (int j = 0; j < S; ++j) {
-    const int t = t0 + j <= t_hi ? t0 + j : t_hi;
-    rov_load_scale<U>(a, t, sc[j]);
-    ctl[j] = 0;
+    for (int h = 0; h < H; ++h) {
+      const int p = p0 + 256 * h;
+      ctl[h] = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) xr[j][u] = 0.0;
-    if (p < a.N) {
-      rov_load_states<U>(a, t, p, xr[j]);
-      ctl[j] = a.pooled ? 0 : gmem(a.controls)[(size_t)t * a.N + p];
+      for (int u = 0; u < U; ++u) xr[h][u] = 0.0;
+      if (p < N) {
+        rov_load_states<U>(a, t, p, xr[h]);
+        ctl[h] = a.pooled ? 0 : gmem(a.controls)[(size_t)t * N + p];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int p = p0 + 256 * h, q = tid + 256 * h;
+      double v[U1];
+      rov_features<U>(sc, xr[h], v);
+#pragma unroll
+      for (int u = 0; u < U1; ++u) xs[(size_t)q * U1 + u] = p < N ? v[u] : 0.0;
+      gs[q] = p < N ? rov_group_of<U>(a, xr[h], ctl[h]) : -1;
     }
   }
-  const int c_lo = lane & 15, kq = lane >> 4;
-  int mi0[NT], mi1[NT], mi2[NT];
-  rov_lane_monomials<U, DEG>(c_lo, mi0, mi1, mi2);
   // The tile slot of each entry this thread writes (entries e = tid + 256 m,
   // m < ceil(E / 256)); -1: a b entry, which stays zero.
   constexpr int ME = (E + 255) / 256;
@@ -541,74 +636,78 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
       int i, jj;
       rov_entry_ij(e, K, &i, &jj);
       if (jj < K) {
-        const int I = i >> 4, J = jj >> 4, ii = i & 15, c = jj & 15;
-        const int pi = I * NT - I * (I - 1) / 2 + (J - I);
-        slot[m] = (pi * 4 + (ii >> 2)) * 64 + (((ii & 3) << 4) | c);
+        const int I = i >> 2, J = jj >> 2;
+        const int qi = I * NF - I * (I - 1) / 2 + (J - I);
+        slot[m] = qi * 16 + (i & 3) * 4 + (jj & 3);
       }
     }
   }
+  lds_barrier();
+  RMC_GLOBAL double* part = gmem(a.partial) + ((size_t)(t - 1) * gridDim.x + blockIdx.x) * GE;
+  for (int g = 0; g < G; ++g) {
+    // The block's paths of group g, in path order: wave w lists its rows
+    // 256 h + 64 w + lane, h = 0 .. H - 1, after the lower waves' rows.
+    uint64_t mk[H];
+    int before[H], nw = 0;
 #pragma unroll
-  for (int j = 0; j < S; ++j) {
-    const int t = t0 + j;
-    if (t > t_hi) break;
-    {
-      double v[U1];
-      rov_features<U>(sc[j], xr[j], v);
-      double* row = xs + (size_t)tid * U1;
+    for (int h = 0; h < H; ++h) {
+      mk[h] = __ballot(gs[256 * h + 64 * w + lane] == g);
+      before[h] = nw;
+      nw += __popcll(mk[h]);
+    }
+    if (lane == 0) wcnt[w] = nw;
+    lds_barrier();
+    int off = 0, total = 0;
 #pragma unroll
-      for (int u = 0; u < U1; ++u) row[u] = p < a.N ? v[u] : 0.0;
-      gs[tid] = p < a.N ? rov_group_of<U>(a, xr[j], ctl[j]) : -1;
+    for (int ww = 0; ww < 4; ++ww) {
+      off += ww < w ? wcnt[ww] : 0;
+      total += wcnt[ww];
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+      if ((mk[h] >> lane) & 1ull)
+        lst[off + before[h] + __popcll(mk[h] & ((1ull << lane) - 1ull))] = 256 * h + 64 * w + lane;
+    lds_barrier();
+    double acc[NQ];
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) acc[qi] = 0.0;
+    for (int r0 = 16 * w; r0 < total; r0 += 64) {
+      const int idx = r0 + (lane >> 2);
+      const bool ok = idx < total;
+      double vq[NR], f[NF];
+      rov_quad_load<U, NR>(xs, lst[ok ? idx : 0], c, vq);
+      rov_quad_columns<U, DEG>(vq, c, f);
+#pragma unroll
+      for (int F = 0; F < NF; ++F) f[F] = ok ? f[F] : 0.0;
+      int qi = 0;
+#pragma unroll
+      for (int I = 0; I < NF; ++I)
+#pragma unroll
+        for (int J = I; J < NF; ++J) {
+          acc[qi] = __builtin_amdgcn_mfma_f64_4x4x4f64(f[I], f[J], acc[qi], 0, 0, 0);
+          ++qi;
+        }
+    }
+    // Lane j + 16 i (b = 0) of each row receives the row's four blocks.
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) {
+      const double s = dpp_row_sum4(acc[qi]);
+      if ((lane & 12) == 0) tiles[((size_t)w * NQ + qi) * 16 + (lane >> 4) * 4 + (lane & 3)] = s;
     }
     lds_barrier();
-    RMC_GLOBAL double* part = gmem(a.partial) + ((size_t)(t - 1) * gridDim.x + blockIdx.x) * GE;
-    for (int g = 0; g < G; ++g) {
-      const bool in = gs[w * 64 + lane] == g;
-      const uint64_t mask = __ballot(in);
-      const int cnt = __popcll(mask);
-      if (in) lst[w * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = lane;
-      wave_lds_sync();
-      doublex4 acc[NP];
 #pragma unroll
-      for (int pi = 0; pi < NP; ++pi) acc[pi] = doublex4{0.0, 0.0, 0.0, 0.0};
-      for (int r0 = 0; r0 < cnt; r0 += 4) {
-        const int idx = r0 + kq;
-        const int pt = w * 64 + lst[w * 64 + (idx < cnt ? idx : 0)];
-        const double* xp = xs + (size_t)pt * U1;
-        double v[NT];
+    for (int m = 0; m < ME; ++m) {
+      const int e = tid + 256 * m;
+      if (e < E) {
+        double s = 0.0;
+        if (slot[m] >= 0) {
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
-          const double val = (xp[mi0[I]] * xp[mi1[I]]) * xp[mi2[I]];
-          v[I] = (idx < cnt && I * 16 + c_lo < K) ? val : 0.0;
+          for (int ww = 0; ww < 4; ++ww) s += tiles[(size_t)ww * NQ * 16 + slot[m]];
         }
-        int pi = 0;
-#pragma unroll
-        for (int I = 0; I < NT; ++I)
-#pragma unroll
-          for (int J = I; J < NT; ++J) {
-            acc[pi] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[pi], 0, 0, 0);
-            ++pi;
-          }
+        part[(size_t)g * E + e] = s;
       }
-#pragma unroll
-      for (int pi = 0; pi < NP; ++pi)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          tiles[(((size_t)w * NP + pi) * 4 + r) * 64 + lane] = acc[pi][r];
-      lds_barrier();
-#pragma unroll
-      for (int m = 0; m < ME; ++m) {
-        const int e = tid + 256 * m;
-        if (e < E) {
-          double s = 0.0;
-          if (slot[m] >= 0) {
-#pragma unroll
-            for (int ww = 0; ww < 4; ++ww) s += tiles[(size_t)ww * NP * 256 + slot[m]];
-          }
-          part[(size_t)g * E + e] = s;
-        }
-      }
-      lds_barrier();
     }
+    lds_barrier();
   }
 }
 
@@ -861,9 +960,8 @@ __device__ void rov_solve_step(const RovStepArgs& a, int t, const RMC_GLOBAL dou
 template <int U, int DEG>
 __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __restrict__ pa, int t,
                                                        int decide) {
-  typedef double doublex4 __attribute__((ext_vector_type(4)));
   using B = RovBasis<U, DEG>;
-  constexpr int K = B::K, NT = B::NT, U1 = U + 1;
+  constexpr int K = B::K, NF = B::NF, NR = B::NR, U1 = U + 1;
   // The parameter block is copied to LDS with the first loads, so that the
   // decisions read it there instead of stalling on scalar-cache misses.
   __shared__ RovStepArgs sa;
@@ -879,7 +977,7 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
   __shared__ double seg[256];
   __shared__ double bsum[kRovMaxC * K];
   __shared__ double beta_l[kRovMaxC * K];
-  __shared__ double tiles[4][NT][4][64];
+  __shared__ double tiles[4][2][NF][16];
   __shared__ int gs[kRovStepPaths];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // Loads of both paths first.
@@ -932,44 +1030,54 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
     ys[q] = y;
     gs[q] = g;
   }
-  // Monomials of this lane's columns 16 I + (lane & 15).
-  const int c_lo = lane & 15, kq = lane >> 4;
-  int mi0[NT], mi1[NT], mi2[NT];
-  rov_lane_monomials<U, DEG>(c_lo, mi0, mi1, mi2);
   lds_barrier();
-  // Wave w takes paths [64 w, +64) (into acc[0]) and [256 + 64 w, +64) (into
-  // acc[1]): two independent accumulation chains, added at the end.
-  doublex4 acc[2][NT];
+  // Wave w takes rows 256 h + 64 w + 16 r + (lane >> 2) (h < 2, r < 4): on
+  // v_mfma_f64_4x4x4f64 lane l = c + 4 b + 16 k holds basis columns 4 F + c
+  // (operand A) and the row's value in the column of its group, 4 P + c
+  // (operand B), of path k of block b's four; pass P = 1 only with more
+  // than four groups.
+  const int c = lane & 3;
+  const bool two = G > 4;
+  double acc[2][NF];
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int P = 0; P < 2; ++P)
 #pragma unroll
-    for (int I = 0; I < NT; ++I) acc[h][I] = doublex4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-  for (int r = 0; r < 16; ++r) {
+    for (int F = 0; F < NF; ++F) acc[P][F] = 0.0;
+#pragma unroll 2
+  for (int r = 0; r < 8; ++r) {
+    const int row = (r >> 2) * 256 + w * 64 + (r & 3) * 16 + (lane >> 2);
+    double vq[NR], f[NF];
+    rov_quad_load<U, NR>(xs, row, c, vq);
+    rov_quad_columns<U, DEG>(vq, c, f);
+    const int gr = gs[row];
+    const double yr = ys[row];
+    const double b0 = gr == c ? yr : 0.0;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int pt = h * 256 + w * 64 + 4 * r + kq;
-      const double bv = gs[pt] == c_lo ? ys[pt] : 0.0;
-      const double* xr = xs + (size_t)pt * U1;
+    for (int F = 0; F < NF; ++F) acc[0][F] = __builtin_amdgcn_mfma_f64_4x4x4f64(f[F], b0, acc[0][F], 0, 0, 0);
+    if (two) {
+      const double b1 = gr == 4 + c ? yr : 0.0;
 #pragma unroll
-      for (int I = 0; I < NT; ++I) {
-        const double val = (xr[mi0[I]] * xr[mi1[I]]) * xr[mi2[I]];
-        acc[h][I] = __builtin_amdgcn_mfma_f64_16x16x4f64(I * 16 + c_lo < K ? val : 0.0, bv,
-                                                        acc[h][I], 0, 0, 0);
-      }
+      for (int F = 0; F < NF; ++F)
+        acc[1][F] = __builtin_amdgcn_mfma_f64_4x4x4f64(f[F], b1, acc[1][F], 0, 0, 0);
     }
   }
-  // Item r of lane l of tile I: row 16 I + (l >> 4) + 4 r, group l & 15.
+  // Lane j + 16 i of each row receives the sum of its row's four blocks:
+  // column 4 F + i, group 4 P + j.
 #pragma unroll
-  for (int I = 0; I < NT; ++I)
+  for (int P = 0; P < 2; ++P) {
+    if (P == 1 && !two) break;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) tiles[w][I][r][lane] = acc[0][I][r] + acc[1][I][r];
+    for (int F = 0; F < NF; ++F) {
+      const double sum = dpp_row_sum4(acc[P][F]);
+      if ((lane & 12) == 0) tiles[w][P][F][(lane >> 4) * 4 + c] = sum;
+    }
+  }
   lds_barrier();
   RMC_GLOBAL double* out = gmem(a.bpart) + (size_t)(t & 1) * a.nblk * GK + (size_t)blockIdx.x * GK;
   for (int q = tid; q < GK; q += 256) {
     const int g = q / K, k = q % K;
-    const int I = k >> 4, ii = k & 15, r = ii >> 2, l = ((ii & 3) << 4) | g;
-    out[q] = ((tiles[0][I][r][l] + tiles[1][I][r][l]) + tiles[2][I][r][l]) + tiles[3][I][r][l];
+    const int P = g >> 2, F = k >> 2, l = (k & 3) * 4 + (g & 3);
+    out[q] = ((tiles[0][P][F][l] + tiles[1][P][F][l]) + tiles[2][P][F][l]) + tiles[3][P][F][l];
   }
 }
 
@@ -1199,15 +1307,13 @@ hipError_t launch_rov_moments(const double* states, int N, int U, int T1, double
 hipError_t launch_rov_gram(const RovStepArgs& a, const RovStepArgs* pa, int t_lo, int n_steps,
                            hipStream_t s) {
   if (n_steps <= 0) return hipSuccess;
-  const dim3 grid((a.N + 255) / 256, (n_steps + kRovGramSteps - 1) / kRovGramSteps),
-      inv_grid(n_steps, a.G);
+  const dim3 grid((a.N + kRovGramPaths - 1) / kRovGramPaths, n_steps), inv_grid(n_steps, a.G);
   bool found = false;
 #define ROV_GRAM_UD(UU, DD)                                                                 \
   if (a.U == UU && a.deg == DD) {                                                           \
     found = true;                                                                           \
     hipLaunchKernelGGL((rov_gram_kernel<UU, DD>), grid, dim3(256),                          \
-                       rov_gram_lds(RovBasis<UU, DD>::K, UU), s, pa, t_lo,                  \
-                       t_lo + n_steps - 1);                                                 \
+                       rov_gram_lds(RovBasis<UU, DD>::K, UU), s, pa, t_lo);                 \
   }
   ROV_UD_CASES(ROV_GRAM_UD)
 #undef ROV_GRAM_UD
