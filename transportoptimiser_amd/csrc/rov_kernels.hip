@@ -512,11 +512,11 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// sum_{m < n} src[m * stride] in order m = 0, 1, ..., with the loads of
-// each run of BATCH issued together (BATCH doubles of registers).
+// s + sum_{m < n} src[m * stride], added in order m = 0, 1, ..., with the
+// loads of each run of BATCH issued together (BATCH doubles of registers).
 template <int BATCH>
-__device__ __forceinline__ double rov_sum_rows(const RMC_GLOBAL double* src, size_t stride, int n) {
-  double s = 0.0;
+__device__ __forceinline__ double rov_sum_rows(const RMC_GLOBAL double* src, size_t stride, int n,
+                                               double s = 0.0) {
   for (int m0 = 0; m0 < n; m0 += BATCH) {
     double v[BATCH];
 #pragma unroll
@@ -882,7 +882,7 @@ __global__ __launch_bounds__(1024) void rov_invert_kernel(const RovStepArgs* __r
                        threadIdx.x);
 }
 
-// The inverse rows of step t that thread tid applies in rov_solve_step
+// The inverse rows of step t that thread tid applies in rov_solve_finish
 // (entry q = tid of [G][K]; threads past G K hold zeros), and the groups'
 // solve status; loaded at kernel start so that their latency overlaps.
 template <int K>
@@ -901,25 +901,63 @@ __device__ __forceinline__ void rov_load_inverse_row(const RovStepArgs& a, int t
   }
 }
 
-// The right-hand sides b of step t (sum over the step's paths of group g of
+// The right-hand sides b of a step (sum over the step's paths of group g of
 // phi V[t+1]) from the step kernels' block partials bpart [nblk][G][K], in a
-// fixed order (thread (e, h) adds its 1/H of the blocks in block order, then
-// the H pieces are added in order), then beta_{t,g} = A^{-1}_{t,g} b_{t,g}
-// into beta_l [G][K] (LDS); block 0 also stores it in beta [t].  row and st
-// come from rov_load_inverse_row; seg holds 256 doubles of LDS.
+// fixed order: thread (e, h) adds its 1/H of the blocks in block order into
+// seg (entries past the first 256 go whole into bsum); rov_solve_finish,
+// after a barrier, adds the H pieces in order.  The first kRovPrefetch rows
+// of the thread's share are loaded by rov_partials_issue at kernel start and
+// added by rov_partials_sum, so that the kernel's other loads are issued
+// while they are in flight.
+constexpr int kRovPrefetch = 40;
+
+struct RovPartials {
+  double v[kRovPrefetch];
+  const RMC_GLOBAL double* src;  // the thread's first row, or null
+  int rows;
+};
+
 template <int K>
-__device__ void rov_solve_step(const RovStepArgs& a, int t, const RMC_GLOBAL double* bpart, int nblk,
-                               double* seg, double* bsum, double* beta_l, const double (&row)[K],
-                               int st) {
-  const int GK = a.G * K, tid = threadIdx.x;
+__device__ __forceinline__ void rov_partials_issue(int G, const RMC_GLOBAL double* bpart, int nblk,
+                                                   RovPartials& pf) {
+  const int GK = G * K, tid = threadIdx.x;
   const int H = GK <= 256 ? 256 / GK : 1;
+  pf.src = nullptr;
+  pf.rows = 0;
   if (tid < GK * H) {
     const int e = tid % GK, h = tid / GK;
     const int lo = (int)((long long)nblk * h / H), hi = (int)((long long)nblk * (h + 1) / H);
-    seg[tid] = rov_sum_rows<40>(bpart + (size_t)lo * GK + e, GK, hi - lo);
+    pf.src = bpart + (size_t)lo * GK + e;
+    pf.rows = hi - lo;
+  }
+#pragma unroll
+  for (int i = 0; i < kRovPrefetch; ++i) pf.v[i] = i < pf.rows ? pf.src[(size_t)i * GK] : 0.0;
+}
+
+template <int K>
+__device__ __forceinline__ void rov_partials_sum(int G, const RMC_GLOBAL double* bpart, int nblk,
+                                                 const RovPartials& pf, double* seg, double* bsum) {
+  const int GK = G * K, tid = threadIdx.x;
+  if (pf.src) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kRovPrefetch; ++i)
+      if (i < pf.rows) s += pf.v[i];
+    if (pf.rows > kRovPrefetch)
+      s = rov_sum_rows<16>(pf.src + (size_t)kRovPrefetch * GK, GK, pf.rows - kRovPrefetch, s);
+    seg[tid] = s;
   }
   for (int e = 256 + tid; e < GK; e += 256) bsum[e] = rov_sum_rows<16>(bpart + e, GK, nblk);
-  lds_barrier();
+}
+
+// b from rov_partials_sum's pieces (seg and bsum visible to the block), then
+// beta_{t,g} = A^{-1}_{t,g} b_{t,g} into beta_l [G][K] (LDS); block 0 also
+// stores it in beta [t].  row and st come from rov_load_inverse_row.
+template <int K>
+__device__ void rov_solve_finish(const RovStepArgs& a, int t, const double* seg, double* bsum,
+                                 double* beta_l, const double (&row)[K], int st) {
+  const int GK = a.G * K, tid = threadIdx.x;
+  const int H = GK <= 256 ? 256 / GK : 1;
   if (tid < GK && tid < 256) {
     double s = 0.0;
     for (int h = 0; h < H; ++h) s += seg[h * GK + tid];
@@ -962,14 +1000,14 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
                                                        int decide) {
   using B = RovBasis<U, DEG>;
   constexpr int K = B::K, NF = B::NF, NR = B::NR, U1 = U + 1;
-  // The parameter block is copied to LDS with the first loads, so that the
-  // decisions read it there instead of stalling on scalar-cache misses.
+  // The parameter block is copied to LDS (one word per thread, stored after
+  // the kernel's other loads are issued), so that the decisions read it
+  // there instead of stalling on scalar-cache misses.
   __shared__ RovStepArgs sa;
-  {
-    const RMC_GLOBAL uint32_t* src = gmem(reinterpret_cast<const uint32_t*>(pa));
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&sa);
-    for (int i = threadIdx.x; i < (int)(sizeof(RovStepArgs) / 4); i += 256) dst[i] = src[i];
-  }
+  constexpr int kWords = (int)(sizeof(RovStepArgs) / 4);
+  static_assert(sizeof(RovStepArgs) % 4 == 0 && kWords <= 256, "one word per thread");
+  const uint32_t pword =
+      threadIdx.x < kWords ? gmem(reinterpret_cast<const uint32_t*>(pa))[threadIdx.x] : 0u;
   const RovStepArgs& ga = *pa;
   const int G = ga.G, GK = G * K, N = ga.N;
   __shared__ double xs[kRovStepPaths * U1];
@@ -980,11 +1018,19 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
   __shared__ double tiles[4][2][NF][16];
   __shared__ int gs[kRovStepPaths];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // Loads of both paths first.
+  const int term = decide == 2 ? 1 : 0;
+  // The previous launch's partials (the critical path), then the inverse
+  // rows, then both paths' states.
+  const RMC_GLOBAL double* bprev = gmem(ga.bpart) + (size_t)((t + 1) & 1) * ga.nblk * GK;
+  RovPartials pf;
+  pf.src = nullptr;
+  if (!term) rov_partials_issue<K>(G, bprev, ga.nblk, pf);
+  double arow[K];
+  int ast = 1;
+  if (!term) rov_load_inverse_row<K>(ga, t + 1, arow, &ast);
   const int p0 = blockIdx.x * kRovStepPaths + tid;
   double xn[2][U], xc[2][U], later[2] = {0.0, 0.0}, sc_n[U][2], sc_c[U][2];
   int ctl[2] = {0, 0};
-  const int term = decide == 2 ? 1 : 0;
   const bool need_later = rov_decide_reads_later(ga, term);
   rov_load_scale<U>(ga, t + 1, sc_n);
   rov_load_scale<U>(ga, t, sc_c);
@@ -1001,14 +1047,11 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
       for (int u = 0; u < U; ++u) xn[h][u] = xc[h][u] = 0.0;
     }
   }
-  double arow[K];
-  int ast = 1;
-  if (!term) rov_load_inverse_row<K>(ga, t + 1, arow, &ast);
+  if (tid < kWords) reinterpret_cast<uint32_t*>(&sa)[tid] = pword;
+  if (!term) rov_partials_sum<K>(G, bprev, ga.nblk, pf, seg, bsum);
   lds_barrier();
   const RovStepArgs& a = sa;
-  if (!term)
-    rov_solve_step<K>(a, t + 1, gmem(a.bpart) + (size_t)((t + 1) & 1) * a.nblk * GK, a.nblk, seg,
-                      bsum, beta_l, arow, ast);
+  if (!term) rov_solve_finish<K>(a, t + 1, seg, bsum, beta_l, arow, ast);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int p = p0 + h * 256, q = tid + h * 256;
@@ -1114,8 +1157,12 @@ __global__ __launch_bounds__(256) void rov_final_kernel(const RovStepArgs* __res
     double arow[K];
     int ast;
     rov_load_inverse_row<K>(a, 1, arow, &ast);
-    rov_solve_step<K>(a, 1, gmem(a.bpart) + (size_t)(1 & 1) * a.nblk * a.G * K, a.nblk, seg, bsum,
-                      beta_l, arow, ast);
+    const RMC_GLOBAL double* b1 = gmem(a.bpart) + (size_t)(1 & 1) * a.nblk * a.G * K;
+    RovPartials pf;
+    rov_partials_issue<K>(a.G, b1, a.nblk, pf);
+    rov_partials_sum<K>(a.G, b1, a.nblk, pf, seg, bsum);
+    lds_barrier();
+    rov_solve_finish<K>(a, 1, seg, bsum, beta_l, arow, ast);
   }
   double x1[kRovMaxC], x2[kRovMaxC], n[kRovMaxC];
 #pragma unroll
