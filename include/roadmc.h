@@ -112,7 +112,7 @@ typedef struct {
 
 /* ---- context ---------------------------------------------------------- */
 
-/* SimulateGPU::deviceCount (SimulateGPU.h:197). */
+/* SimulateGPU::deviceCount (SimulateGPU.h:44). */
 int roadmc_device_count(int* count);
 int roadmc_ctx_create(int device, roadmc_ctx** out);
 void roadmc_ctx_destroy(roadmc_ctx* ctx);
@@ -123,7 +123,7 @@ int roadmc_debug_fail_allocs(roadmc_ctx* ctx, int n);
 
 /* ---- MTE ---------------------------------------------------------------- */
 
-/* SimulateGPU::simulateMTECUDA (SimulateGPU.h:305-309, SimulateGPU.cu:
+/* SimulateGPU::simulateMTECUDA (SimulateGPU.h:152-156, SimulateGPU.cu:
  * 3650-3864): all species of one road on the fused tiled kernels. */
 int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
                    roadmc_mte_result* result);
@@ -337,13 +337,13 @@ typedef struct {
 /* Number of basis functions of a degree-d polynomial in n variables. */
 int roadmc_rov_basis_size(int32_t n_vars, int32_t degree, int32_t* size);
 
-/* SimulateGPU::simulateROVCUDA (SimulateGPU.h:340-344): forward paths, then
+/* SimulateGPU::simulateROVCUDA (SimulateGPU.h:187-191): forward paths, then
  * backward induction with per-step FP64 regression, captured in one HIP
  * graph. */
 int roadmc_rov_run(roadmc_ctx* ctx, const roadmc_rov_problem* problem,
                    roadmc_rov_result* result);
 
-/* SimulateGPU::expPV (SimulateGPU.h:206, SimulateGPU.cu:3163-3259): expected
+/* SimulateGPU::expPV (SimulateGPU.h:53, SimulateGPU.cu:3163-3259): expected
  * present value sum_t (1 + g)^t x_t / (1 + r)^t of one uncontrolled MRJD
  * price with Brownian SD s*dt (the reference's parameterisation), and its SD
  * over paths (1/N; the reference omits the 1/N, Uncertainty.cpp:75). */

@@ -1,7 +1,7 @@
 """Python host API over libroadmc, shaped like the reference's dispatch layer.
 
 ``Context.simulate_mte`` plays the role of ``SimulateGPU::simulateMTECUDA``
-(include/gpuCode/SimulateGPU.h:305-309): it takes the species-road arrays and
+(include/gpuCode/SimulateGPU.h:152-156): it takes the species-road arrays and
 returns the per-path end populations (the reference's ``endPops`` N x S
 matrix) together with the statistics ``Simulator::simulateMTE`` derives from
 them (Simulator.cpp:184-212).

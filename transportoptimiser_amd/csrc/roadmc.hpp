@@ -1,5 +1,5 @@
 // C++ host interface over the C ABI, with the call shape of the reference's
-// SimulateGPU class (include/gpuCode/SimulateGPU.h:197-344) so that a
+// SimulateGPU class (include/gpuCode/SimulateGPU.h:44-191) so that a
 // Simulator-style caller can switch to it: the same argument order and
 // meaning, caller-sized outputs, and errors thrown as `const char*` as
 // gpuAssert does (SimulateGPU.cu:14-26).  Eigen and the Simulator object graph
@@ -11,6 +11,7 @@
 #ifndef ROADMC_HPP
 #define ROADMC_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -39,6 +40,11 @@ struct SpeciesRoadPatches {
   double speciesParams[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double noiseSD = 0.1;
   double thresholdRatio = 0.5;
+  // Patch populations and capacities: the species-state simulateROVCUDA
+  // reads them from the road's habitat patches (SimulateGPU.cu:3965-3989);
+  // simulateMTECUDA takes them as arguments, as the reference does.
+  std::vector<double> initPops;
+  std::vector<double> capacities;
 };
 
 // Column-major int matrix (Eigen::MatrixXi's storage order).
@@ -81,6 +87,32 @@ struct RovInputs {
   uint64_t seed = 0;
 };
 
+// Inputs of the species-state valuation (ROVMethod ALGO4 / ALGO6) that the
+// reference reads through sim->getRoad()->getOptimiser(), its traffic
+// program and its commodities (SimulateGPU.cu:3880-4070).
+struct RovSpeciesInputs {
+  std::vector<double> flows;                      // per control; the last is the highest flow
+  int nFuels = 0;                                 // uncertainties 0 .. nFuels - 1 are fuels
+  std::vector<roadmc_uncertainty> uncertainties;  // fuels, then one per commodity
+  std::vector<double> oreMean, oreSD;             // per commodity
+  std::vector<double> fuelCost;                   // per vehicle class
+  std::vector<int32_t> fuelIdx;                   // per vehicle class: its fuel
+  double unitCost = 0.0;
+  double unitRevenue = 0.0;
+  int nYears = 50;
+  double timeStep = 1.0;
+  double rrr = 7.0;
+  int64_t noPaths = 10000;
+  int dimRes = 20;
+  double profitScaling = 2.0;  // SimulateGPU.cu:3900
+  double hysteresis = 0.975;   // SimulateGPU.cu:2763
+  uint64_t seed = 0;
+  uint32_t roadId = 0;
+  int method = ROADMC_ROVS_ALGO4;
+  int checkpointYears = 0;
+  int precision = ROADMC_PREC_FP32;
+};
+
 // The road attributes simulateROVCUDA stores (SimulateGPU.cu:5461-5490).
 struct RovStats {
   double value = 0.0;
@@ -102,14 +134,14 @@ struct SimulatorInputs {
 
 class SimulateGPU {
  public:
-  // SimulateGPU::deviceCount (SimulateGPU.h:197).
+  // SimulateGPU::deviceCount (SimulateGPU.h:44).
   static int deviceCount() {
     int n = 0;
     roadmc_device_count(&n);
     return n;
   }
 
-  // SimulateGPU::simulateMTECUDA (SimulateGPU.h:305-309): endPops must be
+  // SimulateGPU::simulateMTECUDA (SimulateGPU.h:152-156): endPops must be
   // noPaths x srp.size(); column s receives species s's end totals.
   static void simulateMTECUDA(const SimulatorInputs& sim, std::vector<SpeciesRoadPatches>& srp,
                               std::vector<std::vector<double>>& initPops,
@@ -155,7 +187,7 @@ class SimulateGPU {
     if (roadmc_mte_run(ctx, &pr, &res) != ROADMC_OK) fail(roadmc_last_error(ctx));
   }
 
-  // SimulateGPU::simulateROVCUDA (SimulateGPU.h:340-344) for the price-state
+  // SimulateGPU::simulateROVCUDA (SimulateGPU.h:187-191) for the price-state
   // model: condExp is (nYears+1) x noPaths and optCont nYears x noPaths
   // (step-major, like the reference's T x N matrices); regressions receives
   // nYears * controls * K coefficients.  Outputs are resized here.
@@ -217,7 +249,116 @@ class SimulateGPU {
     stats = st;
   }
 
-  // SimulateGPU::expPV (SimulateGPU.h:206): stores the expected present value
+  // SimulateGPU::simulateROVCUDA (SimulateGPU.h:187-191, SimulateGPU.cu:
+  // 3870-5525) for the reference's species states, ALGO4 or ALGO6 by
+  // sim.method.  Outputs are resized here to the reference's shapes:
+  // adjPops[t] is noPaths x species (the highest-flow adjusted populations of
+  // year t), unitProfits, condExp and optCont are noPaths x nYears, and
+  // regressions holds each year's and control's fitted grid, nYears x
+  // controls x (dims dimRes + dimRes^dims) with dims = species + 1 (the
+  // grid axes, then the grid values; the reference's vector reserves a
+  // second dimRes^dims block per grid that it never fills).  plotResults is
+  // accepted and ignored (the reference only plots with it).  stats receives
+  // the value, its SD and the first control, which the reference stores on
+  // the road (SimulateGPU.cu:5485-5488).
+  static void simulateROVCUDA(const RovSpeciesInputs& sim, std::vector<SpeciesRoadPatches>& srp,
+                              std::vector<Matrix>& adjPops, Matrix& unitProfits, Matrix& condExp,
+                              MatrixI& optCont, std::vector<double>& regressions,
+                              bool plotResults, RovStats& stats, int device = 0) {
+    (void)plotResults;
+    const int S = (int)srp.size();
+    const int C = (int)sim.flows.size();
+    if (S < 1) fail("simulateROVCUDA: no species");
+    const int64_t P = (int64_t)srp[0].initPops.size();
+    const int nComm = (int)sim.uncertainties.size() - sim.nFuels;
+    if (nComm < 0 || (int)sim.oreMean.size() != nComm || (int)sim.oreSD.size() != nComm)
+      fail("simulateROVCUDA: one ore mean and SD per commodity uncertainty");
+    if (sim.fuelCost.size() != sim.fuelIdx.size())
+      fail("simulateROVCUDA: fuel cost and fuel index per vehicle class");
+    std::vector<roadmc_rovs_species> sp((size_t)S);
+    std::vector<std::vector<double>> surv((size_t)S);
+    for (int s = 0; s < S; ++s) {
+      const SpeciesRoadPatches& r = srp[(size_t)s];
+      if ((int64_t)r.initPops.size() != P || (int64_t)r.capacities.size() != P ||
+          r.transProbs.rows != P || r.transProbs.cols != P || (int)r.survProbs.size() != C)
+        fail("simulateROVCUDA: every species needs P patches and one survival matrix per control");
+      surv[(size_t)s].reserve((size_t)(C * P * P));
+      for (const Matrix& m : r.survProbs) {
+        if (m.rows != P || m.cols != P) fail("simulateROVCUDA: survival matrices must be P x P");
+        surv[(size_t)s].insert(surv[(size_t)s].end(), m.data.begin(), m.data.end());
+      }
+      roadmc_rovs_species& o = sp[(size_t)s];
+      o.init_pop = r.initPops.data();
+      o.capacity = r.capacities.data();
+      o.trans = r.transProbs.data.data();
+      o.surv = surv[(size_t)s].data();
+      for (int i = 0; i < 8; ++i) o.params[i] = r.speciesParams[i];
+    }
+    roadmc_rovs_problem pr{};
+    pr.n_patches = (int32_t)P;
+    pr.n_species = S;
+    pr.species = sp.data();
+    pr.n_controls = C;
+    pr.flow = sim.flows.data();
+    pr.n_fuels = sim.nFuels;
+    pr.n_commodities = nComm;
+    pr.uncertainties = sim.uncertainties.data();
+    pr.ore_mean = sim.oreMean.data();
+    pr.ore_sd = sim.oreSD.data();
+    pr.n_vehicles = (int32_t)sim.fuelCost.size();
+    pr.fuel_cost = sim.fuelCost.data();
+    pr.fuel_idx = sim.fuelIdx.data();
+    pr.unit_cost = sim.unitCost;
+    pr.unit_revenue = sim.unitRevenue;
+    pr.n_years = sim.nYears;
+    pr.time_step = sim.timeStep;
+    pr.rrr = sim.rrr;
+    pr.n_paths = sim.noPaths;
+    pr.dim_res = sim.dimRes;
+    pr.profit_scaling = sim.profitScaling;
+    pr.hysteresis = sim.hysteresis;
+    pr.seed = sim.seed;
+    pr.road_id = sim.roadId;
+    pr.method = sim.method;
+    pr.checkpoint_years = sim.checkpointYears;
+    pr.precision = sim.precision;
+    const int64_t N = sim.noPaths, T = sim.nYears;
+    const int D = S + 1;
+    int64_t grid = (int64_t)D * sim.dimRes, cells = 1;
+    for (int d = 0; d < D; ++d) cells *= sim.dimRes;
+    grid += cells;
+    // [T][N] row-major is the column-major noPaths x nYears matrix itself.
+    Matrix ce(N, T), up(N, T);
+    MatrixI oc(N, T);
+    std::vector<double> adj((size_t)(T * S * N));
+    std::vector<double> reg((size_t)(T * C * grid));
+    RovStats st;
+    roadmc_rovs_result res{};
+    res.value = &st.value;
+    res.value_sd = &st.valueSD;
+    res.first_control = &st.firstControl;
+    res.cond_exp = ce.data.data();
+    res.opt_control = oc.data.data();
+    res.unit_profit = up.data.data();
+    res.adj_pops = adj.data();
+    res.grids = reg.data();
+    roadmc_ctx* ctx = context(device);
+    if (roadmc_rovs_run(ctx, &pr, &res) != ROADMC_OK) fail(roadmc_last_error(ctx));
+    // Year t's [S][N] block is the column-major noPaths x species matrix.
+    std::vector<Matrix> ap((size_t)T);
+    for (int64_t t = 0; t < T; ++t) {
+      ap[(size_t)t] = Matrix(N, S);
+      std::copy(adj.begin() + t * S * N, adj.begin() + (t + 1) * S * N, ap[(size_t)t].data.begin());
+    }
+    adjPops = std::move(ap);
+    unitProfits = std::move(up);
+    condExp = std::move(ce);
+    optCont = std::move(oc);
+    regressions = std::move(reg);
+    stats = st;
+  }
+
+  // SimulateGPU::expPV (SimulateGPU.h:53): stores the expected present value
   // and its SD on the uncertainty.
   static void expPV(Uncertainty& u, int nYears, double timeStep, double rrr, double growth,
                     int64_t noPaths, uint64_t seed, int device = 0) {

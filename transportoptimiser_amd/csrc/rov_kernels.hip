@@ -347,15 +347,16 @@ __device__ __forceinline__ void rov_quad_columns(const double (&vq)[RovBasis<U, 
   rov_quad_columns_impl<U, DEG>(vq, c, f, std::make_integer_sequence<int, RovBasis<U, DEG>::NF>{});
 }
 
-// This lane's values of path row `row` of xs ([.][U1], column 0 = 1) for
-// rov_quad_columns: v[c + 4 r] (zero past U).
+// This lane's values of path row `row` of xs (value m of the row at
+// xs[row rs + m cs], value 0 = 1) for rov_quad_columns: v[c + 4 r] (zero past U).
 template <int U, int NR>
-__device__ __forceinline__ void rov_quad_load(const double* xs, int row, int c, double (&vq)[NR]) {
+__device__ __forceinline__ void rov_quad_load(const double* xs, int row, int c, double (&vq)[NR],
+                                              int rs, int cs) {
   constexpr int U1 = U + 1;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const int m = c + 4 * r;
-    vq[r] = m < U1 ? xs[(size_t)row * U1 + (m < U1 ? m : 0)] : 0.0;
+    vq[r] = m < U1 ? xs[(size_t)row * rs + (size_t)(m < U1 ? m : 0) * cs] : 0.0;
   }
 }
 
@@ -539,12 +540,15 @@ __device__ __forceinline__ double rov_sum_rows(const RMC_GLOBAL double* src, siz
 // step (rov_step_kernel): the reduction of b of the step before, beta =
 // A^{-1} b, the decisions, and the partial sums of the next b.
 
-// LDS of rov_gram_kernel: values [kRovGramPaths][U+1], the waves' sums
-// [4][NQ][16]; then ints: groups and the group's path list, kRovGramPaths
-// each, and 4 wave counts.
+// LDS of rov_gram_kernel: values [U+1][kRovGramPaths + 1] (component-major,
+// the odd row length spreading a quad's four components over distinct
+// banks), the waves' sums [4][NQ][16]; then ints: groups and the group's path
+// list, kRovGramPaths each, and 4 wave counts.
+constexpr int kRovGramXs = kRovGramPaths + 1;
+
 __host__ __device__ inline size_t rov_gram_lds_doubles(int K, int U) {
   const int nf = (K + 3) / 4, nq = nf * (nf + 1) / 2;
-  return (size_t)kRovGramPaths * (U + 1) + (size_t)4 * nq * 16;
+  return (size_t)kRovGramXs * (U + 1) + (size_t)4 * nq * 16;
 }
 
 size_t rov_gram_lds(int K, int U) {
@@ -591,8 +595,8 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
   const RovStepArgs& a = *pa;
   const int G = a.G, GE = G * E, N = a.N;
   extern __shared__ double sm[];
-  double* xs = sm;                       // [PB][U1], column 0 = 1
-  double* tiles = xs + PB * U1;          // [4][NQ][16]
+  double* xs = sm;                       // [U1][kRovGramXs], row 0 = 1
+  double* tiles = xs + kRovGramXs * U1;  // [4][NQ][16]
   int* gs = reinterpret_cast<int*>(sm + rov_gram_lds_doubles(K, U));  // [PB]
   int* lst = gs + PB;                    // [PB]
   int* wcnt = lst + PB;                  // [4]
@@ -620,7 +624,7 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
       double v[U1];
       rov_features<U>(sc, xr[h], v);
 #pragma unroll
-      for (int u = 0; u < U1; ++u) xs[(size_t)q * U1 + u] = p < N ? v[u] : 0.0;
+      for (int u = 0; u < U1; ++u) xs[(size_t)u * kRovGramXs + q] = p < N ? v[u] : 0.0;
       gs[q] = p < N ? rov_group_of<U>(a, xr[h], ctl[h]) : -1;
     }
   }
@@ -675,7 +679,7 @@ __global__ __launch_bounds__(256) void rov_gram_kernel(const RovStepArgs* __rest
       const int idx = r0 + (lane >> 2);
       const bool ok = idx < total;
       double vq[NR], f[NF];
-      rov_quad_load<U, NR>(xs, lst[ok ? idx : 0], c, vq);
+      rov_quad_load<U, NR>(xs, lst[ok ? idx : 0], c, vq, 1, kRovGramXs);
       rov_quad_columns<U, DEG>(vq, c, f);
 #pragma unroll
       for (int F = 0; F < NF; ++F) f[F] = ok ? f[F] : 0.0;
@@ -1090,7 +1094,7 @@ __global__ __launch_bounds__(256) void rov_step_kernel(const RovStepArgs* __rest
   for (int r = 0; r < 8; ++r) {
     const int row = (r >> 2) * 256 + w * 64 + (r & 3) * 16 + (lane >> 2);
     double vq[NR], f[NF];
-    rov_quad_load<U, NR>(xs, row, c, vq);
+    rov_quad_load<U, NR>(xs, row, c, vq, U1, 1);
     rov_quad_columns<U, DEG>(vq, c, f);
     const int gr = gs[row];
     const double yr = ys[row];

@@ -1,5 +1,5 @@
 """ROV problems: exogenous price uncertainties valued by regression Monte
-Carlo (SimulateGPU::simulateROVCUDA, include/gpuCode/SimulateGPU.h:340-344;
+Carlo (SimulateGPU::simulateROVCUDA, include/gpuCode/SimulateGPU.h:187-191;
 expected PV, SimulateGPU::expPV, :206)."""
 from __future__ import annotations
 
