@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "adj_kernels.h"
+#include "gmem.h"
 #include "mte_common.h"
 
 namespace roadmc {
@@ -84,23 +85,22 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   double* grb = gr + Pp;
   const int nrb = BM / TM;
 
-  const T* cols = static_cast<const T*>(jb.cols);
-  const T* capK = cols;
-  const T* capKinv = cols + Pp;
-  const T* rate = cols + 2 * Pp;
+  const RMC_GLOBAL T* cols = gmem(static_cast<const T*>(jb.cols));
+  const RMC_GLOBAL T* capKinv = cols + Pp;
+  const RMC_GLOBAL T* rate = cols + 2 * Pp;
   const T l2e = sizeof(T) == 4 ? (T)1.4426950408889634 : (T)1;
   const T sig = (T)(jb.sigma * (sizeof(T) == 4 ? 1.4426950408889634 : 1.0));
   const T half_var = (T)(0.5 * jb.sigma * jb.sigma) * l2e;
 
-  T* ns = static_cast<T*>(jb.ns);
-  T* np = static_cast<T*>(jb.np);
-  T* mu = static_cast<T*>(jb.mu);
-  T* lam = static_cast<T*>(jb.lam);
+  RMC_GLOBAL T* ns = gmem(static_cast<T*>(jb.ns));
+  RMC_GLOBAL T* np = gmem(static_cast<T*>(jb.np));
+  RMC_GLOBAL T* mu = gmem(static_cast<T*>(jb.mu));
+  RMC_GLOBAL T* lam = gmem(static_cast<T*>(jb.lam));
   const size_t plane = (size_t)N * Pp;  // one year of staging
 
   for (int c = tid; c < Pp; c += nthreads) gr[c] = 0.0;
   // Checkpoint of the segment (state entering year y0).
-  const T* ck = static_cast<const T*>(jb.ckpt) + (size_t)(jb.y0 / jb.L) * plane;
+  const RMC_GLOBAL T* ck = gmem(static_cast<const T*>(jb.ckpt)) + (size_t)(jb.y0 / jb.L) * plane;
   for (int idx = tid; idx < BM * Pp; idx += nthreads) {
     const int r = idx / Pp, c = idx - r * Pp;
     A[r * LDA + TR::perm(c)] = (p0 + r < N) ? ck[(size_t)(p0 + r) * Pp + c] : (T)0;
@@ -165,8 +165,8 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
       // Opaque copies: the epilogue's addresses are formed here, after the
       // product, instead of being hoisted into registers live across it.
       int lanee = lane, p0e = p0;
-      const T* colse = cols;
-      T* npe = np;
+      const RMC_GLOBAL T* colse = cols;
+      RMC_GLOBAL T* npe = np;
       asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe));
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -242,10 +242,10 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     __syncthreads();
     const int i = yr - 1 - jb.y0;  // staging slot of n'_yr and mu_yr
     int lanee = lane, p0e = p0;
-    const T* colse = cols;
-    T* npe = np;
-    T* mue = mu;
-    T* lame = lam;
+    const RMC_GLOBAL T* colse = cols;
+    RMC_GLOBAL T* npe = np;
+    RMC_GLOBAL T* mue = mu;
+    RMC_GLOBAL T* lame = lam;
     asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe), "+s"(mue), "+s"(lame));
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     }
   }
   __syncthreads();
-  for (int c = tid; c < Pp; c += nthreads) jb.grad_part[(size_t)tile * Pp + c] = gr[c];
+  for (int c = tid; c < Pp; c += nthreads) gmem(jb.grad_part)[(size_t)tile * Pp + c] = gr[c];
 }
 
 // grad_r[c] += sum over tiles of the segment's partials: one wave per

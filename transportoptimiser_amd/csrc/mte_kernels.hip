@@ -28,6 +28,7 @@
 // REF_GPU  the growth-rate SDE of SimulateGPU.cu:686-709.
 #include <hip/hip_runtime.h>
 
+#include "gmem.h"
 #include "mte_common.h"
 #include "mte_kernels.h"
 #include "rng.h"
@@ -72,9 +73,9 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
       smem + align16((size_t)BM * LDA * sizeof(T)) +
       align16((size_t)BM * sizeof(T)));
 
-  const T* cols = reinterpret_cast<const T*>(jb.cols);
+  const RMC_GLOBAL T* cols = gmem(reinterpret_cast<const T*>(jb.cols));
   // cols holds K, 1/K, r and the initial population, Pp values each.
-  const T* init = cols + 3 * Pp;
+  const RMC_GLOBAL T* init = cols + 3 * Pp;
 
   const T mu = (T)jb.params[1], sd = (T)jb.params[2], rev = (T)jb.params[4];
   const T jmp = (T)jb.params[5], jprob = (T)jb.params[6], lv = (T)jb.params[7];
@@ -160,11 +161,11 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     // ops instead of being hoisted out of the year loop into registers that
     // would then spill.
     int LDAy = LDA, p0y = p0;
-    const T* colsy = cols;
+    const RMC_GLOBAL T* colsy = cols;
     asm volatile("" : "+s"(LDAy), "+s"(p0y), "+s"(colsy));
     if (!TR::kSplit && jb.ckpt && year % jb.ckpt_every == 0) {
       // Adjoint checkpoint: the state entering this year, natural column order.
-      T* ck = static_cast<T*>(jb.ckpt) + ((size_t)(year / jb.ckpt_every) * jb.N) * Pp;
+      RMC_GLOBAL T* ck = gmem(static_cast<T*>(jb.ckpt)) + ((size_t)(year / jb.ckpt_every) * jb.N) * Pp;
       for (int idx = tid; idx < BM * Pp; idx += nthreads) {
         const int r = idx / Pp, c = idx - r * Pp;
         if (p0 + r < jb.N) ck[(size_t)(p0 + r) * Pp + c] = A[r * LDA + TR::perm(c)];
@@ -260,7 +261,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     // Fresh opaque copies for the epilogue: nothing it derives from them can
     // be computed before (and kept live across) the product loop.
     int LDAe = LDA, p0e = p0, lanee = lane;
-    const T* colse = cols;
+    const RMC_GLOBAL T* colse = cols;
     asm volatile("" : "+s"(LDAe), "+s"(p0e), "+s"(colse), "+v"(lanee));
     const bool keep_pre = (year == jb.T - 1) && (jb.endpoint == kEndPre);
     // FP32 LOGNORMAL: n = x (a - b x) exp2(s' z - h') with a = 1 + r and
@@ -353,7 +354,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     if (path < jb.N) {
       double tot = 0.0;
       for (int c = 0; c < Pp; ++c) tot += state_at(r, c);
-      if (jb.end_pops) jb.end_pops[path] = tot;
+      if (jb.end_pops) gmem(jb.end_pops)[path] = tot;
       cnt += (tot >= jb.thr_total) ? 1.0 : 0.0;
       sum += tot;
       sq += tot * tot;
@@ -368,7 +369,7 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
     red[wave * 3 + 2] = sq;
   }
   __syncthreads();
-  double* part = jb.partial + (size_t)tile * (3 + Pp);
+  RMC_GLOBAL double* part = gmem(jb.partial) + (size_t)tile * (3 + Pp);
   if (tid < 3) {
     double v = 0.0;
     for (int w = 0; w < nwaves; ++w) v += red[w * 3 + tid];
@@ -441,7 +442,7 @@ __global__ void mte_reduce_kernel(const MteJob* __restrict__ jobs,
     // Unrolled so that eight tiles' loads are in flight; the sum keeps its
     // tile order.
 #pragma unroll 8
-    for (int t = 0; t < jb.n_tiles; ++t) s += jb.partial[(size_t)t * W + e];
+    for (int t = 0; t < jb.n_tiles; ++t) s += gmem(jb.partial)[(size_t)t * W + e];
     o[e] = s;
   }
 }
