@@ -393,7 +393,8 @@ def main() -> int:
     ap.add_argument("--paths", type=int, default=20000)
     ap.add_argument("--years", type=int, default=50)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--autotune", action="store_true", help="tune tile rows first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile settings instead of timing them first")
     ap.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32",
                     help="products of the measured generation")
     ap.add_argument("--no-other", action="store_true",
@@ -427,11 +428,25 @@ def main() -> int:
         f"sum P^2 = {sum(p * p for p in sizes)}, {flops / 1e12:.1f} TFLOP per generation")
 
     ctx = Context(0)
-    if args.autotune:
-        log(f"autotuned tile rows: {ctx.autotune(n_paths=args.paths, n_years=5)}")
-    tiles = ctx.tiles()
-
     prec = nat.PREC_BF16X3 if args.precision == "bf16x3" else nat.PREC_FP32
+    tune = None
+    if not args.no_autotune:
+        # Every valid (rows, k-split, workgroups per CU) of each patch-count
+        # bucket, timed on 8 synthetic roads spread over the bucket's patch
+        # counts with the generation's paths and 3 years.
+        t = time.perf_counter()
+        tune_paths, tune_years = args.paths, 3
+        trials = ctx.autotune(n_paths=tune_paths, n_years=tune_years, precision=prec)
+        tune = {"n_paths": tune_paths, "n_years": tune_years, "trials": len(trials),
+                "seconds": round(time.perf_counter() - t, 2),
+                "chosen_ms": {t_["bucket_max_p"]: round(t_["ms"], 4) for t_ in trials
+                              if t_["chosen"]}}
+        log(f"autotune: {len(trials)} trials in {tune['seconds']} s")
+        for t_ in trials:
+            log(f"  P<={t_['bucket_max_p']:4d} rows {t_['rows']:3d} split {t_['split']} "
+                f"per_cu {t_['per_cu']}: {t_['ms']:.4f} ms{'  <- chosen' if t_['chosen'] else ''}")
+    table = ctx.tile_table(prec)
+    tiles = {b: t_["rows"] for b, t_ in table.items()}
     for r in roads:
         r.precision = prec
     t = time.perf_counter()
@@ -594,7 +609,8 @@ def main() -> int:
                                f"{len(roads[0].species)} species, {args.years} years, "
                                f"{args.paths} paths, one persistent graph launch",
                    "n_roads": len(roads), "paths": args.paths, "years": args.years,
-                   "species": len(roads[0].species), "tile_rows": tiles},
+                   "species": len(roads[0].species), "tile_rows": tiles,
+                   "tiles": table, "autotune": tune},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2),
                      "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,

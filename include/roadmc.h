@@ -154,10 +154,35 @@ int roadmc_batch_fetch(roadmc_batch* batch, roadmc_mte_result* results);
 int roadmc_batch_kernel_ms(roadmc_batch* batch, double* ms);
 void roadmc_batch_destroy(roadmc_batch* batch);
 
-/* Tile shapes: rows of paths per workgroup for each patch-count bucket.
- * roadmc_autotune times the candidates on synthetic roads and stores the
- * winners in the context; roadmc_get_tiles reports them. */
+/* Tile settings of the MTE launches, per precision (ROADMC_PREC_FP32 or
+ * ROADMC_PREC_BF16X3; FP64 tiles follow LDS) and patch-count bucket
+ * (P <= 128, 256, 512, 1024): rows of paths per workgroup (a power of two in
+ * [32, 256], rows x P <= 32768); split 1 lets remainder tiles (the tiles left
+ * after an even share per SIMD) be computed as four k-quarters, 0 not;
+ * per_cu 1 or 2 workgroups per CU of the persistent launch (0: two when LDS
+ * allows; a batch takes the smallest of its buckets' values).
+ * Rows and per_cu never change results (each path's products and draws are
+ * the same in any tile); a split changes the k-summation order of the split
+ * tiles' paths, so their end totals move within FP32 rounding.
+ * roadmc_autotune_ex times every valid setting of each bucket on a synthetic
+ * road of the bucket's largest P (n_paths paths, n_years years), stores the
+ * fastest in the context, and reports up to max_trials trials (n_trials: all
+ * of them).  roadmc_autotune does so for FP32 and BF16X3 without a report;
+ * roadmc_get_tiles reports the FP32 rows. */
+typedef struct {
+  int32_t precision;     /* ROADMC_PREC_FP32 or ROADMC_PREC_BF16X3 */
+  int32_t bucket_max_p;  /* 128, 256, 512 or 1024 */
+  int32_t rows, split, per_cu;
+  int32_t chosen;        /* 1: the bucket's fastest, now in the context */
+  double ms;             /* average persistent-kernel time of two launches */
+} roadmc_tile_trial;
+int roadmc_autotune_ex(roadmc_ctx* ctx, int32_t precision, int32_t n_paths, int32_t n_years,
+                       roadmc_tile_trial* trials, int32_t max_trials, int32_t* n_trials);
 int roadmc_autotune(roadmc_ctx* ctx, int32_t n_paths, int32_t n_years);
+int roadmc_set_tile(roadmc_ctx* ctx, int32_t precision, int32_t bucket_max_p, int32_t rows,
+                    int32_t split, int32_t per_cu);
+int roadmc_get_tile(const roadmc_ctx* ctx, int32_t precision, int32_t bucket_max_p,
+                    int32_t* rows, int32_t* split, int32_t* per_cu);
 int roadmc_get_tiles(const roadmc_ctx* ctx, int32_t* bucket_max_p,
                      int32_t* rows, int32_t* n_buckets);
 

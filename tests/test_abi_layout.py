@@ -32,6 +32,7 @@ PAIRS = [
     ("roadmc_species_file", nat.SpeciesFile),
     ("roadmc_list", nat.RoadmcList),
     ("roadmc_run_settings", nat.RunSettings),
+    ("roadmc_tile_trial", nat.TileTrial),
 ]
 
 

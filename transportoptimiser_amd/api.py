@@ -210,9 +210,33 @@ class Context:
     def prepare_batch(self, problems: list[RoadProblem]) -> Batch:
         return Batch(self, problems)
 
-    def autotune(self, n_paths: int = 20000, n_years: int = 10) -> dict:
-        nat.check(nat.lib().roadmc_autotune(self.handle, n_paths, n_years), self.handle)
-        return self.tiles()
+    def autotune(self, n_paths: int = 20000, n_years: int = 10,
+                 precision: int = nat.PREC_FP32) -> list[dict]:
+        """Times every valid tile setting of each patch-count bucket
+        (``roadmc_autotune_ex``), keeps the fastest in the context and
+        returns the trials."""
+        cap = 256
+        trials = (nat.TileTrial * cap)()
+        n = C.c_int32()
+        nat.check(nat.lib().roadmc_autotune_ex(self.handle, precision, n_paths, n_years, trials,
+                                               cap, C.byref(n)), self.handle)
+        return [{"precision": t.precision, "bucket_max_p": t.bucket_max_p, "rows": t.rows,
+                 "split": t.split, "per_cu": t.per_cu, "chosen": bool(t.chosen), "ms": t.ms}
+                for t in trials[:min(n.value, cap)]]
+
+    def set_tile(self, precision: int, bucket_max_p: int, rows: int, split: int = 1,
+                 per_cu: int = 0) -> None:
+        nat.check(nat.lib().roadmc_set_tile(self.handle, precision, bucket_max_p, rows, split,
+                                            per_cu), self.handle)
+
+    def tile(self, precision: int, bucket_max_p: int) -> dict:
+        r, s_, pc = C.c_int32(), C.c_int32(), C.c_int32()
+        nat.check(nat.lib().roadmc_get_tile(self.handle, precision, bucket_max_p, C.byref(r),
+                                            C.byref(s_), C.byref(pc)), self.handle)
+        return {"rows": r.value, "split": s_.value, "per_cu": pc.value}
+
+    def tile_table(self, precision: int = nat.PREC_FP32) -> dict:
+        return {b: self.tile(precision, b) for b in (128, 256, 512, 1024)}
 
     def tiles(self) -> dict:
         mx = (C.c_int32 * 8)()

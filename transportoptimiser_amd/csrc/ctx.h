@@ -25,7 +25,16 @@ struct roadmc_ctx {
   std::string err;
   int fail_allocs = 0;  // test hook: next n allocations fail
   size_t lds_limit_f32 = 0, lds_limit_f64 = 0, lds_limit_b3 = 0;
-  int tile_rows[4] = {256, 128, 64, 32};  // FP32 rows per bucket P<=128..1024
+  // Tile settings of the MTE launches per precision (0: FP32, 1: split bf16)
+  // and patch-count bucket (P <= 128, 256, 512, 1024): rows of paths per
+  // workgroup; whether remainder tiles may be split over k (1) or not (0);
+  // workgroups per CU of the persistent launch (0: as many as LDS allows,
+  // else 1 or 2).  Set by roadmc_set_tile or roadmc_autotune_ex.
+  struct TileSetting {
+    int rows, split, per_cu;
+  };
+  TileSetting tiles[2][4] = {{{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {32, 1, 0}},
+                             {{256, 1, 0}, {128, 1, 0}, {64, 1, 0}, {32, 1, 0}}};
   rmc_job_buffers mte_bufs;
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;

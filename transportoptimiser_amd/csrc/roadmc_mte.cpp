@@ -124,14 +124,17 @@ void pack_species(const roadmc_species& sp, int Pp, int Kp, double tol, bool fp6
   }
 }
 
-int rows_for(const roadmc_ctx* ctx, bool fp64, int Pp, int64_t N) {
-  if (fp64) {
+// Row of ctx->tiles for a precision (FP64 has none: its rows follow LDS).
+int tile_slot(int prec) { return prec == roadmc::kPrecB3 ? 1 : 0; }
+
+int rows_for(const roadmc_ctx* ctx, int prec, int Pp, int64_t N) {
+  if (prec == ROADMC_PREC_FP64) {
     int bm = (roadmc::kMteTileElemsF64 / Pp) / roadmc::kMteRowsF64 * roadmc::kMteRowsF64;
     bm = std::max(roadmc::kMteRowsF64, std::min(bm, 256));
     const int cap = round_up((int)std::min<int64_t>(N, 256), roadmc::kMteRowsF64);
     return std::min(bm, cap);
   }
-  int bm = ctx->tile_rows[bucket_of(Pp)];
+  int bm = ctx->tiles[tile_slot(prec)][bucket_of(Pp)].rows;
   const int lim = (roadmc::kMteTileElemsF32 / Pp) / roadmc::kMteRowsF32 * roadmc::kMteRowsF32;
   bm = std::max(roadmc::kMteRowsF32, std::min(bm, lim));
   const int cap = round_up((int)std::min<int64_t>(N, 256), roadmc::kMteRowsF32);
@@ -168,7 +171,7 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       jb.Pk = js.prec == roadmc::kPrecB3 ? jb.Pp : round_up(jb.P, 16);
       jb.N = (int32_t)pr.n_paths;
       jb.T = pr.n_years;
-      jb.BM = rows_for(ctx, js.fp64, jb.Pp, pr.n_paths);
+      jb.BM = rows_for(ctx, js.prec, jb.Pp, pr.n_paths);
       jb.law = pr.growth_law;
       jb.endpoint = pr.endpoint;
       jb.n_tiles = (int)((pr.n_paths + jb.BM - 1) / jb.BM);
@@ -198,7 +201,8 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
       const int ntiles = (jb.BM / roadmc::kMteRowsF32) * (jb.Pp / 32);
       const int rem = ntiles % 4;
       jb.split = 0;
-      if (!js.fp64 && rem && (jb.Pk >> 3) >= 8 &&
+      if (!js.fp64 && rem && ctx->tiles[tile_slot(js.prec)][bucket_of(jb.Pp)].split &&
+          (jb.Pk >> 3) >= 8 &&
           roadmc::mte_lds_bytes(js.prec, jb.BM, jb.Pp, rem) <= kLdsPerCu)
         jb.split = 1;
       js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.prec, jb.BM, jb.Pp,
@@ -553,8 +557,14 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
   if (!rc) rc = ensure_lds_limit(ctx, bt->js.prec, bt->js.lds);
   if (!rc) {
     // One workgroup per CU fits the LDS of the largest tile; small tiles
-    // leave room for a second one.
-    const int per_cu = bt->js.lds * 2 <= kLdsPerCu ? 2 : 1;
+    // leave room for a second one, unless a bucket of the batch asks for one.
+    int want = 2;
+    if (!bt->js.fp64)
+      for (const MteJob& jb : bt->js.jobs) {
+        const int pc = ctx->tiles[tile_slot(bt->js.prec)][bucket_of(jb.Pp)].per_cu;
+        want = std::min(want, pc == 0 ? 2 : pc);
+      }
+    const int per_cu = (want == 2 && bt->js.lds * 2 <= kLdsPerCu) ? 2 : 1;
     bt->grid = ctx->n_cu * per_cu;
     bt->roads.assign(roads, roads + n_roads);
     rc = capture_batch_graph(bt);
@@ -640,68 +650,182 @@ extern "C" int roadmc_mte_run_batch(roadmc_ctx* ctx, const roadmc_mte_problem* r
 }
 
 // ---------------------------------------------------------------------------
-// Tile autotuner: for each patch-count bucket, time every row count the LDS
-// allows on a synthetic road and keep the fastest per path.
+// Tile autotuner: for each patch-count bucket, every valid setting (rows of
+// paths per workgroup, k-split of remainder tiles, workgroups per CU) is
+// timed on a batch of synthetic roads whose patch counts spread over the
+// bucket (so that ragged counts leave remainder tiles, as a generation's
+// roads do), with the persistent kernel's own events, and the fastest is
+// kept.
 
-extern "C" int roadmc_autotune(roadmc_ctx* ctx, int32_t n_paths, int32_t n_years) {
-  if (!ctx) return ROADMC_ERR_INVALID;
-  if (n_paths < 256 || n_years < 1)
-    return rmc_fail(ctx, ROADMC_ERR_INVALID, "autotune needs n_paths >= 256");
-  const int saved[kBuckets] = {ctx->tile_rows[0], ctx->tile_rows[1], ctx->tile_rows[2],
-                               ctx->tile_rows[3]};
-  int best_rows[kBuckets];
-  for (int b = 0; b < kBuckets; ++b) {
-    const int P = kBucketMax[b];
-    std::vector<double> trans((size_t)P * P), init(P, 100.0), cap(P, 200.0), gr(P, 0.05);
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j)
-        trans[(size_t)j * P + i] = (i == j ? 0.5 : 0.5 / (P - 1));
-    roadmc_species sp{};
+namespace {
+
+bool valid_precision(int prec) { return prec == ROADMC_PREC_FP32 || prec == ROADMC_PREC_BF16X3; }
+
+int bucket_index(int bucket_max_p) {
+  for (int b = 0; b < kBuckets; ++b)
+    if (kBucketMax[b] == bucket_max_p) return b;
+  return -1;
+}
+
+// A road of P patches with seeded random transitions (rows summing to one),
+// random crossing survival and capacities: the values do not change the
+// work, only keep the arithmetic ordinary.
+struct TuneRoad {
+  std::vector<double> trans, surv, init, cap, gr;
+  roadmc_species sp{};
+  explicit TuneRoad(int P) : trans((size_t)P * P), surv((size_t)P * P), init(P), cap(P), gr(P) {
+    uint64_t x = 0x9e3779b97f4a7c15ull ^ (uint64_t)P;
+    auto unif = [&x]() {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int i = 0; i < P; ++i) {
+      double row = 0.0;
+      for (int j = 0; j < P; ++j) row += trans[(size_t)j * P + i] = 0.05 + unif();
+      for (int j = 0; j < P; ++j) trans[(size_t)j * P + i] /= row;
+      cap[i] = 500.0 + 4500.0 * unif();
+      init[i] = 0.8 * cap[i];
+      gr[i] = 0.02 + 0.08 * unif();
+    }
+    for (double& v : surv) v = 1.0 - 0.3 * unif();
     sp.n_patches = P;
     sp.init_pop = init.data();
     sp.capacity = cap.data();
     sp.growth = gr.data();
     sp.trans = trans.data();
+    sp.surv = surv.data();
     sp.noise_sd = 0.1;
     sp.threshold_ratio = 0.5;
-    roadmc_mte_problem pr{};
-    pr.n_species = 1;
-    pr.species = &sp;
-    pr.n_years = n_years;
-    pr.time_step = 1.0;
-    pr.n_paths = n_paths;
-    pr.seed = 12345;
-    double best = 1e300;
-    best_rows[b] = saved[b];
-    for (int rows = 32; rows <= 256; rows *= 2) {
+  }
+};
+
+}  // namespace
+
+extern "C" int roadmc_set_tile(roadmc_ctx* ctx, int32_t precision, int32_t bucket_max_p,
+                               int32_t rows, int32_t split, int32_t per_cu) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  const int b = bucket_index(bucket_max_p);
+  if (!valid_precision(precision) || b < 0)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID,
+                    "roadmc_set_tile: precision FP32 or BF16X3, bucket 128, 256, 512 or 1024");
+  if (rows < roadmc::kMteRowsF32 || rows > 256 || (rows & (rows - 1)) ||
+      rows * kBucketMax[b] > roadmc::kMteTileElemsF32)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID,
+                    "roadmc_set_tile: rows a power of two in [32, 256] with rows x P <= 32768");
+  if ((split != 0 && split != 1) || per_cu < 0 || per_cu > 2)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "roadmc_set_tile: split 0 or 1, per_cu 0, 1 or 2");
+  ctx->tiles[tile_slot(precision)][b] = {rows, split, per_cu};
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_get_tile(const roadmc_ctx* ctx, int32_t precision, int32_t bucket_max_p,
+                               int32_t* rows, int32_t* split, int32_t* per_cu) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  const int b = bucket_index(bucket_max_p);
+  if (!valid_precision(precision) || b < 0) return ROADMC_ERR_INVALID;
+  const roadmc_ctx::TileSetting& t = ctx->tiles[tile_slot(precision)][b];
+  if (rows) *rows = t.rows;
+  if (split) *split = t.split;
+  if (per_cu) *per_cu = t.per_cu;
+  return ROADMC_OK;
+}
+
+extern "C" int roadmc_autotune_ex(roadmc_ctx* ctx, int32_t precision, int32_t n_paths,
+                                  int32_t n_years, roadmc_tile_trial* trials, int32_t max_trials,
+                                  int32_t* n_trials) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  if (!valid_precision(precision))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "autotune: precision FP32 or BF16X3");
+  if (n_paths < 256 || n_years < 1)
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "autotune needs n_paths >= 256 and n_years >= 1");
+  if (max_trials < 0 || (max_trials > 0 && !trials))
+    return rmc_fail(ctx, ROADMC_ERR_INVALID, "autotune: trials is NULL");
+  const int slot = tile_slot(precision);
+  roadmc_ctx::TileSetting saved[kBuckets], best[kBuckets];
+  for (int b = 0; b < kBuckets; ++b) saved[b] = best[b] = ctx->tiles[slot][b];
+  auto restore = [&]() {
+    for (int b = 0; b < kBuckets; ++b) ctx->tiles[slot][b] = saved[b];
+  };
+  int count = 0;
+  constexpr int kRoads = 8;
+  for (int b = 0; b < kBuckets; ++b) {
+    const int P = kBucketMax[b], lo = b ? kBucketMax[b - 1] : 64;
+    // Patch counts lo + (i + 1) (P - lo) / kRoads, rounded up to 8: ragged,
+    // the last one the bucket's largest.
+    std::vector<TuneRoad> roads;
+    roads.reserve(kRoads);
+    for (int i = 0; i < kRoads; ++i)
+      roads.emplace_back(std::min(P, round_up(lo + (i + 1) * (P - lo) / kRoads, 8)));
+    std::vector<roadmc_mte_problem> prs(kRoads);
+    for (int i = 0; i < kRoads; ++i) {
+      roadmc_mte_problem& pr = prs[(size_t)i];
+      pr = roadmc_mte_problem{};
+      pr.n_species = 1;
+      pr.species = &roads[(size_t)i].sp;
+      pr.n_years = n_years;
+      pr.time_step = 1.0;
+      pr.n_paths = n_paths;
+      pr.precision = precision;
+      pr.seed = 12345;
+      pr.road_id = (uint32_t)i;
+    }
+    double best_ms = 1e300;
+    int best_trial = -1;
+    for (int rows = roadmc::kMteRowsF32; rows <= 256; rows *= 2) {
       if (rows * P > roadmc::kMteTileElemsF32) break;
-      ctx->tile_rows[b] = rows;
-      roadmc_batch* bt = nullptr;
-      int rc = roadmc_batch_prepare(ctx, &pr, 1, &bt);
-      if (rc) {
-        for (int i = 0; i < kBuckets; ++i) ctx->tile_rows[i] = saved[i];
-        return rc;
-      }
-      double ms = 0.0;
-      rc = roadmc_batch_launch(bt);  // warm-up
-      if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
-      if (!rc) rc = roadmc_batch_launch(bt);
-      if (!rc) rc = roadmc_batch_launch(bt);
-      if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
-      roadmc_batch_destroy(bt);
-      if (rc) {
-        for (int i = 0; i < kBuckets; ++i) ctx->tile_rows[i] = saved[i];
-        return rc;
-      }
-      if (ms < best) {
-        best = ms;
-        best_rows[b] = rows;
+      // Two workgroups per CU only where the largest road's tile (with a
+      // split's partial products) leaves room for a second.
+      const size_t lds_max = roadmc::mte_lds_bytes(precision, rows, P, 3);
+      for (int split = 0; split <= 1; ++split) {
+        const size_t lds = split ? lds_max : roadmc::mte_lds_bytes(precision, rows, P, 0);
+        for (int per_cu = 1; per_cu <= (lds * 2 <= kLdsPerCu ? 2 : 1); ++per_cu) {
+          ctx->tiles[slot][b] = {rows, split, per_cu};
+          roadmc_batch* bt = nullptr;
+          int rc = roadmc_batch_prepare(ctx, prs.data(), kRoads, &bt);
+          double ms = 0.0;
+          if (!rc) rc = roadmc_batch_launch(bt);  // warm-up
+          if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
+          if (!rc) rc = roadmc_batch_launch(bt);
+          if (!rc) rc = roadmc_batch_launch(bt);
+          if (!rc) rc = roadmc_batch_kernel_ms(bt, &ms);
+          if (bt) roadmc_batch_destroy(bt);
+          if (rc) {
+            restore();
+            return rc;
+          }
+          if (count < max_trials) {
+            roadmc_tile_trial& tr = trials[count];
+            tr.precision = precision;
+            tr.bucket_max_p = P;
+            tr.rows = rows;
+            tr.split = split;
+            tr.per_cu = per_cu;
+            tr.chosen = 0;
+            tr.ms = ms;
+          }
+          if (ms < best_ms) {
+            best_ms = ms;
+            best[b] = {rows, split, per_cu};
+            best_trial = count;
+          }
+          ++count;
+        }
       }
     }
-    ctx->tile_rows[b] = saved[b];
+    if (best_trial >= 0 && best_trial < max_trials) trials[best_trial].chosen = 1;
+    ctx->tiles[slot][b] = saved[b];
   }
-  for (int b = 0; b < kBuckets; ++b) ctx->tile_rows[b] = best_rows[b];
+  for (int b = 0; b < kBuckets; ++b) ctx->tiles[slot][b] = best[b];
+  if (n_trials) *n_trials = count;
   return ROADMC_OK;
+}
+
+extern "C" int roadmc_autotune(roadmc_ctx* ctx, int32_t n_paths, int32_t n_years) {
+  int rc = roadmc_autotune_ex(ctx, ROADMC_PREC_FP32, n_paths, n_years, nullptr, 0, nullptr);
+  if (!rc) rc = roadmc_autotune_ex(ctx, ROADMC_PREC_BF16X3, n_paths, n_years, nullptr, 0, nullptr);
+  return rc;
 }
 
 extern "C" int roadmc_get_tiles(const roadmc_ctx* ctx, int32_t* bucket_max_p, int32_t* rows,
@@ -710,7 +834,7 @@ extern "C" int roadmc_get_tiles(const roadmc_ctx* ctx, int32_t* bucket_max_p, in
   *n_buckets = kBuckets;
   for (int b = 0; b < kBuckets; ++b) {
     if (bucket_max_p) bucket_max_p[b] = kBucketMax[b];
-    if (rows) rows[b] = ctx->tile_rows[b];
+    if (rows) rows[b] = ctx->tiles[0][b].rows;
   }
   return ROADMC_OK;
 }
@@ -1011,7 +1135,7 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   // FP32: 32-row tiles.  The segment kernels hold one tile per workgroup, and
   // finer tiles even out the last round of workgroups: 38.9 ms against
   // 41.7 ms with 128-row tiles at 256 patches, 100 years, 50,000 paths.
-  jb.BM = adj->fp64 ? rows_for(ctx, true, jb.Pp, pr->n_paths) : roadmc::kMteRowsF32;
+  jb.BM = adj->fp64 ? rows_for(ctx, ROADMC_PREC_FP64, jb.Pp, pr->n_paths) : roadmc::kMteRowsF32;
   jb.law = ROADMC_LAW_LOGNORMAL;
   jb.endpoint = ROADMC_END_POST_GROWTH;
   jb.n_tiles = (int)((pr->n_paths + jb.BM - 1) / jb.BM);

@@ -231,6 +231,12 @@ NAME_LEN = 64
 MAX_HABITAT_TYPES = 8
 
 
+class TileTrial(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("bucket_max_p", C.c_int32), ("rows", C.c_int32),
+                ("split", C.c_int32), ("per_cu", C.c_int32), ("chosen", C.c_int32),
+                ("ms", C.c_double)]
+
+
 class CommodityFile(C.Structure):
     _fields_ = [("name", C.c_char * NAME_LEN)] + [
         (k, C.c_double) for k in ("current", "mean", "sd", "reversion", "jump_prob",
@@ -326,6 +332,11 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_batch_kernel_ms": (C.c_int, [C.c_void_p, _dp]),
     "roadmc_batch_destroy": (None, [C.c_void_p]),
     "roadmc_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "roadmc_autotune_ex": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(TileTrial), C.c_int32, _ip]),
+    "roadmc_set_tile": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32]),
+    "roadmc_get_tile": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _ip, _ip]),
     "roadmc_get_tiles": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "roadmc_dense_to_csc": (C.c_int, [_fp, C.c_int32, C.c_int32, C.c_float, _fp, _ip, _ip, _ip]),
     "roadmc_normal_cdf_inverse": (C.c_int, [C.c_double, _dp]),
