@@ -47,8 +47,13 @@ extern "C" {
 #define ROADMC_PREC_BF16X3 2
 
 /* Largest patch count the tiled kernels take (the population tile of one
- * workgroup must fit in LDS). */
+ * workgroup must fit in LDS): the batched entry points, FP64 and BF16X3,
+ * the adjoint and the reference-design baseline. */
 #define ROADMC_MAX_PATCHES 1024
+/* Largest patch count of roadmc_mte_run in FP32: above ROADMC_MAX_PATCHES
+ * the state of every path is kept in HBM and each year is one blocked
+ * product (the matrix must fit the device: P^2 floats). */
+#define ROADMC_MAX_PATCHES_WIDE 16384
 
 typedef struct roadmc_ctx roadmc_ctx;
 
@@ -58,7 +63,8 @@ struct roadmc_road_species; /* declared with roadmc_build_species_road below */
  * (include/road/SpeciesRoadPatches.h:294-306) and the packed parameters of
  * SimulateGPU.cu:3690-3714. */
 typedef struct {
-  int32_t n_patches;        /* P, 1..ROADMC_MAX_PATCHES */
+  int32_t n_patches;        /* P, 1..ROADMC_MAX_PATCHES (roadmc_mte_run in FP32:
+                               ..ROADMC_MAX_PATCHES_WIDE) */
   const double* init_pop;   /* [P] initial population per patch */
   const double* capacity;   /* [P] carrying capacity, > 0 */
   const double* growth;     /* [P] intrinsic growth rate (LOGNORMAL law);

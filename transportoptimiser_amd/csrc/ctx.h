@@ -50,6 +50,11 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } adj_bufs;
+  // MTE beyond the LDS tile: the destination-major movement matrix, the
+  // per-patch columns, the two state buffers and the growth-rate state.
+  struct {
+    rmc_devbuf mt, cols, s0, s1, grm;
+  } wide_bufs;
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h, tail, in;
   } road_bufs;

@@ -18,6 +18,7 @@
 #include "mte_kernels.h"
 #include "rng.h"
 #include "roadmc.h"
+#include "mte_wide.h"
 
 using roadmc::MteJob;
 
@@ -41,7 +42,8 @@ bool finite_array(const double* p, size_t n) {
   return true;
 }
 
-int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr) {
+int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr,
+                     int max_patches = ROADMC_MAX_PATCHES) {
   if (!pr) return rmc_fail(ctx, ROADMC_ERR_INVALID, "problem is NULL");
   if (pr->n_species < 1 || pr->n_species > 63 || !pr->species)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_species must be in [1, 63]");
@@ -63,9 +65,12 @@ int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr) {
   for (int s = 0; s < pr->n_species; ++s) {
     const roadmc_species& sp = pr->species[s];
     const int P = sp.n_patches;
-    if (P < 1 || P > ROADMC_MAX_PATCHES)
+    if (P < 1 || P > max_patches)
       return rmc_fail(ctx, ROADMC_ERR_INVALID,
-                      "n_patches must be in [1, ROADMC_MAX_PATCHES]");
+                      max_patches > ROADMC_MAX_PATCHES
+                          ? "n_patches must be in [1, ROADMC_MAX_PATCHES_WIDE]"
+                          : "n_patches must be in [1, ROADMC_MAX_PATCHES] (roadmc_mte_run in "
+                            "FP32 takes up to ROADMC_MAX_PATCHES_WIDE)");
     if (!sp.init_pop || !sp.capacity || (!sp.trans && !sp.road))
       return rmc_fail(ctx, ROADMC_ERR_INVALID,
                       "init_pop, capacity and trans (or a road) are required");
@@ -465,13 +470,184 @@ bool wants_end_pops(const roadmc_mte_result* results, int n) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// More patches than one workgroup's LDS tile takes: the state of every path
+// in HBM, one blocked product per year (mte_wide.hip), species one after the
+// other; the tallies and the results go through the tiled path's partial
+// layout, reduction and result code.
+
+namespace {
+
+int mte_run_wide(roadmc_ctx* ctx, const roadmc_mte_problem* pr, roadmc_mte_result* result) {
+  if (pr->precision != ROADMC_PREC_FP32)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "more than ROADMC_MAX_PATCHES patches: FP32 products only");
+  constexpr int kB = roadmc::kWideBlock;
+  const int64_t N = pr->n_paths;
+  const int S = pr->n_species;
+  JobSet js;
+  js.prec = ROADMC_PREC_FP32;
+  js.fp64 = false;
+  const bool want_end = result->end_pops != nullptr;
+  int maxPp = 0;
+  for (int s = 0; s < S; ++s) {
+    const roadmc_species& sp = pr->species[s];
+    MteJob jb{};
+    jb.P = sp.n_patches;
+    jb.Pp = round_up(jb.P, kB);
+    jb.Pk = jb.Pp;
+    jb.N = (int32_t)N;
+    jb.T = pr->n_years;
+    jb.BM = kB;
+    jb.law = pr->growth_law;
+    jb.endpoint = pr->endpoint;
+    jb.n_tiles = (int)((N + kB - 1) / kB);
+    for (int i = 0; i < 8; ++i) jb.params[i] = sp.params[i];
+    jb.sigma = sp.noise_sd;
+    jb.dt = pr->time_step;
+    double tot0 = 0.0;
+    for (int j = 0; j < jb.P; ++j) tot0 += sp.init_pop[j];
+    jb.thr_total = sp.threshold_ratio * tot0;
+    jb.tag0 = rmc_tag(pr->road_id, (uint32_t)s, RMC_KIND_PATCH);
+    jb.tag1 = rmc_tag(pr->road_id, (uint32_t)s, RMC_KIND_PATH);
+    jb.key0 = (uint32_t)(pr->seed & 0xffffffffu);
+    jb.key1 = (uint32_t)(pr->seed >> 32);
+    js.part_off.push_back((int64_t)js.part_elems);
+    js.out_off.push_back((int64_t)js.out_elems);
+    js.endpop_off.push_back(want_end ? (int64_t)js.endpop_elems : -1);
+    js.part_elems += (size_t)jb.n_tiles * (3 + jb.Pp);
+    js.out_elems += 3 + (size_t)jb.Pp;
+    if (want_end) js.endpop_elems += (size_t)N;
+    js.jobs.push_back(jb);
+    js.road_of.push_back(0);
+    js.species_of.push_back(s);
+    maxPp = std::max(maxPp, jb.Pp);
+  }
+  JobBuffers& b = ctx->mte_bufs;
+  auto& W = ctx->wide_bufs;
+  const size_t plane = (size_t)N * maxPp;
+  int rc;
+  if ((rc = rmc_buf_ensure(ctx, b.partial, js.part_elems * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, b.out, js.out_elems * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, b.endpop, std::max<size_t>(js.endpop_elems, 1) * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, b.jobs, js.jobs.size() * sizeof(MteJob))) ||
+      (rc = rmc_buf_ensure(ctx, b.off, js.jobs.size() * sizeof(int64_t))) ||
+      (rc = rmc_buf_ensure(ctx, W.mt, (size_t)maxPp * maxPp * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.cols, 4 * (size_t)maxPp * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.s0, plane * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.s1, plane * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.grm, (size_t)N * sizeof(float))))
+    return rc;
+  for (size_t j = 0; j < js.jobs.size(); ++j) {
+    MteJob& jb = js.jobs[j];
+    jb.partial = static_cast<double*>(b.partial.ptr) + js.part_off[j];
+    jb.end_pops = js.endpop_off[j] >= 0 ? static_cast<double*>(b.endpop.ptr) + js.endpop_off[j]
+                                        : nullptr;
+  }
+  hipStream_t st = ctx->stream;
+  RMC_HIP(ctx, hipMemcpyAsync(b.jobs.ptr, js.jobs.data(), js.jobs.size() * sizeof(MteJob),
+                              hipMemcpyHostToDevice, st));
+  RMC_HIP(ctx, hipMemcpyAsync(b.off.ptr, js.out_off.data(), js.out_off.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice, st));
+  const double l2e = 1.4426950408889634;
+  for (int s = 0; s < S; ++s) {
+    const roadmc_species& sp = pr->species[s];
+    const MteJob& jb = js.jobs[(size_t)s];
+    const int P = jb.P, Pp = jb.Pp;
+    float* mt = static_cast<float*>(W.mt.ptr);
+    // The previous species' uploads leave the pinned staging before it is
+    // refilled.
+    RMC_HIP(ctx, hipStreamSynchronize(st));
+    const size_t mt_bytes = sp.trans ? (size_t)Pp * Pp * sizeof(float) : 0;
+    if ((rc = rmc_pinned_ensure(ctx, mt_bytes + 4 * (size_t)Pp * sizeof(float)))) return rc;
+    float* cols_h = reinterpret_cast<float*>(static_cast<char*>(ctx->pinned) + mt_bytes);
+    if (sp.trans) {
+      float* mt_h = static_cast<float*>(ctx->pinned);
+      for (int j = 0; j < Pp; ++j)
+        for (int k = 0; k < Pp; ++k)
+          mt_h[(size_t)j * Pp + k] =
+              (j < P && k < P) ? (float)movement(sp, P, k, j, pr->drop_tol) : 0.f;
+      RMC_HIP(ctx, hipMemcpyAsync(mt, mt_h, mt_bytes, hipMemcpyHostToDevice, st));
+    } else {
+      const roadmc_road_species* rd = sp.road;
+      const int control = sp.road_control < 0 ? rd->n_controls - 1 : sp.road_control;
+      if ((rc = roadmc::road_reserve(ctx, P, roadmc::road_visible_segments(rd), rd->n_controls)))
+        return rc;
+      roadmc::RoadStaging stg;
+      roadmc::RoadArgs ra;
+      if ((rc = roadmc::road_launch(ctx, rd, true, control >= 0, true, stg, &ra))) return rc;
+      const double* surv =
+          control >= 0 ? ra.surv + (size_t)control * P * P : nullptr;
+      RMC_HIP(ctx, roadmc::launch_mte_wide_pack(ra.trans, surv, P, Pp, pr->drop_tol, mt, st));
+      roadmc::RoadTail tail;
+      RMC_HIP(ctx, hipMemcpyAsync(&tail, roadmc::road_tail(ctx), sizeof(tail),
+                                  hipMemcpyDeviceToHost, st));
+      RMC_HIP(ctx, hipStreamSynchronize(st));
+      if (tail.bad) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite transition weight");
+    }
+    for (int j = 0; j < Pp; ++j) {
+      const bool in = j < P;
+      cols_h[j] = in ? (float)sp.capacity[j] : 1.f;
+      cols_h[Pp + j] = in ? (float)(1.0 / sp.capacity[j]) : 1.f;
+      cols_h[2 * Pp + j] = (in && sp.growth) ? (float)sp.growth[j] : 0.f;
+      cols_h[3 * Pp + j] = in ? (float)sp.init_pop[j] : 0.f;
+    }
+    RMC_HIP(ctx, hipMemcpyAsync(W.cols.ptr, cols_h, 4 * (size_t)Pp * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+    roadmc::MteWideArgs a{};
+    a.mt = mt;
+    a.cols = static_cast<const float*>(W.cols.ptr);
+    a.grm = static_cast<float*>(W.grm.ptr);
+    a.P = P;
+    a.Pp = Pp;
+    a.N = (int32_t)N;
+    a.law = pr->growth_law;
+    a.sig = (float)(sp.noise_sd * l2e);
+    a.half_var = (float)(0.5 * sp.noise_sd * sp.noise_sd * l2e);
+    a.dt = (float)pr->time_step;
+    a.mu = (float)sp.params[1];
+    a.sd = (float)sp.params[2];
+    a.rev = (float)sp.params[4];
+    a.jmp = (float)sp.params[5];
+    a.jprob = (float)sp.params[6];
+    a.lv = (float)sp.params[7];
+    a.tag0 = jb.tag0;
+    a.tag1 = jb.tag1;
+    a.key0 = jb.key0;
+    a.key1 = jb.key1;
+    float* cur = static_cast<float*>(W.s0.ptr);
+    float* nxt = static_cast<float*>(W.s1.ptr);
+    RMC_HIP(ctx, roadmc::launch_mte_wide_init(a, (float)sp.params[0], cur, st));
+    for (int y = 0; y < pr->n_years; ++y) {
+      a.year = y;
+      a.keep_pre = (y == pr->n_years - 1 && pr->endpoint == ROADMC_END_PRE_GROWTH) ? 1 : 0;
+      a.s_in = cur;
+      a.s_out = nxt;
+      if (pr->growth_law == ROADMC_LAW_REF_GPU) RMC_HIP(ctx, roadmc::launch_mte_wide_grm(a, st));
+      RMC_HIP(ctx, roadmc::launch_mte_wide_year(a, st));
+      std::swap(cur, nxt);
+    }
+    RMC_HIP(ctx, roadmc::launch_mte_wide_tally(cur, (int)N, Pp, jb.thr_total, jb.partial,
+                                               jb.end_pops, st));
+  }
+  RMC_HIP(ctx, roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
+                                         static_cast<double*>(b.out.ptr),
+                                         static_cast<int64_t*>(b.off.ptr), st));
+  return fetch_results(ctx, js, b, pr, result);
+}
+
+}  // namespace
+
 extern "C" int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem,
                               roadmc_mte_result* result) {
   if (!ctx) return ROADMC_ERR_INVALID;
   if (!result) return rmc_fail(ctx, ROADMC_ERR_INVALID, "result is NULL");
-  int rc = validate_problem(ctx, problem);
+  int rc = validate_problem(ctx, problem, ROADMC_MAX_PATCHES_WIDE);
   if (rc) return rc;
   RMC_HIP(ctx, hipSetDevice(ctx->device));
+  for (int s = 0; s < problem->n_species; ++s)
+    if (problem->species[s].n_patches > ROADMC_MAX_PATCHES)
+      return mte_run_wide(ctx, problem, result);
   JobSet js;
   plan_jobs(ctx, problem, 1, result->end_pops != nullptr, js);
   JobBuffers& b = ctx->mte_bufs;

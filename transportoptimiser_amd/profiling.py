@@ -34,6 +34,10 @@ the reference's neighbour count) and evaluates it at 10^6 points.
 species-state ROV (the reference's ALGO4, or ALGO6 with the forward
 recomputation) on the synthetic landscape R times.
 
+``python transportoptimiser_amd/profiling.py wide [--patches P --paths N --years T]``
+runs one road of one species with P > 1024 patches (the HBM-state path of
+roadmc_mte_run) R times, printing the host time and TFLOP/s of each call.
+
 ``python transportoptimiser_amd/profiling.py dbstats <results.db>`` prints the
 per-kernel call count and duration statistics of a rocprofv3 kernel trace
 (the rocpd SQLite output).
@@ -265,6 +269,28 @@ def traffic(a) -> None:
     print(f"hbm_bytes_per_launch: {hbm:.6e}")
 
 
+def wide(patches: int, paths: int, years: int, repeat: int) -> None:
+    import os
+    import time
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import landscape
+    from transportoptimiser_amd.api import Context
+    ctx = Context(0)
+    sp = landscape.make_species(np.random.default_rng(6), patches)
+    pr = landscape.RoadProblem([sp], n_years=years, n_paths=paths, seed=6)
+    Pp = (patches + 127) // 128 * 128
+    flops = 2.0 * paths * Pp * Pp * years
+    for i in range(repeat):
+        t = time.perf_counter()
+        out = ctx.simulate_mte(pr, end_pops=False, patch_mean=False)
+        ms = (time.perf_counter() - t) * 1e3
+        print(f"wide call {i}: P={patches} N={paths} T={years}: {ms:.2f} ms, "
+              f"{flops / ms / 1e9:.1f} TFLOP/s (padded to {Pp}), MTE {out.mte_prob[0]:.4f}",
+              flush=True)
+    ctx.close()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -299,6 +325,11 @@ def main() -> int:
     rv2.add_argument("--repeat", type=int, default=3)
     rv2.add_argument("--method", type=int, default=4, choices=(4, 6))
     rv2.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
+    wd = sub.add_parser("wide")
+    wd.add_argument("--patches", type=int, default=2048)
+    wd.add_argument("--paths", type=int, default=20000)
+    wd.add_argument("--years", type=int, default=10)
+    wd.add_argument("--repeat", type=int, default=3)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     k = sub.add_parser("counters")
@@ -327,6 +358,8 @@ def main() -> int:
         road(a.patches, a.points, a.repeat)
     elif a.cmd == "uniform":
         uniform(a.patches, a.roads, a.precision)
+    elif a.cmd == "wide":
+        wide(a.patches, a.paths, a.years, a.repeat)
     elif a.cmd == "dbstats":
         dbstats(a.path)
     elif a.cmd == "counters":
