@@ -180,11 +180,13 @@ def test_invalid_inputs_write_nothing(ctx):
     with pytest.raises(nat.RoadmcError) as e:
         ctx.simulate_mte(pr)
     assert e.value.code == nat.ERR_NUMERIC
+    # Above the LDS tile only roadmc_mte_run takes a road (test_gpu_mte_wide.py);
+    # the batched entry points keep ROADMC_MAX_PATCHES.
     big = _problem(8, 4, 2)
     big.species[0] = SpeciesData(trans=np.eye(1025), surv=None, init_pop=np.ones(1025),
                                  capacity=np.ones(1025), growth=None)
     with pytest.raises(nat.RoadmcError) as e:
-        ctx.simulate_mte(big)
+        ctx.simulate_mte_batch([big])
     assert e.value.code == nat.ERR_INVALID
     nan = _problem(16, 10, 3)
     nan.species[0].trans = nan.species[0].trans.copy()

@@ -165,9 +165,10 @@ def make_roads(n: int, seed: int, n_paths: int = 8000, n_years: int = 50,
 
 def rov_problem(unit_cost: float, n_paths: int, n_steps: int, seed: int) -> RovProblem:
     """The config-5 valuation of a road with the given unit cost.  Targets
-    use realized cash flows under the fitted policy: with a few thousand paths
-    the regressed-value recursion compounds cubic-fit extrapolation errors
-    over 50 steps."""
+    use realized cash flows under the fitted policy (with few paths the
+    regressed-value recursion compounds cubic-fit extrapolation errors over
+    50 steps; realized values keep the targets comparable at any path
+    count)."""
     from . import native
     pr = config5(seed=5, n_paths=n_paths, n_steps=n_steps)
     pr.unit_cost = unit_cost
@@ -176,8 +177,9 @@ def rov_problem(unit_cost: float, n_paths: int, n_steps: int, seed: int) -> RovP
     return pr
 
 
-def simulate(ctx, roads, rov_paths: int = 4000, rov_steps: int = 50) -> np.ndarray:
-    """Full Monte Carlo targets (MTE of species 0, ROV / 1000) for roads."""
+def simulate(ctx, roads, rov_paths: int = 100000, rov_steps: int = 50) -> np.ndarray:
+    """Full Monte Carlo targets (MTE of species 0, ROV / 1000) for roads; the
+    valuation at config 5's size (100,000 paths, 50 steps)."""
     outs = ctx.simulate_mte_batch([r.problem for r in roads], end_pops=False, patch_mean=False)
     y = np.zeros((len(roads), 2))
     for i, (r, o) in enumerate(zip(roads, outs)):
