@@ -444,7 +444,8 @@ typedef struct {
  *    discounted, interpolated continuation (<= 0), switching from the running
  *    best only below hysteresis x best;
  *  - time 0 (firstPeriodInduction, :2210-2330).
- * P <= 512, n_species + 1 <= ROADMC_KNN_MAX_DIMS, n_controls <=
+ * P <= 512 (ALGO4 with FP32 products: P <= ROADMC_MAX_PATCHES_WIDE, the forward
+ * populations then kept in HBM), n_species + 1 <= ROADMC_KNN_MAX_DIMS, n_controls <=
  * ROADMC_MAX_CONTROLS, dim_res^(n_species+1) <= 2^24. */
 typedef struct {
   int32_t n_patches;                      /* P */

@@ -98,11 +98,8 @@ def test_largest_landscape(ctx, oracle, prec):
 
 
 def test_invalid_arguments(ctx):
+    # (More than 512 patches: test_wide_refuses_algo6_and_split_bf16.)
     from transportoptimiser_amd import native as nat
-    pr = rs.synthetic_problem(n_patches=600, n_paths=100, n_years=5)
-    with pytest.raises(nat.RoadmcError) as e:
-        rs.simulate_rov_species(ctx, pr)
-    assert e.value.code == nat.ERR_INVALID
     pr = rs.synthetic_problem(n_patches=16, n_paths=100, n_years=2)
     with pytest.raises(nat.RoadmcError):
         rs.simulate_rov_species(ctx, pr)
@@ -230,3 +227,39 @@ def test_algo6_realized_recursion_at_the_tile_limits(ctx, P, S, prec):
     assert len(np.unique(oc[1:T - 1])) > 1
     for t in range(1, T - 1):
         np.testing.assert_allclose(V[t], Q[oc[t]] * up[t] + d * V[t + 1], rtol=1e-10, atol=1e-12)
+
+
+# ---- beyond one workgroup's tile: the HBM-state forward pass (ALGO4, FP32) ----
+
+@pytest.mark.parametrize("P,S,C,N,T,R", [
+    (600, 1, 3, 500, 6, 6),
+    (1100, 2, 2, 300, 5, 5),
+])
+def test_wide_forward_matches_oracle(ctx, oracle, P, S, C, N, T, R):
+    pr = rs.synthetic_problem(seed=P + S, n_patches=P, n_species=S, n_controls=C, n_paths=N,
+                              n_years=T, dim_res=R)
+    g = _run(ctx, pr)
+    ctl, tp, aa, up = oracle.rovs_forward(pr)
+    np.testing.assert_array_equal(g["controls"], ctl)
+    np.testing.assert_allclose(g["unit_profit"], up, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(g["total_pops"], tp, rtol=1e-4)
+    np.testing.assert_allclose(g["aars"], aa, rtol=1e-4)
+    V, oc, out = oracle.rovs_backward(pr, g["controls"], g["total_pops"], g["aars"],
+                                      g["unit_profit"])
+    scale = np.abs(V).max() + 1e-300
+    np.testing.assert_allclose(g["cond_exp"], V, rtol=1e-8, atol=1e-10 * scale)
+    assert (g["opt_control"] == oc).mean() > 0.9999
+    assert g["value"] == pytest.approx(out["value"], rel=1e-8)
+    assert g["first_control"] == out["control"]
+
+
+def test_wide_refuses_algo6_and_split_bf16(ctx):
+    pr = rs.synthetic_problem(seed=4, n_patches=600, n_paths=100, n_years=4, dim_res=4)
+    pr.method = 6
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED
+    pr.method, pr.precision = 6, nat.PREC_BF16X3
+    with pytest.raises(nat.RoadmcError) as e:
+        rs.simulate_rov_species(ctx, pr)
+    assert e.value.code == nat.ERR_UNSUPPORTED

@@ -50,10 +50,11 @@ struct roadmc_ctx {
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h;
   } adj_bufs;
-  // MTE beyond the LDS tile: the destination-major movement matrix, the
-  // per-patch columns, the two state buffers and the growth-rate state.
+  // MTE and the species-state forward pass beyond the LDS tile: the
+  // destination-major movement matrices, the per-patch columns, the two
+  // state buffers, the growth-rate state and the per-block row sums.
   struct {
-    rmc_devbuf mt, cols, s0, s1, grm;
+    rmc_devbuf mt, cols, s0, s1, grm, rs, tot;
   } wide_bufs;
   struct {
     rmc_devbuf a, b, c, d, e, f, g, h, tail, in;

@@ -47,6 +47,32 @@ hipError_t launch_mte_wide_tally(const float* s, int N, int Pp, double thr, doub
 hipError_t launch_mte_wide_pack(const double* trans, const double* surv, int P, int Pp,
                                 double tol, float* mt, hipStream_t s);
 
+// One year of one species of the species-state valuation's forward pass
+// beyond its LDS tile: for every control c the product S_in M_c (the block
+// grid's z index), the per-block row sums of each product (for the surviving
+// fractions), and for each path the growth of its drawn control's product
+// into S_out (SimulateGPU.cu:839-916).
+struct RovsWideArgs {
+  const float* mt;        // [C][Pp][Pp] destination-major trans o surv_c
+  const float* cols;      // [4][Pp]: capacity, 1/capacity, (unused), initial population
+  const float* s_in;      // [N][Pp]
+  float* s_out;           // [N][Pp]
+  const float* grm;       // [N] growth-rate means of the year (after its SDE step)
+  const int* controls;    // [N] drawn control of each path this year
+  float* rs_part;         // [Pp / kWideBlock][C][N] row sums of each block's products
+  float* tot_part;        // [Pp / kWideBlock][N] row sums of the grown block
+  int32_t P, Pp, N, C, year;
+  float lv;               // local variability of the growth rate
+  uint32_t tag0, key0, key1;
+};
+
+hipError_t launch_rovs_wide_year(const RovsWideArgs& a, hipStream_t s);
+// Surviving fractions aars[c][n] = (sum over blocks of rs_part) / tot_prev[n]
+// and the new totals tot_next[n] (sums over blocks of tot_part), in block order.
+hipError_t launch_rovs_wide_tally(const float* rs_part, const float* tot_part, int nblk, int C,
+                                  int N, const float* tot_prev, float* tot_next, float* aars,
+                                  hipStream_t s);
+
 }  // namespace roadmc
 
 #endif

@@ -42,16 +42,23 @@ __device__ __forceinline__ void wide_stage(float* row, int g, const floatx4& lo,
 
 }  // namespace
 
-template <int LAW>
-__global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
-  __shared__ __attribute__((aligned(16))) float As[2][kWB][kWLd];
-  __shared__ __attribute__((aligned(16))) float Bs[2][kWB][kWLd];
+// LDS of one block's product: two contraction slabs of each operand.
+struct WideSlabs {
+  float a[2][kWB][kWLd];
+  float b[2][kWB][kWLd];
+};
+
+// The 128 x 128 block (paths row0.., patches col0..) of S M with the state
+// S path-major [N][Pp] and M destination-major (mt[j Pp + k] = M[k][j]), on
+// v_mfma_f32_32x32x2_f32: wave w = 2 wr + wc forms rows wr 64 .. +64 and
+// columns wc 64 .. +64 as acc[i][j], the 32 x 32 tile (i, j) of its quarter.
+// Item r of acc[i][j] in lane l = 32 h + l32 is row 8 (r / 4) + 4 h + r % 4
+// of the tile, column l32.
+__device__ __forceinline__ void wide_product(const RMC_GLOBAL float* S, const RMC_GLOBAL float* Mt,
+                                             int Pp, int N, int row0, int col0, WideSlabs& sl,
+                                             floatx16 (&acc)[2][2]) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1, h = lane >> 5, l32 = lane & 31;
-  const int row0 = blockIdx.x * kWB, col0 = blockIdx.y * kWB;
-  const int Pp = a.Pp, N = a.N;
-  const RMC_GLOBAL float* S = gmem(a.s_in);
-  const RMC_GLOBAL float* Mt = gmem(a.mt);
   // Staging: thread t moves row t >> 1, k-group t & 1 (8 floats) of both slabs.
   const int sr = tid >> 1, sg = tid & 1;
   const bool a_ok = row0 + sr < N;
@@ -66,10 +73,9 @@ __global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
     rb1 = *reinterpret_cast<const RMC_GLOBAL floatx4*>(b_src + k0 + 4);
   };
   auto stage = [&](int buf) {
-    wide_stage(&As[buf][sr][0], sg, ra0, ra1);
-    wide_stage(&Bs[buf][sr][0], sg, rb0, rb1);
+    wide_stage(&sl.a[buf][sr][0], sg, ra0, ra1);
+    wide_stage(&sl.b[buf][sr][0], sg, rb0, rb1);
   };
-  floatx16 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -78,16 +84,16 @@ __global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
   fetch(0);
   stage(0);
   __syncthreads();
-  for (int sl = 0; sl < nslab; ++sl) {
-    const int buf = sl & 1;
-    if (sl + 1 < nslab) fetch((sl + 1) * kWK);
+  for (int k = 0; k < nslab; ++k) {
+    const int buf = k & 1;
+    if (k + 1 < nslab) fetch((k + 1) * kWK);
 #pragma unroll
     for (int g = 0; g < kWK / 8; ++g) {
       floatx4 av[2], bv[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        av[i] = *reinterpret_cast<const floatx4*>(&As[buf][wr * 64 + i * 32 + l32][8 * g + 4 * h]);
-        bv[i] = *reinterpret_cast<const floatx4*>(&Bs[buf][wc * 64 + i * 32 + l32][8 * g + 4 * h]);
+        av[i] = *reinterpret_cast<const floatx4*>(&sl.a[buf][wr * 64 + i * 32 + l32][8 * g + 4 * h]);
+        bv[i] = *reinterpret_cast<const floatx4*>(&sl.b[buf][wc * 64 + i * 32 + l32][8 * g + 4 * h]);
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -98,11 +104,22 @@ __global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][s], bv[j][s], acc[i][j], 0, 0,
                                                               0);
     }
-    if (sl + 1 < nslab) {
+    if (k + 1 < nslab) {
       stage(buf ^ 1);
       __syncthreads();
     }
   }
+}
+
+template <int LAW>
+__global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
+  __shared__ __attribute__((aligned(16))) WideSlabs sl;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, h = lane >> 5, l32 = lane & 31;
+  const int row0 = blockIdx.x * kWB, col0 = blockIdx.y * kWB;
+  const int Pp = a.Pp, N = a.N;
+  floatx16 acc[2][2];
+  wide_product(gmem(a.s_in), gmem(a.mt), Pp, N, row0, col0, sl, acc);
 
   // Growth and write-back.  Item r of a lane's accumulator is row
   // 8 (r / 4) + 4 h + r % 4 of its 32 x 32 block, column l32; items
@@ -146,6 +163,102 @@ __global__ __launch_bounds__(256) void mte_wide_year_kernel(MteWideArgs a) {
       }
     }
   }
+}
+
+// Sum over the 32 lanes of a half wave (one accumulator column each), in a
+// fixed butterfly order.
+__device__ __forceinline__ float wide_half_sum(float v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void rovs_wide_year_kernel(RovsWideArgs a) {
+  __shared__ __attribute__((aligned(16))) WideSlabs sl;
+  __shared__ float rsum[2][kWB], gsum[2][kWB];
+  __shared__ int ctl[kWB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, h = lane >> 5, l32 = lane & 31;
+  const int row0 = blockIdx.x * kWB, col0 = blockIdx.y * kWB, c = blockIdx.z;
+  const int Pp = a.Pp, N = a.N;
+  if (tid < kWB) ctl[tid] = row0 + tid < N ? gmem(a.controls)[row0 + tid] : -1;
+  floatx16 acc[2][2];
+  wide_product(gmem(a.s_in), gmem(a.mt) + (size_t)c * Pp * Pp, Pp, N, row0, col0, sl, acc);
+  // Growth of the paths that drew this control: g = grMean + z lv per patch,
+  // n = n' (1 + g (K - n') / K / 100); every block's row sums of the product
+  // and of the grown values.
+  const RMC_GLOBAL float* cols = gmem(a.cols);
+  const RMC_GLOBAL float* grm = gmem(a.grm);
+  RMC_GLOBAL float* out = gmem(a.s_out);
+  float rs[2][16], gs[2][16];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) rs[i][r] = gs[i][r] = 0.f;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = col0 + wc * 64 + j * 32 + l32;
+    const float K = cols[col], Ki = cols[Pp + col];
+    const bool live = col < a.P;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rl = wr * 64 + i * 32 + 8 * q + 4 * h;  // block row of item 4q
+        float z[4];
+        rmc_normal4_f((uint32_t)(row0 + rl) >> 2, (uint32_t)a.year, (uint32_t)col, a.tag0, a.key0,
+                      a.key1, z);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int n = row0 + rl + s;
+          const float x = acc[i][j][4 * q + s];
+          rs[i][4 * q + s] += x;
+          if (ctl[rl + s] == c) {
+            const float g = grm[n] + z[s] * a.lv;
+            const float v = live ? x * (1.f + g * (K - x) * Ki * 0.01f) : 0.f;
+            gs[i][4 * q + s] += v;
+            out[(size_t)n * Pp + col] = v;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float vr = wide_half_sum(rs[i][r]), vg = wide_half_sum(gs[i][r]);
+      const int rl = wr * 64 + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+      if (l32 == 0) {
+        rsum[wc][rl] = vr;
+        gsum[wc][rl] = vg;
+      }
+    }
+  __syncthreads();
+  if (tid < kWB && row0 + tid < N) {
+    const int n = row0 + tid;
+    gmem(a.rs_part)[((size_t)blockIdx.y * a.C + c) * N + n] = rsum[0][tid] + rsum[1][tid];
+    if (ctl[tid] == c) gmem(a.tot_part)[(size_t)blockIdx.y * N + n] = gsum[0][tid] + gsum[1][tid];
+  }
+}
+
+__global__ __launch_bounds__(256) void rovs_wide_tally_kernel(const float* __restrict__ rs_part,
+                                                              const float* __restrict__ tot_part,
+                                                              int nblk, int C, int N,
+                                                              const float* __restrict__ tot_prev,
+                                                              float* __restrict__ tot_next,
+                                                              float* __restrict__ aars) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const float tot = gmem(tot_prev)[n];
+  for (int c = 0; c < C; ++c) {
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += gmem(rs_part)[((size_t)b * C + c) * N + n];
+    gmem(aars)[(size_t)c * N + n] = s / tot;
+  }
+  float t = 0.f;
+  for (int b = 0; b < nblk; ++b) t += gmem(tot_part)[(size_t)b * N + n];
+  gmem(tot_next)[n] = t;
 }
 
 __global__ __launch_bounds__(256) void mte_wide_init_kernel(MteWideArgs a, float g0, float* s0) {
@@ -260,6 +373,21 @@ hipError_t launch_mte_wide_tally(const float* s, int N, int Pp, double thr, doub
                                  double* end_pops, hipStream_t st) {
   hipLaunchKernelGGL(mte_wide_tally_kernel, dim3((N + kWB - 1) / kWB), dim3(256), 0, st, s, N, Pp,
                      thr, partial, end_pops);
+  return hipGetLastError();
+}
+
+hipError_t launch_rovs_wide_year(const RovsWideArgs& a, hipStream_t s) {
+  if (a.Pp % kWB != 0 || a.C < 1) return hipErrorInvalidValue;
+  const dim3 grid((a.N + kWB - 1) / kWB, a.Pp / kWB, a.C);
+  hipLaunchKernelGGL(rovs_wide_year_kernel, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rovs_wide_tally(const float* rs_part, const float* tot_part, int nblk, int C,
+                                  int N, const float* tot_prev, float* tot_next, float* aars,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(rovs_wide_tally_kernel, dim3((N + 255) / 256), dim3(256), 0, s, rs_part,
+                     tot_part, nblk, C, N, tot_prev, tot_next, aars);
   return hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
 
 #include "ctx.h"
 #include "knn_kernels.h"
+#include "mte_common.h"
+#include "mte_wide.h"
 #include "mte_kernels.h"
 #include "rng.h"
 #include "roadmc.h"
@@ -22,7 +24,14 @@ int validate(roadmc_ctx* ctx, const roadmc_rovs_problem* p, const roadmc_rovs_re
   auto bad = [&](const char* m) { return rmc_fail(ctx, ROADMC_ERR_INVALID, m); };
   if (!p || !r) return bad("problem or result is NULL");
   if (!r->value) return bad("result.value is NULL");
-  if (p->n_patches < 1 || p->n_patches > roadmc::kRovsMaxPatches) return bad("n_patches must be in [1, 512]");
+  // Beyond one workgroup's tile (512 patches) the ALGO4 forward pass keeps
+  // the populations in HBM (rovs_forward_wide), in FP32.
+  const bool algo6 = p->method == ROADMC_ROVS_ALGO6;
+  if (p->n_patches < 1 || p->n_patches > ROADMC_MAX_PATCHES_WIDE)
+    return bad("n_patches must be in [1, ROADMC_MAX_PATCHES_WIDE]");
+  if (p->n_patches > roadmc::kRovsMaxPatches && (algo6 || p->precision != ROADMC_PREC_FP32))
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED,
+                    "more than 512 patches: ALGO4 with FP32 forward products only");
   if (p->n_species < 1 || p->n_species + 1 > ROADMC_KNN_MAX_DIMS || !p->species)
     return bad("n_species must be in [1, 7]");
   if (p->n_controls < 1 || p->n_controls > ROADMC_MAX_CONTROLS || !p->flow)
@@ -74,6 +83,113 @@ int validate(roadmc_ctx* ctx, const roadmc_rovs_problem* p, const roadmc_rovs_re
   return ROADMC_OK;
 }
 
+// One species' forward pass beyond the LDS tile: the populations of every
+// path in HBM ([N][Pp], Pp = P rounded up to 128), and per year the growth-
+// rate step, every control's product with the drawn control's growth
+// (rovs_wide_year_kernel) and the surviving fractions and totals
+// (rovs_wide_tally_kernel), into the tiled pass's total_pops [T+1][N] and
+// aars [T][C][N].
+int rovs_forward_wide(roadmc_ctx* ctx, const roadmc_rovs_problem* pr, int sp, const int* controls,
+                      float* total_pops, float* aars, uint32_t k0, uint32_t k1) {
+  const roadmc_rovs_species& sc = pr->species[sp];
+  const int P = pr->n_patches, C = pr->n_controls, T = pr->n_years, N = (int)pr->n_paths;
+  const int Pp = round_up(P, roadmc::kWideBlock), nb = Pp / roadmc::kWideBlock;
+  auto& W = ctx->wide_bufs;
+  const size_t plane = (size_t)N * Pp, mt_elems = (size_t)C * Pp * Pp;
+  int rc;
+  if ((rc = rmc_buf_ensure(ctx, W.mt, mt_elems * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.cols, 4 * (size_t)Pp * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.s0, plane * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.s1, plane * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.grm, (size_t)N * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.rs, (size_t)nb * C * N * sizeof(float))) ||
+      (rc = rmc_buf_ensure(ctx, W.tot, (size_t)nb * N * sizeof(float))))
+    return rc;
+  hipStream_t s = ctx->stream;
+  // The previous user of the pinned staging is done before it is refilled.
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  const size_t mt_bytes = mt_elems * sizeof(float), cols_bytes = 4 * (size_t)Pp * sizeof(float);
+  if ((rc = rmc_pinned_ensure(ctx, mt_bytes + cols_bytes + (size_t)N * sizeof(float)))) return rc;
+  float* mt_h = static_cast<float*>(ctx->pinned);
+  float* cols_h = reinterpret_cast<float*>(static_cast<char*>(ctx->pinned) + mt_bytes);
+  float* tot0_h = reinterpret_cast<float*>(static_cast<char*>(ctx->pinned) + mt_bytes + cols_bytes);
+  // mt[c][j][k] = trans o surv_c (origin k, destination j), as the tiled
+  // pass packs it (the FP64 product rounded to binary32).
+  for (int c = 0; c < C; ++c) {
+    const double* sv = sc.surv + (size_t)c * P * P;
+    float* m = mt_h + (size_t)c * Pp * Pp;
+    for (int j = 0; j < Pp; ++j)
+      for (int k = 0; k < Pp; ++k)
+        m[(size_t)j * Pp + k] =
+            (j < P && k < P) ? (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]) : 0.f;
+  }
+  double tot = 0.0;
+  for (int j = 0; j < Pp; ++j) {
+    const bool in = j < P;
+    cols_h[j] = in ? (float)sc.capacity[j] : 1.f;
+    cols_h[Pp + j] = in ? (float)(1.0 / sc.capacity[j]) : 1.f;
+    cols_h[2 * Pp + j] = 0.f;
+    cols_h[3 * Pp + j] = in ? (float)sc.init_pop[j] : 0.f;
+    if (in) tot += (double)(float)sc.init_pop[j];
+  }
+  std::fill(tot0_h, tot0_h + N, (float)tot);
+  RMC_HIP(ctx, hipMemcpyAsync(W.mt.ptr, mt_h, mt_bytes, hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(W.cols.ptr, cols_h, cols_bytes, hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(total_pops, tot0_h, (size_t)N * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  const double* prm = sc.params;
+  roadmc::MteWideArgs g{};
+  g.cols = static_cast<const float*>(W.cols.ptr);
+  g.grm = static_cast<float*>(W.grm.ptr);
+  g.P = P;
+  g.Pp = Pp;
+  g.N = N;
+  g.law = roadmc::kLawRefGpu;  // the init pass then sets the growth-rate state
+  g.dt = (float)pr->time_step;
+  g.mu = (float)prm[1];
+  g.sd = (float)prm[2];
+  g.rev = (float)prm[4];
+  g.jmp = (float)prm[5];
+  g.jprob = (float)prm[6];
+  g.tag1 = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATH);
+  g.key0 = k0;
+  g.key1 = k1;
+  float* cur = static_cast<float*>(W.s0.ptr);
+  float* nxt = static_cast<float*>(W.s1.ptr);
+  RMC_HIP(ctx, roadmc::launch_mte_wide_init(g, (float)prm[0], cur, s));
+  roadmc::RovsWideArgs a{};
+  a.mt = static_cast<const float*>(W.mt.ptr);
+  a.cols = static_cast<const float*>(W.cols.ptr);
+  a.grm = static_cast<const float*>(W.grm.ptr);
+  a.rs_part = static_cast<float*>(W.rs.ptr);
+  a.tot_part = static_cast<float*>(W.tot.ptr);
+  a.P = P;
+  a.Pp = Pp;
+  a.N = N;
+  a.C = C;
+  a.lv = (float)prm[7];
+  a.tag0 = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATCH);
+  a.key0 = k0;
+  a.key1 = k1;
+  for (int t = 0; t < T; ++t) {
+    g.year = t;
+    RMC_HIP(ctx, roadmc::launch_mte_wide_grm(g, s));
+    a.year = t;
+    a.s_in = cur;
+    a.s_out = nxt;
+    a.controls = controls + (size_t)t * N;
+    RMC_HIP(ctx, roadmc::launch_rovs_wide_year(a, s));
+    RMC_HIP(ctx, roadmc::launch_rovs_wide_tally(a.rs_part, a.tot_part, nb, C, N,
+                                                total_pops + (size_t)t * N,
+                                                total_pops + (size_t)(t + 1) * N,
+                                                aars + (size_t)t * C * N, s));
+    std::swap(cur, nxt);
+  }
+  // The pinned staging stays in use until the uploads above are done.
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  return ROADMC_OK;
+}
+
 }  // namespace
 
 extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
@@ -86,6 +202,7 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   const int N = (int)pr->n_paths, U = pr->n_fuels + pr->n_commodities, D = S + 1, R = pr->dim_res;
   // Split-bf16 products contract over Pp (an even count of 16-k chunks).
   const bool split = pr->precision == ROADMC_PREC_BF16X3;
+  const bool wide = P > roadmc::kRovsMaxPatches;
   const int Pp = round_up(P, 32), Pk = split ? Pp : round_up(P, 16);
   size_t G = 1;
   for (int d = 0; d < D; ++d) G *= (size_t)R;
@@ -114,8 +231,8 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     rmc_devbuf* b;
     size_t bytes;
   } need[] = {
-      {&B.mpack, (size_t)S * C * Pp * Pk * sizeof(float)},
-      {&B.cols, (size_t)S * 3 * Pp * sizeof(float)},
+      {&B.mpack, wide ? 8 : (size_t)S * C * Pp * Pk * sizeof(float)},
+      {&B.cols, wide ? 8 : (size_t)S * 3 * Pp * sizeof(float)},
       {&B.prices, (size_t)(T + 1) * Uc * N * sizeof(double)},
       {&B.controls, (size_t)T * N * sizeof(int)},
       {&B.tp, (size_t)S * (T + 1) * N * sizeof(float)},
@@ -141,11 +258,12 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     if ((rc = rmc_buf_ensure(ctx, *n.b, n.bytes))) return rc;
   hipStream_t s = ctx->stream;
 
-  // Packed movement matrices trans o surv_c (FP32, MFMA B order) and columns.
-  std::vector<float> mp((size_t)S * C * Pp * Pk, 0.f), cols((size_t)S * 3 * Pp);
+  // Packed movement matrices trans o surv_c (FP32, MFMA B order) and columns
+  // (the wide forward pass packs its own, species by species).
+  std::vector<float> mp(wide ? 0 : (size_t)S * C * Pp * Pk, 0.f), cols(wide ? 0 : (size_t)S * 3 * Pp);
   std::vector<float> rvec(algo6 ? (size_t)S * C * Pp : 0, 0.f);
   std::vector<float> init_total(S);
-  for (int sp = 0; sp < S; ++sp) {
+  for (int sp = 0; sp < (wide ? 0 : S); ++sp) {
     const roadmc_rovs_species& sc = pr->species[sp];
     for (int c = 0; c < C; ++c) {
       float* m = mp.data() + ((size_t)sp * C + c) * Pp * Pk;
@@ -172,10 +290,14 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     }
     init_total[sp] = (float)tot;
   }
-  RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice, s));
-  if (split)
-    RMC_HIP(ctx, roadmc::launch_mte_split_b3(static_cast<float*>(B.mpack.ptr), mp.size(), s));
-  RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  if (!wide) {
+    RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float),
+                                hipMemcpyHostToDevice, s));
+    if (split)
+      RMC_HIP(ctx, roadmc::launch_mte_split_b3(static_cast<float*>(B.mpack.ptr), mp.size(), s));
+    RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float),
+                                hipMemcpyHostToDevice, s));
+  }
   if (algo6)
     RMC_HIP(ctx, hipMemcpyAsync(B.rvec.ptr, rvec.data(), rvec.size() * sizeof(float),
                                 hipMemcpyHostToDevice, s));
@@ -207,7 +329,12 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   RMC_HIP(ctx, roadmc::launch_rov_forward(fa, s));
 
   // Species populations under the drawn controls.
-  for (int sp = 0; sp < S; ++sp) {
+  for (int sp = 0; sp < S && wide; ++sp)
+    if ((rc = rovs_forward_wide(ctx, pr, sp, static_cast<const int*>(B.controls.ptr),
+                                static_cast<float*>(B.tp.ptr) + (size_t)sp * (T + 1) * N,
+                                static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N, k0, k1)))
+      return rc;
+  for (int sp = 0; sp < (wide ? 0 : S); ++sp) {
     roadmc::RovsForwardArgs a{};
     a.N = N;
     a.T = T;
