@@ -98,7 +98,9 @@ def run_extras(ctx, roads, prep_ms) -> dict:
     the ROV valuation (3 uncertainties, 50 steps, 100k paths, cubic basis),
     the adjoint (256 patches, 100 years, 50k paths, checkpoints every 10),
     the road -> matrix builder (4096 patches), the generation prepared from
-    road geometry, the species-state ROV (ALGO4) and the grid-surrogate fit."""
+    road geometry, the species-state ROV (ALGO4), the grid-surrogate fit, and
+    the paths beyond the LDS tiles (an MTE road and the species-state ROV at
+    2048 patches)."""
     from transportoptimiser_amd import landscape
     from transportoptimiser_amd.adjoint import mte_gradient
     from transportoptimiser_amd.rov import config5, simulate_rov
@@ -283,6 +285,33 @@ def run_extras(ctx, roads, prep_ms) -> dict:
     out["knn_grid_fit"] = {"ms": round(ms, 2),
                            "what": "kNN local-linear 20^3 grid from 4,096 samples, k = 97; host call"}
     log(f"knn grid fit: {out['knn_grid_fit']}")
+    # Beyond the LDS tiles: one MTE road of 2048 patches (state in HBM, one
+    # blocked product per year) and the species-state ROV's forward pass at
+    # 2048 patches; kernel time from the profiler is in profiles/.
+    spw = landscape.make_species(np.random.default_rng(6), 2048)
+    prw = landscape.RoadProblem([spw], n_years=10, n_paths=20000, seed=6)
+    ctx.simulate_mte(prw, end_pops=False, patch_mean=False)
+    t = time.perf_counter()
+    ow = ctx.simulate_mte(prw, end_pops=False, patch_mean=False)
+    ms = (time.perf_counter() - t) * 1e3
+    fw = 2.0 * 20000 * 2048 ** 2 * 10
+    out["mte_wide_p2048"] = {
+        "ms": round(ms, 2), "tflops_whole_call": round(fw / ms / 1e9, 1),
+        "mte": round(float(ow.mte_prob[0]), 4),
+        "what": "1 species x 2048 patches, 10 years, 20,000 paths, FP32 (above the 1024-patch "
+                "tile: state in HBM); host call incl. host validation and packing of the matrix"}
+    log(f"mte wide: {out['mte_wide_p2048']}")
+    prs = rsp.synthetic_problem(seed=5, n_patches=2048, n_controls=3, n_paths=20000, n_years=20,
+                                dim_res=20)
+    rsp.simulate_rov_species(ctx, prs)
+    t = time.perf_counter()
+    rw = rsp.simulate_rov_species(ctx, prs)
+    ms = (time.perf_counter() - t) * 1e3
+    out["rov_species_algo4_p2048"] = {
+        "ms": round(ms, 2), "value": round(rw["value"], 4), "first_control": rw["first_control"],
+        "what": "species-state ROV (ALGO4, FP32) with 2048 patches (above the 512-patch tile), "
+                "3 controls, 20,000 paths, 20 years, 20 x 20 grids; host call"}
+    log(f"rov species wide: {out['rov_species_algo4_p2048']}")
     return out
 
 
