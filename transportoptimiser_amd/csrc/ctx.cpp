@@ -95,10 +95,10 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->mte_bufs.work,  &ctx->mte_bufs.off,  &ctx->mte_bufs.counter,
                         &ctx->rov_bufs.a,     &ctx->rov_bufs.b,    &ctx->rov_bufs.c,
                         &ctx->rov_bufs.d,     &ctx->rov_bufs.e,    &ctx->rov_bufs.f,
-                        &ctx->rov_bufs.g,     &ctx->rov_bufs.h,    &ctx->adj_bufs.a,
-                        &ctx->adj_bufs.b,     &ctx->adj_bufs.c,    &ctx->adj_bufs.d,
-                        &ctx->adj_bufs.e,     &ctx->adj_bufs.f,    &ctx->adj_bufs.g,
-                        &ctx->adj_bufs.h,     &ctx->road_bufs.a,   &ctx->road_bufs.b,
+                        &ctx->rov_bufs.g,     &ctx->rov_bufs.h,    &ctx->adj_bufs.ns,
+                        &ctx->adj_bufs.np,    &ctx->adj_bufs.mu,   &ctx->adj_bufs.lam,
+                        &ctx->adj_bufs.gradr, &ctx->adj_bufs.gradm, &ctx->adj_bufs.gpart,
+                        &ctx->adj_bufs.rpart, &ctx->adj_bufs.ckpt_spare, &ctx->road_bufs.a,   &ctx->road_bufs.b,
                         &ctx->road_bufs.c,    &ctx->road_bufs.d,   &ctx->road_bufs.e,
                         &ctx->road_bufs.f,    &ctx->road_bufs.g,   &ctx->road_bufs.h,
                         &ctx->road_bufs.tail, &ctx->road_bufs.in,

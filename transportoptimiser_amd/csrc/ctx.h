@@ -47,8 +47,12 @@ struct roadmc_ctx {
     long long key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     rmc_devbuf param;
   } rov_graph;
+  // Adjoint backward-pass scratch, kept between calls: one segment's states
+  // and multipliers (ns, np, mu), lambda, the gradients and their partials.
+  // ckpt_spare holds the checkpoints of the last destroyed adjoint handle for
+  // the next forward pass to reuse (several GB at full size).
   struct {
-    rmc_devbuf a, b, c, d, e, f, g, h;
+    rmc_devbuf ns, np, mu, lam, gradr, gradm, gpart, rpart, ckpt_spare;
   } adj_bufs;
   // MTE and the species-state forward pass beyond the LDS tile: the
   // destination-major movement matrices, the per-patch columns, the two

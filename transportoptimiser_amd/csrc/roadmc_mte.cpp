@@ -1233,7 +1233,7 @@ struct roadmc_adjoint {
   int P = 0, L = 10;
   std::vector<double> trans;  // column-major, for dJ/dm = -T o dJ/dM
   rmc_job_buffers bufs;       // mpack, cols, partial, out, endpop(unused), jobs, work, off
-  rmc_devbuf mtpack, ckpt, ns, np, mu, lam, gradr, gradm, gpart, rpart;
+  rmc_devbuf mtpack, ckpt;    // the backward scratch lives in ctx->adj_bufs
 };
 
 namespace {
@@ -1267,11 +1267,17 @@ int adjoint_upload(roadmc_adjoint* adj, const roadmc_species& sp, double tol) {
 extern "C" void roadmc_adjoint_destroy(roadmc_adjoint* adj) {
   if (!adj) return;
   (void)hipStreamSynchronize(adj->ctx->stream);
+  // The checkpoints go to the context for the next forward pass when they
+  // are larger than the spare it holds.
+  rmc_devbuf& spare = adj->ctx->adj_bufs.ckpt_spare;
+  if (adj->ckpt.ptr && adj->ckpt.bytes > spare.bytes) {
+    rmc_buf_free(spare);
+    spare = adj->ckpt;
+    adj->ckpt = rmc_devbuf{};
+  }
   rmc_devbuf* all[] = {&adj->bufs.mpack, &adj->bufs.cols, &adj->bufs.partial, &adj->bufs.out,
                        &adj->bufs.endpop, &adj->bufs.jobs, &adj->bufs.work, &adj->bufs.off,
-                       &adj->bufs.counter, &adj->mtpack, &adj->ckpt, &adj->ns, &adj->np,
-                       &adj->mu, &adj->lam, &adj->gradr, &adj->gradm, &adj->gpart,
-                       &adj->rpart};
+                       &adj->bufs.counter, &adj->mtpack, &adj->ckpt};
   for (rmc_devbuf* d : all) rmc_buf_free(*d);
   delete adj;
 }
@@ -1329,7 +1335,14 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   const size_t es = adj->fp64 ? 8 : 4;
   const int nck = (jb.T + checkpoint_every - 1) / checkpoint_every;
   rc = adj->fp64 ? adjoint_upload<double>(adj, sp, 0.0) : adjoint_upload<float>(adj, sp, 0.0);
-  if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->ckpt, (size_t)nck * jb.N * jb.Pp * es);
+  const size_t ck_bytes = (size_t)nck * jb.N * jb.Pp * es;
+  rmc_devbuf& spare = ctx->adj_bufs.ckpt_spare;
+  if (!rc && spare.ptr && spare.bytes >= ck_bytes) {
+    adj->ckpt = spare;  // a destroyed handle's checkpoints (no queued work reads them)
+    spare = rmc_devbuf{};
+  } else if (!rc) {
+    rc = rmc_buf_alloc_exact(ctx, adj->ckpt, ck_bytes);
+  }
   if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.partial,
                                     (size_t)jb.n_tiles * (3 + jb.Pp) * sizeof(double));
   if (!rc) rc = rmc_buf_alloc_exact(ctx, adj->bufs.out, (3 + (size_t)jb.Pp) * sizeof(double));
@@ -1397,30 +1410,33 @@ extern "C" int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
   const size_t es = adj->fp64 ? 8 : 4;
   const size_t plane = (size_t)N * Pp;
   const int nsplit = 128;
+  // Scratch shared by every adjoint handle of the context: the calls on one
+  // context are serial and this one ends with a stream synchronization.
+  auto& sc = ctx->adj_bufs;
   int rc;
-  if ((rc = rmc_buf_ensure(ctx, adj->ns, (size_t)L * plane * es)) ||
-      (rc = rmc_buf_ensure(ctx, adj->np, (size_t)L * plane * es)) ||
-      (rc = rmc_buf_ensure(ctx, adj->mu, (size_t)L * plane * es)) ||
-      (rc = rmc_buf_ensure(ctx, adj->lam, plane * es)) ||
-      (rc = rmc_buf_ensure(ctx, adj->gradr, (size_t)Pp * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, adj->gradm, (size_t)Pp * Pp * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, adj->gpart, (size_t)nsplit * Pp * Pp * sizeof(double))) ||
-      (rc = rmc_buf_ensure(ctx, adj->rpart, (size_t)j0.n_tiles * Pp * sizeof(double))))
+  if ((rc = rmc_buf_ensure(ctx, sc.ns, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, sc.np, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, sc.mu, (size_t)L * plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, sc.lam, plane * es)) ||
+      (rc = rmc_buf_ensure(ctx, sc.gradr, (size_t)Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, sc.gradm, (size_t)Pp * Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, sc.gpart, (size_t)nsplit * Pp * Pp * sizeof(double))) ||
+      (rc = rmc_buf_ensure(ctx, sc.rpart, (size_t)j0.n_tiles * Pp * sizeof(double))))
     return rc;
   hipStream_t s = ctx->stream;
-  RMC_HIP(ctx, hipMemsetAsync(adj->gradr.ptr, 0, (size_t)Pp * sizeof(double), s));
-  RMC_HIP(ctx, hipMemsetAsync(adj->gradm.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
+  RMC_HIP(ctx, hipMemsetAsync(sc.gradr.ptr, 0, (size_t)Pp * sizeof(double), s));
+  RMC_HIP(ctx, hipMemsetAsync(sc.gradm.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
   roadmc::AdjJob a{};
   a.mpack = adj->bufs.mpack.ptr;
   a.mtpack = adj->mtpack.ptr;
   a.cols = adj->bufs.cols.ptr;
   a.ckpt = adj->ckpt.ptr;
-  a.ns = adj->ns.ptr;
-  a.np = adj->np.ptr;
-  a.mu = adj->mu.ptr;
-  a.lam = adj->lam.ptr;
-  a.grad_r = static_cast<double*>(adj->gradr.ptr);
-  a.grad_part = static_cast<double*>(adj->rpart.ptr);
+  a.ns = sc.ns.ptr;
+  a.np = sc.np.ptr;
+  a.mu = sc.mu.ptr;
+  a.lam = sc.lam.ptr;
+  a.grad_r = static_cast<double*>(sc.gradr.ptr);
+  a.grad_part = static_cast<double*>(sc.rpart.ptr);
   a.sigma = j0.sigma;
   a.dobj = d_objective;
   a.P = P;
@@ -1440,14 +1456,14 @@ extern "C" int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
     a.y1 = std::min(j0.T, a.y0 + L) - 1;
     RMC_HIP(ctx, roadmc::launch_adj_segment(adj->fp64, a, j0.n_tiles, lds, s));
     const int K = (a.y1 - a.y0 + 1) * N;
-    RMC_HIP(ctx, roadmc::launch_adj_gradm(adj->fp64, adj->ns.ptr, adj->mu.ptr, K, Pp, nsplit,
-                                          static_cast<double*>(adj->gpart.ptr),
-                                          static_cast<double*>(adj->gradm.ptr), s));
+    RMC_HIP(ctx, roadmc::launch_adj_gradm(adj->fp64, sc.ns.ptr, sc.mu.ptr, K, Pp, nsplit,
+                                          static_cast<double*>(sc.gpart.ptr),
+                                          static_cast<double*>(sc.gradm.ptr), s));
   }
   std::vector<double> gm((size_t)Pp * Pp), gr(Pp);
-  RMC_HIP(ctx, hipMemcpyAsync(gm.data(), adj->gradm.ptr, gm.size() * sizeof(double),
+  RMC_HIP(ctx, hipMemcpyAsync(gm.data(), sc.gradm.ptr, gm.size() * sizeof(double),
                               hipMemcpyDeviceToHost, s));
-  RMC_HIP(ctx, hipMemcpyAsync(gr.data(), adj->gradr.ptr, gr.size() * sizeof(double),
+  RMC_HIP(ctx, hipMemcpyAsync(gr.data(), sc.gradr.ptr, gr.size() * sizeof(double),
                               hipMemcpyDeviceToHost, s));
   RMC_HIP(ctx, hipStreamSynchronize(s));
   for (double v : gr)
