@@ -24,7 +24,9 @@ struct AdjJob {
 };
 
 size_t adj_lds_bytes(bool fp64, int BM, int Pp);
-hipError_t adj_set_lds_limit(bool fp64, size_t bytes);
+// Dynamic LDS limit of the segment kernel form that launch_adj_segment runs
+// for tiles of BM paths over Pp patches.
+hipError_t adj_set_lds_limit(bool fp64, int BM, int Pp, size_t bytes);
 hipError_t launch_adj_segment(bool fp64, const AdjJob& jb, int n_tiles, size_t lds,
                               hipStream_t s);
 // grad += sum_k A[k]^T B[k] for K rows of Pp (Pp a multiple of 64).

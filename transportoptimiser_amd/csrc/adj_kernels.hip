@@ -65,12 +65,16 @@ __device__ __forceinline__ void adj_noise_items(const AdjJob& jb, int p0, int tr
   }
 }
 
-template <class TR>
+// NT bounds the output tiles per wave (the launch picks the smallest form
+// that holds ceil(tiles / waves)): the accumulators and operands of NT tiles
+// are allocated whether or not a wave uses them, so a smaller NT leaves
+// registers for more resident workgroups.
+template <class TR, int NT>
 __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   using AccT = typename TR::AccT;
-  constexpr int TM = TR::TM, TN = TR::TN, KV = TR::KV, NACC = TR::NACC, NT = TR::NT;
+  constexpr int TM = TR::TM, TN = TR::TN, KV = TR::KV, NACC = TR::NACC;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -132,9 +136,18 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
     }
     switch (my_n) {
       case 1: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
-      case 2: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
-      case 3: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
-      case 4: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane); break;
+      case 2:
+        if constexpr (NT >= 2)
+          year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane);
+        break;
+      case 3:
+        if constexpr (NT >= 3)
+          year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane);
+        break;
+      case 4:
+        if constexpr (NT >= 4)
+          year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDA, arow, akoff, lane);
+        break;
       default:
         if constexpr (NT > 4) {
           switch (my_n) {
@@ -464,22 +477,37 @@ size_t adj_lds_bytes(bool fp64, int BM, int Pp) {
   return a + (size_t)Pp * 8 * (1 + BM / rows);
 }
 
-hipError_t adj_set_lds_limit(bool fp64, size_t bytes) {
-  return fp64 ? hipFuncSetAttribute(reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF64>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
-              : hipFuncSetAttribute(reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF32>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+namespace {
+
+// The segment kernel form for a job: FP64 at its traits' tile bound; FP32 at
+// the smallest of 1, 2 and 4 tiles per wave that holds the job's tiles.
+const void* adj_segment_fn(bool fp64, int BM, int Pp) {
+  if (fp64) return reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF64, TraitsF64::NT>);
+  const int nwaves = kMteThreads / 64;
+  const int per_wave = ((BM / TraitsF32::TM) * (Pp / TraitsF32::TN) + nwaves - 1) / nwaves;
+  if (per_wave <= 1) return reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF32, 1>);
+  if (per_wave <= 2) return reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF32, 2>);
+  return reinterpret_cast<const void*>(&adj_segment_kernel<TraitsF32, TraitsF32::NT>);
+}
+
+}  // namespace
+
+hipError_t adj_set_lds_limit(bool fp64, int BM, int Pp, size_t bytes) {
+  return hipFuncSetAttribute(adj_segment_fn(fp64, BM, Pp),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t launch_adj_segment(bool fp64, const AdjJob& jb, int n_tiles, size_t lds,
                               hipStream_t s) {
-  if (fp64)
-    hipLaunchKernelGGL(adj_segment_kernel<TraitsF64>, dim3(n_tiles), dim3(kMteThreads), lds, s,
-                       jb);
-  else
-    hipLaunchKernelGGL(adj_segment_kernel<TraitsF32>, dim3(n_tiles), dim3(kMteThreads), lds, s,
-                       jb);
-  hipError_t e = hipGetLastError();
+  const int nwaves = kMteThreads / 64;
+  const int ntiles_f32 = (jb.BM / TraitsF32::TM) * (jb.Pp / TraitsF32::TN);
+  const int ntiles_f64 = (jb.BM / TraitsF64::TM) * (jb.Pp / TraitsF64::TN);
+  if ((fp64 ? ntiles_f64 : ntiles_f32) > (fp64 ? TraitsF64::NT : TraitsF32::NT) * nwaves)
+    return hipErrorInvalidValue;  // more tiles than the waves can hold
+  void* args[] = {const_cast<AdjJob*>(&jb)};
+  hipError_t e = hipLaunchKernel(adj_segment_fn(fp64, jb.BM, jb.Pp), dim3(n_tiles),
+                                 dim3(kMteThreads), args, lds, s);
+  if (e == hipSuccess) e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(adj_reduce_r_kernel, dim3(jb.P), dim3(64), 0, s, jb.grad_part, n_tiles,
                      jb.Pp, jb.P, jb.grad_r);

@@ -62,16 +62,21 @@ __host__ __device__ inline size_t mte_pack_index(bool fp64, int Kp, int k, int j
   return (((size_t)cb * nkg + kg) * 64 + lane) * 2 + s;
 }
 
+// Output tiles per wave of a job with BM rows of paths over Pp patches; the
+// launches take the maximum over their jobs and run the smallest kernel form
+// that holds it (more resident workgroups where registers allow).
+int mte_tiles_per_wave(int prec, int BM, int Pp);
+// Dynamic LDS limit of every kernel form of a precision.
 hipError_t mte_set_lds_limit(int prec, size_t bytes);
-hipError_t launch_mte_direct(int prec, const MteJob* jobs, const int2* work,
-                             int n_work, size_t lds, hipStream_t s);
+hipError_t launch_mte_direct(int prec, int tiles_per_wave, const MteJob* jobs,
+                             const int2* work, int n_work, size_t lds, hipStream_t s);
 // Converts FP32-packed matrices (n_floats, a multiple of 512, each matrix
 // with an even k-group count) in place to the split-bf16 operand order.
 hipError_t launch_mte_split_b3(float* m, size_t n_floats, hipStream_t s);
 // Work queues: items work[qbeg[q] .. qbeg[q+1]) form queue q; counters[q]
 // must be zero at launch.
 constexpr int kMteQueues = 8;  // one per XCD of an MI355X
-hipError_t launch_mte_persistent(int prec, const MteJob* jobs,
+hipError_t launch_mte_persistent(int prec, int tiles_per_wave, const MteJob* jobs,
                                  const int2* work, const int* qbeg, int nq,
                                  int* counters, int grid, size_t lds,
                                  hipStream_t s);

@@ -47,14 +47,17 @@ size_t mte_lds_bytes(int prec, int BM, int Pp, int n_rem_split) {
 // Simulates one tile (BM paths of one job) for all years and writes the
 // tile's tallies.  Every thread of the workgroup calls it.  The growth law is
 // a template parameter so that the epilogue has no per-element branches.
-template <class TR, int LAW>
+// NT bounds the output tiles per wave: the accumulators and operands of NT
+// tiles are allocated whether or not a wave uses them, so the launch picks the
+// smallest form that holds its jobs' tiles, which leaves registers for more
+// resident workgroups.
+template <class TR, int LAW, int NT>
 __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
                                              char* smem) {
   using T = typename TR::T;
   using VecT = typename TR::VecT;
   using AccT = typename TR::AccT;
-  constexpr int TM = TR::TM, TN = TR::TN, KV = TR::KV, NACC = TR::NACC,
-                NT = TR::NT;
+  constexpr int TM = TR::TM, TN = TR::TN, KV = TR::KV, NACC = TR::NACC;
   const int tid = threadIdx.x, lane = tid & 63;
   // Wave index as a scalar: the per-wave tile loops below then branch on
   // SGPRs instead of exec masks.
@@ -220,13 +223,18 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
       // One branch per year on the (wave-uniform) tile count and row sharing;
       // the product loops themselves are straight-line code.
       switch (my_n * 2 + (shared_a ? 1 : 0)) {
-        case 2: case 3: year_product<TR, 1>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 4: year_product<TR, 2>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 5: year_product<TR, 2, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 6: year_product<TR, 3>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 7: year_product<TR, 3, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 8: year_product<TR, 4>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
-        case 9: year_product<TR, 4, true>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane); break;
+#define RMC_YP(n, sh)                                                                    \
+  if constexpr ((n) <= NT)                                                               \
+    year_product<TR, n, sh>(acc, b0, Bm, A, trow, tcol, LDAy, arow, akoff, lane);        \
+  break
+        case 2: case 3: RMC_YP(1, false);
+        case 4: RMC_YP(2, false);
+        case 5: RMC_YP(2, true);
+        case 6: RMC_YP(3, false);
+        case 7: RMC_YP(3, true);
+        case 8: RMC_YP(4, false);
+        case 9: RMC_YP(4, true);
+#undef RMC_YP
         default:
           if constexpr (NT > 4) {
             switch (my_n) {
@@ -385,21 +393,21 @@ __device__ __forceinline__ void mte_tile_law(const MteJob& jb, const int tile,
   __syncthreads();  // LDS is reused by the next tile
 }
 
-template <class TR>
+template <class TR, int NT>
 __device__ __forceinline__ void mte_tile(const MteJob& jb, const int tile, char* smem) {
   switch (jb.law) {
-    case kLawRefCpu: mte_tile_law<TR, kLawRefCpu>(jb, tile, smem); break;
-    case kLawRefGpu: mte_tile_law<TR, kLawRefGpu>(jb, tile, smem); break;
-    default: mte_tile_law<TR, kLawLognormal>(jb, tile, smem); break;
+    case kLawRefCpu: mte_tile_law<TR, kLawRefCpu, NT>(jb, tile, smem); break;
+    case kLawRefGpu: mte_tile_law<TR, kLawRefGpu, NT>(jb, tile, smem); break;
+    default: mte_tile_law<TR, kLawLognormal, NT>(jb, tile, smem); break;
   }
 }
 
-template <class TR>
+template <class TR, int NT>
 __global__ __launch_bounds__(kMteThreads) void mte_direct_kernel(
     const MteJob* __restrict__ jobs, const int2* __restrict__ work) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int2 w = work[blockIdx.x];
-  mte_tile<TR>(jobs[w.x], w.y, smem);
+  mte_tile<TR, NT>(jobs[w.x], w.y, smem);
 }
 
 // Persistent form.  The work items are split into nq queues, one per XCD,
@@ -408,7 +416,7 @@ __global__ __launch_bounds__(kMteThreads) void mte_direct_kernel(
 // XCDs round robin (workgroup b runs on XCD b % 8), so workgroup b first
 // drains queue b % nq and then takes what is left in the others.  Every
 // workgroup leaves each queue once its counter passes the queue length.
-template <class TR>
+template <class TR, int NT>
 __global__ __launch_bounds__(kMteThreads) void mte_persistent_kernel(
     const MteJob* __restrict__ jobs, const int2* __restrict__ work,
     const int* __restrict__ qbeg, int nq, int* __restrict__ counters) {
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(kMteThreads) void mte_persistent_kernel(
       __syncthreads();
       if (w >= len) break;
       const int2 it = work[base + w];
-      mte_tile<TR>(jobs[it.x], it.y, smem);
+      mte_tile<TR, NT>(jobs[it.x], it.y, smem);
     }
   }
 }
@@ -452,64 +460,72 @@ __global__ void fill_i32_kernel(int* p, int v, int n) {
   if (i < n) p[i] = v;
 }
 
-template <class TR>
-static hipError_t set_lds_limit_t(size_t bytes) {
-  const hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&mte_direct_kernel<TR>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&mte_persistent_kernel<TR>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+namespace {
+
+// Kernel forms: FP64 and split-bf16 at their traits' tile bound; FP32 at 1, 2
+// or 4 tiles per wave.  direct selects mte_direct_kernel, else the persistent
+// kernel.
+template <class TR, int NT>
+const void* mte_fn(bool direct) {
+  return direct ? reinterpret_cast<const void*>(&mte_direct_kernel<TR, NT>)
+                : reinterpret_cast<const void*>(&mte_persistent_kernel<TR, NT>);
+}
+
+const void* mte_form(int prec, int tiles_per_wave, bool direct) {
+  switch (prec) {
+    case kPrecF64: return mte_fn<TraitsF64, TraitsF64::NT>(direct);
+    case kPrecB3: return mte_fn<TraitsB3, TraitsB3::NT>(direct);
+    default:
+      if (tiles_per_wave <= 1) return mte_fn<TraitsF32, 1>(direct);
+      if (tiles_per_wave <= 2) return mte_fn<TraitsF32, 2>(direct);
+      return mte_fn<TraitsF32, TraitsF32::NT>(direct);
+  }
+}
+
+int mte_max_tiles_per_wave(int prec) {
+  return prec == kPrecF64 ? TraitsF64::NT : prec == kPrecB3 ? TraitsB3::NT : TraitsF32::NT;
+}
+
+}  // namespace
+
+int mte_tiles_per_wave(int prec, int BM, int Pp) {
+  const int tm = prec == kPrecF64 ? TraitsF64::TM : TraitsF32::TM;
+  const int tn = prec == kPrecF64 ? TraitsF64::TN : TraitsF32::TN;
+  const int nwaves = kMteThreads / 64;
+  return ((BM / tm) * (Pp / tn) + nwaves - 1) / nwaves;
 }
 
 hipError_t mte_set_lds_limit(int prec, size_t bytes) {
-  switch (prec) {
-    case kPrecF64: return set_lds_limit_t<TraitsF64>(bytes);
-    case kPrecB3: return set_lds_limit_t<TraitsB3>(bytes);
-    default: return set_lds_limit_t<TraitsF32>(bytes);
-  }
+  for (int tpw : {1, 2, 4})
+    for (bool direct : {true, false}) {
+      const hipError_t e = hipFuncSetAttribute(mte_form(prec, tpw, direct),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)bytes);
+      if (e != hipSuccess) return e;
+    }
+  return hipSuccess;
 }
 
-hipError_t launch_mte_direct(int prec, const MteJob* jobs, const int2* work,
+hipError_t launch_mte_direct(int prec, int tiles_per_wave, const MteJob* jobs, const int2* work,
                              int n_work, size_t lds, hipStream_t s) {
   if (n_work <= 0) return hipSuccess;
-  switch (prec) {
-    case kPrecF64:
-      hipLaunchKernelGGL(mte_direct_kernel<TraitsF64>, dim3(n_work), dim3(kMteThreads), lds, s,
-                         jobs, work);
-      break;
-    case kPrecB3:
-      hipLaunchKernelGGL(mte_direct_kernel<TraitsB3>, dim3(n_work), dim3(kMteThreads), lds, s,
-                         jobs, work);
-      break;
-    default:
-      hipLaunchKernelGGL(mte_direct_kernel<TraitsF32>, dim3(n_work), dim3(kMteThreads), lds, s,
-                         jobs, work);
-      break;
-  }
-  return hipGetLastError();
+  if (tiles_per_wave > mte_max_tiles_per_wave(prec)) return hipErrorInvalidValue;
+  void* args[] = {&jobs, &work};
+  const hipError_t e = hipLaunchKernel(mte_form(prec, tiles_per_wave, true), dim3(n_work),
+                                       dim3(kMteThreads), args, lds, s);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
-hipError_t launch_mte_persistent(int prec, const MteJob* jobs,
+hipError_t launch_mte_persistent(int prec, int tiles_per_wave, const MteJob* jobs,
                                  const int2* work, const int* qbeg, int nq,
                                  int* counters, int grid, size_t lds,
                                  hipStream_t s) {
   if (nq <= 0 || grid <= 0) return hipSuccess;
-  switch (prec) {
-    case kPrecF64:
-      hipLaunchKernelGGL(mte_persistent_kernel<TraitsF64>, dim3(grid), dim3(kMteThreads), lds,
-                         s, jobs, work, qbeg, nq, counters);
-      break;
-    case kPrecB3:
-      hipLaunchKernelGGL(mte_persistent_kernel<TraitsB3>, dim3(grid), dim3(kMteThreads), lds,
-                         s, jobs, work, qbeg, nq, counters);
-      break;
-    default:
-      hipLaunchKernelGGL(mte_persistent_kernel<TraitsF32>, dim3(grid), dim3(kMteThreads), lds,
-                         s, jobs, work, qbeg, nq, counters);
-      break;
-  }
-  return hipGetLastError();
+  if (tiles_per_wave > mte_max_tiles_per_wave(prec)) return hipErrorInvalidValue;
+  void* args[] = {&jobs, &work, &qbeg, &nq, &counters};
+  const hipError_t e = hipLaunchKernel(mte_form(prec, tiles_per_wave, false), dim3(grid),
+                                       dim3(kMteThreads), args, lds, s);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 // In-place conversion of FP32-packed movement matrices to the split-bf16

@@ -159,6 +159,7 @@ struct JobSet {
   size_t mpack_elems = 0, cols_elems = 0, part_elems = 0, out_elems = 0,
          endpop_elems = 0;
   size_t lds = 0;
+  int tpw = 1;  // most output tiles per wave over the jobs
 };
 
 void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roads,
@@ -212,6 +213,7 @@ void plan_jobs(const roadmc_ctx* ctx, const roadmc_mte_problem* roads, int n_roa
         jb.split = 1;
       js.lds = std::max(js.lds, roadmc::mte_lds_bytes(js.prec, jb.BM, jb.Pp,
                                                       jb.split ? rem : 0));
+      js.tpw = std::max(js.tpw, roadmc::mte_tiles_per_wave(js.prec, jb.BM, jb.Pp));
       js.jobs.push_back(jb);
       js.road_of.push_back(r);
       js.species_of.push_back(s);
@@ -655,7 +657,7 @@ extern "C" int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem
                : upload_jobs<float>(ctx, problem, js, b, false);
   if (rc) return rc;
   if ((rc = ensure_lds_limit(ctx, js.prec, js.lds))) return rc;
-  RMC_HIP(ctx, roadmc::launch_mte_direct(js.prec, static_cast<MteJob*>(b.jobs.ptr),
+  RMC_HIP(ctx, roadmc::launch_mte_direct(js.prec, js.tpw, static_cast<MteJob*>(b.jobs.ptr),
                                          static_cast<int2*>(b.work.ptr), (int)js.work.size(),
                                          js.lds, ctx->stream));
   RMC_HIP(ctx, roadmc::launch_mte_reduce(static_cast<MteJob*>(b.jobs.ptr), (int)js.jobs.size(),
@@ -691,7 +693,7 @@ int capture_batch_graph(roadmc_batch* bt) {
   hipError_t e = roadmc::launch_fill_i32(static_cast<int*>(b.counter.ptr), 0, 16, s);
   if (e == hipSuccess)
     e = roadmc::launch_mte_persistent(
-        js.prec, static_cast<MteJob*>(b.jobs.ptr), static_cast<int2*>(b.work.ptr),
+        js.prec, js.tpw, static_cast<MteJob*>(b.jobs.ptr), static_cast<int2*>(b.work.ptr),
         static_cast<int*>(b.counter.ptr) + 16, (int)js.qbeg.size() - 1,
         static_cast<int*>(b.counter.ptr), bt->grid, js.lds, s);
   if (e == hipSuccess)
@@ -1375,7 +1377,11 @@ extern "C" int roadmc_mte_adjoint_fwd(roadmc_ctx* ctx, const roadmc_mte_problem*
   if (e == hipSuccess)
     e = hipMemcpyAsync(adj->bufs.off.ptr, &off0, sizeof(off0), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = roadmc::launch_mte_direct(adj->fp64 ? roadmc::kPrecF64 : roadmc::kPrecF32, static_cast<MteJob*>(adj->bufs.jobs.ptr),
+    e = roadmc::launch_mte_direct(adj->fp64 ? roadmc::kPrecF64 : roadmc::kPrecF32,
+                                  roadmc::mte_tiles_per_wave(adj->fp64 ? roadmc::kPrecF64
+                                                                       : roadmc::kPrecF32,
+                                                             jb.BM, jb.Pp),
+                                  static_cast<MteJob*>(adj->bufs.jobs.ptr),
                                   static_cast<int2*>(adj->bufs.work.ptr), jb.n_tiles, lds, s);
   if (e == hipSuccess)
     e = roadmc::launch_mte_reduce(static_cast<MteJob*>(adj->bufs.jobs.ptr), 1,
@@ -1449,7 +1455,7 @@ extern "C" int roadmc_mte_adjoint_bwd(roadmc_adjoint* adj, double d_objective,
   a.key0 = j0.key0;
   a.key1 = j0.key1;
   const size_t lds = roadmc::adj_lds_bytes(adj->fp64, j0.BM, Pp);
-  RMC_HIP(ctx, roadmc::adj_set_lds_limit(adj->fp64, lds));
+  RMC_HIP(ctx, roadmc::adj_set_lds_limit(adj->fp64, j0.BM, Pp, lds));
   const int nseg = (j0.T + L - 1) / L;
   for (int sg = nseg - 1; sg >= 0; --sg) {
     a.y0 = sg * L;
