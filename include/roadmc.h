@@ -502,6 +502,28 @@ typedef struct {
 int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
                     roadmc_rovs_result* result);
 
+/* Smoothed-validity value of the species-state valuation under a fixed
+ * policy, and its gradient with respect to every species' survival matrices.
+ * No counterpart in the reference: there a control is valid only when every
+ * adjusted population is at its threshold (SimulateGPU.cu:1046-1054,
+ * 2757-2771), an indicator, so the value's pathwise derivative with respect to
+ * mortality is zero almost everywhere.  Here the indicator is replaced by
+ * prod_s sigmoid((A_s - threshold_s) / tau) and the decisions by the given
+ * policy [T][N] (for instance roadmc_rovs_run's opt_control):
+ *   value = (1/N) sum_paths sum_{t<T} d^t UP_t (Q_0 + (Q_u - Q_0) prod_s sigma_s,t)
+ * with u = policy[t][path], A_s,t the adjusted population of species s under u
+ * (total population x surviving fraction), d = 1 / (1 + rrr dt / 100), UP the
+ * unit profit of roadmc_rovs_run's streams, and every species' populations
+ * moved by trans o surv_u and grown as roadmc_rovs_run's forward pass does.
+ * grad_surv [S][C][P*P] (column-major like surv, or NULL for the value only)
+ * receives dvalue/dsurv; the crossing mortality m = 1 - surv has the opposite
+ * gradient.  FP32 products, P <= 512 (ROADMC_ERR_UNSUPPORTED otherwise), tau >
+ * 0, every policy entry in [0, n_controls); problem->method and dim_res are
+ * not used. */
+int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
+                            const int32_t* policy, double tau, double* value,
+                            double* grad_surv);
+
 /* ---- On-disk formats ------------------------------------------------------- */
 
 /* The surrogate training data file of a scenario ("surrogate_data"):

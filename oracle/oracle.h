@@ -213,6 +213,13 @@ int orc_rovs_backward(const orc_rovs_spec* s, const int32_t* controls, const dou
 int orc_rovs_backward_method(const orc_rovs_spec* s, int32_t method, const int32_t* controls,
                              const double* total_pops, const double* aars, const double* profit,
                              double* V, int32_t* optc, double* grids, double out[3]);
+/* Smoothed-validity value J of the species-state valuation under a fixed
+ * policy [T][N] (the validity indicator replaced by prod_s sigmoid((A_s -
+ * threshold_s) / tau), realized cash flows d^t UP_t (Q_0 + (Q_u - Q_0) v)) and
+ * dJ/dsurv [S][C][P*P] (column-major, may be NULL) by reverse mode over each
+ * path.  Returns 0, -1 out of memory, -2 bad arguments. */
+int orc_rovs_smooth_grad(const orc_rovs_spec* s, const int32_t* policy, double tau, double* J,
+                         double* grad_surv);
 
 #ifdef __cplusplus
 }

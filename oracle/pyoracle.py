@@ -480,6 +480,8 @@ def _rovs_lib():
     h.orc_rovs_backward_method.argtypes = [C.POINTER(OrcRovsSpec), C.c_int32, _ip, _dp, _dp, _dp,
                                            _dp, _ip, _dp, _dp]
     h.orc_rovs_backward_method.restype = C.c_int
+    h.orc_rovs_smooth_grad.argtypes = [C.POINTER(OrcRovsSpec), _ip, C.c_double, _dp, _dp]
+    h.orc_rovs_smooth_grad.restype = C.c_int
     return h
 
 
@@ -497,6 +499,26 @@ def rovs_forward(pr):
     if rc != 0:
         raise RuntimeError(f"orc_rovs_forward failed: {rc}")
     return ctl, tp, aa, up
+
+
+def rovs_smooth_grad(pr, policy, tau, grad=True):
+    """(J, dJ/dsurv [S][C][P][P] with rows the origin, or None) of the
+    smoothed-validity value under the policy [T][N]."""
+    keep = []
+    s = _rovs_spec(pr, keep)
+    T, N, S, Cc, P = pr.n_years, pr.n_paths, len(pr.species), len(pr.flow), pr.n_patches
+    pol = np.ascontiguousarray(policy, dtype=np.int32)
+    if pol.shape != (T, N):
+        raise ValueError("policy must be (n_years, n_paths)")
+    J = np.zeros(1)
+    g = np.zeros((S, Cc, P, P)) if grad else None
+    rc = _rovs_lib().orc_rovs_smooth_grad(C.byref(s), pol.ctypes.data_as(_ip), float(tau),
+                                          J.ctypes.data_as(_dp),
+                                          g.ctypes.data_as(_dp) if grad else None)
+    if rc != 0:
+        raise RuntimeError(f"orc_rovs_smooth_grad failed: {rc}")
+    # column-major [j][k] per matrix -> rows = origin k
+    return float(J[0]), (g.transpose(0, 1, 3, 2).copy() if grad else None)
 
 
 def rovs_backward(pr, controls, total_pops, aars, profit, method=4, grids=False):

@@ -511,3 +511,122 @@ int orc_rovs_backward_method(const orc_rovs_spec* s, int32_t method, const int32
   if (method != 4 && method != 6) return -3;
   return rv_backward(s, method, controls, tp, aars, profit, V, optc, grids, out);
 }
+
+/* Smoothed-validity value of the species-state valuation under a fixed
+ * policy and its gradient with respect to the survival matrices.  Not in the
+ * reference: its validity test (every adjusted population at its threshold,
+ * SimulateGPU.cu:1046-1054, 2757-2771) is an indicator, so the value's
+ * pathwise derivative with respect to mortality vanishes.  Here that
+ * indicator becomes prod_s sigmoid((A_s - theta_s) / tau) and the decisions
+ * are the given policy, so that
+ *   J = (1/N) sum_p sum_{t<T} d^t UP_t [Q_0 + (Q_u - Q_0) prod_s sigma_s,t],
+ *   u = policy[t][p], A_s,t = sum_j (n_s,t M_s,u)_j (the adjusted population
+ *   under u), d = 1 / (1 + rrr dt / 100),
+ * with every species' populations moved under the policy's controls and grown
+ * as in orc_rovs_forward (same streams); UP is orc_rovs_forward's unit profit.
+ * The gradient is reverse mode over each path: with w_t = lambda_t+1 o G'(x_t)
+ * + c_t (G'(x) = 1 + g (K - 2x) / K / 100, c_t = dJ_p/dA_s,t), lambda_t =
+ * M_u w_t and dJ/dM_u(k, j) += n_t(k) w_t(j); dJ/dsurv = trans o dJ/dM.
+ * grad_surv [S][C][P*P] column-major like surv (may be NULL). */
+int orc_rovs_smooth_grad(const orc_rovs_spec* s, const int32_t* policy, double tau, double* J,
+                         double* grad_surv) {
+  const int P = s->P, S = s->S, C = s->C, T = s->T;
+  const int64_t N = s->N;
+  if (!(tau > 0.0) || N < 1) return -2;
+  const double disc = 1.0 / (1.0 + s->rrr * s->dt / 100.0);
+  int32_t* ctl = (int32_t*)malloc(sizeof(int32_t) * (size_t)T * N);
+  double* tp = (double*)malloc(sizeof(double) * (size_t)S * (T + 1) * N);
+  double* aa = (double*)malloc(sizeof(double) * (size_t)S * T * C * N);
+  double* up = (double*)malloc(sizeof(double) * (size_t)T * N);
+  double* n = (double*)malloc(sizeof(double) * (size_t)S * (T + 1) * P);  /* n_s,t */
+  double* x = (double*)malloc(sizeof(double) * (size_t)S * T * P);        /* moved, pre-growth */
+  double* gm = (double*)malloc(sizeof(double) * (size_t)S * T);          /* growth-rate means */
+  double* A = (double*)malloc(sizeof(double) * (size_t)S * T);
+  double* y = (double*)malloc(sizeof(double) * (size_t)C * P);
+  double* lam = (double*)malloc(sizeof(double) * (size_t)P * 2);
+  int rc = 0;
+  if (!ctl || !tp || !aa || !up || !n || !x || !gm || !A || !y || !lam) {
+    rc = -1;
+    goto done;
+  }
+  if ((rc = orc_rovs_forward(s, ctl, tp, aa, up)) != 0) goto done;
+  if (grad_surv) memset(grad_surv, 0, sizeof(double) * (size_t)S * C * P * P);
+  double jsum = 0.0;
+  for (int64_t p = 0; p < N; ++p) {
+    /* forward of every species under the policy */
+    for (int sp = 0; sp < S; ++sp) {
+      double* n0 = n + (size_t)sp * (T + 1) * P;
+      double grm = s->params[sp * 8 + 0], rs[8];
+      for (int j = 0; j < P; ++j) n0[j] = s->init_pop[(size_t)sp * P + j];
+      for (int t = 0; t < T; ++t) {
+        const int u = policy[(size_t)t * N + p];
+        rv_products(s, sp, n0 + (size_t)t * P, y, rs);
+        A[sp * T + t] = rs[u];
+        memcpy(x + ((size_t)sp * T + t) * P, y + (size_t)u * P, sizeof(double) * P);
+        rv_grow(s, sp, p, t, y + (size_t)u * P, n0 + (size_t)(t + 1) * P, &grm);
+        gm[sp * T + t] = grm;
+      }
+    }
+    /* the path's value and the coefficients dJ_p / dA_s,t */
+    double f = 1.0, jp = 0.0;
+    for (int t = 0; t < T; ++t) {
+      const int u = policy[(size_t)t * N + p];
+      const double upt = up[(size_t)t * N + p];
+      double v = 1.0;
+      for (int sp = 0; sp < S; ++sp)
+        v *= 1.0 / (1.0 + exp(-(A[sp * T + t] - s->params[sp * 8 + 3]) / tau));
+      jp += f * upt * (s->flow[0] + (s->flow[u] - s->flow[0]) * v);
+      for (int sp = 0; sp < S; ++sp) {
+        const double sg = 1.0 / (1.0 + exp(-(A[sp * T + t] - s->params[sp * 8 + 3]) / tau));
+        A[sp * T + t] = f * upt * (s->flow[u] - s->flow[0]) * v * (1.0 - sg) / tau;  /* c_t */
+      }
+      f *= disc;
+    }
+    jsum += jp;
+    if (!grad_surv) continue;
+    /* reverse sweep per species */
+    for (int sp = 0; sp < S; ++sp) {
+      const double* tr = s->trans + (size_t)sp * P * P;
+      const double* K = s->capacity + (size_t)sp * P;
+      const double lv = s->params[sp * 8 + 7];
+      const uint32_t tag0 = rv_tag(s->road_id, (uint32_t)sp, 0);
+      double* l = lam;       /* lambda_t+1 */
+      double* w = lam + P;   /* w_t */
+      for (int j = 0; j < P; ++j) l[j] = 0.0;
+      for (int t = T - 1; t >= 0; --t) {
+        const int u = policy[(size_t)t * N + p];
+        const double* xt = x + ((size_t)sp * T + t) * P;
+        const double* nt = n + ((size_t)sp * (T + 1) + t) * P;
+        const double* sv = s->surv + ((size_t)sp * C + u) * P * P;
+        double* gs = grad_surv + ((size_t)sp * C + u) * P * P;
+        for (int j = 0; j < P; ++j) {
+          const double z = orc_patch_normal(s->seed, tag0, (uint32_t)p, (uint32_t)t, (uint32_t)j, 0);
+          const double g = gm[sp * T + t] + z * lv;
+          w[j] = l[j] * (1.0 + g * (K[j] - 2.0 * xt[j]) / K[j] / 100.0) + A[sp * T + t];
+        }
+        for (int k = 0; k < P; ++k) {
+          double acc = 0.0;
+          for (int j = 0; j < P; ++j) {
+            const size_t cm = (size_t)j * P + k;
+            acc += tr[cm] * sv[cm] * w[j];
+            gs[cm] += tr[cm] * nt[k] * w[j] / (double)N;
+          }
+          l[k] = acc;
+        }
+      }
+    }
+  }
+  *J = jsum / (double)N;
+done:
+  free(ctl);
+  free(tp);
+  free(aa);
+  free(up);
+  free(n);
+  free(x);
+  free(gm);
+  free(A);
+  free(y);
+  free(lam);
+  return rc;
+}

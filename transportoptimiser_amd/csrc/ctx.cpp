@@ -110,7 +110,11 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->rovs_bufs.blk,  &ctx->rovs_bufs.xs,  &ctx->rovs_bufs.ys,
                         &ctx->rovs_bufs.u,    &ctx->rovs_bufs.small, &ctx->rovs_bufs.grid,
                         &ctx->rovs_bufs.args, &ctx->rovs_bufs.ckpt, &ctx->rovs_bufs.grm,
-                        &ctx->rovs_bufs.rvec, &ctx->rovs_bufs.grids};
+                        &ctx->rovs_bufs.rvec, &ctx->rovs_bufs.grids,
+                        &ctx->rovs_grad_bufs.mtpack, &ctx->rovs_grad_bufs.policy,
+                        &ctx->rovs_grad_bufs.coef, &ctx->rovs_grad_bufs.w,
+                        &ctx->rovs_grad_bufs.gpart, &ctx->rovs_grad_bufs.grad,
+                        &ctx->rovs_grad_bufs.jpart};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
   for (auto& l : ctx->road_lanes) {
     for (rmc_devbuf* b : {&l.cross, &l.trans, &l.surv, &l.blockmax, &l.part}) rmc_buf_free(*b);

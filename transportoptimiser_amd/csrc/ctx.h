@@ -82,6 +82,13 @@ struct roadmc_ctx {
     rmc_devbuf mpack, cols, prices, controls, tp, aars, profit, V, optc, adj, flag, blk, xs, ys,
         u, small, grid, args, ckpt, grm, rvec, grids;
   } rovs_bufs;
+  // The smoothed-validity gradient of the species-state valuation: the
+  // transposed movement matrices, the policy, dJ/dA coefficients, the
+  // per-control w planes, the contraction's partials and result, the
+  // per-block values.
+  struct {
+    rmc_devbuf mtpack, policy, coef, w, gpart, grad, jpart;
+  } rovs_grad_bufs;
 };
 
 int rmc_fail(roadmc_ctx* ctx, int code, const std::string& msg);

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "adj_kernels.h"
 #include "knn_kernels.h"
 #include "mte_common.h"
 #include "mte_wide.h"
@@ -563,5 +564,257 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
   if (res->aars)
     for (size_t i = 0; i < aaf.size(); ++i) res->aars[i] = aaf[i];
   if (res->grids) std::memcpy(res->grids, grids_host.data(), grids_host.size() * sizeof(double));
+  return ROADMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothed-validity value under a fixed policy and its gradient with respect
+// to the survival matrices (no counterpart in the reference).
+
+extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
+                                       const int32_t* policy, double tau, double* value,
+                                       double* grad_surv) {
+  if (!ctx) return ROADMC_ERR_INVALID;
+  double v0 = 0.0;
+  roadmc_rovs_result probe{};
+  probe.value = &v0;
+  int rc = validate(ctx, pr, &probe);
+  if (rc) return rc;
+  auto bad = [&](const char* m) { return rmc_fail(ctx, ROADMC_ERR_INVALID, m); };
+  if (!policy || !value) return bad("policy or value is NULL");
+  if (!(tau > 0.0) || !std::isfinite(tau)) return bad("tau must be positive and finite");
+  if (pr->n_patches > roadmc::kRovsMaxPatches)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the smoothed gradient takes P <= 512");
+  if (pr->precision != ROADMC_PREC_FP32)
+    return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the smoothed gradient runs FP32 products");
+  const int P = pr->n_patches, S = pr->n_species, C = pr->n_controls, T = pr->n_years;
+  const int N = (int)pr->n_paths, U = pr->n_fuels + pr->n_commodities;
+  for (size_t i = 0; i < (size_t)T * N; ++i)
+    if (policy[i] < 0 || policy[i] >= C) return bad("policy controls must be in [0, n_controls)");
+  RMC_HIP(ctx, hipSetDevice(ctx->device));
+  // Pp a multiple of 64: the gradient contraction works in 64 x 64 blocks.
+  const int Pp = round_up(P, 64), Pk = round_up(P, 16);
+  const uint32_t k0 = (uint32_t)(pr->seed & 0xffffffffu), k1 = (uint32_t)(pr->seed >> 32);
+  const int nblk = (N + 255) / 256, nsplit = 128;
+  auto& B = ctx->rovs_bufs;
+  auto& G = ctx->rovs_grad_bufs;
+  const size_t Uc = std::max(U, 1);
+  const bool want_grad = grad_surv != nullptr;
+  struct Need {
+    rmc_devbuf* b;
+    size_t bytes;
+  } need[] = {
+      {&B.mpack, (size_t)S * C * Pp * Pk * sizeof(float)},
+      {&G.mtpack, (size_t)S * C * Pp * Pk * sizeof(float)},
+      {&B.cols, (size_t)S * 3 * Pp * sizeof(float)},
+      {&B.prices, (size_t)(T + 1) * Uc * N * sizeof(double)},
+      {&B.controls, (size_t)T * N * sizeof(int)},
+      {&G.policy, (size_t)T * N * sizeof(int)},
+      {&B.tp, (size_t)S * (T + 1) * N * sizeof(float)},
+      {&B.aars, (size_t)S * T * C * N * sizeof(float)},
+      {&B.profit, (size_t)T * N * sizeof(double)},
+      {&B.ckpt, want_grad ? (size_t)S * T * N * Pp * sizeof(float) : 8},
+      {&B.grm, want_grad ? (size_t)S * T * N * sizeof(float) : 8},
+      {&G.coef, (size_t)S * T * N * sizeof(float)},
+      {&G.jpart, (size_t)nblk * sizeof(double)},
+      {&G.w, want_grad ? (size_t)C * T * N * Pp * sizeof(float) : 8},
+      {&G.gpart, want_grad ? (size_t)nsplit * Pp * Pp * sizeof(double) : 8},
+      {&G.grad, want_grad ? (size_t)Pp * Pp * sizeof(double) : 8},
+  };
+  for (auto& n : need)
+    if ((rc = rmc_buf_ensure(ctx, *n.b, n.bytes))) return rc;
+  hipStream_t s = ctx->stream;
+
+  // trans o surv_c and its transpose in MFMA B order; the per-patch columns.
+  std::vector<float> mp((size_t)S * C * Pp * Pk, 0.f), mt((size_t)S * C * Pp * Pk, 0.f);
+  std::vector<float> cols((size_t)S * 3 * Pp), init_total(S);
+  for (int sp = 0; sp < S; ++sp) {
+    const roadmc_rovs_species& sc = pr->species[sp];
+    for (int c = 0; c < C; ++c) {
+      const size_t off = ((size_t)sp * C + c) * Pp * Pk;
+      const double* sv = sc.surv + (size_t)c * P * P;
+      for (int k = 0; k < P; ++k)
+        for (int j = 0; j < P; ++j) {
+          const float v = (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]);
+          mp[off + roadmc::mte_pack_index(false, Pk, k, j)] = v;
+          mt[off + roadmc::mte_pack_index(false, Pk, j, k)] = v;
+        }
+    }
+    float* cl = cols.data() + (size_t)sp * 3 * Pp;
+    double tot = 0.0;
+    for (int j = 0; j < Pp; ++j) {
+      const bool in = j < P;
+      cl[j] = in ? (float)sc.capacity[j] : 1.f;
+      cl[Pp + j] = in ? (float)(1.0 / sc.capacity[j]) : 1.f;
+      cl[2 * Pp + j] = in ? (float)sc.init_pop[j] : 0.f;
+      if (in) tot += (double)(float)sc.init_pop[j];
+    }
+    init_total[sp] = (float)tot;
+  }
+  RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(G.mtpack.ptr, mt.data(), mt.size() * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(G.policy.ptr, policy, (size_t)T * N * sizeof(int),
+                              hipMemcpyHostToDevice, s));
+
+  // Prices (the valuation's streams; its drawn controls are not used) and
+  // unit profit, as roadmc_rovs_run computes them.
+  roadmc::RovForwardArgs fa{};
+  fa.N = N;
+  fa.T = T;
+  fa.U = U;
+  fa.C = C;
+  fa.dt = pr->time_step;
+  for (int u = 0; u < U; ++u) {
+    const roadmc_uncertainty& q = pr->uncertainties[u];
+    double* pp = fa.par + u * 7;
+    pp[0] = q.kind;
+    pp[1] = q.x0;
+    pp[2] = q.mean;
+    pp[3] = q.sd;
+    pp[4] = q.reversion;
+    pp[5] = q.jump_size;
+    pp[6] = q.jump_prob;
+    fa.chol[u * roadmc::kRovMaxU + u] = 1.0;
+  }
+  fa.tag = rmc_tag(pr->road_id, 0, RMC_KIND_ROV);
+  fa.k0 = k0;
+  fa.k1 = k1;
+  fa.states = static_cast<double*>(B.prices.ptr);
+  fa.controls = static_cast<int*>(B.controls.ptr);
+  RMC_HIP(ctx, roadmc::launch_rov_forward(fa, s));
+  roadmc::RovsProfitArgs pa{};
+  pa.N = N;
+  pa.T = T;
+  pa.U = U;
+  pa.n_vehicles = pr->n_vehicles;
+  pa.n_fuels = pr->n_fuels;
+  pa.n_comm = pr->n_commodities;
+  pa.unit_cost = pr->unit_cost;
+  pa.unit_revenue = pr->unit_revenue;
+  for (int v = 0; v < pr->n_vehicles; ++v) {
+    pa.fuel_cost[v] = pr->fuel_cost[v];
+    pa.fuel_idx[v] = pr->fuel_idx[v];
+  }
+  for (int c = 0; c < pr->n_commodities; ++c) {
+    pa.ore_mean[c] = pr->ore_mean[c];
+    pa.ore_sd[c] = pr->ore_sd[c];
+  }
+  pa.tag = rmc_tag(pr->road_id, 0, RMC_KIND_ORE);
+  pa.key0 = k0;
+  pa.key1 = k1;
+  pa.prices = static_cast<const double*>(B.prices.ptr);
+  pa.profit = static_cast<double*>(B.profit.ptr);
+  RMC_HIP(ctx, roadmc::launch_rovs_profit(pa, s));
+
+  // Every species' populations under the policy, each year's tile kept.
+  std::vector<uint32_t> tag0(S);
+  for (int sp = 0; sp < S; ++sp) {
+    roadmc::RovsForwardArgs a{};
+    a.N = N;
+    a.T = T;
+    a.P = P;
+    a.Pp = Pp;
+    a.Pk = Pk;
+    a.C = C;
+    a.mpack = static_cast<const float*>(B.mpack.ptr) + (size_t)sp * C * Pp * Pk;
+    a.cols = static_cast<const float*>(B.cols.ptr) + (size_t)sp * 3 * Pp;
+    for (int i = 0; i < 8; ++i) a.params[i] = pr->species[sp].params[i];
+    a.dt = pr->time_step;
+    a.init_total = init_total[sp];
+    tag0[sp] = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATCH);
+    a.tag0 = tag0[sp];
+    a.tag1 = rmc_tag(pr->road_id, (uint32_t)sp, RMC_KIND_PATH);
+    a.key0 = k0;
+    a.key1 = k1;
+    a.controls = static_cast<const int*>(G.policy.ptr);
+    a.total_pops = static_cast<float*>(B.tp.ptr) + (size_t)sp * (T + 1) * N;
+    a.aars = static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N;
+    a.ck = 1;
+    a.split = 0;
+    if (want_grad) {
+      a.ckpt = static_cast<float*>(B.ckpt.ptr) + (size_t)sp * T * N * Pp;
+      a.grm_out = static_cast<float*>(B.grm.ptr) + (size_t)sp * T * N;
+    }
+    RMC_HIP(ctx, roadmc::launch_rovs_forward(a, s));
+  }
+
+  // The value and the coefficients dJ_p / dA.
+  roadmc::RovsSmoothArgs va{};
+  va.N = N;
+  va.T = T;
+  va.S = S;
+  va.C = C;
+  va.disc = 1.0 / (1.0 + pr->rrr * pr->time_step / 100.0);
+  va.tau = tau;
+  for (int sp = 0; sp < S; ++sp) va.thr[sp] = pr->species[sp].params[3];
+  for (int c = 0; c < C; ++c) va.flow[c] = pr->flow[c];
+  va.total_pops = static_cast<const float*>(B.tp.ptr);
+  va.aars = static_cast<const float*>(B.aars.ptr);
+  va.profit = static_cast<const double*>(B.profit.ptr);
+  va.policy = static_cast<const int*>(G.policy.ptr);
+  va.coef = static_cast<float*>(G.coef.ptr);
+  va.jpart = static_cast<double*>(G.jpart.ptr);
+  RMC_HIP(ctx, roadmc::launch_rovs_smooth_value(va, s));
+  std::vector<double> jpart(nblk);
+  RMC_HIP(ctx, hipMemcpyAsync(jpart.data(), G.jpart.ptr, nblk * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  double jsum = 0.0;
+  for (double v : jpart) jsum += v;
+  if (!std::isfinite(jsum)) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite value");
+  *value = jsum / (double)N;
+  if (!want_grad) return ROADMC_OK;
+
+  // Per species: the reverse sweep, then per control dJ/dM_c = sum over the
+  // years and paths under c of n_t^T w_t (rows in the tile's stored order).
+  std::vector<double> gm((size_t)Pp * Pp);
+  std::vector<int> pos(P);
+  for (int k = 0; k < P; ++k) pos[k] = roadmc::rovs_tile_col(k, false);
+  for (int sp = 0; sp < S; ++sp) {
+    const roadmc_rovs_species& sc = pr->species[sp];
+    roadmc::RovsAdjArgs a{};
+    a.N = N;
+    a.T = T;
+    a.P = P;
+    a.Pp = Pp;
+    a.Pk = Pk;
+    a.C = C;
+    a.mpack = static_cast<const float*>(B.mpack.ptr) + (size_t)sp * C * Pp * Pk;
+    a.mtpack = static_cast<const float*>(G.mtpack.ptr) + (size_t)sp * C * Pp * Pk;
+    a.cols = static_cast<const float*>(B.cols.ptr) + (size_t)sp * 3 * Pp;
+    a.ckpt = static_cast<const float*>(B.ckpt.ptr) + (size_t)sp * T * N * Pp;
+    a.grm = static_cast<const float*>(B.grm.ptr) + (size_t)sp * T * N;
+    a.policy = static_cast<const int*>(G.policy.ptr);
+    a.coef = static_cast<const float*>(G.coef.ptr) + (size_t)sp * T * N;
+    a.lv = (float)sc.params[7];
+    a.tag0 = tag0[sp];
+    a.key0 = k0;
+    a.key1 = k1;
+    a.w_out = static_cast<float*>(G.w.ptr);
+    RMC_HIP(ctx, roadmc::launch_rovs_adj(a, s));
+    for (int c = 0; c < C; ++c) {
+      RMC_HIP(ctx, hipMemsetAsync(G.grad.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
+      RMC_HIP(ctx, roadmc::launch_adj_gradm(false, a.ckpt,
+                                            static_cast<const float*>(G.w.ptr) +
+                                                (size_t)c * T * N * Pp,
+                                            T * N, Pp, nsplit, static_cast<double*>(G.gpart.ptr),
+                                            static_cast<double*>(G.grad.ptr), s));
+      RMC_HIP(ctx, hipMemcpyAsync(gm.data(), G.grad.ptr, gm.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+      RMC_HIP(ctx, hipStreamSynchronize(s));
+      // dJ/dsurv = trans o dJ/dM / N, column-major like surv.
+      double* out = grad_surv + ((size_t)sp * C + c) * P * P;
+      for (int j = 0; j < P; ++j)
+        for (int k = 0; k < P; ++k)
+          out[(size_t)j * P + k] =
+              sc.trans[(size_t)j * P + k] * gm[(size_t)pos[k] * Pp + j] / (double)N;
+    }
+  }
+  for (size_t i = 0; i < (size_t)S * C * P * P; ++i)
+    if (!std::isfinite(grad_surv[i])) return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite gradient");
   return ROADMC_OK;
 }

@@ -123,6 +123,47 @@ struct RovsRecomputeArgs {
   int split;             // split-bf16 products, as the forward pass ran them
 };
 size_t rovs_recompute_lds(int Pp, int C, int S, bool split);
+
+// Smoothed-validity value under a fixed policy (no counterpart in the
+// reference): per path, the discounted cash flows d^t UP_t (Q_0 + (Q_u - Q_0)
+// prod_s sigmoid((A_s,t - thr_s) / tau)), A_s,t = total_pops x aars of the
+// policy's control u, summed per block in a fixed order into jpart
+// [ceil(N / 256)]; coef [S][T][N] = dJ_p / dA_s,t.
+struct RovsSmoothArgs {
+  int N, T, S, C;
+  double disc, tau;
+  double thr[kRovsMaxS];
+  double flow[kRovsMaxC];
+  const float* total_pops;  // [S][T+1][N] under the policy
+  const float* aars;        // [S][T][C][N]
+  const double* profit;     // [T][N]
+  const int* policy;        // [T][N]
+  float* coef;              // [S][T][N]
+  double* jpart;            // [ceil(N / 256)]
+};
+hipError_t launch_rovs_smooth_value(const RovsSmoothArgs& a, hipStream_t s);
+
+// Reverse sweep of one species' populations under the policy, years T-1 .. 0,
+// one 32-path tile per workgroup: x_t = n_t M_u (every control's product, the
+// path's selected), w_t = lambda_t+1 o G'(x_t) + coef_t on the patches (zero
+// on padding), lambda_t = w_t M_u^T; w_t goes to w_out[u] (the other
+// controls' planes get zero rows) for dJ/dM_c = sum over (t, paths with u = c)
+// of n_t^T w_t.  FP32 products on the forward pass's packed operands.
+struct RovsAdjArgs {
+  int N, T, P, Pp, Pk, C;
+  const float* mpack;     // [C][Pp * Pk] trans o surv_c, MFMA B order
+  const float* mtpack;    // [C][Pp * Pk] its transpose, same order
+  const float* cols;      // [3][Pp]: K, 1/K, initial population
+  const float* ckpt;      // [T][N][Pp] populations entering each year, tile order
+  const float* grm;       // [T][N] growth-rate means of each year's growth
+  const int* policy;      // [T][N]
+  const float* coef;      // [T][N] this species' dJ_p / dA_t
+  float lv;               // local variability
+  uint32_t tag0, key0, key1;
+  float* w_out;           // [C][T][N][Pp]
+};
+size_t rovs_adj_lds(int Pp);
+hipError_t launch_rovs_adj(const RovsAdjArgs& a, hipStream_t s);
 // Position of patch k within a row of the population tile.
 int rovs_tile_col(int k, bool split);
 hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s);

@@ -875,6 +875,195 @@ static hipError_t launch_rovs_recompute_t(const RovsRecomputeArgs& a, hipStream_
   return hipGetLastError();
 }
 
+// Smoothed-validity value of each path under the policy and the
+// coefficients dJ_p / dA_s,t (RovsSmoothArgs); one thread per path, the
+// block's values summed in a fixed tree order.
+__global__ __launch_bounds__(256) void rovs_smooth_value_kernel(RovsSmoothArgs a) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int n = blockIdx.x * 256 + tid;
+  double jp = 0.0;
+  if (n < a.N) {
+    double f = 1.0;
+    for (int t = 0; t < a.T; ++t) {
+      const int u = a.policy[(size_t)t * a.N + n];
+      const double up = a.profit[(size_t)t * a.N + n];
+      double sg[kRovsMaxS];
+      double v = 1.0;
+      for (int sp = 0; sp < a.S; ++sp) {
+        const double A = (double)a.total_pops[((size_t)sp * (a.T + 1) + t) * a.N + n] *
+                         (double)a.aars[(((size_t)sp * a.T + t) * a.C + u) * a.N + n];
+        sg[sp] = 1.0 / (1.0 + exp(-(A - a.thr[sp]) / a.tau));
+        v *= sg[sp];
+      }
+      const double dq = a.flow[u] - a.flow[0];
+      jp += f * up * (a.flow[0] + dq * v);
+      for (int sp = 0; sp < a.S; ++sp)
+        a.coef[((size_t)sp * a.T + t) * a.N + n] = (float)(f * up * dq * v * (1.0 - sg[sp]) / a.tau);
+      f *= a.disc;
+    }
+  }
+  red[tid] = jp;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) a.jpart[blockIdx.x] = red[0];
+}
+
+hipError_t launch_rovs_smooth_value(const RovsSmoothArgs& a, hipStream_t s) {
+  if (a.N < 1 || a.S < 1 || a.S > kRovsMaxS || a.C < 1 || a.C > kRovsMaxC || !(a.tau > 0))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rovs_smooth_value_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+size_t rovs_adj_lds(int Pp) {
+  return (size_t)kRovsRows * (Pp + TF::PAD) * sizeof(float)  // population, then w, tile
+         + 2 * kRovsRows * sizeof(float)                     // growth-rate means, coefficients
+         + kRovsRows * sizeof(int);                          // controls of the year
+}
+
+// Reverse sweep of one species under the policy (RovsAdjArgs).  lambda stays
+// in the accumulator layout of the wave's column tiles from year to year: the
+// tile assignment of the two products is the same, so each lane keeps the
+// entries it needs for the next year's w.
+__global__ __launch_bounds__(kRovsMaxThreads) void rovs_adj_kernel(RovsAdjArgs a) {
+  constexpr int NT = kRovsNT;
+  extern __shared__ char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nthreads = blockDim.x, NW = nthreads >> 6;
+  const int Pp = a.Pp, LDA = Pp + TF::PAD, C = a.C, N = a.N;
+  const int p0 = blockIdx.x * kRovsRows;
+  float* A = reinterpret_cast<float*>(smem);
+  float* gm = A + kRovsRows * LDA;
+  float* cf = gm + kRovsRows;
+  int* ctl = reinterpret_cast<int*>(cf + kRovsRows);
+  const float* K = a.cols;
+  const float* Kinv = a.cols + Pp;
+  const int ntc = Pp / TF::TN;
+  const int base = ntc / NW, extra = ntc % NW;
+  const int my_n = base + (wave < extra ? 1 : 0);
+  const int my_t0 = wave * base + (wave < extra ? wave : extra);
+  int tcol[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) tcol[t] = my_t0 + t;
+  const int arow = TF::a_row(lane), akoff = TF::KV * TF::k_lane(lane);
+  const int nkg = a.Pk >> 3;
+  const size_t mbytes = (size_t)Pp * a.Pk * sizeof(float);
+  floatx16 lam[NT], sel[NT], acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) lam[t] = floatx16{};
+
+  for (int year = a.T - 1; year >= 0; --year) {
+    // The tile entering the year (stored positions, as the forward pass
+    // wrote it), the year's controls, coefficients and growth-rate means.
+    const float* src = a.ckpt + (size_t)year * N * Pp;
+    for (int idx = tid; idx < kRovsRows * Pp; idx += nthreads) {
+      const int r = idx / Pp, c = idx - r * Pp;
+      A[r * LDA + c] = (p0 + r < N) ? src[(size_t)(p0 + r) * Pp + c] : 0.f;
+    }
+    if (tid < kRovsRows) {
+      const bool in = p0 + tid < N;
+      const size_t at = (size_t)year * N + p0 + tid;
+      ctl[tid] = in ? a.policy[at] : 0;
+      cf[tid] = in ? a.coef[at] : 0.f;
+      gm[tid] = in ? a.grm[at] : 0.f;
+    }
+    __syncthreads();
+    int LDAy = LDA, laney = lane;
+    asm volatile("" : "+s"(LDAy), "+v"(laney));
+    // x_t: the product under the path's control.
+#pragma unroll
+    for (int t = 0; t < NT; ++t) sel[t] = floatx16{};
+    for (int c = 0; c < C; ++c) {
+      const PackedB B = packed_b(a.mpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
+      if (my_n == 1)
+        rovs_product<TF, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+      else if (my_n == 2)
+        rovs_product<TF, 2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+#pragma unroll
+      for (int r = 0; r < TF::NACC; ++r) {
+        if (ctl[TF::acc_row(laney, r)] == c) {
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            if (t < my_n) sel[t][r] = acc[t][r];
+        }
+      }
+    }
+    __syncthreads();  // every wave has read n_t before w_t overwrites it
+    // w_t = lambda_t+1 o G'(x_t) + coef_t (the growth noise drawn as the
+    // forward pass drew it), into the tile and the per-control planes.
+    int LDAe = LDA, p0e = p0, lanee = lane;
+    asm volatile("" : "+s"(LDAe), "+s"(p0e), "+v"(lanee));
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (t >= my_n) break;
+      const int col = tcol[t] * TF::TN + TF::acc_col(lanee);
+      const int pcol = TF::perm(col);
+      const float Kc = K[col], Ki = Kinv[col];
+      const bool real = col < a.P;
+#pragma unroll
+      for (int r4 = 0; r4 < TF::NACC; r4 += 4) {
+        float z[4];
+        const int row0 = TF::acc_row(lanee, r4);
+        TF::normal4((uint32_t)(p0e + row0) >> 2, (uint32_t)year, (uint32_t)col, a.tag0, a.key0,
+                    a.key1, z);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int row = TF::acc_row(lanee, r4 + s4);
+          const float x = sel[t][r4 + s4];
+          const float g = gm[row] + z[s4] * a.lv;
+          const float w =
+              real ? lam[t][r4 + s4] * (1.f + g * (Kc - 2.f * x) * Ki * 0.01f) + cf[row] : 0.f;
+          A[row * LDAe + pcol] = w;
+          const int path = p0e + row;
+          if (path < N) {
+            const int u = ctl[row];
+            for (int c = 0; c < C; ++c)
+              a.w_out[(((size_t)c * a.T + year) * N + path) * Pp + col] = (c == u) ? w : 0.f;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // lambda_t = w_t M_u^T.
+#pragma unroll
+    for (int t = 0; t < NT; ++t) lam[t] = floatx16{};
+    for (int c = 0; c < C; ++c) {
+      const PackedB B = packed_b(a.mtpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
+      if (my_n == 1)
+        rovs_product<TF, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+      else if (my_n == 2)
+        rovs_product<TF, 2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+#pragma unroll
+      for (int r = 0; r < TF::NACC; ++r) {
+        if (ctl[TF::acc_row(laney, r)] == c) {
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            if (t < my_n) lam[t][r] = acc[t][r];
+        }
+      }
+    }
+    __syncthreads();  // every wave has read w_t before the next year's tile
+  }
+}
+
+hipError_t launch_rovs_adj(const RovsAdjArgs& a, hipStream_t s) {
+  if (a.N < 1 || a.T < 1 || a.C < 1 || a.C > kRovsMaxC || a.Pp % TF::TN || a.Pp > kRovsMaxPatches ||
+      a.P > a.Pp || a.Pk % 16 || a.Pk > a.Pp || a.Pk < a.P)
+    return hipErrorInvalidValue;
+  const size_t lds = rovs_adj_lds(a.Pp);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_adj_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
+  hipLaunchKernelGGL(rovs_adj_kernel, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s) {
   if (a.D < 2 || a.D > 8 || a.S != a.D - 1 || a.C < 1 || a.C > kRovsMaxC || a.Pp % TF::TN ||
       a.Pp > kRovsMaxPatches || a.t < 1 || a.t > a.T - 2 || a.y0 > a.t || a.y0 % a.ck ||

@@ -351,6 +351,8 @@ SIGNATURES: dict[str, tuple] = {
     "roadmc_knn_surrogate_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32,
                                             _dp, _dp]),
     "roadmc_rovs_run": (C.c_int, [C.c_void_p, C.POINTER(RovsProblem), C.POINTER(RovsResult)]),
+    "roadmc_rovs_smooth_grad": (C.c_int, [C.c_void_p, C.POINTER(RovsProblem), _ip, C.c_double,
+                                          _dp, _dp]),
     "roadmc_surrogate_data_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp,
                                               C.c_double, _ip]),
     "roadmc_surrogate_data_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp,

@@ -152,6 +152,25 @@ def simulate_rov_species(ctx, pr: RovsProblem, *, paths: bool = False,
     return out
 
 
+def smooth_value_gradient(ctx, pr: RovsProblem, policy, tau: float, *, grad: bool = True):
+    """``roadmc_rovs_smooth_grad``: the smoothed-validity value under the
+    policy [T][N] (validity prod_s sigmoid((A_s - threshold_s) / tau)) and,
+    with ``grad``, its gradient with respect to each species' survival
+    matrices, (S, C, P, P) with rows the origin patch like ``surv``."""
+    m = _Marshal(pr)
+    T, N, S, C_, P = pr.n_years, pr.n_paths, len(pr.species), len(pr.flow), pr.n_patches
+    pol = np.ascontiguousarray(policy, dtype=np.int32)
+    if pol.shape != (T, N):
+        raise ValueError("policy must be (n_years, n_paths)")
+    value = np.zeros(1)
+    g = np.zeros((S, C_, P, P)) if grad else None
+    nat.check(nat.lib().roadmc_rovs_smooth_grad(
+        ctx.handle, C.byref(m.problem), pol.ctypes.data_as(nat._ip), float(tau),
+        value.ctypes.data_as(nat._dp), g.ctypes.data_as(nat._dp) if grad else None), ctx.handle)
+    # column-major [j][k] per matrix -> rows = origin k
+    return float(value[0]), (g.transpose(0, 1, 3, 2).copy() if grad else None)
+
+
 def synthetic_problem(seed: int = 5, n_patches: int = 64, n_species: int = 1,
                       n_controls: int = 3, n_paths: int = 10000, n_years: int = 30,
                       dim_res: int = 12, threshold_ratio: float = 0.965) -> RovsProblem:
