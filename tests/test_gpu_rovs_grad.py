@@ -62,6 +62,31 @@ def test_smooth_gradient_under_the_fitted_policy(ctx, oracle):
     _check(val, g, ref_val, ref_g)
 
 
+def test_smooth_gradient_against_its_central_differences(ctx):
+    # Beyond the oracle's reach (P = 128, 4000 paths, 20 years): the GPU
+    # gradient's largest entries against central differences of the GPU
+    # value (h = 1e-2 on the survival entry; FP32 populations leave about
+    # 1e-6 relative noise in the value).
+    import copy
+    pr = _problem(128, 1, 3, 4000, 20)
+    pol = np.random.default_rng(9).integers(0, 3, size=(20, 4000)).astype(np.int32)
+    tau = 0.05 * pr.species[0].params[3]
+    print("smooth gradient P=128 N=4000 T=20 against central differences", flush=True)
+    _, g = rs.smooth_value_gradient(ctx, pr, pol, tau)
+    h = 1e-2
+    for flat in np.argsort(np.abs(g).ravel())[-3:]:
+        s_, c, k, j = np.unravel_index(flat, g.shape)
+        p1, p2 = copy.deepcopy(pr), copy.deepcopy(pr)
+        p1.species[s_].surv[c, k, j] += h
+        p2.species[s_].surv[c, k, j] -= h
+        v1, _ = rs.smooth_value_gradient(ctx, p1, pol, tau, grad=False)
+        v2, _ = rs.smooth_value_gradient(ctx, p2, pol, tau, grad=False)
+        fd = (v1 - v2) / (2 * h)
+        print(f"  entry {(s_, c, k, j)}: gradient {g[s_, c, k, j]:.6f}, central difference {fd:.6f}",
+              flush=True)
+        assert g[s_, c, k, j] == pytest.approx(fd, rel=2e-2)
+
+
 def test_smooth_gradient_refusals(ctx):
     pr = _problem(24, 1, 3, 100, 5)
     pol = np.zeros((5, 100), dtype=np.int32)

@@ -176,6 +176,31 @@ def knn(n: int, dims: int, res: int, repeat: int) -> None:
     ctx.close()
 
 
+def rovs_grad(patches: int, paths: int, years: int, repeat: int, tau_frac: float) -> None:
+    """Smoothed-validity gradient of the species-state valuation (1 species,
+    3 controls, thresholds at 0.7 of the initial total) under ALGO4's fitted
+    policy, tau = tau_frac x the threshold (kernel trace target)."""
+    import os
+    import time
+
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from transportoptimiser_amd import rov_species as rs
+    from transportoptimiser_amd.api import Context
+    ctx = Context(0)
+    pr = rs.synthetic_problem(seed=5, n_patches=patches, n_species=1, n_controls=3,
+                              n_paths=paths, n_years=years, dim_res=20, threshold_ratio=0.7)
+    pol = rs.simulate_rov_species(ctx, pr, paths=True)["opt_control"]
+    tau = tau_frac * pr.species[0].params[3]
+    for i in range(repeat):
+        t = time.perf_counter()
+        v, g = rs.smooth_value_gradient(ctx, pr, pol, tau)
+        print(f"rovs grad call {i}: P={patches} C=3 N={paths} T={years} tau={tau_frac} x threshold: "
+              f"{(time.perf_counter() - t) * 1e3:.2f} ms, value {v:.4f}, "
+              f"|dV/dsurv| {np.abs(g).sum():.6e}", flush=True)
+    ctx.close()
+
+
 def rovs(patches: int, species: int, controls: int, paths: int, years: int, res: int,
          repeat: int, method: int = 4, precision: str = "fp32") -> None:
     """Species-state ROV (ALGO4 or ALGO6) on the synthetic landscape, thresholds at 0.7
@@ -325,6 +350,12 @@ def main() -> int:
     rv2.add_argument("--repeat", type=int, default=3)
     rv2.add_argument("--method", type=int, default=4, choices=(4, 6))
     rv2.add_argument("--precision", choices=["fp32", "bf16x3"], default="fp32")
+    rg = sub.add_parser("rovsgrad")
+    rg.add_argument("--patches", type=int, default=256)
+    rg.add_argument("--paths", type=int, default=100000)
+    rg.add_argument("--years", type=int, default=50)
+    rg.add_argument("--repeat", type=int, default=3)
+    rg.add_argument("--tau", type=float, default=0.05)
     wd = sub.add_parser("wide")
     wd.add_argument("--patches", type=int, default=2048)
     wd.add_argument("--paths", type=int, default=20000)
@@ -352,6 +383,8 @@ def main() -> int:
     elif a.cmd == "rovs":
         rovs(a.patches, a.species, a.controls, a.paths, a.years, a.res, a.repeat, a.method,
              a.precision)
+    elif a.cmd == "rovsgrad":
+        rovs_grad(a.patches, a.paths, a.years, a.repeat, a.tau)
     elif a.cmd == "knn":
         knn(a.samples, a.dims, a.res, a.repeat)
     elif a.cmd == "road":
