@@ -242,6 +242,20 @@ def run_extras(ctx, roads, prep_ms) -> dict:
         "what": "species-state ROV (ALGO4): 1 species x 256 patches, 3 flow controls, "
                 "100,000 paths, 50 years, 20 x 20 kNN grids; host call"}
     log(f"rov species: {out['rov_species_algo4']}")
+    # Its mortality gradient with validity smoothed (tau = 0.05 x threshold)
+    # under ALGO4's fitted policy.
+    pol5 = rsp.simulate_rov_species(ctx, pr5, paths=True)["opt_control"]
+    tau5 = 0.05 * pr5.species[0].params[3]
+    rsp.smooth_value_gradient(ctx, pr5, pol5, tau5)
+    t = time.perf_counter()
+    v5, g5 = rsp.smooth_value_gradient(ctx, pr5, pol5, tau5)
+    ms = (time.perf_counter() - t) * 1e3
+    out["rov_species_smooth_grad"] = {
+        "ms": round(ms, 2), "value": round(v5, 4), "grad_abs_sum": float(np.abs(g5).sum()),
+        "what": "smoothed-validity value of the same problem under ALGO4's fitted policy "
+                "and its gradient w.r.t. every control's survival matrix (tau = 0.05 x "
+                "threshold); host call"}
+    log(f"rov species smoothed gradient: {out['rov_species_smooth_grad']}")
     pr5.method = 6
     rsp.simulate_rov_species(ctx, pr5)
     t = time.perf_counter()
