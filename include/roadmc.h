@@ -518,8 +518,8 @@ int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
  * grad_surv [S][C][P*P] (column-major like surv, or NULL for the value only)
  * receives dvalue/dsurv; the crossing mortality m = 1 - surv has the opposite
  * gradient.  FP32 products, P <= 512 (ROADMC_ERR_UNSUPPORTED otherwise), tau >
- * 0, every policy entry in [0, n_controls); problem->method and dim_res are
- * not used. */
+ * 0, n_years x n_paths <= 2^30, every policy entry in [0, n_controls);
+ * problem->method and dim_res are not used. */
 int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_problem* problem,
                             const int32_t* policy, double tau, double* value,
                             double* grad_surv);

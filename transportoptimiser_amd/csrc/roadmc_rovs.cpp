@@ -589,6 +589,8 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
     return rmc_fail(ctx, ROADMC_ERR_UNSUPPORTED, "the smoothed gradient runs FP32 products");
   const int P = pr->n_patches, S = pr->n_species, C = pr->n_controls, T = pr->n_years;
   const int N = (int)pr->n_paths, U = pr->n_fuels + pr->n_commodities;
+  // The contraction counts its (year, path) rows in 32-bit integers.
+  if ((long long)T * N > (1LL << 30)) return bad("n_years x n_paths must be <= 2^30");
   for (size_t i = 0; i < (size_t)T * N; ++i)
     if (policy[i] < 0 || policy[i] >= C) return bad("policy controls must be in [0, n_controls)");
   RMC_HIP(ctx, hipSetDevice(ctx->device));
