@@ -87,6 +87,46 @@ def test_smooth_gradient_against_its_central_differences(ctx):
         assert g[s_, c, k, j] == pytest.approx(fd, rel=2e-2)
 
 
+def test_config4_objective_has_a_rov_gradient(ctx):
+    # Config 4's objective (mean end population + the smoothed species-state
+    # value) on one landscape: the ROV term's gradient with respect to the
+    # crossing mortality m (surv_c = 1 - m q_c / q_max) is non-zero and
+    # matches central differences of the GPU value under m -> m +- h.
+    import copy
+    from transportoptimiser_amd import adjoint as adj
+    from transportoptimiser_amd import landscape
+    rng = np.random.default_rng(21)
+    sd = landscape.make_species(rng, 64, mort_max=0.4)
+    mte = landscape.RoadProblem([sd], n_years=20, n_paths=2000, seed=4)
+    rp = rs.synthetic_problem(seed=6, n_patches=64, n_species=1, n_controls=3, n_paths=2000,
+                              n_years=10)
+    params = rp.species[0].params.copy()
+    params[3] = 0.97 * sd.init_pop.sum()
+    rp.species = [adj.rov_species_on(sd, rp.flow, params)]
+    pol = np.random.default_rng(2).integers(0, 3, size=(10, 2000)).astype(np.int32)
+    tau = 0.05 * params[3]
+    print("config 4 objective with the smoothed ROV term", flush=True)
+    J, gm, gr = adj.objective_gradient(ctx, mte, rp, pol, tau, rov_weight=0.5)
+    J0, gm0, _ = adj.mte_gradient(ctx, mte, 10)
+    g_rov = (gm - gm0) / 0.5
+    v, _ = rs.smooth_value_gradient(ctx, rp, pol, tau, grad=False)
+    assert J == pytest.approx(J0 + 0.5 * v, rel=1e-12)
+    assert np.abs(g_rov).max() > 0
+    h = 1e-3
+    for flat in np.argsort(np.abs(g_rov).ravel())[-3:]:
+        k, j = np.unravel_index(flat, g_rov.shape)
+        vals = []
+        for dm in (h, -h):
+            sd2 = copy.deepcopy(sd)
+            sd2.surv[k, j] -= dm  # m + dm
+            rp2 = copy.deepcopy(rp)
+            rp2.species = [adj.rov_species_on(sd2, rp.flow, params)]
+            vals.append(rs.smooth_value_gradient(ctx, rp2, pol, tau, grad=False)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print(f"  m[{k}, {j}]: dV/dm {g_rov[k, j]:.6f}, central difference {fd:.6f}", flush=True)
+        assert g_rov[k, j] == pytest.approx(fd, rel=2e-2)
+
+
 def test_smooth_gradient_refusals(ctx):
     pr = _problem(24, 1, 3, 100, 5)
     pol = np.zeros((5, 100), dtype=np.int32)

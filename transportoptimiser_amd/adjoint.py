@@ -1,5 +1,7 @@
 """Adjoint of the MTE simulation: gradient of the mean total end population
-with respect to crossing mortality and growth rates (roadmc_mte_adjoint_*)."""
+with respect to crossing mortality and growth rates (roadmc_mte_adjoint_*),
+and config 4's objective with the smoothed species-state value added
+(roadmc_rovs_smooth_grad)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -57,3 +59,42 @@ def mte_gradient(ctx, problem: RoadProblem, checkpoint_every: int = 10):
     with Adjoint(ctx, problem, checkpoint_every) as a:
         gm, gr = a.backward(1.0)
         return a.objective, gm, gr
+
+
+def rov_species_on(sd, flow, params):
+    """The species-state valuation's species on an MTE species' landscape: the
+    same transitions, capacities and initial populations; under flow level q_c
+    the crossing mortality m = 1 - sd.surv scales with q_c / max q
+    (surv_c = 1 - m q_c / q_max, the rule of rov_species.synthetic_problem).
+    ``params``: the valuation's 8 species parameters (growth current, mean,
+    noise SD, threshold, reversion, jump size, jump probability, local
+    variability)."""
+    from .rov_species import RovsSpecies
+    q = np.asarray(flow, dtype=np.float64)
+    q = q / q.max()
+    m = 1.0 - np.asarray(sd.surv, dtype=np.float64)
+    surv = np.stack([1.0 - m * qc for qc in q])
+    return RovsSpecies(np.asarray(sd.init_pop, dtype=np.float64),
+                       np.asarray(sd.capacity, dtype=np.float64),
+                       np.asarray(sd.trans, dtype=np.float64), surv,
+                       np.asarray(params, dtype=np.float64))
+
+
+def objective_gradient(ctx, problem: RoadProblem, rov_problem, policy, tau: float,
+                       rov_weight: float = 1.0, checkpoint_every: int = 10):
+    """Config 4's objective with a non-zero ROV term:
+    J = mean total end population of ``problem`` (one species) + rov_weight x
+    the smoothed-validity species-state value of ``rov_problem`` under
+    ``policy`` [T][N] (validity prod_s sigmoid((A_s - threshold_s) / tau)),
+    whose species 0 is ``rov_species_on(problem.species[0], ...)``.  Returns
+    (J, dJ/dm (P, P), dJ/dr (P,)) with m the crossing mortality of the MTE
+    species (surv = 1 - m, rows the origin): the MTE adjoint plus rov_weight x
+    dV/dm = -sum_c (q_c / q_max) dV/dsurv_c."""
+    from .rov_species import smooth_value_gradient
+    obj, gm, gr = mte_gradient(ctx, problem, checkpoint_every)
+    v, gs = smooth_value_gradient(ctx, rov_problem, policy, tau)
+    q = np.asarray(rov_problem.flow, dtype=np.float64)
+    q = q / q.max()
+    g_rov = -np.tensordot(q, gs[0], axes=(0, 0))
+    return obj + rov_weight * v, gm + rov_weight * g_rov, gr
+
