@@ -85,6 +85,30 @@ def test_surrogate_gain_over_full_monte_carlo(ctx):
     print(f"reference kNN grid surrogate: mean |dMTE| {knn_err[:, 0].mean():.4f}, mean |dROV|/1000 "
           f"{knn_err[:, 1].mean():.4f}; MLP (no fallback): mean |dMTE| {mlp_err[:, 0].mean():.4f}, "
           f"mean |dROV|/1000 {mlp_err[:, 1].mean():.4f}", flush=True)
+    # The gain at equal error: Monte Carlo on the first n paths of each road
+    # (the same streams), the valuation on proportionally fewer paths; the
+    # smallest n whose mean |dMTE| against the full run is within the
+    # surrogate's sets the equal-error cost.
+    import dataclasses
+    sur_err = err[:, 0].mean()
+    n_full = test[0].problem.n_paths
+    equal = None
+    for n in (6000, 4000, 3000, 2000, 1000):
+        sub = [dataclasses.replace(r, problem=dataclasses.replace(r.problem, n_paths=n))
+               for r in test]
+        sg.simulate(ctx, sub[:8], rov_paths=100000 * n // n_full)
+        t = time.perf_counter()
+        Y_n = sg.simulate(ctx, sub, rov_paths=100000 * n // n_full)
+        t_n = time.perf_counter() - t
+        e_n = np.abs(Y_n[:, 0] - Y_full[:, 0]).mean()
+        print(f"MC on {n} paths: {t_n:.3f}s, mean |dMTE| {e_n:.4f} against {n_full} paths",
+              flush=True)
+        if e_n <= sur_err:
+            equal = (n, t_n)
+    assert equal is not None
+    print(f"equal error ({sur_err:.4f}): MC on {equal[0]} paths {equal[1]:.3f}s, surrogate "
+          f"{t_sur:.3f}s ({equal[1] / t_sur:.1f}x)", flush=True)
+    assert equal[1] / t_sur > 2.0
     assert mlp_err[:, 0].mean() < knn_err[:, 0].mean()
     assert np.all(err[fall] == 0.0)  # fallback roads are the full MC values
     assert fall.mean() < 0.3
