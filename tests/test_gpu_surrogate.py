@@ -115,3 +115,37 @@ def test_surrogate_gain_over_full_monte_carlo(ctx):
     assert err[kept, 0].mean() < 0.01
     assert rel_rov.mean() < 0.05
     assert t_full / t_sur > 3.0
+
+
+def test_surrogate_on_species_state_values(ctx):
+    # The harder ROV target: the reference's species-state valuation (ALGO4,
+    # 10,000 paths) of each road's own species, which depends on its
+    # mortality (the validity thresholds) as well as its unit cost.  The
+    # ensemble against the reference's own kNN grid surrogate over (loss
+    # fraction, unit cost) on 256 fresh roads.
+    from transportoptimiser_amd import knn
+    from transportoptimiser_amd import native as nat
+    t = time.perf_counter()
+    train = sg.make_roads(2048, seed=13)
+    Y = sg.simulate(ctx, train, rov_paths=10000, rov_model="species")
+    t_data = time.perf_counter() - t
+    X = np.stack([r.features for r in train])
+    sur = sg.Surrogate(ctx, members=4, seed=0, max_spread=0.01)
+    loss = sur.fit(X, Y, steps=3000, lr=3e-3)
+    test = sg.make_roads(256, seed=14)
+    Y_full = sg.simulate(ctx, test, rov_paths=10000, rov_model="species")
+    Xt = np.stack([r.features for r in test])
+    mlp = np.abs(sur.predict(Xt)[0] - Y_full)
+    g_rov = knn.fit(ctx, X[:, [1, 7]], Y[:, 1], 20,
+                    knn.neighbour_count(len(X), nat.KNN_RULE_ROV), normalize=True,
+                    nonpositive=True)
+    knn_rov = np.abs(g_rov.predict(ctx, Xt[:, [1, 7]]) - Y_full[:, 1])
+    rel = mlp[:, 1] / np.abs(Y_full[:, 1])
+    print(f"species-state ROV targets: data {t_data:.1f}s, train loss {loss.mean():.4f}; "
+          f"values {Y_full[:, 1].min():.3f} .. {Y_full[:, 1].max():.3f} (x 1000); MLP mean |dROV| "
+          f"{mlp[:, 1].mean():.4f} (relative {rel.mean():.3f}), kNN grid {knn_rov.mean():.4f}; "
+          f"MLP mean |dMTE| {mlp[:, 0].mean():.4f}", flush=True)
+    assert np.all(Y_full[:, 1] <= 0.0)
+    assert mlp[:, 1].mean() < knn_rov.mean()
+    assert mlp[:, 0].mean() < 0.01
+

@@ -62,22 +62,10 @@ def mte_gradient(ctx, problem: RoadProblem, checkpoint_every: int = 10):
 
 
 def rov_species_on(sd, flow, params):
-    """The species-state valuation's species on an MTE species' landscape: the
-    same transitions, capacities and initial populations; under flow level q_c
-    the crossing mortality m = 1 - sd.surv scales with q_c / max q
-    (surv_c = 1 - m q_c / q_max, the rule of rov_species.synthetic_problem).
-    ``params``: the valuation's 8 species parameters (growth current, mean,
-    noise SD, threshold, reversion, jump size, jump probability, local
-    variability)."""
-    from .rov_species import RovsSpecies
-    q = np.asarray(flow, dtype=np.float64)
-    q = q / q.max()
-    m = 1.0 - np.asarray(sd.surv, dtype=np.float64)
-    surv = np.stack([1.0 - m * qc for qc in q])
-    return RovsSpecies(np.asarray(sd.init_pop, dtype=np.float64),
-                       np.asarray(sd.capacity, dtype=np.float64),
-                       np.asarray(sd.trans, dtype=np.float64), surv,
-                       np.asarray(params, dtype=np.float64))
+    """The species-state valuation's species on an MTE species' landscape
+    (rov_species.species_on)."""
+    from .rov_species import species_on
+    return species_on(sd, flow, params)
 
 
 def objective_gradient(ctx, problem: RoadProblem, rov_problem, policy, tau: float,

@@ -152,6 +152,24 @@ def simulate_rov_species(ctx, pr: RovsProblem, *, paths: bool = False,
     return out
 
 
+def species_on(sd, flow, params) -> RovsSpecies:
+    """The valuation's species on an MTE species' landscape (a
+    landscape.SpeciesData): the same transitions, capacities and initial
+    populations; under flow level q_c the crossing mortality m = 1 - sd.surv
+    scales with q_c / max q (surv_c = 1 - m q_c / q_max, the rule of
+    synthetic_problem).  ``params``: the 8 species parameters (growth current,
+    mean, noise SD, threshold, reversion, jump size, jump probability, local
+    variability)."""
+    q = np.asarray(flow, dtype=np.float64)
+    q = q / q.max()
+    m = 1.0 - np.asarray(sd.surv, dtype=np.float64)
+    surv = np.stack([1.0 - m * qc for qc in q])
+    return RovsSpecies(np.asarray(sd.init_pop, dtype=np.float64),
+                       np.asarray(sd.capacity, dtype=np.float64),
+                       np.asarray(sd.trans, dtype=np.float64), surv,
+                       np.asarray(params, dtype=np.float64))
+
+
 def smooth_value_gradient(ctx, pr: RovsProblem, policy, tau: float, *, grad: bool = True):
     """``roadmc_rovs_smooth_grad``: the smoothed-validity value under the
     policy [T][N] (validity prod_s sigmoid((A_s - threshold_s) / tau)) and,

@@ -6,7 +6,11 @@ include/main/RoadGA.cpp:1553-2038; fit in SimulateGPU.cu:5533-5968).  Here the
 surrogate is an ensemble of MLPs (8 road/landscape features -> 64 -> 64 ->
 (MTE, ROV), tanh) trained on the GPU by libroadmc's kernels on roads that the
 simulator evaluated; a road whose ensemble members disagree on MTE by more
-than `max_spread` is simulated in full instead.
+than `max_spread` is simulated in full instead.  The ROV target is the
+config-5 price-state valuation of the road's unit cost, or (rov_model
+"species") the reference's species-state valuation (ALGO4) of the road's own
+species, which depends on the road's mortality as well as its unit cost, as
+the reference's ROV surrogate over (adjusted populations, unit profit) does.
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ import numpy as np
 from . import native as nat
 from .landscape import RoadProblem, make_species
 from .rov import RovProblem, config5, simulate_rov
+from .rov_species import Price, RovsProblem, simulate_rov_species, species_on
 
 N_FEATURES = 8
 N_OUTPUTS = 2
@@ -177,14 +182,50 @@ def rov_problem(unit_cost: float, n_paths: int, n_steps: int, seed: int) -> RovP
     return pr
 
 
-def simulate(ctx, roads, rov_paths: int = 100000, rov_steps: int = 50) -> np.ndarray:
-    """Full Monte Carlo targets (MTE of species 0, ROV / 1000) for roads; the
-    valuation at config 5's size (100,000 paths, 50 steps)."""
+# Validity threshold of the species-state valuation, as a fraction of the
+# initial total: the highest flow is valid on 15-85% of the forward
+# path-years across the study's roads.
+ROV_THRESHOLD_RATIO = 0.85
+
+
+def species_rov_problem(sd, unit_cost: float, n_paths: int, n_years: int, seed: int,
+                        road_id: int = 0) -> RovsProblem:
+    """The species-state valuation (the reference's ALGO4) of a road: its
+    species on the road's landscape (species_on: flows 0, 50, 100, crossing
+    mortality scaling with the flow), growth 5% a year with local variability
+    1, the validity threshold at ROV_THRESHOLD_RATIO of the initial total; one
+    fuel and one ore price (the prices of rov_species.synthetic_problem); unit
+    cost 0.6 + 2.5 x the road's unit cost, unit revenue 2."""
+    flow = np.array([0.0, 50.0, 100.0])
+    params = np.array([5.0, 5.0, 0.1, ROV_THRESHOLD_RATIO * float(np.sum(sd.init_pop)), 0.5, 0.1,
+                       0.05, 1.0])
+    return RovsProblem(
+        species=[species_on(sd, flow, params)], flow=flow,
+        fuels=[Price(1.5, 1.5, 0.05, 0.3, 0.1, 0.02)],
+        commodities=[Price(2.0, 2.0, 0.08, 0.3, 0.1, 0.02)],
+        ore_mean=np.array([0.5]), ore_sd=np.array([0.05]),
+        fuel_cost=np.array([0.2]), fuel_idx=np.array([0], dtype=np.int32),
+        unit_cost=0.6 + 2.5 * unit_cost, unit_revenue=2.0, n_years=n_years, n_paths=n_paths,
+        dim_res=12, seed=seed, road_id=road_id)
+
+
+def simulate(ctx, roads, rov_paths: int = 100000, rov_steps: int = 50,
+             rov_model: str = "price") -> np.ndarray:
+    """Full Monte Carlo targets (MTE of species 0, ROV / 1000) for roads.  The
+    ROV is the config-5 price-state valuation of the road's unit cost
+    (rov_model "price", rov_problem; config 5's 100,000 paths by default) or
+    the species-state valuation of the road's species ("species",
+    species_rov_problem)."""
     outs = ctx.simulate_mte_batch([r.problem for r in roads], end_pops=False, patch_mean=False)
     y = np.zeros((len(roads), 2))
     for i, (r, o) in enumerate(zip(roads, outs)):
-        v = simulate_rov(ctx, rov_problem(r.unit_cost, rov_paths, rov_steps,
-                                          seed=1000 + r.problem.road_id))
+        seed = 1000 + r.problem.road_id
+        if rov_model == "species":
+            v = simulate_rov_species(ctx, species_rov_problem(
+                r.problem.species[0], r.unit_cost, rov_paths, rov_steps, seed,
+                r.problem.road_id))
+        else:
+            v = simulate_rov(ctx, rov_problem(r.unit_cost, rov_paths, rov_steps, seed=seed))
         y[i] = (o.mte_prob[0], v["value"] / ROV_SCALE)
     return y
 
