@@ -60,3 +60,21 @@ def test_parallel_generation_equals_serial():
     b = ls.config3(seed=2, n_roads=6, n_paths=10, workers=3)
     for x, y in zip(a, b):
         assert np.array_equal(x.species[1].surv, y.species[1].surv)
+
+
+def test_valuation_species_on_a_landscape():
+    # The species-state valuation's species built from an MTE species: the
+    # same transitions, capacities and initial populations; crossing
+    # mortality scaled by the flow level (none at zero flow, all of it at the
+    # highest).
+    from transportoptimiser_amd import rov_species as rs
+    sd = ls.make_species(np.random.default_rng(3), 40, mort_max=0.4)
+    flow = np.array([0.0, 25.0, 100.0])
+    params = np.arange(8.0)
+    v = rs.species_on(sd, flow, params)
+    assert v.surv.shape == (3, 40, 40)
+    assert np.all(v.surv[0] == 1.0)
+    np.testing.assert_allclose(v.surv[2], sd.surv, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(1.0 - v.surv[1], 0.25 * (1.0 - sd.surv), rtol=1e-12, atol=1e-15)
+    assert v.trans is not sd.trans and np.array_equal(v.trans, sd.trans)
+    assert np.array_equal(v.params, params)
