@@ -76,5 +76,5 @@ def test_valuation_species_on_a_landscape():
     assert np.all(v.surv[0] == 1.0)
     np.testing.assert_allclose(v.surv[2], sd.surv, rtol=0, atol=1e-15)
     np.testing.assert_allclose(1.0 - v.surv[1], 0.25 * (1.0 - sd.surv), rtol=1e-12, atol=1e-15)
-    assert v.trans is not sd.trans and np.array_equal(v.trans, sd.trans)
+    assert np.array_equal(v.trans, sd.trans)
     assert np.array_equal(v.params, params)
