@@ -45,9 +45,12 @@ def test_smooth_gradient_matches_oracle(ctx, oracle, P, S, C, N, T):
     val, g = rs.smooth_value_gradient(ctx, pr, pol, tau)
     ref_val, ref_g = oracle.rovs_smooth_grad(pr, pol, tau)
     _check(val, g, ref_val, ref_g)
-    # the value alone: the same number
+    # the value alone: the same number; a second call: the same bits (every
+    # sum in a fixed order)
     v2, none = rs.smooth_value_gradient(ctx, pr, pol, tau, grad=False)
     assert none is None and v2 == val
+    v3, g3 = rs.smooth_value_gradient(ctx, pr, pol, tau)
+    assert v3 == val and np.array_equal(g3, g)
 
 
 def test_smooth_gradient_under_the_fitted_policy(ctx, oracle):
