@@ -29,9 +29,11 @@ size_t adj_lds_bytes(bool fp64, int BM, int Pp);
 hipError_t adj_set_lds_limit(bool fp64, int BM, int Pp, size_t bytes);
 hipError_t launch_adj_segment(bool fp64, const AdjJob& jb, int n_tiles, size_t lds,
                               hipStream_t s);
-// grad += sum_k A[k]^T B[k] for K rows of Pp (Pp a multiple of 64).
+// grad += sum_k A[r_k]^T B[r_k] for K rows of Pp (Pp a multiple of 64), r_k = k,
+// or rows[k] when rows is given (a list of K row indices).
 hipError_t launch_adj_gradm(bool fp64, const void* A, const void* B, int Ktot, int Pp,
-                            int nsplit, double* partial, double* grad, hipStream_t s);
+                            int nsplit, double* partial, double* grad, hipStream_t s,
+                            const int* rows = nullptr);
 
 }  // namespace roadmc
 

@@ -337,7 +337,8 @@ constexpr int kGmBlock = 128, kGmSlab = 16;
 __global__ __launch_bounds__(256) void adj_gradm_f32_kernel(const float* __restrict__ A,
                                                             const float* __restrict__ B,
                                                             int Ktot, int Pp, int kchunk,
-                                                            double* __restrict__ partial) {
+                                                            double* __restrict__ partial,
+                                                            const int* __restrict__ rows) {
   typedef float floatx4 __attribute__((ext_vector_type(4)));
   typedef float floatx16 __attribute__((ext_vector_type(16)));
   __shared__ __attribute__((aligned(16))) float sa[2][kGmSlab][kGmBlock];
@@ -362,8 +363,9 @@ __global__ __launch_bounds__(256) void adj_gradm_f32_kernel(const float* __restr
       const int r = v >> 5, c4 = (v & 31) * 4;
       const int k = k0 + r;
       const bool ok = k < ke;
-      ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + (size_t)k * Pp + i0 + c4) : floatx4{};
-      rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + (size_t)k * Pp + j0 + c4) : floatx4{};
+      const size_t row = ok ? (rows ? (size_t)rows[k] : (size_t)k) : 0;
+      ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + row * Pp + i0 + c4) : floatx4{};
+      rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + row * Pp + j0 + c4) : floatx4{};
     }
   };
   auto store_slab = [&](int buf, const floatx4 (&ra)[2], const floatx4 (&rb)[2]) {
@@ -420,7 +422,8 @@ template <class TR>
 __global__ __launch_bounds__(256) void adj_gradm_kernel(const typename TR::T* __restrict__ A,
                                                         const typename TR::T* __restrict__ B,
                                                         int Ktot, int Pp, int kchunk,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial,
+                                                        const int* __restrict__ rows) {
   using T = typename TR::T;
   using AccT = typename TR::AccT;
   constexpr int TM = TR::TM, NACC = TR::NACC;
@@ -445,10 +448,11 @@ __global__ __launch_bounds__(256) void adj_gradm_kernel(const typename TR::T* __
   for (int k = kb; k < ke; k += KS) {
     const int kk = k + kl;
     const bool ok = kk < ke;
+    const size_t row = ok ? (rows ? (size_t)rows[kk] : (size_t)kk) : 0;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-      const T a = ok ? A[(size_t)kk * Pp + ti[t] + al] : (T)0;
-      const T b = ok ? B[(size_t)kk * Pp + tj[t] + al] : (T)0;
+      const T a = ok ? A[row * Pp + ti[t] + al] : (T)0;
+      const T b = ok ? B[row * Pp + tj[t] + al] : (T)0;
       acc[t] = TR::mfma(a, b, acc[t]);
     }
   }
@@ -515,25 +519,26 @@ hipError_t launch_adj_segment(bool fp64, const AdjJob& jb, int n_tiles, size_t l
 }
 
 hipError_t launch_adj_gradm(bool fp64, const void* A, const void* B, int Ktot, int Pp,
-                            int nsplit, double* partial, double* grad, hipStream_t s) {
+                            int nsplit, double* partial, double* grad, hipStream_t s,
+                            const int* rows) {
   if (fp64) {
     const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
     const int nb = Pp / 64;
     hipLaunchKernelGGL(adj_gradm_kernel<TraitsF64>, dim3(nb * nb * nsplit), dim3(256), 0, s,
                        static_cast<const double*>(A), static_cast<const double*>(B), Ktot, Pp,
-                       kchunk, partial);
+                       kchunk, partial, rows);
   } else if (Pp % kGmBlock == 0) {
     const int kchunk = ((Ktot + nsplit - 1) / nsplit + kGmSlab - 1) / kGmSlab * kGmSlab;
     const int nb = Pp / kGmBlock;
     hipLaunchKernelGGL(adj_gradm_f32_kernel, dim3(nb * nb * nsplit), dim3(256), 0, s,
                        static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
-                       kchunk, partial);
+                       kchunk, partial, rows);
   } else {
     const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
     const int nb = Pp / 64;
     hipLaunchKernelGGL(adj_gradm_kernel<TraitsF32>, dim3(nb * nb * nsplit), dim3(256), 0, s,
                        static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
-                       kchunk, partial);
+                       kchunk, partial, rows);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -114,7 +114,8 @@ extern "C" void roadmc_ctx_destroy(roadmc_ctx* ctx) {
                         &ctx->rovs_grad_bufs.mtpack, &ctx->rovs_grad_bufs.policy,
                         &ctx->rovs_grad_bufs.coef, &ctx->rovs_grad_bufs.w,
                         &ctx->rovs_grad_bufs.gpart, &ctx->rovs_grad_bufs.grad,
-                        &ctx->rovs_grad_bufs.jpart};
+                        &ctx->rovs_grad_bufs.jpart, &ctx->rovs_grad_bufs.rows,
+                        &ctx->rovs_grad_bufs.blk};
   for (rmc_devbuf* b : bufs) rmc_buf_free(*b);
   for (auto& l : ctx->road_lanes) {
     for (rmc_devbuf* b : {&l.cross, &l.trans, &l.surv, &l.blockmax, &l.part}) rmc_buf_free(*b);

@@ -83,11 +83,11 @@ struct roadmc_ctx {
         u, small, grid, args, ckpt, grm, rvec, grids;
   } rovs_bufs;
   // The smoothed-validity gradient of the species-state valuation: the
-  // transposed movement matrices, the policy, dJ/dA coefficients, the
-  // per-control w planes, the contraction's partials and result, the
-  // per-block values.
+  // transposed movement matrices, the policy, dJ/dA coefficients, the w
+  // plane, the contraction's partials and result, the per-block values, the
+  // rows of each control and the counts that place them.
   struct {
-    rmc_devbuf mtpack, policy, coef, w, gpart, grad, jpart;
+    rmc_devbuf mtpack, policy, coef, w, gpart, grad, jpart, rows, blk;
   } rovs_grad_bufs;
 };
 

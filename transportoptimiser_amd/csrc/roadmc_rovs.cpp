@@ -619,7 +619,9 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
       {&B.grm, want_grad ? (size_t)S * T * N * sizeof(float) : 8},
       {&G.coef, (size_t)S * T * N * sizeof(float)},
       {&G.jpart, (size_t)nblk * sizeof(double)},
-      {&G.w, want_grad ? (size_t)C * T * N * Pp * sizeof(float) : 8},
+      {&G.w, want_grad ? (size_t)T * N * Pp * sizeof(float) : 8},
+      {&G.rows, want_grad ? (size_t)T * N * sizeof(int) : 8},
+      {&G.blk, want_grad ? ((size_t)2 * ((T * N + 1023) / 1024) * C + 8) * sizeof(int) : 8},
       {&G.gpart, want_grad ? (size_t)nsplit * Pp * Pp * sizeof(double) : 8},
       {&G.grad, want_grad ? (size_t)Pp * Pp * sizeof(double) : 8},
   };
@@ -771,6 +773,27 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
   *value = jsum / (double)N;
   if (!want_grad) return ROADMC_OK;
 
+  // The (year, path) rows of each control, in order: the contraction of
+  // control c runs over its own rows only.
+  const int nrows = T * N, nblk_rows = (nrows + 1023) / 1024;
+  int* blk_count = static_cast<int*>(G.blk.ptr);
+  int* blk_off = blk_count + (size_t)nblk_rows * C;
+  int* d_total = blk_off + (size_t)nblk_rows * C;
+  RMC_HIP(ctx, roadmc::launch_ctl_rows(static_cast<const int*>(G.policy.ptr), nrows, C, blk_count,
+                                       blk_off, d_total, nullptr, nullptr, s, false));
+  std::vector<int> total(C), first(C);
+  RMC_HIP(ctx, hipMemcpyAsync(total.data(), d_total, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  for (int c = 0, run = 0; c < C; ++c) {
+    first[c] = run;
+    run += total[c];
+  }
+  if (first[C - 1] + total[C - 1] != nrows)
+    return rmc_fail(ctx, ROADMC_ERR_HIP, "control row lists do not cover the rows");
+  RMC_HIP(ctx, roadmc::launch_ctl_rows(static_cast<const int*>(G.policy.ptr), nrows, C, blk_count,
+                                       blk_off, d_total, first.data(),
+                                       static_cast<int*>(G.rows.ptr), s, true));
+
   // Per species: the reverse sweep, then per control dJ/dM_c = sum over the
   // years and paths under c of n_t^T w_t (rows in the tile's stored order).
   std::vector<double> gm((size_t)Pp * Pp);
@@ -799,15 +822,18 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
     a.w_out = static_cast<float*>(G.w.ptr);
     RMC_HIP(ctx, roadmc::launch_rovs_adj(a, s));
     for (int c = 0; c < C; ++c) {
-      RMC_HIP(ctx, hipMemsetAsync(G.grad.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
-      RMC_HIP(ctx, roadmc::launch_adj_gradm(false, a.ckpt,
-                                            static_cast<const float*>(G.w.ptr) +
-                                                (size_t)c * T * N * Pp,
-                                            T * N, Pp, nsplit, static_cast<double*>(G.gpart.ptr),
-                                            static_cast<double*>(G.grad.ptr), s));
-      RMC_HIP(ctx, hipMemcpyAsync(gm.data(), G.grad.ptr, gm.size() * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
-      RMC_HIP(ctx, hipStreamSynchronize(s));
+      if (total[c] == 0) {  // no path ever under c: no dependence on its survival
+        std::fill(gm.begin(), gm.end(), 0.0);
+      } else {
+        RMC_HIP(ctx, hipMemsetAsync(G.grad.ptr, 0, (size_t)Pp * Pp * sizeof(double), s));
+        RMC_HIP(ctx, roadmc::launch_adj_gradm(false, a.ckpt, G.w.ptr, total[c], Pp, nsplit,
+                                              static_cast<double*>(G.gpart.ptr),
+                                              static_cast<double*>(G.grad.ptr), s,
+                                              static_cast<const int*>(G.rows.ptr) + first[c]));
+        RMC_HIP(ctx, hipMemcpyAsync(gm.data(), G.grad.ptr, gm.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+        RMC_HIP(ctx, hipStreamSynchronize(s));
+      }
       // dJ/dsurv = trans o dJ/dM / N, column-major like surv.
       double* out = grad_surv + ((size_t)sp * C + c) * P * P;
       for (int j = 0; j < P; ++j)

@@ -146,9 +146,9 @@ hipError_t launch_rovs_smooth_value(const RovsSmoothArgs& a, hipStream_t s);
 // Reverse sweep of one species' populations under the policy, years T-1 .. 0,
 // one 32-path tile per workgroup: x_t = n_t M_u (every control's product, the
 // path's selected), w_t = lambda_t+1 o G'(x_t) + coef_t on the patches (zero
-// on padding), lambda_t = w_t M_u^T; w_t goes to w_out[u] (the other
-// controls' planes get zero rows) for dJ/dM_c = sum over (t, paths with u = c)
-// of n_t^T w_t.  FP32 products on the forward pass's packed operands.
+// on padding), lambda_t = w_t M_u^T; w_t goes to w_out for dJ/dM_c = sum over
+// (t, paths with u = c) of n_t^T w_t.  FP32 products on the forward pass's
+// packed operands.
 struct RovsAdjArgs {
   int N, T, P, Pp, Pk, C;
   const float* mpack;     // [C][Pp * Pk] trans o surv_c, MFMA B order
@@ -160,10 +160,18 @@ struct RovsAdjArgs {
   const float* coef;      // [T][N] this species' dJ_p / dA_t
   float lv;               // local variability
   uint32_t tag0, key0, key1;
-  float* w_out;           // [C][T][N][Pp]
+  float* w_out;           // [T][N][Pp]
 };
 size_t rovs_adj_lds(int Pp);
 hipError_t launch_rovs_adj(const RovsAdjArgs& a, hipStream_t s);
+
+// Lists of the n rows of ctl [n] by control, stable.  First call (scatter
+// false): blk_count and blk_off [ceil(n / 1024)][C] and the totals [C].  Second
+// call (scatter true): with host_off [C] the start of each control's list in
+// rows [n], each row's index at its place.
+hipError_t launch_ctl_rows(const int* ctl, int n, int C, int* blk_count, int* blk_off,
+                           int* total, const int* host_off, int* rows, hipStream_t s,
+                           bool scatter);
 // Position of patch k within a row of the population tile.
 int rovs_tile_col(int k, bool split);
 hipError_t launch_rovs_recompute(const RovsRecomputeArgs& a, hipStream_t s);

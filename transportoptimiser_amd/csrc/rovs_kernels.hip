@@ -1020,11 +1020,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_adj_kernel(RovsAdjArgs a
               real ? lam[t][r4 + s4] * (1.f + g * (Kc - 2.f * x) * Ki * 0.01f) + cf[row] : 0.f;
           A[row * LDAe + pcol] = w;
           const int path = p0e + row;
-          if (path < N) {
-            const int u = ctl[row];
-            for (int c = 0; c < C; ++c)
-              a.w_out[(((size_t)c * a.T + year) * N + path) * Pp + col] = (c == u) ? w : 0.f;
-          }
+          if (path < N) a.w_out[((size_t)year * N + path) * Pp + col] = w;
         }
       }
     }
@@ -1049,6 +1045,100 @@ __global__ __launch_bounds__(kRovsMaxThreads) void rovs_adj_kernel(RovsAdjArgs a
     }
     __syncthreads();  // every wave has read w_t before the next year's tile
   }
+}
+
+// Stable lists of the rows of each control (rows of one control in
+// increasing order, the controls' lists at the caller's offsets): per block
+// of kCtlBlock rows the counts per control, then per control the blocks'
+// offsets (one workgroup per control: thread sums over contiguous runs of
+// blocks, then a scan of the thread sums), then each row's position from the
+// ballots of its wave and the counts of the waves before it.
+constexpr int kCtlBlock = 1024;
+
+__global__ __launch_bounds__(kCtlBlock) void ctl_count_kernel(const int* __restrict__ ctl, int n,
+                                                              int C, int* __restrict__ blk_count) {
+  __shared__ int cnt[kRovsMaxC];
+  const int tid = threadIdx.x;
+  if (tid < C) cnt[tid] = 0;
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * kCtlBlock + tid;
+  if (i < n) atomicAdd(&cnt[ctl[i]], 1);
+  __syncthreads();
+  if (tid < C) blk_count[(size_t)blockIdx.x * C + tid] = cnt[tid];
+}
+
+__global__ __launch_bounds__(1024) void ctl_scan_kernel(const int* __restrict__ blk_count,
+                                                        int nblk, int C,
+                                                        int* __restrict__ blk_off,
+                                                        int* __restrict__ total) {
+  __shared__ int part[1024];
+  const int c = blockIdx.x, tid = threadIdx.x;
+  const int per = (nblk + 1023) / 1024;
+  const int b0 = min(nblk, tid * per), b1 = min(nblk, b0 + per);
+  int sum = 0;
+  for (int b = b0; b < b1; ++b) sum += blk_count[(size_t)b * C + c];
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int b = b0; b < b1; ++b) {
+    blk_off[(size_t)b * C + c] = run;
+    run += blk_count[(size_t)b * C + c];
+  }
+  if (tid == 1023) total[c] = part[1023];
+}
+
+struct CtlOffsets {
+  int c[kRovsMaxC];
+};
+
+__global__ __launch_bounds__(kCtlBlock) void ctl_scatter_kernel(const int* __restrict__ ctl, int n,
+                                                                int C,
+                                                                const int* __restrict__ blk_off,
+                                                                CtlOffsets off,
+                                                                int* __restrict__ rows) {
+  __shared__ int wc[kCtlBlock / 64][kRovsMaxC];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long i = (long long)blockIdx.x * kCtlBlock + tid;
+  const int u = i < n ? ctl[i] : -1;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int rank = 0;
+  for (int c = 0; c < C; ++c) {
+    const unsigned long long m = __ballot(u == c);
+    if (lane == 0) wc[wave][c] = __popcll(m);
+    if (u == c) rank = __popcll(m & below);
+  }
+  __syncthreads();
+  if (u >= 0) {
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wc[w][u];
+    rows[(size_t)off.c[u] + blk_off[(size_t)blockIdx.x * C + u] + base + rank] = (int)i;
+  }
+}
+
+hipError_t launch_ctl_rows(const int* ctl, int n, int C, int* blk_count, int* blk_off,
+                           int* total, const int* host_off, int* rows, hipStream_t s,
+                           bool scatter) {
+  if (n < 1 || C < 1 || C > kRovsMaxC) return hipErrorInvalidValue;
+  const int nblk = (n + kCtlBlock - 1) / kCtlBlock;
+  if (!scatter) {
+    hipLaunchKernelGGL(ctl_count_kernel, dim3(nblk), dim3(kCtlBlock), 0, s, ctl, n, C, blk_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ctl_scan_kernel, dim3(C), dim3(1024), 0, s, blk_count, nblk, C, blk_off,
+                       total);
+    return hipGetLastError();
+  }
+  CtlOffsets off{};
+  for (int c = 0; c < C; ++c) off.c[c] = host_off[c];
+  hipLaunchKernelGGL(ctl_scatter_kernel, dim3(nblk), dim3(kCtlBlock), 0, s, ctl, n, C, blk_off,
+                     off, rows);
+  return hipGetLastError();
 }
 
 hipError_t launch_rovs_adj(const RovsAdjArgs& a, hipStream_t s) {
