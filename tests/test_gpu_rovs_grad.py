@@ -53,6 +53,19 @@ def test_smooth_gradient_matches_oracle(ctx, oracle, P, S, C, N, T):
     assert v3 == val and np.array_equal(g3, g)
 
 
+def test_smooth_gradient_with_an_unused_control(ctx, oracle):
+    # Control 1 never drawn: its survival matrix gets an exactly zero
+    # gradient (no rows to contract), the others match the oracle.
+    pr = _problem(40, 1, 3, 500, 6)
+    pol = 2 * np.random.default_rng(7).integers(0, 2, size=(6, 500)).astype(np.int32)
+    tau = 0.02 * pr.species[0].params[3]
+    print("smooth gradient with control 1 unused", flush=True)
+    val, g = rs.smooth_value_gradient(ctx, pr, pol, tau)
+    ref_val, ref_g = oracle.rovs_smooth_grad(pr, pol, tau)
+    assert np.all(g[:, 1] == 0.0) and np.all(ref_g[:, 1] == 0.0)
+    _check(val, g[:, [0, 2]], ref_val, ref_g[:, [0, 2]])
+
+
 def test_smooth_gradient_under_the_fitted_policy(ctx, oracle):
     # The natural policy: ALGO4's decisions on its own forward states.
     pr = _problem(32, 1, 3, 2000, 10)
