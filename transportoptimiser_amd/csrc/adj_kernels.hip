@@ -103,11 +103,15 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   const size_t plane = (size_t)N * Pp;  // one year of staging
 
   for (int c = tid; c < Pp; c += nthreads) gr[c] = 0.0;
-  // Checkpoint of the segment (state entering year y0).
+  // Checkpoint of the segment (state entering year y0), which is also the
+  // segment's first staged state ns[0].
   const RMC_GLOBAL T* ck = gmem(static_cast<const T*>(jb.ckpt)) + (size_t)(jb.y0 / jb.L) * plane;
   for (int idx = tid; idx < BM * Pp; idx += nthreads) {
     const int r = idx / Pp, c = idx - r * Pp;
-    A[r * LDA + TR::perm(c)] = (p0 + r < N) ? ck[(size_t)(p0 + r) * Pp + c] : (T)0;
+    const bool in = p0 + r < N;
+    const T v = in ? ck[(size_t)(p0 + r) * Pp + c] : (T)0;
+    A[r * LDA + TR::perm(c)] = v;
+    if (in) ns[(size_t)(p0 + r) * Pp + c] = v;
   }
   __syncthreads();
 
@@ -165,13 +169,11 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
   const PackedB Mp = packed_b(jb.mpack, nkg, (size_t)Pp * Pp * sizeof(T));
   const PackedB Mtp = packed_b(jb.mtpack, nkg, (size_t)Pp * Pp * sizeof(T));
 
-  // Recompute the segment: ns[i] = n_{y0+i}, np[i] = n'_{y0+i+1}.
+  // Recompute the segment: ns[i] = n_{y0+i}, np[i] = n'_{y0+i+1}; the
+  // epilogue of year yr < y1 stores the next state ns[i + 1] from registers.
   for (int yr = jb.y0; yr <= jb.y1; ++yr) {
     const int i = yr - jb.y0;
-    for (int idx = tid; idx < BM * Pp; idx += nthreads) {
-      const int r = idx / Pp, c = idx - r * Pp;
-      if (p0 + r < N) ns[i * plane + (size_t)(p0 + r) * Pp + c] = A[r * LDA + TR::perm(c)];
-    }
+    const bool stage_next = yr < jb.y1;
     product(Mp);
     __syncthreads();
     {
@@ -180,7 +182,8 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
       int lanee = lane, p0e = p0;
       const RMC_GLOBAL T* colse = cols;
       RMC_GLOBAL T* npe = np;
-      asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe));
+      RMC_GLOBAL T* nse = ns;
+      asm volatile("" : "+v"(lanee), "+s"(p0e), "+s"(colse), "+s"(npe), "+s"(nse));
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         if (t < my_n) {
@@ -195,8 +198,12 @@ __global__ __launch_bounds__(kMteThreads) void adj_segment_kernel(AdjJob jb) {
               const int row = trow[t] * TM + TR::acc_row(lanee, r4 + s);
               const uint32_t path = (uint32_t)(p0e + row);
               const T x = acc[t][r4 + s];
-              if ((int)path < N) npe[i * plane + (size_t)path * Pp + col] = x;
-              A[row * LDA + TR::perm(col)] = x * ((T)1 + rr * ((T)1 - x * Ki)) * e[s];
+              const T nv = x * ((T)1 + rr * ((T)1 - x * Ki)) * e[s];
+              if ((int)path < N) {
+                npe[i * plane + (size_t)path * Pp + col] = x;
+                if (stage_next) nse[(i + 1) * plane + (size_t)path * Pp + col] = nv;
+              }
+              A[row * LDA + TR::perm(col)] = nv;
             }
           }
         }
@@ -334,6 +341,8 @@ __global__ __launch_bounds__(64) void adj_reduce_r_kernel(const double* __restri
 // order (FP64).  FP32 only (the FP64 checker keeps adj_gradm_f64_kernel).
 constexpr int kGmBlock = 128, kGmSlab = 16;
 
+// GATHER: the k-th row is rows[k] (a list of row indices) instead of k.
+template <bool GATHER>
 __global__ __launch_bounds__(256) void adj_gradm_f32_kernel(const float* __restrict__ A,
                                                             const float* __restrict__ B,
                                                             int Ktot, int Pp, int kchunk,
@@ -363,9 +372,14 @@ __global__ __launch_bounds__(256) void adj_gradm_f32_kernel(const float* __restr
       const int r = v >> 5, c4 = (v & 31) * 4;
       const int k = k0 + r;
       const bool ok = k < ke;
-      const size_t row = ok ? (rows ? (size_t)rows[k] : (size_t)k) : 0;
-      ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + row * Pp + i0 + c4) : floatx4{};
-      rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + row * Pp + j0 + c4) : floatx4{};
+      if constexpr (GATHER) {
+        const size_t row = ok ? (size_t)rows[k] : 0;
+        ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + row * Pp + i0 + c4) : floatx4{};
+        rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + row * Pp + j0 + c4) : floatx4{};
+      } else {
+        ra[h] = ok ? *reinterpret_cast<const floatx4*>(A + (size_t)k * Pp + i0 + c4) : floatx4{};
+        rb[h] = ok ? *reinterpret_cast<const floatx4*>(B + (size_t)k * Pp + j0 + c4) : floatx4{};
+      }
     }
   };
   auto store_slab = [&](int buf, const floatx4 (&ra)[2], const floatx4 (&rb)[2]) {
@@ -530,9 +544,14 @@ hipError_t launch_adj_gradm(bool fp64, const void* A, const void* B, int Ktot, i
   } else if (Pp % kGmBlock == 0) {
     const int kchunk = ((Ktot + nsplit - 1) / nsplit + kGmSlab - 1) / kGmSlab * kGmSlab;
     const int nb = Pp / kGmBlock;
-    hipLaunchKernelGGL(adj_gradm_f32_kernel, dim3(nb * nb * nsplit), dim3(256), 0, s,
-                       static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
-                       kchunk, partial, rows);
+    if (rows)
+      hipLaunchKernelGGL(adj_gradm_f32_kernel<true>, dim3(nb * nb * nsplit), dim3(256), 0, s,
+                         static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
+                         kchunk, partial, rows);
+    else
+      hipLaunchKernelGGL(adj_gradm_f32_kernel<false>, dim3(nb * nb * nsplit), dim3(256), 0, s,
+                         static_cast<const float*>(A), static_cast<const float*>(B), Ktot, Pp,
+                         kchunk, partial, rows);
   } else {
     const int kchunk = ((Ktot + nsplit - 1) / nsplit + 3) / 4 * 4;
     const int nb = Pp / 64;
