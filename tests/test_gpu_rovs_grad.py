@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # P, S, C, N, T
+    (3, 1, 2, 5, 3),      # fewer paths than a tile, one column tile of padding
     (24, 1, 3, 600, 8),
     (60, 2, 2, 400, 6),
     (100, 1, 3, 300, 5),
