@@ -275,6 +275,17 @@ static int run_rovs(void) {
           "orc_rovs_backward_method");
     if (!finite_all(V, (size_t)T * N, "V") || !finite_all(out, 3, "out")) return 1;
   }
+  {
+    /* the smoothed-validity value and gradient under the drawn controls */
+    double J = 0.0;
+    double* gs = malloc(sizeof(double) * (size_t)S * C * P * P);
+    CHECK(gs != NULL, "malloc");
+    CHECK(orc_rovs_smooth_grad(&sp, controls, 0.05 * sp.params[3], &J, gs) == 0,
+          "orc_rovs_smooth_grad");
+    if (!finite_all(&J, 1, "J") || !finite_all(gs, (size_t)S * C * P * P, "grad_surv")) return 1;
+    CHECK(orc_rovs_smooth_grad(&sp, controls, 0.0, &J, gs) == -2, "orc_rovs_smooth_grad tau");
+    free(gs);
+  }
   free(controls);
   free(optc);
   free(tp);
