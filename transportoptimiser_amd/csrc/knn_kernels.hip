@@ -30,38 +30,76 @@
 namespace roadmc {
 
 // Weighted local-linear fit (multiLocLinReg, SimulateGPU.cu:2934-3053) over
-// the k neighbours nbd / nbi, sorted by (squared distance, index), summed in
-// that order by one thread: basis (1, x - q), weights
+// the k neighbours nbd / nbi, sorted by (squared distance, index), each sum
+// taken in that order (by one thread, or by one thread per sum in the select
+// kernel, with the same roundings): basis (1, x - q), weights
 // exp(-(d/h)^2 / 2) / sqrt(2 pi), h = the k-th neighbour's distance; FP64
 // Cholesky, the weighted mean when the system is singular.
+// The sums of the fit are B_i = sum_r t_B,i(r) and A_ij = sum_r t_A,ij(r)
+// (j <= i), neighbour r's terms written by knn_terms in the order B_0 .. B_D,
+// then A row by row; every sum runs over r in neighbour order.
+template <int D>
+constexpr int knn_nterms() {
+  return (D + 1) + (D + 1) * (D + 2) / 2;
+}
+
+template <int D>
+__device__ __forceinline__ void knn_terms(const double* nbd, const int* nbi, int r, double h,
+                                          const double* q, const double* __restrict__ u,
+                                          const double* __restrict__ y, double* t) {
+  constexpr int D1 = D + 1;
+  const double dist = sqrt(nbd[r]);
+  const int j = nbi[r];
+  const double tt = h > 0.0 ? dist / h : 0.0;
+  const double w = exp(-tt * tt / 2) / sqrt(2.0 * 3.14159265358979323846);
+  double v[D1];
+  v[0] = 1.0;
+#pragma unroll
+  for (int dd = 0; dd < D; ++dd) v[dd + 1] = u[(size_t)j * D + dd] - q[dd];
+  const double yr = y[j];
+  int o = 0;
+#pragma unroll
+  for (int i = 0; i < D1; ++i) t[o++] = (i == 0 ? yr : yr * v[i]) * w;
+#pragma unroll
+  for (int i = 0; i < D1; ++i)
+#pragma unroll
+    for (int jj = 0; jj <= i; ++jj) t[o++] = (i == 0 && jj == 0 ? 1.0 : v[i] * v[jj]) * w;
+}
+
+template <int D>
+__device__ double knn_solve(const double* sums);
+
 template <int D>
 __device__ double knn_regress(const double* nbd, const int* nbi, int k, const double* q,
                               const double* __restrict__ u, const double* __restrict__ y) {
-  constexpr int D1 = D + 1;
-  double A[D1][D1], B[D1];
+  constexpr int NT = knn_nterms<D>();
+  double sums[NT];
 #pragma unroll
-  for (int i = 0; i < D1; ++i) {
-    B[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < D1; ++j) A[i][j] = 0.0;
-  }
+  for (int i = 0; i < NT; ++i) sums[i] = 0.0;
   const double h = sqrt(nbd[k - 1]);
   for (int r = 0; r < k; ++r) {
-    const double dist = sqrt(nbd[r]);
-    const int j = nbi[r];
-    const double tt = h > 0.0 ? dist / h : 0.0;
-    const double w = exp(-tt * tt / 2) / sqrt(2.0 * 3.14159265358979323846);
-    double v[D1];
-    v[0] = 1.0;
+    double t[NT];
+    knn_terms<D>(nbd, nbi, r, h, q, u, y, t);
 #pragma unroll
-    for (int dd = 0; dd < D; ++dd) v[dd + 1] = u[(size_t)j * D + dd] - q[dd];
-    const double yr = y[j];
+    for (int i = 0; i < NT; ++i) sums[i] += t[i];
+  }
+  return knn_solve<D>(sums);
+}
+
+// Cholesky solve of the summed system; the weighted mean B_0 / A_00 when it
+// is singular.
+template <int D>
+__device__ double knn_solve(const double* sums) {
+  constexpr int D1 = D + 1;
+  double A[D1][D1], B[D1];
+  {
+    int o = 0;
 #pragma unroll
-    for (int i = 0; i < D1; ++i) {
-      B[i] += (i == 0 ? yr : yr * v[i]) * w;
+    for (int i = 0; i < D1; ++i) B[i] = sums[o++];
 #pragma unroll
-      for (int jj = 0; jj <= i; ++jj) A[i][jj] += (i == 0 && jj == 0 ? 1.0 : v[i] * v[jj]) * w;
-    }
+    for (int i = 0; i < D1; ++i)
+#pragma unroll
+      for (int jj = 0; jj <= i; ++jj) A[i][jj] = sums[o++];
   }
   double amax = 0.0;
 #pragma unroll
@@ -203,6 +241,16 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
   __shared__ int ci[kKnnMaxK], nbi[kKnnMaxK];
   __shared__ int s_b, s_cum, s_hb, s_cnt, s_ecnt;
   __shared__ int wcnt[4];
+  // The fit's per-neighbour terms (in the sort buffer, free by then) and
+  // sums: for D <= 3 one thread per neighbour writes its terms, then one
+  // thread per sum adds them in neighbour order; beyond, one thread does all.
+  constexpr bool kParFit = D <= 3;
+  constexpr int NTERM = knn_nterms<D>();
+  static_assert(!kParFit || NTERM * kKnnMaxK * sizeof(double) <=
+                                kKnnSelectCap * sizeof(unsigned long long),
+                "the fit's terms must fit the sort buffer");
+  double(*fterm)[kKnnMaxK] = reinterpret_cast<double(*)[kKnnMaxK]>(ekey);
+  __shared__ double fsum[NTERM];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = a.n, k = a.k, R = a.R;
   double q[D];
@@ -227,57 +275,158 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
   auto keyof = [&](int j) -> unsigned long long {
     return (unsigned long long)__double_as_longlong(dist2(j));
   };
-  // 1. Radix passes: prefix holds the resolved high bits of the k-th key,
-  //    kk its rank among the keys that share them, hb their number.
+  // Radix passes over every step-th sample: prefix holds the resolved high
+  // bits of the k-th key among them, kk its rank among the keys that share
+  // them, hb their number.  At least min_passes passes, then until the
+  // bucket fits LDS.
   unsigned long long prefix = 0ull, hmask = 0ull;
   int kk = k, hb = n;
-  for (int pass = 0; pass < 6; ++pass) {
-    const int shift = pass < 5 ? 53 - 11 * pass : 0;
-    const unsigned int dmask = pass < 5 ? kBins - 1 : 511u;
-    for (int i = tid; i < kBins; i += 256) hist[i] = 0u;
+  auto radix = [&](const int step, const int min_passes) {
+    prefix = 0ull;
+    hmask = 0ull;
+    kk = k;
+    hb = n;
+    for (int pass = 0; pass < 6; ++pass) {
+      const int shift = pass < 5 ? 53 - 11 * pass : 0;
+      const unsigned int dmask = pass < 5 ? kBins - 1 : 511u;
+      for (int i = tid; i < kBins; i += 256) hist[i] = 0u;
+      __syncthreads();
+      for (int j = tid * step; j < n; j += 256 * step) {
+        const unsigned long long key = keyof(j);
+        if ((key & hmask) == prefix) atomicAdd(&hist[(unsigned)(key >> shift) & dmask], 1u);
+      }
+      __syncthreads();
+      {
+        unsigned int sum = 0u;
+#pragma unroll
+        for (int i = 0; i < kBins / 256; ++i) sum += hist[tid * (kBins / 256) + i];
+        part[tid] = sum;
+      }
+      __syncthreads();
+      if (w == 0) {
+        // First bin whose cumulative count reaches kk: an inclusive scan over
+        // the lanes' four partial sums, then within the lane's 32 bins.
+        const int s4 = (int)(part[4 * lane] + part[4 * lane + 1] + part[4 * lane + 2] +
+                             part[4 * lane + 3]);
+        int inc = s4;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const int o = __shfl_up(inc, off);
+          if (lane >= off) inc += o;
+        }
+        const unsigned long long reach = __ballot(inc >= kk);  // hb >= kk keys counted
+        if (lane == __ffsll((long long)reach) - 1) {
+          int cum = inc - s4, c = 4 * lane;
+          while (cum + (int)part[c] < kk) cum += (int)part[c++];
+          int b = c * (kBins / 256);
+          while (cum + (int)hist[b] < kk) cum += (int)hist[b++];
+          s_b = b;
+          s_cum = cum;
+          s_hb = (int)hist[b];
+        }
+      }
+      __syncthreads();
+      prefix |= (unsigned long long)s_b << shift;
+      hmask |= (unsigned long long)dmask << shift;
+      kk -= s_cum;
+      hb = s_hb;
+      __syncthreads();
+      if (pass + 1 >= min_passes && hb <= kKnnSelectCap) break;
+    }
+  };
+  // Bitonic sort of the first h entries of ekey / eidx by (key, index).
+  auto sort_bucket = [&](const int h) {
+    int S2 = 1;
+    while (S2 < h) S2 <<= 1;
+    for (int i = h + tid; i < S2; i += 256) {
+      ekey[i] = ~0ull;
+      eidx[i] = 0x7fffffff;
+    }
+    __syncthreads();
+    for (int size = 2; size <= S2; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = tid; i < S2 / 2; i += 256) {
+          const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+          const unsigned long long kl = ekey[lo], kh = ekey[hi];
+          const int il = eidx[lo], ih = eidx[hi];
+          const bool gt = kl > kh || (kl == kh && il > ih);
+          if (gt == ((lo & size) == 0)) {
+            ekey[lo] = kh;
+            ekey[hi] = kl;
+            eidx[lo] = ih;
+            eidx[hi] = il;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  };
+  // The fit over nbd / nbi (ready in LDS), by every thread of the workgroup.
+  auto fit = [&]() {
+    if constexpr (kParFit) {
+      const double h = sqrt(nbd[k - 1]);
+      if (tid < k) {
+        double t[NTERM];
+        knn_terms<D>(nbd, nbi, tid, h, q, a.u, a.y, t);
+#pragma unroll
+        for (int i = 0; i < NTERM; ++i) fterm[i][tid] = t[i];
+      }
+      __syncthreads();
+      if (tid < NTERM) {
+        double sm = 0.0;
+        for (int r = 0; r < k; ++r) sm += fterm[tid][r];
+        fsum[tid] = sm;
+      }
+      __syncthreads();
+      if (tid != 0) return;
+      double v = knn_solve<D>(fsum);
+      if (a.clamp && v > 0.0) v = 0.0;  // rovCorrection (SimulateGPU.cu:3055-3073)
+      a.val[blockIdx.x] = v;
+    } else {
+      if (tid != 0) return;
+      double v = knn_regress<D>(nbd, nbi, k, q, a.u, a.y);
+      if (a.clamp && v > 0.0) v = 0.0;
+      a.val[blockIdx.x] = v;
+    }
+  };
+  // 0. A bound from a subsample: the k-th smallest key of every step-th
+  //    sample is at least the k-th smallest of all, so the samples whose key
+  //    is at most the top of that key's radix bucket include the k nearest
+  //    (with every tie at the bound).  One pass gathers them; when they fit
+  //    LDS, sorting them by (key, index) gives the k neighbours in the order
+  //    of a stable sort of all samples.  Otherwise the full radix select runs.
+  const int step = min(8, n / (4 * k));
+  if (step >= 2) {
+    radix(step, 2);
+    const unsigned long long bound = prefix | ~hmask;
+    if (tid == 0) s_ecnt = 0;
     __syncthreads();
     for (int j = tid; j < n; j += 256) {
       const unsigned long long key = keyof(j);
-      if ((key & hmask) == prefix) atomicAdd(&hist[(unsigned)(key >> shift) & dmask], 1u);
-    }
-    __syncthreads();
-    {
-      unsigned int sum = 0u;
-#pragma unroll
-      for (int i = 0; i < kBins / 256; ++i) sum += hist[tid * (kBins / 256) + i];
-      part[tid] = sum;
-    }
-    __syncthreads();
-    if (w == 0) {
-      // First bin whose cumulative count reaches kk: an inclusive scan over
-      // the lanes' four partial sums, then within the lane's 32 bins.
-      const int s4 = (int)(part[4 * lane] + part[4 * lane + 1] + part[4 * lane + 2] +
-                           part[4 * lane + 3]);
-      int inc = s4;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(inc, off);
-        if (lane >= off) inc += o;
-      }
-      const unsigned long long reach = __ballot(inc >= kk);  // hb >= kk keys counted
-      if (lane == __ffsll((long long)reach) - 1) {
-        int cum = inc - s4, c = 4 * lane;
-        while (cum + (int)part[c] < kk) cum += (int)part[c++];
-        int b = c * (kBins / 256);
-        while (cum + (int)hist[b] < kk) cum += (int)hist[b++];
-        s_b = b;
-        s_cum = cum;
-        s_hb = (int)hist[b];
+      if (key <= bound) {
+        const int pos = atomicAdd(&s_ecnt, 1);
+        if (pos < kKnnSelectCap) {
+          ekey[pos] = key;
+          eidx[pos] = j;
+        }
       }
     }
     __syncthreads();
-    prefix |= (unsigned long long)s_b << shift;
-    hmask |= (unsigned long long)dmask << shift;
-    kk -= s_cum;
-    hb = s_hb;
+    const int cnt = s_ecnt;  // at least k
+    if (cnt <= kKnnSelectCap) {
+      sort_bucket(cnt);
+      if (tid < k) {
+        nbd[tid] = __longlong_as_double((long long)ekey[tid]);
+        nbi[tid] = eidx[tid];
+      }
+      __syncthreads();
+      fit();
+      return;
+    }
     __syncthreads();
-    if (hb <= kKnnSelectCap) break;
   }
+  // 1. The full radix select over all samples.
+  radix(1, 1);
   // 2. Keys below the prefix (exactly k - kk of them) and the bucket.
   if (tid == 0) {
     s_cnt = 0;
@@ -304,30 +453,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
   __syncthreads();
   const int nl = k - kk;  // == s_cnt
   if (in_lds) {
-    int S2 = 1;
-    while (S2 < hb) S2 <<= 1;
-    for (int i = hb + tid; i < S2; i += 256) {
-      ekey[i] = ~0ull;
-      eidx[i] = 0x7fffffff;
-    }
-    __syncthreads();
-    for (int size = 2; size <= S2; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int i = tid; i < S2 / 2; i += 256) {
-          const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
-          const unsigned long long kl = ekey[lo], kh = ekey[hi];
-          const int il = eidx[lo], ih = eidx[hi];
-          const bool gt = kl > kh || (kl == kh && il > ih);
-          if (gt == ((lo & size) == 0)) {
-            ekey[lo] = kh;
-            ekey[hi] = kl;
-            eidx[lo] = ih;
-            eidx[hi] = il;
-          }
-        }
-        __syncthreads();
-      }
-    }
+    sort_bucket(hb);
     if (tid < nl) {
       const double d = cd[tid];
       const int i = ci[tid];
@@ -373,10 +499,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
     }
   }
   __syncthreads();
-  if (tid != 0) return;
-  double v = knn_regress<D>(nbd, nbi, k, q, a.u, a.y);
-  if (a.clamp && v > 0.0) v = 0.0;
-  a.val[blockIdx.x] = v;
+  fit();
 }
 
 // Multilinear interpolation of one point in its grid cell.
