@@ -42,6 +42,16 @@ struct KnnFitBatch {
 inline size_t knn_fit_lds(int cap, int k) {
   return ((size_t)cap * kKnnThreads + (size_t)k) * (sizeof(double) + sizeof(int));
 }
+// The fit's per-neighbour terms and sums after the lists (D <= 3): nterms x k
+// doubles and nterms more, starting knn_fit_term_offset doubles past nbd (the
+// k neighbours' distances, then their k indices, rounded up to doubles).
+__host__ __device__ inline size_t knn_fit_term_offset(int cap, int k) {
+  (void)cap;
+  return (size_t)k + ((size_t)k + 1) / 2;
+}
+inline size_t knn_fit_term_lds(int k, int nterms) {
+  return ((size_t)nterms * k + nterms + 1) * sizeof(double);
+}
 
 hipError_t launch_knn_fit(const KnnFitArgs& a, hipStream_t s);
 // Up to kKnnMaxBatch fits of the same dimension d in at most two launches

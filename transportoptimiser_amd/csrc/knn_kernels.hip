@@ -209,11 +209,38 @@ __global__ __launch_bounds__(kKnnThreads) void knn_fit_kernel(KnnFitBatch batch)
       nbi[r] = mi;
     }
   }
-  if (lane != 0) return;
-  // 3. The fit, summed in neighbour order by lane 0.
-  double v = knn_regress<D>(nbd, nbi, k, q, a.u, a.y);
-  if (a.clamp && v > 0.0) v = 0.0;  // rovCorrection (SimulateGPU.cu:3055-3073)
-  a.val[blockIdx.x] = v;
+  // 3. The fit: for D <= 3 each lane writes the terms of neighbours lane and
+  //    lane + 64 (after the lists, in the launch's extra LDS), then one lane
+  //    per sum adds them in neighbour order; beyond, lane 0 does all.
+  if constexpr (D <= 3) {
+    constexpr int NTERM = knn_nterms<D>();
+    double* fterm = nbd + knn_fit_term_offset(cap, k);  // [NTERM][k]
+    double* fsum = fterm + (size_t)NTERM * k;
+    __syncthreads();  // nbd / nbi written by lane 0
+    const double h = sqrt(nbd[k - 1]);
+    for (int r = lane; r < k; r += kKnnThreads) {
+      double t[NTERM];
+      knn_terms<D>(nbd, nbi, r, h, q, a.u, a.y, t);
+#pragma unroll
+      for (int i = 0; i < NTERM; ++i) fterm[(size_t)i * k + r] = t[i];
+    }
+    __syncthreads();
+    if (lane < NTERM) {
+      double sm = 0.0;
+      for (int r = 0; r < k; ++r) sm += fterm[(size_t)lane * k + r];
+      fsum[lane] = sm;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    double v = knn_solve<D>(fsum);
+    if (a.clamp && v > 0.0) v = 0.0;  // rovCorrection (SimulateGPU.cu:3055-3073)
+    a.val[blockIdx.x] = v;
+  } else {
+    if (lane != 0) return;
+    double v = knn_regress<D>(nbd, nbi, k, q, a.u, a.y);
+    if (a.clamp && v > 0.0) v = 0.0;
+    a.val[blockIdx.x] = v;
+  }
 }
 
 // Many samples per grid point: one 256-thread workgroup per point.  The keys
@@ -550,7 +577,8 @@ static hipError_t fit_d(const KnnFitArgs* fits, int nfits, hipStream_t s) {
     } else {
       wave.f[wave.nfits++] = a;
       gwave = std::max(gwave, a.G);
-      lds = std::max(lds, knn_fit_lds(a.cap, a.k));
+      lds = std::max(lds, knn_fit_lds(a.cap, a.k) +
+                              (D <= 3 ? knn_fit_term_lds(a.k, knn_nterms<D>()) : 0));
     }
   }
   if (sel.nfits) {
