@@ -90,6 +90,14 @@ struct RovsTile {
   }
 };
 
+// Waves per SIMD the forward and recompute kernels are compiled for: three
+// for FP32 products (167 of 168 VGPRs, a few spilled in the epilogue), two
+// for split bf16, which would spill its product loop at three.
+template <class TR>
+constexpr int rovs_min_waves() {
+  return TR::kSplit ? 2 : 3;
+}
+
 static int rovs_pad(bool split) { return split ? TraitsB3::PAD : TF::PAD; }
 
 size_t rovs_forward_lds(int Pp, int C, bool split) {
@@ -128,7 +136,7 @@ __device__ __forceinline__ void rovs_product(floatx16* acc, const PackedB& B, co
 }
 
 template <class TR>
-__global__ __launch_bounds__(kRovsMaxThreads) void rovs_forward_kernel(RovsForwardArgs a) {
+__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_forward_kernel(RovsForwardArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -712,7 +720,7 @@ __device__ double rovs_policy_step(const RovsRecomputeArgs& a, const float* st, 
 // states of the decision, every control's total after one migration, are
 // n . r_c with r_c the row sums of trans o surv_c (fixed-order sums).
 template <class TR>
-__global__ __launch_bounds__(kRovsMaxThreads) void rovs_recompute_kernel(RovsRecomputeArgs a) {
+__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_recompute_kernel(RovsRecomputeArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -929,7 +937,7 @@ size_t rovs_adj_lds(int Pp) {
 // in the accumulator layout of the wave's column tiles from year to year: the
 // tile assignment of the two products is the same, so each lane keeps the
 // entries it needs for the next year's w.
-__global__ __launch_bounds__(kRovsMaxThreads) void rovs_adj_kernel(RovsAdjArgs a) {
+__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(3))) void rovs_adj_kernel(RovsAdjArgs a) {
   constexpr int NT = kRovsNT;
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
