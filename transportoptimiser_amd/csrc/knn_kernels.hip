@@ -260,9 +260,14 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
   const KnnFitArgs& a = batch.f[blockIdx.y];
   if ((long long)blockIdx.x >= a.G) return;
   constexpr int kBins = 2048;
-  __shared__ unsigned int hist[kBins];
-  __shared__ unsigned int part[256];
   __shared__ unsigned long long ekey[kKnnSelectCap];
+  // The radix passes' histogram and its per-thread partial sums live in the
+  // sort buffer, which holds nothing while they run (27.8 KB of LDS in all at
+  // D = 2: five workgroups per CU).
+  static_assert((kBins + 256) * sizeof(unsigned int) <= sizeof(ekey),
+                "the histogram must fit the sort buffer");
+  unsigned int* hist = reinterpret_cast<unsigned int*>(ekey);
+  unsigned int* part = hist + kBins;
   __shared__ int eidx[kKnnSelectCap];
   __shared__ double cd[kKnnMaxK], nbd[kKnnMaxK];
   __shared__ int ci[kKnnMaxK], nbi[kKnnMaxK];
