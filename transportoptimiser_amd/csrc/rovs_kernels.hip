@@ -418,15 +418,26 @@ __global__ __launch_bounds__(kRovsScanBlock) void rovs_count_kernel(const RovsSt
 }
 
 // Exclusive scan of the block counts per control (one workgroup).
-__global__ __launch_bounds__(256) void rovs_scan_kernel(const RovsStepArgs* __restrict__ a, int nblk) {
-  const int c = threadIdx.x;
-  if (c >= a->C) return;
+// Exclusive scan of each control's block counts: one wave per control, 64
+// blocks per round (a wave prefix sum, then the round's total carried on).
+__global__ __launch_bounds__(64 * kRovsMaxC) void rovs_scan_kernel(const RovsStepArgs* __restrict__ a,
+                                                                  int nblk) {
+  const int c = threadIdx.x >> 6, lane = threadIdx.x & 63, C = a->C;
+  if (c >= C) return;
   int run = 0;
-  for (int b = 0; b < nblk; ++b) {
-    a->blk_off[b * a->C + c] = run;
-    run += a->blk_count[b * a->C + c];
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const int v = b < nblk ? a->blk_count[b * C + c] : 0;
+    int inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(inc, off);
+      if (lane >= off) inc += o;
+    }
+    if (b < nblk) a->blk_off[b * C + c] = run + inc - v;
+    run += __shfl(inc, 63);
   }
-  a->count[c] = run;
+  if (lane == 0) a->count[c] = run;
 }
 
 // Scatter in path order: a path's position is its block's offset plus the
@@ -461,7 +472,7 @@ hipError_t launch_rovs_collect(const RovsStepArgs* a, int N, int t, hipStream_t 
   hipLaunchKernelGGL(rovs_count_kernel, dim3(nblk), dim3(kRovsScanBlock), 0, s, a, t);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(rovs_scan_kernel, dim3(1), dim3(256), 0, s, a, nblk);
+  hipLaunchKernelGGL(rovs_scan_kernel, dim3(1), dim3(64 * kRovsMaxC), 0, s, a, nblk);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(rovs_scatter_kernel, dim3(nblk), dim3(kRovsScanBlock), 0, s, a, t);
   return hipGetLastError();
@@ -499,14 +510,23 @@ __global__ __launch_bounds__(256) void rovs_minmax_kernel(const RovsStepArgs* __
   }
 }
 
-__global__ void rovs_minmax_final_kernel(const RovsStepArgs* __restrict__ a, int nb) {
-  const int d = threadIdx.x;
-  if (d >= a->D) return;
+// One wave per state dimension over the block partials (minimum and maximum
+// are exact, so the order of the reduction does not matter).
+__global__ __launch_bounds__(64 * (kRovsMaxS + 1)) void rovs_minmax_final_kernel(
+    const RovsStepArgs* __restrict__ a, int nb) {
+  const int d = threadIdx.x >> 6, lane = threadIdx.x & 63, D = a->D;
+  if (d >= D) return;
   double lo = __builtin_inf(), hi = -__builtin_inf();
-  for (int b = 0; b < nb; ++b) {
-    lo = fmin(lo, a->mm[((size_t)b * a->D + d) * 2 + 0]);
-    hi = fmax(hi, a->mm[((size_t)b * a->D + d) * 2 + 1]);
+  for (int b = lane; b < nb; b += 64) {
+    lo = fmin(lo, a->mm[((size_t)b * D + d) * 2 + 0]);
+    hi = fmax(hi, a->mm[((size_t)b * D + d) * 2 + 1]);
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off));
+    hi = fmax(hi, __shfl_xor(hi, off));
+  }
+  if (lane != 0) return;
   if (!(lo <= hi)) lo = hi = 0.0;  // no samples at all
   a->range[d * 2 + 0] = lo;
   a->range[d * 2 + 1] = hi;
@@ -519,7 +539,8 @@ hipError_t launch_rovs_ranges(const RovsStepArgs* a, int N, int D, hipStream_t s
   hipLaunchKernelGGL(rovs_minmax_kernel, dim3(kRovsMinMaxBlocks, D), dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(rovs_minmax_final_kernel, dim3(1), dim3(64), 0, s, a, kRovsMinMaxBlocks);
+  hipLaunchKernelGGL(rovs_minmax_final_kernel, dim3(1), dim3(64 * (kRovsMaxS + 1)), 0, s, a,
+                     kRovsMinMaxBlocks);
   return hipGetLastError();
 }
 
@@ -1195,22 +1216,36 @@ __global__ __launch_bounds__(1024) void rovs_first_kernel(const RovsStepArgs* __
     __syncthreads();
     return r;
   };
-  for (int c = 0; c < C; ++c) {
-    double s = 0.0, n = 0.0;
-    for (int p = tid; p < N; p += 1024)
-      if (c0[p] == c) {
-        s += V1[p];
-        n += 1.0;
+  // One pass for every control's sum and count, the unit profits and the
+  // adjusted populations: each thread adds the same values in the same order
+  // (p = tid, tid + 1024, ...) as one pass per sum would.
+  double cs[kRovsMaxC], cn[kRovsMaxC];
+#pragma unroll
+  for (int c = 0; c < kRovsMaxC; ++c) cs[c] = cn[c] = 0.0;
+  double up = 0.0;
+  for (int p = tid; p < N; p += 1024) {
+    const int cp = c0[p];
+    const double v = V1[p];
+#pragma unroll
+    for (int c = 0; c < kRovsMaxC; ++c)
+      if (cp == c) {
+        cs[c] += v;
+        cn[c] += 1.0;
       }
-    s = block_sum(s);
-    n = block_sum(n);
+    up += a->profit[p];
+    if (a->adj_pops)
+      for (int sp = 0; sp < S; ++sp) a->adj_pops[(size_t)sp * N + p] = rovs_adj(a, sp, 0, C - 1, p);
+  }
+#pragma unroll
+  for (int c = 0; c < kRovsMaxC; ++c) {
+    if (c >= C) break;
+    const double sc = block_sum(cs[c]);
+    const double nc = block_sum(cn[c]);
     if (tid == 0) {
-      sums[c] = s;
-      cnts[c] = n;
+      sums[c] = sc;
+      cnts[c] = nc;
     }
   }
-  double up = 0.0;
-  for (int p = tid; p < N; p += 1024) up += a->profit[p];
   up = block_sum(up) / N;
   if (tid == 0) {
     double best = 0.0;
@@ -1240,24 +1275,20 @@ __global__ __launch_bounds__(1024) void rovs_first_kernel(const RovsStepArgs* __
   m = block_sum(m);
   const double n = cnts[bc] > 0.0 ? cnts[bc] : 1.0;
   const double mean = m / n;
-  for (int p = tid; p < N; p += 1024)
+  for (int p = tid; p < N; p += 1024) {
     if (c0[p] == bc) {
       const double d = V1[p] * a->disc + base - mean;
       q += d * d;
     }
-  q = block_sum(q);
-  for (int p = tid; p < N; p += 1024) {
-    a->V[p] = res[0];
+    a->V[p] = res[0];  // time 0's row of V, apart from the V1 row read here
     a->optc[p] = bc;
   }
+  q = block_sum(q);
   if (tid == 0) {
     out[0] = res[0];
     out[1] = sqrt(q / n);
     out[2] = (double)bc;
   }
-  if (a->adj_pops)
-    for (int p = tid; p < N; p += 1024)
-      for (int sp = 0; sp < S; ++sp) a->adj_pops[(size_t)sp * N + p] = rovs_adj(a, sp, 0, C - 1, p);
 }
 
 hipError_t launch_rovs_first(const RovsStepArgs* a, double* out, hipStream_t s) {
