@@ -464,13 +464,18 @@ extern "C" int roadmc_rovs_run(roadmc_ctx* ctx, const roadmc_rovs_problem* pr,
     ra.adj_pops = h.adj_pops;
   }
   RMC_HIP(ctx, roadmc::launch_rovs_terminal(d_args, N, s));
-  std::vector<int> counts(C);
+  // Each step's control counts come back through the pinned staging (resized
+  // while the stream is idle).  The range kernels read the counts on the
+  // device, so they are queued before the host waits for them.
+  RMC_HIP(ctx, hipStreamSynchronize(s));
+  if ((rc = rmc_pinned_ensure(ctx, roadmc::kRovsMaxC * sizeof(int)))) return rc;
+  int* const counts = static_cast<int*>(ctx->pinned);
   for (int t = T - 2; t >= 1; --t) {
     RMC_HIP(ctx, roadmc::launch_rovs_collect(d_args, N, t, s));
-    RMC_HIP(ctx, hipMemcpyAsync(counts.data(), h.count, C * sizeof(int), hipMemcpyDeviceToHost, s));
-    RMC_HIP(ctx, hipStreamSynchronize(s));
+    RMC_HIP(ctx, hipMemcpyAsync(counts, h.count, C * sizeof(int), hipMemcpyDeviceToHost, s));
     RMC_HIP(ctx, roadmc::launch_rovs_ranges(d_args, N, D, s));
     RMC_HIP(ctx, roadmc::launch_rovs_normalise(d_args, N, D, R, s));
+    RMC_HIP(ctx, hipStreamSynchronize(s));
     std::vector<roadmc::KnnFitArgs> fits;
     for (int c = 0; c < C; ++c) {
       const int n = counts[c];
