@@ -196,6 +196,40 @@ def test_invalid_inputs_write_nothing(ctx):
     assert e.value.code == nat.ERR_NUMERIC
 
 
+def test_batch_validation_finds_non_finite_matrices_in_road_order(ctx):
+    # The batch checks its host matrices over threads, then every road in
+    # order: a non-finite entry in any road's or species' matrix is found, and
+    # an earlier road's other error still comes first.
+    def road(i, S=2):
+        return _problem(24 + 8 * i, 10, 2, S=S, seed=40 + i, road_id=i)
+
+    good = [road(i) for i in range(6)]
+    print("batch validation: 6 roads, NaN in species 1's surv of road 4", flush=True)
+    bad_surv = [road(i) for i in range(6)]
+    sp = bad_surv[4].species[1]
+    sp.surv = np.ones_like(sp.trans) if sp.surv is None else sp.surv.copy()
+    sp.surv[1, 3] = np.inf
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte_batch(bad_surv)
+    assert e.value.code == nat.ERR_NUMERIC
+    # an out-of-range patch count in road 1 before the NaN of road 3
+    order = [road(i) for i in range(5)]
+    order[1].species[0] = SpeciesData(trans=np.eye(1025), surv=None, init_pop=np.ones(1025),
+                                      capacity=np.ones(1025), growth=None)
+    order[3].species[0].trans = order[3].species[0].trans.copy()
+    order[3].species[0].trans[0, 0] = np.nan
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte_batch(order)
+    assert e.value.code == nat.ERR_INVALID
+    # the NaN road before it: the NaN first
+    order[1], order[3] = order[3], order[1]
+    with pytest.raises(nat.RoadmcError) as e:
+        ctx.simulate_mte_batch(order)
+    assert e.value.code == nat.ERR_NUMERIC
+    res = ctx.simulate_mte_batch(good)
+    assert len(res) == 6 and all(np.isfinite(r.mte_prob).all() for r in res)
+
+
 def test_injected_allocation_failure(ctx):
     from transportoptimiser_amd import api
     pr = _problem(20, 50, 3)

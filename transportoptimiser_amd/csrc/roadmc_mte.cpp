@@ -42,8 +42,21 @@ bool finite_array(const double* p, size_t n) {
   return true;
 }
 
+// Whether species s's host matrices (trans, and surv when given) are finite;
+// true when there are none or the species' patch count is out of range (the
+// other checks reject those first).
+bool species_matrices_finite(const roadmc_species& sp, int max_patches) {
+  const int P = sp.n_patches;
+  if (!sp.trans || P < 1 || P > max_patches) return true;
+  return finite_array(sp.trans, (size_t)P * P) &&
+         (!sp.surv || finite_array(sp.surv, (size_t)P * P));
+}
+
+// mats_finite, when given, holds species_matrices_finite for each species
+// (computed beforehand, e.g. in parallel); the checks and their order are the
+// same either way.
 int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr,
-                     int max_patches = ROADMC_MAX_PATCHES) {
+                     int max_patches = ROADMC_MAX_PATCHES, const char* mats_finite = nullptr) {
   if (!pr) return rmc_fail(ctx, ROADMC_ERR_INVALID, "problem is NULL");
   if (pr->n_species < 1 || pr->n_species > 63 || !pr->species)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "n_species must be in [1, 63]");
@@ -86,9 +99,8 @@ int validate_problem(roadmc_ctx* ctx, const roadmc_mte_problem* pr,
     for (int j = 0; j < P; ++j)
       if (!(sp.capacity[j] > 0.0) || !std::isfinite(sp.capacity[j]))
         return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "capacity must be finite and > 0");
-    if (!finite_array(sp.init_pop, P) ||
-        (sp.trans && !finite_array(sp.trans, (size_t)P * P)) ||
-        (sp.trans && sp.surv && !finite_array(sp.surv, (size_t)P * P)) ||
+    const bool mats_ok = mats_finite ? mats_finite[s] != 0 : species_matrices_finite(sp, max_patches);
+    if (!finite_array(sp.init_pop, P) || !mats_ok ||
         (sp.growth && !finite_array(sp.growth, P)) || !finite_array(sp.params, 8) ||
         !std::isfinite(sp.noise_sd) || !std::isfinite(sp.threshold_ratio))
       return rmc_fail(ctx, ROADMC_ERR_NUMERIC, "non-finite input");
@@ -720,8 +732,57 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
   if (!out || !roads || n_roads < 1)
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "roads/out invalid");
   *out = nullptr;
+  // The host matrices' finiteness (most of the validation's reads) over
+  // threads, species split by matrix size; then the checks road by road, in
+  // order, with those flags.
+  std::vector<size_t> mat_off((size_t)n_roads + 1, 0);
   for (int r = 0; r < n_roads; ++r) {
-    int rc = validate_problem(ctx, &roads[r]);
+    const int ns = roads[r].species && roads[r].n_species >= 1 && roads[r].n_species <= 63
+                       ? roads[r].n_species
+                       : 0;
+    mat_off[r + 1] = mat_off[r] + (size_t)ns;
+  }
+  std::vector<char> mat_ok(std::max<size_t>(mat_off[n_roads], 1), 1);
+  {
+    std::vector<std::pair<int, int>> items;  // (road, species)
+    size_t total = 0;
+    for (int r = 0; r < n_roads; ++r)
+      for (size_t s = 0; s < mat_off[r + 1] - mat_off[r]; ++s) {
+        items.emplace_back(r, (int)s);
+        const int P = roads[r].species[s].n_patches;
+        total += (size_t)std::max(P, 0) * std::max(P, 0) + 4096;
+      }
+    auto check = [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; ++i) {
+        const int r = items[i].first, sidx = items[i].second;
+        mat_ok[mat_off[r] + sidx] =
+            species_matrices_finite(roads[r].species[sidx], ROADMC_MAX_PATCHES) ? 1 : 0;
+      }
+    };
+    const size_t ni = items.size();
+    const int nth = (int)std::min<size_t>(
+        std::max<size_t>(ni, 1), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<size_t> bounds(nth + 1, ni);
+    bounds[0] = 0;
+    for (size_t i = 0, acc = 0, t = 1; i < ni && t < (size_t)nth; ++i) {
+      const int P = std::max(roads[items[i].first].species[items[i].second].n_patches, 0);
+      acc += (size_t)P * P + 4096;
+      if (acc * nth >= total * t) bounds[t++] = i + 1;
+    }
+    std::vector<std::thread> pool;
+    try {
+      for (int t = 1; t < nth; ++t) pool.emplace_back(check, bounds[t], bounds[t + 1]);
+      check(bounds[0], bounds[1]);
+    } catch (...) {
+      for (auto& th : pool) th.join();
+      pool.clear();
+      check(0, ni);  // no threads to be had: check serially
+    }
+    for (auto& th : pool) th.join();
+  }
+  for (int r = 0; r < n_roads; ++r) {
+    int rc = validate_problem(ctx, &roads[r], ROADMC_MAX_PATCHES,
+                              mat_off[r + 1] > mat_off[r] ? &mat_ok[mat_off[r]] : nullptr);
     if (rc) return rc;
     if (roads[r].precision != roads[0].precision)
       return rmc_fail(ctx, ROADMC_ERR_INVALID, "all roads of a batch need one precision");

@@ -156,8 +156,11 @@ def make_species(rng: np.random.Generator, n_patches: int, *, side_km: float = 1
     cap = rng.uniform(500.0, 5000.0, size=P)
     init = 0.8 * cap
     params = np.array([0.05, 0.05, 0.1, threshold_ratio, 0.5, 0.05, 0.1, 0.2])
-    return SpeciesData(trans=trans, surv=surv, init_pop=init, capacity=cap, growth=growth,
-                       params=params, noise_sd=noise_sd, threshold_ratio=threshold_ratio)
+    # Column-major, the layout of the C ABI's matrices (Eigen's default, in
+    # which the reference keeps them): passed to the library without a copy.
+    return SpeciesData(trans=np.asfortranarray(trans), surv=np.asfortranarray(surv),
+                       init_pop=init, capacity=cap, growth=growth, params=params,
+                       noise_sd=noise_sd, threshold_ratio=threshold_ratio)
 
 
 def config1(seed: int = 1, n_paths: int = 10000, n_years: int = 50) -> RoadProblem:
