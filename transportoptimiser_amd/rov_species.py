@@ -212,6 +212,10 @@ def synthetic_problem(seed: int = 5, n_patches: int = 64, n_species: int = 1,
         cap = rng.uniform(500.0, 5000.0, size=P)
         init = 0.8 * cap
         params = np.array([5.0, 5.0, 0.1, threshold_ratio * init.sum(), 0.5, 0.1, 0.05, 1.0])
+        # Each matrix column-major, the C ABI's layout (Eigen's default, in
+        # which the reference keeps them): passed to the library without a copy.
+        trans = np.asfortranarray(trans)
+        surv = np.ascontiguousarray(surv.transpose(0, 2, 1)).transpose(0, 2, 1)
         species.append(RovsSpecies(init, cap, trans, surv, params))
     return RovsProblem(
         species=species, flow=flow,
