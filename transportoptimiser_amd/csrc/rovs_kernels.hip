@@ -830,6 +830,15 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
       if (tid < kRovsRows) ctl[tid] = (p0 + tid < N) ? a.controls[(size_t)u * N + p0 + tid] : 0;
       __syncthreads();
     }
+    // The controls some path of the tile takes this year (the same mask in
+    // every wave): only their products are formed.  The surviving fractions
+    // come from the row-sum dots above, so a product no path takes is unused.
+    unsigned used = 0u;
+    {
+      const int cl = lane < kRovsRows ? ctl[lane] : -1;
+      for (int c = 0; c < C; ++c)
+        if (__ballot(cl == c) != 0ull) used |= 1u << c;
+    }
     // Migration under each path's control and growth, species by species.
     for (int sp = 0; sp < S; ++sp) {
       float* A = A0 + (size_t)sp * kRovsRows * LDA;
@@ -841,6 +850,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
       for (int t = 0; t < kRovsNT; ++t) sel[t] = floatx16{};
       for (int c = 0; c < C; ++c) {
+        if (!((used >> c) & 1u)) continue;
         const PackedB B = packed_b(a.mpack + ((size_t)sp * C + c) * Pp * a.Pk, nkg, mbytes);
         if (my_n == 1)
           rovs_product<TR, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
