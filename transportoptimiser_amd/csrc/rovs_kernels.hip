@@ -1014,10 +1014,19 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     int LDAy = LDA, laney = lane;
     asm volatile("" : "+s"(LDAy), "+v"(laney));
+    // The controls some path of the tile takes this year (the same mask in
+    // every wave): only their products are formed in both sweeps below.
+    unsigned used = 0u;
+    {
+      const int cl = lane < kRovsRows ? ctl[lane] : -1;
+      for (int c = 0; c < C; ++c)
+        if (__ballot(cl == c) != 0ull) used |= 1u << c;
+    }
     // x_t: the product under the path's control.
 #pragma unroll
     for (int t = 0; t < NT; ++t) sel[t] = floatx16{};
     for (int c = 0; c < C; ++c) {
+      if (!((used >> c) & 1u)) continue;
       const PackedB B = packed_b(a.mpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
       if (my_n == 1)
         rovs_product<TF, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
@@ -1068,6 +1077,7 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
     for (int t = 0; t < NT; ++t) lam[t] = floatx16{};
     for (int c = 0; c < C; ++c) {
+      if (!((used >> c) & 1u)) continue;
       const PackedB B = packed_b(a.mtpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
       if (my_n == 1)
         rovs_product<TF, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
