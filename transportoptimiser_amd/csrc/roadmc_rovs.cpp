@@ -613,6 +613,7 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
   } need[] = {
       {&B.mpack, (size_t)S * C * Pp * Pk * sizeof(float)},
       {&G.mtpack, (size_t)S * C * Pp * Pk * sizeof(float)},
+      {&B.rvec, (size_t)S * C * Pp * sizeof(float)},
       {&B.cols, (size_t)S * 3 * Pp * sizeof(float)},
       {&B.prices, (size_t)(T + 1) * Uc * N * sizeof(double)},
       {&B.controls, (size_t)T * N * sizeof(int)},
@@ -635,19 +636,25 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
   hipStream_t s = ctx->stream;
 
   // trans o surv_c and its transpose in MFMA B order; the per-patch columns.
+  // The row sums of each origin (for the forward pass's surviving fractions),
+  // at the origin's column of the population tile.
   std::vector<float> mp((size_t)S * C * Pp * Pk, 0.f), mt((size_t)S * C * Pp * Pk, 0.f);
-  std::vector<float> cols((size_t)S * 3 * Pp), init_total(S);
+  std::vector<float> cols((size_t)S * 3 * Pp), init_total(S), rvec((size_t)S * C * Pp, 0.f);
   for (int sp = 0; sp < S; ++sp) {
     const roadmc_rovs_species& sc = pr->species[sp];
     for (int c = 0; c < C; ++c) {
       const size_t off = ((size_t)sp * C + c) * Pp * Pk;
       const double* sv = sc.surv + (size_t)c * P * P;
-      for (int k = 0; k < P; ++k)
+      for (int k = 0; k < P; ++k) {
+        double rs = 0.0;
         for (int j = 0; j < P; ++j) {
           const float v = (float)(sc.trans[(size_t)j * P + k] * sv[(size_t)j * P + k]);
           mp[off + roadmc::mte_pack_index(false, Pk, k, j)] = v;
           mt[off + roadmc::mte_pack_index(false, Pk, j, k)] = v;
+          rs += v;
         }
+        rvec[((size_t)sp * C + c) * Pp + roadmc::rovs_tile_col(k, false)] = (float)rs;
+      }
     }
     float* cl = cols.data() + (size_t)sp * 3 * Pp;
     double tot = 0.0;
@@ -663,6 +670,8 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
   RMC_HIP(ctx, hipMemcpyAsync(B.mpack.ptr, mp.data(), mp.size() * sizeof(float),
                               hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(G.mtpack.ptr, mt.data(), mt.size() * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  RMC_HIP(ctx, hipMemcpyAsync(B.rvec.ptr, rvec.data(), rvec.size() * sizeof(float),
                               hipMemcpyHostToDevice, s));
   RMC_HIP(ctx, hipMemcpyAsync(B.cols.ptr, cols.data(), cols.size() * sizeof(float),
                               hipMemcpyHostToDevice, s));
@@ -744,6 +753,9 @@ extern "C" int roadmc_rovs_smooth_grad(roadmc_ctx* ctx, const roadmc_rovs_proble
     a.aars = static_cast<float*>(B.aars.ptr) + (size_t)sp * T * C * N;
     a.ck = 1;
     a.split = 0;
+    // Controls fixed by the policy: the dots' form (surviving fractions from
+    // the row sums, products only for the controls a tile takes).
+    a.rvec = static_cast<const float*>(B.rvec.ptr) + (size_t)sp * C * Pp;
     if (want_grad) {
       a.ckpt = static_cast<float*>(B.ckpt.ptr) + (size_t)sp * T * N * Pp;
       a.grm_out = static_cast<float*>(B.grm.ptr) + (size_t)sp * T * N;

@@ -40,9 +40,14 @@ struct RovsForwardArgs {
   // Split-bf16 products (mpack in that order, Pk == Pp); the checkpoints then
   // hold each entry's hi | lo << 16 bits.
   int split;
+  // Optional (FP32 only): the row sums of each control's matrix, [C][Pp] in
+  // the tile's column order.  When given, the surviving fractions are dots of
+  // the year-start tile with them, and a tile forms only the products of the
+  // controls its paths take that year (for controls fixed by a policy).
+  const float* rvec;
 };
 
-size_t rovs_forward_lds(int Pp, int C, bool split);
+size_t rovs_forward_lds(int Pp, int C, bool split, bool dots = false);
 hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s);
 
 // Unit profit per path and year: unit_cost + sum_v fuel_cost_v price_{fuel_idx_v}

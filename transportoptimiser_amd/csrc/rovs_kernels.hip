@@ -100,14 +100,16 @@ constexpr int rovs_min_waves() {
 
 static int rovs_pad(bool split) { return split ? TraitsB3::PAD : TF::PAD; }
 
-size_t rovs_forward_lds(int Pp, int C, bool split) {
+size_t rovs_forward_lds(int Pp, int C, bool split, bool dots) {
   const int LDA = Pp + rovs_pad(split);
   return (size_t)kRovsRows * LDA * sizeof(float)      // state tile
          + kRovsRows * sizeof(float)                  // growth-rate means
          + (size_t)rovs_waves(Pp) * kRovsRows * sizeof(float)  // per-wave row sums
          + kRovsRows * sizeof(float)                  // row totals
          + (size_t)C * kRovsRows * sizeof(float)      // AARs
-         + kRovsRows * sizeof(int);                   // controls
+         + kRovsRows * sizeof(int)                    // controls
+         // the dots' partial sums: 2 strides of columns per wave
+         + (dots ? (size_t)2 * rovs_waves(Pp) * C * kRovsRows * sizeof(float) : 0);
 }
 
 // Sum over the 32 columns held by the 32 lanes of a half-wave.
@@ -135,7 +137,10 @@ __device__ __forceinline__ void rovs_product(floatx16* acc, const PackedB& B, co
   }
 }
 
-template <class TR>
+// DOTS: the surviving fractions from dots with a.rvec and products only for
+// the controls the tile's paths take (a.rvec given); otherwise from the row
+// sums of every control's product.
+template <class TR, bool DOTS>
 __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_forward_kernel(RovsForwardArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -149,6 +154,8 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
   float* tot = rsum + NW * kRovsRows;
   float* aar = tot + kRovsRows;     // [C][32]
   int* ctl = reinterpret_cast<int*>(aar + C * kRovsRows);
+  const int nq = nthreads / kRovsRows;                   // column strides of the dots
+  float* spart = reinterpret_cast<float*>(ctl + kRovsRows);  // [nq][C][32] (DOTS)
   const float* K = a.cols;
   const float* Kinv = a.cols + Pp;
   const float* init = a.cols + 2 * Pp;
@@ -198,15 +205,52 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
       }
     }
     __syncthreads();
+    unsigned used = ~0u;
+    if constexpr (DOTS) {
+      // The surviving fractions' numerators as dots of the year-start tile
+      // with each control's row sums (a stride of columns per thread, summed
+      // over the strides in order after the products), and the controls some
+      // path of the tile takes (the same mask in every wave).
+      const int r = tid & (kRovsRows - 1), q = tid / kRovsRows;
+      const RovsTile<TR> ts{A, LDAy};
+      float accs[kRovsMaxC];
+#pragma unroll
+      for (int c = 0; c < kRovsMaxC; ++c) accs[c] = 0.f;
+      for (int i = q; i < Pp; i += nq) {
+        const float v = ts.get(r, i);
+#pragma unroll
+        for (int c = 0; c < kRovsMaxC; ++c)
+          if (c < C) accs[c] += v * a.rvec[(size_t)c * Pp + i];
+      }
+#pragma unroll
+      for (int c = 0; c < kRovsMaxC; ++c)
+        if (c < C) spart[(q * C + c) * kRovsRows + r] = accs[c];
+      used = 0u;
+      const int cl = laney < kRovsRows ? ctl[laney] : -1;
+      for (int c = 0; c < C; ++c)
+        if (__ballot(cl == c) != 0ull) used |= 1u << c;
+    }
     floatx16 sel[kRovsNT], acc[kRovsNT];
 #pragma unroll
     for (int t = 0; t < kRovsNT; ++t) sel[t] = floatx16{};
     for (int c = 0; c < C; ++c) {
+      if (!((used >> c) & 1u)) continue;
       const PackedB B = packed_b(a.mpack + (size_t)c * Pp * a.Pk, nkg, mbytes);
       if (my_n == 1)
         rovs_product<TR, 1>(acc, B, A, tcol, LDAy, arow, akoff, laney);
       else if (my_n == 2)
         rovs_product<TR, 2>(acc, B, A, tcol, LDAy, arow, akoff, laney);
+      if constexpr (DOTS) {
+        // The select of each path's own control only.
+#pragma unroll
+        for (int r = 0; r < TF::NACC; ++r)
+          if (ctl[TF::acc_row(laney, r)] == c) {
+#pragma unroll
+            for (int t = 0; t < kRovsNT; ++t)
+              if (t < my_n) sel[t][r] = acc[t][r];
+          }
+        continue;
+      }
       // Row sums of this control's product over the wave's columns, and the
       // select of each path's own control.
 #pragma unroll
@@ -243,7 +287,15 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
       grm[tid] = g + rev * (mu - g) * dt + g * (zb * sd) + (__expf(jsz) - 1.f) * g * jump;
       if (a.grm_out && p0 + tid < a.N) a.grm_out[(size_t)year * a.N + p0 + tid] = grm[tid];
     }
-    __syncthreads();
+    __syncthreads();  // (DOTS: also every wave's products have read the tile)
+    if constexpr (DOTS) {
+      if (tid < C * kRovsRows) {
+        const int c = tid / kRovsRows, r = tid - c * kRovsRows;
+        float sm = 0.f;
+        for (int qq = 0; qq < nq; ++qq) sm += spart[(qq * C + c) * kRovsRows + r];
+        aar[c * kRovsRows + r] = sm / tot[r];
+      }
+    }
     // Growth (SimulateGPU.cu:902-916): g = grMean + z lv per patch,
     // n = n' (1 + g (K - n') / K / 100), written back to the tile.
     int LDAe = LDA, p0e = p0, lanee = lane;
@@ -296,22 +348,30 @@ __global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu
   }
 }
 
-template <class TR>
+template <class TR, bool DOTS>
 static hipError_t launch_rovs_forward_t(const RovsForwardArgs& a, hipStream_t s) {
-  const size_t lds = rovs_forward_lds(a.Pp, a.C, TR::kSplit);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_forward_kernel<TR>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const size_t lds = rovs_forward_lds(a.Pp, a.C, TR::kSplit, DOTS);
+  hipError_t e =
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&rovs_forward_kernel<TR, DOTS>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const int blocks = (a.N + kRovsRows - 1) / kRovsRows;
-  hipLaunchKernelGGL(rovs_forward_kernel<TR>, dim3(blocks), dim3(64 * rovs_waves(a.Pp)), lds, s,
-                     a);
+  hipLaunchKernelGGL((rovs_forward_kernel<TR, DOTS>), dim3(blocks), dim3(64 * rovs_waves(a.Pp)),
+                     lds, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_rovs_forward(const RovsForwardArgs& a, hipStream_t s) {
   // Split-bf16 products contract over Pp (an even count of 16-k chunks).
   if (a.split && a.Pk != a.Pp) return hipErrorInvalidValue;
-  return a.split ? launch_rovs_forward_t<TraitsB3>(a, s) : launch_rovs_forward_t<TF>(a, s);
+  if (a.rvec) {
+    // The dots' form: FP32 products, at most kRovsMaxC controls (C x 32 <= 256
+    // threads finish the sums).
+    if (a.split || a.C < 1 || a.C > kRovsMaxC) return hipErrorInvalidValue;
+    return launch_rovs_forward_t<TF, true>(a, s);
+  }
+  return a.split ? launch_rovs_forward_t<TraitsB3, false>(a, s)
+                 : launch_rovs_forward_t<TF, false>(a, s);
 }
 
 // Unit profit (computePathStates, SimulateGPU.cu:985-1002) with the ore
