@@ -42,9 +42,9 @@ bool finite_array(const double* p, size_t n) {
   return true;
 }
 
-// Whether species s's host matrices (trans, and surv when given) are finite;
-// true when there are none or the species' patch count is out of range (the
-// other checks reject those first).
+// Whether a species' host matrices (trans, and surv when given) are finite;
+// true when there are none or its patch count is out of range (the other
+// checks reject those first).
 bool species_matrices_finite(const roadmc_species& sp, int max_patches) {
   const int P = sp.n_patches;
   if (!sp.trans || P < 1 || P > max_patches) return true;
@@ -749,8 +749,8 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
     for (int r = 0; r < n_roads; ++r)
       for (size_t s = 0; s < mat_off[r + 1] - mat_off[r]; ++s) {
         items.emplace_back(r, (int)s);
-        const int P = roads[r].species[s].n_patches;
-        total += (size_t)std::max(P, 0) * std::max(P, 0) + 4096;
+        const size_t P = (size_t)std::clamp(roads[r].species[s].n_patches, 0, ROADMC_MAX_PATCHES);
+        total += P * P + 4096;
       }
     auto check = [&](size_t i0, size_t i1) {
       for (size_t i = i0; i < i1; ++i) {
@@ -765,8 +765,9 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
     std::vector<size_t> bounds(nth + 1, ni);
     bounds[0] = 0;
     for (size_t i = 0, acc = 0, t = 1; i < ni && t < (size_t)nth; ++i) {
-      const int P = std::max(roads[items[i].first].species[items[i].second].n_patches, 0);
-      acc += (size_t)P * P + 4096;
+      const size_t P = (size_t)std::clamp(roads[items[i].first].species[items[i].second].n_patches,
+                                          0, ROADMC_MAX_PATCHES);
+      acc += P * P + 4096;
       if (acc * nth >= total * t) bounds[t++] = i + 1;
     }
     std::vector<std::thread> pool;

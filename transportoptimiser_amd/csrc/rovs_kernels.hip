@@ -141,7 +141,9 @@ __device__ __forceinline__ void rovs_product(floatx16* acc, const PackedB& B, co
 // the controls the tile's paths take (a.rvec given); otherwise from the row
 // sums of every control's product.
 template <class TR, bool DOTS>
-__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_forward_kernel(RovsForwardArgs a) {
+__global__ __launch_bounds__(kRovsMaxThreads)
+__attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_forward_kernel(
+    RovsForwardArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -477,7 +479,6 @@ __global__ __launch_bounds__(kRovsScanBlock) void rovs_count_kernel(const RovsSt
   if (threadIdx.x < a->C) a->blk_count[blockIdx.x * a->C + threadIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of the block counts per control (one workgroup).
 // Exclusive scan of each control's block counts: one wave per control, 64
 // blocks per round (a wave prefix sum, then the round's total carried on).
 __global__ __launch_bounds__(64 * kRovsMaxC) void rovs_scan_kernel(const RovsStepArgs* __restrict__ a,
@@ -801,7 +802,9 @@ __device__ double rovs_policy_step(const RovsRecomputeArgs& a, const float* st, 
 // states of the decision, every control's total after one migration, are
 // n . r_c with r_c the row sums of trans o surv_c (fixed-order sums).
 template <class TR>
-__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_recompute_kernel(RovsRecomputeArgs a) {
+__global__ __launch_bounds__(kRovsMaxThreads)
+__attribute__((amdgpu_waves_per_eu(rovs_min_waves<TR>()))) void rovs_recompute_kernel(
+    RovsRecomputeArgs a) {
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1028,7 +1031,9 @@ size_t rovs_adj_lds(int Pp) {
 // in the accumulator layout of the wave's column tiles from year to year: the
 // tile assignment of the two products is the same, so each lane keeps the
 // entries it needs for the next year's w.
-__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(3))) void rovs_adj_kernel(RovsAdjArgs a) {
+// Three waves per SIMD, as the FP32 forward pass (168 VGPRs, a few spilled).
+__global__ __launch_bounds__(kRovsMaxThreads) __attribute__((amdgpu_waves_per_eu(3))) void
+rovs_adj_kernel(RovsAdjArgs a) {
   constexpr int NT = kRovsNT;
   extern __shared__ char smem[];
   const int tid = threadIdx.x, lane = tid & 63;
