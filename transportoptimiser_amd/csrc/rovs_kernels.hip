@@ -21,7 +21,10 @@
 //     product for the AARs (lane shuffles, then a fixed-order sum over the
 //     waves), each path's own control kept by a register select, then the
 //     growth-rate SDE and the logistic growth with the reference's /100 in
-//     the epilogue, noise from Philox in registers;
+//     the epilogue, noise from Philox in registers.  Under controls fixed by
+//     a policy (the smoothed gradient), a second form takes the AARs as dots
+//     of the year-start tile with each control's row sums and forms only the
+//     products of the controls the tile's paths take;
 //   - per backward step: one kernel forms the states and counts each
 //     control's included paths per block, one scans the counts, one
 //     scatters the samples in path order (so that kNN ties resolve as in a
