@@ -307,6 +307,20 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
   auto keyof = [&](int j) -> unsigned long long {
     return (unsigned long long)__double_as_longlong(dist2(j));
   };
+  // fn(j, key) for the samples j = j0, j0 + stride, ... below n, four at a
+  // time: the four loads are issued together (indices clamped to n - 1, so
+  // none is conditional), then the keys below n are used.  The callers do not
+  // depend on the order of the calls.
+  auto for_keys = [&](const int j0, const int stride, auto&& fn) {
+    for (int jb = j0; jb < n; jb += 4 * stride) {
+      unsigned long long key[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) key[u] = keyof(min(jb + u * stride, n - 1));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (jb + u * stride < n) fn(jb + u * stride, key[u]);
+    }
+  };
   // Radix passes over every step-th sample: prefix holds the resolved high
   // bits of the k-th key among them, kk its rank among the keys that share
   // them, hb their number.  At least min_passes passes, then until the
@@ -323,10 +337,9 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
       const unsigned int dmask = pass < 5 ? kBins - 1 : 511u;
       for (int i = tid; i < kBins; i += 256) hist[i] = 0u;
       __syncthreads();
-      for (int j = tid * step; j < n; j += 256 * step) {
-        const unsigned long long key = keyof(j);
+      for_keys(tid * step, 256 * step, [&](int, unsigned long long key) {
         if ((key & hmask) == prefix) atomicAdd(&hist[(unsigned)(key >> shift) & dmask], 1u);
-      }
+      });
       __syncthreads();
       {
         unsigned int sum = 0u;
@@ -433,8 +446,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
     const unsigned long long bound = prefix | ~hmask;
     if (tid == 0) s_ecnt = 0;
     __syncthreads();
-    for (int j = tid; j < n; j += 256) {
-      const unsigned long long key = keyof(j);
+    for_keys(tid, 256, [&](int j, unsigned long long key) {
       if (key <= bound) {
         const int pos = atomicAdd(&s_ecnt, 1);
         if (pos < kKnnSelectCap) {
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
           eidx[pos] = j;
         }
       }
-    }
+    });
     __syncthreads();
     const int cnt = s_ecnt;  // at least k
     if (cnt <= kKnnSelectCap) {
