@@ -52,6 +52,66 @@ bool species_matrices_finite(const roadmc_species& sp, int max_patches) {
          (!sp.surv || finite_array(sp.surv, (size_t)P * P));
 }
 
+// fn(i0, i1) over contiguous ranges of [0, n) on up to 16 host threads
+// (serially when no thread can be had).
+template <class F>
+void parallel_ranges(size_t n, const F& fn) {
+  const size_t nth = std::min<size_t>(
+      std::max<size_t>(n, 1), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  std::vector<std::thread> pool;
+  try {
+    for (size_t t = 1; t < nth; ++t) pool.emplace_back(fn, n * t / nth, n * (t + 1) / nth);
+    fn(0, n / nth);
+  } catch (...) {
+    for (auto& th : pool) th.join();
+    pool.clear();
+    fn(0, n);
+    return;
+  }
+  for (auto& th : pool) th.join();
+}
+
+// species_matrices_finite for every species of roads[0 .. n_roads), in
+// chunks of at most 2^20 entries over host threads: ok[off[r] + s] for
+// species s of road r (off[r + 1] - off[r] species, none for a road whose
+// species list is invalid).
+void matrices_finite(const roadmc_mte_problem* roads, int n_roads, int max_patches,
+                     std::vector<size_t>& off, std::vector<char>& ok) {
+  off.assign((size_t)n_roads + 1, 0);
+  for (int r = 0; r < n_roads; ++r) {
+    const int ns = roads[r].species && roads[r].n_species >= 1 && roads[r].n_species <= 63
+                       ? roads[r].n_species
+                       : 0;
+    off[r + 1] = off[r] + (size_t)ns;
+  }
+  ok.assign(std::max<size_t>(off[n_roads], 1), 1);
+  struct Chunk {
+    size_t flag;
+    const double* p;
+    size_t n;
+  };
+  constexpr size_t kChunk = (size_t)1 << 20;
+  std::vector<Chunk> chunks;
+  for (int r = 0; r < n_roads; ++r)
+    for (size_t s = 0; s < off[r + 1] - off[r]; ++s) {
+      const roadmc_species& sp = roads[r].species[s];
+      const int P = sp.n_patches;
+      if (!sp.trans || P < 1 || P > max_patches) continue;
+      const size_t n = (size_t)P * P;
+      for (const double* m : {sp.trans, sp.surv}) {
+        if (!m) continue;
+        for (size_t i = 0; i < n; i += kChunk)
+          chunks.push_back({off[r] + s, m + i, std::min(kChunk, n - i)});
+      }
+    }
+  std::vector<char> chunk_ok(std::max<size_t>(chunks.size(), 1), 1);
+  parallel_ranges(chunks.size(), [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; ++i) chunk_ok[i] = finite_array(chunks[i].p, chunks[i].n) ? 1 : 0;
+  });
+  for (size_t i = 0; i < chunks.size(); ++i)
+    if (!chunk_ok[i]) ok[chunks[i].flag] = 0;
+}
+
 // mats_finite, when given, holds species_matrices_finite for each species
 // (computed beforehand, e.g. in parallel); the checks and their order are the
 // same either way.
@@ -577,10 +637,13 @@ int mte_run_wide(roadmc_ctx* ctx, const roadmc_mte_problem* pr, roadmc_mte_resul
     float* cols_h = reinterpret_cast<float*>(static_cast<char*>(ctx->pinned) + mt_bytes);
     if (sp.trans) {
       float* mt_h = static_cast<float*>(ctx->pinned);
-      for (int j = 0; j < Pp; ++j)
-        for (int k = 0; k < Pp; ++k)
-          mt_h[(size_t)j * Pp + k] =
-              (j < P && k < P) ? (float)movement(sp, P, k, j, pr->drop_tol) : 0.f;
+      // Destination rows over host threads (each writes its own rows).
+      parallel_ranges((size_t)Pp, [&](size_t j0, size_t j1) {
+        for (int j = (int)j0; j < (int)j1; ++j)
+          for (int k = 0; k < Pp; ++k)
+            mt_h[(size_t)j * Pp + k] =
+                (j < P && k < P) ? (float)movement(sp, P, k, j, pr->drop_tol) : 0.f;
+      });
       RMC_HIP(ctx, hipMemcpyAsync(mt, mt_h, mt_bytes, hipMemcpyHostToDevice, st));
     } else {
       const roadmc_road_species* rd = sp.road;
@@ -656,7 +719,16 @@ extern "C" int roadmc_mte_run(roadmc_ctx* ctx, const roadmc_mte_problem* problem
                               roadmc_mte_result* result) {
   if (!ctx) return ROADMC_ERR_INVALID;
   if (!result) return rmc_fail(ctx, ROADMC_ERR_INVALID, "result is NULL");
-  int rc = validate_problem(ctx, problem, ROADMC_MAX_PATCHES_WIDE);
+  int rc;
+  if (!problem) {
+    rc = validate_problem(ctx, problem, ROADMC_MAX_PATCHES_WIDE);
+  } else {
+    std::vector<size_t> mat_off;
+    std::vector<char> mat_ok;
+    matrices_finite(problem, 1, ROADMC_MAX_PATCHES_WIDE, mat_off, mat_ok);
+    rc = validate_problem(ctx, problem, ROADMC_MAX_PATCHES_WIDE,
+                          mat_off[1] > 0 ? mat_ok.data() : nullptr);
+  }
   if (rc) return rc;
   RMC_HIP(ctx, hipSetDevice(ctx->device));
   for (int s = 0; s < problem->n_species; ++s)
@@ -733,54 +805,10 @@ extern "C" int roadmc_batch_prepare(roadmc_ctx* ctx, const roadmc_mte_problem* r
     return rmc_fail(ctx, ROADMC_ERR_INVALID, "roads/out invalid");
   *out = nullptr;
   // The host matrices' finiteness (most of the validation's reads) over
-  // threads, species split by matrix size; then the checks road by road, in
-  // order, with those flags.
-  std::vector<size_t> mat_off((size_t)n_roads + 1, 0);
-  for (int r = 0; r < n_roads; ++r) {
-    const int ns = roads[r].species && roads[r].n_species >= 1 && roads[r].n_species <= 63
-                       ? roads[r].n_species
-                       : 0;
-    mat_off[r + 1] = mat_off[r] + (size_t)ns;
-  }
-  std::vector<char> mat_ok(std::max<size_t>(mat_off[n_roads], 1), 1);
-  {
-    std::vector<std::pair<int, int>> items;  // (road, species)
-    size_t total = 0;
-    for (int r = 0; r < n_roads; ++r)
-      for (size_t s = 0; s < mat_off[r + 1] - mat_off[r]; ++s) {
-        items.emplace_back(r, (int)s);
-        const size_t P = (size_t)std::clamp(roads[r].species[s].n_patches, 0, ROADMC_MAX_PATCHES);
-        total += P * P + 4096;
-      }
-    auto check = [&](size_t i0, size_t i1) {
-      for (size_t i = i0; i < i1; ++i) {
-        const int r = items[i].first, sidx = items[i].second;
-        mat_ok[mat_off[r] + sidx] =
-            species_matrices_finite(roads[r].species[sidx], ROADMC_MAX_PATCHES) ? 1 : 0;
-      }
-    };
-    const size_t ni = items.size();
-    const int nth = (int)std::min<size_t>(
-        std::max<size_t>(ni, 1), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
-    std::vector<size_t> bounds(nth + 1, ni);
-    bounds[0] = 0;
-    for (size_t i = 0, acc = 0, t = 1; i < ni && t < (size_t)nth; ++i) {
-      const size_t P = (size_t)std::clamp(roads[items[i].first].species[items[i].second].n_patches,
-                                          0, ROADMC_MAX_PATCHES);
-      acc += P * P + 4096;
-      if (acc * nth >= total * t) bounds[t++] = i + 1;
-    }
-    std::vector<std::thread> pool;
-    try {
-      for (int t = 1; t < nth; ++t) pool.emplace_back(check, bounds[t], bounds[t + 1]);
-      check(bounds[0], bounds[1]);
-    } catch (...) {
-      for (auto& th : pool) th.join();
-      pool.clear();
-      check(0, ni);  // no threads to be had: check serially
-    }
-    for (auto& th : pool) th.join();
-  }
+  // threads; then the checks road by road, in order, with those flags.
+  std::vector<size_t> mat_off;
+  std::vector<char> mat_ok;
+  matrices_finite(roads, n_roads, ROADMC_MAX_PATCHES, mat_off, mat_ok);
   for (int r = 0; r < n_roads; ++r) {
     int rc = validate_problem(ctx, &roads[r], ROADMC_MAX_PATCHES,
                               mat_off[r + 1] > mat_off[r] ? &mat_ok[mat_off[r]] : nullptr);
