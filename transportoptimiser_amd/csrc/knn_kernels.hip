@@ -322,15 +322,15 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
     }
   };
   // Radix passes over every step-th sample: prefix holds the resolved high
-  // bits of the k-th key among them, kk its rank among the keys that share
+  // bits of the rank-th key among them, kk its rank among the keys that share
   // them, hb their number.  At least min_passes passes, then until the
   // bucket fits LDS.
   unsigned long long prefix = 0ull, hmask = 0ull;
   int kk = k, hb = n;
-  auto radix = [&](const int step, const int min_passes) {
+  auto radix = [&](const int step, const int min_passes, const int rank) {
     prefix = 0ull;
     hmask = 0ull;
-    kk = k;
+    kk = rank;
     hb = n;
     for (int pass = 0; pass < 6; ++pass) {
       const int shift = pass < 5 ? 53 - 11 * pass : 0;
@@ -434,18 +434,23 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
       a.val[blockIdx.x] = v;
     }
   };
-  // 0. A bound from a subsample: the k-th smallest key of every step-th
-  //    sample is at least the k-th smallest of all, so the samples whose key
-  //    is at most the top of that key's radix bucket include the k nearest
-  //    (with every tie at the bound).  One pass gathers them; when they fit
-  //    LDS, sorting them by (key, index) gives the k neighbours in the order
-  //    of a stable sort of all samples.  Otherwise the full radix select runs.
-  //    Every 4th sample: about 4k samples are gathered, a 512-entry sort at
-  //    k = 114 (every 8th gathers twice as many, and the sort of 1 024
-  //    entries took longer than the extra subsample reads save).
+  // 0. A bound from a subsample of every step-th sample: the top of the
+  //    radix bucket of its ks-th smallest key.  When at least k samples have
+  //    a key at or below it, they include the k nearest (with every tie at
+  //    the bound): one pass gathers them and, when they fit LDS, sorting them
+  //    by (key, index) gives the k neighbours in the order of a stable sort
+  //    of all samples.  Otherwise the full radix select runs.  ks is k / step
+  //    plus five standard deviations of the number of subsample members
+  //    among the k nearest (binomial, p = 1 / step), so the bound holds at
+  //    least k samples but in rare cases; about step x ks samples are
+  //    gathered (a 256-entry sort at k = 114).  With ks = k every sample
+  //    below the bound's rank in the subsample is kept and the sort is four
+  //    times as long, which cost more than the subsample reads it saves.
   const int step = min(4, n / (4 * k));
   if (step >= 2) {
-    radix(step, 2);
+    const double pk = (double)k / step;
+    const int ks = min(k, (int)ceil(pk + 5.0 * sqrt(pk * (1.0 - 1.0 / step)) + 1.0));
+    radix(step, 2, ks);
     const unsigned long long bound = prefix | ~hmask;
     if (tid == 0) s_ecnt = 0;
     __syncthreads();
@@ -459,8 +464,8 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
       }
     });
     __syncthreads();
-    const int cnt = s_ecnt;  // at least k
-    if (cnt <= kKnnSelectCap) {
+    const int cnt = s_ecnt;
+    if (cnt >= k && cnt <= kKnnSelectCap) {
       sort_bucket(cnt);
       if (tid < k) {
         nbd[tid] = __longlong_as_double((long long)ekey[tid]);
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(256) void knn_select_kernel(KnnFitBatch batch) {
     __syncthreads();
   }
   // 1. The full radix select over all samples.
-  radix(1, 1);
+  radix(1, 1, k);
   // 2. Keys below the prefix (exactly k - kk of them) and the bucket.
   if (tid == 0) {
     s_cnt = 0;
