@@ -75,6 +75,27 @@ def test_fit_with_crowded_distances_matches_oracle(ctx, oracle, offset):
                                atol=1e-12)
 
 
+def test_fit_when_the_subsample_bound_gathers_too_few(ctx, oracle):
+    # Every 4th sample (the select kernel's subsample) sits in a tight cluster
+    # at a grid point; the others are spread over the square.  The bound from
+    # the subsample then holds fewer than k samples, and the full radix select
+    # must give the neighbours instead.
+    rng = np.random.default_rng(21)
+    n, R, k = 6000, 6, 60
+    x = rng.uniform(0.0, 1.0, size=(n, 2))
+    x[:4] = [[0.0, 0.0], [1.0, 1.0], [0.0, 1.0], [1.0, 0.0]]  # the grid spans [0, 1]^2
+    members = np.arange(4, n, 4)
+    g1 = np.linspace(0.0, 1.0, R)
+    cell = rng.integers(0, R * R, size=members.size)
+    centres = np.stack([g1[cell // R], g1[cell % R]], axis=1)
+    x[members] = np.clip(centres + rng.uniform(-1e-7, 1e-7, size=centres.shape), 0.0, 1.0)
+    y = rng.normal(size=n)
+    print(f"knn fit: clustered subsample, n={n} R={R} k={k}", flush=True)
+    g = knn.fit(ctx, x, y, R, k)
+    np.testing.assert_allclose(g.surrogate, oracle.knn_fit(x, y, R, k, 0), rtol=1e-9,
+                               atol=1e-12)
+
+
 def test_degenerate_inputs(ctx, oracle):
     rng = np.random.default_rng(6)
     # fewer than 3 samples
