@@ -26,7 +26,7 @@ struct KnnEvalArgs {
 constexpr int kKnnThreads = 64;  // one wave per workgroup
 constexpr int kKnnMaxK = 128;
 
-constexpr int kKnnSelectMin = 2048;  // above this many samples: the radix-select kernel
+constexpr int kKnnSelectMin = 512;   // above this many samples: the radix-select kernel
 constexpr int kKnnSelectCap = 2048;  // largest bucket of the k-th key sorted in LDS
 constexpr int kKnnMaxBatch = 8;      // fits of one launch (blockIdx.y)
 
